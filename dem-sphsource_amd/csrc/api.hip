@@ -1,0 +1,1428 @@
+// api.hip — the C ABI of include/dsph.h: context, device memory, the phase
+// sequence of one timestep, HIP-graph replay and read-back.
+//
+// Replaces the reference's DeviceData ownership and phase drivers:
+// dataBase::buildDeviceData / setSpatialGrids (core/dataBase.cpp:415-449),
+// neighborSearch (core/neighborSearch.cu:314-381), fluidIntegrate
+// (core/integrate.cu:385-421), the DEM phase drivers (:940-998),
+// solverBase::initialize/update (core/solverBase.cpp:13-67) and the
+// uploadState read-backs (core/deviceStructs.h).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/dsph.h"
+#include "dsph_internal.h"
+
+using namespace dsph;
+
+namespace {
+
+struct ApiError {
+    int code;
+    std::string msg;
+};
+
+#define HIPCHK(x)                                                                       \
+    do {                                                                                \
+        hipError_t e_ = (x);                                                            \
+        if (e_ != hipSuccess)                                                           \
+            throw ApiError{DSPH_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_)}; \
+    } while (0)
+
+struct StepGraph {
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    int fcur_after = 0;
+    int dcur_after = 0;
+};
+
+// kernel-family timers (HIP events on the context stream)
+enum Fam { F_GRID = 0, F_FSEARCH, F_FSLIST, F_SSEARCH, F_REINIT, F_CONT, F_MOM, F_FSTREAM, F_DEM, F_NFAM };
+const char* kFamNames[F_NFAM] = {"grid_hash_sort", "fluid_search",  "fluid_solid_list",
+                                 "solid_search",   "density_reinit", "continuity",
+                                 "momentum",       "fluid_stream",  "dem"};
+
+}  // namespace
+
+struct dsph_ctx {
+    int dev = 0;
+    hipStream_t st = nullptr;
+    int prec = DSPH_PREC_MIXED;
+    uint32_t flags = 0;
+    std::string err;
+    bool uploaded = false, initialized = false;
+    int nf = 0, ns = 0, nc = 0, nw = 0;
+    GridP g{};
+    int key_bits = 1;
+    double hmax = 0;
+    double gv[3] = {0, 0, -9.81};
+    // fluid
+    FluidSet fset[2];
+    int fcur = 0, dcur = 0;
+    int* fslot = nullptr;
+    int* fmember = nullptr;
+    unsigned* fkey = nullptr;
+    unsigned* fkey_sorted = nullptr;
+    int* fperm = nullptr;
+    int* fcs = nullptr;
+    int* fce = nullptr;
+    // solids
+    Solids so;
+    unsigned* skey = nullptr;
+    unsigned* skey_sorted = nullptr;
+    int* sperm = nullptr;
+    int* scs = nullptr;
+    int* sce = nullptr;
+    Clumps cl;
+    double4* wpt = nullptr;
+    double4* wn = nullptr;
+    int* wmat = nullptr;
+    double* whist = nullptr;
+    ContactTables tb{};
+    int nmat = 0, ncomb = 0;
+    Lists L;
+    Bonds B;
+    unsigned* radix_tmp = nullptr;
+    int* scan_tmp = nullptr;
+    unsigned long long* status = nullptr;
+    int reorder_period = 1;
+    double graph_dt = -1.0;  // time step baked into the captured graphs
+    // host copies used at initialisation (bond creation)
+    std::vector<double> h_spos, h_srad;
+    std::vector<int> h_smat, h_sclus;
+    std::vector<double> h_bonded;
+    std::vector<void*> allocs;
+    std::map<uint64_t, StepGraph> graphs;
+    // timing
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[F_NFAM];
+    size_t ev_used[F_NFAM] = {0};
+
+    bool mixed() const { return prec == DSPH_PREC_MIXED; }
+};
+
+namespace {
+
+template <class T>
+T* dalloc(dsph_ctx* c, size_t n) {
+    if (n == 0) n = 1;
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, n * sizeof(T));
+    if (e != hipSuccess)
+        throw ApiError{DSPH_E_NOMEM, "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed"};
+    c->allocs.push_back(p);
+    return (T*)p;
+}
+
+void dfree(dsph_ctx* c, void* p) {
+    if (!p) return;
+    auto it = std::find(c->allocs.begin(), c->allocs.end(), p);
+    if (it != c->allocs.end()) c->allocs.erase(it);
+    (void)hipFree(p);
+}
+
+template <class T>
+void h2d(dsph_ctx* c, T* d, const T* h, size_t n) {
+    if (n) HIPCHK(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, c->st));
+}
+template <class T>
+void d2h(dsph_ctx* c, T* h, const T* d, size_t n) {
+    if (n) HIPCHK(hipMemcpyAsync(h, d, n * sizeof(T), hipMemcpyDeviceToHost, c->st));
+}
+
+void free_all(dsph_ctx* c) {
+    for (auto& kv : c->graphs) {
+        if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+        if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
+    }
+    c->graphs.clear();
+    for (void* p : c->allocs) (void)hipFree(p);
+    c->allocs.clear();
+}
+
+int fail(dsph_ctx* c, const ApiError& e) {
+    if (c) c->err = e.msg;
+    return e.code;
+}
+
+// --------------------------------------------------------------- timing
+struct FamTimer {
+    dsph_ctx* c;
+    int f;
+    FamTimer(dsph_ctx* c_, int f_) : c(c_), f(f_) {
+        if (!c->timing) return;
+        if (c->ev_used[f] == c->ev[f].size()) {
+            hipEvent_t a, b;
+            HIPCHK(hipEventCreate(&a));
+            HIPCHK(hipEventCreate(&b));
+            c->ev[f].push_back({a, b});
+        }
+        HIPCHK(hipEventRecord(c->ev[f][c->ev_used[f]].first, c->st));
+    }
+    ~FamTimer() {
+        if (!c->timing) return;
+        (void)hipEventRecord(c->ev[f][c->ev_used[f]].second, c->st);
+        c->ev_used[f]++;
+    }
+};
+
+// --------------------------------------------------------------- capacities
+void alloc_ff(dsph_ctx* c, int K) {
+    dfree(c, c->L.nbr);
+    c->L.K = K;
+    c->L.nbr = dalloc<int>(c, (size_t)K * std::max(c->nf, 1));
+}
+
+void alloc_fs(dsph_ctx* c, int64_t cap) {
+    dfree(c, c->L.fs_i);
+    dfree(c, c->L.fs_a);
+    dfree(c, c->L.fsF);
+    dfree(c, c->L.fsr_idx);
+    c->L.fs_cap = cap;
+    c->L.fs_i = dalloc<int>(c, cap);
+    c->L.fs_a = dalloc<int>(c, cap);
+    c->L.fsF = dalloc<double4>(c, cap);
+    c->L.fsr_idx = dalloc<int>(c, cap);
+}
+
+// grows the solid–solid arrays, keeping the saved history (offsets are
+// separate arrays and unaffected)
+void alloc_ss(dsph_ctx* c, int64_t cap, bool keep_history) {
+    int* old_hj = c->L.h_j;
+    double* old_hS = c->L.hS;
+    int64_t old_cap = c->L.ss_cap;
+    for (void* p : {(void*)c->L.ss_i, (void*)c->L.ss_j, (void*)c->L.ssF, (void*)c->L.ssT,
+                    (void*)c->L.ssS, (void*)c->L.ssr_idx})
+        dfree(c, p);
+    c->L.ss_cap = cap;
+    c->L.ss_i = dalloc<int>(c, cap);
+    c->L.ss_j = dalloc<int>(c, cap);
+    c->L.ssF = dalloc<double4>(c, cap);
+    c->L.ssT = dalloc<double4>(c, cap);
+    c->L.ssS = dalloc<double>(c, 9 * cap);
+    c->L.ssr_idx = dalloc<int>(c, cap);
+    c->L.h_j = dalloc<int>(c, cap);
+    c->L.hS = dalloc<double>(c, 9 * cap);
+    if (keep_history && old_hj) {
+        int64_t n = std::min(old_cap, cap);
+        HIPCHK(hipMemcpyAsync(c->L.h_j, old_hj, n * sizeof(int), hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(c->L.hS, old_hS, 9 * n * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+    }
+    dfree(c, old_hj);
+    dfree(c, old_hS);
+}
+
+void invalidate_graphs(dsph_ctx* c) {
+    for (auto& kv : c->graphs) {
+        if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+        if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
+    }
+    c->graphs.clear();
+}
+
+// ----------------------------------------------------------------- phases
+FluidArgs fluid_args(dsph_ctx* c) {
+    FluidArgs a;
+    a.f = c->fset[c->fcur];
+    a.nf = c->nf;
+    a.cur = c->dcur;
+    a.L = c->L;
+    a.so = c->so;
+    a.gx = c->gv[0];
+    a.gy = c->gv[1];
+    a.gz = c->gv[2];
+    a.flags = c->flags;
+    return a;
+}
+
+DemArgs dem_args(dsph_ctx* c) {
+    DemArgs a;
+    a.so = c->so;
+    a.ns = c->ns;
+    a.c = c->cl;
+    a.nc = c->nc;
+    a.L = c->L;
+    a.b = c->B;
+    a.wpt = c->wpt;
+    a.wn = c->wn;
+    a.wmat = c->wmat;
+    a.nw = c->nw;
+    a.whist = c->whist;
+    a.tb = c->tb;
+    a.gx = c->gv[0];
+    a.gy = c->gv[1];
+    a.gz = c->gv[2];
+    a.flags = c->flags;
+    return a;
+}
+
+bool is_reorder(dsph_ctx* c, int i_step, int gap) {
+    return c->reorder_period > 0 && ((i_step / gap) % c->reorder_period) == 0;
+}
+
+void neighbor_search(dsph_ctx* c, int i_step, int gap, bool save_history) {
+    hipStream_t s = c->st;
+    long long* stw = (long long*)c->status;
+    {
+        FamTimer t(c, F_GRID);
+        hash_fluid(s, c->g, c->fset[c->fcur].ph, c->fset[c->fcur].orig, c->nf, c->fkey, stw);
+        hash_solid(s, c->g, c->so.pr, c->ns, c->skey, stw);
+        // ranges of an empty species are filled with -1 as well
+        if (c->nf > 0)
+            radix_sort_iota(s, c->fkey, c->nf, c->key_bits, c->fkey_sorted, c->fperm, c->radix_tmp);
+        cell_spans(s, c->fkey_sorted, c->nf, c->fcs, c->fce, c->g.ncell);
+        if (c->ns > 0)
+            radix_sort_iota(s, c->skey, c->ns, c->key_bits, c->skey_sorted, c->sperm, c->radix_tmp);
+        cell_spans(s, c->skey_sorted, c->ns, c->scs, c->sce, c->g.ncell);
+        if (i_step % gap == 0 && c->nf > 0) {
+            if (is_reorder(c, i_step, gap)) {
+                reorder_fluid(s, c->fset[c->fcur], c->fset[1 - c->fcur], c->fperm, c->fslot,
+                              c->fmember, c->nf, c->mixed(), c->dcur);
+                c->fcur ^= 1;
+            } else {
+                member_slots(s, c->fperm, c->fslot, c->fmember, c->nf);
+            }
+        }
+    }
+    if (i_step % gap == 0 && c->nf > 0) {
+        {
+            FamTimer t(c, F_FSEARCH);
+            fluid_search(s, c->g, c->fset[c->fcur], c->nf, c->fcs, c->fce, c->fmember, c->scs,
+                         c->sce, c->sperm, c->so, c->hmax, c->L, stw);
+        }
+        FamTimer t(c, F_FSLIST);
+        exclusive_scan(s, c->L.fscnt, c->L.fsoff, c->nf, c->scan_tmp);
+        fluid_solid_fill(s, c->g, c->fset[c->fcur], c->nf, c->scs, c->sce, c->sperm, c->so, c->L,
+                         stw);
+        if (c->ns > 0)
+            reverse_movable(s, c->L.fs_a, c->L.fsoff + c->nf, c->L.fs_cap, c->so.vel, c->ns,
+                            c->L.fsr_cnt, c->L.fsr_off, c->L.fsr_idx, c->scan_tmp);
+    }
+    if (c->ns > 0) {
+        FamTimer t(c, F_SSEARCH);
+        if (save_history) solid_save_history(s, c->L, c->ns);
+        solid_search(s, c->g, c->so, c->ns, c->scs, c->sce, c->sperm, c->L, c->scan_tmp, stw);
+        reverse_solid(s, c->L, c->ns, c->scan_tmp);
+    }
+}
+
+void read_status(dsph_ctx* c, unsigned long long* w) {
+    HIPCHK(hipMemcpyAsync(w, c->status, ST_WORDS * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+}
+
+// search with the reference's grow-on-demand semantics (setNum realloc,
+// deviceStructs.h:483-491, 668-691): on overflow the lists grow and the
+// search is repeated without a second reorder or history save
+void search_checked(dsph_ctx* c, int i_step, int gap) {
+    neighbor_search(c, i_step, gap, true);
+    for (int attempt = 0; attempt < 8; attempt++) {
+        unsigned long long w[ST_WORDS];
+        read_status(c, w);
+        if (!(w[ST_FF_OVERFLOW] | w[ST_FS_OVERFLOW] | w[ST_SS_OVERFLOW])) return;
+        if (w[ST_FF_OVERFLOW])
+            alloc_ff(c, (int)std::max<unsigned long long>(w[ST_FF_MAX] + 16, (unsigned long long)c->L.K * 2));
+        if (w[ST_FS_OVERFLOW]) alloc_fs(c, (int64_t)(w[ST_FS_TOTAL] * 3 / 2 + 1024));
+        if (w[ST_SS_OVERFLOW]) alloc_ss(c, (int64_t)(w[ST_SS_TOTAL] * 3 / 2 + 1024), true);
+        unsigned long long zero[3] = {0, 0, 0};
+        HIPCHK(hipMemcpyAsync(c->status, zero, sizeof(zero), hipMemcpyHostToDevice, c->st));
+        invalidate_graphs(c);
+        const int rp = c->reorder_period;
+        c->reorder_period = 0;
+        neighbor_search(c, i_step, gap, false);
+        c->reorder_period = rp;
+    }
+    throw ApiError{DSPH_E_CAPACITY, "neighbor list capacity could not be satisfied"};
+}
+
+void solid_before(dsph_ctx* c, double dt) {
+    FamTimer t(c, F_DEM);
+    DemArgs a = dem_args(c);
+    dem_solid_kick(c->st, a, 0.5 * dt);
+    dem_clump_kick(c->st, a, 0.5 * dt);
+    dem_clump_drift(c->st, a, 0.5 * dt);
+    dem_solid_drift(c->st, a, 0.5 * dt);
+}
+
+void solid_after(dsph_ctx* c, double dt) {
+    FamTimer t(c, F_DEM);
+    DemArgs a = dem_args(c);
+    dem_solid_drift(c->st, a, 0.5 * dt);
+    dem_clump_drift(c->st, a, 0.5 * dt);
+    dem_clump_kick(c->st, a, 0.5 * dt);
+    dem_solid_kick(c->st, a, 0.5 * dt);
+}
+
+void contacts(dsph_ctx* c, double dt) {
+    FamTimer t(c, F_DEM);
+    DemArgs a = dem_args(c);
+    dem_contacts(c->st, a, dt);
+    dem_bonds(c->st, a, dt);
+    dem_solid_sum(c->st, a);
+    if (c->nw > 0) dem_walls(c->st, a, dt);
+    dem_clump_sum(c->st, a);
+}
+
+void fluid_integrate(dsph_ctx* c, double dt, int i_step, int gap) {
+    if (c->nf <= 0) return;
+    const bool mx = c->mixed();
+    const double dtf = dt * double(gap);
+    if (i_step % (5 * gap) == 0 || i_step == 1) {
+        FamTimer t(c, F_REINIT);
+        density_reinit(c->st, fluid_args(c), i_step, mx);
+    }
+    if (i_step % gap == 0) {
+        {
+            FamTimer t(c, F_CONT);
+            continuity(c->st, fluid_args(c), 0.5 * dtf, mx);
+        }
+        c->dcur ^= 1;
+        {
+            FamTimer t(c, F_FSTREAM);
+            fluid_drift(c->st, fluid_args(c), 0.5 * dtf);
+        }
+        {
+            FamTimer t(c, F_MOM);
+            momentum(c->st, fluid_args(c), mx, true);
+        }
+        {
+            FamTimer t(c, F_FSTREAM);
+            fluid_kick_drift(c->st, fluid_args(c), dtf, mx);
+        }
+        {
+            FamTimer t(c, F_CONT);
+            continuity(c->st, fluid_args(c), 0.5 * dtf, mx);
+        }
+        c->dcur ^= 1;
+    } else {
+        FamTimer t(c, F_MOM);
+        fluid_solid_force(c->st, fluid_args(c), mx);
+    }
+}
+
+void step_body(dsph_ctx* c, int i_step, double dt, int gap) {
+    neighbor_search(c, i_step, gap, true);
+    solid_before(c, dt);
+    fluid_integrate(c, dt, i_step, gap);
+    contacts(c, dt);
+    solid_after(c, dt);
+}
+
+uint64_t step_key(dsph_ctx* c, int i_step, int gap) {
+    uint64_t k = 0;
+    k |= (uint64_t)(i_step % gap == 0);
+    k |= (uint64_t)(i_step % (5 * gap) == 0 || i_step == 1) << 1;
+    k |= (uint64_t)(i_step == 1) << 2;
+    k |= (uint64_t)(i_step % gap == 0 && is_reorder(c, i_step, gap)) << 3;
+    k |= (uint64_t)c->fcur << 4;
+    k |= (uint64_t)c->dcur << 5;
+    k |= (uint64_t)(uint32_t)gap << 8;
+    return k;
+}
+
+// ------------------------------------------------------------- validation
+void require(bool cond, const std::string& msg) {
+    if (!cond) throw ApiError{DSPH_E_INVALID, msg};
+}
+
+}  // namespace
+
+namespace {
+// fluid–solid pairs in reference order (CSR by fluid caller index) as pair
+// ids into the device arrays
+std::vector<int64_t> fs_reference_order(dsph_ctx* c, std::vector<int>& fluid_idx,
+                                        std::vector<int>& solid_idx, std::vector<int>* count) {
+    const int nf = c->nf;
+    std::vector<int> off(nf + 1), orig(nf);
+    d2h(c, off.data(), c->L.fsoff, nf + 1);
+    d2h(c, orig.data(), c->fset[c->fcur].orig, nf);
+    HIPCHK(hipStreamSynchronize(c->st));
+    int64_t tot = off[nf];
+    if (tot > c->L.fs_cap) throw ApiError{DSPH_E_CAPACITY, "fluid-solid list overflowed"};
+    std::vector<int> a(tot);
+    d2h(c, a.data(), c->L.fs_a, tot);
+    HIPCHK(hipStreamSynchronize(c->st));
+    std::vector<int> slot_of(nf);
+    for (int s = 0; s < nf; s++) slot_of[orig[s]] = s;
+    std::vector<int64_t> ids;
+    ids.reserve(tot);
+    fluid_idx.clear();
+    solid_idx.clear();
+    if (count) count->assign(nf, 0);
+    for (int i = 0; i < nf; i++) {
+        int s = slot_of[i];
+        for (int k = off[s]; k < off[s + 1]; k++) {
+            ids.push_back(k);
+            fluid_idx.push_back(i);
+            solid_idx.push_back(a[k]);
+        }
+        if (count) (*count)[i] = off[s + 1] - off[s];
+    }
+    return ids;
+}
+}  // namespace
+
+// ==========================================================================
+extern "C" {
+
+const char* dsph_version(void) { return "dsph-mi355x 0.1 (gfx950)"; }
+
+const char* dsph_last_error(const dsph_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int dsph_create(int device_index, int precision, dsph_ctx** out) {
+    if (!out) return DSPH_E_INVALID;
+    *out = nullptr;
+    if (precision != DSPH_PREC_FP64 && precision != DSPH_PREC_MIXED) return DSPH_E_INVALID;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return DSPH_E_NODEVICE;
+    if (device_index < 0 || device_index >= n) return DSPH_E_NODEVICE;
+    dsph_ctx* c = new dsph_ctx();
+    c->dev = device_index;
+    c->prec = precision;
+    if (hipSetDevice(device_index) != hipSuccess || hipStreamCreate(&c->st) != hipSuccess) {
+        delete c;
+        return DSPH_E_HIP;
+    }
+    *out = c;
+    return DSPH_OK;
+}
+
+int dsph_destroy(dsph_ctx* c) {
+    if (!c) return DSPH_OK;
+    (void)hipSetDevice(c->dev);
+    (void)hipStreamSynchronize(c->st);
+    free_all(c);
+    for (int f = 0; f < F_NFAM; f++)
+        for (auto& e : c->ev[f]) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+    (void)hipStreamDestroy(c->st);
+    delete c;
+    return DSPH_OK;
+}
+
+int dsph_set_flags(dsph_ctx* c, uint32_t flags) {
+    if (!c) return DSPH_E_INVALID;
+    c->flags = flags;
+    invalidate_graphs(c);
+    return DSPH_OK;
+}
+
+int dsph_set_reorder_period(dsph_ctx* c, int32_t p) {
+    if (!c || p < 0) return DSPH_E_INVALID;
+    c->reorder_period = p;
+    invalidate_graphs(c);
+    return DSPH_OK;
+}
+
+int dsph_set_gravity(dsph_ctx* c, const double g[3]) {
+    if (!c || !g) return DSPH_E_INVALID;
+    c->gv[0] = g[0];
+    c->gv[1] = g[1];
+    c->gv[2] = g[2];
+    invalidate_graphs(c);
+    return DSPH_OK;
+}
+
+int dsph_upload_scene(dsph_ctx* c, const dsph_scene* s) {
+    if (!c || !s) return DSPH_E_INVALID;
+    try {
+        HIPCHK(hipSetDevice(c->dev));
+        require(s->n_fluid >= 0 && s->n_solid >= 0 && s->n_clump >= 0 && s->n_wall >= 0,
+                "negative particle count");
+        require(s->n_fluid == 0 || (s->fluid_pos && s->fluid_h && s->fluid_rho0 && s->fluid_c && s->fluid_nu),
+                "fluid arrays missing");
+        require(s->n_solid == 0 || (s->solid_pos && s->solid_radius && s->solid_eff_radius && s->solid_inv_mass),
+                "solid arrays missing");
+        require(s->n_clump == 0 || (s->clump_centroid && s->clump_inv_inertia && s->clump_inv_mass &&
+                                    s->clump_pebble_start && s->clump_pebble_end),
+                "clump arrays missing");
+        require(s->n_wall == 0 || (s->wall_point && s->wall_normal), "wall arrays missing");
+        require(s->n_material >= 1, "n_material must be >= 1");
+        for (int k = 0; k < 3; k++) require(s->domain_size[k] > 0, "domain size must be positive");
+        const int nmat = s->n_material;
+        for (int i = 0; i < s->n_solid; i++) {
+            int m = s->solid_material ? s->solid_material[i] : 0;
+            int d = s->solid_dummy ? s->solid_dummy[i] : 0;
+            require(d == 1 || (m >= 0 && m < nmat),
+                    "solid " + std::to_string(i) + " has material outside [0, n_material)");
+        }
+        for (int i = 0; i < s->n_wall; i++) {
+            int m = s->wall_material ? s->wall_material[i] : 0;
+            require(m >= 0 && m < nmat, "wall material outside [0, n_material)");
+        }
+        for (int k = 0; k < s->n_clump; k++)
+            require(s->clump_pebble_start[k] >= 0 && s->clump_pebble_end[k] <= s->n_solid &&
+                        s->clump_pebble_start[k] <= s->clump_pebble_end[k],
+                    "clump pebble range out of bounds");
+        for (int i = 0; i < s->n_fluid; i++) require(s->fluid_h[i] > 0, "fluid smoothing length must be > 0");
+
+        invalidate_graphs(c);
+        free_all(c);
+        c->L = Lists();
+        c->B = Bonds();
+        c->so = Solids();
+        c->cl = Clumps();
+        c->fset[0] = FluidSet();
+        c->fset[1] = FluidSet();
+        c->fcur = c->dcur = 0;
+        c->initialized = false;
+        const int nf = c->nf = s->n_fluid;
+        const int ns = c->ns = s->n_solid;
+        const int nc = c->nc = s->n_clump;
+        const int nw = c->nw = s->n_wall;
+        c->gv[0] = s->gravity[0];
+        c->gv[1] = s->gravity[1];
+        c->gv[2] = s->gravity[2];
+
+        // ---- grid: setSpatialGrids (dataBase.cpp:415-432)
+        double mf = 0, ms = 0;
+        for (int i = 0; i < nf; i++) mf = std::max(mf, s->fluid_h[i]);
+        for (int i = 0; i < ns; i++) ms = std::max(ms, s->solid_eff_radius[i]);
+        c->hmax = mf;
+        const double cs = 2 * std::max(mf, ms);
+        GridP& g = c->g;
+        g.minx = s->domain_origin[0];
+        g.miny = s->domain_origin[1];
+        g.minz = s->domain_origin[2];
+        g.maxx = s->domain_origin[0] + s->domain_size[0];
+        g.maxy = s->domain_origin[1] + s->domain_size[1];
+        g.maxz = s->domain_origin[2] + s->domain_size[2];
+        g.gx = s->domain_size[0] > cs ? int(s->domain_size[0] / cs) : 1;
+        g.gy = s->domain_size[1] > cs ? int(s->domain_size[1] / cs) : 1;
+        g.gz = s->domain_size[2] > cs ? int(s->domain_size[2] / cs) : 1;
+        g.csx = s->domain_size[0] / double(g.gx);
+        g.csy = s->domain_size[1] / double(g.gy);
+        g.csz = s->domain_size[2] / double(g.gz);
+        const long long ncell = (long long)g.gx * g.gy * g.gz + 1;
+        require(ncell < (1LL << 31) - 1, "grid has too many cells");
+        g.ncell = (int)ncell;
+        double ext = std::max({std::fabs(g.minx), std::fabs(g.maxx), std::fabs(g.miny), std::fabs(g.maxy),
+                               std::fabs(g.minz), std::fabs(g.maxz)});
+        g.pad = 1e-9 * (1.0 + ext);
+        c->key_bits = 1;
+        while ((1LL << c->key_bits) < ncell) c->key_bits++;
+
+        // ---- fluid (two sets for the sorted-order ping-pong)
+        const bool mx = c->mixed();
+        std::vector<double4> ph(nf), vr(nf), acc(nf, make_double4(0, 0, 0, 0));
+        std::vector<double2> cn(nf);
+        std::vector<double> drho(nf, 0.0), sumw0(nf, 0.0);
+        std::vector<int> orig(nf);
+        std::vector<float4> vr32(mx ? nf : 0), p32(mx ? nf : 0);
+        std::vector<float> drho32(mx ? nf : 0);
+        for (int i = 0; i < nf; i++) {
+            const double* p = s->fluid_pos + 3 * i;
+            ph[i] = make_double4(p[0], p[1], p[2], s->fluid_h[i]);
+            double vx = 0, vy = 0, vz = 0;
+            if (s->fluid_vel) {
+                vx = s->fluid_vel[3 * i];
+                vy = s->fluid_vel[3 * i + 1];
+                vz = s->fluid_vel[3 * i + 2];
+            }
+            vr[i] = make_double4(vx, vy, vz, s->fluid_rho0[i]);
+            cn[i] = make_double2(s->fluid_c[i], s->fluid_nu[i]);
+            if (s->fluid_drho) drho[i] = s->fluid_drho[i];
+            orig[i] = i;
+            if (mx) {
+                double h13 = s->fluid_h[i] / 1.3;
+                double m = h13 * h13 * h13 * s->fluid_rho0[i];
+                vr32[i] = make_float4((float)vx, (float)vy, (float)vz, (float)s->fluid_rho0[i]);
+                p32[i] = make_float4((float)s->fluid_c[i], (float)s->fluid_nu[i], (float)m,
+                                     (float)s->fluid_h[i]);
+                drho32[i] = (float)drho[i];
+            }
+        }
+        for (int k = 0; k < 2; k++) {
+            FluidSet& f = c->fset[k];
+            f.ph = dalloc<double4>(c, nf);
+            f.vr = dalloc<double4>(c, nf);
+            f.drho[0] = dalloc<double>(c, nf);
+            f.drho[1] = dalloc<double>(c, nf);
+            f.cn = dalloc<double2>(c, nf);
+            f.sumw0 = dalloc<double>(c, nf);
+            f.acc = dalloc<double4>(c, nf);
+            f.orig = dalloc<int>(c, nf);
+            if (mx) {
+                f.vr32 = dalloc<float4>(c, nf);
+                f.drho32[0] = dalloc<float>(c, nf);
+                f.drho32[1] = dalloc<float>(c, nf);
+                f.p32 = dalloc<float4>(c, nf);
+            }
+        }
+        {
+            FluidSet& f = c->fset[0];
+            h2d(c, f.ph, ph.data(), nf);
+            h2d(c, f.vr, vr.data(), nf);
+            h2d(c, f.drho[0], drho.data(), nf);
+            h2d(c, f.drho[1], drho.data(), nf);
+            h2d(c, f.cn, cn.data(), nf);
+            h2d(c, f.sumw0, sumw0.data(), nf);
+            h2d(c, f.acc, acc.data(), nf);
+            h2d(c, f.orig, orig.data(), nf);
+            if (mx) {
+                h2d(c, f.vr32, vr32.data(), nf);
+                h2d(c, f.drho32[0], drho32.data(), nf);
+                h2d(c, f.drho32[1], drho32.data(), nf);
+                h2d(c, f.p32, p32.data(), nf);
+            }
+        }
+        c->fslot = dalloc<int>(c, nf);
+        h2d(c, c->fslot, orig.data(), nf);
+        c->fmember = dalloc<int>(c, nf);
+        c->fkey = dalloc<unsigned>(c, nf);
+        c->fkey_sorted = dalloc<unsigned>(c, nf);
+        c->fperm = dalloc<int>(c, nf);
+        c->fcs = dalloc<int>(c, g.ncell);
+        c->fce = dalloc<int>(c, g.ncell);
+        c->scs = dalloc<int>(c, g.ncell);
+        c->sce = dalloc<int>(c, g.ncell);
+
+        // ---- solids
+        std::vector<double4> spr(ns), svel(ns), sw(ns), snrm(ns), zero4(std::max({ns, nc, 1}), make_double4(0, 0, 0, 0));
+        std::vector<double> seff(ns);
+        std::vector<int> smat(ns), sclus(ns), sclump(ns), sdum(ns), active;
+        c->h_spos.assign(3 * (size_t)ns, 0);
+        c->h_srad.assign(ns, 0);
+        c->h_smat.assign(ns, 0);
+        c->h_sclus.assign(ns, 0);
+        for (int i = 0; i < ns; i++) {
+            const double* p = s->solid_pos + 3 * i;
+            spr[i] = make_double4(p[0], p[1], p[2], s->solid_radius[i]);
+            const double* v = s->solid_vel ? s->solid_vel + 3 * i : nullptr;
+            svel[i] = make_double4(v ? v[0] : 0, v ? v[1] : 0, v ? v[2] : 0, s->solid_inv_mass[i]);
+            const double* w = s->solid_angvel ? s->solid_angvel + 3 * i : nullptr;
+            sw[i] = make_double4(w ? w[0] : 0, w ? w[1] : 0, w ? w[2] : 0, 0);
+            snrm[i] = make_double4(0, 0, 0, 0);
+            seff[i] = s->solid_eff_radius[i];
+            smat[i] = s->solid_material ? s->solid_material[i] : 0;
+            sclus[i] = s->solid_cluster ? s->solid_cluster[i] : -1;
+            sclump[i] = s->solid_clump ? s->solid_clump[i] : -1;
+            sdum[i] = s->solid_dummy ? s->solid_dummy[i] : 0;
+            if (sdum[i] != 1) active.push_back(i);
+            for (int k = 0; k < 3; k++) c->h_spos[3 * i + k] = p[k];
+            c->h_srad[i] = s->solid_radius[i];
+            c->h_smat[i] = smat[i];
+            c->h_sclus[i] = sclus[i];
+        }
+        Solids& so = c->so;
+        so.pr = dalloc<double4>(c, ns);
+        so.vel = dalloc<double4>(c, ns);
+        so.acc = dalloc<double4>(c, ns);
+        so.w = dalloc<double4>(c, ns);
+        so.tq = dalloc<double4>(c, ns);
+        so.nrm = dalloc<double4>(c, ns);
+        so.eff = dalloc<double>(c, ns);
+        so.mat = dalloc<int>(c, ns);
+        so.clus = dalloc<int>(c, ns);
+        so.clump = dalloc<int>(c, ns);
+        so.dummy = dalloc<int>(c, ns);
+        so.active = dalloc<int>(c, active.size());
+        so.nactive = (int)active.size();
+        h2d(c, so.pr, spr.data(), ns);
+        h2d(c, so.vel, svel.data(), ns);
+        h2d(c, so.acc, zero4.data(), ns);
+        h2d(c, so.w, sw.data(), ns);
+        h2d(c, so.tq, zero4.data(), ns);
+        h2d(c, so.nrm, snrm.data(), ns);
+        h2d(c, so.eff, seff.data(), ns);
+        h2d(c, so.mat, smat.data(), ns);
+        h2d(c, so.clus, sclus.data(), ns);
+        h2d(c, so.clump, sclump.data(), ns);
+        h2d(c, so.dummy, sdum.data(), ns);
+        h2d(c, so.active, active.data(), active.size());
+        if (s->solid_normal) {
+            for (int i = 0; i < ns; i++)
+                snrm[i] = make_double4(s->solid_normal[3 * i], s->solid_normal[3 * i + 1], s->solid_normal[3 * i + 2], 0);
+            h2d(c, so.nrm, snrm.data(), ns);
+        }
+        c->skey = dalloc<unsigned>(c, ns);
+        c->skey_sorted = dalloc<unsigned>(c, ns);
+        c->sperm = dalloc<int>(c, ns);
+
+        // ---- clumps
+        std::vector<double4> cpos(nc), cvel(nc), cw(nc), cq(nc);
+        std::vector<double> cinv(6 * (size_t)nc);
+        std::vector<int> cst(nc), cen(nc);
+        for (int k = 0; k < nc; k++) {
+            const double* p = s->clump_centroid + 3 * k;
+            cpos[k] = make_double4(p[0], p[1], p[2], 0);
+            const double* v = s->clump_vel ? s->clump_vel + 3 * k : nullptr;
+            cvel[k] = make_double4(v ? v[0] : 0, v ? v[1] : 0, v ? v[2] : 0, s->clump_inv_mass[k]);
+            const double* w = s->clump_angvel ? s->clump_angvel + 3 * k : nullptr;
+            cw[k] = make_double4(w ? w[0] : 0, w ? w[1] : 0, w ? w[2] : 0, 0);
+            if (s->clump_orientation) {
+                const double* q = s->clump_orientation + 4 * k;
+                cq[k] = make_double4(q[0], q[1], q[2], q[3]);
+            } else {
+                cq[k] = make_double4(1, 0, 0, 0);
+            }
+            for (int m = 0; m < 6; m++) cinv[6 * k + m] = s->clump_inv_inertia[6 * k + m];
+            cst[k] = s->clump_pebble_start[k];
+            cen[k] = s->clump_pebble_end[k];
+        }
+        Clumps& cl = c->cl;
+        cl.pos = dalloc<double4>(c, nc);
+        cl.vel = dalloc<double4>(c, nc);
+        cl.acc = dalloc<double4>(c, nc);
+        cl.w = dalloc<double4>(c, nc);
+        cl.tq = dalloc<double4>(c, nc);
+        cl.q = dalloc<double4>(c, nc);
+        cl.invI = dalloc<double>(c, 6 * (size_t)nc);
+        cl.start = dalloc<int>(c, nc);
+        cl.end = dalloc<int>(c, nc);
+        h2d(c, cl.pos, cpos.data(), nc);
+        h2d(c, cl.vel, cvel.data(), nc);
+        h2d(c, cl.acc, zero4.data(), nc);
+        h2d(c, cl.w, cw.data(), nc);
+        h2d(c, cl.tq, zero4.data(), nc);
+        h2d(c, cl.q, cq.data(), nc);
+        h2d(c, cl.invI, cinv.data(), 6 * (size_t)nc);
+        h2d(c, cl.start, cst.data(), nc);
+        h2d(c, cl.end, cen.data(), nc);
+
+        // ---- infinite walls and their per-solid spring history
+        std::vector<double4> wp(nw), wnv(nw);
+        std::vector<int> wm(nw);
+        for (int k = 0; k < nw; k++) {
+            wp[k] = make_double4(s->wall_point[3 * k], s->wall_point[3 * k + 1], s->wall_point[3 * k + 2], 0);
+            wnv[k] = make_double4(s->wall_normal[3 * k], s->wall_normal[3 * k + 1], s->wall_normal[3 * k + 2], 0);
+            wm[k] = s->wall_material ? s->wall_material[k] : 0;
+        }
+        c->wpt = dalloc<double4>(c, nw);
+        c->wn = dalloc<double4>(c, nw);
+        c->wmat = dalloc<int>(c, nw);
+        h2d(c, c->wpt, wp.data(), nw);
+        h2d(c, c->wn, wnv.data(), nw);
+        h2d(c, c->wmat, wm.data(), nw);
+        const size_t nh = 9 * (size_t)ns * nw;
+        c->whist = dalloc<double>(c, nh);
+        if (nh) HIPCHK(hipMemsetAsync(c->whist, 0, nh * sizeof(double), c->st));
+
+        // ---- contact tables
+        c->nmat = nmat;
+        c->ncomb = nmat * (nmat + 1) / 2;
+        const int ncb = c->ncomb;
+        std::vector<double> hz(8 * ncb, 0.0), li(11 * ncb, 0.0), bo(6 * ncb, 0.0);
+        for (int k = 0; k < ncb; k++) {
+            hz[2 * ncb + k] = 1.0;  // restitution default (hostStructs.h:305)
+            bo[0 * ncb + k] = 1.0;  // bond multiplier default (hostStructs.h:346)
+            bo[2 * ncb + k] = 2.0;  // kn/ks default
+        }
+        if (s->hertz) std::copy(s->hertz, s->hertz + 8 * ncb, hz.begin());
+        if (s->linear) std::copy(s->linear, s->linear + 11 * ncb, li.begin());
+        if (s->bonded) std::copy(s->bonded, s->bonded + 6 * ncb, bo.begin());
+        c->h_bonded = bo;
+        double* dhz = dalloc<double>(c, hz.size());
+        double* dli = dalloc<double>(c, li.size());
+        double* dbo = dalloc<double>(c, bo.size());
+        h2d(c, dhz, hz.data(), hz.size());
+        h2d(c, dli, li.data(), li.size());
+        h2d(c, dbo, bo.data(), bo.size());
+        c->tb = ContactTables{nmat, ncb, dhz, dli, dbo};
+
+        // ---- lists
+        Lists& L = c->L;
+        L.cnt = dalloc<int>(c, nf);
+        L.fscnt = dalloc<int>(c, nf);
+        L.fsoff = dalloc<int>(c, (size_t)nf + 1);
+        fill_int(c->st, L.fsoff, (int64_t)nf + 1, 0);
+        fill_int(c->st, L.cnt, nf, 0);
+        alloc_ff(c, nf > 0 ? 128 : 1);
+        alloc_fs(c, std::max<int64_t>(4096, 8LL * nf));
+        L.fsr_cnt = dalloc<int>(c, ns);
+        L.fsr_off = dalloc<int>(c, (size_t)ns + 1);
+        fill_int(c->st, L.fsr_off, (int64_t)ns + 1, 0);
+        L.sscnt = dalloc<int>(c, ns);
+        L.ssoff = dalloc<int>(c, (size_t)ns + 1);
+        L.hoff = dalloc<int>(c, (size_t)ns + 1);
+        L.ssr_cnt = dalloc<int>(c, ns);
+        L.ssr_off = dalloc<int>(c, (size_t)ns + 1);
+        fill_int(c->st, L.ssoff, (int64_t)ns + 1, 0);
+        fill_int(c->st, L.hoff, (int64_t)ns + 1, 0);
+        fill_int(c->st, L.ssr_off, (int64_t)ns + 1, 0);
+        alloc_ss(c, std::max<int64_t>(4096, 6LL * (int64_t)active.size() + 4096), false);
+
+        // ---- scratch
+        const int64_t nmax = std::max<int64_t>(std::max(nf, ns), 1);
+        c->radix_tmp = dalloc<unsigned>(c, radix_tmp_size(nmax));
+        int64_t sc = std::max<int64_t>(scan_tmp_size(nmax + 1), scan_tmp_size(256 * ((nmax + 1023) / 1024)));
+        c->scan_tmp = dalloc<int>(c, sc + 64);
+        c->status = dalloc<unsigned long long>(c, ST_WORDS);
+        HIPCHK(hipMemsetAsync(c->status, 0, ST_WORDS * sizeof(unsigned long long), c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        c->uploaded = true;
+        return DSPH_OK;
+    } catch (const ApiError& e) {
+        return fail(c, e);
+    }
+}
+
+int dsph_set_capacity(dsph_ctx* c, int32_t ff, int64_t fs, int64_t ss) {
+    if (!c || !c->uploaded) return DSPH_E_STATE;
+    try {
+        HIPCHK(hipStreamSynchronize(c->st));
+        if (ff > 0) alloc_ff(c, ff);
+        if (fs > 0) alloc_fs(c, fs);
+        if (ss > 0) alloc_ss(c, ss, true);
+        invalidate_graphs(c);
+        return DSPH_OK;
+    } catch (const ApiError& e) {
+        return fail(c, e);
+    }
+}
+
+int dsph_initialize(dsph_ctx* c, int compute_normals) {
+    if (!c || !c->uploaded) return DSPH_E_STATE;
+    try {
+        search_checked(c, 0, 1);
+        // addBondData, initial branch (dataBase.cpp:270-308): bonds are the
+        // initial contacts in the same cluster that touch physically
+        int64_t tot = 0;
+        {
+            int t = 0;
+            d2h(c, &t, c->L.ssoff + c->ns, 1);
+            HIPCHK(hipStreamSynchronize(c->st));
+            tot = t;
+        }
+        std::vector<int> si(tot), sj(tot);
+        d2h(c, si.data(), c->L.ss_i, tot);
+        d2h(c, sj.data(), c->L.ss_j, tot);
+        HIPCHK(hipStreamSynchronize(c->st));
+        std::vector<int> bi, bj;
+        std::vector<double4> bn, bpt;
+        const int ncb = c->ncomb;
+        for (int64_t k = 0; k < tot; k++) {
+            const int i = si[k], j = sj[k];
+            const double* pi = &c->h_spos[3 * (size_t)i];
+            const double* pj = &c->h_spos[3 * (size_t)j];
+            const double dx = pi[0] - pj[0], dy = pi[1] - pj[1], dz = pi[2] - pj[2];
+            const double d = std::sqrt(dx * dx + dy * dy + dz * dz);
+            const double ri = c->h_srad[i], rj = c->h_srad[j];
+            if (ri + rj + 1.e-10 <= d) continue;
+            const int cij = combined_index(c->nmat, c->h_smat[i], c->h_smat[j]);
+            const double Eb = c->h_bonded[1 * ncb + cij];
+            if (Eb > 1.e-10 && c->h_sclus[i] >= 0 && c->h_sclus[i] == c->h_sclus[j]) {
+                const double nx = dx / d, ny = dy / d, nz = dz / d;
+                const double delta = ri + rj - d;
+                const double a = rj - 0.5 * delta;
+                bi.push_back(i);
+                bj.push_back(j);
+                bn.push_back(make_double4(nx, ny, nz, 0));
+                bpt.push_back(make_double4(pj[0] + a * nx, pj[1] + a * ny, pj[2] + a * nz, 0));
+            }
+        }
+        const int nb = (int)bi.size();
+        c->B.n = nb;
+        c->B.i = dalloc<int>(c, nb);
+        c->B.j = dalloc<int>(c, nb);
+        c->B.on = dalloc<int>(c, nb);
+        c->B.nrm = dalloc<double4>(c, nb);
+        c->B.pt = dalloc<double4>(c, nb);
+        c->B.Fs = dalloc<double4>(c, nb);
+        c->B.Tb = dalloc<double4>(c, nb);
+        c->B.Fn = dalloc<double>(c, nb);
+        c->B.Tt = dalloc<double>(c, nb);
+        if (nb) {
+            std::vector<int> ones(nb, 1);
+            std::vector<double4> z4(nb, make_double4(0, 0, 0, 0));
+            std::vector<double> z(nb, 0.0);
+            h2d(c, c->B.i, bi.data(), nb);
+            h2d(c, c->B.j, bj.data(), nb);
+            h2d(c, c->B.on, ones.data(), nb);
+            h2d(c, c->B.nrm, bn.data(), nb);
+            h2d(c, c->B.pt, bpt.data(), nb);
+            h2d(c, c->B.Fs, z4.data(), nb);
+            h2d(c, c->B.Tb, z4.data(), nb);
+            h2d(c, c->B.Fn, z.data(), nb);
+            h2d(c, c->B.Tt, z.data(), nb);
+        }
+        if (compute_normals)
+            solid_normals(c->st, c->g, c->so, c->ns, c->scs, c->sce, c->sperm, c->hmax);
+        HIPCHK(hipStreamSynchronize(c->st));
+        invalidate_graphs(c);
+        c->initialized = true;
+        return DSPH_OK;
+    } catch (const ApiError& e) {
+        return fail(c, e);
+    }
+}
+
+#define API_GUARD(c)                                   \
+    if (!c) return DSPH_E_INVALID;                     \
+    if (!c->uploaded) return DSPH_E_STATE;
+
+int dsph_neighbor_search(dsph_ctx* c, int32_t i_step, int32_t gap) {
+    API_GUARD(c);
+    if (gap < 1) return DSPH_E_INVALID;
+    try {
+        neighbor_search(c, i_step, gap, true);
+        return DSPH_OK;
+    } catch (const ApiError& e) {
+        return fail(c, e);
+    }
+}
+
+int dsph_solid_before_contact(dsph_ctx* c, double dt) {
+    API_GUARD(c);
+    try {
+        solid_before(c, dt);
+        return DSPH_OK;
+    } catch (const ApiError& e) {
+        return fail(c, e);
+    }
+}
+
+int dsph_fluid_integrate(dsph_ctx* c, double dt, int32_t i_step, int32_t gap) {
+    API_GUARD(c);
+    if (gap < 1) return DSPH_E_INVALID;
+    try {
+        fluid_integrate(c, dt, i_step, gap);
+        return DSPH_OK;
+    } catch (const ApiError& e) {
+        return fail(c, e);
+    }
+}
+
+int dsph_contact_forces(dsph_ctx* c, double dt) {
+    API_GUARD(c);
+    try {
+        contacts(c, dt);
+        return DSPH_OK;
+    } catch (const ApiError& e) {
+        return fail(c, e);
+    }
+}
+
+int dsph_solid_after_contact(dsph_ctx* c, double dt) {
+    API_GUARD(c);
+    try {
+        solid_after(c, dt);
+        return DSPH_OK;
+    } catch (const ApiError& e) {
+        return fail(c, e);
+    }
+}
+
+int dsph_step(dsph_ctx* c, int32_t i_step, double dt, int32_t gap, int check_capacity) {
+    API_GUARD(c);
+    if (!c->initialized) return DSPH_E_STATE;
+    if (gap < 1) return DSPH_E_INVALID;
+    try {
+        if (check_capacity)
+            search_checked(c, i_step, gap);
+        else
+            neighbor_search(c, i_step, gap, true);
+        solid_before(c, dt);
+        fluid_integrate(c, dt, i_step, gap);
+        contacts(c, dt);
+        solid_after(c, dt);
+        return DSPH_OK;
+    } catch (const ApiError& e) {
+        return fail(c, e);
+    }
+}
+
+int dsph_step_n(dsph_ctx* c, int32_t i0, int32_t n, double dt, int32_t gap) {
+    API_GUARD(c);
+    if (!c->initialized) return DSPH_E_STATE;
+    if (gap < 1 || n < 0) return DSPH_E_INVALID;
+    try {
+        const bool timing = c->timing;
+        if (dt != c->graph_dt) {
+            invalidate_graphs(c);
+            c->graph_dt = dt;
+        }
+        for (int k = 0; k < n; k++) {
+            const int is = i0 + k;
+            if (timing) {  // events cannot live inside a captured graph
+                step_body(c, is, dt, gap);
+                continue;
+            }
+            const uint64_t key = step_key(c, is, gap);
+            auto it = c->graphs.find(key);
+            if (it == c->graphs.end()) {
+                StepGraph sg;
+                HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+                try {
+                    step_body(c, is, dt, gap);
+                } catch (...) {
+                    hipGraph_t tmp;
+                    (void)hipStreamEndCapture(c->st, &tmp);
+                    throw;
+                }
+                HIPCHK(hipStreamEndCapture(c->st, &sg.graph));
+                HIPCHK(hipGraphInstantiate(&sg.exec, sg.graph, nullptr, nullptr, 0));
+                sg.fcur_after = c->fcur;
+                sg.dcur_after = c->dcur;
+                c->graphs[key] = sg;
+                HIPCHK(hipGraphLaunch(sg.exec, c->st));
+            } else {
+                HIPCHK(hipGraphLaunch(it->second.exec, c->st));
+                c->fcur = it->second.fcur_after;
+                c->dcur = it->second.dcur_after;
+            }
+        }
+        unsigned long long w[ST_WORDS];
+        read_status(c, w);
+        if (w[ST_FF_OVERFLOW] | w[ST_FS_OVERFLOW] | w[ST_SS_OVERFLOW]) {
+            c->err = "neighbor list overflow during dsph_step_n; enlarge capacities (dsph_set_capacity) and rerun";
+            return DSPH_E_CAPACITY;
+        }
+        return DSPH_OK;
+    } catch (const ApiError& e) {
+        return fail(c, e);
+    }
+}
+
+int dsph_synchronize(dsph_ctx* c) {
+    if (!c) return DSPH_E_INVALID;
+    return hipStreamSynchronize(c->st) == hipSuccess ? DSPH_OK : DSPH_E_HIP;
+}
+
+int dsph_add_external_force(dsph_ctx* c, int32_t idx, const double f[3]) {
+    API_GUARD(c);
+    if (idx < 0 || idx >= c->ns || !f) return DSPH_E_INVALID;
+    try {
+        dem_external_force(c->st, dem_args(c), idx, f[0], f[1], f[2]);
+        return DSPH_OK;
+    } catch (const ApiError& e) {
+        return fail(c, e);
+    }
+}
+
+int dsph_add_global_damping(dsph_ctx* c, int32_t idx, double cd) {
+    API_GUARD(c);
+    if (idx < 0 || idx >= c->ns) return DSPH_E_INVALID;
+    try {
+        dem_global_damping(c->st, dem_args(c), idx, cd);
+        return DSPH_OK;
+    } catch (const ApiError& e) {
+        return fail(c, e);
+    }
+}
+
+// ------------------------------------------------------------- read-back
+int dsph_download_fluid(dsph_ctx* c, double* pos, double* vel, double* acc, double* drho,
+                        double* sumw0) {
+    API_GUARD(c);
+    try {
+        const int n = c->nf;
+        const FluidSet& f = c->fset[c->fcur];
+        std::vector<double4> ph(n), vr(n), ac(n);
+        std::vector<double> dr(n), sw(n);
+        std::vector<int> orig(n);
+        d2h(c, ph.data(), f.ph, n);
+        d2h(c, vr.data(), f.vr, n);
+        d2h(c, ac.data(), f.acc, n);
+        d2h(c, dr.data(), f.drho[c->dcur], n);
+        d2h(c, sw.data(), f.sumw0, n);
+        d2h(c, orig.data(), f.orig, n);
+        HIPCHK(hipStreamSynchronize(c->st));
+        for (int s = 0; s < n; s++) {
+            const int i = orig[s];
+            if (pos) { pos[3 * i] = ph[s].x; pos[3 * i + 1] = ph[s].y; pos[3 * i + 2] = ph[s].z; }
+            if (vel) { vel[3 * i] = vr[s].x; vel[3 * i + 1] = vr[s].y; vel[3 * i + 2] = vr[s].z; }
+            if (acc) { acc[3 * i] = ac[s].x; acc[3 * i + 1] = ac[s].y; acc[3 * i + 2] = ac[s].z; }
+            if (drho) drho[i] = dr[s];
+            if (sumw0) sumw0[i] = sw[s];
+        }
+        return DSPH_OK;
+    } catch (const ApiError& e) {
+        return fail(c, e);
+    }
+}
+
+static void put4(double* dst, const std::vector<double4>& v) {
+    if (!dst) return;
+    for (size_t i = 0; i < v.size(); i++) {
+        dst[3 * i] = v[i].x;
+        dst[3 * i + 1] = v[i].y;
+        dst[3 * i + 2] = v[i].z;
+    }
+}
+
+int dsph_download_solid(dsph_ctx* c, double* pos, double* vel, double* acc, double* angvel,
+                        double* torque, double* normal) {
+    API_GUARD(c);
+    try {
+        const int n = c->ns;
+        std::vector<double4> a(n), b(n), d(n), e(n), f(n), g(n);
+        d2h(c, a.data(), c->so.pr, n);
+        d2h(c, b.data(), c->so.vel, n);
+        d2h(c, d.data(), c->so.acc, n);
+        d2h(c, e.data(), c->so.w, n);
+        d2h(c, f.data(), c->so.tq, n);
+        d2h(c, g.data(), c->so.nrm, n);
+        HIPCHK(hipStreamSynchronize(c->st));
+        put4(pos, a);
+        put4(vel, b);
+        put4(acc, d);
+        put4(angvel, e);
+        put4(torque, f);
+        put4(normal, g);
+        return DSPH_OK;
+    } catch (const ApiError& e) {
+        return fail(c, e);
+    }
+}
+
+int dsph_download_clump(dsph_ctx* c, double* centroid, double* vel, double* acc, double* angvel,
+                        double* torque, double* orientation) {
+    API_GUARD(c);
+    try {
+        const int n = c->nc;
+        std::vector<double4> a(n), b(n), d(n), e(n), f(n), q(n);
+        d2h(c, a.data(), c->cl.pos, n);
+        d2h(c, b.data(), c->cl.vel, n);
+        d2h(c, d.data(), c->cl.acc, n);
+        d2h(c, e.data(), c->cl.w, n);
+        d2h(c, f.data(), c->cl.tq, n);
+        d2h(c, q.data(), c->cl.q, n);
+        HIPCHK(hipStreamSynchronize(c->st));
+        put4(centroid, a);
+        put4(vel, b);
+        put4(acc, d);
+        put4(angvel, e);
+        put4(torque, f);
+        if (orientation)
+            for (int k = 0; k < n; k++) {
+                orientation[4 * k] = q[k].x;
+                orientation[4 * k + 1] = q[k].y;
+                orientation[4 * k + 2] = q[k].z;
+                orientation[4 * k + 3] = q[k].w;
+            }
+        return DSPH_OK;
+    } catch (const ApiError& e) {
+        return fail(c, e);
+    }
+}
+
+
+int dsph_download_fluid_solid_force(dsph_ctx* c, double* force, int64_t cap, int64_t* n_out) {
+    API_GUARD(c);
+    try {
+        std::vector<int> fi, sa;
+        std::vector<int64_t> ids = fs_reference_order(c, fi, sa, nullptr);
+        if (n_out) *n_out = (int64_t)ids.size();
+        if (!force) return DSPH_OK;
+        if (cap < (int64_t)ids.size()) return DSPH_E_INVALID;
+        int64_t tot = ids.size();
+        std::vector<double4> F(tot);
+        d2h(c, F.data(), c->L.fsF, tot);
+        HIPCHK(hipStreamSynchronize(c->st));
+        for (size_t k = 0; k < ids.size(); k++) {
+            const double4 f = F[ids[k]];
+            force[3 * k] = f.x;
+            force[3 * k + 1] = f.y;
+            force[3 * k + 2] = f.z;
+        }
+        return DSPH_OK;
+    } catch (const ApiError& e) {
+        return fail(c, e);
+    }
+}
+
+int dsph_download_contacts(dsph_ctx* c, double* force, double* torque, double* sliding,
+                           double* rolling, double* torsion, int64_t cap, int64_t* n_out) {
+    API_GUARD(c);
+    try {
+        int t = 0;
+        d2h(c, &t, c->L.ssoff + c->ns, 1);
+        HIPCHK(hipStreamSynchronize(c->st));
+        int64_t n = std::min<int64_t>(t, c->L.ss_cap);
+        if (n_out) *n_out = n;
+        if ((force || torque || sliding || rolling || torsion) && cap < n) return DSPH_E_INVALID;
+        std::vector<double4> F(n), T(n);
+        std::vector<double> S(9 * n);
+        d2h(c, F.data(), c->L.ssF, n);
+        d2h(c, T.data(), c->L.ssT, n);
+        d2h(c, S.data(), c->L.ssS, 9 * n);
+        HIPCHK(hipStreamSynchronize(c->st));
+        put4(force, F);
+        put4(torque, T);
+        for (int64_t k = 0; k < n; k++)
+            for (int m = 0; m < 3; m++) {
+                if (sliding) sliding[3 * k + m] = S[9 * k + m];
+                if (rolling) rolling[3 * k + m] = S[9 * k + 3 + m];
+                if (torsion) torsion[3 * k + m] = S[9 * k + 6 + m];
+            }
+        return DSPH_OK;
+    } catch (const ApiError& e) {
+        return fail(c, e);
+    }
+}
+
+int dsph_download_bonds(dsph_ctx* c, double* fn, double* tt, double* shear, double* bending,
+                        int32_t* on, int64_t cap, int64_t* n_out) {
+    API_GUARD(c);
+    try {
+        const int n = c->B.n;
+        if (n_out) *n_out = n;
+        if ((fn || tt || shear || bending || on) && cap < n) return DSPH_E_INVALID;
+        std::vector<double> a(n), b(n);
+        std::vector<double4> s(n), t(n);
+        std::vector<int> o(n);
+        d2h(c, a.data(), c->B.Fn, n);
+        d2h(c, b.data(), c->B.Tt, n);
+        d2h(c, s.data(), c->B.Fs, n);
+        d2h(c, t.data(), c->B.Tb, n);
+        d2h(c, o.data(), c->B.on, n);
+        HIPCHK(hipStreamSynchronize(c->st));
+        for (int k = 0; k < n; k++) {
+            if (fn) fn[k] = a[k];
+            if (tt) tt[k] = b[k];
+            if (on) on[k] = o[k];
+        }
+        put4(shear, s);
+        put4(bending, t);
+        return DSPH_OK;
+    } catch (const ApiError& e) {
+        return fail(c, e);
+    }
+}
+
+int dsph_download_list(dsph_ctx* c, int kind, int32_t* pointed, int32_t* pointing, int64_t cap,
+                       int64_t* n_out, int32_t* count) {
+    API_GUARD(c);
+    try {
+        std::vector<int> A, Bv, cnt;
+        if (kind == DSPH_LIST_FLUID_FLUID) {
+            const int nf = c->nf, K = c->L.K;
+            std::vector<int> nbr((size_t)K * nf), cn(nf), orig(nf);
+            d2h(c, nbr.data(), c->L.nbr, (size_t)K * nf);
+            d2h(c, cn.data(), c->L.cnt, nf);
+            d2h(c, orig.data(), c->fset[c->fcur].orig, nf);
+            HIPCHK(hipStreamSynchronize(c->st));
+            std::vector<int> slot_of(nf);
+            for (int s = 0; s < nf; s++) slot_of[orig[s]] = s;
+            cnt.assign(nf, 0);
+            for (int i = 0; i < nf; i++) {
+                const int s = slot_of[i];
+                if (cn[s] > K) throw ApiError{DSPH_E_CAPACITY, "fluid-fluid list overflowed"};
+                for (int k = 0; k < cn[s]; k++) {
+                    const int jo = orig[nbr[(size_t)k * nf + s]];
+                    if (jo > i) {  // the reference keeps the j > i half
+                        A.push_back(i);
+                        Bv.push_back(jo);
+                        cnt[i]++;
+                    }
+                }
+            }
+        } else if (kind == DSPH_LIST_FLUID_SOLID) {
+            fs_reference_order(c, A, Bv, &cnt);
+        } else if (kind == DSPH_LIST_SOLID_SOLID) {
+            int t = 0;
+            d2h(c, &t, c->L.ssoff + c->ns, 1);
+            HIPCHK(hipStreamSynchronize(c->st));
+            if (t > c->L.ss_cap) throw ApiError{DSPH_E_CAPACITY, "solid-solid list overflowed"};
+            A.resize(t);
+            Bv.resize(t);
+            cnt.resize(c->ns);
+            d2h(c, A.data(), c->L.ss_i, t);
+            d2h(c, Bv.data(), c->L.ss_j, t);
+            d2h(c, cnt.data(), c->L.sscnt, c->ns);
+            HIPCHK(hipStreamSynchronize(c->st));
+        } else if (kind == DSPH_LIST_BOND) {
+            A.resize(c->B.n);
+            Bv.resize(c->B.n);
+            d2h(c, A.data(), c->B.i, c->B.n);
+            d2h(c, Bv.data(), c->B.j, c->B.n);
+            HIPCHK(hipStreamSynchronize(c->st));
+        } else {
+            return DSPH_E_INVALID;
+        }
+        const int64_t n = (int64_t)A.size();
+        if (n_out) *n_out = n;
+        if ((pointed || pointing) && cap < n) return DSPH_E_INVALID;
+        for (int64_t k = 0; k < n; k++) {
+            if (pointed) pointed[k] = A[k];
+            if (pointing) pointing[k] = Bv[k];
+        }
+        if (count)
+            for (size_t i = 0; i < cnt.size(); i++) count[i] = cnt[i];
+        return DSPH_OK;
+    } catch (const ApiError& e) {
+        return fail(c, e);
+    }
+}
+
+int dsph_grid_info(dsph_ctx* c, int32_t* gs, double* cs, int32_t* n_cells) {
+    API_GUARD(c);
+    if (gs) { gs[0] = c->g.gx; gs[1] = c->g.gy; gs[2] = c->g.gz; }
+    if (cs) { cs[0] = c->g.csx; cs[1] = c->g.csy; cs[2] = c->g.csz; }
+    if (n_cells) *n_cells = c->g.ncell;
+    return DSPH_OK;
+}
+
+int dsph_download_grid(dsph_ctx* c, int species, int32_t* hash, int32_t* perm, int32_t* start,
+                       int32_t* end) {
+    API_GUARD(c);
+    if (species != 0 && species != 1) return DSPH_E_INVALID;
+    try {
+        const int n = species == 0 ? c->nf : c->ns;
+        if (hash) d2h(c, (unsigned*)hash, species == 0 ? c->fkey : c->skey, n);
+        if (perm) d2h(c, perm, species == 0 ? c->fperm : c->sperm, n);
+        if (start) d2h(c, start, species == 0 ? c->fcs : c->scs, c->g.ncell);
+        if (end) d2h(c, end, species == 0 ? c->fce : c->sce, c->g.ncell);
+        HIPCHK(hipStreamSynchronize(c->st));
+        return DSPH_OK;
+    } catch (const ApiError& e) {
+        return fail(c, e);
+    }
+}
+
+int dsph_status(dsph_ctx* c, int64_t words[DSPH_STATUS_WORDS]) {
+    API_GUARD(c);
+    try {
+        unsigned long long w[ST_WORDS];
+        read_status(c, w);
+        for (int k = 0; k < ST_WORDS; k++) words[k] = (int64_t)w[k];
+        return DSPH_OK;
+    } catch (const ApiError& e) {
+        return fail(c, e);
+    }
+}
+
+int dsph_set_timing(dsph_ctx* c, int enabled) {
+    if (!c) return DSPH_E_INVALID;
+    c->timing = enabled != 0;
+    for (int f = 0; f < F_NFAM; f++) c->ev_used[f] = 0;
+    return DSPH_OK;
+}
+
+int dsph_kernel_times(dsph_ctx* c, char* names, int64_t names_cap, double* ms, int32_t cap) {
+    if (!c) return DSPH_E_INVALID;
+    if (hipStreamSynchronize(c->st) != hipSuccess) return DSPH_E_HIP;
+    std::string all;
+    int m = 0;
+    for (int f = 0; f < F_NFAM && m < cap; f++) {
+        double tot = 0;
+        for (size_t k = 0; k < c->ev_used[f]; k++) {
+            float t = 0;
+            if (hipEventElapsedTime(&t, c->ev[f][k].first, c->ev[f][k].second) == hipSuccess) tot += t;
+        }
+        // per family: total ms and launch count
+        ms[2 * m] = tot;
+        ms[2 * m + 1] = (double)c->ev_used[f];
+        all += kFamNames[f];
+        all.push_back('\0');
+        m++;
+    }
+    if (names && names_cap > 0) {
+        int64_t n = std::min<int64_t>((int64_t)all.size(), names_cap);
+        memcpy(names, all.data(), n);
+    }
+    for (int f = 0; f < F_NFAM; f++) c->ev_used[f] = 0;
+    return m;
+}
+
+}  // extern "C"

@@ -1,0 +1,227 @@
+// dsph_internal.h — device data layout and kernel-launch interface shared by
+// the HIP translation units of libdsph.  Not part of the public ABI.
+//
+// Layout in HBM (all per-particle arrays are SoA of 16/32-byte records so a
+// neighbour gather is one aligned vector load):
+//   fluid  — kept in cell-sorted ("slot") order, two copies ping-ponged on
+//            reorder: ph (x,y,z,h) f64x4, vr (vx,vy,vz,rho0) f64x4,
+//            drho f64 (Jacobi ping-pong), cn (c,nu) f64x2, sumw0 f64, acc f64x4,
+//            orig (slot -> caller index) i32; FP32 shadows for the mixed
+//            pair loops: vr32 (vx,vy,vz,rho0), drho32, p32 (c,nu,m,h).
+//   solids — caller order: pr (x,y,z,r), vel (vx,vy,vz,1/m), acc, w, torque,
+//            normal (f64x4), eff radius f64, material/cluster/clump/dummy i32.
+//   lists  — fluid–fluid: full (both directions) neighbour list in ELL
+//            column-major layout nbr[k*NF + slot]; fluid–solid and
+//            solid–solid: CSR by pointed particle with device-side totals.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "dsph_math.h"
+
+namespace dsph {
+
+// status words (mirrors include/dsph.h)
+enum {
+    ST_FF_OVERFLOW = 0,
+    ST_FS_OVERFLOW = 1,
+    ST_SS_OVERFLOW = 2,
+    ST_OUT_OF_DOMAIN = 3,
+    ST_FF_MAX = 4,
+    ST_FS_TOTAL = 5,
+    ST_SS_TOTAL = 6,
+    ST_WORDS = 8
+};
+
+// the reference's spatialGrid (deviceStructs.h:976-1014)
+struct GridP {
+    double minx, miny, minz;
+    double maxx, maxy, maxz;
+    double csx, csy, csz;
+    int gx, gy, gz, ncell;
+    double pad;  // conservative enlargement of cell boxes for pruning
+};
+
+struct FluidSet {
+    double4* ph = nullptr;
+    double4* vr = nullptr;
+    double* drho[2] = {nullptr, nullptr};
+    double2* cn = nullptr;
+    double* sumw0 = nullptr;
+    double4* acc = nullptr;
+    int* orig = nullptr;
+    float4* vr32 = nullptr;
+    float* drho32[2] = {nullptr, nullptr};
+    float4* p32 = nullptr;
+};
+
+struct Solids {
+    double4* pr = nullptr;   // x, y, z, radius
+    double4* vel = nullptr;  // vx, vy, vz, inverse mass
+    double4* acc = nullptr;
+    double4* w = nullptr;
+    double4* tq = nullptr;
+    double4* nrm = nullptr;
+    double* eff = nullptr;
+    int* mat = nullptr;
+    int* clus = nullptr;
+    int* clump = nullptr;
+    int* dummy = nullptr;
+    int* active = nullptr;  // indices of non-dummy solids
+    int nactive = 0;
+};
+
+struct Clumps {
+    double4* pos = nullptr;
+    double4* vel = nullptr;  // w = inverse mass
+    double4* acc = nullptr;
+    double4* w = nullptr;
+    double4* tq = nullptr;
+    double4* q = nullptr;    // q0, q1, q2, q3
+    double* invI = nullptr;  // 6 per clump (body frame)
+    int* start = nullptr;
+    int* end = nullptr;
+};
+
+struct Lists {
+    // fluid–fluid ELL
+    int K = 0;
+    int* nbr = nullptr;
+    int* cnt = nullptr;
+    // fluid–solid CSR by fluid slot
+    int64_t fs_cap = 0;
+    int* fscnt = nullptr;
+    int* fsoff = nullptr;  // NF + 1
+    int* fs_i = nullptr;   // fluid slot
+    int* fs_a = nullptr;   // solid index
+    double4* fsF = nullptr;
+    // reverse fluid–solid for movable solids: CSR by solid
+    int* fsr_cnt = nullptr;
+    int* fsr_off = nullptr;  // NS + 1
+    int* fsr_idx = nullptr;  // fs pair ids, capacity fs_cap
+    // solid–solid CSR by solid index (the reference's own list)
+    int64_t ss_cap = 0;
+    int* sscnt = nullptr;
+    int* ssoff = nullptr;  // NS + 1
+    int* ss_i = nullptr;
+    int* ss_j = nullptr;
+    double4* ssF = nullptr;
+    double4* ssT = nullptr;
+    double* ssS = nullptr;  // 9 per contact: sliding, rolling, torsion
+    // previous step's list (spring history)
+    int* hoff = nullptr;    // NS + 1
+    int* h_j = nullptr;
+    double* hS = nullptr;
+    // reverse solid–solid: CSR by pointing solid
+    int* ssr_cnt = nullptr;
+    int* ssr_off = nullptr;
+    int* ssr_idx = nullptr;
+};
+
+struct Bonds {
+    int n = 0;
+    int* i = nullptr;
+    int* j = nullptr;
+    int* on = nullptr;
+    double4* nrm = nullptr;
+    double4* pt = nullptr;
+    double4* Fs = nullptr;
+    double4* Tb = nullptr;
+    double* Fn = nullptr;
+    double* Tt = nullptr;
+};
+
+// ---- launch wrappers (implemented in the .hip files) -----------------------
+// generic
+void fill_int(hipStream_t s, int* p, int64_t n, int v);
+void copy_int(hipStream_t s, int* dst, const int* src, int64_t n);
+// exclusive scan of n ints into out[0..n] (out[n] = total); tmp >= scan_tmp_size(n)
+int64_t scan_tmp_size(int64_t n);
+void exclusive_scan(hipStream_t s, const int* in, int* out, int64_t n, int* tmp);
+// stable LSD radix sort of (key, index) pairs; keys < 2^bits; vals out = perm
+int64_t radix_tmp_size(int64_t n);
+void radix_sort_iota(hipStream_t s, const unsigned* keys_in, int64_t n, int bits,
+                     unsigned* keys_out, int* perm_out, unsigned* tmp);
+void cell_spans(hipStream_t s, const unsigned* sorted, int64_t n, int* start, int* end,
+                int ncell);
+
+// search (k_search.hip)
+void hash_fluid(hipStream_t s, const GridP& g, const double4* ph, const int* orig, int n,
+                unsigned* key_orig, long long* status);
+void hash_solid(hipStream_t s, const GridP& g, const double4* pr, int n, unsigned* key,
+                long long* status);
+void reorder_fluid(hipStream_t s, const FluidSet& src, const FluidSet& dst, const int* perm,
+                   int* slot_of, int* member, int n, bool mixed, int cur);
+void member_slots(hipStream_t s, const int* perm, const int* slot_of, int* member, int n);
+void fluid_search(hipStream_t s, const GridP& g, const FluidSet& f, int nf, const int* fcs,
+                  const int* fce, const int* fmember, const int* scs, const int* sce,
+                  const int* sperm, const Solids& so, double hmax, Lists& L, long long* status);
+void fluid_solid_fill(hipStream_t s, const GridP& g, const FluidSet& f, int nf,
+                      const int* scs, const int* sce, const int* sperm, const Solids& so,
+                      Lists& L, long long* status);
+void reverse_movable(hipStream_t s, const int* pair_key, const int* pair_off_total, int64_t cap,
+                     const double4* key_vel, int nkeys, int* cnt, int* off, int* idx,
+                     int* scan_tmp);
+void solid_save_history(hipStream_t s, Lists& L, int ns);
+void solid_search(hipStream_t s, const GridP& g, const Solids& so, int ns, const int* scs,
+                  const int* sce, const int* sperm, Lists& L, int* scan_tmp, long long* status);
+void reverse_solid(hipStream_t s, Lists& L, int ns, int* scan_tmp);
+void solid_normals(hipStream_t s, const GridP& g, const Solids& so, int ns, const int* scs,
+                   const int* sce, const int* sperm, double h);
+
+// fluid (k_fluid.hip)
+struct FluidArgs {
+    FluidSet f;
+    int nf;
+    int cur;  // current drho buffer
+    Lists L;
+    Solids so;
+    double gx, gy, gz;
+    uint32_t flags;
+};
+void density_reinit(hipStream_t s, const FluidArgs& a, int i_step, bool mixed);
+void continuity(hipStream_t s, const FluidArgs& a, double dt, bool mixed);  // writes 1-cur
+void fluid_drift(hipStream_t s, const FluidArgs& a, double dt);
+void momentum(hipStream_t s, const FluidArgs& a, bool mixed, bool write_fs_force);
+void fluid_solid_force(hipStream_t s, const FluidArgs& a, bool mixed);
+void fluid_kick_drift(hipStream_t s, const FluidArgs& a, double dt, bool mixed);
+
+// DEM (k_dem.hip)
+struct DemArgs {
+    Solids so;
+    int ns;
+    Clumps c;
+    int nc;
+    Lists L;
+    Bonds b;
+    const double4* wpt;
+    const double4* wn;
+    const int* wmat;
+    int nw;
+    double* whist;  // 9 * ns * nw
+    ContactTables tb;
+    double gx, gy, gz;
+    uint32_t flags;
+};
+void dem_contacts(hipStream_t s, const DemArgs& a, double dt);
+void dem_bonds(hipStream_t s, const DemArgs& a, double dt);
+void dem_solid_sum(hipStream_t s, const DemArgs& a);
+void dem_walls(hipStream_t s, const DemArgs& a, double dt);
+void dem_clump_sum(hipStream_t s, const DemArgs& a);
+void dem_solid_kick(hipStream_t s, const DemArgs& a, double dt);
+void dem_solid_drift(hipStream_t s, const DemArgs& a, double dt);
+void dem_clump_kick(hipStream_t s, const DemArgs& a, double dt);
+void dem_clump_drift(hipStream_t s, const DemArgs& a, double dt);
+void dem_external_force(hipStream_t s, const DemArgs& a, int idx, double fx, double fy, double fz);
+void dem_global_damping(hipStream_t s, const DemArgs& a, int idx, double cd);
+
+inline int blocks_for(int64_t n, int bs = 256) {
+    int64_t b = (n + bs - 1) / bs;
+    return (int)(b < 1 ? 1 : b);
+}
+
+}  // namespace dsph

@@ -1,0 +1,283 @@
+"""ctypes binding of libdsph.so — the MI355X DEM–SPH hot path behind include/dsph.h.
+
+``Solver`` mirrors the reference's ``solverBase`` driving loop
+(core/solverBase.cpp:3-67): upload a scene (dataBase::buildDeviceData),
+``initialize`` (first neighbour search, bonds, solid normals), then ``step``
+per timestep (neighborSearch -> solidIntegrateBeforeContact -> fluidIntegrate
+-> calSolidContactAfterFluidIntegrate -> solidIntegrateAfterContact).
+
+There is no fallback: if the HIP library is not built or no GPU is present,
+constructing a ``Solver`` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from dsph_scene import CScene, Scene  # noqa: F401  (re-exported)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdsph.so")
+
+PREC_FP64, PREC_MIXED = 0, 1
+FLAG_WALL_CONTINUE, FLAG_CLUMP_QUATERNION, FLAG_ART_VISCOSITY = 1, 2, 4
+LIST_FLUID_FLUID, LIST_FLUID_SOLID, LIST_SOLID_SOLID, LIST_BOND = 0, 1, 2, 3
+ST_FF_OVERFLOW, ST_FS_OVERFLOW, ST_SS_OVERFLOW, ST_OUT_OF_DOMAIN = 0, 1, 2, 3
+ST_FF_MAX, ST_FS_TOTAL, ST_SS_TOTAL = 4, 5, 6
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_i64p = C.POINTER(C.c_int64)
+_vp = C.c_void_p
+
+# name -> argument types after the context pointer (all return int)
+_SIGS = {
+    "dsph_upload_scene": [C.POINTER(CScene)],
+    "dsph_set_capacity": [C.c_int32, C.c_int64, C.c_int64],
+    "dsph_set_flags": [C.c_uint32],
+    "dsph_set_gravity": [_dp],
+    "dsph_set_reorder_period": [C.c_int32],
+    "dsph_initialize": [C.c_int],
+    "dsph_neighbor_search": [C.c_int32, C.c_int32],
+    "dsph_solid_before_contact": [C.c_double],
+    "dsph_fluid_integrate": [C.c_double, C.c_int32, C.c_int32],
+    "dsph_contact_forces": [C.c_double],
+    "dsph_solid_after_contact": [C.c_double],
+    "dsph_step": [C.c_int32, C.c_double, C.c_int32, C.c_int],
+    "dsph_step_n": [C.c_int32, C.c_int32, C.c_double, C.c_int32],
+    "dsph_synchronize": [],
+    "dsph_add_external_force": [C.c_int32, _dp],
+    "dsph_add_global_damping": [C.c_int32, C.c_double],
+    "dsph_download_fluid": [_dp] * 5,
+    "dsph_download_solid": [_dp] * 6,
+    "dsph_download_clump": [_dp] * 6,
+    "dsph_download_fluid_solid_force": [_dp, C.c_int64, _i64p],
+    "dsph_download_contacts": [_dp] * 5 + [C.c_int64, _i64p],
+    "dsph_download_bonds": [_dp] * 4 + [_ip, C.c_int64, _i64p],
+    "dsph_download_list": [C.c_int, _ip, _ip, C.c_int64, _i64p, _ip],
+    "dsph_grid_info": [_ip, _dp, _ip],
+    "dsph_download_grid": [C.c_int, _ip, _ip, _ip, _ip],
+    "dsph_status": [_i64p],
+    "dsph_set_timing": [C.c_int],
+    "dsph_kernel_times": [C.c_char_p, C.c_int64, _dp, C.c_int32],
+}
+
+EXPORTS = ["dsph_create", "dsph_destroy", "dsph_last_error", "dsph_version"] + list(_SIGS)
+
+
+class DsphError(RuntimeError):
+    def __init__(self, fn, code, msg):
+        super().__init__(f"{fn} failed with code {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libdsph.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"libdsph.so is not built ({path}); run `make -C dem-sphsource_amd`")
+    lib = C.CDLL(path)
+    lib.dsph_create.argtypes = [C.c_int, C.c_int, C.POINTER(_vp)]
+    lib.dsph_create.restype = C.c_int
+    lib.dsph_destroy.argtypes = [_vp]
+    lib.dsph_destroy.restype = C.c_int
+    lib.dsph_last_error.argtypes = [_vp]
+    lib.dsph_last_error.restype = C.c_char_p
+    lib.dsph_version.restype = C.c_char_p
+    for name, args in _SIGS.items():
+        f = getattr(lib, name)
+        f.argtypes = [_vp] + args
+        f.restype = C.c_int
+    _lib = lib
+    return lib
+
+
+def _p(a, t=_dp):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+class Solver:
+    """One device context running one scene."""
+
+    def __init__(self, precision: int = PREC_MIXED, device: int = 0):
+        self.L = load_library()
+        h = _vp()
+        rc = self.L.dsph_create(device, precision, C.byref(h))
+        if rc != 0:
+            raise DsphError("dsph_create", rc, "no usable HIP device" if rc == -6 else "")
+        self.h = h
+        self.precision = precision
+        self.nf = self.ns = self.nc = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.dsph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _call(self, name, *args):
+        rc = getattr(self.L, name)(self.h, *args)
+        if rc < 0:
+            raise DsphError(name, rc, (self.L.dsph_last_error(self.h) or b"").decode())
+        return rc
+
+    # ---- set-up
+    def upload(self, scene: Scene):
+        self.scene = scene
+        cs = scene.cstruct()
+        self._call("dsph_upload_scene", C.byref(cs))
+        self.nf, self.ns, self.nc = scene.n_fluid, scene.n_solid, scene.n_clump
+
+    def set_flags(self, flags):
+        self._call("dsph_set_flags", flags)
+
+    def set_capacity(self, ff=0, fs=0, ss=0):
+        self._call("dsph_set_capacity", ff, fs, ss)
+
+    def set_reorder_period(self, p):
+        self._call("dsph_set_reorder_period", p)
+
+    def initialize(self, compute_normals=True):
+        self._call("dsph_initialize", int(compute_normals))
+
+    # ---- phases
+    def neighbor_search(self, i_step, gap=1):
+        self._call("dsph_neighbor_search", i_step, gap)
+
+    def solid_before_contact(self, dt):
+        self._call("dsph_solid_before_contact", dt)
+
+    def fluid_integrate(self, dt, i_step, gap=1):
+        self._call("dsph_fluid_integrate", dt, i_step, gap)
+
+    def contact_forces(self, dt):
+        self._call("dsph_contact_forces", dt)
+
+    def solid_after_contact(self, dt):
+        self._call("dsph_solid_after_contact", dt)
+
+    def step(self, i_step, dt, gap=1, check_capacity=True):
+        self._call("dsph_step", i_step, dt, gap, int(check_capacity))
+
+    def step_n(self, i_step0, n, dt, gap=1):
+        self._call("dsph_step_n", i_step0, n, dt, gap)
+
+    def run(self, n_steps, dt, gap=1, first=1, graphs=False):
+        if graphs:
+            self.step_n(first, n_steps, dt, gap)
+        else:
+            for s in range(first, first + n_steps):
+                self.step(s, dt, gap)
+
+    def synchronize(self):
+        self._call("dsph_synchronize")
+
+    def add_external_force(self, idx, f):
+        self._call("dsph_add_external_force", idx, _p(np.asarray(f, float)))
+
+    def add_global_damping(self, idx, cd):
+        self._call("dsph_add_global_damping", idx, cd)
+
+    # ---- read-back (caller's original order)
+    def fluid(self):
+        n = self.nf
+        pos, vel, acc = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+        drho, sw = np.zeros(n), np.zeros(n)
+        self._call("dsph_download_fluid", _p(pos), _p(vel), _p(acc), _p(drho), _p(sw))
+        return dict(pos=pos, vel=vel, acc=acc, drho=drho, sumw0=sw)
+
+    def solid(self):
+        n = self.ns
+        keys = ("pos", "vel", "acc", "angvel", "torque", "normal")
+        out = {k: np.zeros((n, 3)) for k in keys}
+        self._call("dsph_download_solid", *[_p(out[k]) for k in keys])
+        return out
+
+    def clump(self):
+        n = self.nc
+        keys = ("centroid", "vel", "acc", "angvel", "torque")
+        out = {k: np.zeros((n, 3)) for k in keys}
+        out["orientation"] = np.zeros((n, 4))
+        self._call("dsph_download_clump", *[_p(out[k]) for k in keys], _p(out["orientation"]))
+        return out
+
+    def list(self, kind):
+        n = C.c_int64(0)
+        self._call("dsph_download_list", kind, None, None, 0, C.byref(n), None)
+        m = n.value
+        a, b = np.zeros(max(m, 1), np.int32), np.zeros(max(m, 1), np.int32)
+        cnt_len = self.nf if kind in (0, 1) else self.ns
+        cnt = np.zeros(max(cnt_len, 1), np.int32)
+        self._call("dsph_download_list", kind, _p(a, _ip), _p(b, _ip), m, C.byref(n),
+                   _p(cnt, _ip) if kind != LIST_BOND else None)
+        return a[:m], b[:m], cnt[:cnt_len]
+
+    def fs_force(self):
+        n = C.c_int64(0)
+        self._call("dsph_download_fluid_solid_force", None, 0, C.byref(n))
+        f = np.zeros((max(n.value, 1), 3))
+        self._call("dsph_download_fluid_solid_force", _p(f), n.value, C.byref(n))
+        return f[: n.value]
+
+    def contacts(self):
+        n = C.c_int64(0)
+        self._call("dsph_download_contacts", None, None, None, None, None, 0, C.byref(n))
+        m = n.value
+        keys = ("force", "torque", "sliding", "rolling", "torsion")
+        out = {k: np.zeros((max(m, 1), 3)) for k in keys}
+        self._call("dsph_download_contacts", *[_p(out[k]) for k in keys], m, C.byref(n))
+        return {k: v[:m] for k, v in out.items()}
+
+    def bonds(self):
+        n = C.c_int64(0)
+        self._call("dsph_download_bonds", None, None, None, None, None, 0, C.byref(n))
+        m = n.value
+        fn, tt = np.zeros(max(m, 1)), np.zeros(max(m, 1))
+        fs, tb = np.zeros((max(m, 1), 3)), np.zeros((max(m, 1), 3))
+        on = np.zeros(max(m, 1), np.int32)
+        self._call("dsph_download_bonds", _p(fn), _p(tt), _p(fs), _p(tb), _p(on, _ip), m, C.byref(n))
+        return dict(normal_force=fn[:m], torsion=tt[:m], shear=fs[:m], bending=tb[:m],
+                    is_bonded=on[:m])
+
+    def grid_info(self):
+        gs = np.zeros(3, np.int32)
+        cs = np.zeros(3)
+        nc = C.c_int32(0)
+        self._call("dsph_grid_info", _p(gs, _ip), _p(cs), C.byref(nc))
+        return gs, cs, nc.value
+
+    def grid(self, species):
+        gs, cs, ncell = self.grid_info()
+        n = self.nf if species == 0 else self.ns
+        h, p = np.zeros(max(n, 1), np.int32), np.zeros(max(n, 1), np.int32)
+        s, e = np.zeros(ncell, np.int32), np.zeros(ncell, np.int32)
+        self._call("dsph_download_grid", species, _p(h, _ip), _p(p, _ip), _p(s, _ip), _p(e, _ip))
+        return dict(hash=h[:n], perm=p[:n], start=s, end=e)
+
+    def status(self):
+        w = np.zeros(8, np.int64)
+        self._call("dsph_status", _p(w, _i64p))
+        return w
+
+    def set_timing(self, on=True):
+        self._call("dsph_set_timing", int(on))
+
+    def kernel_times(self):
+        """{family: (total_ms, launches)} since the last call / set_timing."""
+        names = C.create_string_buffer(4096)
+        ms = np.zeros(64)
+        m = self._call("dsph_kernel_times", names, 4096, _p(ms), 32)
+        labels = names.raw.split(b"\0")[:m]
+        return {labels[k].decode(): (ms[2 * k], int(ms[2 * k + 1])) for k in range(m)}

@@ -1,0 +1,250 @@
+/*
+ * dsph.h — C ABI of the MI355X-native DEM–SPH per-timestep hot path.
+ *
+ * The reference solver (kaiqideng/DEM-SPHSource) has no FFI layer: its host
+ * classes call five phase functions that take a `DeviceData&` and launch CUDA
+ * kernels.  This header replaces that downward seam with plain C entry points
+ * (opaque context, pointers + sizes, int status codes) so that the
+ * solverBase/dataBase host API, a ctypes binding or any other FFI can drive
+ * the HIP kernels.  Each entry point names the reference interface it
+ * replaces (file:line in the reference tree).
+ *
+ * Conventions
+ *  - every function returns DSPH_OK (0) or a negative DSPH_E* code; the text
+ *    of the last error is returned by dsph_last_error(ctx);
+ *  - one host thread per context; no global mutable state; the library owns
+ *    every device buffer; host arrays passed in are copied during the call;
+ *  - particle indices crossing this boundary are always the caller's
+ *    ("original") indices: the order in which particles were uploaded, as in
+ *    the reference's HostData vectors.  The device keeps particles in its
+ *    own (cell-sorted) order internally;
+ *  - vectors are interleaved xyz doubles (the reference's double3 AoS).
+ */
+#ifndef DSPH_H
+#define DSPH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define DSPH_OK             0
+#define DSPH_E_INVALID    (-1)  /* bad argument / inconsistent scene       */
+#define DSPH_E_HIP        (-2)  /* HIP runtime error                         */
+#define DSPH_E_CAPACITY   (-3)  /* a neighbor/contact list overflowed        */
+#define DSPH_E_STATE      (-4)  /* call out of order (e.g. step before init) */
+#define DSPH_E_NOMEM      (-5)  /* device or host allocation failed          */
+#define DSPH_E_NODEVICE   (-6)  /* no HIP device / bad device index          */
+
+/* ---- precision of the fluid pair loops ------------------------------- */
+#define DSPH_PREC_FP64   0  /* everything FP64 (the reference's arithmetic)   */
+#define DSPH_PREC_MIXED  1  /* FP32 pair loops with compensated sums,
+                               FP64 integration, springs and quaternions      */
+
+/* ---- behaviour flags (reference quirks, SURVEY Q-items) -------------- */
+/* default 0 = the reference's own semantics                                 */
+#define DSPH_FLAG_WALL_CONTINUE   (1u << 0) /* wall loop skips a non-touching
+                                               wall instead of returning
+                                               (integrate.cu:535-541)        */
+#define DSPH_FLAG_CLUMP_QUATERNION (1u << 1)/* integrate clump orientation
+                                               q <- normalize(q + 1/2 q*w dt)
+                                               (myQua.h:46-55); the reference
+                                               keeps q frozen                */
+#define DSPH_FLAG_ART_VISCOSITY   (1u << 2) /* add the artificial-viscosity
+                                               term (SPHKernelDEMContactModel
+                                               .h:74-77), unused upstream   */
+
+/* ---- list kinds for dsph_download_list -------------------------------- */
+#define DSPH_LIST_FLUID_FLUID  0 /* fluid2Fluid forward half list            */
+#define DSPH_LIST_FLUID_SOLID  1 /* fluid2Solid list                         */
+#define DSPH_LIST_SOLID_SOLID  2 /* solid2Solid contact list                 */
+#define DSPH_LIST_BOND         3 /* solidBond2Solid (pointed, pointing)      */
+
+/* ---- status words (dsph_status) -------------------------------------- */
+#define DSPH_STATUS_FF_OVERFLOW   0 /* max fluid-fluid neighbors seen > cap */
+#define DSPH_STATUS_FS_OVERFLOW   1 /* fluid-solid pairs needed > capacity   */
+#define DSPH_STATUS_SS_OVERFLOW   2 /* solid-solid pairs needed > capacity   */
+#define DSPH_STATUS_OUT_OF_DOMAIN 3 /* particles hashed to the spare cell    */
+#define DSPH_STATUS_FF_MAX        4 /* max fluid-fluid neighbors of a particle*/
+#define DSPH_STATUS_FS_TOTAL      5 /* fluid-solid pairs found               */
+#define DSPH_STATUS_SS_TOTAL      6 /* solid-solid contacts found            */
+#define DSPH_STATUS_WORDS         8
+
+/*
+ * Scene upload format: the reference's HostData SoA (hostStructs.h:7-399)
+ * flattened to plain arrays.  NULL optional arrays mean "zeros" (or the
+ * reference constructor default noted).  n_comb = n_material*(n_material+1)/2
+ * and the contact tables are field-major (all E, then all G, ...).
+ */
+typedef struct dsph_scene {
+    /* dataBase::setDomain / setGravity (dataBase.h:93-109) */
+    double domain_origin[3];
+    double domain_size[3];
+    double gravity[3];
+
+    /* HostFluid (hostStructs.h:49-77), filled by dataBase::addFluid */
+    int32_t n_fluid;
+    const double* fluid_pos;    /* [3n] */
+    const double* fluid_vel;    /* [3n] */
+    const double* fluid_h;      /* [n]  smoothing length (effectiveRadii)  */
+    const double* fluid_rho0;   /* [n]                                      */
+    const double* fluid_drho;   /* [n]  optional                            */
+    const double* fluid_c;      /* [n]  speed of sound                      */
+    const double* fluid_nu;     /* [n]  kinematic viscosity                 */
+
+    /* HostSolid (hostStructs.h:79-118): walls, spheres, pebbles */
+    int32_t n_solid;
+    const double* solid_pos;        /* [3n] */
+    const double* solid_vel;        /* [3n] optional */
+    const double* solid_angvel;     /* [3n] optional */
+    const double* solid_normal;     /* [3n] optional: kept when dsph_initialize
+                                       is told not to compute normals */
+    const double* solid_radius;     /* [n]  */
+    const double* solid_eff_radius; /* [n]  search radius (2.6r wall, 1.1r DEM) */
+    const double* solid_inv_mass;   /* [n]  0 = fixed */
+    const int32_t* solid_material;  /* [n]  optional (0) */
+    const int32_t* solid_cluster;   /* [n]  optional (-1) */
+    const int32_t* solid_clump;     /* [n]  optional (-1) */
+    const int32_t* solid_dummy;     /* [n]  optional (0) */
+
+    /* HostClump (hostStructs.h:120-150) */
+    int32_t n_clump;
+    const double* clump_centroid;     /* [3n] */
+    const double* clump_vel;          /* [3n] optional */
+    const double* clump_angvel;       /* [3n] optional */
+    const double* clump_orientation;  /* [4n] optional (1,0,0,0) */
+    const double* clump_inv_inertia;  /* [6n] xx,yy,zz,xy,xz,yz (body frame) */
+    const double* clump_inv_mass;     /* [n]  */
+    const int32_t* clump_pebble_start;/* [n]  first solid index */
+    const int32_t* clump_pebble_end;  /* [n]  one past the last */
+
+    /* HostInfiniteWall (hostStructs.h:152-171) */
+    int32_t n_wall;
+    const double* wall_point;   /* [3n] */
+    const double* wall_normal;  /* [3n] */
+    const int32_t* wall_material;/* [n] optional (0) */
+
+    /* HostSolidContactModel (hostStructs.h:339-375) */
+    int32_t n_material;         /* reference default 10 (solverBase.cpp:16) */
+    const double* hertz;        /* [8*n_comb]  E,G,res,kr/ks,kt/ks,mu_s,mu_r,mu_t */
+    const double* linear;       /* [11*n_comb] kn,ks,kr,kt,dn,ds,dr,dt,mu_s,mu_r,mu_t */
+    const double* bonded;       /* [6*n_comb]  gamma,E,kn/ks,sigma_s,C,mu */
+} dsph_scene;
+
+typedef struct dsph_ctx dsph_ctx;
+
+/* ---- lifecycle ---------------------------------------------------------
+ * replaces: cudaSetDevice in solverBase::initialize (solverBase.cpp:19-26)
+ * and DeviceData ownership in dataBase (dataBase.h:26-29, 239-240). */
+int         dsph_create(int device_index, int precision, dsph_ctx** out);
+int         dsph_destroy(dsph_ctx* ctx);
+const char* dsph_last_error(const dsph_ctx* ctx);
+const char* dsph_version(void);
+
+/* ---- set-up ------------------------------------------------------------
+ * replaces: dataBase::buildDeviceData + setSpatialGrids
+ * (dataBase.cpp:415-449).  Copies the scene, sizes the hashed grid and
+ * allocates list storage (fluid-fluid neighbors per particle, fluid-solid and
+ * solid-solid totals; pass 0 for the reference defaults 40*N_f-derived).   */
+int dsph_upload_scene(dsph_ctx* ctx, const dsph_scene* scene);
+int dsph_set_capacity(dsph_ctx* ctx, int32_t ff_per_particle,
+                      int64_t fs_total, int64_t ss_total);
+int dsph_set_flags(dsph_ctx* ctx, uint32_t flags);
+int dsph_set_gravity(dsph_ctx* ctx, const double g[3]); /* dataBase.h:105-109 */
+/* sorted-order particle reorder period in fluid searches (1 = every search) */
+int dsph_set_reorder_period(dsph_ctx* ctx, int32_t period);
+
+/* replaces: solverBase::initialize lines 34-37 (solverBase.cpp:81-84):
+ * neighborSearch(dev,0,1), addBondData (dataBase.cpp:270-346) and
+ * setSolidNormal (dataBase.cpp:348-413).  Normals passed in the scene are
+ * kept when compute_normals == 0. */
+int dsph_initialize(dsph_ctx* ctx, int compute_normals);
+
+/* ---- the five phases (asynchronous on the context's stream) ------------ */
+/* neighborSearch (neighborSearch.h:28, neighborSearch.cu:314-381)          */
+int dsph_neighbor_search(dsph_ctx* ctx, int32_t i_step, int32_t gap);
+/* solidIntegrateBeforeContact (integrate.h:43, integrate.cu:964-980)       */
+int dsph_solid_before_contact(dsph_ctx* ctx, double dt);
+/* fluidIntegrate (integrate.h:19, integrate.cu:385-421)                    */
+int dsph_fluid_integrate(dsph_ctx* ctx, double dt, int32_t i_step, int32_t gap);
+/* calSolidContactAfterFluidIntegrate (integrate.h:41, integrate.cu:940-962)*/
+int dsph_contact_forces(dsph_ctx* ctx, double dt);
+/* solidIntegrateAfterContact (integrate.h:45, integrate.cu:982-998)        */
+int dsph_solid_after_contact(dsph_ctx* ctx, double dt);
+
+/* solverBase::update without the user hook and output
+ * (solverBase.cpp:47-59): the five phases in order.  check_capacity != 0
+ * synchronises after the search and grows lists that overflowed, re-running
+ * the search (the reference's setNum realloc, deviceStructs.h:483-491).   */
+int dsph_step(dsph_ctx* ctx, int32_t i_step, double dt, int32_t gap,
+              int check_capacity);
+/* n consecutive steps i_step0 .. i_step0+n-1 replayed from captured HIP
+ * graphs (no host synchronisation inside); returns DSPH_E_CAPACITY if any
+ * list overflowed during the run.  No reference counterpart (the reference
+ * syncs three times per step, neighborSearch.cu:339-340,373). */
+int dsph_step_n(dsph_ctx* ctx, int32_t i_step0, int32_t n, double dt,
+                int32_t gap);
+int dsph_synchronize(dsph_ctx* ctx);
+
+/* ---- runtime mutation (dataBase.cpp:179-224) ---------------------------- */
+int dsph_add_external_force(dsph_ctx* ctx, int32_t solid_index,
+                            const double force[3]);
+int dsph_add_global_damping(dsph_ctx* ctx, int32_t solid_index, double c_d);
+
+/* ---- read-back (synchronous), caller's original order ------------------
+ * replaces: fluid/solid/clump uploadState (deviceStructs.h:199-205,
+ * 287-297, 357-365).  Any pointer may be NULL to skip that field.          */
+int dsph_download_fluid(dsph_ctx* ctx, double* pos, double* vel,
+                        double* acc, double* drho, double* sumw0);
+int dsph_download_solid(dsph_ctx* ctx, double* pos, double* vel,
+                        double* acc, double* angvel, double* torque,
+                        double* normal);
+int dsph_download_clump(dsph_ctx* ctx, double* centroid, double* vel,
+                        double* acc, double* angvel, double* torque,
+                        double* orientation);
+/* fluid2Solid per-pair force (integrate.cu:282-330), aligned with the
+ * DSPH_LIST_FLUID_SOLID list order; n = number of pairs                    */
+int dsph_download_fluid_solid_force(dsph_ctx* ctx, double* force, int64_t cap,
+                                    int64_t* n_out);
+/* solid2Solid per-contact force/torque/springs, aligned with the
+ * DSPH_LIST_SOLID_SOLID order; any pointer may be NULL                    */
+int dsph_download_contacts(dsph_ctx* ctx, double* force, double* torque,
+                           double* sliding, double* rolling, double* torsion,
+                           int64_t cap, int64_t* n_out);
+/* bonds: normal force, torsion torque, shear force, bending torque, state  */
+int dsph_download_bonds(dsph_ctx* ctx, double* normal_force, double* torsion,
+                        double* shear, double* bending, int32_t* is_bonded,
+                        int64_t cap, int64_t* n_out);
+
+/* Lists in the reference's own order and index space: CSR by the pointed
+ * particle ascending, each segment in stencil (z,y,x) order with cell
+ * members in ascending original index (neighborSearch.cu:115-312).
+ * count[] (may be NULL) receives the per-pointed-particle counts.          */
+int dsph_download_list(dsph_ctx* ctx, int kind, int32_t* pointed,
+                       int32_t* pointing, int64_t cap, int64_t* n_out,
+                       int32_t* count);
+
+/* Hashed grid of the last search (neighborSearch.cu:19-113):
+ * hash[] per particle in original order; perm[] = stable sort permutation
+ * (sorted slot -> original index, objectHash::index); cell_start/cell_end
+ * per cell (-1 = empty), n_cells = Gx*Gy*Gz + 1.  species 0 fluid, 1 solid. */
+int dsph_grid_info(dsph_ctx* ctx, int32_t* grid_size3, double* cell_size3,
+                   int32_t* n_cells);
+int dsph_download_grid(dsph_ctx* ctx, int species, int32_t* hash,
+                       int32_t* perm, int32_t* cell_start, int32_t* cell_end);
+
+int dsph_status(dsph_ctx* ctx, int64_t words[DSPH_STATUS_WORDS]);
+
+/* profiling aid: average device time (ms) of the last dsph_step_n run per
+ * kernel family measured with HIP events on the context stream; names is a
+ * NUL-separated list.  Returns the number of entries written. */
+int dsph_kernel_times(dsph_ctx* ctx, char* names, int64_t names_cap,
+                      double* ms, int32_t cap);
+int dsph_set_timing(dsph_ctx* ctx, int enabled);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DSPH_H */

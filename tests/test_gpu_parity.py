@@ -1,0 +1,316 @@
+"""Parity of the HIP path (through the C ABI) with the FP64 CPU oracle.
+
+Bars: cell hashes, permutations, cell ranges and every neighbour / contact
+list bit-exact; one-step fields within 1e-10 (FP64 path) and 1e-5 (mixed
+path) norm-wise relative error; multi-step integrals within 1%.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import dsph
+import oracle
+import refmodel
+import scenes
+
+pytestmark = pytest.mark.gpu
+
+TOL_FP64 = 1e-10   # one-step fields, FP64 path, norm-wise relative
+TOL_MIXED = 1e-5   # one-step fields, mixed path, norm-wise relative
+TOL_INTEGRAL = 0.01
+
+
+def note(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def make_pair(scene, precision, flags=0, normals=True):
+    g = dsph.Solver(precision)
+    g.set_flags(flags)
+    g.upload(scene)
+    g.initialize(normals)
+    o = oracle.Oracle()
+    o.set_flags(flags)
+    o.upload(scene)
+    o.initialize(normals)
+    return g, o
+
+
+def small_scenes():
+    out = {}
+    out["dam_break_dx004"] = scenes.dam_break(0.04)
+    out["random_box"] = scenes.random_fluid_box(box=(0.06, 0.05, 0.05), dx=0.005, seed_stream=31,
+                                                h_jitter=0.1)
+    cb = scenes.coupled_dam_break(dx=0.02, n_dem=(4, 3, 3), dem_spacing=0.025,
+                                  dem_origin=(0.45, 0.2, 0.012), name="coupled_small")
+    out["coupled_small"] = cb
+    out["granular_small"] = scenes.granular_column(n_sphere=600, n_clump=60, n_bonded=216)
+    return out
+
+
+SCENES = small_scenes()
+
+
+# ------------------------------------------------------------------ grid
+@pytest.mark.parametrize("name", list(SCENES))
+def test_grid_bit_exact(name):
+    note(f"grid {name}")
+    sc = SCENES[name]
+    g, o = make_pair(sc, dsph.PREC_FP64)
+    gi, oi = g.grid_info(), o.grid_info()
+    assert np.array_equal(gi[0], oi[0]) and np.array_equal(gi[1], oi[1]) and gi[2] == oi[2]
+    for species in (0, 1):
+        a, b = g.grid(species), o.grid(species)
+        for k in ("hash", "perm", "start", "end"):
+            assert np.array_equal(a[k], b[k]), (species, k)
+
+
+# ----------------------------------------------------------------- lists
+def compare_lists(g, o):
+    for kind in (dsph.LIST_FLUID_FLUID, dsph.LIST_FLUID_SOLID, dsph.LIST_SOLID_SOLID,
+                 dsph.LIST_BOND):
+        ga, gb, gc = g.list(kind)
+        oa, ob, oc = o.list(kind)
+        assert np.array_equal(ga, oa), kind
+        assert np.array_equal(gb, ob), kind
+        if kind != dsph.LIST_BOND:
+            assert np.array_equal(gc, oc), kind
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+@pytest.mark.parametrize("prec", [dsph.PREC_FP64, dsph.PREC_MIXED])
+def test_lists_bit_exact_after_initialize(name, prec):
+    note(f"lists {name} prec={prec}")
+    g, o = make_pair(SCENES[name], prec)
+    compare_lists(g, o)
+
+
+def scene_with_state(sc, g):
+    """The scene with positions replaced by the GPU's current ones."""
+    import copy
+    s2 = copy.copy(sc)
+    f, s = g.fluid(), g.solid()
+    s2.fluid_pos = f["pos"]
+    s2.solid_pos = s["pos"]
+    return s2
+
+
+@pytest.mark.parametrize("name", ["random_box", "granular_small", "coupled_small"])
+def test_lists_bit_exact_on_evolved_positions(name):
+    note(f"evolved lists {name}")
+    sc = SCENES[name]
+    g = dsph.Solver(dsph.PREC_FP64)
+    g.upload(sc)
+    g.initialize()
+    for s in range(1, 31):
+        g.step(s, sc.dt)
+    g.neighbor_search(31, 1)
+    o = oracle.Oracle()
+    o.upload(scene_with_state(sc, g))
+    o.neighbor_search(31, 1)
+    for kind in (dsph.LIST_FLUID_FLUID, dsph.LIST_FLUID_SOLID, dsph.LIST_SOLID_SOLID):
+        ga, gb, _ = g.list(kind)
+        oa, ob, _ = o.list(kind)
+        assert np.array_equal(ga, oa) and np.array_equal(gb, ob), kind
+    for species in (0, 1):
+        a, b = g.grid(species), o.grid(species)
+        for k in ("hash", "perm", "start", "end"):
+            assert np.array_equal(a[k], b[k])
+
+
+# --------------------------------------------------------- one-step fields
+def fields(x):
+    f, s = x.fluid(), x.solid()
+    return dict(fpos=f["pos"], fvel=f["vel"], facc=f["acc"], drho=f["drho"], spos=s["pos"],
+                svel=s["vel"], sacc=s["acc"], sw=s["angvel"], stq=s["torque"], snrm=s["normal"])
+
+
+def compare_fields(a, b, tol, skip=()):
+    worst = {}
+    for k in a:
+        if k in skip or a[k].size == 0:
+            continue
+        e = refmodel.rel_err(a[k], b[k], floor=1e-300)
+        worst[k] = e
+    bad = {k: v for k, v in worst.items() if v > tol}
+    assert not bad, (bad, worst)
+    return worst
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+@pytest.mark.parametrize("prec,tol", [(dsph.PREC_FP64, TOL_FP64), (dsph.PREC_MIXED, TOL_MIXED)])
+def test_one_step_fields(name, prec, tol):
+    note(f"one step {name} prec={prec}")
+    sc = SCENES[name]
+    g, o = make_pair(sc, prec)
+    a, b = fields(g), fields(o)
+    compare_fields(a, b, 1e-13, skip=())  # identical initial state and normals
+    g.step(1, sc.dt)
+    o.step(1, sc.dt)
+    w = compare_fields(fields(g), fields(o), tol)
+    note(f"  worst rel err {max(w.values()):.3e}")
+    fa, fb = g.fs_force(), o.fs_force()
+    assert fa.shape == fb.shape
+    if fa.size:
+        assert refmodel.rel_err(fa, fb) < tol
+    ca, cb = g.contacts(), o.contacts()
+    for k in ca:
+        if ca[k].size:
+            assert refmodel.rel_err(ca[k], cb[k], floor=1e-300) < max(tol, 1e-9), k
+
+
+@pytest.mark.parametrize("prec", [dsph.PREC_FP64, dsph.PREC_MIXED])
+def test_sumw0_and_density_reinit(prec):
+    sc = SCENES["random_box"]
+    g, o = make_pair(sc, prec)
+    tol = TOL_FP64 if prec == dsph.PREC_FP64 else TOL_MIXED
+    for s in range(1, 6):  # step 1 stores sumW0, step 5 re-initialises
+        g.step(s, sc.dt)
+        o.step(s, sc.dt)
+    assert refmodel.rel_err(g.fluid()["sumw0"], o.fluid()["sumw0"]) < tol
+    assert refmodel.rel_err(g.fluid()["drho"], o.fluid()["drho"]) < 20 * tol
+
+
+# ----------------------------------------------------------- integrals
+def integrals(x, sc):
+    f = x.fluid()
+    m = (np.asarray(sc.fluid_h) / 1.3) ** 3 * np.asarray(sc.fluid_rho0)
+    rho = np.asarray(sc.fluid_rho0) + f["drho"]
+    ke = 0.5 * np.sum(m * np.sum(f["vel"] ** 2, axis=1))
+    pe = np.sum(m * 9.81 * f["pos"][:, 2])
+    out = dict(mass_rho=np.sum(m / rho * rho), ke=ke, pe=pe,
+               front=np.max(f["pos"][:, 0]) if len(m) else 0.0,
+               mean_rho=np.mean(rho) if len(m) else 0.0)
+    if sc.n_solid:
+        s = x.solid()
+        mov = np.asarray(sc.solid_inv_mass) > 0
+        if mov.any():
+            out["dem_centroid_z"] = float(np.mean(s["pos"][mov, 2]))
+            out["dem_centroid_x"] = float(np.mean(s["pos"][mov, 0]))
+    return out
+
+
+@pytest.mark.parametrize("name,steps", [("dam_break_dx004", 400), ("coupled_small", 300),
+                                        ("granular_small", 300)])
+@pytest.mark.parametrize("prec", [dsph.PREC_FP64, dsph.PREC_MIXED])
+def test_multistep_integrals(name, steps, prec):
+    note(f"integrals {name} {steps} steps prec={prec}")
+    sc = SCENES[name]
+    g, o = make_pair(sc, prec)
+    g.run(steps, sc.dt, graphs=True)
+    o.run(steps, sc.dt)
+    a, b = integrals(g, sc), integrals(o, sc)
+    for k in a:
+        ref = b[k]
+        err = abs(a[k] - ref) / max(abs(ref), 1e-12)
+        if k == "ke":
+            err = abs(a[k] - ref) / max(abs(ref), 1e-9 * b["pe"] if b["pe"] else 1e-12)
+        assert err < TOL_INTEGRAL, (k, a[k], ref)
+
+
+def test_dem_state_after_many_steps_fp64():
+    sc = SCENES["granular_small"]
+    g, o = make_pair(sc, dsph.PREC_FP64)
+    g.run(200, sc.dt)
+    o.run(200, sc.dt)
+    a, b = g.solid(), o.solid()
+    assert refmodel.rel_err(a["pos"], b["pos"]) < 1e-9
+    assert refmodel.rel_err(a["vel"], b["vel"]) < 1e-6
+    ba, bb = g.bonds(), o.bonds()
+    assert np.array_equal(ba["is_bonded"], bb["is_bonded"])
+    assert refmodel.rel_err(ba["normal_force"], bb["normal_force"], floor=1e-300) < 1e-6
+    ca, cb = g.clump(), o.clump()
+    assert refmodel.rel_err(ca["centroid"], cb["centroid"]) < 1e-9
+
+
+# ------------------------------------------------------------- flags
+@pytest.mark.parametrize("flags", [dsph.FLAG_WALL_CONTINUE, dsph.FLAG_CLUMP_QUATERNION,
+                                   dsph.FLAG_WALL_CONTINUE | dsph.FLAG_CLUMP_QUATERNION])
+def test_flags_match_oracle(flags):
+    sc = scenes.granular_column(n_sphere=200, n_clump=40, n_bonded=64)
+    scenes.add_infinite_wall(sc, (0.6, 0, 0), (-1, 0, 0), 0)
+    g, o = make_pair(sc, dsph.PREC_FP64, flags)
+    g.run(100, sc.dt)
+    o.run(100, sc.dt)
+    assert refmodel.rel_err(g.solid()["pos"], o.solid()["pos"]) < 1e-9
+    qa, qb = g.clump()["orientation"], o.clump()["orientation"]
+    assert np.allclose(qa, qb, atol=1e-10)
+
+
+# --------------------------------------------- graphs, determinism, capacity
+@pytest.mark.parametrize("prec", [dsph.PREC_FP64, dsph.PREC_MIXED])
+def test_graph_replay_equals_eager(prec):
+    sc = SCENES["coupled_small"]
+    a = dsph.Solver(prec)
+    a.upload(sc)
+    a.initialize()
+    b = dsph.Solver(prec)
+    b.upload(sc)
+    b.initialize()
+    a.run(23, sc.dt, graphs=False)
+    b.run(23, sc.dt, graphs=True)
+    fa, fb = fields(a), fields(b)
+    for k in fa:
+        assert np.array_equal(fa[k], fb[k]), k
+
+
+def test_bitwise_reproducible():
+    sc = SCENES["granular_small"]
+    runs = []
+    for _ in range(2):
+        g = dsph.Solver(dsph.PREC_MIXED)
+        g.upload(sc)
+        g.initialize()
+        g.run(40, sc.dt, graphs=True)
+        runs.append(fields(g))
+    for k in runs[0]:
+        assert np.array_equal(runs[0][k], runs[1][k]), k
+
+
+def test_capacity_overflow_grows_and_matches():
+    sc = SCENES["coupled_small"]
+    ref = dsph.Solver(dsph.PREC_FP64)
+    ref.upload(sc)
+    ref.initialize()
+    ref.run(5, sc.dt)
+    g = dsph.Solver(dsph.PREC_FP64)
+    g.upload(sc)
+    g.set_capacity(4, 16, 4)
+    g.initialize()
+    g.run(5, sc.dt)  # eager steps grow the lists on overflow
+    a, b = fields(g), fields(ref)
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+    h = dsph.Solver(dsph.PREC_FP64)
+    h.upload(sc)
+    h.initialize()
+    h.set_capacity(4, 0, 0)
+    with pytest.raises(dsph.DsphError) as e:
+        h.step_n(1, 2, sc.dt)
+    assert e.value.code == -3
+
+
+def test_invalid_material_rejected():
+    sc = scenes.granular_column(n_sphere=10, n_clump=0, n_bonded=0)
+    sc.solid_material = np.asarray(sc.solid_material).copy()
+    sc.solid_material[3] = sc.n_material  # out of the table
+    g = dsph.Solver(dsph.PREC_FP64)
+    with pytest.raises(dsph.DsphError) as e:
+        g.upload(sc)
+    assert e.value.code == -1
+
+
+def test_out_of_domain_particles_counted():
+    sc = scenes.random_fluid_box(box=(0.04, 0.04, 0.04), dx=0.005, seed_stream=41,
+                                 with_walls=False)
+    sc.fluid_pos = np.asarray(sc.fluid_pos).copy()
+    sc.fluid_pos[5] = [5.0, 5.0, 5.0]
+    g, o = make_pair(sc, dsph.PREC_FP64)
+    assert g.status()[dsph.ST_OUT_OF_DOMAIN] == 1
+    for species in (0,):
+        a, b = g.grid(species), o.grid(species)
+        for k in ("hash", "perm", "start", "end"):
+            assert np.array_equal(a[k], b[k])
+    compare_lists(g, o)

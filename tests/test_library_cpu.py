@@ -1,0 +1,84 @@
+"""CPU-side checks of the C ABI library: it loads, it exports every entry
+point include/dsph.h declares, and host-side argument checks work without a
+GPU (no compute calls)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import dsph
+import scenes
+from conftest import ROOT
+
+HEADER = os.path.join(ROOT, "include", "dsph.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(dsph_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_header_declares_expected_api():
+    names = declared_functions()
+    for must in ("dsph_create", "dsph_upload_scene", "dsph_initialize", "dsph_neighbor_search",
+                 "dsph_fluid_integrate", "dsph_contact_forces", "dsph_step", "dsph_step_n",
+                 "dsph_download_list", "dsph_download_grid"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = dsph.load_library()
+    out = subprocess.run(["nm", "-D", "--defined-only", dsph.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r" T (dsph_[a-z_0-9]+)", out))
+    missing = [n for n in declared_functions() if n not in exported]
+    assert not missing, missing
+    for n in declared_functions():
+        assert hasattr(lib, n)
+    assert set(dsph.EXPORTS) <= set(declared_functions())
+
+
+def test_version_and_argument_checks_without_compute():
+    lib = dsph.load_library()
+    assert b"gfx950" in lib.dsph_version()
+    h = C.c_void_p()
+    # invalid precision is rejected before any device call
+    assert lib.dsph_create(0, 7, C.byref(h)) == -1
+    # null context paths
+    assert lib.dsph_destroy(None) == 0
+    assert lib.dsph_upload_scene(None, None) == -1
+
+
+def test_scene_struct_matches_header_field_order():
+    text = open(HEADER).read()
+    body = text[text.index("typedef struct dsph_scene"):text.index("} dsph_scene;")]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    names = re.findall(r"\b([a-z_0-9]+)\s*(?:\[3\])?;", body)
+    py = [f[0] for f in dsph.CScene._fields_]
+    assert names == py
+
+
+def test_scene_builders_match_documented_sizes():
+    sc = scenes.dam_break(0.02)
+    assert (sc.n_fluid, sc.n_solid) == (8000, 32256)
+    big = scenes.coupled_dam_break(0.005)
+    assert big.n_fluid == 512000
+    assert big.n_solid == 452376 + 2000
+    assert big.n_wall == 5
+    assert abs(big.dt - 5.80e-5) < 1e-7
+    # deterministic inputs
+    again = scenes.coupled_dam_break(0.005)
+    assert np.array_equal(big.solid_pos, again.solid_pos)
+
+
+def test_lattice_is_the_reference_lattice():
+    # getRegularPackedPoints: x outer, z inner, spacing size/num, centred
+    p = scenes.regular_packed_points((0, 0, 0), (0.1, 0.2, 0.3), 0.1)
+    assert p.shape == (6, 3)
+    assert np.allclose(p[0], [0.05, 0.05, 0.05]) and np.allclose(p[1], [0.05, 0.05, 0.15])
+    one = scenes.regular_packed_points((1, 1, 1), (0.01, 0.2, 0.3), 0.1)
+    assert np.allclose(one, [[1.005, 1.1, 1.15]])
