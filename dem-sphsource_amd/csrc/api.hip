@@ -72,6 +72,10 @@ struct dsph_ctx {
     int* fperm = nullptr;
     int* fcs = nullptr;
     int* fce = nullptr;
+    int* fblk_nb = nullptr;     // blocks per cell
+    int* fblk_off = nullptr;    // ncell + 1; [ncell] = number of blocks
+    int2* fblocks = nullptr;    // (cell, first sorted position)
+    bool member_identity = false;
     // solids
     Solids so;
     unsigned* skey = nullptr;
@@ -286,21 +290,27 @@ void neighbor_search(dsph_ctx* c, int i_step, int gap, bool save_history) {
                 reorder_fluid(s, c->fset[c->fcur], c->fset[1 - c->fcur], c->fperm, c->fslot,
                               c->fmember, c->nf, c->mixed(), c->dcur);
                 c->fcur ^= 1;
+                c->member_identity = true;
             } else {
                 member_slots(s, c->fperm, c->fslot, c->fmember, c->nf);
+                c->member_identity = false;
             }
+            fluid_blocks(s, c->fcs, c->fce, c->g.ncell, c->fblk_nb, c->fblk_off, c->fblocks,
+                         c->scan_tmp);
         }
     }
     if (i_step % gap == 0 && c->nf > 0) {
+        const int* mem = c->member_identity ? nullptr : c->fmember;
+        const int* nblk = c->fblk_off + c->g.ncell;
         {
             FamTimer t(c, F_FSEARCH);
-            fluid_search(s, c->g, c->fset[c->fcur], c->nf, c->fcs, c->fce, c->fmember, c->scs,
-                         c->sce, c->sperm, c->so, c->hmax, c->L, stw);
+            fluid_search(s, c->g, c->fset[c->fcur], c->nf, c->fcs, c->fce, mem, c->scs, c->sce,
+                         c->sperm, c->so, c->hmax, c->L, c->fblocks, nblk, stw);
         }
         FamTimer t(c, F_FSLIST);
         exclusive_scan(s, c->L.fscnt, c->L.fsoff, c->nf, c->scan_tmp);
-        fluid_solid_fill(s, c->g, c->fset[c->fcur], c->nf, c->scs, c->sce, c->sperm, c->so, c->L,
-                         stw);
+        fluid_solid_fill(s, c->g, c->fset[c->fcur], c->nf, c->fcs, c->fce, mem, c->scs, c->sce,
+                         c->sperm, c->so, c->L, c->fblocks, nblk, stw);
         if (c->ns > 0)
             reverse_movable(s, c->L.fs_a, c->L.fsoff + c->nf, c->L.fs_cap, c->so.vel, c->ns,
                             c->L.fsr_cnt, c->L.fsr_off, c->L.fsr_idx, c->scan_tmp);
@@ -617,8 +627,6 @@ int dsph_upload_scene(dsph_ctx* c, const dsph_scene* s) {
         std::vector<double2> cn(nf);
         std::vector<double> drho(nf, 0.0), sumw0(nf, 0.0);
         std::vector<int> orig(nf);
-        std::vector<float4> vr32(mx ? nf : 0), p32(mx ? nf : 0);
-        std::vector<float> drho32(mx ? nf : 0);
         for (int i = 0; i < nf; i++) {
             const double* p = s->fluid_pos + 3 * i;
             ph[i] = make_double4(p[0], p[1], p[2], s->fluid_h[i]);
@@ -632,14 +640,6 @@ int dsph_upload_scene(dsph_ctx* c, const dsph_scene* s) {
             cn[i] = make_double2(s->fluid_c[i], s->fluid_nu[i]);
             if (s->fluid_drho) drho[i] = s->fluid_drho[i];
             orig[i] = i;
-            if (mx) {
-                double h13 = s->fluid_h[i] / 1.3;
-                double m = h13 * h13 * h13 * s->fluid_rho0[i];
-                vr32[i] = make_float4((float)vx, (float)vy, (float)vz, (float)s->fluid_rho0[i]);
-                p32[i] = make_float4((float)s->fluid_c[i], (float)s->fluid_nu[i], (float)m,
-                                     (float)s->fluid_h[i]);
-                drho32[i] = (float)drho[i];
-            }
         }
         for (int k = 0; k < 2; k++) {
             FluidSet& f = c->fset[k];
@@ -651,11 +651,10 @@ int dsph_upload_scene(dsph_ctx* c, const dsph_scene* s) {
             f.sumw0 = dalloc<double>(c, nf);
             f.acc = dalloc<double4>(c, nf);
             f.orig = dalloc<int>(c, nf);
-            if (mx) {
-                f.vr32 = dalloc<float4>(c, nf);
-                f.drho32[0] = dalloc<float>(c, nf);
-                f.drho32[1] = dalloc<float>(c, nf);
-                f.p32 = dalloc<float4>(c, nf);
+            const size_t rec = mx ? sizeof(float4) : sizeof(double4);
+            for (int b = 0; b < 2; b++) {
+                f.pv[b] = dalloc<char>(c, rec * std::max(nf, 1));
+                f.pq[b] = dalloc<char>(c, rec * std::max(nf, 1));
             }
         }
         {
@@ -668,12 +667,6 @@ int dsph_upload_scene(dsph_ctx* c, const dsph_scene* s) {
             h2d(c, f.sumw0, sumw0.data(), nf);
             h2d(c, f.acc, acc.data(), nf);
             h2d(c, f.orig, orig.data(), nf);
-            if (mx) {
-                h2d(c, f.vr32, vr32.data(), nf);
-                h2d(c, f.drho32[0], drho32.data(), nf);
-                h2d(c, f.drho32[1], drho32.data(), nf);
-                h2d(c, f.p32, p32.data(), nf);
-            }
         }
         c->fslot = dalloc<int>(c, nf);
         h2d(c, c->fslot, orig.data(), nf);
@@ -683,6 +676,10 @@ int dsph_upload_scene(dsph_ctx* c, const dsph_scene* s) {
         c->fperm = dalloc<int>(c, nf);
         c->fcs = dalloc<int>(c, g.ncell);
         c->fce = dalloc<int>(c, g.ncell);
+        c->fblk_nb = dalloc<int>(c, g.ncell);
+        c->fblk_off = dalloc<int>(c, (size_t)g.ncell + 1);
+        c->fblocks = dalloc<int2>(c, std::max(nf, 1));
+        fill_int(c->st, c->fblk_off, (int64_t)g.ncell + 1, 0);
         c->scs = dalloc<int>(c, g.ncell);
         c->sce = dalloc<int>(c, g.ncell);
 
@@ -854,11 +851,18 @@ int dsph_upload_scene(dsph_ctx* c, const dsph_scene* s) {
         // ---- scratch
         const int64_t nmax = std::max<int64_t>(std::max(nf, ns), 1);
         c->radix_tmp = dalloc<unsigned>(c, radix_tmp_size(nmax));
-        int64_t sc = std::max<int64_t>(scan_tmp_size(nmax + 1), scan_tmp_size(256 * ((nmax + 1023) / 1024)));
+        int64_t sc = std::max<int64_t>(scan_tmp_size(std::max<int64_t>(nmax, g.ncell) + 1),
+                                       scan_tmp_size(256 * ((nmax + 1023) / 1024)));
         c->scan_tmp = dalloc<int>(c, sc + 64);
         c->status = dalloc<unsigned long long>(c, ST_WORDS);
         HIPCHK(hipMemsetAsync(c->status, 0, ST_WORDS * sizeof(unsigned long long), c->st));
+        {
+            FluidArgs fa = fluid_args(c);
+            fluid_records(c->st, fa, 0, mx);
+            fluid_records(c->st, fa, 1, mx);
+        }
         HIPCHK(hipStreamSynchronize(c->st));
+        HIPCHK(hipGetLastError());
         c->uploaded = true;
         return DSPH_OK;
     } catch (const ApiError& e) {

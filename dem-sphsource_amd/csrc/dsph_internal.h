@@ -7,7 +7,8 @@
 //            reorder: ph (x,y,z,h) f64x4, vr (vx,vy,vz,rho0) f64x4,
 //            drho f64 (Jacobi ping-pong), cn (c,nu) f64x2, sumw0 f64, acc f64x4,
 //            orig (slot -> caller index) i32; FP32 shadows for the mixed
-//            pair loops: vr32 (vx,vy,vz,rho0), drho32, p32 (c,nu,m,h).
+//            pair-loop records pv (v, rho) and pq (P, c, V, nu rho) in the
+//            pair precision.
 //   solids — caller order: pr (x,y,z,r), vel (vx,vy,vz,1/m), acc, w, torque,
 //            normal (f64x4), eff radius f64, material/cluster/clump/dummy i32.
 //   lists  — fluid–fluid: full (both directions) neighbour list in ELL
@@ -47,17 +48,28 @@ struct GridP {
 };
 
 struct FluidSet {
-    double4* ph = nullptr;
-    double4* vr = nullptr;
-    double* drho[2] = {nullptr, nullptr};
-    double2* cn = nullptr;
+    double4* ph = nullptr;              // x, y, z, h
+    double4* vr = nullptr;              // vx, vy, vz, rho0 (FP64 master)
+    double* drho[2] = {nullptr, nullptr};  // FP64 master, Jacobi ping-pong
+    double2* cn = nullptr;              // c, nu
     double* sumw0 = nullptr;
     double4* acc = nullptr;
-    int* orig = nullptr;
-    float4* vr32 = nullptr;
-    float* drho32[2] = {nullptr, nullptr};
-    float4* p32 = nullptr;
+    int* orig = nullptr;                // slot -> caller index
+    // pair-loop records in the pair-loop precision (float4 mixed / double4
+    // FP64), ping-ponged with drho so the Jacobi update never overwrites what
+    // neighbours read: pv = (vx, vy, vz, rho), pq = (P, c, V = m/rho, nu rho)
+    void* pv[2] = {nullptr, nullptr};
+    void* pq[2] = {nullptr, nullptr};
 };
+
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X; this
+// maps the hardware block id to a logical one so each XCD (and its L2) works
+// on a contiguous 1/8 of the cell-sorted particles.
+__device__ __forceinline__ int xcd_block() {
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int q = nb >> 3, r = nb & 7, x = b & 7;
+    return x * q + (x < r ? x : r) + (b >> 3);
+}
 
 struct Solids {
     double4* pr = nullptr;   // x, y, z, radius
@@ -157,12 +169,18 @@ void hash_solid(hipStream_t s, const GridP& g, const double4* pr, int n, unsigne
 void reorder_fluid(hipStream_t s, const FluidSet& src, const FluidSet& dst, const int* perm,
                    int* slot_of, int* member, int n, bool mixed, int cur);
 void member_slots(hipStream_t s, const int* perm, const int* slot_of, int* member, int n);
+// cell blocks of <= 64 sorted fluid particles (nb: ncell, off: ncell + 1)
+void fluid_blocks(hipStream_t s, const int* fcs, const int* fce, int ncell, int* nb, int* off,
+                  int2* blocks, int* scan_tmp);
+// fmember == nullptr: slots are the sorted positions (right after a reorder)
 void fluid_search(hipStream_t s, const GridP& g, const FluidSet& f, int nf, const int* fcs,
                   const int* fce, const int* fmember, const int* scs, const int* sce,
-                  const int* sperm, const Solids& so, double hmax, Lists& L, long long* status);
-void fluid_solid_fill(hipStream_t s, const GridP& g, const FluidSet& f, int nf,
-                      const int* scs, const int* sce, const int* sperm, const Solids& so,
-                      Lists& L, long long* status);
+                  const int* sperm, const Solids& so, double hmax, Lists& L, const int2* blocks,
+                  const int* nblk, long long* status);
+void fluid_solid_fill(hipStream_t s, const GridP& g, const FluidSet& f, int nf, const int* fcs,
+                      const int* fce, const int* fmember, const int* scs, const int* sce,
+                      const int* sperm, const Solids& so, Lists& L, const int2* blocks,
+                      const int* nblk, long long* status);
 void reverse_movable(hipStream_t s, const int* pair_key, const int* pair_off_total, int64_t cap,
                      const double4* key_vel, int nkeys, int* cnt, int* off, int* idx,
                      int* scan_tmp);
@@ -183,6 +201,7 @@ struct FluidArgs {
     double gx, gy, gz;
     uint32_t flags;
 };
+void fluid_records(hipStream_t s, const FluidArgs& a, int buf, bool mixed);
 void density_reinit(hipStream_t s, const FluidArgs& a, int i_step, bool mixed);
 void continuity(hipStream_t s, const FluidArgs& a, double dt, bool mixed);  // writes 1-cur
 void fluid_drift(hipStream_t s, const FluidArgs& a, double dt);

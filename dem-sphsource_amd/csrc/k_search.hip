@@ -124,9 +124,9 @@ __global__ void k_hash_solid(GridP g, const double4* __restrict__ pr, int n,
     if ((threadIdx.x & 63) == 0 && m) atomicAdd(&st[ST_OUT_OF_DOMAIN], (unsigned long long)__popcll(m));
 }
 
+template <class R4>
 __global__ void k_reorder(FluidSet src, FluidSet dst, const int* __restrict__ perm,
-                          int* __restrict__ slot_of, int* __restrict__ member, int n, int cur,
-                          int mixed) {
+                          int* __restrict__ slot_of, int* __restrict__ member, int n, int cur) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     int o = perm[t];
@@ -138,11 +138,8 @@ __global__ void k_reorder(FluidSet src, FluidSet dst, const int* __restrict__ pe
     dst.sumw0[t] = src.sumw0[old];
     dst.acc[t] = src.acc[old];
     dst.orig[t] = o;
-    if (mixed) {
-        dst.vr32[t] = src.vr32[old];
-        dst.drho32[cur][t] = src.drho32[cur][old];
-        dst.p32[t] = src.p32[old];
-    }
+    ((R4*)dst.pv[cur])[t] = ((const R4*)src.pv[cur])[old];
+    ((R4*)dst.pq[cur])[t] = ((const R4*)src.pq[cur])[old];
     slot_of[o] = t;
     member[t] = t;
 }
@@ -153,108 +150,189 @@ __global__ void k_member(const int* __restrict__ perm, const int* __restrict__ s
     if (t < n) member[t] = slot_of[perm[t]];
 }
 
-// setFluidInteractionsKernel (neighborSearch.cu:115-220): full fluid–fluid
-// list into ELL + fluid–solid count.  Membership: |r| <= 2 max(h_i, h_j)
-// (fluid), |r| <= 2 h_i (solid).
-__global__ void __launch_bounds__(kBS) k_fluid_search(
-    GridP g, const double4* __restrict__ ph, int nf, const int* __restrict__ fcs,
-    const int* __restrict__ fce, const int* __restrict__ fmember, const int* __restrict__ scs,
-    const int* __restrict__ sce, const int* __restrict__ sperm, const double4* __restrict__ spr,
-    double hmax, int K, int* __restrict__ nbr, int* __restrict__ cnt, int* __restrict__ fscnt,
-    unsigned long long* st) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    int c = 0, cs = 0;
-    if (s < nf) {
-        const double4 p = ph[s];
-        const double hi = p.w;
-        int ax, ay, az;
-        grid_pos(g, p.x, p.y, p.z, ax, ay, az);
-        const double Rf = 2.0 * fmax(hi, hmax) * (1.0 + 1e-9);
-        const double Rs = 2.0 * hi * (1.0 + 1e-9);
-        const double Rf2 = Rf * Rf, Rs2 = Rs * Rs;
-        for (int zz = -1; zz <= 1; zz++)
-            for (int yy = -1; yy <= 1; yy++)
-                for (int xx = -1; xx <= 1; xx++) {
-                    const int h = (az + zz) * g.gy * g.gx + (ay + yy) * g.gx + (ax + xx);
-                    if (h < 0 || h >= g.ncell) continue;
-                    const double bd2 = (h == g.ncell - 1) ? 0.0 : box_dist2(g, h, p.x, p.y, p.z);
-                    if (bd2 <= Rf2) {
-                        const int st0 = fcs[h];
-                        if (st0 >= 0) {
-                            const int en = fce[h];
-                            for (int t = st0; t < en; t++) {
-                                const int j = fmember[t];
-                                if (j == s) continue;
-                                const double4 q = ph[j];
-                                if (within(p.x, p.y, p.z, q.x, q.y, q.z, 2.0 * fmax(hi, q.w))) {
-                                    if (c < K) nbr[(int64_t)c * nf + s] = j;
-                                    c++;
-                                }
-                            }
-                        }
-                    }
-                    if (bd2 <= Rs2) {
-                        const int st0 = scs[h];
-                        if (st0 >= 0) {
-                            const int en = sce[h];
-                            for (int t = st0; t < en; t++) {
-                                const double4 q = spr[sperm[t]];
-                                if (within(p.x, p.y, p.z, q.x, q.y, q.z, 2.0 * hi)) cs++;
-                            }
-                        }
-                    }
-                }
-        cnt[s] = c;
-        fscnt[s] = cs;
-    }
-    wave_max_atomic(&st[ST_FF_MAX], (unsigned long long)c);
-    if (c > K) atomicOr(&st[ST_FF_OVERFLOW], 1ull);
-}
-
-// fill pass of the fluid–solid CSR (same loop order as the count)
-__global__ void __launch_bounds__(kBS) k_fs_fill(
-    GridP g, const double4* __restrict__ ph, int nf, const int* __restrict__ scs,
-    const int* __restrict__ sce, const int* __restrict__ sperm, const double4* __restrict__ spr,
-    const int* __restrict__ fscnt, const int* __restrict__ fsoff, int64_t cap,
-    int* __restrict__ fs_i, int* __restrict__ fs_a, double4* __restrict__ fsF,
-    unsigned long long* st) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s == 0) {
-        long long tot = fsoff[nf];
-        st[ST_FS_TOTAL] = (unsigned long long)tot;
-        if (tot > cap) atomicOr(&st[ST_FS_OVERFLOW], 1ull);
-    }
-    if (s >= nf || fscnt[s] == 0) return;
-    const double4 p = ph[s];
+// Per-lane stencil walk of one fluid particle (the reference's own loop,
+// neighborSearch.cu:135-214) — used for particles in the spare cell, whose
+// stencils differ lane by lane.  COUNT: append fluid neighbours to the ELL
+// row and count solid neighbours; FILL: write the solid neighbours.
+template <bool FILL>
+__device__ void lane_search(const GridP& g, const double4 p, int s, int nf,
+                            const int* __restrict__ fcs, const int* __restrict__ fce,
+                            const int* __restrict__ fmember, const double4* __restrict__ ph,
+                            const int* __restrict__ scs, const int* __restrict__ sce,
+                            const int* __restrict__ sperm, const double4* __restrict__ spr,
+                            double hmax, int K, int* __restrict__ nbr, int& c, int& cs,
+                            int64_t w, int64_t cap, int* __restrict__ fs_i, int* __restrict__ fs_a,
+                            double4* __restrict__ fsF) {
     const double hi = p.w;
     int ax, ay, az;
     grid_pos(g, p.x, p.y, p.z, ax, ay, az);
-    const double Rs = 2.0 * hi * (1.0 + 1e-9);
-    const double Rs2 = Rs * Rs;
-    int64_t w = fsoff[s];
     for (int zz = -1; zz <= 1; zz++)
         for (int yy = -1; yy <= 1; yy++)
             for (int xx = -1; xx <= 1; xx++) {
                 const int h = (az + zz) * g.gy * g.gx + (ay + yy) * g.gx + (ax + xx);
                 if (h < 0 || h >= g.ncell) continue;
-                const double bd2 = (h == g.ncell - 1) ? 0.0 : box_dist2(g, h, p.x, p.y, p.z);
-                if (bd2 > Rs2) continue;
-                const int st0 = scs[h];
-                if (st0 < 0) continue;
-                const int en = sce[h];
-                for (int t = st0; t < en; t++) {
+                if (!FILL) {
+                    const int st0 = fcs[h];
+                    if (st0 >= 0)
+                        for (int t = st0; t < fce[h]; t++) {
+                            const int j = fmember ? fmember[t] : t;
+                            if (j == s) continue;
+                            const double4 q = ph[j];
+                            if (within(p.x, p.y, p.z, q.x, q.y, q.z, 2.0 * fmax(hi, q.w))) {
+                                if (c < K) nbr[(int64_t)c * nf + s] = j;
+                                c++;
+                            }
+                        }
+                }
+                const int st1 = scs[h];
+                if (st1 < 0) continue;
+                for (int t = st1; t < sce[h]; t++) {
                     const int a = sperm[t];
                     const double4 q = spr[a];
-                    if (within(p.x, p.y, p.z, q.x, q.y, q.z, 2.0 * hi)) {
+                    if (!within(p.x, p.y, p.z, q.x, q.y, q.z, 2.0 * hi)) continue;
+                    if (FILL) {
                         if (w < cap) {
                             fs_i[w] = s;
                             fs_a[w] = a;
                             fsF[w] = make_double4(0, 0, 0, 0);
                         }
                         w++;
+                    } else {
+                        cs++;
                     }
                 }
             }
+}
+
+// Cell blocks: every cell holding fluid is cut into blocks of <= 64
+// consecutive sorted particles, one wavefront each.
+__global__ void k_blk_count(const int* __restrict__ cs, const int* __restrict__ ce, int ncell,
+                            int* __restrict__ nb) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    const int st = cs[c];
+    nb[c] = st >= 0 ? (ce[c] - st + 63) >> 6 : 0;
+}
+__global__ void k_blk_fill(const int* __restrict__ cs, const int* __restrict__ ce, int ncell,
+                           const int* __restrict__ off, int2* __restrict__ blocks) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    const int st = cs[c];
+    if (st < 0) return;
+    const int n = ce[c] - st;
+    const int o = off[c];
+    for (int b = 0; b * 64 < n; b++) blocks[o + b] = make_int2(c, st + 64 * b);
+}
+
+// setFluidInteractionsKernel (neighborSearch.cu:115-220), one wavefront per
+// cell block.  All lanes of a block share the cell, so the 27-cell stencil
+// walk is wave-uniform: cell ranges and candidate records are loaded once
+// for the wave, and a stencil cell is skipped when no lane's box distance can
+// reach it (ballot).  Output: the full fluid–fluid list in stencil order
+// (ELL row of the particle), the fluid–solid count.
+template <bool FILL>
+__global__ void __launch_bounds__(kBS) k_fluid_blocks(
+    GridP g, const double4* __restrict__ ph, int nf, const int* __restrict__ fcs,
+    const int* __restrict__ fce, const int* __restrict__ fmember, const int* __restrict__ scs,
+    const int* __restrict__ sce, const int* __restrict__ sperm, const double4* __restrict__ spr,
+    double hmax, int K, int* __restrict__ nbr, int* __restrict__ cnt, int* __restrict__ fscnt,
+    const int2* __restrict__ blocks, const int* __restrict__ nblk_p, const int* __restrict__ fsoff,
+    int64_t cap, int* __restrict__ fs_i, int* __restrict__ fs_a, double4* __restrict__ fsF,
+    unsigned long long* st) {
+    const int lane = threadIdx.x & 63;
+    const int wv = xcd_block() * (kBS / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nwv = gridDim.x * (kBS / 64);
+    const int nblk = *nblk_p;
+    unsigned long long cmax = 0;
+    bool ovf = false;
+    for (int b = wv; b < nblk; b += nwv) {
+        const int2 blk = blocks[b];
+        const int cell = __builtin_amdgcn_readfirstlane(blk.x);
+        const int t0 = __builtin_amdgcn_readfirstlane(blk.y);
+        const int nmem = min(64, fce[cell] - t0);
+        const bool act = lane < nmem;
+        const int t = t0 + (act ? lane : 0);
+        const int s = fmember ? fmember[t] : t;
+        const double4 p = ph[s];
+        int c = 0, cs = 0;
+        int64_t w = 0;
+        if (FILL) {
+            const int mine = act ? fscnt[s] : 0;
+            if (__ballot(mine > 0) == 0) continue;
+            w = fsoff[s];
+        }
+        if (cell == g.ncell - 1) {  // spare cell: stencils differ per lane
+            if (act)
+                lane_search<FILL>(g, p, s, nf, fcs, fce, fmember, ph, scs, sce, sperm, spr, hmax, K,
+                                  nbr, c, cs, w, cap, fs_i, fs_a, fsF);
+        } else {
+            const int gxy = g.gx * g.gy;
+            const int cz = cell / gxy, rem = cell - cz * gxy;
+            const int cy = rem / g.gx, cx = rem - cy * g.gx;
+            const double hi = p.w;
+            const double Rf = 2.0 * fmax(hi, hmax) * (1.0 + 1e-9);
+            const double Rs = 2.0 * hi * (1.0 + 1e-9);
+            for (int zz = -1; zz <= 1; zz++)
+                for (int yy = -1; yy <= 1; yy++)
+                    for (int xx = -1; xx <= 1; xx++) {
+                        const int h = (cz + zz) * gxy + (cy + yy) * g.gx + (cx + xx);
+                        if (h < 0 || h >= g.ncell) continue;
+                        const double bd2 = (h == g.ncell - 1) ? 0.0 : box_dist2(g, h, p.x, p.y, p.z);
+                        if (!FILL && __ballot(act && bd2 <= Rf * Rf) != 0) {
+                            const int st0 = __builtin_amdgcn_readfirstlane(fcs[h]);
+                            if (st0 >= 0) {
+                                const int en = __builtin_amdgcn_readfirstlane(fce[h]);
+                                for (int u = st0; u < en; u++) {
+                                    const int j = __builtin_amdgcn_readfirstlane(fmember ? fmember[u] : u);
+                                    const double4 q = ph[j];
+                                    if (act && j != s &&
+                                        within(p.x, p.y, p.z, q.x, q.y, q.z, 2.0 * fmax(hi, q.w))) {
+                                        if (c < K) nbr[(int64_t)c * nf + s] = j;
+                                        c++;
+                                    }
+                                }
+                            }
+                        }
+                        if (__ballot(act && bd2 <= Rs * Rs) != 0) {
+                            const int st1 = __builtin_amdgcn_readfirstlane(scs[h]);
+                            if (st1 < 0) continue;
+                            const int en = __builtin_amdgcn_readfirstlane(sce[h]);
+                            for (int u = st1; u < en; u++) {
+                                const int a = __builtin_amdgcn_readfirstlane(sperm[u]);
+                                const double4 q = spr[a];
+                                if (!(act && within(p.x, p.y, p.z, q.x, q.y, q.z, 2.0 * hi))) continue;
+                                if (FILL) {
+                                    if (w < cap) {
+                                        fs_i[w] = s;
+                                        fs_a[w] = a;
+                                        fsF[w] = make_double4(0, 0, 0, 0);
+                                    }
+                                    w++;
+                                } else {
+                                    cs++;
+                                }
+                            }
+                        }
+                    }
+        }
+        if (!FILL && act) {
+            cnt[s] = c;
+            fscnt[s] = cs;
+        }
+        cmax = cmax > (unsigned long long)c ? cmax : (unsigned long long)c;
+        ovf = ovf || c > K;
+    }
+    if (!FILL) {
+        wave_max_atomic(&st[ST_FF_MAX], cmax);
+        if (ovf) atomicOr(&st[ST_FF_OVERFLOW], 1ull);
+    }
+}
+
+__global__ void k_fs_status(const int* __restrict__ fsoff, int nf, int64_t cap,
+                            unsigned long long* st) {
+    const long long tot = fsoff[nf];
+    st[ST_FS_TOTAL] = (unsigned long long)tot;
+    if (tot > cap) st[ST_FS_OVERFLOW] |= 1ull;
 }
 
 // reverse CSR of a pair list by its second index (optionally only keys with
@@ -456,9 +534,13 @@ void hash_solid(hipStream_t s, const GridP& g, const double4* pr, int n, unsigne
 
 void reorder_fluid(hipStream_t s, const FluidSet& src, const FluidSet& dst, const int* perm,
                    int* slot_of, int* member, int n, bool mixed, int cur) {
-    if (n > 0)
-        hipLaunchKernelGGL(k_reorder, dim3(blocks_for(n)), dim3(kBS), 0, s, src, dst, perm, slot_of,
-                           member, n, cur, (int)mixed);
+    if (n <= 0) return;
+    if (mixed)
+        hipLaunchKernelGGL(k_reorder<float4>, dim3(blocks_for(n)), dim3(kBS), 0, s, src, dst, perm,
+                           slot_of, member, n, cur);
+    else
+        hipLaunchKernelGGL(k_reorder<double4>, dim3(blocks_for(n)), dim3(kBS), 0, s, src, dst, perm,
+                           slot_of, member, n, cur);
 }
 
 void member_slots(hipStream_t s, const int* perm, const int* slot_of, int* member, int n) {
@@ -466,21 +548,42 @@ void member_slots(hipStream_t s, const int* perm, const int* slot_of, int* membe
         hipLaunchKernelGGL(k_member, dim3(blocks_for(n)), dim3(kBS), 0, s, perm, slot_of, member, n);
 }
 
-void fluid_search(hipStream_t s, const GridP& g, const FluidSet& f, int nf, const int* fcs,
-                  const int* fce, const int* fmember, const int* scs, const int* sce,
-                  const int* sperm, const Solids& so, double hmax, Lists& L, long long* status) {
-    if (nf <= 0) return;
-    hipLaunchKernelGGL(k_fluid_search, dim3(blocks_for(nf)), dim3(kBS), 0, s, g, f.ph, nf, fcs, fce,
-                       fmember, scs, sce, sperm, so.pr, hmax, L.K, L.nbr, L.cnt, L.fscnt,
-                       (unsigned long long*)status);
+void fluid_blocks(hipStream_t s, const int* fcs, const int* fce, int ncell, int* nb, int* off,
+                  int2* blocks, int* scan_tmp) {
+    hipLaunchKernelGGL(k_blk_count, dim3(blocks_for(ncell)), dim3(kBS), 0, s, fcs, fce, ncell, nb);
+    exclusive_scan(s, nb, off, ncell, scan_tmp);
+    hipLaunchKernelGGL(k_blk_fill, dim3(blocks_for(ncell)), dim3(kBS), 0, s, fcs, fce, ncell,
+                       (const int*)off, blocks);
 }
 
-void fluid_solid_fill(hipStream_t s, const GridP& g, const FluidSet& f, int nf, const int* scs,
-                      const int* sce, const int* sperm, const Solids& so, Lists& L,
-                      long long* status) {
+static int wave_grid(int nf) {
+    // about one wavefront per cell block (4 per workgroup); waves stride over
+    // the block list, which lives on the device
+    int b = (nf + 127) / 128;
+    return b < 1 ? 1 : (b > 16384 ? 16384 : b);
+}
+
+void fluid_search(hipStream_t s, const GridP& g, const FluidSet& f, int nf, const int* fcs,
+                  const int* fce, const int* fmember, const int* scs, const int* sce,
+                  const int* sperm, const Solids& so, double hmax, Lists& L, const int2* blocks,
+                  const int* nblk, long long* status) {
     if (nf <= 0) return;
-    hipLaunchKernelGGL(k_fs_fill, dim3(blocks_for(nf)), dim3(kBS), 0, s, g, f.ph, nf, scs, sce, sperm,
-                       so.pr, L.fscnt, L.fsoff, L.fs_cap, L.fs_i, L.fs_a, L.fsF,
+    hipLaunchKernelGGL(k_fluid_blocks<false>, dim3(wave_grid(nf)), dim3(kBS), 0, s, g, f.ph, nf, fcs,
+                       fce, fmember, scs, sce, sperm, so.pr, hmax, L.K, L.nbr, L.cnt, L.fscnt, blocks,
+                       nblk, (const int*)nullptr, (int64_t)0, (int*)nullptr, (int*)nullptr,
+                       (double4*)nullptr, (unsigned long long*)status);
+}
+
+void fluid_solid_fill(hipStream_t s, const GridP& g, const FluidSet& f, int nf, const int* fcs,
+                      const int* fce, const int* fmember, const int* scs, const int* sce,
+                      const int* sperm, const Solids& so, Lists& L, const int2* blocks,
+                      const int* nblk, long long* status) {
+    if (nf <= 0) return;
+    hipLaunchKernelGGL(k_fs_status, dim3(1), dim3(1), 0, s, (const int*)L.fsoff, nf, L.fs_cap,
+                       (unsigned long long*)status);
+    hipLaunchKernelGGL(k_fluid_blocks<true>, dim3(wave_grid(nf)), dim3(kBS), 0, s, g, f.ph, nf, fcs,
+                       fce, fmember, scs, sce, sperm, so.pr, 0.0, L.K, L.nbr, L.cnt, L.fscnt, blocks,
+                       nblk, (const int*)L.fsoff, L.fs_cap, L.fs_i, L.fs_a, L.fsF,
                        (unsigned long long*)status);
 }
 
