@@ -38,7 +38,6 @@ struct ApiError {
 struct StepGraph {
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
-    int fcur_after = 0;
     int dcur_after = 0;
 };
 
@@ -64,7 +63,7 @@ struct dsph_ctx {
     double gv[3] = {0, 0, -9.81};
     // fluid
     FluidSet fset[2];
-    int fcur = 0, dcur = 0;
+    int dcur = 0;  // current dRho / pair-record buffer (Jacobi ping-pong)
     int* fslot = nullptr;
     int* fmember = nullptr;
     unsigned* fkey = nullptr;
@@ -75,7 +74,17 @@ struct dsph_ctx {
     int* fblk_nb = nullptr;     // blocks per cell
     int* fblk_off = nullptr;    // ncell + 1; [ncell] = number of blocks
     int2* fblocks = nullptr;    // (cell, first sorted position)
-    bool member_identity = false;
+    unsigned* hslot = nullptr;  // reference hash per fluid slot
+    double4* fxb = nullptr;     // fluid positions at the last superset build
+    double4* sxb = nullptr;     // solid positions at the last superset build
+    unsigned long long* ctl = nullptr;
+    double skin_h = 0.3;        // superset skin / max fluid h
+    // active (non-dummy) solid grid for the DEM contact search
+    unsigned* akey = nullptr;
+    unsigned* akey_sorted = nullptr;
+    int* aperm = nullptr;
+    int* acs = nullptr;
+    int* ace = nullptr;
     // solids
     Solids so;
     unsigned* skey = nullptr;
@@ -183,6 +192,15 @@ void alloc_ff(dsph_ctx* c, int K) {
     c->L.nbr = dalloc<int>(c, (size_t)K * std::max(c->nf, 1));
 }
 
+void alloc_sup(dsph_ctx* c, int K, int KS) {
+    dfree(c, c->L.sup);
+    dfree(c, c->L.fsup);
+    c->L.supK = K;
+    c->L.fsupK = KS;
+    c->L.sup = dalloc<int>(c, (size_t)K * std::max(c->nf, 1));
+    c->L.fsup = dalloc<int>(c, (size_t)KS * std::max(c->nf, 1));
+}
+
 void alloc_fs(dsph_ctx* c, int64_t cap) {
     dfree(c, c->L.fs_i);
     dfree(c, c->L.fs_a);
@@ -234,7 +252,7 @@ void invalidate_graphs(dsph_ctx* c) {
 // ----------------------------------------------------------------- phases
 FluidArgs fluid_args(dsph_ctx* c) {
     FluidArgs a;
-    a.f = c->fset[c->fcur];
+    a.f = c->fset[0];
     a.nf = c->nf;
     a.cur = c->dcur;
     a.L = c->L;
@@ -267,16 +285,40 @@ DemArgs dem_args(dsph_ctx* c) {
     return a;
 }
 
-bool is_reorder(dsph_ctx* c, int i_step, int gap) {
-    return c->reorder_period > 0 && ((i_step / gap) % c->reorder_period) == 0;
+SearchArgs search_args(dsph_ctx* c) {
+    SearchArgs a;
+    a.g = c->g;
+    a.nf = c->nf;
+    a.ns = c->ns;
+    a.ph = c->fset[0].ph;
+    a.fcs = c->fcs;
+    a.fce = c->fce;
+    a.fmember = c->fmember;
+    a.forig = c->fset[0].orig;
+    a.hslot = c->hslot;
+    a.scs = c->scs;
+    a.sce = c->sce;
+    a.sperm = c->sperm;
+    a.skey = c->skey;
+    a.spr = c->so.pr;
+    a.hmax = c->hmax;
+    a.skin = c->skin_h * c->hmax;
+    a.L = c->L;
+    a.ctl = c->ctl;
+    a.st = c->status;
+    return a;
 }
 
+// neighborSearch (neighborSearch.cu:314-381): reference grid of both species
+// every step; on fluid steps the Verlet superset (rebuilt when needed, with
+// the sorted-order reorder) and the exact lists; DEM contacts every step.
 void neighbor_search(dsph_ctx* c, int i_step, int gap, bool save_history) {
     hipStream_t s = c->st;
     long long* stw = (long long*)c->status;
+    const bool fluid_step = i_step % gap == 0 && c->nf > 0;
     {
         FamTimer t(c, F_GRID);
-        hash_fluid(s, c->g, c->fset[c->fcur].ph, c->fset[c->fcur].orig, c->nf, c->fkey, stw);
+        hash_fluid(s, c->g, c->fset[0].ph, c->fset[0].orig, c->nf, c->fkey, stw);
         hash_solid(s, c->g, c->so.pr, c->ns, c->skey, stw);
         // ranges of an empty species are filled with -1 as well
         if (c->nf > 0)
@@ -285,32 +327,24 @@ void neighbor_search(dsph_ctx* c, int i_step, int gap, bool save_history) {
         if (c->ns > 0)
             radix_sort_iota(s, c->skey, c->ns, c->key_bits, c->skey_sorted, c->sperm, c->radix_tmp);
         cell_spans(s, c->skey_sorted, c->ns, c->scs, c->sce, c->g.ncell);
-        if (i_step % gap == 0 && c->nf > 0) {
-            if (is_reorder(c, i_step, gap)) {
-                reorder_fluid(s, c->fset[c->fcur], c->fset[1 - c->fcur], c->fperm, c->fslot,
-                              c->fmember, c->nf, c->mixed(), c->dcur);
-                c->fcur ^= 1;
-                c->member_identity = true;
-            } else {
-                member_slots(s, c->fperm, c->fslot, c->fmember, c->nf);
-                c->member_identity = false;
-            }
-            fluid_blocks(s, c->fcs, c->fce, c->g.ncell, c->fblk_nb, c->fblk_off, c->fblocks,
-                         c->scan_tmp);
+        if (fluid_step) {
+            rebuild_decision(s, c->fset[0].ph, c->fxb, c->nf, c->so.pr, c->sxb, c->ns,
+                             c->skin_h * c->hmax, c->ctl, stw);
+            if (c->reorder_period > 0)
+                reorder_fluid(s, c->fset[0], c->fset[1], c->fperm, c->fslot, c->nf, c->mixed(),
+                              c->dcur, c->ctl);
         }
+        if (c->nf > 0)
+            member_slots(s, c->fperm, c->fslot, c->fkey_sorted, c->fmember, c->hslot, c->nf);
     }
-    if (i_step % gap == 0 && c->nf > 0) {
-        const int* mem = c->member_identity ? nullptr : c->fmember;
-        const int* nblk = c->fblk_off + c->g.ncell;
+    if (fluid_step) {
+        const SearchArgs sa = search_args(c);
         {
             FamTimer t(c, F_FSEARCH);
-            fluid_search(s, c->g, c->fset[c->fcur], c->nf, c->fcs, c->fce, mem, c->scs, c->sce,
-                         c->sperm, c->so, c->hmax, c->L, c->fblocks, nblk, stw);
+            fluid_superset(s, sa, c->fblk_nb, c->fblk_off, c->fblocks, c->scan_tmp, c->fxb, c->sxb);
         }
         FamTimer t(c, F_FSLIST);
-        exclusive_scan(s, c->L.fscnt, c->L.fsoff, c->nf, c->scan_tmp);
-        fluid_solid_fill(s, c->g, c->fset[c->fcur], c->nf, c->fcs, c->fce, mem, c->scs, c->sce,
-                         c->sperm, c->so, c->L, c->fblocks, nblk, stw);
+        fluid_filter(s, sa, c->scan_tmp);
         if (c->ns > 0)
             reverse_movable(s, c->L.fs_a, c->L.fsoff + c->nf, c->L.fs_cap, c->so.vel, c->ns,
                             c->L.fsr_cnt, c->L.fsr_off, c->L.fsr_idx, c->scan_tmp);
@@ -318,9 +352,16 @@ void neighbor_search(dsph_ctx* c, int i_step, int gap, bool save_history) {
     if (c->ns > 0) {
         FamTimer t(c, F_SSEARCH);
         if (save_history) solid_save_history(s, c->L, c->ns);
-        solid_search(s, c->g, c->so, c->ns, c->scs, c->sce, c->sperm, c->L, c->scan_tmp, stw);
+        active_grid(s, c->so, c->skey, c->g.ncell, c->key_bits, c->akey, c->akey_sorted, c->aperm,
+                    c->acs, c->ace, c->radix_tmp);
+        solid_search(s, c->g, c->so, c->ns, c->acs, c->ace, c->aperm, c->L, c->scan_tmp, stw);
         reverse_solid(s, c->L, c->ns, c->scan_tmp);
     }
+}
+
+void force_rebuild(dsph_ctx* c) {
+    unsigned long long one = 1;
+    HIPCHK(hipMemcpyAsync(c->ctl + CTL_FORCE, &one, sizeof(one), hipMemcpyHostToDevice, c->st));
 }
 
 void read_status(dsph_ctx* c, unsigned long long* w) {
@@ -336,18 +377,18 @@ void search_checked(dsph_ctx* c, int i_step, int gap) {
     for (int attempt = 0; attempt < 8; attempt++) {
         unsigned long long w[ST_WORDS];
         read_status(c, w);
-        if (!(w[ST_FF_OVERFLOW] | w[ST_FS_OVERFLOW] | w[ST_SS_OVERFLOW])) return;
+        if (!(w[ST_FF_OVERFLOW] | w[ST_FS_OVERFLOW] | w[ST_SS_OVERFLOW] | w[ST_SUP_OVERFLOW])) return;
+        if (w[ST_SUP_OVERFLOW]) alloc_sup(c, c->L.supK * 3 / 2 + 16, c->L.fsupK * 3 / 2 + 16);
         if (w[ST_FF_OVERFLOW])
             alloc_ff(c, (int)std::max<unsigned long long>(w[ST_FF_MAX] + 16, (unsigned long long)c->L.K * 2));
         if (w[ST_FS_OVERFLOW]) alloc_fs(c, (int64_t)(w[ST_FS_TOTAL] * 3 / 2 + 1024));
         if (w[ST_SS_OVERFLOW]) alloc_ss(c, (int64_t)(w[ST_SS_TOTAL] * 3 / 2 + 1024), true);
         unsigned long long zero[3] = {0, 0, 0};
         HIPCHK(hipMemcpyAsync(c->status, zero, sizeof(zero), hipMemcpyHostToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(c->status + ST_SUP_OVERFLOW, zero, sizeof(zero[0]), hipMemcpyHostToDevice, c->st));
         invalidate_graphs(c);
-        const int rp = c->reorder_period;
-        c->reorder_period = 0;
+        force_rebuild(c);
         neighbor_search(c, i_step, gap, false);
-        c->reorder_period = rp;
     }
     throw ApiError{DSPH_E_CAPACITY, "neighbor list capacity could not be satisfied"};
 }
@@ -430,8 +471,6 @@ uint64_t step_key(dsph_ctx* c, int i_step, int gap) {
     k |= (uint64_t)(i_step % gap == 0);
     k |= (uint64_t)(i_step % (5 * gap) == 0 || i_step == 1) << 1;
     k |= (uint64_t)(i_step == 1) << 2;
-    k |= (uint64_t)(i_step % gap == 0 && is_reorder(c, i_step, gap)) << 3;
-    k |= (uint64_t)c->fcur << 4;
     k |= (uint64_t)c->dcur << 5;
     k |= (uint64_t)(uint32_t)gap << 8;
     return k;
@@ -445,35 +484,126 @@ void require(bool cond, const std::string& msg) {
 }  // namespace
 
 namespace {
+// The reference emits each list segment in stencil order (z, y, x over the 27
+// cells around the particle's cell, neighborSearch.cu:188-214) with cell
+// members in ascending index.  The device keeps list rows in the order of
+// the last superset build; read-back re-sorts them by (stencil rank on the
+// reference grid, index) from the current reference hashes.
+void stencil_hashes(const GridP& g, double x, double y, double z, int H[27]) {
+    const int ax = (int)((x - g.minx) / g.csx);
+    const int ay = (int)((y - g.miny) / g.csy);
+    const int az = (int)((z - g.minz) / g.csz);
+    int k = 0;
+    for (int zz = -1; zz <= 1; zz++)
+        for (int yy = -1; yy <= 1; yy++)
+            for (int xx = -1; xx <= 1; xx++) {
+                const int h = (az + zz) * g.gy * g.gx + (ay + yy) * g.gx + (ax + xx);
+                H[k++] = (h < 0 || h >= g.ncell) ? -1 : h;
+            }
+}
+int stencil_rank(const int H[27], int h) {
+    for (int k = 0; k < 27; k++)
+        if (H[k] == h) return k;
+    return 27;
+}
+
+// one read-back row entry: the neighbour's cell, its index, a payload
+struct RowEnt {
+    int key, idx;
+    int64_t id;
+    int rank;
+};
+// Puts a row in the reference's order.  When stencil offsets coincide (grids
+// narrower than three cells in x or y) a cell is visited once per offset and
+// its members appear once per visit: the row then holds that many copies of
+// each entry, emitted one copy per visit.  Entries off the stencil go last.
+void order_row(const GridP& g, const int H[27], std::vector<RowEnt>& row) {
+    for (auto& e : row) e.rank = stencil_rank(H, e.key);
+    auto lt = [](const RowEnt& x, const RowEnt& y) {
+        if (x.rank != y.rank) return x.rank < y.rank;
+        if (x.idx != y.idx) return x.idx < y.idx;
+        return x.id < y.id;
+    };
+    if (!g.collide) {
+        std::sort(row.begin(), row.end(), lt);
+        return;
+    }
+    std::sort(row.begin(), row.end(), [](const RowEnt& x, const RowEnt& y) {
+        return x.idx != y.idx ? x.idx < y.idx : x.id < y.id;
+    });
+    std::vector<RowEnt> out;
+    std::vector<char> used(row.size(), 0);
+    for (int k = 0; k < 27; k++) {
+        if (H[k] < 0) continue;
+        int last = -1;
+        for (size_t e = 0; e < row.size(); e++)
+            if (!used[e] && row[e].key == H[k] && row[e].idx != last) {
+                used[e] = 1;
+                last = row[e].idx;
+                out.push_back(row[e]);
+            }
+    }
+    for (size_t e = 0; e < row.size(); e++)
+        if (!used[e]) out.push_back(row[e]);
+    row.swap(out);
+}
+
+struct FluidHost {
+    std::vector<int> orig, slot_of;
+    std::vector<double4> ph;
+    std::vector<unsigned> fkey;  // reference hash per caller index
+};
+FluidHost fluid_host(dsph_ctx* c) {
+    const int nf = c->nf;
+    FluidHost f;
+    f.orig.resize(nf);
+    f.ph.resize(nf);
+    f.fkey.resize(nf);
+    d2h(c, f.orig.data(), c->fset[0].orig, nf);
+    d2h(c, f.ph.data(), c->fset[0].ph, nf);
+    d2h(c, f.fkey.data(), c->fkey, nf);
+    HIPCHK(hipStreamSynchronize(c->st));
+    f.slot_of.resize(nf);
+    for (int s2 = 0; s2 < nf; s2++) f.slot_of[f.orig[s2]] = s2;
+    return f;
+}
+
 // fluid–solid pairs in reference order (CSR by fluid caller index) as pair
 // ids into the device arrays
 std::vector<int64_t> fs_reference_order(dsph_ctx* c, std::vector<int>& fluid_idx,
                                         std::vector<int>& solid_idx, std::vector<int>* count) {
     const int nf = c->nf;
-    std::vector<int> off(nf + 1), orig(nf);
+    const FluidHost fh = fluid_host(c);
+    std::vector<int> off(nf + 1);
     d2h(c, off.data(), c->L.fsoff, nf + 1);
-    d2h(c, orig.data(), c->fset[c->fcur].orig, nf);
     HIPCHK(hipStreamSynchronize(c->st));
     int64_t tot = off[nf];
     if (tot > c->L.fs_cap) throw ApiError{DSPH_E_CAPACITY, "fluid-solid list overflowed"};
     std::vector<int> a(tot);
+    std::vector<unsigned> skey(c->ns);
     d2h(c, a.data(), c->L.fs_a, tot);
+    d2h(c, skey.data(), c->skey, c->ns);
     HIPCHK(hipStreamSynchronize(c->st));
-    std::vector<int> slot_of(nf);
-    for (int s = 0; s < nf; s++) slot_of[orig[s]] = s;
     std::vector<int64_t> ids;
     ids.reserve(tot);
     fluid_idx.clear();
     solid_idx.clear();
     if (count) count->assign(nf, 0);
+    std::vector<RowEnt> row;
     for (int i = 0; i < nf; i++) {
-        int s = slot_of[i];
-        for (int k = off[s]; k < off[s + 1]; k++) {
-            ids.push_back(k);
+        const int s2 = fh.slot_of[i];
+        int H[27];
+        stencil_hashes(c->g, fh.ph[s2].x, fh.ph[s2].y, fh.ph[s2].z, H);
+        row.clear();
+        for (int k = off[s2]; k < off[s2 + 1]; k++)
+            row.push_back({(int)skey[a[k]], a[k], (int64_t)k, 0});
+        order_row(c->g, H, row);
+        for (auto& e : row) {
+            ids.push_back(e.id);
             fluid_idx.push_back(i);
-            solid_idx.push_back(a[k]);
+            solid_idx.push_back(e.idx);
         }
-        if (count) (*count)[i] = off[s + 1] - off[s];
+        if (count) (*count)[i] = (int)row.size();
     }
     return ids;
 }
@@ -533,6 +663,34 @@ int dsph_set_reorder_period(dsph_ctx* c, int32_t p) {
     return DSPH_OK;
 }
 
+int dsph_set_skin(dsph_ctx* c, double skin_over_h) {
+    if (!c || !(skin_over_h >= 0.0)) return DSPH_E_INVALID;
+    c->skin_h = skin_over_h;
+    invalidate_graphs(c);
+    if (c->uploaded) {
+        try {
+            force_rebuild(c);
+        } catch (const ApiError& e) {
+            return fail(c, e);
+        }
+    }
+    return DSPH_OK;
+}
+
+int dsph_set_superset_capacity(dsph_ctx* c, int32_t ff, int32_t fs) {
+    if (!c || !c->uploaded) return DSPH_E_STATE;
+    if (ff < 0 || fs < 0) return DSPH_E_INVALID;
+    try {
+        HIPCHK(hipStreamSynchronize(c->st));
+        alloc_sup(c, ff > 0 ? ff : c->L.supK, fs > 0 ? fs : c->L.fsupK);
+        invalidate_graphs(c);
+        force_rebuild(c);
+        return DSPH_OK;
+    } catch (const ApiError& e) {
+        return fail(c, e);
+    }
+}
+
 int dsph_set_gravity(dsph_ctx* c, const double g[3]) {
     if (!c || !g) return DSPH_E_INVALID;
     c->gv[0] = g[0];
@@ -583,7 +741,7 @@ int dsph_upload_scene(dsph_ctx* c, const dsph_scene* s) {
         c->cl = Clumps();
         c->fset[0] = FluidSet();
         c->fset[1] = FluidSet();
-        c->fcur = c->dcur = 0;
+        c->dcur = 0;
         c->initialized = false;
         const int nf = c->nf = s->n_fluid;
         const int ns = c->ns = s->n_solid;
@@ -615,6 +773,7 @@ int dsph_upload_scene(dsph_ctx* c, const dsph_scene* s) {
         const long long ncell = (long long)g.gx * g.gy * g.gz + 1;
         require(ncell < (1LL << 31) - 1, "grid has too many cells");
         g.ncell = (int)ncell;
+        g.collide = g.gx < 3 || g.gy < 3;
         double ext = std::max({std::fabs(g.minx), std::fabs(g.maxx), std::fabs(g.miny), std::fabs(g.maxy),
                                std::fabs(g.minz), std::fabs(g.maxz)});
         g.pad = 1e-9 * (1.0 + ext);
@@ -680,6 +839,11 @@ int dsph_upload_scene(dsph_ctx* c, const dsph_scene* s) {
         c->fblk_off = dalloc<int>(c, (size_t)g.ncell + 1);
         c->fblocks = dalloc<int2>(c, std::max(nf, 1));
         fill_int(c->st, c->fblk_off, (int64_t)g.ncell + 1, 0);
+        c->hslot = dalloc<unsigned>(c, nf);
+        c->fxb = dalloc<double4>(c, nf);
+        h2d(c, c->fxb, ph.data(), nf);
+        c->ctl = dalloc<unsigned long long>(c, CTL_WORDS);
+        HIPCHK(hipMemsetAsync(c->ctl, 0, CTL_WORDS * sizeof(unsigned long long), c->st));
         c->scs = dalloc<int>(c, g.ncell);
         c->sce = dalloc<int>(c, g.ncell);
 
@@ -744,6 +908,13 @@ int dsph_upload_scene(dsph_ctx* c, const dsph_scene* s) {
         c->skey = dalloc<unsigned>(c, ns);
         c->skey_sorted = dalloc<unsigned>(c, ns);
         c->sperm = dalloc<int>(c, ns);
+        c->sxb = dalloc<double4>(c, ns);
+        h2d(c, c->sxb, spr.data(), ns);
+        c->akey = dalloc<unsigned>(c, active.size());
+        c->akey_sorted = dalloc<unsigned>(c, active.size());
+        c->aperm = dalloc<int>(c, active.size());
+        c->acs = dalloc<int>(c, g.ncell);
+        c->ace = dalloc<int>(c, g.ncell);
 
         // ---- clumps
         std::vector<double4> cpos(nc), cvel(nc), cw(nc), cq(nc);
@@ -834,6 +1005,11 @@ int dsph_upload_scene(dsph_ctx* c, const dsph_scene* s) {
         fill_int(c->st, L.fsoff, (int64_t)nf + 1, 0);
         fill_int(c->st, L.cnt, nf, 0);
         alloc_ff(c, nf > 0 ? 128 : 1);
+        alloc_sup(c, nf > 0 ? 160 : 1, nf > 0 ? 96 : 1);
+        L.sup_cnt = dalloc<int>(c, nf);
+        L.fsup_cnt = dalloc<int>(c, nf);
+        fill_int(c->st, L.sup_cnt, nf, 0);
+        fill_int(c->st, L.fsup_cnt, nf, 0);
         alloc_fs(c, std::max<int64_t>(4096, 8LL * nf));
         L.fsr_cnt = dalloc<int>(c, ns);
         L.fsr_off = dalloc<int>(c, (size_t)ns + 1);
@@ -844,6 +1020,7 @@ int dsph_upload_scene(dsph_ctx* c, const dsph_scene* s) {
         L.ssr_cnt = dalloc<int>(c, ns);
         L.ssr_off = dalloc<int>(c, (size_t)ns + 1);
         fill_int(c->st, L.ssoff, (int64_t)ns + 1, 0);
+        fill_int(c->st, L.sscnt, ns, 0);  // dummies are never searched: their count stays 0
         fill_int(c->st, L.hoff, (int64_t)ns + 1, 0);
         fill_int(c->st, L.ssr_off, (int64_t)ns + 1, 0);
         alloc_ss(c, std::max<int64_t>(4096, 6LL * (int64_t)active.size() + 4096), false);
@@ -861,6 +1038,7 @@ int dsph_upload_scene(dsph_ctx* c, const dsph_scene* s) {
             fluid_records(c->st, fa, 0, mx);
             fluid_records(c->st, fa, 1, mx);
         }
+        force_rebuild(c);
         HIPCHK(hipStreamSynchronize(c->st));
         HIPCHK(hipGetLastError());
         c->uploaded = true;
@@ -878,6 +1056,7 @@ int dsph_set_capacity(dsph_ctx* c, int32_t ff, int64_t fs, int64_t ss) {
         if (fs > 0) alloc_fs(c, fs);
         if (ss > 0) alloc_ss(c, ss, true);
         invalidate_graphs(c);
+        force_rebuild(c);
         return DSPH_OK;
     } catch (const ApiError& e) {
         return fail(c, e);
@@ -1065,19 +1244,17 @@ int dsph_step_n(dsph_ctx* c, int32_t i0, int32_t n, double dt, int32_t gap) {
                 }
                 HIPCHK(hipStreamEndCapture(c->st, &sg.graph));
                 HIPCHK(hipGraphInstantiate(&sg.exec, sg.graph, nullptr, nullptr, 0));
-                sg.fcur_after = c->fcur;
                 sg.dcur_after = c->dcur;
                 c->graphs[key] = sg;
                 HIPCHK(hipGraphLaunch(sg.exec, c->st));
             } else {
                 HIPCHK(hipGraphLaunch(it->second.exec, c->st));
-                c->fcur = it->second.fcur_after;
                 c->dcur = it->second.dcur_after;
             }
         }
         unsigned long long w[ST_WORDS];
         read_status(c, w);
-        if (w[ST_FF_OVERFLOW] | w[ST_FS_OVERFLOW] | w[ST_SS_OVERFLOW]) {
+        if (w[ST_FF_OVERFLOW] | w[ST_FS_OVERFLOW] | w[ST_SS_OVERFLOW] | w[ST_SUP_OVERFLOW]) {
             c->err = "neighbor list overflow during dsph_step_n; enlarge capacities (dsph_set_capacity) and rerun";
             return DSPH_E_CAPACITY;
         }
@@ -1120,7 +1297,7 @@ int dsph_download_fluid(dsph_ctx* c, double* pos, double* vel, double* acc, doub
     API_GUARD(c);
     try {
         const int n = c->nf;
-        const FluidSet& f = c->fset[c->fcur];
+        const FluidSet& f = c->fset[0];
         std::vector<double4> ph(n), vr(n), ac(n);
         std::vector<double> dr(n), sw(n);
         std::vector<int> orig(n);
@@ -1301,25 +1478,30 @@ int dsph_download_list(dsph_ctx* c, int kind, int32_t* pointed, int32_t* pointin
         std::vector<int> A, Bv, cnt;
         if (kind == DSPH_LIST_FLUID_FLUID) {
             const int nf = c->nf, K = c->L.K;
-            std::vector<int> nbr((size_t)K * nf), cn(nf), orig(nf);
+            const FluidHost fh = fluid_host(c);
+            std::vector<int> nbr((size_t)K * nf), cn(nf);
             d2h(c, nbr.data(), c->L.nbr, (size_t)K * nf);
             d2h(c, cn.data(), c->L.cnt, nf);
-            d2h(c, orig.data(), c->fset[c->fcur].orig, nf);
             HIPCHK(hipStreamSynchronize(c->st));
-            std::vector<int> slot_of(nf);
-            for (int s = 0; s < nf; s++) slot_of[orig[s]] = s;
             cnt.assign(nf, 0);
+            std::vector<RowEnt> row;
             for (int i = 0; i < nf; i++) {
-                const int s = slot_of[i];
-                if (cn[s] > K) throw ApiError{DSPH_E_CAPACITY, "fluid-fluid list overflowed"};
-                for (int k = 0; k < cn[s]; k++) {
-                    const int jo = orig[nbr[(size_t)k * nf + s]];
-                    if (jo > i) {  // the reference keeps the j > i half
-                        A.push_back(i);
-                        Bv.push_back(jo);
-                        cnt[i]++;
-                    }
+                const int s2 = fh.slot_of[i];
+                if (cn[s2] > K) throw ApiError{DSPH_E_CAPACITY, "fluid-fluid list overflowed"};
+                int H[27];
+                stencil_hashes(c->g, fh.ph[s2].x, fh.ph[s2].y, fh.ph[s2].z, H);
+                row.clear();
+                for (int k = 0; k < cn[s2]; k++) {
+                    const int jo = fh.orig[nbr[(size_t)k * nf + s2]];
+                    if (jo > i)  // the reference keeps the j > i half
+                        row.push_back({(int)fh.fkey[jo], jo, 0, 0});
                 }
+                order_row(c->g, H, row);
+                for (auto& e : row) {
+                    A.push_back(i);
+                    Bv.push_back(e.idx);
+                }
+                cnt[i] = (int)row.size();
             }
         } else if (kind == DSPH_LIST_FLUID_SOLID) {
             fs_reference_order(c, A, Bv, &cnt);

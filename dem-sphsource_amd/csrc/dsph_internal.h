@@ -35,7 +35,20 @@ enum {
     ST_FF_MAX = 4,
     ST_FS_TOTAL = 5,
     ST_SS_TOTAL = 6,
-    ST_WORDS = 8
+    ST_SUP_OVERFLOW = 7,
+    ST_REBUILDS = 8,
+    ST_WORDS = 12
+};
+
+// device control words of the neighbour search
+enum {
+    CTL_DISP_F = 0,   // max fluid displacement since the last superset build (f64 bits)
+    CTL_DISP_S = 1,   // max solid displacement (f64 bits)
+    CTL_FORCE = 2,    // host request: rebuild at the next search
+    CTL_REBUILD = 3,  // this search rebuilds the superset
+    CTL_WALK_ALL = 4, // some particle is out of the domain: walk the stencil exactly
+    CTL_OUT_AT_BUILD = 5,  // the last build had particles out of the domain
+    CTL_WORDS = 8
 };
 
 // the reference's spatialGrid (deviceStructs.h:976-1014)
@@ -44,7 +57,8 @@ struct GridP {
     double maxx, maxy, maxz;
     double csx, csy, csz;
     int gx, gy, gz, ncell;
-    double pad;  // conservative enlargement of cell boxes for pruning
+    int collide;  // gx < 3 or gy < 3: some of the 27 linear stencil offsets coincide
+    double pad;   // conservative enlargement of cell boxes for pruning
 };
 
 struct FluidSet {
@@ -115,6 +129,13 @@ struct Lists {
     int* fsr_cnt = nullptr;
     int* fsr_off = nullptr;  // NS + 1
     int* fsr_idx = nullptr;  // fs pair ids, capacity fs_cap
+    // Verlet supersets, ELL column-major like nbr (rebuilt only when needed)
+    int supK = 0;
+    int* sup = nullptr;
+    int* sup_cnt = nullptr;
+    int fsupK = 0;
+    int* fsup = nullptr;
+    int* fsup_cnt = nullptr;
     // solid–solid CSR by solid index (the reference's own list)
     int64_t ss_cap = 0;
     int* sscnt = nullptr;
@@ -162,31 +183,48 @@ void cell_spans(hipStream_t s, const unsigned* sorted, int64_t n, int* start, in
                 int ncell);
 
 // search (k_search.hip)
+struct SearchArgs {
+    GridP g;
+    int nf, ns;
+    const double4* ph;      // fluid position + h, slot order
+    const int* fcs;         // fluid cell ranges over sorted positions
+    const int* fce;
+    const int* fmember;     // slot of each sorted position
+    const int* forig;       // caller index of each fluid slot
+    const unsigned* hslot;  // reference hash of each fluid slot
+    const int* scs;         // solid cell ranges
+    const int* sce;
+    const int* sperm;
+    const unsigned* skey;   // reference hash of each solid
+    const double4* spr;
+    double hmax, skin;
+    Lists L;
+    unsigned long long* ctl;
+    unsigned long long* st;
+};
 void hash_fluid(hipStream_t s, const GridP& g, const double4* ph, const int* orig, int n,
                 unsigned* key_orig, long long* status);
 void hash_solid(hipStream_t s, const GridP& g, const double4* pr, int n, unsigned* key,
                 long long* status);
-void reorder_fluid(hipStream_t s, const FluidSet& src, const FluidSet& dst, const int* perm,
-                   int* slot_of, int* member, int n, bool mixed, int cur);
-void member_slots(hipStream_t s, const int* perm, const int* slot_of, int* member, int n);
-// cell blocks of <= 64 sorted fluid particles (nb: ncell, off: ncell + 1)
-void fluid_blocks(hipStream_t s, const int* fcs, const int* fce, int ncell, int* nb, int* off,
-                  int2* blocks, int* scan_tmp);
-// fmember == nullptr: slots are the sorted positions (right after a reorder)
-void fluid_search(hipStream_t s, const GridP& g, const FluidSet& f, int nf, const int* fcs,
-                  const int* fce, const int* fmember, const int* scs, const int* sce,
-                  const int* sperm, const Solids& so, double hmax, Lists& L, const int2* blocks,
-                  const int* nblk, long long* status);
-void fluid_solid_fill(hipStream_t s, const GridP& g, const FluidSet& f, int nf, const int* fcs,
-                      const int* fce, const int* fmember, const int* scs, const int* sce,
-                      const int* sperm, const Solids& so, Lists& L, const int2* blocks,
-                      const int* nblk, long long* status);
+void rebuild_decision(hipStream_t s, const double4* fph, const double4* fxb, int nf,
+                      const double4* spr, const double4* sxb, int ns, double skin,
+                      unsigned long long* ctl, long long* status);
+void reorder_fluid(hipStream_t s, const FluidSet& f, const FluidSet& scratch, const int* perm,
+                   int* slot_of, int n, bool mixed, int cur, const unsigned long long* ctl);
+void member_slots(hipStream_t s, const int* perm, const int* slot_of, const unsigned* sorted_key,
+                  int* member, unsigned* hslot, int n);
+void fluid_superset(hipStream_t s, const SearchArgs& a, int* nb, int* off, int2* blocks,
+                    int* scan_tmp, double4* fxb, double4* sxb);
+void fluid_filter(hipStream_t s, const SearchArgs& a, int* scan_tmp);
 void reverse_movable(hipStream_t s, const int* pair_key, const int* pair_off_total, int64_t cap,
                      const double4* key_vel, int nkeys, int* cnt, int* off, int* idx,
                      int* scan_tmp);
 void solid_save_history(hipStream_t s, Lists& L, int ns);
-void solid_search(hipStream_t s, const GridP& g, const Solids& so, int ns, const int* scs,
-                  const int* sce, const int* sperm, Lists& L, int* scan_tmp, long long* status);
+void active_grid(hipStream_t s, const Solids& so, const unsigned* skey, int ncell, int key_bits,
+                 unsigned* akey, unsigned* akey_sorted, int* aperm, int* acs, int* ace,
+                 unsigned* radix_tmp);
+void solid_search(hipStream_t s, const GridP& g, const Solids& so, int ns, const int* acs,
+                  const int* ace, const int* aperm, Lists& L, int* scan_tmp, long long* status);
 void reverse_solid(hipStream_t s, Lists& L, int ns, int* scan_tmp);
 void solid_normals(hipStream_t s, const GridP& g, const Solids& so, int ns, const int* scs,
                    const int* sce, const int* sperm, double h);

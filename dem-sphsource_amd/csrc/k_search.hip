@@ -1,22 +1,35 @@
-// k_search.hip — hashed-grid neighbour search on the reference's grid.
+// k_search.hip — hashed-grid neighbour search.
 //
 // Follows core/neighborSearch.cu (hash :19-36, fluid lists :115-220, solid
 // contact lists with spring history :222-312, driver :314-381) and the
 // solid-normal estimate of core/dataBase.cpp:348-413.
 //
-// Bit-exactness: the hash and every distance test are evaluated in FP64
+// Bit-exactness: the hash and every membership test are evaluated in FP64
 // with FMA contraction off, IEEE division and correctly rounded sqrt, so the
 // cell of a particle and the membership of a pair are the reference's.  The
 // distance test skips the sqrt when d^2 is outside [R^2(1-1e-14),
 // R^2(1+1e-14)] (sqrt_rn is monotone, so the answer is the same).
 //
-// MI355X design: one thread per particle in cell-sorted order (a wave covers
-// ~64 consecutive particles of one or two cells, so its 27-cell stencils and
-// neighbour gathers coincide); the reference's inflated cells (cell =
-// 2 max effective radius over all species) are pruned per lane with an exact
-// box-distance test, so a fluid particle only walks the cells that can hold a
-// neighbour.  The fluid–fluid list is the full (both-direction) list written
-// once in stencil order into an ELL table — no transpose sort.
+// MI355X design.  The reference grid's cells are sized by the largest
+// effective radius of any species (2.2 r of the DEM spheres), so a fluid
+// particle would test ~27 x 85 candidates for ~80 neighbours at 1M
+// particles.  Fluid lists therefore come from a Verlet superset: pairs within
+// 2 max(h_i,h_j) + skin (fluid) and 2 h_i + skin (solid), built by a
+// cell-block search only when the largest displacements since the last build
+// could have brought a new pair inside the cut-off
+// (d_fluid + max(d_fluid, d_solid) >= skin, decided on the device).  Every
+// step an exact FP64 filter of the superset yields the reference's lists.
+// For in-domain particles the reference's stencil membership is exactly
+// geometric, so the sets are identical; particles whose stencil reaches the
+// spare (out-of-domain) cell, and every particle while any is out of the
+// domain, take the reference's own per-lane stencil walk instead.
+//
+// The cell-block build puts one wavefront on <= 64 particles of one cell: the
+// 27-cell stencil walk is wave-uniform, candidate records are loaded once
+// per wave, and a stencil cell is skipped when no lane can reach it (ballot).
+// After a build the fluid particles are reordered into cell-sorted slots
+// (gather + copy-back, so device pointers never change and a captured HIP
+// graph stays valid).
 #include "dsph_internal.h"
 
 namespace dsph {
@@ -81,6 +94,18 @@ __device__ __forceinline__ double box_dist2(const GridP& g, int h, double px, do
     return ax * ax + ay * ay + az * az;
 }
 
+// does the reference stencil of a particle at (x,y,z) include the spare cell?
+__device__ __forceinline__ bool stencil_has_spare(const GridP& g, double x, double y, double z) {
+    int ax, ay, az;
+    grid_pos(g, x, y, z, ax, ay, az);
+    const int spare = g.ncell - 1;
+    for (int zz = -1; zz <= 1; zz++)
+        for (int yy = -1; yy <= 1; yy++)
+            for (int xx = -1; xx <= 1; xx++)
+                if ((az + zz) * g.gy * g.gx + (ay + yy) * g.gx + (ax + xx) == spare) return true;
+    return false;
+}
+
 __device__ __forceinline__ void wave_max_atomic(unsigned long long* addr, unsigned long long v) {
 #pragma unroll
     for (int d = 32; d > 0; d >>= 1) {
@@ -124,9 +149,51 @@ __global__ void k_hash_solid(GridP g, const double4* __restrict__ pr, int n,
     if ((threadIdx.x & 63) == 0 && m) atomicAdd(&st[ST_OUT_OF_DOMAIN], (unsigned long long)__popcll(m));
 }
 
+// --------------------------------------------------- rebuild decision
+__device__ __forceinline__ unsigned long long dbits(double d) {
+    return (unsigned long long)__double_as_longlong(d);
+}
+
+// largest displacement since the last build (positive doubles order like
+// their bit patterns)
+__global__ void k_disp(const double4* __restrict__ p, const double4* __restrict__ xb, int n,
+                       unsigned long long* slot) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    double d = 0.0;
+    if (s < n) {
+        const double4 a = p[s], b = xb[s];
+        const double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+        d = sqrt(dx * dx + dy * dy + dz * dz);
+        if (!(d == d)) d = 1e300;  // NaN forces a rebuild
+    }
+    wave_max_atomic(slot, dbits(d));
+}
+
+__global__ void k_decide(unsigned long long* ctl, unsigned long long* st, double skin) {
+    const double df = __longlong_as_double((long long)ctl[CTL_DISP_F]);
+    const double ds = __longlong_as_double((long long)ctl[CTL_DISP_S]);
+    const bool out = st[ST_OUT_OF_DOMAIN] != 0;
+    const bool move = df + (df > ds ? df : ds) >= skin * (1.0 - 1e-9);
+    // out-of-domain particles are missing from a superset built while they
+    // are out: rebuild until a build sees every particle inside
+    const bool stale = ctl[CTL_OUT_AT_BUILD] != 0;
+    const unsigned long long r =
+        (ctl[CTL_FORCE] != 0 || out || stale || skin <= 0.0 || move) ? 1ull : 0ull;
+    ctl[CTL_REBUILD] = r;
+    ctl[CTL_WALK_ALL] = out ? 1ull : 0ull;
+    if (r) ctl[CTL_OUT_AT_BUILD] = out ? 1ull : 0ull;
+    ctl[CTL_FORCE] = 0;
+    ctl[CTL_DISP_F] = 0;
+    ctl[CTL_DISP_S] = 0;
+    st[ST_REBUILDS] += r;
+}
+
+// --------------------------------------------- sorted-order reorder
 template <class R4>
 __global__ void k_reorder(FluidSet src, FluidSet dst, const int* __restrict__ perm,
-                          int* __restrict__ slot_of, int* __restrict__ member, int n, int cur) {
+                          int* __restrict__ slot_of, int n, int cur,
+                          const unsigned long long* __restrict__ ctl) {
+    if (ctl[CTL_REBUILD] == 0) return;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     int o = perm[t];
@@ -141,80 +208,58 @@ __global__ void k_reorder(FluidSet src, FluidSet dst, const int* __restrict__ pe
     ((R4*)dst.pv[cur])[t] = ((const R4*)src.pv[cur])[old];
     ((R4*)dst.pq[cur])[t] = ((const R4*)src.pq[cur])[old];
     slot_of[o] = t;
-    member[t] = t;
 }
 
-__global__ void k_member(const int* __restrict__ perm, const int* __restrict__ slot_of,
-                         int* __restrict__ member, int n) {
+template <class R4>
+__global__ void k_copyback(FluidSet dst, FluidSet src, int n, int cur,
+                           const unsigned long long* __restrict__ ctl) {
+    if (ctl[CTL_REBUILD] == 0) return;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) member[t] = slot_of[perm[t]];
+    if (t >= n) return;
+    dst.ph[t] = src.ph[t];
+    dst.vr[t] = src.vr[t];
+    dst.drho[cur][t] = src.drho[cur][t];
+    dst.cn[t] = src.cn[t];
+    dst.sumw0[t] = src.sumw0[t];
+    dst.acc[t] = src.acc[t];
+    dst.orig[t] = src.orig[t];
+    ((R4*)dst.pv[cur])[t] = ((const R4*)src.pv[cur])[t];
+    ((R4*)dst.pq[cur])[t] = ((const R4*)src.pq[cur])[t];
 }
 
-// Per-lane stencil walk of one fluid particle (the reference's own loop,
-// neighborSearch.cu:135-214) — used for particles in the spare cell, whose
-// stencils differ lane by lane.  COUNT: append fluid neighbours to the ELL
-// row and count solid neighbours; FILL: write the solid neighbours.
-template <bool FILL>
-__device__ void lane_search(const GridP& g, const double4 p, int s, int nf,
-                            const int* __restrict__ fcs, const int* __restrict__ fce,
-                            const int* __restrict__ fmember, const double4* __restrict__ ph,
-                            const int* __restrict__ scs, const int* __restrict__ sce,
-                            const int* __restrict__ sperm, const double4* __restrict__ spr,
-                            double hmax, int K, int* __restrict__ nbr, int& c, int& cs,
-                            int64_t w, int64_t cap, int* __restrict__ fs_i, int* __restrict__ fs_a,
-                            double4* __restrict__ fsF) {
-    const double hi = p.w;
-    int ax, ay, az;
-    grid_pos(g, p.x, p.y, p.z, ax, ay, az);
-    for (int zz = -1; zz <= 1; zz++)
-        for (int yy = -1; yy <= 1; yy++)
-            for (int xx = -1; xx <= 1; xx++) {
-                const int h = (az + zz) * g.gy * g.gx + (ay + yy) * g.gx + (ax + xx);
-                if (h < 0 || h >= g.ncell) continue;
-                if (!FILL) {
-                    const int st0 = fcs[h];
-                    if (st0 >= 0)
-                        for (int t = st0; t < fce[h]; t++) {
-                            const int j = fmember ? fmember[t] : t;
-                            if (j == s) continue;
-                            const double4 q = ph[j];
-                            if (within(p.x, p.y, p.z, q.x, q.y, q.z, 2.0 * fmax(hi, q.w))) {
-                                if (c < K) nbr[(int64_t)c * nf + s] = j;
-                                c++;
-                            }
-                        }
-                }
-                const int st1 = scs[h];
-                if (st1 < 0) continue;
-                for (int t = st1; t < sce[h]; t++) {
-                    const int a = sperm[t];
-                    const double4 q = spr[a];
-                    if (!within(p.x, p.y, p.z, q.x, q.y, q.z, 2.0 * hi)) continue;
-                    if (FILL) {
-                        if (w < cap) {
-                            fs_i[w] = s;
-                            fs_a[w] = a;
-                            fsF[w] = make_double4(0, 0, 0, 0);
-                        }
-                        w++;
-                    } else {
-                        cs++;
-                    }
-                }
-            }
+// slot of every sorted position, and the reference hash of every slot
+__global__ void k_member(const int* __restrict__ perm, const int* __restrict__ slot_of,
+                         const unsigned* __restrict__ sorted_key, int* __restrict__ member,
+                         unsigned* __restrict__ hslot, int n) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int s = slot_of[perm[t]];
+    member[t] = s;
+    hslot[s] = sorted_key[t];
 }
 
-// Cell blocks: every cell holding fluid is cut into blocks of <= 64
-// consecutive sorted particles, one wavefront each.
+__global__ void k_store_xb(const double4* __restrict__ p, double4* __restrict__ xb, int n,
+                           const unsigned long long* __restrict__ ctl) {
+    if (ctl[CTL_REBUILD] == 0) return;
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) xb[s] = p[s];
+}
+
+// ------------------------------------------------------- cell blocks
+// every cell holding fluid is cut into blocks of <= 64 consecutive sorted
+// particles, one wavefront each
 __global__ void k_blk_count(const int* __restrict__ cs, const int* __restrict__ ce, int ncell,
-                            int* __restrict__ nb) {
+                            int* __restrict__ nb, const unsigned long long* __restrict__ ctl) {
+    if (ctl[CTL_REBUILD] == 0) return;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncell) return;
     const int st = cs[c];
     nb[c] = st >= 0 ? (ce[c] - st + 63) >> 6 : 0;
 }
 __global__ void k_blk_fill(const int* __restrict__ cs, const int* __restrict__ ce, int ncell,
-                           const int* __restrict__ off, int2* __restrict__ blocks) {
+                           const int* __restrict__ off, int2* __restrict__ blocks,
+                           const unsigned long long* __restrict__ ctl) {
+    if (ctl[CTL_REBUILD] == 0) return;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncell) return;
     const int st = cs[c];
@@ -224,107 +269,256 @@ __global__ void k_blk_fill(const int* __restrict__ cs, const int* __restrict__ c
     for (int b = 0; b * 64 < n; b++) blocks[o + b] = make_int2(c, st + 64 * b);
 }
 
-// setFluidInteractionsKernel (neighborSearch.cu:115-220), one wavefront per
-// cell block.  All lanes of a block share the cell, so the 27-cell stencil
-// walk is wave-uniform: cell ranges and candidate records are loaded once
-// for the wave, and a stencil cell is skipped when no lane's box distance can
-// reach it (ballot).  Output: the full fluid–fluid list in stencil order
-// (ELL row of the particle), the fluid–solid count.
-template <bool FILL>
-__global__ void __launch_bounds__(kBS) k_fluid_blocks(
-    GridP g, const double4* __restrict__ ph, int nf, const int* __restrict__ fcs,
-    const int* __restrict__ fce, const int* __restrict__ fmember, const int* __restrict__ scs,
-    const int* __restrict__ sce, const int* __restrict__ sperm, const double4* __restrict__ spr,
-    double hmax, int K, int* __restrict__ nbr, int* __restrict__ cnt, int* __restrict__ fscnt,
-    const int2* __restrict__ blocks, const int* __restrict__ nblk_p, const int* __restrict__ fsoff,
-    int64_t cap, int* __restrict__ fs_i, int* __restrict__ fs_a, double4* __restrict__ fsF,
-    unsigned long long* st) {
+// Verlet superset build (setFluidInteractionsKernel's loop, radius + skin),
+// one wavefront per cell block.  Spare-cell particles and spare-cell
+// candidates are left out: they are walked exactly every step.
+__global__ void __launch_bounds__(kBS) k_build(SearchArgs a, const int2* __restrict__ blocks,
+                                               const int* __restrict__ nblk_p) {
+    if (a.ctl[CTL_REBUILD] == 0) return;
+    const GridP& g = a.g;
     const int lane = threadIdx.x & 63;
     const int wv = xcd_block() * (kBS / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nwv = gridDim.x * (kBS / 64);
     const int nblk = *nblk_p;
-    unsigned long long cmax = 0;
+    const int spare = g.ncell - 1;
+    const double skin = a.skin;
+    const int supK = a.L.supK, fsupK = a.L.fsupK;
     bool ovf = false;
     for (int b = wv; b < nblk; b += nwv) {
         const int2 blk = blocks[b];
         const int cell = __builtin_amdgcn_readfirstlane(blk.x);
         const int t0 = __builtin_amdgcn_readfirstlane(blk.y);
-        const int nmem = min(64, fce[cell] - t0);
+        const int nmem = min(64, a.fce[cell] - t0);
         const bool act = lane < nmem;
-        const int t = t0 + (act ? lane : 0);
-        const int s = fmember ? fmember[t] : t;
-        const double4 p = ph[s];
-        int c = 0, cs = 0;
-        int64_t w = 0;
-        if (FILL) {
-            const int mine = act ? fscnt[s] : 0;
-            if (__ballot(mine > 0) == 0) continue;
-            w = fsoff[s];
+        const int s = a.fmember[t0 + (act ? lane : 0)];
+        if (cell == spare) {  // walked exactly while out of the domain
+            if (act) {
+                a.L.sup_cnt[s] = 0;
+                a.L.fsup_cnt[s] = 0;
+            }
+            continue;
         }
-        if (cell == g.ncell - 1) {  // spare cell: stencils differ per lane
-            if (act)
-                lane_search<FILL>(g, p, s, nf, fcs, fce, fmember, ph, scs, sce, sperm, spr, hmax, K,
-                                  nbr, c, cs, w, cap, fs_i, fs_a, fsF);
-        } else {
-            const int gxy = g.gx * g.gy;
-            const int cz = cell / gxy, rem = cell - cz * gxy;
-            const int cy = rem / g.gx, cx = rem - cy * g.gx;
-            const double hi = p.w;
-            const double Rf = 2.0 * fmax(hi, hmax) * (1.0 + 1e-9);
-            const double Rs = 2.0 * hi * (1.0 + 1e-9);
-            for (int zz = -1; zz <= 1; zz++)
-                for (int yy = -1; yy <= 1; yy++)
-                    for (int xx = -1; xx <= 1; xx++) {
-                        const int h = (cz + zz) * gxy + (cy + yy) * g.gx + (cx + xx);
-                        if (h < 0 || h >= g.ncell) continue;
-                        const double bd2 = (h == g.ncell - 1) ? 0.0 : box_dist2(g, h, p.x, p.y, p.z);
-                        if (!FILL && __ballot(act && bd2 <= Rf * Rf) != 0) {
-                            const int st0 = __builtin_amdgcn_readfirstlane(fcs[h]);
-                            if (st0 >= 0) {
-                                const int en = __builtin_amdgcn_readfirstlane(fce[h]);
-                                for (int u = st0; u < en; u++) {
-                                    const int j = __builtin_amdgcn_readfirstlane(fmember ? fmember[u] : u);
-                                    const double4 q = ph[j];
-                                    if (act && j != s &&
-                                        within(p.x, p.y, p.z, q.x, q.y, q.z, 2.0 * fmax(hi, q.w))) {
-                                        if (c < K) nbr[(int64_t)c * nf + s] = j;
-                                        c++;
-                                    }
-                                }
-                            }
-                        }
-                        if (__ballot(act && bd2 <= Rs * Rs) != 0) {
-                            const int st1 = __builtin_amdgcn_readfirstlane(scs[h]);
-                            if (st1 < 0) continue;
-                            const int en = __builtin_amdgcn_readfirstlane(sce[h]);
-                            for (int u = st1; u < en; u++) {
-                                const int a = __builtin_amdgcn_readfirstlane(sperm[u]);
-                                const double4 q = spr[a];
-                                if (!(act && within(p.x, p.y, p.z, q.x, q.y, q.z, 2.0 * hi))) continue;
-                                if (FILL) {
-                                    if (w < cap) {
-                                        fs_i[w] = s;
-                                        fs_a[w] = a;
-                                        fsF[w] = make_double4(0, 0, 0, 0);
-                                    }
-                                    w++;
-                                } else {
-                                    cs++;
+        const double4 p = a.ph[s];
+        const double hi = p.w;
+        const double Rf = (2.0 * fmax(hi, a.hmax) + skin) * (1.0 + 1e-9);
+        const double Rs = (2.0 * hi + skin) * (1.0 + 1e-9);
+        int c = 0, cs = 0;
+        const int gxy = g.gx * g.gy;
+        const int cz = cell / gxy, rem = cell - cz * gxy;
+        const int cy = rem / g.gx, cx = rem - cy * g.gx;
+        for (int zz = -1; zz <= 1; zz++)
+            for (int yy = -1; yy <= 1; yy++)
+                for (int xx = -1; xx <= 1; xx++) {
+                    const int h = (cz + zz) * gxy + (cy + yy) * g.gx + (cx + xx);
+                    if (h < 0 || h >= spare) continue;
+                    const double bd2 = box_dist2(g, h, p.x, p.y, p.z);
+                    if (__ballot(act && bd2 <= Rf * Rf) != 0) {
+                        const int st0 = __builtin_amdgcn_readfirstlane(a.fcs[h]);
+                        if (st0 >= 0) {
+                            const int en = __builtin_amdgcn_readfirstlane(a.fce[h]);
+                            for (int u = st0; u < en; u++) {
+                                const int j = __builtin_amdgcn_readfirstlane(a.fmember[u]);
+                                const double4 q = a.ph[j];
+                                if (act && j != s &&
+                                    within(p.x, p.y, p.z, q.x, q.y, q.z, 2.0 * fmax(hi, q.w) + skin)) {
+                                    if (c < supK) a.L.sup[(int64_t)c * a.nf + s] = j;
+                                    c++;
                                 }
                             }
                         }
                     }
+                    if (__ballot(act && bd2 <= Rs * Rs) != 0) {
+                        const int st1 = __builtin_amdgcn_readfirstlane(a.scs[h]);
+                        if (st1 < 0) continue;
+                        const int en = __builtin_amdgcn_readfirstlane(a.sce[h]);
+                        for (int u = st1; u < en; u++) {
+                            const int ai = __builtin_amdgcn_readfirstlane(a.sperm[u]);
+                            const double4 q = a.spr[ai];
+                            if (act && within(p.x, p.y, p.z, q.x, q.y, q.z, 2.0 * hi + skin)) {
+                                if (cs < fsupK) a.L.fsup[(int64_t)cs * a.nf + s] = ai;
+                                cs++;
+                            }
+                        }
+                    }
+                }
+        if (act) {
+            a.L.sup_cnt[s] = c;
+            a.L.fsup_cnt[s] = cs;
         }
-        if (!FILL && act) {
-            cnt[s] = c;
-            fscnt[s] = cs;
+        ovf = ovf || c > supK || cs > fsupK;
+    }
+    if (ovf) atomicOr(&a.st[ST_SUP_OVERFLOW], 1ull);
+}
+
+// number of the 27 linear stencil offsets around cell (ax, ay, az) that give
+// hash h, for a hash h of a real or spare cell (so in [0, ncell)); the
+// reference visits a cell once per offset that reaches it
+// (neighborSearch.cu:135-146)
+__device__ __forceinline__ int stencil_count(const GridP& g, int ax, int ay, int az, int h) {
+    const int base = az * g.gy * g.gx + ay * g.gx + ax;
+    int m = 0;
+    for (int zz = -1; zz <= 1; zz++)
+        for (int yy = -1; yy <= 1; yy++)
+            for (int xx = -1; xx <= 1; xx++)
+                m += (base + zz * g.gy * g.gx + yy * g.gx + xx) == h;
+    return m;
+}
+
+// Fluid neighbours of slot s as the reference's pair loops see them.  The
+// reference lists (i, j), j > i, for each j its stencil walk from i finds
+// within range (neighborSearch.cu:150-170), and the loops of particle i run
+// over that half plus every (j, i), j < i, that j's walk found
+// (integrate.cu:55-161).  Between two particles of real cells the stencil
+// relation is symmetric and this is every particle of i's stencil within
+// range; with a particle in the spare cell the two directions differ (its
+// hash is the spare cell, its stencil is centred on its position), so every
+// candidate is kept in its reference direction only.  Candidates: i's
+// stencil cells, the spare cell and, for i in the spare cell, the cells whose
+// stencils reach the spare cell.  A cell reached by several offsets (grids
+// narrower than three cells in x or y) is counted once per offset.
+__device__ void ff_walk(const SearchArgs& a, const double4 p, int s, int& c) {
+    const GridP& g = a.g;
+    const int spare = g.ncell - 1;
+    const int hs = (int)a.hslot[s];
+    const int oi = a.forig[s];
+    const double hi = p.w;
+    int ax, ay, az;
+    grid_pos(g, p.x, p.y, p.z, ax, ay, az);
+    int cand[55];
+    int nc = 0;
+    auto push = [&](int h, bool dedup) {
+        if (h < 0 || h >= g.ncell) return;
+        if (dedup)
+            for (int k = 0; k < nc; k++)
+                if (cand[k] == h) return;
+        cand[nc++] = h;
+    };
+    for (int zz = -1; zz <= 1; zz++)
+        for (int yy = -1; yy <= 1; yy++)
+            for (int xx = -1; xx <= 1; xx++)
+                push((az + zz) * g.gy * g.gx + (ay + yy) * g.gx + (ax + xx), g.collide != 0);
+    push(spare, true);
+    if (hs == spare)
+        for (int zz = -1; zz <= 1; zz++)
+            for (int yy = -1; yy <= 1; yy++)
+                for (int xx = -1; xx <= 1; xx++)
+                    push(spare - (zz * g.gy * g.gx + yy * g.gx + xx), true);
+    for (int k = 0; k < nc; k++) {
+        const int h = cand[k];
+        const int st0 = a.fcs[h];
+        if (st0 < 0) continue;
+        const int mh = stencil_count(g, ax, ay, az, h);  // offsets of i's stencil reaching h
+        const bool sym = h != spare && hs != spare;
+        for (int t = st0; t < a.fce[h]; t++) {
+            const int j = a.fmember[t];
+            if (j == s) continue;
+            const double4 q = a.ph[j];
+            if (!within(p.x, p.y, p.z, q.x, q.y, q.z, 2.0 * fmax(hi, q.w))) continue;
+            int m = mh;
+            if (a.forig[j] < oi && !sym) {
+                int bx, by, bz;
+                grid_pos(g, q.x, q.y, q.z, bx, by, bz);
+                m = stencil_count(g, bx, by, bz, hs);  // offsets of j's stencil reaching i
+            }
+            for (; m > 0; m--) {
+                if (c < a.L.K) a.L.nbr[(int64_t)c * a.nf + s] = j;
+                c++;
+            }
         }
-        cmax = cmax > (unsigned long long)c ? cmax : (unsigned long long)c;
-        ovf = ovf || c > K;
+    }
+}
+
+// The reference's per-lane stencil walk (neighborSearch.cu:135-214).
+// COUNT: fluid neighbours into the ELL row, solid neighbours counted;
+// FILL: solid neighbours written to the CSR list.
+template <bool FILL>
+__device__ void lane_walk(const SearchArgs& a, const double4 p, int s, int& c, int& cs,
+                          int64_t& w) {
+    const GridP& g = a.g;
+    const double hi = p.w;
+    if (!FILL) ff_walk(a, p, s, c);
+    int ax, ay, az;
+    grid_pos(g, p.x, p.y, p.z, ax, ay, az);
+    for (int zz = -1; zz <= 1; zz++)
+        for (int yy = -1; yy <= 1; yy++)
+            for (int xx = -1; xx <= 1; xx++) {
+                const int h = (az + zz) * g.gy * g.gx + (ay + yy) * g.gx + (ax + xx);
+                if (h < 0 || h >= g.ncell) continue;
+                const int st1 = a.scs[h];
+                if (st1 < 0) continue;
+                for (int t = st1; t < a.sce[h]; t++) {
+                    const int ai = a.sperm[t];
+                    const double4 q = a.spr[ai];
+                    if (!within(p.x, p.y, p.z, q.x, q.y, q.z, 2.0 * hi)) continue;
+                    if (FILL) {
+                        if (w < a.L.fs_cap) {
+                            a.L.fs_i[w] = s;
+                            a.L.fs_a[w] = ai;
+                            a.L.fsF[w] = make_double4(0, 0, 0, 0);
+                        }
+                        w++;
+                    } else {
+                        cs++;
+                    }
+                }
+            }
+}
+
+// exact lists of this step from the superset (or the exact walk)
+template <bool FILL>
+__global__ void __launch_bounds__(kBS) k_filter(SearchArgs a) {
+    const int s = xcd_block() * blockDim.x + threadIdx.x;
+    int c = 0, cs = 0;
+    if (s < a.nf) {
+        const double4 p = a.ph[s];
+        const unsigned spare = (unsigned)(a.g.ncell - 1);
+        const bool walk = a.ctl[CTL_WALK_ALL] != 0 || a.g.collide || a.hslot[s] == spare ||
+                          stencil_has_spare(a.g, p.x, p.y, p.z);
+        int64_t w = FILL ? (int64_t)a.L.fsoff[s] : 0;
+        if (walk) {
+            lane_walk<FILL>(a, p, s, c, cs, w);
+        } else {
+            const double hi = p.w;
+            if (!FILL) {
+                const int n = min(a.L.sup_cnt[s], a.L.supK);
+                const int* __restrict__ sup = a.L.sup + s;
+                for (int k = 0; k < n; k++) {
+                    const int j = sup[(int64_t)k * a.nf];
+                    if (a.hslot[j] == spare) continue;
+                    const double4 q = a.ph[j];
+                    if (within(p.x, p.y, p.z, q.x, q.y, q.z, 2.0 * fmax(hi, q.w))) {
+                        if (c < a.L.K) a.L.nbr[(int64_t)c * a.nf + s] = j;
+                        c++;
+                    }
+                }
+            }
+            const int n2 = min(a.L.fsup_cnt[s], a.L.fsupK);
+            const int* __restrict__ fsup = a.L.fsup + s;
+            for (int k = 0; k < n2; k++) {
+                const int ai = fsup[(int64_t)k * a.nf];
+                if (a.skey[ai] == spare) continue;
+                const double4 q = a.spr[ai];
+                if (!within(p.x, p.y, p.z, q.x, q.y, q.z, 2.0 * hi)) continue;
+                if (FILL) {
+                    if (w < a.L.fs_cap) {
+                        a.L.fs_i[w] = s;
+                        a.L.fs_a[w] = ai;
+                        a.L.fsF[w] = make_double4(0, 0, 0, 0);
+                    }
+                    w++;
+                } else {
+                    cs++;
+                }
+            }
+        }
+        if (!FILL) {
+            a.L.cnt[s] = c;
+            a.L.fscnt[s] = cs;
+        }
     }
     if (!FILL) {
-        wave_max_atomic(&st[ST_FF_MAX], cmax);
-        if (ovf) atomicOr(&st[ST_FF_OVERFLOW], 1ull);
+        wave_max_atomic(&a.st[ST_FF_MAX], (unsigned long long)c);
+        if (c > a.L.K) atomicOr(&a.st[ST_FF_OVERFLOW], 1ull);
     }
 }
 
@@ -335,6 +529,7 @@ __global__ void k_fs_status(const int* __restrict__ fsoff, int nf, int64_t cap,
     if (tot > cap) st[ST_FS_OVERFLOW] |= 1ull;
 }
 
+// ------------------------------------------------ reverse (transposed) lists
 // reverse CSR of a pair list by its second index (optionally only keys with
 // a positive inverse mass); within a key the pair ids are ascending, as the
 // reference's stable transpose sort gives (neighborSearch.cu:345-354)
@@ -377,6 +572,7 @@ __global__ void k_rev_sort(const int* __restrict__ off, int n, int* __restrict__
     }
 }
 
+// ------------------------------------------------------- DEM contacts
 // interactionSolid2Solid::save (deviceStructs.h:693-706): keep the previous
 // list's offsets, partners and springs for the history lookup
 __global__ void k_ss_save(const int* __restrict__ ssoff, int ns, const int* __restrict__ ss_j,
@@ -395,26 +591,32 @@ __global__ void k_ss_save(const int* __restrict__ ssoff, int ns, const int* __re
     }
 }
 
-// setSolid2SolidInteractionsKernel (neighborSearch.cu:222-312): pass 0
-// counts, pass 1 fills + carries the springs of the same (i, j) over
+__global__ void k_active_keys(const int* __restrict__ active, int n, const unsigned* __restrict__ skey,
+                              unsigned* __restrict__ akey) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) akey[k] = skey[active[k]];
+}
+
+// setSolid2SolidInteractionsKernel (neighborSearch.cu:222-312) over the
+// non-dummy solids only (dummies are never contacts): pass 0 counts, pass 1
+// fills and carries the springs of the same (i, j) over from the previous
+// step's list.  The grid of active solids keeps the reference's order
+// (stencil order, ascending index within a cell).
 template <bool FILL>
 __global__ void __launch_bounds__(kBS) k_ss_search(
-    GridP g, Solids so, int ns, const int* __restrict__ scs, const int* __restrict__ sce,
-    const int* __restrict__ sperm, int* __restrict__ sscnt, const int* __restrict__ ssoff,
+    GridP g, Solids so, int ns, const int* __restrict__ acs, const int* __restrict__ ace,
+    const int* __restrict__ aperm, int* __restrict__ sscnt, const int* __restrict__ ssoff,
     int64_t cap, int* __restrict__ ss_i, int* __restrict__ ss_j, double4* __restrict__ ssF,
     double4* __restrict__ ssT, double* __restrict__ ssS, const int* __restrict__ hoff,
     const int* __restrict__ h_j, const double* __restrict__ hS, unsigned long long* st) {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (FILL && a == 0) {
+    const int k0 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (FILL && k0 == 0) {
         long long tot = ssoff[ns];
         st[ST_SS_TOTAL] = (unsigned long long)tot;
         if (tot > cap) atomicOr(&st[ST_SS_OVERFLOW], 1ull);
     }
-    if (a >= ns) return;
-    if (so.dummy[a] == 1) {
-        if (!FILL) sscnt[a] = 0;
-        return;
-    }
+    if (k0 >= so.nactive) return;
+    const int a = so.active[k0];
     if (FILL && sscnt[a] == 0) return;
     const double4 p = so.pr[a];
     const double ra = so.eff[a];
@@ -434,13 +636,12 @@ __global__ void __launch_bounds__(kBS) k_ss_search(
             for (int xx = -1; xx <= 1; xx++) {
                 const int h = (az + zz) * g.gy * g.gx + (ay + yy) * g.gx + (ax + xx);
                 if (h < 0 || h >= g.ncell) continue;
-                const int st0 = scs[h];
+                const int st0 = acs[h];
                 if (st0 < 0) continue;
-                const int en = sce[h];
+                const int en = ace[h];
                 for (int t = st0; t < en; t++) {
-                    const int b = sperm[t];
+                    const int b = so.active[aperm[t]];
                     if (b <= a) continue;
-                    if (so.dummy[b] == 1) continue;
                     if (ca >= 0 && ca == so.clump[b]) continue;
                     if (fixa && so.vel[b].w < 1.e-10) continue;
                     const double4 q = so.pr[b];
@@ -514,6 +715,11 @@ __global__ void k_normals(GridP g, Solids so, int ns, const int* __restrict__ sc
     so.nrm[a] = make_double4(n.x, n.y, n.z, 0.0);
 }
 
+int cap_grid(int64_t n, int cap) {
+    int b = blocks_for(n);
+    return b > cap ? cap : b;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- wrappers
@@ -532,28 +738,35 @@ void hash_solid(hipStream_t s, const GridP& g, const double4* pr, int n, unsigne
                            (unsigned long long*)status);
 }
 
-void reorder_fluid(hipStream_t s, const FluidSet& src, const FluidSet& dst, const int* perm,
-                   int* slot_of, int* member, int n, bool mixed, int cur) {
+void rebuild_decision(hipStream_t s, const double4* fph, const double4* fxb, int nf,
+                      const double4* spr, const double4* sxb, int ns, double skin,
+                      unsigned long long* ctl, long long* status) {
+    if (nf > 0)
+        hipLaunchKernelGGL(k_disp, dim3(blocks_for(nf)), dim3(kBS), 0, s, fph, fxb, nf, ctl + CTL_DISP_F);
+    if (ns > 0)
+        hipLaunchKernelGGL(k_disp, dim3(blocks_for(ns)), dim3(kBS), 0, s, spr, sxb, ns, ctl + CTL_DISP_S);
+    hipLaunchKernelGGL(k_decide, dim3(1), dim3(1), 0, s, ctl, (unsigned long long*)status, skin);
+}
+
+void reorder_fluid(hipStream_t s, const FluidSet& f, const FluidSet& scratch, const int* perm,
+                   int* slot_of, int n, bool mixed, int cur, const unsigned long long* ctl) {
     if (n <= 0) return;
-    if (mixed)
-        hipLaunchKernelGGL(k_reorder<float4>, dim3(blocks_for(n)), dim3(kBS), 0, s, src, dst, perm,
-                           slot_of, member, n, cur);
-    else
-        hipLaunchKernelGGL(k_reorder<double4>, dim3(blocks_for(n)), dim3(kBS), 0, s, src, dst, perm,
-                           slot_of, member, n, cur);
+    if (mixed) {
+        hipLaunchKernelGGL(k_reorder<float4>, dim3(blocks_for(n)), dim3(kBS), 0, s, f, scratch, perm,
+                           slot_of, n, cur, ctl);
+        hipLaunchKernelGGL(k_copyback<float4>, dim3(blocks_for(n)), dim3(kBS), 0, s, f, scratch, n, cur, ctl);
+    } else {
+        hipLaunchKernelGGL(k_reorder<double4>, dim3(blocks_for(n)), dim3(kBS), 0, s, f, scratch, perm,
+                           slot_of, n, cur, ctl);
+        hipLaunchKernelGGL(k_copyback<double4>, dim3(blocks_for(n)), dim3(kBS), 0, s, f, scratch, n, cur, ctl);
+    }
 }
 
-void member_slots(hipStream_t s, const int* perm, const int* slot_of, int* member, int n) {
+void member_slots(hipStream_t s, const int* perm, const int* slot_of, const unsigned* sorted_key,
+                  int* member, unsigned* hslot, int n) {
     if (n > 0)
-        hipLaunchKernelGGL(k_member, dim3(blocks_for(n)), dim3(kBS), 0, s, perm, slot_of, member, n);
-}
-
-void fluid_blocks(hipStream_t s, const int* fcs, const int* fce, int ncell, int* nb, int* off,
-                  int2* blocks, int* scan_tmp) {
-    hipLaunchKernelGGL(k_blk_count, dim3(blocks_for(ncell)), dim3(kBS), 0, s, fcs, fce, ncell, nb);
-    exclusive_scan(s, nb, off, ncell, scan_tmp);
-    hipLaunchKernelGGL(k_blk_fill, dim3(blocks_for(ncell)), dim3(kBS), 0, s, fcs, fce, ncell,
-                       (const int*)off, blocks);
+        hipLaunchKernelGGL(k_member, dim3(blocks_for(n)), dim3(kBS), 0, s, perm, slot_of, sorted_key,
+                           member, hslot, n);
 }
 
 static int wave_grid(int nf) {
@@ -563,28 +776,31 @@ static int wave_grid(int nf) {
     return b < 1 ? 1 : (b > 16384 ? 16384 : b);
 }
 
-void fluid_search(hipStream_t s, const GridP& g, const FluidSet& f, int nf, const int* fcs,
-                  const int* fce, const int* fmember, const int* scs, const int* sce,
-                  const int* sperm, const Solids& so, double hmax, Lists& L, const int2* blocks,
-                  const int* nblk, long long* status) {
-    if (nf <= 0) return;
-    hipLaunchKernelGGL(k_fluid_blocks<false>, dim3(wave_grid(nf)), dim3(kBS), 0, s, g, f.ph, nf, fcs,
-                       fce, fmember, scs, sce, sperm, so.pr, hmax, L.K, L.nbr, L.cnt, L.fscnt, blocks,
-                       nblk, (const int*)nullptr, (int64_t)0, (int*)nullptr, (int*)nullptr,
-                       (double4*)nullptr, (unsigned long long*)status);
+void fluid_superset(hipStream_t s, const SearchArgs& a, int* nb, int* off, int2* blocks,
+                    int* scan_tmp, double4* fxb, double4* sxb) {
+    if (a.nf <= 0) return;
+    const int nc = a.g.ncell;
+    hipLaunchKernelGGL(k_blk_count, dim3(blocks_for(nc)), dim3(kBS), 0, s, a.fcs, a.fce, nc, nb,
+                       (const unsigned long long*)a.ctl);
+    exclusive_scan(s, nb, off, nc, scan_tmp);
+    hipLaunchKernelGGL(k_blk_fill, dim3(blocks_for(nc)), dim3(kBS), 0, s, a.fcs, a.fce, nc,
+                       (const int*)off, blocks, (const unsigned long long*)a.ctl);
+    hipLaunchKernelGGL(k_build, dim3(wave_grid(a.nf)), dim3(kBS), 0, s, a, (const int2*)blocks,
+                       (const int*)(off + nc));
+    hipLaunchKernelGGL(k_store_xb, dim3(blocks_for(a.nf)), dim3(kBS), 0, s, a.ph, fxb, a.nf,
+                       (const unsigned long long*)a.ctl);
+    if (a.ns > 0)
+        hipLaunchKernelGGL(k_store_xb, dim3(blocks_for(a.ns)), dim3(kBS), 0, s, a.spr, sxb, a.ns,
+                           (const unsigned long long*)a.ctl);
 }
 
-void fluid_solid_fill(hipStream_t s, const GridP& g, const FluidSet& f, int nf, const int* fcs,
-                      const int* fce, const int* fmember, const int* scs, const int* sce,
-                      const int* sperm, const Solids& so, Lists& L, const int2* blocks,
-                      const int* nblk, long long* status) {
-    if (nf <= 0) return;
-    hipLaunchKernelGGL(k_fs_status, dim3(1), dim3(1), 0, s, (const int*)L.fsoff, nf, L.fs_cap,
-                       (unsigned long long*)status);
-    hipLaunchKernelGGL(k_fluid_blocks<true>, dim3(wave_grid(nf)), dim3(kBS), 0, s, g, f.ph, nf, fcs,
-                       fce, fmember, scs, sce, sperm, so.pr, 0.0, L.K, L.nbr, L.cnt, L.fscnt, blocks,
-                       nblk, (const int*)L.fsoff, L.fs_cap, L.fs_i, L.fs_a, L.fsF,
-                       (unsigned long long*)status);
+void fluid_filter(hipStream_t s, const SearchArgs& a, int* scan_tmp) {
+    if (a.nf <= 0) return;
+    hipLaunchKernelGGL(k_filter<false>, dim3(blocks_for(a.nf)), dim3(kBS), 0, s, a);
+    exclusive_scan(s, a.L.fscnt, a.L.fsoff, a.nf, scan_tmp);
+    hipLaunchKernelGGL(k_fs_status, dim3(1), dim3(1), 0, s, (const int*)a.L.fsoff, a.nf, a.L.fs_cap,
+                       a.st);
+    hipLaunchKernelGGL(k_filter<true>, dim3(blocks_for(a.nf)), dim3(kBS), 0, s, a);
 }
 
 static void reverse_build(hipStream_t s, const int* key, const int* total_p, int64_t cap,
@@ -592,8 +808,7 @@ static void reverse_build(hipStream_t s, const int* key, const int* total_p, int
                           int* scan_tmp) {
     if (nkeys <= 0) return;
     fill_int(s, cnt, nkeys, 0);
-    int gb = blocks_for(cap);
-    if (gb > 2048) gb = 2048;
+    const int gb = cap_grid(cap, 2048);
     if (cap > 0) hipLaunchKernelGGL(k_rev_count, dim3(gb), dim3(kBS), 0, s, key, total_p, cap, kvel, cnt);
     exclusive_scan(s, cnt, off, nkeys, scan_tmp);
     fill_int(s, cnt, nkeys, 0);
@@ -611,23 +826,35 @@ void reverse_movable(hipStream_t s, const int* pair_key, const int* pair_total, 
 
 void solid_save_history(hipStream_t s, Lists& L, int ns) {
     if (ns <= 0) return;
-    int gb = blocks_for(L.ss_cap > ns + 1 ? L.ss_cap : ns + 1);
-    if (gb > 2048) gb = 2048;
-    hipLaunchKernelGGL(k_ss_save, dim3(gb), dim3(kBS), 0, s, (const int*)L.ssoff, ns,
-                       (const int*)L.ss_j, (const double*)L.ssS, L.ss_cap, L.hoff, L.h_j, L.hS);
+    hipLaunchKernelGGL(k_ss_save, dim3(cap_grid(L.ss_cap > ns + 1 ? L.ss_cap : ns + 1, 2048)), dim3(kBS),
+                       0, s, (const int*)L.ssoff, ns, (const int*)L.ss_j, (const double*)L.ssS, L.ss_cap,
+                       L.hoff, L.h_j, L.hS);
 }
 
-void solid_search(hipStream_t s, const GridP& g, const Solids& so, int ns, const int* scs,
-                  const int* sce, const int* sperm, Lists& L, int* scan_tmp, long long* status) {
+void active_grid(hipStream_t s, const Solids& so, const unsigned* skey, int ncell, int key_bits,
+                 unsigned* akey, unsigned* akey_sorted, int* aperm, int* acs, int* ace,
+                 unsigned* radix_tmp) {
+    const int n = so.nactive;
+    if (n > 0) {
+        hipLaunchKernelGGL(k_active_keys, dim3(blocks_for(n)), dim3(kBS), 0, s, (const int*)so.active, n,
+                           skey, akey);
+        radix_sort_iota(s, akey, n, key_bits, akey_sorted, aperm, radix_tmp);
+    }
+    cell_spans(s, akey_sorted, n, acs, ace, ncell);
+}
+
+void solid_search(hipStream_t s, const GridP& g, const Solids& so, int ns, const int* acs,
+                  const int* ace, const int* aperm, Lists& L, int* scan_tmp, long long* status) {
     if (ns <= 0) return;
-    hipLaunchKernelGGL(k_ss_search<false>, dim3(blocks_for(ns)), dim3(kBS), 0, s, g, so, ns, scs, sce,
-                       sperm, L.sscnt, (const int*)L.ssoff, L.ss_cap, L.ss_i, L.ss_j, L.ssF, L.ssT,
-                       L.ssS, (const int*)L.hoff, (const int*)L.h_j, (const double*)L.hS,
+    const int nb = blocks_for(so.nactive > 0 ? so.nactive : 1);
+    hipLaunchKernelGGL(k_ss_search<false>, dim3(nb), dim3(kBS), 0, s, g, so, ns, acs, ace, aperm, L.sscnt,
+                       (const int*)L.ssoff, L.ss_cap, L.ss_i, L.ss_j, L.ssF, L.ssT, L.ssS,
+                       (const int*)L.hoff, (const int*)L.h_j, (const double*)L.hS,
                        (unsigned long long*)status);
     exclusive_scan(s, L.sscnt, L.ssoff, ns, scan_tmp);
-    hipLaunchKernelGGL(k_ss_search<true>, dim3(blocks_for(ns)), dim3(kBS), 0, s, g, so, ns, scs, sce,
-                       sperm, L.sscnt, (const int*)L.ssoff, L.ss_cap, L.ss_i, L.ss_j, L.ssF, L.ssT,
-                       L.ssS, (const int*)L.hoff, (const int*)L.h_j, (const double*)L.hS,
+    hipLaunchKernelGGL(k_ss_search<true>, dim3(nb), dim3(kBS), 0, s, g, so, ns, acs, ace, aperm, L.sscnt,
+                       (const int*)L.ssoff, L.ss_cap, L.ss_i, L.ss_j, L.ssF, L.ssT, L.ssS,
+                       (const int*)L.hoff, (const int*)L.h_j, (const double*)L.hS,
                        (unsigned long long*)status);
 }
 

@@ -25,7 +25,8 @@ PREC_FP64, PREC_MIXED = 0, 1
 FLAG_WALL_CONTINUE, FLAG_CLUMP_QUATERNION, FLAG_ART_VISCOSITY = 1, 2, 4
 LIST_FLUID_FLUID, LIST_FLUID_SOLID, LIST_SOLID_SOLID, LIST_BOND = 0, 1, 2, 3
 ST_FF_OVERFLOW, ST_FS_OVERFLOW, ST_SS_OVERFLOW, ST_OUT_OF_DOMAIN = 0, 1, 2, 3
-ST_FF_MAX, ST_FS_TOTAL, ST_SS_TOTAL = 4, 5, 6
+ST_FF_MAX, ST_FS_TOTAL, ST_SS_TOTAL, ST_SUP_OVERFLOW, ST_REBUILDS = 4, 5, 6, 7, 8
+STATUS_WORDS = 12
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -39,6 +40,8 @@ _SIGS = {
     "dsph_set_flags": [C.c_uint32],
     "dsph_set_gravity": [_dp],
     "dsph_set_reorder_period": [C.c_int32],
+    "dsph_set_skin": [C.c_double],
+    "dsph_set_superset_capacity": [C.c_int32, C.c_int32],
     "dsph_initialize": [C.c_int],
     "dsph_neighbor_search": [C.c_int32, C.c_int32],
     "dsph_solid_before_contact": [C.c_double],
@@ -148,6 +151,13 @@ class Solver:
 
     def set_reorder_period(self, p):
         self._call("dsph_set_reorder_period", p)
+
+    def set_skin(self, skin_over_h):
+        """Verlet skin of the fluid superset, in units of the largest h."""
+        self._call("dsph_set_skin", float(skin_over_h))
+
+    def set_superset_capacity(self, ff=0, fs=0):
+        self._call("dsph_set_superset_capacity", ff, fs)
 
     def initialize(self, compute_normals=True):
         self._call("dsph_initialize", int(compute_normals))
@@ -267,7 +277,7 @@ class Solver:
         return dict(hash=h[:n], perm=p[:n], start=s, end=e)
 
     def status(self):
-        w = np.zeros(8, np.int64)
+        w = np.zeros(STATUS_WORDS, np.int64)
         self._call("dsph_status", _p(w, _i64p))
         return w
 

@@ -70,7 +70,9 @@ extern "C" {
 #define DSPH_STATUS_FF_MAX        4 /* max fluid-fluid neighbors of a particle*/
 #define DSPH_STATUS_FS_TOTAL      5 /* fluid-solid pairs found               */
 #define DSPH_STATUS_SS_TOTAL      6 /* solid-solid contacts found            */
-#define DSPH_STATUS_WORDS         8
+#define DSPH_STATUS_SUP_OVERFLOW  7 /* a fluid Verlet superset row overflowed */
+#define DSPH_STATUS_REBUILDS      8 /* superset rebuilds since upload        */
+#define DSPH_STATUS_WORDS        12
 
 /*
  * Scene upload format: the reference's HostData SoA (hostStructs.h:7-399)
@@ -151,10 +153,19 @@ const char* dsph_version(void);
 int dsph_upload_scene(dsph_ctx* ctx, const dsph_scene* scene);
 int dsph_set_capacity(dsph_ctx* ctx, int32_t ff_per_particle,
                       int64_t fs_total, int64_t ss_total);
+/* per-particle rows of the fluid–fluid and fluid–solid Verlet supersets */
+int dsph_set_superset_capacity(dsph_ctx* ctx, int32_t ff_per_particle,
+                               int32_t fs_per_particle);
 int dsph_set_flags(dsph_ctx* ctx, uint32_t flags);
 int dsph_set_gravity(dsph_ctx* ctx, const double g[3]); /* dataBase.h:105-109 */
-/* sorted-order particle reorder period in fluid searches (1 = every search) */
+/* 0 disables the sorted-order reorder of fluid particles that follows every
+ * superset rebuild (1, the default, keeps it) */
 int dsph_set_reorder_period(dsph_ctx* ctx, int32_t period);
+/* Verlet skin of the fluid neighbour superset in units of the largest fluid
+ * smoothing length (default 0.3; 0 rebuilds the superset every search).
+ * Lists stay exact for any value; the skin only trades superset size for
+ * rebuild frequency. */
+int dsph_set_skin(dsph_ctx* ctx, double skin_over_h);
 
 /* replaces: solverBase::initialize lines 34-37 (solverBase.cpp:81-84):
  * neighborSearch(dev,0,1), addBondData (dataBase.cpp:270-346) and
