@@ -49,19 +49,20 @@ def build_scene(dx):
     return scenes.coupled_dam_break(dx=dx)
 
 
-def run_gpu(scene, precision, steps, warmup, timing):
+def run_gpu(scene, precision, steps, warmup, timing, eager=False):
     import dsph
     g = dsph.Solver(precision)
     g.upload(scene)
     g.initialize()
     i = 1
     if warmup:
-        g.step_n(i, warmup, scene.dt)
+        g.run(warmup, scene.dt, first=i, graphs=not eager)
         i += warmup
     g.synchronize()
     g.set_timing(timing)
     t0 = time.perf_counter()
-    g.step_n(i, steps, scene.dt)  # ends with a status read (device sync)
+    # with graphs the run ends with a status read (device sync)
+    g.run(steps, scene.dt, first=i, graphs=not eager)
     g.synchronize()
     t1 = time.perf_counter()
     times = g.kernel_times() if timing else {}
@@ -95,15 +96,42 @@ def roofline(times, prec_key, nf, ff_directed, fs_pairs):
     return best
 
 
-def traffic_from_profile(kernel):
-    """HBM bytes per launch from the committed rocprofv3 PMC summary."""
-    path = os.path.join(ROOT, "profiles", "pmc_traffic_coupled_dam_break.json")
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "traffic_pair_loops_coupled_dam_break.txt")
+
+
+def traffic_from_profile(kernel, prec_key):
+    """HBM bytes per launch of a kernel family from the committed rocprofv3
+    PMC summary (FETCH_SIZE + WRITE_SIZE, calibrated against a plain copy);
+    lines read "TRAFFIC <family> <precision> <bytes>"."""
     try:
-        with open(path) as f:
-            d = json.load(f)
-        return d.get(kernel)
-    except (OSError, ValueError):
-        return None
+        with open(TRAFFIC_SUMMARY) as f:
+            for line in f:
+                w = line.split()
+                if len(w) == 4 and w[0] == "TRAFFIC" and w[1] == kernel and w[2] == prec_key:
+                    return int(float(w[3]))
+    except OSError:
+        pass
+    return None
+
+
+def copy_probe(mib):
+    """A plain device copy of known size (read mib MiB, write mib MiB), the
+    calibration kernel for the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        torch.neg(x, out=y)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = 20
+    for _ in range(reps):
+        torch.neg(x, out=y)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    print(json.dumps({"copy_probe_mib": mib, "bytes_read": 4 * n, "bytes_written": 4 * n,
+                      "GBps": round(8 * n / dt / 1e9, 1)}), flush=True)
 
 
 def cpu_baseline(scene, budget_s, max_steps):
@@ -140,6 +168,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-timing", action="store_true")
+    ap.add_argument("--eager", action="store_true",
+                    help="launch every step's kernels directly instead of replaying HIP graphs")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -148,6 +180,10 @@ def main():
               "design only — run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
+
     import dsph
     scene = build_scene(args.dx)
     npart = scene.n_particles
@@ -155,19 +191,21 @@ def main():
     print(f"# workload: {scene.n_fluid} fluid + {scene.n_solid} solid, dt={scene.dt:.3e}",
           file=sys.stderr, flush=True)
 
-    g, secs, times = run_gpu(scene, dsph.PREC_MIXED, args.steps, args.warmup, not args.no_timing)
+    g, secs, times = run_gpu(scene, dsph.PREC_MIXED, args.steps, args.warmup, not args.no_timing,
+                             args.eager)
     value = npart * args.steps / secs
     print(f"# mixed: {secs / args.steps * 1e3:.3f} ms/step  {value:.4g} particle-steps/s",
           file=sys.stderr, flush=True)
     ff_directed, fs_pairs = pair_counts(g)
     roof = roofline(times, "mixed", scene.n_fluid, ff_directed, fs_pairs) if times else None
     if roof:
-        roof["traffic"] = traffic_from_profile(roof["kernel"])
+        roof["traffic"] = traffic_from_profile(roof["kernel"], "mixed")
     g.close()
 
     fp64 = None
     if not args.no_fp64 and args.fp64_steps > 0:
-        g64, s64, t64 = run_gpu(scene, dsph.PREC_FP64, args.fp64_steps, 3, not args.no_timing)
+        g64, s64, t64 = run_gpu(scene, dsph.PREC_FP64, args.fp64_steps, 3, not args.no_timing,
+                                args.eager)
         fp64 = {"value": round(npart * args.fp64_steps / s64, 1),
                 "ms_per_step": round(s64 / args.fp64_steps * 1e3, 4), "steps": args.fp64_steps}
         r64 = roofline(t64, "fp64", scene.n_fluid, ff_directed, fs_pairs) if t64 else None

@@ -66,9 +66,12 @@ struct dsph_ctx {
     int dcur = 0;  // current dRho / pair-record buffer (Jacobi ping-pong)
     int* fslot = nullptr;
     int* fmember = nullptr;
-    unsigned* fkey = nullptr;
-    unsigned* fkey_sorted = nullptr;
-    int* fperm = nullptr;
+    unsigned* fkey = nullptr;         // reference hash per caller index
+    unsigned* frank = nullptr;        // sort key: Morton rank of that cell
+    unsigned* fkey_sorted = nullptr;  // sorted ranks
+    int* fperm = nullptr;             // sorted position -> caller index
+    unsigned* crank = nullptr;        // cell -> Morton rank (spare cell last)
+    unsigned* cinv = nullptr;         // Morton rank -> cell
     int* fcs = nullptr;
     int* fce = nullptr;
     int* fblk_nb = nullptr;     // blocks per cell
@@ -318,15 +321,16 @@ void neighbor_search(dsph_ctx* c, int i_step, int gap, bool save_history) {
     const bool fluid_step = i_step % gap == 0 && c->nf > 0;
     {
         FamTimer t(c, F_GRID);
-        hash_fluid(s, c->g, c->fset[0].ph, c->fset[0].orig, c->nf, c->fkey, stw);
+        hash_fluid(s, c->g, c->fset[0].ph, c->fset[0].orig, c->nf, c->crank, c->fkey, c->frank,
+                   stw);
         hash_solid(s, c->g, c->so.pr, c->ns, c->skey, stw);
         // ranges of an empty species are filled with -1 as well
         if (c->nf > 0)
-            radix_sort_iota(s, c->fkey, c->nf, c->key_bits, c->fkey_sorted, c->fperm, c->radix_tmp);
-        cell_spans(s, c->fkey_sorted, c->nf, c->fcs, c->fce, c->g.ncell);
+            radix_sort_iota(s, c->frank, c->nf, c->key_bits, c->fkey_sorted, c->fperm, c->radix_tmp);
+        cell_spans(s, c->fkey_sorted, c->nf, c->fcs, c->fce, c->g.ncell, c->cinv);
         if (c->ns > 0)
             radix_sort_iota(s, c->skey, c->ns, c->key_bits, c->skey_sorted, c->sperm, c->radix_tmp);
-        cell_spans(s, c->skey_sorted, c->ns, c->scs, c->sce, c->g.ncell);
+        cell_spans(s, c->skey_sorted, c->ns, c->scs, c->sce, c->g.ncell, nullptr);
         if (fluid_step) {
             rebuild_decision(s, c->fset[0].ph, c->fxb, c->nf, c->so.pr, c->sxb, c->ns,
                              c->skin_h * c->hmax, c->ctl, stw);
@@ -335,13 +339,14 @@ void neighbor_search(dsph_ctx* c, int i_step, int gap, bool save_history) {
                               c->dcur, c->ctl);
         }
         if (c->nf > 0)
-            member_slots(s, c->fperm, c->fslot, c->fkey_sorted, c->fmember, c->hslot, c->nf);
+            member_slots(s, c->fperm, c->fslot, c->fkey_sorted, c->cinv, c->fmember, c->hslot, c->nf);
     }
     if (fluid_step) {
         const SearchArgs sa = search_args(c);
         {
             FamTimer t(c, F_FSEARCH);
-            fluid_superset(s, sa, c->fblk_nb, c->fblk_off, c->fblocks, c->scan_tmp, c->fxb, c->sxb);
+            fluid_superset(s, sa, c->cinv, c->fblk_nb, c->fblk_off, c->fblocks, c->scan_tmp, c->fxb,
+                           c->sxb);
         }
         FamTimer t(c, F_FSLIST);
         fluid_filter(s, sa, c->scan_tmp);
@@ -484,6 +489,38 @@ void require(bool cond, const std::string& msg) {
 }  // namespace
 
 namespace {
+// Slot order of the fluid: cells in Morton (z-order) order of their (x, y, z)
+// indices, members of a cell in ascending index.  A cell's members stay
+// contiguous, as the reference's hash sort keeps them, while spatially close
+// cells get close ranks, so the neighbours of a run of consecutive particles
+// are a compact block that stays in one XCD's L2.  The spare cell ranks last.
+uint64_t spread3(uint64_t v) {
+    v &= 0x1fffff;
+    v = (v | v << 32) & 0x1f00000000ffffULL;
+    v = (v | v << 16) & 0x1f0000ff0000ffULL;
+    v = (v | v << 8) & 0x100f00f00f00f00fULL;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+    v = (v | v << 2) & 0x1249249249249249ULL;
+    return v;
+}
+void morton_cell_ranks(const GridP& g, std::vector<unsigned>& rank, std::vector<unsigned>& inv) {
+    const int64_t nreal = (int64_t)g.ncell - 1;
+    std::vector<std::pair<uint64_t, unsigned>> code(nreal);
+    for (int64_t h = 0; h < nreal; h++) {
+        const int64_t x = h % g.gx, y = (h / g.gx) % g.gy, z = h / ((int64_t)g.gx * g.gy);
+        code[h] = {spread3(x) | spread3(y) << 1 | spread3(z) << 2, (unsigned)h};
+    }
+    std::sort(code.begin(), code.end());
+    rank.assign(g.ncell, 0);
+    inv.assign(g.ncell, 0);
+    for (int64_t r = 0; r < nreal; r++) {
+        rank[code[r].second] = (unsigned)r;
+        inv[r] = code[r].second;
+    }
+    rank[nreal] = (unsigned)nreal;
+    inv[nreal] = (unsigned)nreal;
+}
+
 // The reference emits each list segment in stencil order (z, y, x over the 27
 // cells around the particle's cell, neighborSearch.cu:188-214) with cell
 // members in ascending index.  The device keeps list rows in the order of
@@ -831,7 +868,16 @@ int dsph_upload_scene(dsph_ctx* c, const dsph_scene* s) {
         h2d(c, c->fslot, orig.data(), nf);
         c->fmember = dalloc<int>(c, nf);
         c->fkey = dalloc<unsigned>(c, nf);
+        c->frank = dalloc<unsigned>(c, nf);
         c->fkey_sorted = dalloc<unsigned>(c, nf);
+        {
+            std::vector<unsigned> rank, inv;
+            morton_cell_ranks(g, rank, inv);
+            c->crank = dalloc<unsigned>(c, g.ncell);
+            c->cinv = dalloc<unsigned>(c, g.ncell);
+            h2d(c, c->crank, rank.data(), g.ncell);
+            h2d(c, c->cinv, inv.data(), g.ncell);
+        }
         c->fperm = dalloc<int>(c, nf);
         c->fcs = dalloc<int>(c, g.ncell);
         c->fce = dalloc<int>(c, g.ncell);
@@ -1555,11 +1601,34 @@ int dsph_download_grid(dsph_ctx* c, int species, int32_t* hash, int32_t* perm, i
     if (species != 0 && species != 1) return DSPH_E_INVALID;
     try {
         const int n = species == 0 ? c->nf : c->ns;
-        if (hash) d2h(c, (unsigned*)hash, species == 0 ? c->fkey : c->skey, n);
-        if (perm) d2h(c, perm, species == 0 ? c->fperm : c->sperm, n);
-        if (start) d2h(c, start, species == 0 ? c->fcs : c->scs, c->g.ncell);
-        if (end) d2h(c, end, species == 0 ? c->fce : c->sce, c->g.ncell);
+        if (species == 1) {
+            if (hash) d2h(c, (unsigned*)hash, c->skey, n);
+            if (perm) d2h(c, perm, c->sperm, n);
+            if (start) d2h(c, start, c->scs, c->g.ncell);
+            if (end) d2h(c, end, c->sce, c->g.ncell);
+            HIPCHK(hipStreamSynchronize(c->st));
+            return DSPH_OK;
+        }
+        // the fluid is sorted by Morton cell rank on the device; the
+        // reference's arrays (stable sort by hash) are rebuilt from the hashes
+        std::vector<unsigned> key(n);
+        d2h(c, key.data(), c->fkey, n);
         HIPCHK(hipStreamSynchronize(c->st));
+        std::vector<int> cnt((size_t)c->g.ncell + 1, 0), p(n);
+        for (int i = 0; i < n; i++) cnt[key[i] + 1]++;
+        for (int h = 0; h < c->g.ncell; h++) cnt[h + 1] += cnt[h];
+        for (int i = 0; i < n; i++) p[cnt[key[i]]++] = i;
+        if (hash)
+            for (int i = 0; i < n; i++) hash[i] = (int32_t)key[i];
+        if (perm)
+            for (int i = 0; i < n; i++) perm[i] = p[i];
+        if (start) std::fill(start, start + c->g.ncell, -1);
+        if (end) std::fill(end, end + c->g.ncell, -1);
+        for (int t = 0; t < n; t++) {
+            const unsigned k = key[p[t]];
+            if (start && (t == 0 || key[p[t - 1]] != k)) start[k] = t;
+            if (end && (t == n - 1 || key[p[t + 1]] != k)) end[k] = t + 1;
+        }
         return DSPH_OK;
     } catch (const ApiError& e) {
         return fail(c, e);

@@ -179,8 +179,10 @@ void exclusive_scan(hipStream_t s, const int* in, int* out, int64_t n, int* tmp)
 int64_t radix_tmp_size(int64_t n);
 void radix_sort_iota(hipStream_t s, const unsigned* keys_in, int64_t n, int bits,
                      unsigned* keys_out, int* perm_out, unsigned* tmp);
+// start/end of each cell over sorted keys; with inv, the keys are cell
+// ranks and inv maps a rank to its cell
 void cell_spans(hipStream_t s, const unsigned* sorted, int64_t n, int* start, int* end,
-                int ncell);
+                int ncell, const unsigned* inv);
 
 // search (k_search.hip)
 struct SearchArgs {
@@ -203,7 +205,7 @@ struct SearchArgs {
     unsigned long long* st;
 };
 void hash_fluid(hipStream_t s, const GridP& g, const double4* ph, const int* orig, int n,
-                unsigned* key_orig, long long* status);
+                const unsigned* crank, unsigned* key_orig, unsigned* rank_orig, long long* status);
 void hash_solid(hipStream_t s, const GridP& g, const double4* pr, int n, unsigned* key,
                 long long* status);
 void rebuild_decision(hipStream_t s, const double4* fph, const double4* fxb, int nf,
@@ -211,10 +213,10 @@ void rebuild_decision(hipStream_t s, const double4* fph, const double4* fxb, int
                       unsigned long long* ctl, long long* status);
 void reorder_fluid(hipStream_t s, const FluidSet& f, const FluidSet& scratch, const int* perm,
                    int* slot_of, int n, bool mixed, int cur, const unsigned long long* ctl);
-void member_slots(hipStream_t s, const int* perm, const int* slot_of, const unsigned* sorted_key,
-                  int* member, unsigned* hslot, int n);
-void fluid_superset(hipStream_t s, const SearchArgs& a, int* nb, int* off, int2* blocks,
-                    int* scan_tmp, double4* fxb, double4* sxb);
+void member_slots(hipStream_t s, const int* perm, const int* slot_of, const unsigned* sorted_rank,
+                  const unsigned* cinv, int* member, unsigned* hslot, int n);
+void fluid_superset(hipStream_t s, const SearchArgs& a, const unsigned* cinv, int* nb, int* off,
+                    int2* blocks, int* scan_tmp, double4* fxb, double4* sxb);
 void fluid_filter(hipStream_t s, const SearchArgs& a, int* scan_tmp);
 void reverse_movable(hipStream_t s, const int* pair_key, const int* pair_off_total, int64_t cap,
                      const double4* key_vel, int nkeys, int* cnt, int* off, int* idx,

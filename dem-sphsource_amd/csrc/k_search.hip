@@ -122,13 +122,16 @@ __global__ void k_status_reset(unsigned long long* st) {
 }
 
 __global__ void k_hash_fluid(GridP g, const double4* __restrict__ ph, const int* __restrict__ orig,
-                             int n, unsigned* __restrict__ key, unsigned long long* st) {
+                             int n, const unsigned* __restrict__ crank, unsigned* __restrict__ key,
+                             unsigned* __restrict__ rank, unsigned long long* st) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long out = 0;
     if (s < n) {
         double4 p = ph[s];
         int h = point_hash(g, p.x, p.y, p.z);
-        key[orig[s]] = (unsigned)h;
+        const int o = orig[s];
+        key[o] = (unsigned)h;
+        rank[o] = crank[h];
         out = h == g.ncell - 1;
     }
     unsigned long long m = __ballot(out != 0);
@@ -229,13 +232,13 @@ __global__ void k_copyback(FluidSet dst, FluidSet src, int n, int cur,
 
 // slot of every sorted position, and the reference hash of every slot
 __global__ void k_member(const int* __restrict__ perm, const int* __restrict__ slot_of,
-                         const unsigned* __restrict__ sorted_key, int* __restrict__ member,
-                         unsigned* __restrict__ hslot, int n) {
+                         const unsigned* __restrict__ sorted_rank, const unsigned* __restrict__ cinv,
+                         int* __restrict__ member, unsigned* __restrict__ hslot, int n) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int s = slot_of[perm[t]];
     member[t] = s;
-    hslot[s] = sorted_key[t];
+    hslot[s] = cinv[sorted_rank[t]];
 }
 
 __global__ void k_store_xb(const double4* __restrict__ p, double4* __restrict__ xb, int n,
@@ -247,25 +250,28 @@ __global__ void k_store_xb(const double4* __restrict__ p, double4* __restrict__ 
 
 // ------------------------------------------------------- cell blocks
 // every cell holding fluid is cut into blocks of <= 64 consecutive sorted
-// particles, one wavefront each
-__global__ void k_blk_count(const int* __restrict__ cs, const int* __restrict__ ce, int ncell,
-                            int* __restrict__ nb, const unsigned long long* __restrict__ ctl) {
+// particles, one wavefront each; blocks are listed in cell rank order
+__global__ void k_blk_count(const int* __restrict__ cs, const int* __restrict__ ce,
+                            const unsigned* __restrict__ cinv, int ncell, int* __restrict__ nb,
+                            const unsigned long long* __restrict__ ctl) {
     if (ctl[CTL_REBUILD] == 0) return;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncell) return;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= ncell) return;
+    const int c = (int)cinv[r];
     const int st = cs[c];
-    nb[c] = st >= 0 ? (ce[c] - st + 63) >> 6 : 0;
+    nb[r] = st >= 0 ? (ce[c] - st + 63) >> 6 : 0;
 }
-__global__ void k_blk_fill(const int* __restrict__ cs, const int* __restrict__ ce, int ncell,
-                           const int* __restrict__ off, int2* __restrict__ blocks,
-                           const unsigned long long* __restrict__ ctl) {
+__global__ void k_blk_fill(const int* __restrict__ cs, const int* __restrict__ ce,
+                           const unsigned* __restrict__ cinv, int ncell, const int* __restrict__ off,
+                           int2* __restrict__ blocks, const unsigned long long* __restrict__ ctl) {
     if (ctl[CTL_REBUILD] == 0) return;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncell) return;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= ncell) return;
+    const int c = (int)cinv[r];
     const int st = cs[c];
     if (st < 0) return;
     const int n = ce[c] - st;
-    const int o = off[c];
+    const int o = off[r];
     for (int b = 0; b * 64 < n; b++) blocks[o + b] = make_int2(c, st + 64 * b);
 }
 
@@ -724,11 +730,11 @@ int cap_grid(int64_t n, int cap) {
 
 // ---------------------------------------------------------------- wrappers
 void hash_fluid(hipStream_t s, const GridP& g, const double4* ph, const int* orig, int n,
-                unsigned* key, long long* status) {
+                const unsigned* crank, unsigned* key, unsigned* rank, long long* status) {
     hipLaunchKernelGGL(k_status_reset, dim3(1), dim3(ST_WORDS), 0, s, (unsigned long long*)status);
     if (n > 0)
-        hipLaunchKernelGGL(k_hash_fluid, dim3(blocks_for(n)), dim3(kBS), 0, s, g, ph, orig, n, key,
-                           (unsigned long long*)status);
+        hipLaunchKernelGGL(k_hash_fluid, dim3(blocks_for(n)), dim3(kBS), 0, s, g, ph, orig, n, crank,
+                           key, rank, (unsigned long long*)status);
 }
 
 void hash_solid(hipStream_t s, const GridP& g, const double4* pr, int n, unsigned* key,
@@ -762,11 +768,11 @@ void reorder_fluid(hipStream_t s, const FluidSet& f, const FluidSet& scratch, co
     }
 }
 
-void member_slots(hipStream_t s, const int* perm, const int* slot_of, const unsigned* sorted_key,
-                  int* member, unsigned* hslot, int n) {
+void member_slots(hipStream_t s, const int* perm, const int* slot_of, const unsigned* sorted_rank,
+                  const unsigned* cinv, int* member, unsigned* hslot, int n) {
     if (n > 0)
-        hipLaunchKernelGGL(k_member, dim3(blocks_for(n)), dim3(kBS), 0, s, perm, slot_of, sorted_key,
-                           member, hslot, n);
+        hipLaunchKernelGGL(k_member, dim3(blocks_for(n)), dim3(kBS), 0, s, perm, slot_of, sorted_rank,
+                           cinv, member, hslot, n);
 }
 
 static int wave_grid(int nf) {
@@ -776,14 +782,14 @@ static int wave_grid(int nf) {
     return b < 1 ? 1 : (b > 16384 ? 16384 : b);
 }
 
-void fluid_superset(hipStream_t s, const SearchArgs& a, int* nb, int* off, int2* blocks,
-                    int* scan_tmp, double4* fxb, double4* sxb) {
+void fluid_superset(hipStream_t s, const SearchArgs& a, const unsigned* cinv, int* nb, int* off,
+                    int2* blocks, int* scan_tmp, double4* fxb, double4* sxb) {
     if (a.nf <= 0) return;
     const int nc = a.g.ncell;
-    hipLaunchKernelGGL(k_blk_count, dim3(blocks_for(nc)), dim3(kBS), 0, s, a.fcs, a.fce, nc, nb,
-                       (const unsigned long long*)a.ctl);
+    hipLaunchKernelGGL(k_blk_count, dim3(blocks_for(nc)), dim3(kBS), 0, s, a.fcs, a.fce, cinv, nc,
+                       nb, (const unsigned long long*)a.ctl);
     exclusive_scan(s, nb, off, nc, scan_tmp);
-    hipLaunchKernelGGL(k_blk_fill, dim3(blocks_for(nc)), dim3(kBS), 0, s, a.fcs, a.fce, nc,
+    hipLaunchKernelGGL(k_blk_fill, dim3(blocks_for(nc)), dim3(kBS), 0, s, a.fcs, a.fce, cinv, nc,
                        (const int*)off, blocks, (const unsigned long long*)a.ctl);
     hipLaunchKernelGGL(k_build, dim3(wave_grid(a.nf)), dim3(kBS), 0, s, a, (const int2*)blocks,
                        (const int*)(off + nc));
@@ -840,7 +846,7 @@ void active_grid(hipStream_t s, const Solids& so, const unsigned* skey, int ncel
                            skey, akey);
         radix_sort_iota(s, akey, n, key_bits, akey_sorted, aperm, radix_tmp);
     }
-    cell_spans(s, akey_sorted, n, acs, ace, ncell);
+    cell_spans(s, akey_sorted, n, acs, ace, ncell, nullptr);
 }
 
 void solid_search(hipStream_t s, const GridP& g, const Solids& so, int ns, const int* acs,
