@@ -43,7 +43,7 @@ extern "C" {
 #define DSPH_PREC_MIXED  1  /* FP32 pair loops with compensated sums,
                                FP64 integration, springs and quaternions      */
 
-/* ---- behaviour flags (reference quirks, SURVEY Q-items) -------------- */
+/* ---- behaviour flags (reference quirks) ------------------------------ */
 /* default 0 = the reference's own semantics                                 */
 #define DSPH_FLAG_WALL_CONTINUE   (1u << 0) /* wall loop skips a non-touching
                                                wall instead of returning

@@ -631,7 +631,8 @@ int orc_upload_scene(orc_ctx* o, const dsph_scene* s) {
     if (s->hertz) std::copy(s->hertz, s->hertz + 8 * o->ncomb, o->hertz.begin());
     if (s->linear) std::copy(s->linear, s->linear + 11 * o->ncomb, o->linear.begin());
     if (s->bonded) std::copy(s->bonded, s->bonded + 6 * o->ncomb, o->bonded.begin());
-    /* invalid material pairs would index out of the tables */
+    /* invalid material pairs would index out of the tables (the reference reads
+       out of bounds there, deviceStructs.h:960-963): rejected */
     for (int i = 0; i < o->ns; i++)
         if (!o->sdummy[i] && (o->smat[i] < 0 || o->smat[i] >= o->nmat)) return DSPH_E_INVALID;
     for (int i = 0; i < o->nw; i++)

@@ -297,7 +297,7 @@ def test_capacity_overflow_grows_and_matches():
 def test_invalid_material_rejected():
     sc = scenes.granular_column(n_sphere=10, n_clump=0, n_bonded=0)
     sc.solid_material = np.asarray(sc.solid_material).copy()
-    sc.solid_material[3] = sc.n_material  # out of the table
+    sc.solid_material[3] = sc.n_material  # out of the contact tables
     g = dsph.Solver(dsph.PREC_FP64)
     with pytest.raises(dsph.DsphError) as e:
         g.upload(sc)
