@@ -49,7 +49,11 @@ def build_scene(dx):
     return scenes.coupled_dam_break(dx=dx)
 
 
-def run_gpu(scene, precision, steps, warmup, timing, eager=False):
+def run_gpu(scene, precision, steps, warmup, timing_steps, eager=False):
+    """Times `steps` steps after `warmup` (HIP-graph replay unless eager), then
+    runs `timing_steps` more steps eagerly with per-family HIP events on the
+    context stream (events cannot live inside a captured graph; kernel
+    durations do not depend on how the kernels were launched)."""
     import dsph
     g = dsph.Solver(precision)
     g.upload(scene)
@@ -59,14 +63,19 @@ def run_gpu(scene, precision, steps, warmup, timing, eager=False):
         g.run(warmup, scene.dt, first=i, graphs=not eager)
         i += warmup
     g.synchronize()
-    g.set_timing(timing)
     t0 = time.perf_counter()
     # with graphs the run ends with a status read (device sync)
     g.run(steps, scene.dt, first=i, graphs=not eager)
     g.synchronize()
     t1 = time.perf_counter()
-    times = g.kernel_times() if timing else {}
-    g.set_timing(False)
+    i += steps
+    times = {}
+    if timing_steps:
+        g.set_timing(True)
+        g.run(timing_steps, scene.dt, first=i, graphs=False)
+        g.synchronize()
+        times = g.kernel_times()
+        g.set_timing(False)
     return g, t1 - t0, times
 
 
@@ -159,15 +168,17 @@ def cpu_baseline(scene, budget_s, max_steps):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--dx", type=float, default=0.005)
-    ap.add_argument("--fp64-steps", type=int, default=30)
+    ap.add_argument("--fp64-steps", type=int, default=1000)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--cpu-max-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-timing", action="store_true")
+    ap.add_argument("--timing-steps", type=int, default=100,
+                    help="eager steps after the timed region that carry the per-kernel HIP events")
     ap.add_argument("--eager", action="store_true",
                     help="launch every step's kernels directly instead of replaying HIP graphs")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -191,8 +202,8 @@ def main():
     print(f"# workload: {scene.n_fluid} fluid + {scene.n_solid} solid, dt={scene.dt:.3e}",
           file=sys.stderr, flush=True)
 
-    g, secs, times = run_gpu(scene, dsph.PREC_MIXED, args.steps, args.warmup, not args.no_timing,
-                             args.eager)
+    tsteps = 0 if args.no_timing else args.timing_steps
+    g, secs, times = run_gpu(scene, dsph.PREC_MIXED, args.steps, args.warmup, tsteps, args.eager)
     value = npart * args.steps / secs
     print(f"# mixed: {secs / args.steps * 1e3:.3f} ms/step  {value:.4g} particle-steps/s",
           file=sys.stderr, flush=True)
@@ -204,7 +215,7 @@ def main():
 
     fp64 = None
     if not args.no_fp64 and args.fp64_steps > 0:
-        g64, s64, t64 = run_gpu(scene, dsph.PREC_FP64, args.fp64_steps, 3, not args.no_timing,
+        g64, s64, t64 = run_gpu(scene, dsph.PREC_FP64, args.fp64_steps, args.warmup, tsteps,
                                 args.eager)
         fp64 = {"value": round(npart * args.fp64_steps / s64, 1),
                 "ms_per_step": round(s64 / args.fp64_steps * 1e3, 4), "steps": args.fp64_steps}

@@ -79,6 +79,7 @@ struct dsph_ctx {
     int2* fblocks = nullptr;    // (cell, first sorted position)
     unsigned* hslot = nullptr;  // reference hash per fluid slot
     double4* fxb = nullptr;     // fluid positions at the last superset build
+    double4* fsp = nullptr;     // fluid positions at the last fluid search
     double4* sxb = nullptr;     // solid positions at the last superset build
     unsigned long long* ctl = nullptr;
     double skin_h = 0.3;        // superset skin / max fluid h
@@ -288,6 +289,21 @@ DemArgs dem_args(dsph_ctx* c) {
     return a;
 }
 
+// Half-width of the superset build stencil: every particle within the
+// superset radius 2 h_max + skin of a particle lies within sw cells of that
+// particle's cell on each axis.  The (2 sw + 1)^3 linear offsets are distinct
+// while gx, gy >= 2 sw + 1; narrower grids (or very large skins) walk the
+// reference stencil for every row instead.
+void superset_stencil(dsph_ctx* c) {
+    GridP& g = c->g;
+    const double R = (2.0 + c->skin_h) * c->hmax * (1.0 + 1e-6);
+    const double cmin = std::min({g.csx, g.csy, g.csz});
+    int w = cmin > 0 ? (int)std::ceil(R / cmin) : 1;
+    w = std::max(w, 1);
+    g.sw = std::min(w, 8);
+    g.walk_only = w > 8 || g.gx < 2 * g.sw + 1 || g.gy < 2 * g.sw + 1;
+}
+
 SearchArgs search_args(dsph_ctx* c) {
     SearchArgs a;
     a.g = c->g;
@@ -304,6 +320,7 @@ SearchArgs search_args(dsph_ctx* c) {
     a.sperm = c->sperm;
     a.skey = c->skey;
     a.spr = c->so.pr;
+    a.fsp = c->fsp;
     a.hmax = c->hmax;
     a.skin = c->skin_h * c->hmax;
     a.L = c->L;
@@ -442,7 +459,7 @@ void fluid_integrate(dsph_ctx* c, double dt, int i_step, int gap) {
         c->dcur ^= 1;
         {
             FamTimer t(c, F_FSTREAM);
-            fluid_drift(c->st, fluid_args(c), 0.5 * dtf);
+            fluid_drift(c->st, fluid_args(c), 0.5 * dtf, mx);
         }
         {
             FamTimer t(c, F_MOM);
@@ -597,7 +614,7 @@ FluidHost fluid_host(dsph_ctx* c) {
     f.ph.resize(nf);
     f.fkey.resize(nf);
     d2h(c, f.orig.data(), c->fset[0].orig, nf);
-    d2h(c, f.ph.data(), c->fset[0].ph, nf);
+    d2h(c, f.ph.data(), c->fsp, nf);  // stencils of the last search
     d2h(c, f.fkey.data(), c->fkey, nf);
     HIPCHK(hipStreamSynchronize(c->st));
     f.slot_of.resize(nf);
@@ -703,6 +720,7 @@ int dsph_set_reorder_period(dsph_ctx* c, int32_t p) {
 int dsph_set_skin(dsph_ctx* c, double skin_over_h) {
     if (!c || !(skin_over_h >= 0.0)) return DSPH_E_INVALID;
     c->skin_h = skin_over_h;
+    if (c->uploaded) superset_stencil(c);
     invalidate_graphs(c);
     if (c->uploaded) {
         try {
@@ -811,6 +829,7 @@ int dsph_upload_scene(dsph_ctx* c, const dsph_scene* s) {
         require(ncell < (1LL << 31) - 1, "grid has too many cells");
         g.ncell = (int)ncell;
         g.collide = g.gx < 3 || g.gy < 3;
+        superset_stencil(c);
         double ext = std::max({std::fabs(g.minx), std::fabs(g.maxx), std::fabs(g.miny), std::fabs(g.maxy),
                                std::fabs(g.minz), std::fabs(g.maxz)});
         g.pad = 1e-9 * (1.0 + ext);
@@ -851,6 +870,17 @@ int dsph_upload_scene(dsph_ctx* c, const dsph_scene* s) {
             for (int b = 0; b < 2; b++) {
                 f.pv[b] = dalloc<char>(c, rec * std::max(nf, 1));
                 f.pq[b] = dalloc<char>(c, rec * std::max(nf, 1));
+                f.p0[b] = mx ? dalloc<char>(c, sizeof(uint4) * std::max(nf, 1)) : nullptr;
+            }
+            // fixed-point span of the mixed records: the domain plus one
+            // domain length on each side of every axis
+            const double lo[3] = {g.minx, g.miny, g.minz}, hi[3] = {g.maxx, g.maxy, g.maxz};
+            for (int a = 0; a < 3; a++) {
+                const double L = hi[a] - lo[a];
+                const double unit = 3.0 * L / 4294967296.0;
+                f.qo[a] = lo[a] - L;
+                f.qi[a] = 1.0 / unit;
+                f.qr[a] = (float)unit;
             }
         }
         {
@@ -888,6 +918,8 @@ int dsph_upload_scene(dsph_ctx* c, const dsph_scene* s) {
         c->hslot = dalloc<unsigned>(c, nf);
         c->fxb = dalloc<double4>(c, nf);
         h2d(c, c->fxb, ph.data(), nf);
+        c->fsp = dalloc<double4>(c, nf);
+        h2d(c, c->fsp, ph.data(), nf);
         c->ctl = dalloc<unsigned long long>(c, CTL_WORDS);
         HIPCHK(hipMemsetAsync(c->ctl, 0, CTL_WORDS * sizeof(unsigned long long), c->st));
         c->scs = dalloc<int>(c, g.ncell);
@@ -1193,7 +1225,7 @@ int dsph_neighbor_search(dsph_ctx* c, int32_t i_step, int32_t gap) {
     API_GUARD(c);
     if (gap < 1) return DSPH_E_INVALID;
     try {
-        neighbor_search(c, i_step, gap, true);
+        search_checked(c, i_step, gap);  // grows overflowing lists (one status read)
         return DSPH_OK;
     } catch (const ApiError& e) {
         return fail(c, e);
@@ -1313,6 +1345,44 @@ int dsph_step_n(dsph_ctx* c, int32_t i0, int32_t n, double dt, int32_t gap) {
 int dsph_synchronize(dsph_ctx* c) {
     if (!c) return DSPH_E_INVALID;
     return hipStreamSynchronize(c->st) == hipSuccess ? DSPH_OK : DSPH_E_HIP;
+}
+
+int dsph_set_contact_models(dsph_ctx* c, const double* hertz, const double* linear,
+                            const double* bonded) {
+    API_GUARD(c);
+    if (!c->uploaded) return DSPH_E_STATE;
+    try {
+        const int ncb = c->tb.ncomb;
+        if (hertz) h2d(c, const_cast<double*>(c->tb.hertz), hertz, 8 * (size_t)ncb);
+        if (linear) h2d(c, const_cast<double*>(c->tb.linear), linear, 11 * (size_t)ncb);
+        if (bonded) {
+            h2d(c, const_cast<double*>(c->tb.bonded), bonded, 6 * (size_t)ncb);
+            c->h_bonded.assign(bonded, bonded + 6 * (size_t)ncb);
+        }
+        HIPCHK(hipStreamSynchronize(c->st));
+        return DSPH_OK;
+    } catch (const ApiError& e) {
+        return fail(c, e);
+    }
+}
+
+int dsph_set_clump_velocity(dsph_ctx* c, int32_t idx, const double v[3]) {
+    API_GUARD(c);
+    if (!c->uploaded) return DSPH_E_STATE;
+    if (idx < 0 || idx >= c->nc || !v) return DSPH_E_INVALID;
+    try {
+        double4 cur;
+        d2h(c, &cur, c->cl.vel + idx, 1);
+        HIPCHK(hipStreamSynchronize(c->st));
+        cur.x = v[0];
+        cur.y = v[1];
+        cur.z = v[2];  // .w keeps the inverse mass
+        h2d(c, c->cl.vel + idx, &cur, 1);
+        HIPCHK(hipStreamSynchronize(c->st));
+        return DSPH_OK;
+    } catch (const ApiError& e) {
+        return fail(c, e);
+    }
 }
 
 int dsph_add_external_force(dsph_ctx* c, int32_t idx, const double f[3]) {
@@ -1523,17 +1593,27 @@ int dsph_download_list(dsph_ctx* c, int kind, int32_t* pointed, int32_t* pointin
     try {
         std::vector<int> A, Bv, cnt;
         if (kind == DSPH_LIST_FLUID_FLUID) {
-            const int nf = c->nf, K = c->L.K;
+            const int nf = c->nf, KS = std::max(c->L.K, c->L.supK);
             const FluidHost fh = fluid_host(c);
-            std::vector<int> nbr((size_t)K * nf), cn(nf);
-            d2h(c, nbr.data(), c->L.nbr, (size_t)K * nf);
-            d2h(c, cn.data(), c->L.cnt, nf);
-            HIPCHK(hipStreamSynchronize(c->st));
+            std::vector<int> nbr((size_t)KS * nf), cn(nf);
+            if (nf > 0) {
+                // exact rows from the supersets (and the walked rows) into a
+                // scratch ELL of the superset's capacity
+                int *rows = nullptr, *rcnt = nullptr;
+                HIPCHK(hipMalloc(&rows, sizeof(int) * (size_t)KS * nf));
+                HIPCHK(hipMalloc(&rcnt, sizeof(int) * (size_t)nf));
+                fluid_exact_rows(c->st, search_args(c), rows, rcnt);
+                d2h(c, nbr.data(), rows, (size_t)KS * nf);
+                d2h(c, cn.data(), rcnt, nf);
+                HIPCHK(hipStreamSynchronize(c->st));
+                HIPCHK(hipFree(rows));
+                HIPCHK(hipFree(rcnt));
+            }
             cnt.assign(nf, 0);
             std::vector<RowEnt> row;
             for (int i = 0; i < nf; i++) {
                 const int s2 = fh.slot_of[i];
-                if (cn[s2] > K) throw ApiError{DSPH_E_CAPACITY, "fluid-fluid list overflowed"};
+                if (cn[s2] < 0) throw ApiError{DSPH_E_CAPACITY, "fluid-fluid list overflowed"};
                 int H[27];
                 stencil_hashes(c->g, fh.ph[s2].x, fh.ph[s2].y, fh.ph[s2].z, H);
                 row.clear();

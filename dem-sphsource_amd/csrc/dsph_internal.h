@@ -6,13 +6,13 @@
 //   fluid  — kept in cell-sorted ("slot") order, two copies ping-ponged on
 //            reorder: ph (x,y,z,h) f64x4, vr (vx,vy,vz,rho0) f64x4,
 //            drho f64 (Jacobi ping-pong), cn (c,nu) f64x2, sumw0 f64, acc f64x4,
-//            orig (slot -> caller index) i32; FP32 shadows for the mixed
-//            pair-loop records pv (v, rho) and pq (P, c, V, nu rho) in the
-//            pair precision.
+//            orig (slot -> caller index) i32; neighbour records in the pair
+//            precision (see FluidSet).
 //   solids — caller order: pr (x,y,z,r), vel (vx,vy,vz,1/m), acc, w, torque,
 //            normal (f64x4), eff radius f64, material/cluster/clump/dummy i32.
-//   lists  — fluid–fluid: full (both directions) neighbour list in ELL
-//            column-major layout nbr[k*NF + slot]; fluid–solid and
+//   lists  — fluid–fluid: Verlet superset rows (both directions) in ELL
+//            column-major layout sup[k*NF + slot], plus exact rows nbr for
+//            particles walked on the reference stencil; fluid–solid and
 //            solid–solid: CSR by pointed particle with device-side totals.
 #pragma once
 
@@ -57,8 +57,10 @@ struct GridP {
     double maxx, maxy, maxz;
     double csx, csy, csz;
     int gx, gy, gz, ncell;
-    int collide;  // gx < 3 or gy < 3: some of the 27 linear stencil offsets coincide
-    double pad;   // conservative enlargement of cell boxes for pruning
+    int collide;    // gx < 3 or gy < 3: some of the 27 linear stencil offsets coincide
+    int sw;         // half-width in cells of the superset build stencil
+    int walk_only;  // the superset stencil's offsets are not distinct: walk every row
+    double pad;     // conservative enlargement of cell boxes for pruning
 };
 
 struct FluidSet {
@@ -69,11 +71,19 @@ struct FluidSet {
     double* sumw0 = nullptr;
     double4* acc = nullptr;
     int* orig = nullptr;                // slot -> caller index
-    // pair-loop records in the pair-loop precision (float4 mixed / double4
-    // FP64), ping-ponged with drho so the Jacobi update never overwrites what
-    // neighbours read: pv = (vx, vy, vz, rho), pq = (P, c, V = m/rho, nu rho)
+    // neighbour records, ping-ponged with drho so the Jacobi update never
+    // overwrites what neighbours read.  FP64: pv = (v, rho), pq = (P, c,
+    // V = m/rho, gamma = nu rho) as f64x4, the position from ph.  Mixed:
+    // p0 = (x, y, z as 32-bit fixed point, h), pv = (v, P), pq = (c, V, rho,
+    // gamma), 16 bytes each.
+    void* p0[2] = {nullptr, nullptr};
     void* pv[2] = {nullptr, nullptr};
     void* pq[2] = {nullptr, nullptr};
+    // fixed-point map of the mixed records: q = rint((x - qo) * qi), one unit
+    // = qr metres per axis
+    double qo[3] = {0, 0, 0};
+    double qi[3] = {0, 0, 0};
+    float qr[3] = {0, 0, 0};
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs of an MI355X; this
@@ -199,6 +209,7 @@ struct SearchArgs {
     const int* sperm;
     const unsigned* skey;   // reference hash of each solid
     const double4* spr;
+    double4* fsp;           // fluid positions of the last fluid search, slot order
     double hmax, skin;
     Lists L;
     unsigned long long* ctl;
@@ -217,6 +228,9 @@ void member_slots(hipStream_t s, const int* perm, const int* slot_of, const unsi
                   const unsigned* cinv, int* member, unsigned* hslot, int n);
 void fluid_superset(hipStream_t s, const SearchArgs& a, const unsigned* cinv, int* nb, int* off,
                     int2* blocks, int* scan_tmp, double4* fxb, double4* sxb);
+// exact fluid neighbour rows of all particles for read-back (ELL, stride nf,
+// capacity max(K, supK)); out_cnt = -1 marks a walked row that overflowed
+void fluid_exact_rows(hipStream_t s, const SearchArgs& a, int* out, int* out_cnt);
 void fluid_filter(hipStream_t s, const SearchArgs& a, int* scan_tmp);
 void reverse_movable(hipStream_t s, const int* pair_key, const int* pair_off_total, int64_t cap,
                      const double4* key_vel, int nkeys, int* cnt, int* off, int* idx,
@@ -244,7 +258,7 @@ struct FluidArgs {
 void fluid_records(hipStream_t s, const FluidArgs& a, int buf, bool mixed);
 void density_reinit(hipStream_t s, const FluidArgs& a, int i_step, bool mixed);
 void continuity(hipStream_t s, const FluidArgs& a, double dt, bool mixed);  // writes 1-cur
-void fluid_drift(hipStream_t s, const FluidArgs& a, double dt);
+void fluid_drift(hipStream_t s, const FluidArgs& a, double dt, bool mixed);
 void momentum(hipStream_t s, const FluidArgs& a, bool mixed, bool write_fs_force);
 void fluid_solid_force(hipStream_t s, const FluidArgs& a, bool mixed);
 void fluid_kick_drift(hipStream_t s, const FluidArgs& a, double dt, bool mixed);

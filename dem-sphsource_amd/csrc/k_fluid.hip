@@ -5,18 +5,31 @@
 // solveMomentumConservationEquation (:163-280), calFluid2SolidForce
 // (:282-330), fluidVelocityIntegrate / fluidPositionIntegrate (:367-383).
 //
-// One thread per fluid particle in cell-sorted order walks its full
-// neighbour list (ELL, both directions) and its fluid–solid segment; there
-// is no per-pair scratch and no atomics.  Two precisions from one template:
+// One thread per fluid particle in slot order walks its neighbour row and its
+// fluid–solid segment; there is no per-pair scratch and no atomics.  A row is
+// the particle's Verlet superset (all particles within 2h + skin at the last
+// rebuild, both directions): candidates outside the kernel support give
+// grad W = 0 and are skipped, so the contributing pairs and their order are
+// exactly those of the filtered list.  Rows the search walked exactly this
+// step (spare-cell cases) are read from the exact list instead (cnt >= 0).
+// Two precisions from one template:
 //   MIX = false  FP64 throughout (the reference's arithmetic);
-//   MIX = true   pair geometry r_i - r_j subtracted in FP64 then rounded,
-//                everything else per pair in FP32 with Kahan-compensated
-//                neighbour sums; the FP64 masters (position, velocity, dRho)
-//                are updated in FP64 and the FP32 records refreshed.
+//   MIX = true   pair loops in FP32 with Kahan-compensated neighbour sums;
+//                the FP64 masters (position, velocity, dRho) are updated in
+//                FP64 and the FP32 records refreshed.
 //
-// A neighbour costs three gathers: position+h (f64x4) and two records in the
-// pair precision, pv = (v, rho) and pq = (P, c, V = m/rho, gamma = nu rho),
-// computed once per particle when its dRho or velocity changes.
+// Neighbour records (ping-ponged with dRho so the Jacobi continuity never
+// overwrites what neighbours read):
+//   FP64: ph (x, y, z, h) + pv (v, rho) + pq (P, c, V = m/rho, gamma = nu rho),
+//         three f64x4;
+//   MIX:  p0 (x, y, z as 32-bit fixed point, h) + pv (v, P) + pq (c, V, rho,
+//         gamma), three 16-byte records.
+// A candidate's position record (with h) decides the kernel support; the
+// other two records are read only for pairs inside it.  The fixed-point map spans the domain
+//         plus one domain length on each side; r_i - r_j is the exact integer
+//         difference (|d| < 2^24 units converts to FP32 exactly) times the
+//         unit, so pair geometry never suffers the cancellation of absolute
+//         FP32 coordinates.
 //
 // The Riemann algebra is rearranged (same quantities, fewer operations):
 //   grad W_ij = F(r,h) r_ij with F = -105/(16 pi) t^3 / h^5, t = 1 - r/(2h)
@@ -25,8 +38,6 @@
 //   the viscous term's (v_i - v_j)/|r| (e . grad W) = F (v_i - v_j);
 //   2 V_i / m_i = 2 / rho_i.
 // Pairs closer than 1e-10 contribute nothing (grad W = 0 there upstream).
-// The continuity update is Jacobi: neighbours are read from buffer `cur` and
-// the result written to `1 - cur`.
 #include <type_traits>
 
 #include "dsph_internal.h"
@@ -84,44 +95,106 @@ struct Acc3<float> {
     }
 };
 
-// one particle as the pair loops see it
+// one particle as the pair loops see it; the position is FP64 (FP64 path)
+// or fixed point (mixed path)
 template <bool MIX>
 struct Nb {
     using R = typename Prec<MIX>::R;
-    double px, py, pz;
+    using PT = typename std::conditional<MIX, unsigned, double>::type;
+    PT px, py, pz;
     R h, vx, vy, vz, rho, P, c, V, gamma;
 };
 
+// the position record (with h): all the support test needs
+template <bool MIX>
+__device__ __forceinline__ void load_pos(const FluidSet& f, int cur, int j, Nb<MIX>& o) {
+    if constexpr (MIX) {
+        const uint4 a = ((const uint4*)f.p0[cur])[j];
+        o.px = a.x; o.py = a.y; o.pz = a.z; o.h = __uint_as_float(a.w);
+    } else {
+        const double4 p = f.ph[j];
+        o.px = p.x; o.py = p.y; o.pz = p.z; o.h = p.w;
+    }
+}
+// the state records
+template <bool MIX>
+__device__ __forceinline__ void load_state(const FluidSet& f, int cur, int j, Nb<MIX>& o) {
+    if constexpr (MIX) {
+        const float4 b = ((const float4*)f.pv[cur])[j];
+        const float4 c = ((const float4*)f.pq[cur])[j];
+        o.vx = b.x; o.vy = b.y; o.vz = b.z; o.P = b.w;
+        o.c = c.x; o.V = c.y; o.rho = c.z; o.gamma = c.w;
+    } else {
+        const double4 a = ((const double4*)f.pv[cur])[j];
+        const double4 b = ((const double4*)f.pq[cur])[j];
+        o.vx = a.x; o.vy = a.y; o.vz = a.z; o.rho = a.w;
+        o.P = b.x; o.c = b.y; o.V = b.z; o.gamma = b.w;
+    }
+}
 template <bool MIX>
 __device__ __forceinline__ Nb<MIX> load_nb(const FluidSet& f, int cur, int j) {
-    using R4 = typename Prec<MIX>::R4;
-    using R = typename Prec<MIX>::R;
     Nb<MIX> o;
-    const double4 p = f.ph[j];
-    const R4 a = ((const R4*)f.pv[cur])[j];
-    const R4 b = ((const R4*)f.pq[cur])[j];
-    o.px = p.x; o.py = p.y; o.pz = p.z;
-    o.h = (R)p.w;
-    o.vx = a.x; o.vy = a.y; o.vz = a.z; o.rho = a.w;
-    o.P = b.x; o.c = b.y; o.V = b.z; o.gamma = b.w;
+    load_pos<MIX>(f, cur, j, o);
+    load_state<MIX>(f, cur, j, o);
     return o;
 }
 
-// the pair records of particle s from its FP64 masters
+// r_i - r_j in the pair precision
 template <bool MIX>
-__device__ __forceinline__ void write_records(const FluidSet& f, int buf, int s, double vx,
-                                              double vy, double vz, double rho0, double drho,
-                                              double c, double nu, double h) {
-    using R4 = typename Prec<MIX>::R4;
-    using R = typename Prec<MIX>::R;
+__device__ __forceinline__ void rel(const FluidSet& f, const Nb<MIX>& o, const Nb<MIX>& q,
+                                    typename Prec<MIX>::R& rx, typename Prec<MIX>::R& ry,
+                                    typename Prec<MIX>::R& rz) {
+    if constexpr (MIX) {
+        rx = (float)(int)(o.px - q.px) * f.qr[0];
+        ry = (float)(int)(o.py - q.py) * f.qr[1];
+        rz = (float)(int)(o.pz - q.pz) * f.qr[2];
+    } else {
+        rx = o.px - q.px;
+        ry = o.py - q.py;
+        rz = o.pz - q.pz;
+    }
+}
+
+// 32-bit fixed point of one coordinate (clamped to the mapped span)
+__device__ __forceinline__ unsigned fixq(double x, double origin, double inv_unit) {
+    double t = rint((x - origin) * inv_unit);
+    t = t < 0.0 ? 0.0 : (t > 4294967295.0 ? 4294967295.0 : t);
+    return (unsigned)t;
+}
+__device__ __forceinline__ uint4 fix_pos(const FluidSet& f, const double4& p) {
+    return make_uint4(fixq(p.x, f.qo[0], f.qi[0]), fixq(p.y, f.qo[1], f.qi[1]),
+                      fixq(p.z, f.qo[2], f.qi[2]), __float_as_uint((float)p.w));
+}
+
+// the pair records of particle s from its FP64 masters (p = position, h)
+template <bool MIX>
+__device__ __forceinline__ void write_records(const FluidSet& f, int buf, int s, const double4& p,
+                                              double vx, double vy, double vz, double rho0,
+                                              double drho, double c, double nu) {
     const double rho = rho0 + drho;
-    const double h13 = h / 1.3;
+    const double h13 = p.w / 1.3;
     const double m = h13 * h13 * h13 * rho0;
-    R4 a, b;
-    a.x = (R)vx; a.y = (R)vy; a.z = (R)vz; a.w = (R)rho;
-    b.x = (R)(drho * c * c); b.y = (R)c; b.z = (R)(m / rho); b.w = (R)(nu * rho);
-    ((R4*)f.pv[buf])[s] = a;
-    ((R4*)f.pq[buf])[s] = b;
+    if constexpr (MIX) {
+        ((uint4*)f.p0[buf])[s] = fix_pos(f, p);
+        ((float4*)f.pv[buf])[s] = make_float4((float)vx, (float)vy, (float)vz, (float)(drho * c * c));
+        ((float4*)f.pq[buf])[s] = make_float4((float)c, (float)(m / rho), (float)rho, (float)(nu * rho));
+    } else {
+        ((double4*)f.pv[buf])[s] = make_double4(vx, vy, vz, rho);
+        ((double4*)f.pq[buf])[s] = make_double4(drho * c * c, c, m / rho, nu * rho);
+    }
+}
+
+// the neighbour row of slot s: the exact walked row or the Verlet superset;
+// both are ELL column-major with stride nf
+struct Row {
+    const int* p;
+    int n;
+};
+__device__ __forceinline__ Row nb_row(const Lists& L, int s) {
+    const int c = L.cnt[s];
+    if (c >= 0) return {L.nbr + s, c < L.K ? c : L.K};
+    const int m = L.sup_cnt[s];
+    return {L.sup + s, m < L.supK ? m : L.supK};
 }
 
 template <class R>
@@ -208,7 +281,7 @@ __global__ void __launch_bounds__(kBS) k_records(FluidSet f, int n, int buf) {
     if (s >= n) return;
     const double4 v = f.vr[s], p = f.ph[s];
     const double2 cn = f.cn[s];
-    write_records<MIX>(f, buf, s, v.x, v.y, v.z, v.w, f.drho[buf][s], cn.x, cn.y, p.w);
+    write_records<MIX>(f, buf, s, p, v.x, v.y, v.z, v.w, f.drho[buf][s], cn.x, cn.y);
 }
 
 // ------------------------------------------------------------- continuity
@@ -219,16 +292,19 @@ __global__ void __launch_bounds__(kBS) k_continuity(FluidArgs a, double dt) {
     if (s >= a.nf) return;
     const FluidSet& f = a.f;
     const Nb<MIX> o = load_nb<MIX>(f, a.cur, s);
+    const double4 ps = f.ph[s];
     Acc1<R> acc;
-    const int n = min(a.L.cnt[s], a.L.K);
-    const int* __restrict__ nbr = a.L.nbr + s;
-    for (int k = 0; k < n; k++) {
-        const int j = nbr[(int64_t)k * a.nf];
-        const Nb<MIX> q = load_nb<MIX>(f, a.cur, j);
-        const R rx = (R)(o.px - q.px), ry = (R)(o.py - q.py), rz = (R)(o.pz - q.pz);
+    const Row row = nb_row(a.L, s);
+    for (int k = 0; k < row.n; k++) {
+        const int j = row.p[(int64_t)k * a.nf];
+        Nb<MIX> q;
+        load_pos<MIX>(f, a.cur, j, q);
+        R rx, ry, rz;
+        rel<MIX>(f, o, q, rx, ry, rz);
         const R r = dist<R>(rx * rx + ry * ry + rz * rz);
         const R F = grad_factor<R>(r, R(0.5) * (o.h + q.h));
         if (F == R(0)) continue;
+        load_state<MIX>(f, a.cur, j, q);
         const R dvr = (o.vx - q.vx) * rx + (o.vy - q.vy) * ry + (o.vz - q.vz) * rz;
         const R c = R(0.5) * (o.c + q.c);
         acc.add(R(2) * o.rho * q.V * F * (q.rho * dvr + (o.P - q.P) * r * rcp_(c)) *
@@ -239,7 +315,7 @@ __global__ void __launch_bounds__(kBS) k_continuity(FluidArgs a, double dt) {
         const R rho0 = (R)vm.w;
         const int e = a.L.fsoff[s + 1];
         for (int k = a.L.fsoff[s]; k < e; k++) {
-            const SolidP<R> sp = load_solid<R>(a.so, a.L.fs_a[k], o.px, o.py, o.pz);
+            const SolidP<R> sp = load_solid<R>(a.so, a.L.fs_a[k], ps.x, ps.y, ps.z);
             const R r = dist<R>(sp.rx * sp.rx + sp.ry * sp.ry + sp.rz * sp.rz);
             const R F = grad_factor<R>(r, R(0.5) * (o.h + R(2.6) * sp.rad));
             if (F == R(0)) continue;
@@ -255,7 +331,7 @@ __global__ void __launch_bounds__(kBS) k_continuity(FluidArgs a, double dt) {
     const double nd = f.drho[a.cur][s] + dt * (double)acc.get();
     f.drho[1 - a.cur][s] = nd;
     const double2 cn = f.cn[s];
-    write_records<MIX>(f, 1 - a.cur, s, vm.x, vm.y, vm.z, vm.w, nd, cn.x, cn.y, f.ph[s].w);
+    write_records<MIX>(f, 1 - a.cur, s, ps, vm.x, vm.y, vm.z, vm.w, nd, cn.x, cn.y);
 }
 
 // --------------------------------------------------------------- momentum
@@ -266,19 +342,22 @@ __global__ void __launch_bounds__(kBS) k_momentum(FluidArgs a, int write_fs) {
     if (s >= a.nf) return;
     const FluidSet& f = a.f;
     const Nb<MIX> o = load_nb<MIX>(f, a.cur, s);
+    const double4 ps = f.ph[s];
     const R two_over_rho = R(2) * rcp_(o.rho);
     Acc3<R> acc;
-    const int n = min(a.L.cnt[s], a.L.K);
-    const int* __restrict__ nbr = a.L.nbr + s;
-    for (int k = 0; k < n; k++) {
-        const int j = nbr[(int64_t)k * a.nf];
-        const Nb<MIX> q = load_nb<MIX>(f, a.cur, j);
-        const R rx = (R)(o.px - q.px), ry = (R)(o.py - q.py), rz = (R)(o.pz - q.pz);
+    const Row row = nb_row(a.L, s);
+    for (int k = 0; k < row.n; k++) {
+        const int j = row.p[(int64_t)k * a.nf];
+        Nb<MIX> q;
+        load_pos<MIX>(f, a.cur, j, q);
+        R rx, ry, rz;
+        rel<MIX>(f, o, q, rx, ry, rz);
         const R r2 = rx * rx + ry * ry + rz * rz;
         const R ir = r2 > R(0) ? rsq_(r2) : R(0);
         const R r = r2 * ir;
         const R F = grad_factor<R>(r, R(0.5) * (o.h + q.h));
         if (F == R(0)) continue;
+        load_state<MIX>(f, a.cur, j, q);
         const R c = R(0.5) * (o.c + q.c);
         const R dvx = o.vx - q.vx, dvy = o.vy - q.vy, dvz = o.vz - q.vz;
         // U_L - U_R = -(v_i - v_j) . e_ij
@@ -293,11 +372,11 @@ __global__ void __launch_bounds__(kBS) k_momentum(FluidArgs a, int write_fs) {
     }
     if (fs_valid(a.L, a.nf)) {
         const double4 vm = f.vr[s];
-        const double h13 = f.ph[s].w / 1.3;
+        const double h13 = ps.w / 1.3;
         const double mi = h13 * h13 * h13 * vm.w;
         const int e = a.L.fsoff[s + 1];
         for (int k = a.L.fsoff[s]; k < e; k++) {
-            const SolidP<R> sp = load_solid<R>(a.so, a.L.fs_a[k], o.px, o.py, o.pz);
+            const SolidP<R> sp = load_solid<R>(a.so, a.L.fs_a[k], ps.x, ps.y, ps.z);
             R tx = 0, ty = 0, tz = 0;
             const bool on = fs_term<MIX>(o, (R)vm.w, sp, (R)a.gx, (R)a.gy, (R)a.gz, tx, ty, tz);
             if (on) acc.add(tx, ty, tz);
@@ -316,11 +395,12 @@ __global__ void __launch_bounds__(kBS) k_fs_force(FluidArgs a) {
     const int b0 = a.L.fsoff[s], e = a.L.fsoff[s + 1];
     if (b0 == e) return;
     const Nb<MIX> o = load_nb<MIX>(a.f, a.cur, s);
+    const double4 ps = a.f.ph[s];
     const double4 vm = a.f.vr[s];
-    const double h13 = a.f.ph[s].w / 1.3;
+    const double h13 = ps.w / 1.3;
     const double mi = h13 * h13 * h13 * vm.w;
     for (int k = b0; k < e; k++) {
-        const SolidP<R> sp = load_solid<R>(a.so, a.L.fs_a[k], o.px, o.py, o.pz);
+        const SolidP<R> sp = load_solid<R>(a.so, a.L.fs_a[k], ps.x, ps.y, ps.z);
         R tx = 0, ty = 0, tz = 0;
         fs_term<MIX>(o, (R)vm.w, sp, (R)a.gx, (R)a.gy, (R)a.gz, tx, ty, tz);
         a.L.fsF[k] = make_double4(mi * tx, mi * ty, mi * tz, 0.0);
@@ -340,10 +420,9 @@ __global__ void __launch_bounds__(kBS) k_density_reinit(FluidArgs a, int i_step)
     const R sigma = R(kWC) * ih * ih * ih;
     Acc1<R> sw;
     sw.add(sigma);  // W(0, h)
-    const int n = min(a.L.cnt[s], a.L.K);
-    const int* __restrict__ nbr = a.L.nbr + s;
-    for (int k = 0; k < n; k++) {
-        const double4 q = f.ph[nbr[(int64_t)k * a.nf]];
+    const Row row = nb_row(a.L, s);
+    for (int k = 0; k < row.n; k++) {
+        const double4 q = f.ph[row.p[(int64_t)k * a.nf]];
         const R rx = (R)(p.x - q.x), ry = (R)(p.y - q.y), rz = (R)(p.z - q.z);
         const R qq = dist<R>(rx * rx + ry * ry + rz * rz) * ih;
         if (qq >= R(2)) continue;
@@ -377,12 +456,13 @@ __global__ void __launch_bounds__(kBS) k_density_reinit(FluidArgs a, int i_step)
         const double nd = (rs > r1 ? rs : r1) - rho0;
         f.drho[a.cur][s] = nd;
         const double2 cn = f.cn[s];
-        write_records<MIX>(f, a.cur, s, vm.x, vm.y, vm.z, rho0, nd, cn.x, cn.y, p.w);
+        write_records<MIX>(f, a.cur, s, p, vm.x, vm.y, vm.z, rho0, nd, cn.x, cn.y);
     }
 }
 
 // ------------------------------------------------------------ integration
-__global__ void k_drift(FluidSet f, int n, double dt) {
+template <bool MIX>
+__global__ void k_drift(FluidSet f, int n, double dt, int cur) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     double4 p = f.ph[s];
@@ -391,6 +471,9 @@ __global__ void k_drift(FluidSet f, int n, double dt) {
     p.y += dt * v.y;
     p.z += dt * v.z;
     f.ph[s] = p;
+    if constexpr (MIX) {
+        ((uint4*)f.p0[cur])[s] = fix_pos(f, p);
+    }
 }
 
 // v += dt a (fluidVelocityIntegrate) then x += dt/2 v (fluidPositionIntegrate)
@@ -416,6 +499,9 @@ __global__ void k_kick_drift(FluidSet f, int n, double dt, int cur) {
     R4 r = pv[s];
     r.x = (R)v.x; r.y = (R)v.y; r.z = (R)v.z;
     pv[s] = r;
+    if constexpr (MIX) {
+        ((uint4*)f.p0[cur])[s] = fix_pos(f, p);
+    }
 }
 
 }  // namespace
@@ -440,8 +526,8 @@ void continuity(hipStream_t s, const FluidArgs& a, double dt, bool mixed) {
     if (a.nf > 0) DSPH_LAUNCH(k_continuity, a.nf, s, a, dt);
 }
 
-void fluid_drift(hipStream_t s, const FluidArgs& a, double dt) {
-    if (a.nf > 0) hipLaunchKernelGGL(k_drift, dim3(blocks_for(a.nf)), dim3(kBS), 0, s, a.f, a.nf, dt);
+void fluid_drift(hipStream_t s, const FluidArgs& a, double dt, bool mixed) {
+    if (a.nf > 0) DSPH_LAUNCH(k_drift, a.nf, s, a.f, a.nf, dt, a.cur);
 }
 
 void momentum(hipStream_t s, const FluidArgs& a, bool mixed, bool write_fs_force) {

@@ -159,17 +159,32 @@ __device__ __forceinline__ unsigned long long dbits(double d) {
 
 // largest displacement since the last build (positive doubles order like
 // their bit patterns)
-__global__ void k_disp(const double4* __restrict__ p, const double4* __restrict__ xb, int n,
-                       unsigned long long* slot) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    double d = 0.0;
-    if (s < n) {
+// grid-stride over the particles, one atomic per workgroup (thousands of
+// per-wavefront atomics on one word serialise in L2)
+__global__ void __launch_bounds__(kBS) k_disp(const double4* __restrict__ p,
+                                              const double4* __restrict__ xb, int n,
+                                              unsigned long long* slot) {
+    __shared__ unsigned long long wmax[kBS / 64];
+    unsigned long long m = 0;
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
         const double4 a = p[s], b = xb[s];
         const double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
-        d = sqrt(dx * dx + dy * dy + dz * dz);
+        double d = sqrt(dx * dx + dy * dy + dz * dz);
         if (!(d == d)) d = 1e300;  // NaN forces a rebuild
+        const unsigned long long v = dbits(d);
+        m = v > m ? v : m;
     }
-    wave_max_atomic(slot, dbits(d));
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        const unsigned long long u = __shfl_xor(m, d, 64);
+        m = u > m ? u : m;
+    }
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kBS / 64; w++) m = wmax[w] > m ? wmax[w] : m;
+        if (m) atomicMax(slot, m);
+    }
 }
 
 __global__ void k_decide(unsigned long long* ctl, unsigned long long* st, double skin) {
@@ -210,6 +225,7 @@ __global__ void k_reorder(FluidSet src, FluidSet dst, const int* __restrict__ pe
     dst.orig[t] = o;
     ((R4*)dst.pv[cur])[t] = ((const R4*)src.pv[cur])[old];
     ((R4*)dst.pq[cur])[t] = ((const R4*)src.pq[cur])[old];
+    if (src.p0[cur]) ((uint4*)dst.p0[cur])[t] = ((const uint4*)src.p0[cur])[old];
     slot_of[o] = t;
 }
 
@@ -228,6 +244,7 @@ __global__ void k_copyback(FluidSet dst, FluidSet src, int n, int cur,
     dst.orig[t] = src.orig[t];
     ((R4*)dst.pv[cur])[t] = ((const R4*)src.pv[cur])[t];
     ((R4*)dst.pq[cur])[t] = ((const R4*)src.pq[cur])[t];
+    if (src.p0[cur]) ((uint4*)dst.p0[cur])[t] = ((const uint4*)src.p0[cur])[t];
 }
 
 // slot of every sorted position, and the reference hash of every slot
@@ -275,9 +292,22 @@ __global__ void k_blk_fill(const int* __restrict__ cs, const int* __restrict__ c
     for (int b = 0; b * 64 < n; b++) blocks[o + b] = make_int2(c, st + 64 * b);
 }
 
+__device__ __forceinline__ double readlane_d(double v, int k) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, k);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 // Verlet superset build (setFluidInteractionsKernel's loop, radius + skin),
 // one wavefront per cell block.  Spare-cell particles and spare-cell
-// candidates are left out: they are walked exactly every step.
+// candidates are left out: they are walked exactly every step.  A candidate
+// cell is read 64 members at a time with one coalesced load per lane and
+// then broadcast lane by lane (readlane), so the scan over the members has
+// no memory latency in its loop.  Membership is d^2 <= (R (1 + 1e-9))^2 with
+// R = 2 max(h_i, h_j) + skin (or 2 h_i + skin for solids): a superset only
+// has to contain every pair that can come within range before the next
+// rebuild, so it errs on the inclusive side.
 __global__ void __launch_bounds__(kBS) k_build(SearchArgs a, const int2* __restrict__ blocks,
                                                const int* __restrict__ nblk_p) {
     if (a.ctl[CTL_REBUILD] == 0) return;
@@ -288,6 +318,7 @@ __global__ void __launch_bounds__(kBS) k_build(SearchArgs a, const int2* __restr
     const int nblk = *nblk_p;
     const int spare = g.ncell - 1;
     const double skin = a.skin;
+    const double grow = (1.0 + 1e-9) * (1.0 + 1e-9);
     const int supK = a.L.supK, fsupK = a.L.fsupK;
     bool ovf = false;
     for (int b = wv; b < nblk; b += nwv) {
@@ -308,25 +339,37 @@ __global__ void __launch_bounds__(kBS) k_build(SearchArgs a, const int2* __restr
         const double hi = p.w;
         const double Rf = (2.0 * fmax(hi, a.hmax) + skin) * (1.0 + 1e-9);
         const double Rs = (2.0 * hi + skin) * (1.0 + 1e-9);
+        const double Rs2 = (2.0 * hi + skin) * (2.0 * hi + skin) * grow;
         int c = 0, cs = 0;
         const int gxy = g.gx * g.gy;
         const int cz = cell / gxy, rem = cell - cz * gxy;
         const int cy = rem / g.gx, cx = rem - cy * g.gx;
-        for (int zz = -1; zz <= 1; zz++)
-            for (int yy = -1; yy <= 1; yy++)
-                for (int xx = -1; xx <= 1; xx++) {
+        const int w = g.sw;
+        for (int zz = -w; zz <= w; zz++)
+            for (int yy = -w; yy <= w; yy++)
+                for (int xx = -w; xx <= w; xx++) {
                     const int h = (cz + zz) * gxy + (cy + yy) * g.gx + (cx + xx);
                     if (h < 0 || h >= spare) continue;
                     const double bd2 = box_dist2(g, h, p.x, p.y, p.z);
                     if (__ballot(act && bd2 <= Rf * Rf) != 0) {
                         const int st0 = __builtin_amdgcn_readfirstlane(a.fcs[h]);
-                        if (st0 >= 0) {
-                            const int en = __builtin_amdgcn_readfirstlane(a.fce[h]);
-                            for (int u = st0; u < en; u++) {
-                                const int j = __builtin_amdgcn_readfirstlane(a.fmember[u]);
-                                const double4 q = a.ph[j];
-                                if (act && j != s &&
-                                    within(p.x, p.y, p.z, q.x, q.y, q.z, 2.0 * fmax(hi, q.w) + skin)) {
+                        const int en = __builtin_amdgcn_readfirstlane(a.fce[h]);
+                        for (int base = st0; st0 >= 0 && base < en; base += 64) {
+                            const int u = base + lane;
+                            int jl = -1;
+                            double4 ql = make_double4(0, 0, 0, 0);
+                            if (u < en) {
+                                jl = a.fmember[u];
+                                ql = a.ph[jl];
+                            }
+                            const int nb = min(64, en - base);
+                            for (int k = 0; k < nb; k++) {
+                                const int j = __builtin_amdgcn_readlane(jl, k);
+                                const double qx = readlane_d(ql.x, k), qy = readlane_d(ql.y, k),
+                                             qz = readlane_d(ql.z, k), qh = readlane_d(ql.w, k);
+                                const double dx = p.x - qx, dy = p.y - qy, dz = p.z - qz;
+                                const double R = 2.0 * fmax(hi, qh) + skin;
+                                if (act && j != s && dx * dx + dy * dy + dz * dz <= R * R * grow) {
                                     if (c < supK) a.L.sup[(int64_t)c * a.nf + s] = j;
                                     c++;
                                 }
@@ -335,14 +378,25 @@ __global__ void __launch_bounds__(kBS) k_build(SearchArgs a, const int2* __restr
                     }
                     if (__ballot(act && bd2 <= Rs * Rs) != 0) {
                         const int st1 = __builtin_amdgcn_readfirstlane(a.scs[h]);
-                        if (st1 < 0) continue;
                         const int en = __builtin_amdgcn_readfirstlane(a.sce[h]);
-                        for (int u = st1; u < en; u++) {
-                            const int ai = __builtin_amdgcn_readfirstlane(a.sperm[u]);
-                            const double4 q = a.spr[ai];
-                            if (act && within(p.x, p.y, p.z, q.x, q.y, q.z, 2.0 * hi + skin)) {
-                                if (cs < fsupK) a.L.fsup[(int64_t)cs * a.nf + s] = ai;
-                                cs++;
+                        for (int base = st1; st1 >= 0 && base < en; base += 64) {
+                            const int u = base + lane;
+                            int al = -1;
+                            double4 ql = make_double4(0, 0, 0, 0);
+                            if (u < en) {
+                                al = a.sperm[u];
+                                ql = a.spr[al];
+                            }
+                            const int nb = min(64, en - base);
+                            for (int k = 0; k < nb; k++) {
+                                const int ai = __builtin_amdgcn_readlane(al, k);
+                                const double qx = readlane_d(ql.x, k), qy = readlane_d(ql.y, k),
+                                             qz = readlane_d(ql.z, k);
+                                const double dx = p.x - qx, dy = p.y - qy, dz = p.z - qz;
+                                if (act && dx * dx + dy * dy + dz * dz <= Rs2) {
+                                    if (cs < fsupK) a.L.fsup[(int64_t)cs * a.nf + s] = ai;
+                                    cs++;
+                                }
                             }
                         }
                     }
@@ -470,7 +524,11 @@ __device__ void lane_walk(const SearchArgs& a, const double4 p, int s, int& c, i
             }
 }
 
-// exact lists of this step from the superset (or the exact walk)
+// The per-step list pass.  Rows that must follow the reference stencil this
+// step (walk) get their exact fluid neighbours in the ELL list, cnt = count;
+// every other row keeps its superset for the pair loops (cnt = -1), which
+// skip candidates outside the kernel support themselves.  Fluid–solid pairs
+// are filtered exactly for every row (COUNT, then FILL into the CSR).
 template <bool FILL>
 __global__ void __launch_bounds__(kBS) k_filter(SearchArgs a) {
     const int s = xcd_block() * blockDim.x + threadIdx.x;
@@ -478,26 +536,13 @@ __global__ void __launch_bounds__(kBS) k_filter(SearchArgs a) {
     if (s < a.nf) {
         const double4 p = a.ph[s];
         const unsigned spare = (unsigned)(a.g.ncell - 1);
-        const bool walk = a.ctl[CTL_WALK_ALL] != 0 || a.g.collide || a.hslot[s] == spare ||
+        const bool walk = a.ctl[CTL_WALK_ALL] != 0 || a.g.walk_only || a.hslot[s] == spare ||
                           stencil_has_spare(a.g, p.x, p.y, p.z);
         int64_t w = FILL ? (int64_t)a.L.fsoff[s] : 0;
         if (walk) {
             lane_walk<FILL>(a, p, s, c, cs, w);
         } else {
             const double hi = p.w;
-            if (!FILL) {
-                const int n = min(a.L.sup_cnt[s], a.L.supK);
-                const int* __restrict__ sup = a.L.sup + s;
-                for (int k = 0; k < n; k++) {
-                    const int j = sup[(int64_t)k * a.nf];
-                    if (a.hslot[j] == spare) continue;
-                    const double4 q = a.ph[j];
-                    if (within(p.x, p.y, p.z, q.x, q.y, q.z, 2.0 * fmax(hi, q.w))) {
-                        if (c < a.L.K) a.L.nbr[(int64_t)c * a.nf + s] = j;
-                        c++;
-                    }
-                }
-            }
             const int n2 = min(a.L.fsup_cnt[s], a.L.fsupK);
             const int* __restrict__ fsup = a.L.fsup + s;
             for (int k = 0; k < n2; k++) {
@@ -518,14 +563,44 @@ __global__ void __launch_bounds__(kBS) k_filter(SearchArgs a) {
             }
         }
         if (!FILL) {
-            a.L.cnt[s] = c;
+            a.L.cnt[s] = walk ? c : -1;
             a.L.fscnt[s] = cs;
+            a.fsp[s] = p;
         }
     }
     if (!FILL) {
         wave_max_atomic(&a.st[ST_FF_MAX], (unsigned long long)c);
         if (c > a.L.K) atomicOr(&a.st[ST_FF_OVERFLOW], 1ull);
     }
+}
+
+// Exact fluid neighbour rows of every particle for read-back, into a
+// separate ELL buffer of the superset's capacity (an exact row is a subset
+// of its superset row): walked rows are copied, superset rows filtered with
+// the reference's membership test on the positions of the last search.
+// Pair-loop state is not touched.
+__global__ void __launch_bounds__(kBS) k_exact_rows(SearchArgs a, int* __restrict__ out,
+                                                    int* __restrict__ out_cnt) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.nf) return;
+    const int cw = a.L.cnt[s];
+    int c = 0;
+    if (cw > a.L.K) {
+        c = -1;  // the walked row overflowed its capacity
+    } else if (cw >= 0) {
+        for (int k = 0; k < cw; k++) out[(int64_t)k * a.nf + s] = a.L.nbr[(int64_t)k * a.nf + s];
+        c = cw;
+    } else {
+        const double4 p = a.fsp[s];
+        const int n = min(a.L.sup_cnt[s], a.L.supK);
+        for (int k = 0; k < n; k++) {
+            const int j = a.L.sup[(int64_t)k * a.nf + s];
+            const double4 q = a.fsp[j];
+            if (within(p.x, p.y, p.z, q.x, q.y, q.z, 2.0 * fmax(p.w, q.w)))
+                out[(int64_t)(c++) * a.nf + s] = j;
+        }
+    }
+    out_cnt[s] = c;
 }
 
 __global__ void k_fs_status(const int* __restrict__ fsoff, int nf, int64_t cap,
@@ -748,9 +823,11 @@ void rebuild_decision(hipStream_t s, const double4* fph, const double4* fxb, int
                       const double4* spr, const double4* sxb, int ns, double skin,
                       unsigned long long* ctl, long long* status) {
     if (nf > 0)
-        hipLaunchKernelGGL(k_disp, dim3(blocks_for(nf)), dim3(kBS), 0, s, fph, fxb, nf, ctl + CTL_DISP_F);
+        hipLaunchKernelGGL(k_disp, dim3(std::min(blocks_for(nf), 1024)), dim3(kBS), 0, s, fph, fxb, nf,
+                           ctl + CTL_DISP_F);
     if (ns > 0)
-        hipLaunchKernelGGL(k_disp, dim3(blocks_for(ns)), dim3(kBS), 0, s, spr, sxb, ns, ctl + CTL_DISP_S);
+        hipLaunchKernelGGL(k_disp, dim3(std::min(blocks_for(ns), 1024)), dim3(kBS), 0, s, spr, sxb, ns,
+                           ctl + CTL_DISP_S);
     hipLaunchKernelGGL(k_decide, dim3(1), dim3(1), 0, s, ctl, (unsigned long long*)status, skin);
 }
 
@@ -798,6 +875,11 @@ void fluid_superset(hipStream_t s, const SearchArgs& a, const unsigned* cinv, in
     if (a.ns > 0)
         hipLaunchKernelGGL(k_store_xb, dim3(blocks_for(a.ns)), dim3(kBS), 0, s, a.spr, sxb, a.ns,
                            (const unsigned long long*)a.ctl);
+}
+
+void fluid_exact_rows(hipStream_t s, const SearchArgs& a, int* out, int* out_cnt) {
+    if (a.nf > 0)
+        hipLaunchKernelGGL(k_exact_rows, dim3(blocks_for(a.nf)), dim3(kBS), 0, s, a, out, out_cnt);
 }
 
 void fluid_filter(hipStream_t s, const SearchArgs& a, int* scan_tmp) {

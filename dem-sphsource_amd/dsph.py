@@ -51,6 +51,8 @@ _SIGS = {
     "dsph_step": [C.c_int32, C.c_double, C.c_int32, C.c_int],
     "dsph_step_n": [C.c_int32, C.c_int32, C.c_double, C.c_int32],
     "dsph_synchronize": [],
+    "dsph_set_contact_models": [_dp, _dp, _dp],
+    "dsph_set_clump_velocity": [C.c_int32, _dp],
     "dsph_add_external_force": [C.c_int32, _dp],
     "dsph_add_global_damping": [C.c_int32, C.c_double],
     "dsph_download_fluid": [_dp] * 5,
@@ -193,6 +195,17 @@ class Solver:
 
     def synchronize(self):
         self._call("dsph_synchronize")
+
+    def set_contact_models(self, hertz=None, linear=None, bonded=None):
+        """Replace contact tables during the run (field-major, as in Scene)."""
+        def arr(a):
+            return None if a is None else np.ascontiguousarray(np.asarray(a, float).reshape(-1))
+        h, l, b = arr(hertz), arr(linear), arr(bonded)
+        self._call("dsph_set_contact_models", _p(h) if h is not None else None,
+                   _p(l) if l is not None else None, _p(b) if b is not None else None)
+
+    def set_clump_velocity(self, idx, v):
+        self._call("dsph_set_clump_velocity", idx, _p(np.asarray(v, float)))
 
     def add_external_force(self, idx, f):
         self._call("dsph_add_external_force", idx, _p(np.asarray(f, float)))

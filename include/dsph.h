@@ -174,7 +174,9 @@ int dsph_set_skin(dsph_ctx* ctx, double skin_over_h);
 int dsph_initialize(dsph_ctx* ctx, int compute_normals);
 
 /* ---- the five phases (asynchronous on the context's stream) ------------ */
-/* neighborSearch (neighborSearch.h:28, neighborSearch.cu:314-381)          */
+/* neighborSearch (neighborSearch.h:28, neighborSearch.cu:314-381); lists
+ * that overflow are grown and the search re-run (one status read), as the
+ * reference's setNum reallocation does                                      */
 int dsph_neighbor_search(dsph_ctx* ctx, int32_t i_step, int32_t gap);
 /* solidIntegrateBeforeContact (integrate.h:43, integrate.cu:964-980)       */
 int dsph_solid_before_contact(dsph_ctx* ctx, double dt);
@@ -198,6 +200,19 @@ int dsph_step(dsph_ctx* ctx, int32_t i_step, double dt, int32_t gap,
 int dsph_step_n(dsph_ctx* ctx, int32_t i_step0, int32_t n, double dt,
                 int32_t gap);
 int dsph_synchronize(dsph_ctx* ctx);
+
+/* ---- runtime parameter changes ------------------------------------------
+ * replaces: dataBase::setHertzianContactModel / setLinearContactModel /
+ * setBondedContactModel after the run started (dataBase.h:111-167, which
+ * re-copy HostSolidContactModel to the device); the tables keep the upload's
+ * n_material and field-major layout; NULL leaves a table unchanged.        */
+int dsph_set_contact_models(dsph_ctx* ctx, const double* hertz,
+                            const double* linear, const double* bonded);
+/* replaces: dataBase::setClumpVelocity during the run (dataBase.h:169-182):
+ * the clump's translational velocity; its pebbles follow at the next clump
+ * kinematics update. */
+int dsph_set_clump_velocity(dsph_ctx* ctx, int32_t clump_index,
+                            const double velocity[3]);
 
 /* ---- runtime mutation (dataBase.cpp:179-224) ---------------------------- */
 int dsph_add_external_force(dsph_ctx* ctx, int32_t solid_index,
