@@ -863,6 +863,7 @@ int dsph_upload_scene(dsph_ctx* c, const dsph_scene* s) {
             f.drho[0] = dalloc<double>(c, nf);
             f.drho[1] = dalloc<double>(c, nf);
             f.cn = dalloc<double2>(c, nf);
+            f.alpha = dalloc<double>(c, nf);
             f.sumw0 = dalloc<double>(c, nf);
             f.acc = dalloc<double4>(c, nf);
             f.orig = dalloc<int>(c, nf);
@@ -891,6 +892,9 @@ int dsph_upload_scene(dsph_ctx* c, const dsph_scene* s) {
             h2d(c, f.drho[1], drho.data(), nf);
             h2d(c, f.cn, cn.data(), nf);
             h2d(c, f.sumw0, sumw0.data(), nf);
+            std::vector<double> alpha(nf, 0.0);
+            if (s->fluid_alpha) std::copy(s->fluid_alpha, s->fluid_alpha + nf, alpha.begin());
+            h2d(c, f.alpha, alpha.data(), nf);
             h2d(c, f.acc, acc.data(), nf);
             h2d(c, f.orig, orig.data(), nf);
         }

@@ -22,6 +22,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/dsph.h"
 #include "dsph_math.h"
 
 namespace dsph {
@@ -69,6 +70,7 @@ struct FluidSet {
     double* drho[2] = {nullptr, nullptr};  // FP64 master, Jacobi ping-pong
     double2* cn = nullptr;              // c, nu
     double* sumw0 = nullptr;
+    double* alpha = nullptr;            // artificial viscosity coefficient
     double4* acc = nullptr;
     int* orig = nullptr;                // slot -> caller index
     // neighbour records, ping-ponged with drho so the Jacobi update never

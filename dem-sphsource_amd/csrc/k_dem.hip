@@ -157,7 +157,7 @@ __global__ void k_walls(DemArgs a, double dt) {
     if (i >= a.ns) return;
     const double4 v4 = a.so.vel[i];
     if (a.so.dummy[i] == 1 || v4.w < 1.e-10) return;
-    const bool cont = (a.flags & 1u) != 0;  // DSPH_FLAG_WALL_CONTINUE
+    const bool cont = (a.flags & DSPH_FLAG_WALL_CONTINUE) != 0;
     const double4 pr = a.so.pr[i];
     const d3 pi = v3(pr), vi = v3(v4), wi = v3(a.so.w[i]);
     d3 acc = v3(a.so.acc[i]), tq = v3(a.so.tq[i]);
@@ -273,7 +273,7 @@ __global__ void k_clump_drift(DemArgs a, double dt) {
     p.y += dt * v.y;
     p.z += dt * v.z;
     a.c.pos[c] = p;
-    if (a.flags & 2u) {  // DSPH_FLAG_CLUMP_QUATERNION
+    if (a.flags & DSPH_FLAG_CLUMP_QUATERNION) {
         const double4 q4 = a.c.q[c];
         const Quat q = quat_rotate(Quat{q4.x, q4.y, q4.z, q4.w}, v3(a.c.w[c]), dt);
         a.c.q[c] = make_double4(q.q0, q.q1, q.q2, q.q3);

@@ -344,6 +344,11 @@ __global__ void __launch_bounds__(kBS) k_momentum(FluidArgs a, int write_fs) {
     const Nb<MIX> o = load_nb<MIX>(f, a.cur, s);
     const double4 ps = f.ph[s];
     const R two_over_rho = R(2) * rcp_(o.rho);
+    // artificial viscosity (SPHKernelDEMContactModel.h:74-77), opt-in: the
+    // particle's alpha times m_j h_ij c_ij (v_ij . r_ij) / (rho_ij r^2 +
+    // 0.01 h_ij^2) grad W_ij, oriented to remove kinetic energy
+    const bool art = (a.flags & DSPH_FLAG_ART_VISCOSITY) != 0;
+    const R alpha = art ? (R)f.alpha[s] : R(0);
     Acc3<R> acc;
     const Row row = nb_row(a.L, s);
     for (int k = 0; k < row.n; k++) {
@@ -367,8 +372,13 @@ __global__ void __launch_bounds__(kBS) k_momentum(FluidArgs a, int write_fs) {
         const R Ps = (o.rho * q.P + q.rho * o.P + o.rho * q.rho * beta * dU) * rcp_(o.rho + q.rho);
         const R gbar = R(2) * o.gamma * q.gamma * rcp_(o.gamma + q.gamma);
         const R pre = two_over_rho * q.V * F;
-        acc.add(pre * (gbar * dvx - Ps * rx), pre * (gbar * dvy - Ps * ry),
-                pre * (gbar * dvz - Ps * rz));
+        R sr = -pre * Ps;  // the radial factor of the pair term
+        if (art) {
+            const R hb = R(0.5) * (o.h + q.h), rb = R(0.5) * (o.rho + q.rho);
+            const R dvr = dvx * rx + dvy * ry + dvz * rz;
+            sr += q.V * q.rho * alpha * hb * c * dvr * rcp_(rb * r2 + R(0.01) * hb * hb) * F;
+        }
+        acc.add(pre * gbar * dvx + sr * rx, pre * gbar * dvy + sr * ry, pre * gbar * dvz + sr * rz);
     }
     if (fs_valid(a.L, a.nf)) {
         const double4 vm = f.vr[s];

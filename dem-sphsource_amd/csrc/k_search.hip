@@ -221,6 +221,7 @@ __global__ void k_reorder(FluidSet src, FluidSet dst, const int* __restrict__ pe
     dst.drho[cur][t] = src.drho[cur][old];
     dst.cn[t] = src.cn[old];
     dst.sumw0[t] = src.sumw0[old];
+    dst.alpha[t] = src.alpha[old];
     dst.acc[t] = src.acc[old];
     dst.orig[t] = o;
     ((R4*)dst.pv[cur])[t] = ((const R4*)src.pv[cur])[old];
@@ -240,6 +241,7 @@ __global__ void k_copyback(FluidSet dst, FluidSet src, int n, int cur,
     dst.drho[cur][t] = src.drho[cur][t];
     dst.cn[t] = src.cn[t];
     dst.sumw0[t] = src.sumw0[t];
+    dst.alpha[t] = src.alpha[t];
     dst.acc[t] = src.acc[t];
     dst.orig[t] = src.orig[t];
     ((R4*)dst.pv[cur])[t] = ((const R4*)src.pv[cur])[t];

@@ -38,6 +38,7 @@ class CScene(C.Structure):
         ("wall_point", _dp), ("wall_normal", _dp), ("wall_material", _ip),
         ("n_material", C.c_int32),
         ("hertz", _dp), ("linear", _dp), ("bonded", _dp),
+        ("fluid_alpha", _dp),
     ]
 
 
@@ -105,6 +106,7 @@ class Scene:
     hertz: np.ndarray | None = None
     linear: np.ndarray | None = None
     bonded: np.ndarray | None = None
+    fluid_alpha: np.ndarray | None = None  # artificial viscosity per fluid particle
     # driver parameters (not part of the C struct)
     dt: float = 1e-4
     gap: int = 1
@@ -219,5 +221,6 @@ class Scene:
         s.hertz = dp(self.hertz)
         s.linear = dp(self.linear)
         s.bonded = dp(self.bonded)
+        s.fluid_alpha = dp(self.fluid_alpha)
         s._keep = keep  # keep arrays alive with the struct
         return s

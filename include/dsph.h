@@ -53,8 +53,10 @@ extern "C" {
                                                (myQua.h:46-55); the reference
                                                keeps q frozen                */
 #define DSPH_FLAG_ART_VISCOSITY   (1u << 2) /* add the artificial-viscosity
-                                               term (SPHKernelDEMContactModel
-                                               .h:74-77), unused upstream   */
+                                               term of SPHKernelDEMContactModel
+                                               .h:74-77 (never called
+                                               upstream) with the particle's
+                                               alpha (dsph_scene.fluid_alpha)*/
 
 /* ---- list kinds for dsph_download_list -------------------------------- */
 #define DSPH_LIST_FLUID_FLUID  0 /* fluid2Fluid forward half list            */
@@ -133,6 +135,12 @@ typedef struct dsph_scene {
     const double* hertz;        /* [8*n_comb]  E,G,res,kr/ks,kt/ks,mu_s,mu_r,mu_t */
     const double* linear;       /* [11*n_comb] kn,ks,kr,kt,dn,ds,dr,dt,mu_s,mu_r,mu_t */
     const double* bonded;       /* [6*n_comb]  gamma,E,kn/ks,sigma_s,C,mu */
+
+    /* artificial viscosity coefficient per fluid particle, used with
+     * DSPH_FLAG_ART_VISCOSITY; optional (0).  A pair uses the alpha of the
+     * particle being updated: alpha is uniform within a scene.  No
+     * reference counterpart (the reference's helper takes one alpha). */
+    const double* fluid_alpha;  /* [n_fluid] */
 } dsph_scene;
 
 typedef struct dsph_ctx dsph_ctx;

@@ -19,7 +19,10 @@
  *  - solid–solid spring history carries over from the previous step's list
  *    (neighborSearch.cu:287-301), without the capacity-growth reset;
  *  - with no fluid the normal-estimation smoothing length is 0
- *    (dataBase.cpp:359 reads max_element of an empty vector).
+ *    (dataBase.cpp:359 reads max_element of an empty vector);
+ *  - DSPH_FLAG_ART_VISCOSITY adds the artificial viscosity of
+ *    SPHKernelDEMContactModel.h:74-77 (a helper never called upstream) in
+ *    the dissipative orientation, with the updated particle's alpha.
  */
 #include "oracle.h"
 
@@ -333,7 +336,7 @@ struct orc_ctx {
     /* fluid */
     int nf = 0;
     std::vector<V3> fpos, fvel, facc;
-    std::vector<double> fh, rho0, drho, c, nu, sumw0;
+    std::vector<double> fh, rho0, drho, c, nu, sumw0, alpha;
     std::vector<int> fhash, fperm, fsorted;
     /* solid */
     int ns = 0;
@@ -566,6 +569,7 @@ int orc_upload_scene(orc_ctx* o, const dsph_scene* s) {
     int n = o->nf = s->n_fluid;
     o->fpos.resize(n); o->fvel.resize(n); o->facc.assign(n, mk(0, 0, 0));
     o->fh.resize(n); o->rho0.resize(n); o->drho.assign(n, 0); o->c.resize(n); o->nu.resize(n);
+    o->alpha.assign(n, 0);
     o->sumw0.assign(n, 0);
     for (int i = 0; i < n; i++) {
         o->fpos[i] = ld3(s->fluid_pos, i);
@@ -575,6 +579,7 @@ int orc_upload_scene(orc_ctx* o, const dsph_scene* s) {
         if (s->fluid_drho) o->drho[i] = s->fluid_drho[i];
         o->c[i] = s->fluid_c[i];
         o->nu[i] = s->fluid_nu[i];
+        if (s->fluid_alpha) o->alpha[i] = s->fluid_alpha[i];
     }
     n = o->ns = s->n_solid;
     o->spos.resize(n); o->svel.resize(n); o->sacc.assign(n, mk(0, 0, 0)); o->sw.resize(n);
@@ -838,6 +843,7 @@ static void fluid_solid_force(orc_ctx* o) {
 
 /* solveMomentumConservationEquation (integrate.cu:163-280) */
 static void momentum(orc_ctx* o) {
+    const bool art = (o->flags & DSPH_FLAG_ART_VISCOSITY) != 0;
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < o->nf; i++) {
         V3 ri = o->fpos[i], vi = o->fvel[i];
@@ -863,6 +869,15 @@ static void momentum(orc_ctx* o) {
             double Ps = (rhoi * Pj + rhoj * Pi + rhoi * rhoj * beta * (UL - UR)) / (rhoi + rhoj);
             dv += -2 / mi * Vi * Vj * Ps * dW +
                   2 / mi * Vi * Vj * 2 * gi * gj / (gi + gj) * (vi - vj) / len(rij) * dot(e, dW);
+            if (art) {
+                /* getArtificialViscosityAccelerationFromParticleJ
+                 * (SPHKernelDEMContactModel.h:74-77): m_j alpha h_ij c_ij
+                 * (v_ij . r_ij) / (rho_ij r^2 + 0.01 h_ij^2) grad W_ij, signed
+                 * so that the pair term removes kinetic energy */
+                double rho_ij = 0.5 * (rhoi + rhoj);
+                dv += mj * o->alpha[i] * h * c * dot(vi - vj, rij) /
+                      (rho_ij * dot(rij, rij) + 0.01 * h * h) * dW;
+            }
         };
         for (int k = o->ff.seg_begin(i); k < o->ff.seg_end(i); k++) pair(o->ff.pointing[k]);
         if (o->ff.rstart[i] != -1)

@@ -241,6 +241,27 @@ def test_flags_match_oracle(flags):
     assert np.allclose(qa, qb, atol=1e-10)
 
 
+@pytest.mark.parametrize("prec", [dsph.PREC_FP64, dsph.PREC_MIXED])
+def test_artificial_viscosity_matches_oracle(prec):
+    import copy
+    sc = copy.copy(SCENES["random_box"])
+    sc.fluid_alpha = np.full(sc.n_fluid, 0.08)
+    flags = dsph.FLAG_ART_VISCOSITY
+    g, o = make_pair(sc, prec, flags)
+    g.step(1, sc.dt)
+    o.step(1, sc.dt)
+    tol = TOL_FP64 if prec == dsph.PREC_FP64 else TOL_MIXED
+    gf, of = g.fluid(), o.fluid()
+    for k in ("pos", "vel", "acc", "drho"):
+        assert refmodel.rel_err(gf[k], of[k]) < tol, k
+    # and the term is really there: without it the accelerations differ
+    h = dsph.Solver(prec)
+    h.upload(sc)
+    h.initialize()
+    h.step(1, sc.dt)
+    assert refmodel.rel_err(h.fluid()["acc"], of["acc"]) > 100 * tol
+
+
 # --------------------------------------------- graphs, determinism, capacity
 @pytest.mark.parametrize("prec", [dsph.PREC_FP64, dsph.PREC_MIXED])
 def test_graph_replay_equals_eager(prec):

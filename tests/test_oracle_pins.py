@@ -467,3 +467,59 @@ def test_torque_free_clump_keeps_angular_velocity():
     c = o.clump()
     assert np.allclose(c["angvel"][0], w, rtol=0, atol=1e-12)
     assert np.allclose(c["centroid"][0], 0, atol=1e-15)
+
+
+# ------------------------------------------------------ artificial viscosity
+def _av_accel(sc, alpha):
+    """The artificial-viscosity part of the momentum rate: acc with the term
+    minus acc without it, from the same state (it enters only the momentum
+    pass, after the first half-step of continuity and drift)."""
+    out = []
+    for flags in (0, dsph_flags()["ART_VISCOSITY"]):
+        sc.fluid_alpha = np.full(sc.n_fluid, alpha)
+        o = oracle.Oracle()
+        o.set_flags(flags)
+        o.upload(sc)
+        o.initialize()
+        o.step(2, 1e-6)
+        out.append(o.fluid()["acc"])
+        o.close()
+    return out[1] - out[0]
+
+
+def dsph_flags():
+    import dsph
+    return {"ART_VISCOSITY": dsph.FLAG_ART_VISCOSITY}
+
+
+def test_artificial_viscosity_conserves_momentum_and_dissipates():
+    sc = scenes.random_fluid_box(box=(0.05, 0.04, 0.04), dx=0.005, seed_stream=23,
+                                 with_walls=False, vel_scale=0.2, drho_scale=2.0)
+    sc.gravity = np.zeros(3)
+    a = _av_accel(sc, 0.05)
+    m = (np.asarray(sc.fluid_h) / 1.3) ** 3 * np.asarray(sc.fluid_rho0)
+    scale = np.abs(m[:, None] * a).sum()
+    assert scale > 0
+    # pair terms are equal and opposite (uniform alpha and mass)
+    assert np.abs((m[:, None] * a).sum(axis=0)).max() < 1e-12 * scale
+    # the work done on the particles is negative: sum_pairs -|F|(v.r)^2 <= 0
+    v = np.asarray(sc.fluid_vel)
+    assert (m[:, None] * a * v).sum() < 0
+
+
+def test_artificial_viscosity_is_linear_in_alpha():
+    sc = scenes.random_fluid_box(box=(0.04, 0.03, 0.03), dx=0.005, seed_stream=24,
+                                 with_walls=False, vel_scale=0.2, drho_scale=2.0)
+    a1, a2 = _av_accel(sc, 0.03), _av_accel(sc, 0.06)
+    assert np.abs(a2 - 2 * a1).max() < 1e-9 * np.abs(a2).max()
+
+
+def test_artificial_viscosity_slows_an_approaching_pair():
+    sc = scenes.empty_scene((-0.1, -0.1, -0.1), (0.2, 0.2, 0.2))
+    sc.gravity = np.zeros(3)
+    dx = 0.01
+    scenes.add_fluid(sc, [[-0.5 * dx, 0, 0], [0.5 * dx, 0, 0]], 1.3 * dx, 1000.0, 20.0, 1e-6)
+    sc.fluid_vel = np.array([[0.1, 0, 0], [-0.1, 0, 0]])
+    a = _av_accel(sc, 0.1)
+    assert a[0, 0] < 0 < a[1, 0]          # each pushed back along its approach
+    assert np.abs(a[:, 1:]).max() < 1e-12 * abs(a[0, 0])
