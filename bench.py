@@ -49,30 +49,38 @@ def build_scene(dx):
     return scenes.coupled_dam_break(dx=dx)
 
 
-def run_gpu(scene, precision, steps, warmup, timing_steps, eager=False):
+def run_gpu(scene, precision, steps, warmup, timing_steps, eager=False, flags=0):
     """Times `steps` steps after `warmup` (HIP-graph replay unless eager), then
     runs `timing_steps` more steps eagerly with per-family HIP events on the
     context stream (events cannot live inside a captured graph; kernel
-    durations do not depend on how the kernels were launched)."""
+    durations do not depend on how the kernels were launched).  `scene` may
+    be a list of scenes: they are uploaded as one batch and step with the
+    smallest of their time steps."""
     import dsph
     g = dsph.Solver(precision)
-    g.upload(scene)
+    g.set_flags(flags)
+    if isinstance(scene, list):
+        g.upload_batch(scene)
+        dt = min(sc.dt for sc in scene)
+    else:
+        g.upload(scene)
+        dt = scene.dt
     g.initialize()
     i = 1
     if warmup:
-        g.run(warmup, scene.dt, first=i, graphs=not eager)
+        g.run(warmup, dt, first=i, graphs=not eager)
         i += warmup
     g.synchronize()
     t0 = time.perf_counter()
     # with graphs the run ends with a status read (device sync)
-    g.run(steps, scene.dt, first=i, graphs=not eager)
+    g.run(steps, dt, first=i, graphs=not eager)
     g.synchronize()
     t1 = time.perf_counter()
     i += steps
     times = {}
     if timing_steps:
         g.set_timing(True)
-        g.run(timing_steps, scene.dt, first=i, graphs=False)
+        g.run(timing_steps, dt, first=i, graphs=False)
         g.synchronize()
         times = g.kernel_times()
         g.set_timing(False)
@@ -165,6 +173,55 @@ def cpu_baseline(scene, budget_s, max_steps):
                       f"initial search ({dt:.1f} s)"}
 
 
+def other_workload(args):
+    """The batched ensemble or the granular column: one JSON line of the same
+    shape (no CPU baseline: the oracle runs one scene at a time)."""
+    import dsph
+    import scenes
+    if args.workload == "ensemble":
+        scs = scenes.ensemble(args.scenes)
+        flags = dsph.FLAG_ART_VISCOSITY
+        nf = sum(sc.n_fluid for sc in scs)
+        desc = (f"batched dam-break ensemble: {len(scs)} independent scenes in one context "
+                f"(segmented grids), {nf} fluid + {sum(sc.n_solid for sc in scs)} wall/DEM, "
+                f"fluid per scene log-uniform 2k-60k, DEM 0-500, tank 0.8-1.6 m, alpha 0.01-0.1")
+        work = scs
+    else:
+        sc = scenes.granular_column(n_sphere=110000, n_clump=12500, n_bonded=40000)
+        flags, nf = 0, 0
+        desc = (f"granular column collapse: {sc.n_solid} spheres (110k free, 12.5k tetrahedral "
+                f"clumps of 4, a 40k bonded block), no fluid")
+        work, scs = sc, [sc]
+    npart = sum(sc.n_particles for sc in scs)
+    print(f"# workload: {desc}", file=sys.stderr, flush=True)
+    tsteps = 0 if args.no_timing else args.timing_steps
+    g, secs, times = run_gpu(work, dsph.PREC_MIXED, args.steps, args.warmup, tsteps, args.eager, flags)
+    value = npart * args.steps / secs
+    print(f"# mixed: {secs / args.steps * 1e3:.3f} ms/step  {value:.4g} particle-steps/s",
+          file=sys.stderr, flush=True)
+    g.close()
+    fp64 = None
+    if not args.no_fp64 and args.fp64_steps > 0:
+        g64, s64, _ = run_gpu(work, dsph.PREC_FP64, args.fp64_steps, args.warmup, 0, args.eager, flags)
+        fp64 = {"value": round(npart * args.fp64_steps / s64, 1),
+                "ms_per_step": round(s64 / args.fp64_steps * 1e3, 4), "steps": args.fp64_steps}
+        g64.close()
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(secs / args.steps * 1e3, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+        "data": "synthetic: seeded SplitMix64 scenes",
+        "config": {"workload": desc, "particles": npart,
+                   "precision": "mixed (f32 pair loops, f64 integration)"},
+        "fp64_path": fp64,
+        "mixed_over_fp64": round(value / fp64["value"], 3) if fp64 else None,
+        "kernel_ms": {k: [round(v[0] / max(v[1], 1), 4), v[1]] for k, v in times.items()},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -183,6 +240,10 @@ def main():
                     help="launch every step's kernels directly instead of replaying HIP graphs")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration)")
+    ap.add_argument("--workload", default="coupled", choices=["coupled", "ensemble", "granular"],
+                    help="coupled: the 1M-particle coupled dam break (the headline); ensemble: "
+                         "the batched dam-break ensemble; granular: the clump/bond column")
+    ap.add_argument("--scenes", type=int, default=256, help="scenes of the ensemble workload")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -193,6 +254,9 @@ def main():
 
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.workload != "coupled":
+        other_workload(args)
         return
 
     import dsph

@@ -57,9 +57,19 @@ struct dsph_ctx {
     std::string err;
     bool uploaded = false, initialized = false;
     int nf = 0, ns = 0, nc = 0, nw = 0;
-    GridP g{};
+    // scenes: one grid each; particles of scene k are the caller indices
+    // [foff[k], foff[k+1]) (fluid), [soff[k], soff[k+1]) (solid), ...
+    int nscene = 1;
+    std::vector<GridP> grids;
+    GridP* dgrids = nullptr;
+    int ncell = 0;  // cells of all scenes
+    std::vector<int> foff, soff, coff, woff;
+    std::vector<int> h_fscene, h_sscene;  // scene of each caller index
+    int* sscene = nullptr;                // device, solid -> scene
+    int* dwoff = nullptr;                 // device, walls of each scene
+    int nwmax = 0;                        // most walls of one scene
     int key_bits = 1;
-    double hmax = 0;
+    double hmax = 0;  // largest fluid smoothing length of all scenes
     double gv[3] = {0, 0, -9.81};
     // fluid
     FluidSet fset[2];
@@ -280,6 +290,10 @@ DemArgs dem_args(dsph_ctx* c) {
     a.wn = c->wn;
     a.wmat = c->wmat;
     a.nw = c->nw;
+    a.woff = c->dwoff;
+    a.sscene = c->sscene;
+    a.nscene = c->nscene;
+    a.nwmax = c->nwmax;
     a.whist = c->whist;
     a.tb = c->tb;
     a.gx = c->gv[0];
@@ -295,18 +309,33 @@ DemArgs dem_args(dsph_ctx* c) {
 // while gx, gy >= 2 sw + 1; narrower grids (or very large skins) walk the
 // reference stencil for every row instead.
 void superset_stencil(dsph_ctx* c) {
-    GridP& g = c->g;
-    const double R = (2.0 + c->skin_h) * c->hmax * (1.0 + 1e-6);
-    const double cmin = std::min({g.csx, g.csy, g.csz});
-    int w = cmin > 0 ? (int)std::ceil(R / cmin) : 1;
-    w = std::max(w, 1);
-    g.sw = std::min(w, 8);
-    g.walk_only = w > 8 || g.gx < 2 * g.sw + 1 || g.gy < 2 * g.sw + 1;
+    for (GridP& g : c->grids) {
+        g.skin = c->skin_h * g.hmax;
+        const double R = (2.0 + c->skin_h) * g.hmax * (1.0 + 1e-6);
+        const double cmin = std::min({g.csx, g.csy, g.csz});
+        int w = cmin > 0 ? (int)std::ceil(R / cmin) : 1;
+        w = std::max(w, 1);
+        g.sw = std::min(w, 8);
+        g.walk_only = w > 8 || g.gx < 2 * g.sw + 1 || g.gy < 2 * g.sw + 1;
+    }
+    if (c->dgrids) h2d(c, c->dgrids, c->grids.data(), c->grids.size());
+}
+
+// the grids as the kernels see them
+Grids grids_view(dsph_ctx* c) {
+    Grids G;
+    G.g = c->dgrids;
+    G.fscene = c->fset[0].scene;
+    G.sscene = c->sscene;
+    G.nscene = c->nscene;
+    G.g0 = c->grids[0];
+    return G;
 }
 
 SearchArgs search_args(dsph_ctx* c) {
     SearchArgs a;
-    a.g = c->g;
+    a.G = grids_view(c);
+    a.ncell = c->ncell;
     a.nf = c->nf;
     a.ns = c->ns;
     a.ph = c->fset[0].ph;
@@ -321,8 +350,6 @@ SearchArgs search_args(dsph_ctx* c) {
     a.skey = c->skey;
     a.spr = c->so.pr;
     a.fsp = c->fsp;
-    a.hmax = c->hmax;
-    a.skin = c->skin_h * c->hmax;
     a.L = c->L;
     a.ctl = c->ctl;
     a.st = c->status;
@@ -338,19 +365,19 @@ void neighbor_search(dsph_ctx* c, int i_step, int gap, bool save_history) {
     const bool fluid_step = i_step % gap == 0 && c->nf > 0;
     {
         FamTimer t(c, F_GRID);
-        hash_fluid(s, c->g, c->fset[0].ph, c->fset[0].orig, c->nf, c->crank, c->fkey, c->frank,
-                   stw);
-        hash_solid(s, c->g, c->so.pr, c->ns, c->skey, stw);
+        const Grids G = grids_view(c);
+        hash_fluid(s, G, c->fset[0].ph, c->fset[0].orig, c->nf, c->crank, c->fkey, c->frank, stw);
+        hash_solid(s, G, c->so.pr, c->ns, c->skey, stw);
         // ranges of an empty species are filled with -1 as well
         if (c->nf > 0)
             radix_sort_iota(s, c->frank, c->nf, c->key_bits, c->fkey_sorted, c->fperm, c->radix_tmp);
-        cell_spans(s, c->fkey_sorted, c->nf, c->fcs, c->fce, c->g.ncell, c->cinv);
+        cell_spans(s, c->fkey_sorted, c->nf, c->fcs, c->fce, c->ncell, c->cinv);
         if (c->ns > 0)
             radix_sort_iota(s, c->skey, c->ns, c->key_bits, c->skey_sorted, c->sperm, c->radix_tmp);
-        cell_spans(s, c->skey_sorted, c->ns, c->scs, c->sce, c->g.ncell, nullptr);
+        cell_spans(s, c->skey_sorted, c->ns, c->scs, c->sce, c->ncell, nullptr);
         if (fluid_step) {
-            rebuild_decision(s, c->fset[0].ph, c->fxb, c->nf, c->so.pr, c->sxb, c->ns,
-                             c->skin_h * c->hmax, c->ctl, stw);
+            rebuild_decision(s, G, c->fset[0].ph, c->fxb, c->nf, c->so.pr, c->sxb, c->ns,
+                             c->skin_h <= 0.0, c->ctl, stw);
             if (c->reorder_period > 0)
                 reorder_fluid(s, c->fset[0], c->fset[1], c->fperm, c->fslot, c->nf, c->mixed(),
                               c->dcur, c->ctl);
@@ -374,9 +401,9 @@ void neighbor_search(dsph_ctx* c, int i_step, int gap, bool save_history) {
     if (c->ns > 0) {
         FamTimer t(c, F_SSEARCH);
         if (save_history) solid_save_history(s, c->L, c->ns);
-        active_grid(s, c->so, c->skey, c->g.ncell, c->key_bits, c->akey, c->akey_sorted, c->aperm,
+        active_grid(s, c->so, c->skey, c->ncell, c->key_bits, c->akey, c->akey_sorted, c->aperm,
                     c->acs, c->ace, c->radix_tmp);
-        solid_search(s, c->g, c->so, c->ns, c->acs, c->ace, c->aperm, c->L, c->scan_tmp, stw);
+        solid_search(s, grids_view(c), c->so, c->ns, c->acs, c->ace, c->aperm, c->L, c->scan_tmp, stw);
         reverse_solid(s, c->L, c->ns, c->scan_tmp);
     }
 }
@@ -552,7 +579,7 @@ void stencil_hashes(const GridP& g, double x, double y, double z, int H[27]) {
         for (int yy = -1; yy <= 1; yy++)
             for (int xx = -1; xx <= 1; xx++) {
                 const int h = (az + zz) * g.gy * g.gx + (ay + yy) * g.gx + (ax + xx);
-                H[k++] = (h < 0 || h >= g.ncell) ? -1 : h;
+                H[k++] = (h < 0 || h >= g.ncell) ? -1 : g.cell0 + h;  // global cell
             }
 }
 int stencil_rank(const int H[27], int h) {
@@ -647,11 +674,12 @@ std::vector<int64_t> fs_reference_order(dsph_ctx* c, std::vector<int>& fluid_idx
     for (int i = 0; i < nf; i++) {
         const int s2 = fh.slot_of[i];
         int H[27];
-        stencil_hashes(c->g, fh.ph[s2].x, fh.ph[s2].y, fh.ph[s2].z, H);
+        const GridP& g = c->grids[c->h_fscene[i]];
+        stencil_hashes(g, fh.ph[s2].x, fh.ph[s2].y, fh.ph[s2].z, H);
         row.clear();
         for (int k = off[s2]; k < off[s2 + 1]; k++)
             row.push_back({(int)skey[a[k]], a[k], (int64_t)k, 0});
-        order_row(c->g, H, row);
+        order_row(g, H, row);
         for (auto& e : row) {
             ids.push_back(e.id);
             fluid_idx.push_back(i);
@@ -664,6 +692,227 @@ std::vector<int64_t> fs_reference_order(dsph_ctx* c, std::vector<int>& fluid_idx
 }  // namespace
 
 // ==========================================================================
+namespace {
+// ---------------------------------------------------------------- batches
+// Several independent scenes in one context: their arrays are concatenated in
+// the order given (so caller indices of scene k follow those of scene k-1),
+// clump and cluster ids and pebble ranges are shifted to stay distinct, and
+// each scene keeps its own hashed grid (setSpatialGrids, dataBase.cpp:415-432)
+// at its own range of global cell indices.
+struct Merged {
+    std::vector<double> fpos, fvel, fh, frho0, fdrho, fc, fnu, falpha;
+    std::vector<double> spos, svel, sw, snrm, srad, seff, sinvm;
+    std::vector<int> smat, sclus, sclump, sdum;
+    std::vector<double> cpos, cvel, cw, cq, cinvI, cinvm;
+    std::vector<int> cstart, cend;
+    std::vector<double> wpt, wn;
+    std::vector<int> wmat;
+    std::vector<int> fscene, sscene;           // caller order
+    std::vector<int> foff, soff, coff, woff;   // nscene + 1 each
+    std::vector<GridP> grids;
+    bool any_normal = false;
+    dsph_scene flat{};
+};
+
+template <class T>
+void put(std::vector<T>& v, const T* src, size_t n, size_t width, T dflt) {
+    if (src)
+        v.insert(v.end(), src, src + n * width);
+    else
+        v.insert(v.end(), n * width, dflt);
+}
+
+void validate_scene(const dsph_scene* s) {
+    require(s->n_fluid >= 0 && s->n_solid >= 0 && s->n_clump >= 0 && s->n_wall >= 0,
+            "negative particle count");
+    require(s->n_fluid == 0 || (s->fluid_pos && s->fluid_h && s->fluid_rho0 && s->fluid_c && s->fluid_nu),
+            "fluid arrays missing");
+    require(s->n_solid == 0 || (s->solid_pos && s->solid_radius && s->solid_eff_radius && s->solid_inv_mass),
+            "solid arrays missing");
+    require(s->n_clump == 0 || (s->clump_centroid && s->clump_inv_inertia && s->clump_inv_mass &&
+                                s->clump_pebble_start && s->clump_pebble_end),
+            "clump arrays missing");
+    require(s->n_wall == 0 || (s->wall_point && s->wall_normal), "wall arrays missing");
+    require(s->n_material >= 1, "n_material must be >= 1");
+    for (int k = 0; k < 3; k++) require(s->domain_size[k] > 0, "domain size must be positive");
+    const int nmat = s->n_material;
+    for (int i = 0; i < s->n_solid; i++) {
+        int m = s->solid_material ? s->solid_material[i] : 0;
+        int d = s->solid_dummy ? s->solid_dummy[i] : 0;
+        // an invalid material pair would index past the contact tables (the
+        // reference reads out of bounds there, deviceStructs.h:960-963)
+        require(d == 1 || (m >= 0 && m < nmat),
+                "solid " + std::to_string(i) + " has material outside [0, n_material)");
+    }
+    for (int i = 0; i < s->n_wall; i++) {
+        int m = s->wall_material ? s->wall_material[i] : 0;
+        require(m >= 0 && m < nmat, "wall material outside [0, n_material)");
+    }
+    for (int k = 0; k < s->n_clump; k++)
+        require(s->clump_pebble_start[k] >= 0 && s->clump_pebble_end[k] <= s->n_solid &&
+                    s->clump_pebble_start[k] <= s->clump_pebble_end[k],
+                "clump pebble range out of bounds");
+    for (int i = 0; i < s->n_fluid; i++) require(s->fluid_h[i] > 0, "fluid smoothing length must be > 0");
+}
+
+// the grid of one scene: cell = 2 max(all effective radii), G = int(L / cell)
+// (at least 1), cell size L / G per axis, plus the spare cell
+GridP scene_grid(const dsph_scene* s, int cell0) {
+    double mf = 0, ms = 0;
+    for (int i = 0; i < s->n_fluid; i++) mf = std::max(mf, s->fluid_h[i]);
+    for (int i = 0; i < s->n_solid; i++) ms = std::max(ms, s->solid_eff_radius[i]);
+    const double cs = 2 * std::max(mf, ms);
+    GridP g{};
+    g.minx = s->domain_origin[0];
+    g.miny = s->domain_origin[1];
+    g.minz = s->domain_origin[2];
+    g.maxx = s->domain_origin[0] + s->domain_size[0];
+    g.maxy = s->domain_origin[1] + s->domain_size[1];
+    g.maxz = s->domain_origin[2] + s->domain_size[2];
+    g.gx = s->domain_size[0] > cs ? int(s->domain_size[0] / cs) : 1;
+    g.gy = s->domain_size[1] > cs ? int(s->domain_size[1] / cs) : 1;
+    g.gz = s->domain_size[2] > cs ? int(s->domain_size[2] / cs) : 1;
+    g.csx = s->domain_size[0] / double(g.gx);
+    g.csy = s->domain_size[1] / double(g.gy);
+    g.csz = s->domain_size[2] / double(g.gz);
+    const long long ncell = (long long)g.gx * g.gy * g.gz + 1;
+    require(ncell < (1LL << 30), "grid has too many cells");
+    g.ncell = (int)ncell;
+    g.cell0 = cell0;
+    g.collide = g.gx < 3 || g.gy < 3;
+    const double ext = std::max({std::fabs(g.minx), std::fabs(g.maxx), std::fabs(g.miny), std::fabs(g.maxy),
+                                 std::fabs(g.minz), std::fabs(g.maxz)});
+    g.pad = 1e-9 * (1.0 + ext);
+    g.hmax = mf;
+    return g;
+}
+
+// a scene's contact tables with the reference's defaults where none are given
+// (restitution 1, bond multiplier 1, kn/ks 2; hostStructs.h:290-346)
+std::vector<double> contact_tables(const dsph_scene* s) {
+    const size_t ncb = (size_t)s->n_material * (s->n_material + 1) / 2;
+    std::vector<double> t(25 * ncb, 0.0);
+    double* hz = t.data();
+    double* li = hz + 8 * ncb;
+    double* bo = li + 11 * ncb;
+    for (size_t k = 0; k < ncb; k++) {
+        hz[2 * ncb + k] = 1.0;
+        bo[k] = 1.0;
+        bo[2 * ncb + k] = 2.0;
+    }
+    if (s->hertz) std::copy(s->hertz, s->hertz + 8 * ncb, hz);
+    if (s->linear) std::copy(s->linear, s->linear + 11 * ncb, li);
+    if (s->bonded) std::copy(s->bonded, s->bonded + 6 * ncb, bo);
+    return t;
+}
+
+void merge_scenes(const dsph_scene* sc, int k, Merged& m) {
+    require(k >= 1, "a batch needs at least one scene");
+    const std::vector<double> tables0 = contact_tables(sc);
+    m.foff = {0};
+    m.soff = {0};
+    m.coff = {0};
+    m.woff = {0};
+    int cell0 = 0, cluster_base = 0;
+    for (int b = 0; b < k; b++) {
+        const dsph_scene* s = sc + b;
+        validate_scene(s);
+        require(s->n_material == sc[0].n_material, "scenes of a batch share n_material");
+        require(b == 0 || contact_tables(s) == tables0, "scenes of a batch share the contact tables");
+        const size_t nf = s->n_fluid, ns = s->n_solid, nc = s->n_clump, nw = s->n_wall;
+        put(m.fpos, s->fluid_pos, nf, 3, 0.0);
+        put(m.fvel, s->fluid_vel, nf, 3, 0.0);
+        put(m.fh, s->fluid_h, nf, 1, 0.0);
+        put(m.frho0, s->fluid_rho0, nf, 1, 0.0);
+        put(m.fdrho, s->fluid_drho, nf, 1, 0.0);
+        put(m.fc, s->fluid_c, nf, 1, 0.0);
+        put(m.fnu, s->fluid_nu, nf, 1, 0.0);
+        put(m.falpha, s->fluid_alpha, nf, 1, 0.0);
+        m.fscene.insert(m.fscene.end(), nf, b);
+        const int sbase = m.soff.back();
+        put(m.spos, s->solid_pos, ns, 3, 0.0);
+        put(m.svel, s->solid_vel, ns, 3, 0.0);
+        put(m.sw, s->solid_angvel, ns, 3, 0.0);
+        put(m.snrm, s->solid_normal, ns, 3, 0.0);
+        m.any_normal = m.any_normal || (s->solid_normal && ns > 0);
+        put(m.srad, s->solid_radius, ns, 1, 0.0);
+        put(m.seff, s->solid_eff_radius, ns, 1, 0.0);
+        put(m.sinvm, s->solid_inv_mass, ns, 1, 0.0);
+        put(m.smat, s->solid_material, ns, 1, 0);
+        put(m.sdum, s->solid_dummy, ns, 1, 0);
+        int cmax = -1;
+        for (size_t i = 0; i < ns; i++) {
+            const int cl = s->solid_cluster ? s->solid_cluster[i] : -1;
+            m.sclus.push_back(cl >= 0 ? cl + cluster_base : -1);
+            cmax = std::max(cmax, cl);
+            const int cp = s->solid_clump ? s->solid_clump[i] : -1;
+            m.sclump.push_back(cp >= 0 ? cp + m.coff.back() : -1);
+        }
+        cluster_base += cmax + 1;
+        m.sscene.insert(m.sscene.end(), ns, b);
+        put(m.cpos, s->clump_centroid, nc, 3, 0.0);
+        put(m.cvel, s->clump_vel, nc, 3, 0.0);
+        put(m.cw, s->clump_angvel, nc, 3, 0.0);
+        for (size_t i = 0; i < nc; i++) {
+            if (s->clump_orientation)
+                m.cq.insert(m.cq.end(), s->clump_orientation + 4 * i, s->clump_orientation + 4 * i + 4);
+            else
+                m.cq.insert(m.cq.end(), {1.0, 0.0, 0.0, 0.0});
+            m.cstart.push_back(s->clump_pebble_start[i] + sbase);
+            m.cend.push_back(s->clump_pebble_end[i] + sbase);
+        }
+        put(m.cinvI, s->clump_inv_inertia, nc, 6, 0.0);
+        put(m.cinvm, s->clump_inv_mass, nc, 1, 0.0);
+        put(m.wpt, s->wall_point, nw, 3, 0.0);
+        put(m.wn, s->wall_normal, nw, 3, 0.0);
+        put(m.wmat, s->wall_material, nw, 1, 0);
+        m.grids.push_back(scene_grid(s, cell0));
+        require((long long)cell0 + m.grids.back().ncell < (1LL << 31) - 1, "batch has too many cells");
+        cell0 += m.grids.back().ncell;
+        m.foff.push_back(m.foff.back() + (int)nf);
+        m.soff.push_back(m.soff.back() + (int)ns);
+        m.coff.push_back(m.coff.back() + (int)nc);
+        m.woff.push_back(m.woff.back() + (int)nw);
+    }
+    dsph_scene& f = m.flat;
+    f = sc[0];
+    f.n_fluid = m.foff.back();
+    f.fluid_pos = m.fpos.data();
+    f.fluid_vel = m.fvel.data();
+    f.fluid_h = m.fh.data();
+    f.fluid_rho0 = m.frho0.data();
+    f.fluid_drho = m.fdrho.data();
+    f.fluid_c = m.fc.data();
+    f.fluid_nu = m.fnu.data();
+    f.fluid_alpha = m.falpha.data();
+    f.n_solid = m.soff.back();
+    f.solid_pos = m.spos.data();
+    f.solid_vel = m.svel.data();
+    f.solid_angvel = m.sw.data();
+    f.solid_normal = m.any_normal ? m.snrm.data() : nullptr;
+    f.solid_radius = m.srad.data();
+    f.solid_eff_radius = m.seff.data();
+    f.solid_inv_mass = m.sinvm.data();
+    f.solid_material = m.smat.data();
+    f.solid_cluster = m.sclus.data();
+    f.solid_clump = m.sclump.data();
+    f.solid_dummy = m.sdum.data();
+    f.n_clump = m.coff.back();
+    f.clump_centroid = m.cpos.data();
+    f.clump_vel = m.cvel.data();
+    f.clump_angvel = m.cw.data();
+    f.clump_orientation = m.cq.data();
+    f.clump_inv_inertia = m.cinvI.data();
+    f.clump_inv_mass = m.cinvm.data();
+    f.clump_pebble_start = m.cstart.data();
+    f.clump_pebble_end = m.cend.data();
+    f.n_wall = m.woff.back();
+    f.wall_point = m.wpt.data();
+    f.wall_normal = m.wn.data();
+    f.wall_material = m.wmat.data();
+}
+}  // namespace
+
 extern "C" {
 
 const char* dsph_version(void) { return "dsph-mi355x 0.1 (gfx950)"; }
@@ -755,38 +1004,16 @@ int dsph_set_gravity(dsph_ctx* c, const double g[3]) {
     return DSPH_OK;
 }
 
-int dsph_upload_scene(dsph_ctx* c, const dsph_scene* s) {
-    if (!c || !s) return DSPH_E_INVALID;
+int dsph_upload_scene(dsph_ctx* c, const dsph_scene* s) { return dsph_upload_batch(c, s, 1); }
+
+int dsph_upload_batch(dsph_ctx* c, const dsph_scene* scenes, int32_t n_scenes) {
+    if (!c || !scenes || n_scenes < 1) return DSPH_E_INVALID;
     try {
         HIPCHK(hipSetDevice(c->dev));
-        require(s->n_fluid >= 0 && s->n_solid >= 0 && s->n_clump >= 0 && s->n_wall >= 0,
-                "negative particle count");
-        require(s->n_fluid == 0 || (s->fluid_pos && s->fluid_h && s->fluid_rho0 && s->fluid_c && s->fluid_nu),
-                "fluid arrays missing");
-        require(s->n_solid == 0 || (s->solid_pos && s->solid_radius && s->solid_eff_radius && s->solid_inv_mass),
-                "solid arrays missing");
-        require(s->n_clump == 0 || (s->clump_centroid && s->clump_inv_inertia && s->clump_inv_mass &&
-                                    s->clump_pebble_start && s->clump_pebble_end),
-                "clump arrays missing");
-        require(s->n_wall == 0 || (s->wall_point && s->wall_normal), "wall arrays missing");
-        require(s->n_material >= 1, "n_material must be >= 1");
-        for (int k = 0; k < 3; k++) require(s->domain_size[k] > 0, "domain size must be positive");
+        Merged M;
+        merge_scenes(scenes, n_scenes, M);
+        const dsph_scene* s = &M.flat;
         const int nmat = s->n_material;
-        for (int i = 0; i < s->n_solid; i++) {
-            int m = s->solid_material ? s->solid_material[i] : 0;
-            int d = s->solid_dummy ? s->solid_dummy[i] : 0;
-            require(d == 1 || (m >= 0 && m < nmat),
-                    "solid " + std::to_string(i) + " has material outside [0, n_material)");
-        }
-        for (int i = 0; i < s->n_wall; i++) {
-            int m = s->wall_material ? s->wall_material[i] : 0;
-            require(m >= 0 && m < nmat, "wall material outside [0, n_material)");
-        }
-        for (int k = 0; k < s->n_clump; k++)
-            require(s->clump_pebble_start[k] >= 0 && s->clump_pebble_end[k] <= s->n_solid &&
-                        s->clump_pebble_start[k] <= s->clump_pebble_end[k],
-                    "clump pebble range out of bounds");
-        for (int i = 0; i < s->n_fluid; i++) require(s->fluid_h[i] > 0, "fluid smoothing length must be > 0");
 
         invalidate_graphs(c);
         free_all(c);
@@ -806,35 +1033,28 @@ int dsph_upload_scene(dsph_ctx* c, const dsph_scene* s) {
         c->gv[1] = s->gravity[1];
         c->gv[2] = s->gravity[2];
 
-        // ---- grid: setSpatialGrids (dataBase.cpp:415-432)
-        double mf = 0, ms = 0;
-        for (int i = 0; i < nf; i++) mf = std::max(mf, s->fluid_h[i]);
-        for (int i = 0; i < ns; i++) ms = std::max(ms, s->solid_eff_radius[i]);
-        c->hmax = mf;
-        const double cs = 2 * std::max(mf, ms);
-        GridP& g = c->g;
-        g.minx = s->domain_origin[0];
-        g.miny = s->domain_origin[1];
-        g.minz = s->domain_origin[2];
-        g.maxx = s->domain_origin[0] + s->domain_size[0];
-        g.maxy = s->domain_origin[1] + s->domain_size[1];
-        g.maxz = s->domain_origin[2] + s->domain_size[2];
-        g.gx = s->domain_size[0] > cs ? int(s->domain_size[0] / cs) : 1;
-        g.gy = s->domain_size[1] > cs ? int(s->domain_size[1] / cs) : 1;
-        g.gz = s->domain_size[2] > cs ? int(s->domain_size[2] / cs) : 1;
-        g.csx = s->domain_size[0] / double(g.gx);
-        g.csy = s->domain_size[1] / double(g.gy);
-        g.csz = s->domain_size[2] / double(g.gz);
-        const long long ncell = (long long)g.gx * g.gy * g.gz + 1;
-        require(ncell < (1LL << 31) - 1, "grid has too many cells");
-        g.ncell = (int)ncell;
-        g.collide = g.gx < 3 || g.gy < 3;
-        superset_stencil(c);
-        double ext = std::max({std::fabs(g.minx), std::fabs(g.maxx), std::fabs(g.miny), std::fabs(g.maxy),
-                               std::fabs(g.minz), std::fabs(g.maxz)});
-        g.pad = 1e-9 * (1.0 + ext);
+        // ---- grids: one per scene, setSpatialGrids (dataBase.cpp:415-432)
+        c->nscene = n_scenes;
+        c->grids = M.grids;
+        c->ncell = M.grids.back().cell0 + M.grids.back().ncell;
+        c->foff = M.foff;
+        c->soff = M.soff;
+        c->coff = M.coff;
+        c->woff = M.woff;
+        c->h_fscene = M.fscene;
+        c->h_sscene = M.sscene;
+        c->hmax = 0;
+        for (const GridP& gg : c->grids) c->hmax = std::max(c->hmax, gg.hmax);
+        c->dgrids = dalloc<GridP>(c, n_scenes);
+        superset_stencil(c);  // also copies the grids to the device
         c->key_bits = 1;
-        while ((1LL << c->key_bits) < ncell) c->key_bits++;
+        while ((1LL << c->key_bits) < c->ncell) c->key_bits++;
+        c->sscene = dalloc<int>(c, std::max(ns, 1));
+        if (ns) h2d(c, c->sscene, M.sscene.data(), ns);
+        c->dwoff = dalloc<int>(c, n_scenes + 1);
+        h2d(c, c->dwoff, M.woff.data(), n_scenes + 1);
+        c->nwmax = 0;
+        for (int b = 0; b < n_scenes; b++) c->nwmax = std::max(c->nwmax, M.woff[b + 1] - M.woff[b]);
 
         // ---- fluid (two sets for the sorted-order ping-pong)
         const bool mx = c->mixed();
@@ -867,15 +1087,24 @@ int dsph_upload_scene(dsph_ctx* c, const dsph_scene* s) {
             f.sumw0 = dalloc<double>(c, nf);
             f.acc = dalloc<double4>(c, nf);
             f.orig = dalloc<int>(c, nf);
+            f.scene = dalloc<int>(c, std::max(nf, 1));
             const size_t rec = mx ? sizeof(float4) : sizeof(double4);
             for (int b = 0; b < 2; b++) {
                 f.pv[b] = dalloc<char>(c, rec * std::max(nf, 1));
                 f.pq[b] = dalloc<char>(c, rec * std::max(nf, 1));
                 f.p0[b] = mx ? dalloc<char>(c, sizeof(uint4) * std::max(nf, 1)) : nullptr;
             }
-            // fixed-point span of the mixed records: the domain plus one
-            // domain length on each side of every axis
-            const double lo[3] = {g.minx, g.miny, g.minz}, hi[3] = {g.maxx, g.maxy, g.maxz};
+            // fixed-point span of the mixed records: the domains of all
+            // scenes plus one such extent on each side of every axis
+            double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+            for (const GridP& gg : c->grids) {
+                lo[0] = std::min(lo[0], gg.minx);
+                lo[1] = std::min(lo[1], gg.miny);
+                lo[2] = std::min(lo[2], gg.minz);
+                hi[0] = std::max(hi[0], gg.maxx);
+                hi[1] = std::max(hi[1], gg.maxy);
+                hi[2] = std::max(hi[2], gg.maxz);
+            }
             for (int a = 0; a < 3; a++) {
                 const double L = hi[a] - lo[a];
                 const double unit = 3.0 * L / 4294967296.0;
@@ -897,6 +1126,7 @@ int dsph_upload_scene(dsph_ctx* c, const dsph_scene* s) {
             h2d(c, f.alpha, alpha.data(), nf);
             h2d(c, f.acc, acc.data(), nf);
             h2d(c, f.orig, orig.data(), nf);
+            if (nf) h2d(c, f.scene, M.fscene.data(), nf);
         }
         c->fslot = dalloc<int>(c, nf);
         h2d(c, c->fslot, orig.data(), nf);
@@ -905,20 +1135,28 @@ int dsph_upload_scene(dsph_ctx* c, const dsph_scene* s) {
         c->frank = dalloc<unsigned>(c, nf);
         c->fkey_sorted = dalloc<unsigned>(c, nf);
         {
-            std::vector<unsigned> rank, inv;
-            morton_cell_ranks(g, rank, inv);
-            c->crank = dalloc<unsigned>(c, g.ncell);
-            c->cinv = dalloc<unsigned>(c, g.ncell);
-            h2d(c, c->crank, rank.data(), g.ncell);
-            h2d(c, c->cinv, inv.data(), g.ncell);
+            // each scene's cells ranked within its own range of indices
+            std::vector<unsigned> rank(c->ncell), inv(c->ncell);
+            for (const GridP& gg : c->grids) {
+                std::vector<unsigned> r1, i1;
+                morton_cell_ranks(gg, r1, i1);
+                for (int h = 0; h < gg.ncell; h++) {
+                    rank[gg.cell0 + h] = gg.cell0 + r1[h];
+                    inv[gg.cell0 + h] = gg.cell0 + i1[h];
+                }
+            }
+            c->crank = dalloc<unsigned>(c, c->ncell);
+            c->cinv = dalloc<unsigned>(c, c->ncell);
+            h2d(c, c->crank, rank.data(), c->ncell);
+            h2d(c, c->cinv, inv.data(), c->ncell);
         }
         c->fperm = dalloc<int>(c, nf);
-        c->fcs = dalloc<int>(c, g.ncell);
-        c->fce = dalloc<int>(c, g.ncell);
-        c->fblk_nb = dalloc<int>(c, g.ncell);
-        c->fblk_off = dalloc<int>(c, (size_t)g.ncell + 1);
+        c->fcs = dalloc<int>(c, c->ncell);
+        c->fce = dalloc<int>(c, c->ncell);
+        c->fblk_nb = dalloc<int>(c, c->ncell);
+        c->fblk_off = dalloc<int>(c, (size_t)c->ncell + 1);
         c->fblocks = dalloc<int2>(c, std::max(nf, 1));
-        fill_int(c->st, c->fblk_off, (int64_t)g.ncell + 1, 0);
+        fill_int(c->st, c->fblk_off, (int64_t)c->ncell + 1, 0);
         c->hslot = dalloc<unsigned>(c, nf);
         c->fxb = dalloc<double4>(c, nf);
         h2d(c, c->fxb, ph.data(), nf);
@@ -926,8 +1164,8 @@ int dsph_upload_scene(dsph_ctx* c, const dsph_scene* s) {
         h2d(c, c->fsp, ph.data(), nf);
         c->ctl = dalloc<unsigned long long>(c, CTL_WORDS);
         HIPCHK(hipMemsetAsync(c->ctl, 0, CTL_WORDS * sizeof(unsigned long long), c->st));
-        c->scs = dalloc<int>(c, g.ncell);
-        c->sce = dalloc<int>(c, g.ncell);
+        c->scs = dalloc<int>(c, c->ncell);
+        c->sce = dalloc<int>(c, c->ncell);
 
         // ---- solids
         std::vector<double4> spr(ns), svel(ns), sw(ns), snrm(ns), zero4(std::max({ns, nc, 1}), make_double4(0, 0, 0, 0));
@@ -995,8 +1233,8 @@ int dsph_upload_scene(dsph_ctx* c, const dsph_scene* s) {
         c->akey = dalloc<unsigned>(c, active.size());
         c->akey_sorted = dalloc<unsigned>(c, active.size());
         c->aperm = dalloc<int>(c, active.size());
-        c->acs = dalloc<int>(c, g.ncell);
-        c->ace = dalloc<int>(c, g.ncell);
+        c->acs = dalloc<int>(c, c->ncell);
+        c->ace = dalloc<int>(c, c->ncell);
 
         // ---- clumps
         std::vector<double4> cpos(nc), cvel(nc), cw(nc), cq(nc);
@@ -1053,7 +1291,7 @@ int dsph_upload_scene(dsph_ctx* c, const dsph_scene* s) {
         h2d(c, c->wpt, wp.data(), nw);
         h2d(c, c->wn, wnv.data(), nw);
         h2d(c, c->wmat, wm.data(), nw);
-        const size_t nh = 9 * (size_t)ns * nw;
+        const size_t nh = 9 * (size_t)ns * c->nwmax;
         c->whist = dalloc<double>(c, nh);
         if (nh) HIPCHK(hipMemsetAsync(c->whist, 0, nh * sizeof(double), c->st));
 
@@ -1110,7 +1348,7 @@ int dsph_upload_scene(dsph_ctx* c, const dsph_scene* s) {
         // ---- scratch
         const int64_t nmax = std::max<int64_t>(std::max(nf, ns), 1);
         c->radix_tmp = dalloc<unsigned>(c, radix_tmp_size(nmax));
-        int64_t sc = std::max<int64_t>(scan_tmp_size(std::max<int64_t>(nmax, g.ncell) + 1),
+        int64_t sc = std::max<int64_t>(scan_tmp_size(std::max<int64_t>(nmax, c->ncell) + 1),
                                        scan_tmp_size(256 * ((nmax + 1023) / 1024)));
         c->scan_tmp = dalloc<int>(c, sc + 64);
         c->status = dalloc<unsigned long long>(c, ST_WORDS);
@@ -1211,7 +1449,7 @@ int dsph_initialize(dsph_ctx* c, int compute_normals) {
             h2d(c, c->B.Tt, z.data(), nb);
         }
         if (compute_normals)
-            solid_normals(c->st, c->g, c->so, c->ns, c->scs, c->sce, c->sperm, c->hmax);
+            solid_normals(c->st, grids_view(c), c->so, c->ns, c->scs, c->sce, c->sperm);
         HIPCHK(hipStreamSynchronize(c->st));
         invalidate_graphs(c);
         c->initialized = true;
@@ -1619,14 +1857,15 @@ int dsph_download_list(dsph_ctx* c, int kind, int32_t* pointed, int32_t* pointin
                 const int s2 = fh.slot_of[i];
                 if (cn[s2] < 0) throw ApiError{DSPH_E_CAPACITY, "fluid-fluid list overflowed"};
                 int H[27];
-                stencil_hashes(c->g, fh.ph[s2].x, fh.ph[s2].y, fh.ph[s2].z, H);
+                const GridP& g = c->grids[c->h_fscene[i]];
+                stencil_hashes(g, fh.ph[s2].x, fh.ph[s2].y, fh.ph[s2].z, H);
                 row.clear();
                 for (int k = 0; k < cn[s2]; k++) {
                     const int jo = fh.orig[nbr[(size_t)k * nf + s2]];
                     if (jo > i)  // the reference keeps the j > i half
                         row.push_back({(int)fh.fkey[jo], jo, 0, 0});
                 }
-                order_row(c->g, H, row);
+                order_row(g, H, row);
                 for (auto& e : row) {
                     A.push_back(i);
                     Bv.push_back(e.idx);
@@ -1671,11 +1910,27 @@ int dsph_download_list(dsph_ctx* c, int kind, int32_t* pointed, int32_t* pointin
     }
 }
 
+int dsph_batch_info(dsph_ctx* c, int32_t* n_scenes, int32_t* fluid_offset, int32_t* solid_offset,
+                    int32_t* clump_offset, int32_t* wall_offset, int32_t* cell_offset) {
+    API_GUARD(c);
+    const int k = c->nscene;
+    if (n_scenes) *n_scenes = k;
+    for (int b = 0; b <= k; b++) {
+        if (fluid_offset) fluid_offset[b] = c->foff[b];
+        if (solid_offset) solid_offset[b] = c->soff[b];
+        if (clump_offset) clump_offset[b] = c->coff[b];
+        if (wall_offset) wall_offset[b] = c->woff[b];
+        if (cell_offset) cell_offset[b] = b < k ? c->grids[b].cell0 : c->ncell;
+    }
+    return DSPH_OK;
+}
+
 int dsph_grid_info(dsph_ctx* c, int32_t* gs, double* cs, int32_t* n_cells) {
     API_GUARD(c);
-    if (gs) { gs[0] = c->g.gx; gs[1] = c->g.gy; gs[2] = c->g.gz; }
-    if (cs) { cs[0] = c->g.csx; cs[1] = c->g.csy; cs[2] = c->g.csz; }
-    if (n_cells) *n_cells = c->g.ncell;
+    const GridP& g = c->grids[0];  // a batch: the first scene (dsph_batch_info)
+    if (gs) { gs[0] = g.gx; gs[1] = g.gy; gs[2] = g.gz; }
+    if (cs) { cs[0] = g.csx; cs[1] = g.csy; cs[2] = g.csz; }
+    if (n_cells) *n_cells = g.ncell;
     return DSPH_OK;
 }
 
@@ -1688,8 +1943,8 @@ int dsph_download_grid(dsph_ctx* c, int species, int32_t* hash, int32_t* perm, i
         if (species == 1) {
             if (hash) d2h(c, (unsigned*)hash, c->skey, n);
             if (perm) d2h(c, perm, c->sperm, n);
-            if (start) d2h(c, start, c->scs, c->g.ncell);
-            if (end) d2h(c, end, c->sce, c->g.ncell);
+            if (start) d2h(c, start, c->scs, c->ncell);
+            if (end) d2h(c, end, c->sce, c->ncell);
             HIPCHK(hipStreamSynchronize(c->st));
             return DSPH_OK;
         }
@@ -1698,16 +1953,16 @@ int dsph_download_grid(dsph_ctx* c, int species, int32_t* hash, int32_t* perm, i
         std::vector<unsigned> key(n);
         d2h(c, key.data(), c->fkey, n);
         HIPCHK(hipStreamSynchronize(c->st));
-        std::vector<int> cnt((size_t)c->g.ncell + 1, 0), p(n);
+        std::vector<int> cnt((size_t)c->ncell + 1, 0), p(n);
         for (int i = 0; i < n; i++) cnt[key[i] + 1]++;
-        for (int h = 0; h < c->g.ncell; h++) cnt[h + 1] += cnt[h];
+        for (int h = 0; h < c->ncell; h++) cnt[h + 1] += cnt[h];
         for (int i = 0; i < n; i++) p[cnt[key[i]]++] = i;
         if (hash)
             for (int i = 0; i < n; i++) hash[i] = (int32_t)key[i];
         if (perm)
             for (int i = 0; i < n; i++) perm[i] = p[i];
-        if (start) std::fill(start, start + c->g.ncell, -1);
-        if (end) std::fill(end, end + c->g.ncell, -1);
+        if (start) std::fill(start, start + c->ncell, -1);
+        if (end) std::fill(end, end + c->ncell, -1);
         for (int t = 0; t < n; t++) {
             const unsigned k = key[p[t]];
             if (start && (t == 0 || key[p[t - 1]] != k)) start[k] = t;

@@ -52,17 +52,43 @@ enum {
     CTL_WORDS = 8
 };
 
-// the reference's spatialGrid (deviceStructs.h:976-1014)
+// the reference's spatialGrid (deviceStructs.h:976-1014) of one scene.  A
+// batch of independent scenes keeps one grid per scene; its cells occupy the
+// global cell indices [cell0, cell0 + ncell), the spare cell last.  Hashes
+// stored on the device are global (cell0 + the scene's own hash); stencils
+// and spare tests are evaluated in the scene's own cell coordinates, so each
+// scene's lists are those of the scene run alone.
 struct GridP {
     double minx, miny, minz;
     double maxx, maxy, maxz;
     double csx, csy, csz;
     int gx, gy, gz, ncell;
+    int cell0;      // first global cell index of this scene
     int collide;    // gx < 3 or gy < 3: some of the 27 linear stencil offsets coincide
     int sw;         // half-width in cells of the superset build stencil
     int walk_only;  // the superset stencil's offsets are not distinct: walk every row
     double pad;     // conservative enlargement of cell boxes for pruning
+    double hmax;    // largest fluid smoothing length of the scene
+    double skin;    // Verlet skin of the scene's fluid superset (absolute)
 };
+
+// the grids of all scenes and the scene of every particle
+struct Grids {
+    const GridP* g = nullptr;       // per scene, device
+    const int* fscene = nullptr;    // fluid slot -> scene (slot order)
+    const int* sscene = nullptr;    // solid -> scene
+    int nscene = 1;
+    GridP g0{};                     // scene 0 by value (the one-scene case)
+};
+__device__ __forceinline__ const GridP& grid_at(const Grids& G, int scene) {
+    return G.nscene == 1 ? G.g0 : G.g[scene];
+}
+__device__ __forceinline__ int fluid_scene(const Grids& G, int slot) {
+    return G.nscene == 1 ? 0 : G.fscene[slot];
+}
+__device__ __forceinline__ int solid_scene(const Grids& G, int i) {
+    return G.nscene == 1 ? 0 : G.sscene[i];
+}
 
 struct FluidSet {
     double4* ph = nullptr;              // x, y, z, h
@@ -71,6 +97,7 @@ struct FluidSet {
     double2* cn = nullptr;              // c, nu
     double* sumw0 = nullptr;
     double* alpha = nullptr;            // artificial viscosity coefficient
+    int* scene = nullptr;               // scene of each slot (batches)
     double4* acc = nullptr;
     int* orig = nullptr;                // slot -> caller index
     // neighbour records, ping-ponged with drho so the Jacobi update never
@@ -198,7 +225,8 @@ void cell_spans(hipStream_t s, const unsigned* sorted, int64_t n, int* start, in
 
 // search (k_search.hip)
 struct SearchArgs {
-    GridP g;
+    Grids G;
+    int ncell;              // cells of all scenes
     int nf, ns;
     const double4* ph;      // fluid position + h, slot order
     const int* fcs;         // fluid cell ranges over sorted positions
@@ -212,17 +240,18 @@ struct SearchArgs {
     const unsigned* skey;   // reference hash of each solid
     const double4* spr;
     double4* fsp;           // fluid positions of the last fluid search, slot order
-    double hmax, skin;
     Lists L;
     unsigned long long* ctl;
     unsigned long long* st;
 };
-void hash_fluid(hipStream_t s, const GridP& g, const double4* ph, const int* orig, int n,
+void hash_fluid(hipStream_t s, const Grids& G, const double4* ph, const int* orig, int n,
                 const unsigned* crank, unsigned* key_orig, unsigned* rank_orig, long long* status);
-void hash_solid(hipStream_t s, const GridP& g, const double4* pr, int n, unsigned* key,
+void hash_solid(hipStream_t s, const Grids& G, const double4* pr, int n, unsigned* key,
                 long long* status);
-void rebuild_decision(hipStream_t s, const double4* fph, const double4* fxb, int nf,
-                      const double4* spr, const double4* sxb, int ns, double skin,
+// displacements since the last build in units of each particle's scene skin;
+// always != 0 rebuilds at every search
+void rebuild_decision(hipStream_t s, const Grids& G, const double4* fph, const double4* fxb, int nf,
+                      const double4* spr, const double4* sxb, int ns, int always,
                       unsigned long long* ctl, long long* status);
 void reorder_fluid(hipStream_t s, const FluidSet& f, const FluidSet& scratch, const int* perm,
                    int* slot_of, int n, bool mixed, int cur, const unsigned long long* ctl);
@@ -241,11 +270,12 @@ void solid_save_history(hipStream_t s, Lists& L, int ns);
 void active_grid(hipStream_t s, const Solids& so, const unsigned* skey, int ncell, int key_bits,
                  unsigned* akey, unsigned* akey_sorted, int* aperm, int* acs, int* ace,
                  unsigned* radix_tmp);
-void solid_search(hipStream_t s, const GridP& g, const Solids& so, int ns, const int* acs,
+void solid_search(hipStream_t s, const Grids& G, const Solids& so, int ns, const int* acs,
                   const int* ace, const int* aperm, Lists& L, int* scan_tmp, long long* status);
 void reverse_solid(hipStream_t s, Lists& L, int ns, int* scan_tmp);
-void solid_normals(hipStream_t s, const GridP& g, const Solids& so, int ns, const int* scs,
-                   const int* sce, const int* sperm, double h);
+// wall normals with the smoothing length of each solid's scene (grid hmax)
+void solid_normals(hipStream_t s, const Grids& G, const Solids& so, int ns, const int* scs,
+                   const int* sce, const int* sperm);
 
 // fluid (k_fluid.hip)
 struct FluidArgs {
@@ -277,7 +307,13 @@ struct DemArgs {
     const double4* wn;
     const int* wmat;
     int nw;
-    double* whist;  // 9 * ns * nw
+    // walls of scene k are [woff[k], woff[k+1]); a solid meets only its
+    // scene's walls and keeps 9 history doubles per wall of its scene
+    const int* woff;
+    const int* sscene;
+    int nscene;
+    int nwmax;      // most walls of any scene
+    double* whist;  // 9 * ns * nwmax
     ContactTables tb;
     double gx, gy, gz;
     uint32_t flags;

@@ -161,8 +161,11 @@ __global__ void k_walls(DemArgs a, double dt) {
     const double4 pr = a.so.pr[i];
     const d3 pi = v3(pr), vi = v3(v4), wi = v3(a.so.w[i]);
     d3 acc = v3(a.so.acc[i]), tq = v3(a.so.tq[i]);
-    for (int k = 0; k < a.nw; k++) {
-        double* h = a.whist + 9 * ((int64_t)i * a.nw + k);
+    // the walls of the solid's own scene, in their upload order
+    const int sc = a.nscene == 1 ? 0 : a.sscene[i];
+    const int k0 = a.nscene == 1 ? 0 : a.woff[sc], k1 = a.nscene == 1 ? a.nw : a.woff[sc + 1];
+    for (int k = k0; k < k1; k++) {
+        double* h = a.whist + 9 * ((int64_t)i * a.nwmax + (k - k0));
         const d3 n = v3(a.wn[k]);
         const double delta = pr.w - dot(pi - v3(a.wpt[k]), n);
         if (delta < 0.) {

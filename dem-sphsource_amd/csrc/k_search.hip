@@ -121,32 +121,35 @@ __global__ void k_status_reset(unsigned long long* st) {
     if (i == ST_OUT_OF_DOMAIN || i == ST_FF_MAX || i == ST_FS_TOTAL || i == ST_SS_TOTAL) st[i] = 0;
 }
 
-__global__ void k_hash_fluid(GridP g, const double4* __restrict__ ph, const int* __restrict__ orig,
+__global__ void k_hash_fluid(Grids G, const double4* __restrict__ ph, const int* __restrict__ orig,
                              int n, const unsigned* __restrict__ crank, unsigned* __restrict__ key,
                              unsigned* __restrict__ rank, unsigned long long* st) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long out = 0;
     if (s < n) {
+        const GridP& g = grid_at(G, fluid_scene(G, s));
         double4 p = ph[s];
-        int h = point_hash(g, p.x, p.y, p.z);
+        const int hl = point_hash(g, p.x, p.y, p.z);
+        const int h = g.cell0 + hl;
         const int o = orig[s];
         key[o] = (unsigned)h;
         rank[o] = crank[h];
-        out = h == g.ncell - 1;
+        out = hl == g.ncell - 1;
     }
     unsigned long long m = __ballot(out != 0);
     if ((threadIdx.x & 63) == 0 && m) atomicAdd(&st[ST_OUT_OF_DOMAIN], (unsigned long long)__popcll(m));
 }
 
-__global__ void k_hash_solid(GridP g, const double4* __restrict__ pr, int n,
+__global__ void k_hash_solid(Grids G, const double4* __restrict__ pr, int n,
                              unsigned* __restrict__ key, unsigned long long* st) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long out = 0;
     if (s < n) {
+        const GridP& g = grid_at(G, solid_scene(G, s));
         double4 p = pr[s];
-        int h = point_hash(g, p.x, p.y, p.z);
-        key[s] = (unsigned)h;
-        out = h == g.ncell - 1;
+        const int hl = point_hash(g, p.x, p.y, p.z);
+        key[s] = (unsigned)(g.cell0 + hl);
+        out = hl == g.ncell - 1;
     }
     unsigned long long m = __ballot(out != 0);
     if ((threadIdx.x & 63) == 0 && m) atomicAdd(&st[ST_OUT_OF_DOMAIN], (unsigned long long)__popcll(m));
@@ -161,7 +164,8 @@ __device__ __forceinline__ unsigned long long dbits(double d) {
 // their bit patterns)
 // grid-stride over the particles, one atomic per workgroup (thousands of
 // per-wavefront atomics on one word serialise in L2)
-__global__ void __launch_bounds__(kBS) k_disp(const double4* __restrict__ p,
+// displacement in units of the particle's scene skin
+__global__ void __launch_bounds__(kBS) k_disp(Grids G, bool fluid, const double4* __restrict__ p,
                                               const double4* __restrict__ xb, int n,
                                               unsigned long long* slot) {
     __shared__ unsigned long long wmax[kBS / 64];
@@ -169,7 +173,8 @@ __global__ void __launch_bounds__(kBS) k_disp(const double4* __restrict__ p,
     for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
         const double4 a = p[s], b = xb[s];
         const double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
-        double d = sqrt(dx * dx + dy * dy + dz * dz);
+        const double skin = grid_at(G, fluid ? fluid_scene(G, s) : solid_scene(G, s)).skin;
+        double d = sqrt(dx * dx + dy * dy + dz * dz) / (skin > 0.0 ? skin : 1e-300);
         if (!(d == d)) d = 1e300;  // NaN forces a rebuild
         const unsigned long long v = dbits(d);
         m = v > m ? v : m;
@@ -187,16 +192,18 @@ __global__ void __launch_bounds__(kBS) k_disp(const double4* __restrict__ p,
     }
 }
 
-__global__ void k_decide(unsigned long long* ctl, unsigned long long* st, double skin) {
+// rebuild when two particles may have closed the skin: d_f + max(d_f, d_s)
+// >= skin, with displacements in skin units (each scene its own skin)
+__global__ void k_decide(unsigned long long* ctl, unsigned long long* st, int always) {
     const double df = __longlong_as_double((long long)ctl[CTL_DISP_F]);
     const double ds = __longlong_as_double((long long)ctl[CTL_DISP_S]);
     const bool out = st[ST_OUT_OF_DOMAIN] != 0;
-    const bool move = df + (df > ds ? df : ds) >= skin * (1.0 - 1e-9);
+    const bool move = df + (df > ds ? df : ds) >= 1.0 - 1e-9;
     // out-of-domain particles are missing from a superset built while they
     // are out: rebuild until a build sees every particle inside
     const bool stale = ctl[CTL_OUT_AT_BUILD] != 0;
     const unsigned long long r =
-        (ctl[CTL_FORCE] != 0 || out || stale || skin <= 0.0 || move) ? 1ull : 0ull;
+        (ctl[CTL_FORCE] != 0 || out || stale || always || move) ? 1ull : 0ull;
     ctl[CTL_REBUILD] = r;
     ctl[CTL_WALK_ALL] = out ? 1ull : 0ull;
     if (r) ctl[CTL_OUT_AT_BUILD] = out ? 1ull : 0ull;
@@ -222,6 +229,7 @@ __global__ void k_reorder(FluidSet src, FluidSet dst, const int* __restrict__ pe
     dst.cn[t] = src.cn[old];
     dst.sumw0[t] = src.sumw0[old];
     dst.alpha[t] = src.alpha[old];
+    dst.scene[t] = src.scene[old];
     dst.acc[t] = src.acc[old];
     dst.orig[t] = o;
     ((R4*)dst.pv[cur])[t] = ((const R4*)src.pv[cur])[old];
@@ -242,6 +250,7 @@ __global__ void k_copyback(FluidSet dst, FluidSet src, int n, int cur,
     dst.cn[t] = src.cn[t];
     dst.sumw0[t] = src.sumw0[t];
     dst.alpha[t] = src.alpha[t];
+    dst.scene[t] = src.scene[t];
     dst.acc[t] = src.acc[t];
     dst.orig[t] = src.orig[t];
     ((R4*)dst.pv[cur])[t] = ((const R4*)src.pv[cur])[t];
@@ -313,13 +322,10 @@ __device__ __forceinline__ double readlane_d(double v, int k) {
 __global__ void __launch_bounds__(kBS) k_build(SearchArgs a, const int2* __restrict__ blocks,
                                                const int* __restrict__ nblk_p) {
     if (a.ctl[CTL_REBUILD] == 0) return;
-    const GridP& g = a.g;
     const int lane = threadIdx.x & 63;
     const int wv = xcd_block() * (kBS / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nwv = gridDim.x * (kBS / 64);
     const int nblk = *nblk_p;
-    const int spare = g.ncell - 1;
-    const double skin = a.skin;
     const double grow = (1.0 + 1e-9) * (1.0 + 1e-9);
     const int supK = a.L.supK, fsupK = a.L.fsupK;
     bool ovf = false;
@@ -330,7 +336,13 @@ __global__ void __launch_bounds__(kBS) k_build(SearchArgs a, const int2* __restr
         const int nmem = min(64, a.fce[cell] - t0);
         const bool act = lane < nmem;
         const int s = a.fmember[t0 + (act ? lane : 0)];
-        if (cell == spare) {  // walked exactly while out of the domain
+        // the block's cell, and so its scene, is the same for every lane
+        const int sc = __builtin_amdgcn_readfirstlane(fluid_scene(a.G, a.fmember[t0]));
+        const GridP& g = grid_at(a.G, sc);
+        const int lc = cell - g.cell0;  // the cell in its scene's own indices
+        const int spare = g.ncell - 1;
+        const double skin = g.skin;
+        if (lc == spare) {  // walked exactly while out of the domain
             if (act) {
                 a.L.sup_cnt[s] = 0;
                 a.L.fsup_cnt[s] = 0;
@@ -339,20 +351,21 @@ __global__ void __launch_bounds__(kBS) k_build(SearchArgs a, const int2* __restr
         }
         const double4 p = a.ph[s];
         const double hi = p.w;
-        const double Rf = (2.0 * fmax(hi, a.hmax) + skin) * (1.0 + 1e-9);
+        const double Rf = (2.0 * fmax(hi, g.hmax) + skin) * (1.0 + 1e-9);
         const double Rs = (2.0 * hi + skin) * (1.0 + 1e-9);
         const double Rs2 = (2.0 * hi + skin) * (2.0 * hi + skin) * grow;
         int c = 0, cs = 0;
         const int gxy = g.gx * g.gy;
-        const int cz = cell / gxy, rem = cell - cz * gxy;
+        const int cz = lc / gxy, rem = lc - cz * gxy;
         const int cy = rem / g.gx, cx = rem - cy * g.gx;
         const int w = g.sw;
         for (int zz = -w; zz <= w; zz++)
             for (int yy = -w; yy <= w; yy++)
                 for (int xx = -w; xx <= w; xx++) {
-                    const int h = (cz + zz) * gxy + (cy + yy) * g.gx + (cx + xx);
-                    if (h < 0 || h >= spare) continue;
-                    const double bd2 = box_dist2(g, h, p.x, p.y, p.z);
+                    const int hl = (cz + zz) * gxy + (cy + yy) * g.gx + (cx + xx);
+                    if (hl < 0 || hl >= spare) continue;
+                    const int h = g.cell0 + hl;
+                    const double bd2 = box_dist2(g, hl, p.x, p.y, p.z);
                     if (__ballot(act && bd2 <= Rf * Rf) != 0) {
                         const int st0 = __builtin_amdgcn_readfirstlane(a.fcs[h]);
                         const int en = __builtin_amdgcn_readfirstlane(a.fce[h]);
@@ -439,9 +452,10 @@ __device__ __forceinline__ int stencil_count(const GridP& g, int ax, int ay, int
 // stencils reach the spare cell.  A cell reached by several offsets (grids
 // narrower than three cells in x or y) is counted once per offset.
 __device__ void ff_walk(const SearchArgs& a, const double4 p, int s, int& c) {
-    const GridP& g = a.g;
+    const GridP& g = grid_at(a.G, fluid_scene(a.G, s));
+    const int c0 = g.cell0;  // cells below are the scene's own indices
     const int spare = g.ncell - 1;
-    const int hs = (int)a.hslot[s];
+    const int hs = (int)a.hslot[s] - c0;
     const int oi = a.forig[s];
     const double hi = p.w;
     int ax, ay, az;
@@ -467,11 +481,11 @@ __device__ void ff_walk(const SearchArgs& a, const double4 p, int s, int& c) {
                     push(spare - (zz * g.gy * g.gx + yy * g.gx + xx), true);
     for (int k = 0; k < nc; k++) {
         const int h = cand[k];
-        const int st0 = a.fcs[h];
+        const int st0 = a.fcs[c0 + h];
         if (st0 < 0) continue;
         const int mh = stencil_count(g, ax, ay, az, h);  // offsets of i's stencil reaching h
         const bool sym = h != spare && hs != spare;
-        for (int t = st0; t < a.fce[h]; t++) {
+        for (int t = st0; t < a.fce[c0 + h]; t++) {
             const int j = a.fmember[t];
             if (j == s) continue;
             const double4 q = a.ph[j];
@@ -496,7 +510,7 @@ __device__ void ff_walk(const SearchArgs& a, const double4 p, int s, int& c) {
 template <bool FILL>
 __device__ void lane_walk(const SearchArgs& a, const double4 p, int s, int& c, int& cs,
                           int64_t& w) {
-    const GridP& g = a.g;
+    const GridP& g = grid_at(a.G, fluid_scene(a.G, s));
     const double hi = p.w;
     if (!FILL) ff_walk(a, p, s, c);
     int ax, ay, az;
@@ -506,9 +520,9 @@ __device__ void lane_walk(const SearchArgs& a, const double4 p, int s, int& c, i
             for (int xx = -1; xx <= 1; xx++) {
                 const int h = (az + zz) * g.gy * g.gx + (ay + yy) * g.gx + (ax + xx);
                 if (h < 0 || h >= g.ncell) continue;
-                const int st1 = a.scs[h];
+                const int st1 = a.scs[g.cell0 + h];
                 if (st1 < 0) continue;
-                for (int t = st1; t < a.sce[h]; t++) {
+                for (int t = st1; t < a.sce[g.cell0 + h]; t++) {
                     const int ai = a.sperm[t];
                     const double4 q = a.spr[ai];
                     if (!within(p.x, p.y, p.z, q.x, q.y, q.z, 2.0 * hi)) continue;
@@ -537,9 +551,10 @@ __global__ void __launch_bounds__(kBS) k_filter(SearchArgs a) {
     int c = 0, cs = 0;
     if (s < a.nf) {
         const double4 p = a.ph[s];
-        const unsigned spare = (unsigned)(a.g.ncell - 1);
-        const bool walk = a.ctl[CTL_WALK_ALL] != 0 || a.g.walk_only || a.hslot[s] == spare ||
-                          stencil_has_spare(a.g, p.x, p.y, p.z);
+        const GridP& g = grid_at(a.G, fluid_scene(a.G, s));
+        const unsigned spare = (unsigned)(g.cell0 + g.ncell - 1);  // the scene's spare cell
+        const bool walk = a.ctl[CTL_WALK_ALL] != 0 || g.walk_only || a.hslot[s] == spare ||
+                          stencil_has_spare(g, p.x, p.y, p.z);
         int64_t w = FILL ? (int64_t)a.L.fsoff[s] : 0;
         if (walk) {
             lane_walk<FILL>(a, p, s, c, cs, w);
@@ -687,7 +702,7 @@ __global__ void k_active_keys(const int* __restrict__ active, int n, const unsig
 // (stencil order, ascending index within a cell).
 template <bool FILL>
 __global__ void __launch_bounds__(kBS) k_ss_search(
-    GridP g, Solids so, int ns, const int* __restrict__ acs, const int* __restrict__ ace,
+    Grids G, Solids so, int ns, const int* __restrict__ acs, const int* __restrict__ ace,
     const int* __restrict__ aperm, int* __restrict__ sscnt, const int* __restrict__ ssoff,
     int64_t cap, int* __restrict__ ss_i, int* __restrict__ ss_j, double4* __restrict__ ssF,
     double4* __restrict__ ssT, double* __restrict__ ssS, const int* __restrict__ hoff,
@@ -705,6 +720,7 @@ __global__ void __launch_bounds__(kBS) k_ss_search(
     const double ra = so.eff[a];
     const int ca = so.clump[a];
     const bool fixa = so.vel[a].w < 1.e-10;
+    const GridP& g = grid_at(G, solid_scene(G, a));
     int ax, ay, az;
     grid_pos(g, p.x, p.y, p.z, ax, ay, az);
     int c = 0;
@@ -717,8 +733,9 @@ __global__ void __launch_bounds__(kBS) k_ss_search(
     for (int zz = -1; zz <= 1; zz++)
         for (int yy = -1; yy <= 1; yy++)
             for (int xx = -1; xx <= 1; xx++) {
-                const int h = (az + zz) * g.gy * g.gx + (ay + yy) * g.gx + (ax + xx);
-                if (h < 0 || h >= g.ncell) continue;
+                const int hl = (az + zz) * g.gy * g.gx + (ay + yy) * g.gx + (ax + xx);
+                if (hl < 0 || hl >= g.ncell) continue;
+                const int h = g.cell0 + hl;
                 const int st0 = acs[h];
                 if (st0 < 0) continue;
                 const int en = ace[h];
@@ -756,10 +773,12 @@ __global__ void __launch_bounds__(kBS) k_ss_search(
 
 // setSolidNormal (dataBase.cpp:348-413): n = normalize(-sum V_B gradW) over
 // fixed–fixed or same-clump neighbours within 2h
-__global__ void k_normals(GridP g, Solids so, int ns, const int* __restrict__ scs,
-                          const int* __restrict__ sce, const int* __restrict__ sperm, double h) {
+__global__ void k_normals(Grids G, Solids so, int ns, const int* __restrict__ scs,
+                          const int* __restrict__ sce, const int* __restrict__ sperm) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= ns) return;
+    const GridP& g = grid_at(G, solid_scene(G, a));
+    const double h = g.hmax;  // the scene's largest fluid smoothing length
     const double4 p = so.pr[a];
     const int ca = so.clump[a];
     const double ima = so.vel[a].w;
@@ -771,8 +790,9 @@ __global__ void k_normals(GridP g, Solids so, int ns, const int* __restrict__ sc
     for (int zz = -1; zz <= 1; zz++)
         for (int yy = -1; yy <= 1; yy++)
             for (int xx = -1; xx <= 1; xx++) {
-                const int hh = (az + zz) * g.gy * g.gx + (ay + yy) * g.gx + (ax + xx);
-                if (hh < 0 || hh >= g.ncell) continue;
+                const int hl = (az + zz) * g.gy * g.gx + (ay + yy) * g.gx + (ax + xx);
+                if (hl < 0 || hl >= g.ncell) continue;
+                const int hh = g.cell0 + hl;
                 const int st0 = scs[hh];
                 if (st0 < 0) continue;
                 const int en = sce[hh];
@@ -806,31 +826,31 @@ int cap_grid(int64_t n, int cap) {
 }  // namespace
 
 // ---------------------------------------------------------------- wrappers
-void hash_fluid(hipStream_t s, const GridP& g, const double4* ph, const int* orig, int n,
+void hash_fluid(hipStream_t s, const Grids& G, const double4* ph, const int* orig, int n,
                 const unsigned* crank, unsigned* key, unsigned* rank, long long* status) {
     hipLaunchKernelGGL(k_status_reset, dim3(1), dim3(ST_WORDS), 0, s, (unsigned long long*)status);
     if (n > 0)
-        hipLaunchKernelGGL(k_hash_fluid, dim3(blocks_for(n)), dim3(kBS), 0, s, g, ph, orig, n, crank,
+        hipLaunchKernelGGL(k_hash_fluid, dim3(blocks_for(n)), dim3(kBS), 0, s, G, ph, orig, n, crank,
                            key, rank, (unsigned long long*)status);
 }
 
-void hash_solid(hipStream_t s, const GridP& g, const double4* pr, int n, unsigned* key,
+void hash_solid(hipStream_t s, const Grids& G, const double4* pr, int n, unsigned* key,
                 long long* status) {
     if (n > 0)
-        hipLaunchKernelGGL(k_hash_solid, dim3(blocks_for(n)), dim3(kBS), 0, s, g, pr, n, key,
+        hipLaunchKernelGGL(k_hash_solid, dim3(blocks_for(n)), dim3(kBS), 0, s, G, pr, n, key,
                            (unsigned long long*)status);
 }
 
-void rebuild_decision(hipStream_t s, const double4* fph, const double4* fxb, int nf,
-                      const double4* spr, const double4* sxb, int ns, double skin,
+void rebuild_decision(hipStream_t s, const Grids& G, const double4* fph, const double4* fxb, int nf,
+                      const double4* spr, const double4* sxb, int ns, int always,
                       unsigned long long* ctl, long long* status) {
     if (nf > 0)
-        hipLaunchKernelGGL(k_disp, dim3(std::min(blocks_for(nf), 1024)), dim3(kBS), 0, s, fph, fxb, nf,
-                           ctl + CTL_DISP_F);
+        hipLaunchKernelGGL(k_disp, dim3(std::min(blocks_for(nf), 1024)), dim3(kBS), 0, s, G, true, fph,
+                           fxb, nf, ctl + CTL_DISP_F);
     if (ns > 0)
-        hipLaunchKernelGGL(k_disp, dim3(std::min(blocks_for(ns), 1024)), dim3(kBS), 0, s, spr, sxb, ns,
-                           ctl + CTL_DISP_S);
-    hipLaunchKernelGGL(k_decide, dim3(1), dim3(1), 0, s, ctl, (unsigned long long*)status, skin);
+        hipLaunchKernelGGL(k_disp, dim3(std::min(blocks_for(ns), 1024)), dim3(kBS), 0, s, G, false, spr,
+                           sxb, ns, ctl + CTL_DISP_S);
+    hipLaunchKernelGGL(k_decide, dim3(1), dim3(1), 0, s, ctl, (unsigned long long*)status, always);
 }
 
 void reorder_fluid(hipStream_t s, const FluidSet& f, const FluidSet& scratch, const int* perm,
@@ -864,7 +884,7 @@ static int wave_grid(int nf) {
 void fluid_superset(hipStream_t s, const SearchArgs& a, const unsigned* cinv, int* nb, int* off,
                     int2* blocks, int* scan_tmp, double4* fxb, double4* sxb) {
     if (a.nf <= 0) return;
-    const int nc = a.g.ncell;
+    const int nc = a.ncell;
     hipLaunchKernelGGL(k_blk_count, dim3(blocks_for(nc)), dim3(kBS), 0, s, a.fcs, a.fce, cinv, nc,
                        nb, (const unsigned long long*)a.ctl);
     exclusive_scan(s, nb, off, nc, scan_tmp);
@@ -933,7 +953,7 @@ void active_grid(hipStream_t s, const Solids& so, const unsigned* skey, int ncel
     cell_spans(s, akey_sorted, n, acs, ace, ncell, nullptr);
 }
 
-void solid_search(hipStream_t s, const GridP& g, const Solids& so, int ns, const int* acs,
+void solid_search(hipStream_t s, const Grids& g, const Solids& so, int ns, const int* acs,
                   const int* ace, const int* aperm, Lists& L, int* scan_tmp, long long* status) {
     if (ns <= 0) return;
     const int nb = blocks_for(so.nactive > 0 ? so.nactive : 1);
@@ -953,11 +973,10 @@ void reverse_solid(hipStream_t s, Lists& L, int ns, int* scan_tmp) {
                   scan_tmp);
 }
 
-void solid_normals(hipStream_t s, const GridP& g, const Solids& so, int ns, const int* scs,
-                   const int* sce, const int* sperm, double h) {
+void solid_normals(hipStream_t s, const Grids& G, const Solids& so, int ns, const int* scs,
+                   const int* sce, const int* sperm) {
     if (ns > 0)
-        hipLaunchKernelGGL(k_normals, dim3(blocks_for(ns)), dim3(kBS), 0, s, g, so, ns, scs, sce,
-                           sperm, h);
+        hipLaunchKernelGGL(k_normals, dim3(blocks_for(ns)), dim3(kBS), 0, s, G, so, ns, scs, sce, sperm);
 }
 
 }  // namespace dsph

@@ -36,6 +36,8 @@ _vp = C.c_void_p
 # name -> argument types after the context pointer (all return int)
 _SIGS = {
     "dsph_upload_scene": [C.POINTER(CScene)],
+    "dsph_upload_batch": [C.POINTER(CScene), C.c_int32],
+    "dsph_batch_info": [_ip] * 6,
     "dsph_set_capacity": [C.c_int32, C.c_int64, C.c_int64],
     "dsph_set_flags": [C.c_uint32],
     "dsph_set_gravity": [_dp],
@@ -144,6 +146,30 @@ class Solver:
         cs = scene.cstruct()
         self._call("dsph_upload_scene", C.byref(cs))
         self.nf, self.ns, self.nc = scene.n_fluid, scene.n_solid, scene.n_clump
+
+    def upload_batch(self, scenes):
+        """Independent scenes advanced together, each on its own grid; their
+        particles are numbered in scene order (dsph_upload_batch)."""
+        self.scenes = list(scenes)
+        # the structs own the numpy buffers their pointers refer to: keep them
+        # alive until the library has copied the scenes
+        structs = [sc.cstruct() for sc in self.scenes]
+        arr = (CScene * len(structs))(*structs)
+        self._call("dsph_upload_batch", arr, len(structs))
+        del structs
+        self.scene = self.scenes[0]
+        self.nf = sum(sc.n_fluid for sc in self.scenes)
+        self.ns = sum(sc.n_solid for sc in self.scenes)
+        self.nc = sum(sc.n_clump for sc in self.scenes)
+
+    def batch_info(self):
+        n = C.c_int32(0)
+        self._call("dsph_batch_info", C.byref(n), None, None, None, None, None)
+        k = n.value
+        out = {key: np.zeros(k + 1, np.int32) for key in ("fluid", "solid", "clump", "wall", "cell")}
+        self._call("dsph_batch_info", C.byref(n), _p(out["fluid"], _ip), _p(out["solid"], _ip),
+                   _p(out["clump"], _ip), _p(out["wall"], _ip), _p(out["cell"], _ip))
+        return out
 
     def set_flags(self, flags):
         self._call("dsph_set_flags", flags)
@@ -282,7 +308,7 @@ class Solver:
         return gs, cs, nc.value
 
     def grid(self, species):
-        gs, cs, ncell = self.grid_info()
+        ncell = int(self.batch_info()["cell"][-1])  # cells of all scenes
         n = self.nf if species == 0 else self.ns
         h, p = np.zeros(max(n, 1), np.int32), np.zeros(max(n, 1), np.int32)
         s, e = np.zeros(ncell, np.int32), np.zeros(ncell, np.int32)

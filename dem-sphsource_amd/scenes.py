@@ -370,3 +370,50 @@ def random_fluid_box(n=2000, box=(0.1, 0.08, 0.06), dx=0.005, seed_stream=7, h_j
         add_dummy_wall(sc, sp0[keep], 0.5 * dx)
     sc.dt = 0.25 * 1.3 * dx / c0
     return sc
+
+
+def ensemble_scene(k, config_id=4, n_range=(2000, 60000), nd_max=500, L_range=(0.8, 1.6),
+                   alpha_range=(0.01, 0.1), r=0.01, rho_s=2500.0, E=1.0e7, nu_p=0.3, res=0.5,
+                   mu=0.4) -> Scene:
+    """Scene k of the batched dam-break ensemble.  Draws from sub-stream k of
+    stream config_id: fluid count log-uniform in n_range rounded to a cube
+    n^3 (dx = 0.4/n, the 0.4 m column), DEM count uniform in {0..nd_max},
+    tank length uniform in L_range (0.6 m wide and high), artificial
+    viscosity alpha uniform in alpha_range.  The DEM spheres sit on a
+    jittered lattice downstream of the column; Hertz-Mindlin contact and
+    tank planes as in the coupled dam break."""
+    base = 1_000_000 * config_id + 16 * k
+    u = uniform(ROOT_SEED, base, 4)
+    lo, hi = n_range
+    n_f = math.exp(math.log(lo) + (math.log(hi) - math.log(lo)) * u[0])
+    n = max(2, int(round(n_f ** (1.0 / 3.0))))
+    n_d = min(nd_max, int(u[1] * (nd_max + 1)))
+    L = L_range[0] + (L_range[1] - L_range[0]) * u[2]
+    alpha = alpha_range[0] + (alpha_range[1] - alpha_range[0]) * u[3]
+    dx = 0.4 / n
+    sc = dam_break(dx=dx, L=L, name=f"ensemble_{k}")
+    sc.fluid_alpha = np.full(sc.n_fluid, alpha)
+    # DEM spheres on a 0.025 m lattice from x = 0.5 downstream, y centred
+    sp = 0.025
+    nx = max(1, int((L - 0.05 - 0.5) / sp))
+    ny = int(0.5 / sp)
+    idx = np.arange(n_d)
+    pos = np.stack([0.5 + sp * (idx % nx), 0.05 + sp * ((idx // nx) % ny),
+                    0.012 + sp * (idx // (nx * ny))], axis=1).astype(float)
+    if n_d:
+        jit = uniform(ROOT_SEED, base + 4, 3 * n_d, -0.1 * r, 0.1 * r).reshape(n_d, 3)
+        add_solid(sc, pos + jit, r, rho_s, material=0)
+    Es, Gs = hertz_effective(E, nu_p)
+    sc.set_hertz(0, 0, Es, Gs, res, 0.0, 0.0, mu, 0.0, 0.0)
+    add_tank_planes(sc, L, 0.6)
+    k_c = 0.5 * E * math.pi * r
+    dt_dem = 0.2 * math.sqrt(8 * r ** 3 * rho_s / k_c)
+    sc.dt = min(sc.dt, dt_dem) if n_d else sc.dt
+    sc.alpha = alpha
+    return sc
+
+
+def ensemble(n_scenes=256, **kw) -> list:
+    """The batched dam-break ensemble: n_scenes independent scenes (see
+    ensemble_scene); run together they step with the smallest dt."""
+    return [ensemble_scene(k, **kw) for k in range(n_scenes)]

@@ -159,6 +159,24 @@ const char* dsph_version(void);
  * allocates list storage (fluid-fluid neighbors per particle, fluid-solid and
  * solid-solid totals; pass 0 for the reference defaults 40*N_f-derived).   */
 int dsph_upload_scene(dsph_ctx* ctx, const dsph_scene* scene);
+/* Batched ensemble: n_scenes independent scenes in one context, advanced
+ * together by every phase (one kernel launch per phase for all of them) with
+ * one grid per scene (segmented hash grids), so each scene's hashes, lists
+ * and fields are those of the scene run alone.  Particles are numbered in
+ * scene order (the caller indices of scene k follow those of scene k-1);
+ * cluster and clump ids and pebble ranges are shifted accordingly.  Walls
+ * act only on their own scene's solids.  The scenes share n_material, the
+ * contact tables, gravity (those of scenes[0]) and the time step.  No
+ * reference counterpart: the reference runs one scene per process.       */
+int dsph_upload_batch(dsph_ctx* ctx, const dsph_scene* scenes,
+                      int32_t n_scenes);
+/* per-scene offsets of a batch (arrays of n_scenes + 1, any may be NULL):
+ * caller indices of fluid, solids, clumps and walls, and global cell
+ * indices (hashes read back are global: scene k's cells start at
+ * cell_offset[k]) */
+int dsph_batch_info(dsph_ctx* ctx, int32_t* n_scenes, int32_t* fluid_offset,
+                    int32_t* solid_offset, int32_t* clump_offset,
+                    int32_t* wall_offset, int32_t* cell_offset);
 int dsph_set_capacity(dsph_ctx* ctx, int32_t ff_per_particle,
                       int64_t fs_total, int64_t ss_total);
 /* per-particle rows of the fluid–fluid and fluid–solid Verlet supersets */
@@ -263,7 +281,9 @@ int dsph_download_list(dsph_ctx* ctx, int kind, int32_t* pointed,
 /* Hashed grid of the last search (neighborSearch.cu:19-113):
  * hash[] per particle in original order; perm[] = stable sort permutation
  * (sorted slot -> original index, objectHash::index); cell_start/cell_end
- * per cell (-1 = empty), n_cells = Gx*Gy*Gz + 1.  species 0 fluid, 1 solid. */
+ * per cell (-1 = empty), n_cells = Gx*Gy*Gz + 1.  species 0 fluid, 1 solid.
+ * For a batch, grid_info describes scene 0 and the arrays span the cells of
+ * all scenes (dsph_batch_info gives each scene's cell range).            */
 int dsph_grid_info(dsph_ctx* ctx, int32_t* grid_size3, double* cell_size3,
                    int32_t* n_cells);
 int dsph_download_grid(dsph_ctx* ctx, int species, int32_t* hash,

@@ -501,3 +501,93 @@ def test_clump_velocity_set_during_the_run():
     expect = c["vel"][3] + np.cross(c["angvel"][3], rp)
     assert np.allclose(s["vel"][i0:i1], expect, rtol=1e-9, atol=1e-9)
     assert np.linalg.norm(c["vel"][3] - v) < 0.05  # one step of contact and gravity
+
+
+# ------------------------------------------------------- batched scenes
+def batch_scenes():
+    scs = [
+        scenes.random_fluid_box(box=(0.05, 0.04, 0.04), dx=0.005, seed_stream=41, h_jitter=0.1),
+        scenes.coupled_dam_break(dx=0.04, n_dem=(4, 3, 3), dem_origin=(0.45, 0.2, 0.012),
+                                 name="batch_coupled"),
+        scenes.granular_column(n_sphere=300, n_clump=30, n_bonded=64),
+        scenes.dam_break(0.05),
+    ]
+    # the scenes of a batch share one set of contact tables
+    for sc in scs:
+        sc.hertz, sc.linear, sc.bonded = scs[2].hertz, scs[2].linear, scs[2].bonded
+    return scs
+
+
+def test_batch_rejects_different_contact_tables():
+    scs = batch_scenes()
+    scs[1].hertz = np.array(scs[1].hertz, float).copy()
+    scs[1].hertz[5, 0] = 0.9
+    g = dsph.Solver(dsph.PREC_FP64)
+    with pytest.raises(dsph.DsphError) as e:
+        g.upload_batch(scs)
+    assert e.value.code == -1
+
+
+@pytest.mark.parametrize("prec", [dsph.PREC_FP64, dsph.PREC_MIXED])
+def test_batch_matches_scenes_run_alone(prec):
+    """Scenes uploaded as one batch (segmented grids, one launch per phase)
+    give each scene's own grid, lists and fields."""
+    scs = batch_scenes()
+    dt = min(sc.dt for sc in scs)
+    g = dsph.Solver(prec)
+    g.upload_batch(scs)
+    g.initialize()
+    info = g.batch_info()
+    os_ = []
+    for sc in scs:
+        o = oracle.Oracle()
+        o.upload(sc)
+        o.initialize()
+        os_.append(o)
+    fo, so_, co = info["fluid"], info["solid"], info["cell"]
+
+    def check_grid_and_lists():
+        for species, off in ((0, fo), (1, so_)):
+            a = g.grid(species)
+            for k, o in enumerate(os_):
+                b = o.grid(species)
+                n0, n1 = off[k], off[k + 1]
+                assert np.array_equal(a["hash"][n0:n1] - co[k], b["hash"]), (k, species)
+                p = a["perm"][(a["perm"] >= n0) & (a["perm"] < n1)] - n0
+                assert np.array_equal(p, b["perm"]), (k, species)
+                for key in ("start", "end"):
+                    v = a[key][co[k]:co[k + 1]].copy()
+                    v[v >= 0] -= n0
+                    assert np.array_equal(v, b[key]), (k, species, key)
+        for kind, (oa, ob) in ((dsph.LIST_FLUID_FLUID, (fo, fo)), (dsph.LIST_FLUID_SOLID, (fo, so_)),
+                               (dsph.LIST_SOLID_SOLID, (so_, so_))):
+            ga, gb, _ = g.list(kind)
+            for k, o in enumerate(os_):
+                sel = (ga >= oa[k]) & (ga < oa[k + 1])
+                ra, rb, _ = o.list(kind)
+                assert np.array_equal(ga[sel] - oa[k], ra), (k, kind)
+                assert np.array_equal(gb[sel] - ob[k], rb), (k, kind)
+
+    check_grid_and_lists()
+    tol = TOL_FP64 if prec == dsph.PREC_FP64 else TOL_MIXED
+    for s in range(1, 11):
+        g.step(s, dt)
+        for o in os_:
+            o.step(s, dt)
+    if prec == dsph.PREC_FP64:  # mixed positions differ in the last bits
+        check_grid_and_lists()
+    gf, gs = g.fluid(), g.solid()
+    # errors relative to physical scales: the batch steps with the smallest
+    # time step (the granular column's), so velocities and density changes
+    # of the fluid scenes stay near zero; dRho is measured against rho0
+    floors = {"pos": 0.0, "vel": 1e-3, "drho": 1000.0, "angvel": 1e-3}
+    for k, o in enumerate(os_):
+        of, osd = o.fluid(), o.solid()
+        for key in ("pos", "vel", "drho"):
+            a = gf[key][fo[k]:fo[k + 1]]
+            if a.size:
+                assert refmodel.rel_err(a, of[key], floors[key]) < tol, (k, key)
+        for key in ("pos", "vel", "angvel"):
+            a = gs[key][so_[k]:so_[k + 1]]
+            if a.size:
+                assert refmodel.rel_err(a, osd[key], floors[key]) < tol, (k, key)
