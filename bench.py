@@ -49,7 +49,7 @@ def build_scene(dx):
     return scenes.coupled_dam_break(dx=dx)
 
 
-def run_gpu(scene, precision, steps, warmup, timing_steps, eager=False, flags=0):
+def run_gpu(scene, precision, steps, warmup, timing_steps, eager=False, flags=0, tuning=None):
     """Times `steps` steps after `warmup` (HIP-graph replay unless eager), then
     runs `timing_steps` more steps eagerly with per-family HIP events on the
     context stream (events cannot live inside a captured graph; kernel
@@ -65,6 +65,8 @@ def run_gpu(scene, precision, steps, warmup, timing_steps, eager=False, flags=0)
     else:
         g.upload(scene)
         dt = scene.dt
+    if tuning:
+        g.apply_tuning(tuning)
     g.initialize()
     i = 1
     if warmup:
@@ -173,6 +175,17 @@ def cpu_baseline(scene, budget_s, max_steps):
                       f"initial search ({dt:.1f} s)"}
 
 
+def tune(work, precision, flags, args):
+    """The autotuner's choice for one precision (None when disabled)."""
+    if args.no_autotune:
+        return None
+    import dsph
+    cfg, trials = dsph.autotune(work, precision, flags, trial_steps=10, warmup=3)
+    print(f"# autotuned ({'mixed' if precision == dsph.PREC_MIXED else 'fp64'}): {cfg}",
+          file=sys.stderr, flush=True)
+    return cfg
+
+
 def other_workload(args):
     """The batched ensemble or the granular column: one JSON line of the same
     shape (no CPU baseline: the oracle runs one scene at a time)."""
@@ -195,16 +208,21 @@ def other_workload(args):
     npart = sum(sc.n_particles for sc in scs)
     print(f"# workload: {desc}", file=sys.stderr, flush=True)
     tsteps = 0 if args.no_timing else args.timing_steps
-    g, secs, times = run_gpu(work, dsph.PREC_MIXED, args.steps, args.warmup, tsteps, args.eager, flags)
+    tun = tune(work, dsph.PREC_MIXED, flags, args)
+    g, secs, times = run_gpu(work, dsph.PREC_MIXED, args.steps, args.warmup, tsteps, args.eager, flags,
+                             tun)
     value = npart * args.steps / secs
     print(f"# mixed: {secs / args.steps * 1e3:.3f} ms/step  {value:.4g} particle-steps/s",
           file=sys.stderr, flush=True)
     g.close()
     fp64 = None
     if not args.no_fp64 and args.fp64_steps > 0:
-        g64, s64, _ = run_gpu(work, dsph.PREC_FP64, args.fp64_steps, args.warmup, 0, args.eager, flags)
+        tun64 = tune(work, dsph.PREC_FP64, flags, args)
+        g64, s64, _ = run_gpu(work, dsph.PREC_FP64, args.fp64_steps, args.warmup, 0, args.eager, flags,
+                              tun64)
         fp64 = {"value": round(npart * args.fp64_steps / s64, 1),
-                "ms_per_step": round(s64 / args.fp64_steps * 1e3, 4), "steps": args.fp64_steps}
+                "ms_per_step": round(s64 / args.fp64_steps * 1e3, 4), "steps": args.fp64_steps,
+                "tuning": tun64}
         g64.close()
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1,
@@ -213,7 +231,7 @@ def other_workload(args):
         "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
         "data": "synthetic: seeded SplitMix64 scenes",
         "config": {"workload": desc, "particles": npart,
-                   "precision": "mixed (f32 pair loops, f64 integration)"},
+                   "precision": "mixed (f32 pair loops, f64 integration)", "tuning": tun},
         "fp64_path": fp64,
         "mixed_over_fp64": round(value / fp64["value"], 3) if fp64 else None,
         "kernel_ms": {k: [round(v[0] / max(v[1], 1), 4), v[1]] for k, v in times.items()},
@@ -244,6 +262,8 @@ def main():
                     help="coupled: the 1M-particle coupled dam break (the headline); ensemble: "
                          "the batched dam-break ensemble; granular: the clump/bond column")
     ap.add_argument("--scenes", type=int, default=256, help="scenes of the ensemble workload")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shapes and skin instead of autotuning them")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -267,7 +287,9 @@ def main():
           file=sys.stderr, flush=True)
 
     tsteps = 0 if args.no_timing else args.timing_steps
-    g, secs, times = run_gpu(scene, dsph.PREC_MIXED, args.steps, args.warmup, tsteps, args.eager)
+    tun = tune(scene, dsph.PREC_MIXED, 0, args)
+    g, secs, times = run_gpu(scene, dsph.PREC_MIXED, args.steps, args.warmup, tsteps, args.eager,
+                             tuning=tun)
     value = npart * args.steps / secs
     print(f"# mixed: {secs / args.steps * 1e3:.3f} ms/step  {value:.4g} particle-steps/s",
           file=sys.stderr, flush=True)
@@ -279,10 +301,12 @@ def main():
 
     fp64 = None
     if not args.no_fp64 and args.fp64_steps > 0:
+        tun64 = tune(scene, dsph.PREC_FP64, 0, args)
         g64, s64, t64 = run_gpu(scene, dsph.PREC_FP64, args.fp64_steps, args.warmup, tsteps,
-                                args.eager)
+                                args.eager, tuning=tun64)
         fp64 = {"value": round(npart * args.fp64_steps / s64, 1),
-                "ms_per_step": round(s64 / args.fp64_steps * 1e3, 4), "steps": args.fp64_steps}
+                "ms_per_step": round(s64 / args.fp64_steps * 1e3, 4), "steps": args.fp64_steps,
+                "tuning": tun64}
         r64 = roofline(t64, "fp64", scene.n_fluid, ff_directed, fs_pairs) if t64 else None
         if r64:
             fp64["roofline_frac"] = r64["frac"]
@@ -309,7 +333,8 @@ def main():
         "data": "synthetic: deterministic lattice dam break + seeded-jitter DEM lattice",
         "config": {"workload": f"coupled DEM-SPH dam break dx={args.dx}: {scene.n_fluid} fluid + "
                                f"{scene.n_solid - 2000} wall + 2000 DEM spheres",
-                   "particles": npart, "precision": "mixed (f32 pair loops, f64 integration)"},
+                   "particles": npart, "precision": "mixed (f32 pair loops, f64 integration)",
+                   "tuning": tun},
         "moving_particle_steps_per_s": round(moving * args.steps / secs, 1),
         "fp64_path": fp64,
         "mixed_over_fp64": round(value / fp64["value"], 3) if fp64 else None,

@@ -93,6 +93,8 @@ struct dsph_ctx {
     double4* sxb = nullptr;     // solid positions at the last superset build
     unsigned long long* ctl = nullptr;
     double skin_h = 0.3;        // superset skin / max fluid h
+    int tune_bs = 256;          // workgroup of the fluid pair loops
+    int tune_bmem = 64;         // particles per wavefront in the superset build
     // active (non-dummy) solid grid for the DEM contact search
     unsigned* akey = nullptr;
     unsigned* akey_sorted = nullptr;
@@ -268,6 +270,7 @@ FluidArgs fluid_args(dsph_ctx* c) {
     FluidArgs a;
     a.f = c->fset[0];
     a.nf = c->nf;
+    a.bs = c->tune_bs;
     a.cur = c->dcur;
     a.L = c->L;
     a.so = c->so;
@@ -336,6 +339,7 @@ SearchArgs search_args(dsph_ctx* c) {
     SearchArgs a;
     a.G = grids_view(c);
     a.ncell = c->ncell;
+    a.bmem = c->tune_bmem;
     a.nf = c->nf;
     a.ns = c->ns;
     a.ph = c->fset[0].ph;
@@ -993,6 +997,39 @@ int dsph_set_superset_capacity(dsph_ctx* c, int32_t ff, int32_t fs) {
     } catch (const ApiError& e) {
         return fail(c, e);
     }
+}
+
+int dsph_set_tuning(dsph_ctx* c, int32_t key, int32_t value) {
+    if (!c) return DSPH_E_INVALID;
+    if (key == DSPH_TUNE_PAIR_BLOCK) {
+        if (value < 64 || value > 512 || value % 64) return DSPH_E_INVALID;
+        c->tune_bs = value;
+    } else if (key == DSPH_TUNE_BUILD_BLOCK) {
+        if (value < 1 || value > 64) return DSPH_E_INVALID;
+        c->tune_bmem = value;
+        if (c->uploaded) {
+            try {
+                force_rebuild(c);
+            } catch (const ApiError& e) {
+                return fail(c, e);
+            }
+        }
+    } else {
+        return DSPH_E_INVALID;
+    }
+    invalidate_graphs(c);
+    return DSPH_OK;
+}
+
+int dsph_get_tuning(dsph_ctx* c, int32_t key, int32_t* value) {
+    if (!c || !value) return DSPH_E_INVALID;
+    if (key == DSPH_TUNE_PAIR_BLOCK)
+        *value = c->tune_bs;
+    else if (key == DSPH_TUNE_BUILD_BLOCK)
+        *value = c->tune_bmem;
+    else
+        return DSPH_E_INVALID;
+    return DSPH_OK;
 }
 
 int dsph_set_gravity(dsph_ctx* c, const double g[3]) {

@@ -227,6 +227,7 @@ void cell_spans(hipStream_t s, const unsigned* sorted, int64_t n, int* start, in
 struct SearchArgs {
     Grids G;
     int ncell;              // cells of all scenes
+    int bmem;               // particles per wavefront in the superset build (tunable, <= 64)
     int nf, ns;
     const double4* ph;      // fluid position + h, slot order
     const int* fcs;         // fluid cell ranges over sorted positions
@@ -281,6 +282,7 @@ void solid_normals(hipStream_t s, const Grids& G, const Solids& so, int ns, cons
 struct FluidArgs {
     FluidSet f;
     int nf;
+    int bs;   // workgroup size of the pair loops (tunable, <= 512)
     int cur;  // current drho buffer
     Lists L;
     Solids so;

@@ -46,7 +46,8 @@ namespace dsph {
 
 namespace {
 
-constexpr int kBS = 256;
+constexpr int kBS = 256;     // workgroup of the streaming kernels
+constexpr int kMaxBS = 512;  // largest tunable workgroup of the pair loops
 constexpr double kGradC = -105.0 / (16.0 * kPi);  // -5 * 21/(16 pi)
 constexpr double kWC = 21.0 / (16.0 * kPi);
 
@@ -286,7 +287,7 @@ __global__ void __launch_bounds__(kBS) k_records(FluidSet f, int n, int buf) {
 
 // ------------------------------------------------------------- continuity
 template <bool MIX>
-__global__ void __launch_bounds__(kBS) k_continuity(FluidArgs a, double dt) {
+__global__ void __launch_bounds__(kMaxBS) k_continuity(FluidArgs a, double dt) {
     using R = typename Prec<MIX>::R;
     const int s = xcd_block() * blockDim.x + threadIdx.x;
     if (s >= a.nf) return;
@@ -336,7 +337,7 @@ __global__ void __launch_bounds__(kBS) k_continuity(FluidArgs a, double dt) {
 
 // --------------------------------------------------------------- momentum
 template <bool MIX>
-__global__ void __launch_bounds__(kBS) k_momentum(FluidArgs a, int write_fs) {
+__global__ void __launch_bounds__(kMaxBS) k_momentum(FluidArgs a, int write_fs) {
     using R = typename Prec<MIX>::R;
     const int s = xcd_block() * blockDim.x + threadIdx.x;
     if (s >= a.nf) return;
@@ -398,7 +399,7 @@ __global__ void __launch_bounds__(kBS) k_momentum(FluidArgs a, int write_fs) {
 
 // calFluid2SolidForce (integrate.cu:282-330) on steps without a fluid update
 template <bool MIX>
-__global__ void __launch_bounds__(kBS) k_fs_force(FluidArgs a) {
+__global__ void __launch_bounds__(kMaxBS) k_fs_force(FluidArgs a) {
     using R = typename Prec<MIX>::R;
     const int s = xcd_block() * blockDim.x + threadIdx.x;
     if (s >= a.nf || !fs_valid(a.L, a.nf)) return;
@@ -419,7 +420,7 @@ __global__ void __launch_bounds__(kBS) k_fs_force(FluidArgs a) {
 
 // -------------------------------------------------------- density re-init
 template <bool MIX>
-__global__ void __launch_bounds__(kBS) k_density_reinit(FluidArgs a, int i_step) {
+__global__ void __launch_bounds__(kMaxBS) k_density_reinit(FluidArgs a, int i_step) {
     using R = typename Prec<MIX>::R;
     const int s = xcd_block() * blockDim.x + threadIdx.x;
     if (s >= a.nf) return;
@@ -516,24 +517,25 @@ __global__ void k_kick_drift(FluidSet f, int n, double dt, int cur) {
 
 }  // namespace
 
-#define DSPH_LAUNCH(kern, n, s, ...)                                                   \
-    do {                                                                               \
-        if (mixed)                                                                     \
-            hipLaunchKernelGGL(kern<true>, dim3(blocks_for(n)), dim3(kBS), 0, s, __VA_ARGS__);  \
-        else                                                                           \
-            hipLaunchKernelGGL(kern<false>, dim3(blocks_for(n)), dim3(kBS), 0, s, __VA_ARGS__); \
+#define DSPH_LAUNCH_BS(kern, n, bs, s, ...)                                                 \
+    do {                                                                                    \
+        if (mixed)                                                                          \
+            hipLaunchKernelGGL(kern<true>, dim3(blocks_for(n, bs)), dim3(bs), 0, s, __VA_ARGS__);  \
+        else                                                                                \
+            hipLaunchKernelGGL(kern<false>, dim3(blocks_for(n, bs)), dim3(bs), 0, s, __VA_ARGS__); \
     } while (0)
+#define DSPH_LAUNCH(kern, n, s, ...) DSPH_LAUNCH_BS(kern, n, kBS, s, __VA_ARGS__)
 
 void fluid_records(hipStream_t s, const FluidArgs& a, int buf, bool mixed) {
     if (a.nf > 0) DSPH_LAUNCH(k_records, a.nf, s, a.f, a.nf, buf);
 }
 
 void density_reinit(hipStream_t s, const FluidArgs& a, int i_step, bool mixed) {
-    if (a.nf > 0) DSPH_LAUNCH(k_density_reinit, a.nf, s, a, i_step);
+    if (a.nf > 0) DSPH_LAUNCH_BS(k_density_reinit, a.nf, a.bs, s, a, i_step);
 }
 
 void continuity(hipStream_t s, const FluidArgs& a, double dt, bool mixed) {
-    if (a.nf > 0) DSPH_LAUNCH(k_continuity, a.nf, s, a, dt);
+    if (a.nf > 0) DSPH_LAUNCH_BS(k_continuity, a.nf, a.bs, s, a, dt);
 }
 
 void fluid_drift(hipStream_t s, const FluidArgs& a, double dt, bool mixed) {
@@ -541,11 +543,11 @@ void fluid_drift(hipStream_t s, const FluidArgs& a, double dt, bool mixed) {
 }
 
 void momentum(hipStream_t s, const FluidArgs& a, bool mixed, bool write_fs_force) {
-    if (a.nf > 0) DSPH_LAUNCH(k_momentum, a.nf, s, a, (int)write_fs_force);
+    if (a.nf > 0) DSPH_LAUNCH_BS(k_momentum, a.nf, a.bs, s, a, (int)write_fs_force);
 }
 
 void fluid_solid_force(hipStream_t s, const FluidArgs& a, bool mixed) {
-    if (a.nf > 0) DSPH_LAUNCH(k_fs_force, a.nf, s, a);
+    if (a.nf > 0) DSPH_LAUNCH_BS(k_fs_force, a.nf, a.bs, s, a);
 }
 
 void fluid_kick_drift(hipStream_t s, const FluidArgs& a, double dt, bool mixed) {

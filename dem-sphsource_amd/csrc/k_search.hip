@@ -277,21 +277,23 @@ __global__ void k_store_xb(const double4* __restrict__ p, double4* __restrict__ 
 }
 
 // ------------------------------------------------------- cell blocks
-// every cell holding fluid is cut into blocks of <= 64 consecutive sorted
-// particles, one wavefront each; blocks are listed in cell rank order
+// every cell holding fluid is cut into blocks of <= bmem consecutive sorted
+// particles (a tunable <= 64), one wavefront each; blocks are listed in cell
+// rank order
 __global__ void k_blk_count(const int* __restrict__ cs, const int* __restrict__ ce,
-                            const unsigned* __restrict__ cinv, int ncell, int* __restrict__ nb,
-                            const unsigned long long* __restrict__ ctl) {
+                            const unsigned* __restrict__ cinv, int ncell, int bmem,
+                            int* __restrict__ nb, const unsigned long long* __restrict__ ctl) {
     if (ctl[CTL_REBUILD] == 0) return;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= ncell) return;
     const int c = (int)cinv[r];
     const int st = cs[c];
-    nb[r] = st >= 0 ? (ce[c] - st + 63) >> 6 : 0;
+    nb[r] = st >= 0 ? (ce[c] - st + bmem - 1) / bmem : 0;
 }
 __global__ void k_blk_fill(const int* __restrict__ cs, const int* __restrict__ ce,
-                           const unsigned* __restrict__ cinv, int ncell, const int* __restrict__ off,
-                           int2* __restrict__ blocks, const unsigned long long* __restrict__ ctl) {
+                           const unsigned* __restrict__ cinv, int ncell, int bmem,
+                           const int* __restrict__ off, int2* __restrict__ blocks,
+                           const unsigned long long* __restrict__ ctl) {
     if (ctl[CTL_REBUILD] == 0) return;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= ncell) return;
@@ -300,7 +302,7 @@ __global__ void k_blk_fill(const int* __restrict__ cs, const int* __restrict__ c
     if (st < 0) return;
     const int n = ce[c] - st;
     const int o = off[r];
-    for (int b = 0; b * 64 < n; b++) blocks[o + b] = make_int2(c, st + 64 * b);
+    for (int b = 0; b * bmem < n; b++) blocks[o + b] = make_int2(c, st + bmem * b);
 }
 
 __device__ __forceinline__ double readlane_d(double v, int k) {
@@ -333,7 +335,7 @@ __global__ void __launch_bounds__(kBS) k_build(SearchArgs a, const int2* __restr
         const int2 blk = blocks[b];
         const int cell = __builtin_amdgcn_readfirstlane(blk.x);
         const int t0 = __builtin_amdgcn_readfirstlane(blk.y);
-        const int nmem = min(64, a.fce[cell] - t0);
+        const int nmem = min(a.bmem, a.fce[cell] - t0);
         const bool act = lane < nmem;
         const int s = a.fmember[t0 + (act ? lane : 0)];
         // the block's cell, and so its scene, is the same for every lane
@@ -886,10 +888,10 @@ void fluid_superset(hipStream_t s, const SearchArgs& a, const unsigned* cinv, in
     if (a.nf <= 0) return;
     const int nc = a.ncell;
     hipLaunchKernelGGL(k_blk_count, dim3(blocks_for(nc)), dim3(kBS), 0, s, a.fcs, a.fce, cinv, nc,
-                       nb, (const unsigned long long*)a.ctl);
+                       a.bmem, nb, (const unsigned long long*)a.ctl);
     exclusive_scan(s, nb, off, nc, scan_tmp);
     hipLaunchKernelGGL(k_blk_fill, dim3(blocks_for(nc)), dim3(kBS), 0, s, a.fcs, a.fce, cinv, nc,
-                       (const int*)off, blocks, (const unsigned long long*)a.ctl);
+                       a.bmem, (const int*)off, blocks, (const unsigned long long*)a.ctl);
     hipLaunchKernelGGL(k_build, dim3(wave_grid(a.nf)), dim3(kBS), 0, s, a, (const int2*)blocks,
                        (const int*)(off + nc));
     hipLaunchKernelGGL(k_store_xb, dim3(blocks_for(a.nf)), dim3(kBS), 0, s, a.ph, fxb, a.nf,

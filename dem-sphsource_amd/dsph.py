@@ -12,6 +12,7 @@ constructing a ``Solver`` raises.
 from __future__ import annotations
 
 import ctypes as C
+import time
 import os
 
 import numpy as np
@@ -23,6 +24,7 @@ LIB_PATH = os.path.join(HERE, "libdsph.so")
 
 PREC_FP64, PREC_MIXED = 0, 1
 FLAG_WALL_CONTINUE, FLAG_CLUMP_QUATERNION, FLAG_ART_VISCOSITY = 1, 2, 4
+TUNE_PAIR_BLOCK, TUNE_BUILD_BLOCK = 1, 2
 LIST_FLUID_FLUID, LIST_FLUID_SOLID, LIST_SOLID_SOLID, LIST_BOND = 0, 1, 2, 3
 ST_FF_OVERFLOW, ST_FS_OVERFLOW, ST_SS_OVERFLOW, ST_OUT_OF_DOMAIN = 0, 1, 2, 3
 ST_FF_MAX, ST_FS_TOTAL, ST_SS_TOTAL, ST_SUP_OVERFLOW, ST_REBUILDS = 4, 5, 6, 7, 8
@@ -43,6 +45,8 @@ _SIGS = {
     "dsph_set_gravity": [_dp],
     "dsph_set_reorder_period": [C.c_int32],
     "dsph_set_skin": [C.c_double],
+    "dsph_set_tuning": [C.c_int32, C.c_int32],
+    "dsph_get_tuning": [C.c_int32, _ip],
     "dsph_set_superset_capacity": [C.c_int32, C.c_int32],
     "dsph_initialize": [C.c_int],
     "dsph_neighbor_search": [C.c_int32, C.c_int32],
@@ -173,6 +177,23 @@ class Solver:
 
     def set_flags(self, flags):
         self._call("dsph_set_flags", flags)
+
+    def set_tuning(self, key, value):
+        self._call("dsph_set_tuning", key, int(value))
+
+    def get_tuning(self, key):
+        v = C.c_int32(0)
+        self._call("dsph_get_tuning", key, C.byref(v))
+        return v.value
+
+    def apply_tuning(self, cfg):
+        """Apply a configuration returned by autotune()."""
+        if "pair_block" in cfg:
+            self.set_tuning(TUNE_PAIR_BLOCK, cfg["pair_block"])
+        if "build_block" in cfg:
+            self.set_tuning(TUNE_BUILD_BLOCK, cfg["build_block"])
+        if "skin" in cfg:
+            self.set_skin(cfg["skin"])
 
     def set_capacity(self, ff=0, fs=0, ss=0):
         self._call("dsph_set_capacity", ff, fs, ss)
@@ -330,3 +351,73 @@ class Solver:
         m = self._call("dsph_kernel_times", names, 4096, _p(ms), 32)
         labels = names.raw.split(b"\0")[:m]
         return {labels[k].decode(): (ms[2 * k], int(ms[2 * k + 1])) for k in range(m)}
+
+
+# --------------------------------------------------------------- autotuner
+TUNING_SPACE = {
+    "skin": [0.1, 0.2, 0.3, 0.5],               # Verlet skin / h: superset size vs rebuilds
+    "pair_block": [64, 128, 256, 512],          # workgroup of the fluid pair loops
+    "build_block": [32, 48, 64],                # particles per wavefront in the superset build
+}
+
+
+def autotune(scene, precision=PREC_MIXED, flags=0, trial_steps=20, warmup=5, space=None,
+             log=None):
+    """Pick the fastest launch shapes and Verlet skin for a workload.
+
+    Runs the workload (a Scene, or a list of Scenes uploaded as one batch) on
+    a throwaway context and times `trial_steps` graph-replayed steps per
+    candidate, one coordinate of the search space at a time (skin, then the
+    pair-loop workgroup, then the build block), keeping the best value of
+    each before moving to the next.  Every candidate gives the same results;
+    only the time differs.  Returns (config, trials) where config feeds
+    Solver.apply_tuning and trials lists (config, ms per step)."""
+    space = dict(TUNING_SPACE, **(space or {}))
+    g = Solver(precision)
+    g.set_flags(flags)
+    if isinstance(scene, (list, tuple)):
+        g.upload_batch(scene)
+        dt = min(sc.dt for sc in scene)
+    else:
+        g.upload(scene)
+        dt = scene.dt
+    g.initialize()
+    cfg = {"skin": 0.3, "pair_block": 256, "build_block": 64}
+    trials = []
+    step = [1]
+
+    def timed(n):
+        g.synchronize()
+        t0 = time.perf_counter()
+        g.run(n, dt, first=step[0], graphs=True)
+        g.synchronize()
+        step[0] += n
+        return (time.perf_counter() - t0) / n * 1e3
+
+    def measure(c):
+        g.apply_tuning(c)
+        # eager warm-up steps grow any list the new skin overflows
+        g.run(warmup, dt, first=step[0], graphs=False)
+        step[0] += warmup
+        try:
+            ms = timed(trial_steps)
+        except DsphError as e:
+            if e.code != -3:
+                raise
+            g.run(1, dt, first=step[0], graphs=False)  # grow, then time again
+            step[0] += 1
+            ms = timed(trial_steps)
+        trials.append((dict(c), ms))
+        if log:
+            log(f"autotune {c} -> {ms:.4f} ms/step")
+        return ms
+
+    for key in ("skin", "pair_block", "build_block"):
+        best = None
+        for v in space[key]:
+            ms = measure(dict(cfg, **{key: v}))
+            if best is None or ms < best[0]:
+                best = (ms, v)
+        cfg[key] = best[1]
+    g.close()
+    return cfg, trials

@@ -3,10 +3,12 @@
 #include "dataBase.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <iostream>
 #include <stdexcept>
 #include <string>
+#include <utility>
 
 static_assert(sizeof(double3) == 3 * sizeof(double), "double3 must be three packed doubles");
 static_assert(sizeof(symMatrix) == 6 * sizeof(double), "symMatrix must be six packed doubles");
@@ -382,6 +384,9 @@ void dataBase::buildDeviceData() {
     }
     check(dsph_create(gpuPara.deviceIndex, precision_, &dev));
     check(dsph_set_flags(dev, flags_));
+    check(dsph_set_tuning(dev, DSPH_TUNE_PAIR_BLOCK, tunePair_));
+    check(dsph_set_tuning(dev, DSPH_TUNE_BUILD_BLOCK, tuneBuild_));
+    check(dsph_set_skin(dev, tuneSkin_));
     uploadContactModels();
     dsph_scene s{};
     s.domain_origin[0] = domainOrigin.x;
@@ -481,4 +486,61 @@ const HostClump& dataBase::getHostClumpData() {
                                   &c.orientations[0].q0));
     }
     return hos.clumps;
+}
+
+// ---------------------------------------------------------------- tuning
+// One coordinate at a time (skin, pair-loop workgroup, build block): each
+// candidate runs warm-up steps (growing lists if needed) and then trial
+// steps replayed from HIP graphs; the best value of a coordinate is kept
+// before the next one is searched.  Results do not depend on the choice.
+void dataBase::autotune() {
+    const int trial = autotuneSteps_ > 0 ? autotuneSteps_ : 20, warm = 5;
+    const double dt = simPara.timeStep;
+    const int gap = fluidIntegrateGap;
+    buildDeviceData();
+    check(dsph_initialize(dev, 1));
+    int step = 1;
+    auto measure = [&](int pair, int build, double skin) {
+        check(dsph_set_tuning(dev, DSPH_TUNE_PAIR_BLOCK, pair));
+        check(dsph_set_tuning(dev, DSPH_TUNE_BUILD_BLOCK, build));
+        check(dsph_set_skin(dev, skin));
+        // eager warm-up steps grow any list the new skin overflows
+        for (int k = 0; k < warm; k++, step++) check(dsph_step(dev, step, dt, gap, 1));
+        auto timed = [&]() {
+            check(dsph_synchronize(dev));
+            const auto t0 = std::chrono::steady_clock::now();
+            const int rc = dsph_step_n(dev, step, trial, dt, gap);
+            step += trial;
+            return std::make_pair(
+                rc, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / trial);
+        };
+        auto r = timed();
+        if (r.first == DSPH_E_CAPACITY) {  // grow, then time again
+            check(dsph_step(dev, step++, dt, gap, 1));
+            r = timed();
+        }
+        check(r.first);
+        return r.second;
+    };
+    int pair = 256, build = 64;
+    double skin = 0.3, best = 1e300;
+    for (double v : {0.1, 0.2, 0.3, 0.5}) {
+        const double ms = measure(pair, build, v);
+        if (ms < best) best = ms, skin = v;
+    }
+    best = 1e300;
+    for (int v : {64, 128, 256, 512}) {
+        const double ms = measure(v, build, skin);
+        if (ms < best) best = ms, pair = v;
+    }
+    best = 1e300;
+    for (int v : {32, 48, 64}) {
+        const double ms = measure(pair, v, skin);
+        if (ms < best) best = ms, build = v;
+    }
+    std::cout << "Autotuned: pair-loop workgroup " << pair << ", build block " << build << ", skin "
+              << skin << " h (" << best << " ms/step)" << std::endl;
+    setTuning(pair, build, skin);
+    dsph_destroy(dev);
+    dev = nullptr;
 }

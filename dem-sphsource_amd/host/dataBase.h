@@ -70,6 +70,17 @@ protected:
     void setPrecision(int precision) { precision_ = precision; }
     void setBehaviourFlags(unsigned flags) { flags_ = flags; }
     dsph_ctx* context() const { return dev; }
+    // before solve(): time trialSteps steps per candidate launch shape and
+    // Verlet skin on a throwaway copy of the scene and run with the fastest
+    void setAutotune(int trialSteps) { autotuneSteps_ = trialSteps; }
+    // pick launch shapes and skin now (used by solverBase::initialize)
+    void autotune();
+    // the configuration the next device context gets
+    void setTuning(int pairBlock, int buildBlock, double skin) {
+        tunePair_ = pairBlock;
+        tuneBuild_ = buildBlock;
+        tuneSkin_ = skin;
+    }
 
     struct simulationParameter {
         double maximumTime;
@@ -108,6 +119,13 @@ private:
     void uploadContactModels();
     int precision_ = DSPH_PREC_FP64;
     unsigned flags_ = 0;
+    int tunePair_ = 256, tuneBuild_ = 64;
+    double tuneSkin_ = 0.3;
+
+protected:
+    int autotuneSteps_ = 0;
+
+private:
     // flat copies handed to dsph_upload_scene
     std::vector<double> tables_[3];
 };

@@ -30,6 +30,7 @@ void solverBase::initialize() {
     if (simPara.frameInterval < 1) simPara.frameInterval = 1;
     simPara.iStep = 0;
     simPara.iFrame = 0;
+    if (autotuneSteps_ > 0) autotune();
     buildDeviceData();
     check(dsph_initialize(dev, 1));
     ::mkdir(dir.c_str(), 0755);

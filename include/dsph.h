@@ -193,6 +193,19 @@ int dsph_set_reorder_period(dsph_ctx* ctx, int32_t period);
  * rebuild frequency. */
 int dsph_set_skin(dsph_ctx* ctx, double skin_over_h);
 
+/* Launch-shape tunables (no reference counterpart; results do not depend on
+ * them, only the speed does):
+ *   DSPH_TUNE_PAIR_BLOCK   workgroup size of the fluid pair loops, a
+ *                          multiple of 64 in [64, 512] (default 256);
+ *   DSPH_TUNE_BUILD_BLOCK  particles per wavefront in the Verlet superset
+ *                          build, [1, 64] (default 64).
+ * The Verlet skin (dsph_set_skin) and the reorder period complete the
+ * autotuner's search space (dsph.py autotune, dataBase::autotune).        */
+#define DSPH_TUNE_PAIR_BLOCK   1
+#define DSPH_TUNE_BUILD_BLOCK  2
+int dsph_set_tuning(dsph_ctx* ctx, int32_t key, int32_t value);
+int dsph_get_tuning(dsph_ctx* ctx, int32_t key, int32_t* value);
+
 /* replaces: solverBase::initialize lines 34-37 (solverBase.cpp:81-84):
  * neighborSearch(dev,0,1), addBondData (dataBase.cpp:270-346) and
  * setSolidNormal (dataBase.cpp:348-413).  Normals passed in the scene are

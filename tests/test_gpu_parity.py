@@ -591,3 +591,39 @@ def test_batch_matches_scenes_run_alone(prec):
             a = gs[key][so_[k]:so_[k + 1]]
             if a.size:
                 assert refmodel.rel_err(a, osd[key], floors[key]) < tol, (k, key)
+
+
+# ------------------------------------------------------------- tuning
+@pytest.mark.parametrize("prec", [dsph.PREC_FP64, dsph.PREC_MIXED])
+def test_launch_shapes_do_not_change_results(prec):
+    sc = SCENES["coupled_small"]
+    runs = []
+    for pair, build in ((256, 64), (64, 32), (512, 17)):
+        g = dsph.Solver(prec)
+        g.upload(sc)
+        g.set_tuning(dsph.TUNE_PAIR_BLOCK, pair)
+        g.set_tuning(dsph.TUNE_BUILD_BLOCK, build)
+        assert g.get_tuning(dsph.TUNE_PAIR_BLOCK) == pair
+        g.initialize()
+        g.run(15, sc.dt, graphs=True)
+        runs.append((fields(g), g.list(dsph.LIST_FLUID_FLUID)))
+        g.close()
+    for f, lst in runs[1:]:
+        for k in f:
+            assert np.array_equal(f[k], runs[0][0][k]), k
+        assert all(np.array_equal(a, b) for a, b in zip(lst, runs[0][1]))
+
+
+def test_autotune_returns_a_configuration_of_its_space():
+    sc = SCENES["coupled_small"]
+    cfg, trials = dsph.autotune(sc, dsph.PREC_MIXED, trial_steps=3, warmup=2)
+    for key, values in dsph.TUNING_SPACE.items():
+        assert cfg[key] in values
+    assert len(trials) == sum(len(v) for v in dsph.TUNING_SPACE.values())
+    assert all(ms > 0 for _, ms in trials)
+    g = dsph.Solver(dsph.PREC_MIXED)
+    g.upload(sc)
+    g.apply_tuning(cfg)
+    g.initialize()
+    g.run(5, sc.dt, graphs=True)
+    assert np.all(np.isfinite(g.fluid()["pos"]))
