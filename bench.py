@@ -73,11 +73,13 @@ def run_gpu(scene, precision, steps, warmup, timing_steps, eager=False, flags=0,
         g.run(warmup, dt, first=i, graphs=not eager)
         i += warmup
     g.synchronize()
+    rb0 = int(g.status()[8])
     t0 = time.perf_counter()
     # with graphs the run ends with a status read (device sync)
     g.run(steps, dt, first=i, graphs=not eager)
     g.synchronize()
     t1 = time.perf_counter()
+    g.rebuilds_timed = int(g.status()[8]) - rb0
     i += steps
     times = {}
     if timing_steps:
@@ -176,7 +178,12 @@ def cpu_baseline(scene, budget_s, max_steps):
 
 
 def tune(work, precision, flags, args):
-    """The autotuner's choice for one precision (None when disabled)."""
+    """The autotuner's choice for one precision (None when disabled; the
+    --tuning configuration when given)."""
+    import dsph
+    if args.tuning:
+        cfg = json.loads(args.tuning)
+        return cfg.get("fp64" if precision == dsph.PREC_FP64 else "mixed", cfg)
     if args.no_autotune:
         return None
     import dsph
@@ -264,6 +271,9 @@ def main():
     ap.add_argument("--scenes", type=int, default=256, help="scenes of the ensemble workload")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes and skin instead of autotuning them")
+    ap.add_argument("--tuning", default=None, metavar="JSON",
+                    help='fixed configuration instead of autotuning, e.g. \'{"mixed": {"skin": 0.2, '
+                         '"pair_block": 256, "build_block": 32}, "fp64": {...}}\' (profiling runs)')
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -294,6 +304,7 @@ def main():
     print(f"# mixed: {secs / args.steps * 1e3:.3f} ms/step  {value:.4g} particle-steps/s",
           file=sys.stderr, flush=True)
     ff_directed, fs_pairs = pair_counts(g)
+    g_rebuilds = g.rebuilds_timed
     roof = roofline(times, "mixed", scene.n_fluid, ff_directed, fs_pairs) if times else None
     if roof:
         roof["traffic"] = traffic_from_profile(roof["kernel"], "mixed")
@@ -306,7 +317,7 @@ def main():
                                 args.eager, tuning=tun64)
         fp64 = {"value": round(npart * args.fp64_steps / s64, 1),
                 "ms_per_step": round(s64 / args.fp64_steps * 1e3, 4), "steps": args.fp64_steps,
-                "tuning": tun64}
+                "tuning": tun64, "superset_rebuilds": g64.rebuilds_timed}
         r64 = roofline(t64, "fp64", scene.n_fluid, ff_directed, fs_pairs) if t64 else None
         if r64:
             fp64["roofline_frac"] = r64["frac"]
@@ -336,6 +347,7 @@ def main():
                    "particles": npart, "precision": "mixed (f32 pair loops, f64 integration)",
                    "tuning": tun},
         "moving_particle_steps_per_s": round(moving * args.steps / secs, 1),
+        "superset_rebuilds": g_rebuilds,
         "fp64_path": fp64,
         "mixed_over_fp64": round(value / fp64["value"], 3) if fp64 else None,
         "roofline": roof,

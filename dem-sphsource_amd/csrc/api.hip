@@ -8,6 +8,7 @@
 // solverBase::initialize/update (core/solverBase.cpp:13-67) and the
 // uploadState read-backs (core/deviceStructs.h).
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -84,17 +85,17 @@ struct dsph_ctx {
     unsigned* cinv = nullptr;         // Morton rank -> cell
     int* fcs = nullptr;
     int* fce = nullptr;
-    int* fblk_nb = nullptr;     // blocks per cell
-    int* fblk_off = nullptr;    // ncell + 1; [ncell] = number of blocks
-    int2* fblocks = nullptr;    // (cell, first sorted position)
+    int2* fchunks = nullptr;    // superset build chunks: (first, end) sorted positions
+    int nfchunks = 0;
     unsigned* hslot = nullptr;  // reference hash per fluid slot
+    unsigned char* cspare = nullptr;  // per cell: its stencil reaches the spare cell
     double4* fxb = nullptr;     // fluid positions at the last superset build
     double4* fsp = nullptr;     // fluid positions at the last fluid search
     double4* sxb = nullptr;     // solid positions at the last superset build
     unsigned long long* ctl = nullptr;
     double skin_h = 0.3;        // superset skin / max fluid h
     int tune_bs = 256;          // workgroup of the fluid pair loops
-    int tune_bmem = 64;         // particles per wavefront in the superset build
+    int tune_bmem = 64;         // particles per superset build chunk (one wavefront)
     // active (non-dummy) solid grid for the DEM contact search
     unsigned* akey = nullptr;
     unsigned* akey_sorted = nullptr;
@@ -306,6 +307,25 @@ DemArgs dem_args(dsph_ctx* c) {
     return a;
 }
 
+// Chunks of the superset build: runs of <= tune_bmem consecutive sorted
+// particles of one scene.  Scene k's cells rank in [cell0_k, cell0_k +
+// ncell_k), so its particles hold the sorted positions [foff[k], foff[k+1]).
+void build_chunks(dsph_ctx* c) {
+    std::vector<int2> ch;
+    const int m = c->tune_bmem;
+    for (int k = 0; k < c->nscene; k++)
+        for (int t = c->foff[k]; t < c->foff[k + 1]; t += m)
+            ch.push_back(make_int2(t, std::min(t + m, c->foff[k + 1])));
+    if (c->fchunks) {
+        HIPCHK(hipStreamSynchronize(c->st));
+        dfree(c, c->fchunks);
+    }
+    c->fchunks = dalloc<int2>(c, std::max<size_t>(ch.size(), 1));
+    h2d(c, c->fchunks, ch.data(), ch.size());
+    HIPCHK(hipStreamSynchronize(c->st));
+    c->nfchunks = (int)ch.size();
+}
+
 // Half-width of the superset build stencil: every particle within the
 // superset radius 2 h_max + skin of a particle lies within sw cells of that
 // particle's cell on each axis.  The (2 sw + 1)^3 linear offsets are distinct
@@ -339,7 +359,6 @@ SearchArgs search_args(dsph_ctx* c) {
     SearchArgs a;
     a.G = grids_view(c);
     a.ncell = c->ncell;
-    a.bmem = c->tune_bmem;
     a.nf = c->nf;
     a.ns = c->ns;
     a.ph = c->fset[0].ph;
@@ -348,6 +367,7 @@ SearchArgs search_args(dsph_ctx* c) {
     a.fmember = c->fmember;
     a.forig = c->fset[0].orig;
     a.hslot = c->hslot;
+    a.cspare = c->cspare;
     a.scs = c->scs;
     a.sce = c->sce;
     a.sperm = c->sperm;
@@ -393,8 +413,7 @@ void neighbor_search(dsph_ctx* c, int i_step, int gap, bool save_history) {
         const SearchArgs sa = search_args(c);
         {
             FamTimer t(c, F_FSEARCH);
-            fluid_superset(s, sa, c->cinv, c->fblk_nb, c->fblk_off, c->fblocks, c->scan_tmp, c->fxb,
-                           c->sxb);
+            fluid_superset(s, sa, c->fchunks, c->nfchunks, c->fxb, c->sxb);
         }
         FamTimer t(c, F_FSLIST);
         fluid_filter(s, sa, c->scan_tmp);
@@ -1009,6 +1028,7 @@ int dsph_set_tuning(dsph_ctx* c, int32_t key, int32_t value) {
         c->tune_bmem = value;
         if (c->uploaded) {
             try {
+                build_chunks(c);
                 force_rebuild(c);
             } catch (const ApiError& e) {
                 return fail(c, e);
@@ -1060,6 +1080,8 @@ int dsph_upload_batch(dsph_ctx* c, const dsph_scene* scenes, int32_t n_scenes) {
         c->cl = Clumps();
         c->fset[0] = FluidSet();
         c->fset[1] = FluidSet();
+        c->fchunks = nullptr;
+        c->nfchunks = 0;
         c->dcur = 0;
         c->initialized = false;
         const int nf = c->nf = s->n_fluid;
@@ -1125,12 +1147,9 @@ int dsph_upload_batch(dsph_ctx* c, const dsph_scene* scenes, int32_t n_scenes) {
             f.acc = dalloc<double4>(c, nf);
             f.orig = dalloc<int>(c, nf);
             f.scene = dalloc<int>(c, std::max(nf, 1));
-            const size_t rec = mx ? sizeof(float4) : sizeof(double4);
-            for (int b = 0; b < 2; b++) {
-                f.pv[b] = dalloc<char>(c, rec * std::max(nf, 1));
-                f.pq[b] = dalloc<char>(c, rec * std::max(nf, 1));
-                f.p0[b] = mx ? dalloc<char>(c, sizeof(uint4) * std::max(nf, 1)) : nullptr;
-            }
+            f.tid = dalloc<int>(c, nf);
+            const size_t rec = 2 * (mx ? sizeof(float4) : sizeof(double4));
+            for (int b = 0; b < 2; b++) f.rec[b] = dalloc<char>(c, rec * std::max(nf, 1));
             // fixed-point span of the mixed records: the domains of all
             // scenes plus one such extent on each side of every axis
             double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
@@ -1164,6 +1183,49 @@ int dsph_upload_batch(dsph_ctx* c, const dsph_scene* scenes, int32_t n_scenes) {
             h2d(c, f.acc, acc.data(), nf);
             h2d(c, f.orig, orig.data(), nf);
             if (nf) h2d(c, f.scene, M.fscene.data(), nf);
+            // the fluid property table: one entry per distinct (h, c, rho0,
+            // nu), with the particle mass m = (h / 1.3)^3 rho0
+            std::map<std::array<double, 4>, int> uniq;
+            std::vector<int> tid(nf);
+            for (int i = 0; i < nf; i++) {
+                const std::array<double, 4> key = {s->fluid_h[i], s->fluid_c[i], s->fluid_rho0[i],
+                                                   s->fluid_nu[i]};
+                tid[i] = uniq.emplace(key, (int)uniq.size()).first->second;
+            }
+            const size_t ntab = std::max<size_t>(uniq.size(), 1);
+            std::vector<double> t4(4 * ntab, 0.0), tn(ntab, 0.0);
+            for (const auto& kv : uniq) {
+                const double h = kv.first[0], h13 = h / 1.3;
+                const int k = kv.second;
+                t4[4 * k] = h;
+                t4[4 * k + 1] = kv.first[1];
+                t4[4 * k + 2] = h13 * h13 * h13 * kv.first[2];
+                t4[4 * k + 3] = kv.first[2];
+                tn[k] = kv.first[3];
+            }
+            if (mx) {
+                std::vector<float> f4(t4.begin(), t4.end()), fn(tn.begin(), tn.end());
+                float* d4 = dalloc<float>(c, 4 * ntab);
+                float* dn = dalloc<float>(c, ntab);
+                h2d(c, d4, f4.data(), f4.size());
+                h2d(c, dn, fn.data(), fn.size());
+                HIPCHK(hipStreamSynchronize(c->st));
+                for (FluidSet& g : c->fset) {
+                    g.tab = d4;
+                    g.tabnu = dn;
+                }
+            } else {
+                double* d4 = dalloc<double>(c, 4 * ntab);
+                double* dn = dalloc<double>(c, ntab);
+                h2d(c, d4, t4.data(), t4.size());
+                h2d(c, dn, tn.data(), tn.size());
+                for (FluidSet& g : c->fset) {
+                    g.tab = d4;
+                    g.tabnu = dn;
+                }
+            }
+            h2d(c, f.tid, tid.data(), nf);
+            HIPCHK(hipStreamSynchronize(c->st));
         }
         c->fslot = dalloc<int>(c, nf);
         h2d(c, c->fslot, orig.data(), nf);
@@ -1182,6 +1244,24 @@ int dsph_upload_batch(dsph_ctx* c, const dsph_scene* scenes, int32_t n_scenes) {
                     inv[gg.cell0 + h] = gg.cell0 + i1[h];
                 }
             }
+            // cells whose reference stencil (27 linear offsets) reaches the
+            // spare cell: their particles' lists follow the stencil walk
+            std::vector<unsigned char> sp(c->ncell, 0);
+            for (const GridP& gg : c->grids) {
+                const int64_t spare = gg.ncell - 1, gxy = (int64_t)gg.gx * gg.gy;
+                for (int64_t h = 0; h < spare; h++) {
+                    const int64_t x = h % gg.gx, y = (h / gg.gx) % gg.gy, z = h / gxy;
+                    bool hit = false;
+                    for (int zz = -1; zz <= 1; zz++)
+                        for (int yy = -1; yy <= 1; yy++)
+                            for (int xx = -1; xx <= 1; xx++)
+                                hit = hit || (z + zz) * gxy + (y + yy) * gg.gx + (x + xx) == spare;
+                    sp[gg.cell0 + h] = hit;
+                }
+                sp[gg.cell0 + spare] = 1;
+            }
+            c->cspare = dalloc<unsigned char>(c, c->ncell);
+            h2d(c, c->cspare, sp.data(), c->ncell);
             c->crank = dalloc<unsigned>(c, c->ncell);
             c->cinv = dalloc<unsigned>(c, c->ncell);
             h2d(c, c->crank, rank.data(), c->ncell);
@@ -1190,10 +1270,7 @@ int dsph_upload_batch(dsph_ctx* c, const dsph_scene* scenes, int32_t n_scenes) {
         c->fperm = dalloc<int>(c, nf);
         c->fcs = dalloc<int>(c, c->ncell);
         c->fce = dalloc<int>(c, c->ncell);
-        c->fblk_nb = dalloc<int>(c, c->ncell);
-        c->fblk_off = dalloc<int>(c, (size_t)c->ncell + 1);
-        c->fblocks = dalloc<int2>(c, std::max(nf, 1));
-        fill_int(c->st, c->fblk_off, (int64_t)c->ncell + 1, 0);
+        build_chunks(c);
         c->hslot = dalloc<unsigned>(c, nf);
         c->fxb = dalloc<double4>(c, nf);
         h2d(c, c->fxb, ph.data(), nf);

@@ -101,13 +101,17 @@ struct FluidSet {
     double4* acc = nullptr;
     int* orig = nullptr;                // slot -> caller index
     // neighbour records, ping-ponged with drho so the Jacobi update never
-    // overwrites what neighbours read.  FP64: pv = (v, rho), pq = (P, c,
-    // V = m/rho, gamma = nu rho) as f64x4, the position from ph.  Mixed:
-    // p0 = (x, y, z as 32-bit fixed point, h), pv = (v, P), pq = (c, V, rho,
-    // gamma), 16 bytes each.
-    void* p0[2] = {nullptr, nullptr};
-    void* pv[2] = {nullptr, nullptr};
-    void* pq[2] = {nullptr, nullptr};
+    // overwrites what neighbours read.  Two quads per slot:
+    //   FP64  (64 B): (x, y, z, table index), (v, dRho) as f64x4;
+    //   mixed (32 B): (x, y, z as 32-bit fixed point, table index),
+    //                 (v, dRho) as f32x4.
+    void* rec[2] = {nullptr, nullptr};
+    int* tid = nullptr;           // property-table index of each slot
+    // fluid property table, shared by all slots (and both sets): one entry
+    // per distinct (h, c, rho0, nu) of the scene(s), (h, c, m, rho0) as
+    // f64x4 / f32x4 in tab and nu in tabnu
+    const void* tab = nullptr;
+    const void* tabnu = nullptr;
     // fixed-point map of the mixed records: q = rint((x - qo) * qi), one unit
     // = qr metres per axis
     double qo[3] = {0, 0, 0};
@@ -227,7 +231,6 @@ void cell_spans(hipStream_t s, const unsigned* sorted, int64_t n, int* start, in
 struct SearchArgs {
     Grids G;
     int ncell;              // cells of all scenes
-    int bmem;               // particles per wavefront in the superset build (tunable, <= 64)
     int nf, ns;
     const double4* ph;      // fluid position + h, slot order
     const int* fcs;         // fluid cell ranges over sorted positions
@@ -235,6 +238,7 @@ struct SearchArgs {
     const int* fmember;     // slot of each sorted position
     const int* forig;       // caller index of each fluid slot
     const unsigned* hslot;  // reference hash of each fluid slot
+    const unsigned char* cspare;  // per cell: its 27-offset stencil reaches the spare cell
     const int* scs;         // solid cell ranges
     const int* sce;
     const int* sperm;
@@ -258,8 +262,10 @@ void reorder_fluid(hipStream_t s, const FluidSet& f, const FluidSet& scratch, co
                    int* slot_of, int n, bool mixed, int cur, const unsigned long long* ctl);
 void member_slots(hipStream_t s, const int* perm, const int* slot_of, const unsigned* sorted_rank,
                   const unsigned* cinv, int* member, unsigned* hslot, int n);
-void fluid_superset(hipStream_t s, const SearchArgs& a, const unsigned* cinv, int* nb, int* off,
-                    int2* blocks, int* scan_tmp, double4* fxb, double4* sxb);
+// chunks: (first, end) sorted positions of <= 64 consecutive particles of
+// one scene, one wavefront each
+void fluid_superset(hipStream_t s, const SearchArgs& a, const int2* chunks, int nchunk, double4* fxb,
+                    double4* sxb);
 // exact fluid neighbour rows of all particles for read-back (ELL, stride nf,
 // capacity max(K, supK)); out_cnt = -1 marks a walked row that overflowed
 void fluid_exact_rows(hipStream_t s, const SearchArgs& a, int* out, int* out_cnt);

@@ -54,6 +54,10 @@ constexpr double kWC = 21.0 / (16.0 * kPi);
 template <bool MIX> struct Prec { using R = double; using R4 = double4; };
 template <> struct Prec<true> { using R = float; using R4 = float4; };
 
+// Kahan-compensated FP32 sum.  add_if(on, x) is add(x) when on and no-op
+// otherwise, without a branch: the pair loops compute every candidate's
+// term and keep it only inside the kernel support, so the compiler cannot
+// sink a candidate's record loads behind its support test.
 struct KahanF {
     float s = 0.f, c = 0.f;
     __device__ __forceinline__ void add(float x) {
@@ -61,6 +65,13 @@ struct KahanF {
         float t = s + y;
         c = (t - s) - y;
         s = t;
+    }
+    __device__ __forceinline__ void add_if(bool on, float x) {
+        const float y = x - c;
+        const float t = s + y;
+        const float cn = (t - s) - y;
+        s = on ? t : s;
+        c = on ? cn : c;
     }
 };
 
@@ -73,18 +84,25 @@ template <class R>
 struct Acc1 {
     R s = 0;
     __device__ __forceinline__ void add(R x) { s += x; }
+    __device__ __forceinline__ void add_if(bool on, R x) { s = on ? s + x : s; }
     __device__ __forceinline__ R get() const { return s; }
 };
 template <>
 struct Acc1<float> {
     KahanF k;
     __device__ __forceinline__ void add(float x) { k.add(x); }
+    __device__ __forceinline__ void add_if(bool on, float x) { k.add_if(on, x); }
     __device__ __forceinline__ float get() const { return k.s; }
 };
 template <class R>
 struct Acc3 {
     R x = 0, y = 0, z = 0;
     __device__ __forceinline__ void add(R a, R b, R c) { x += a; y += b; z += c; }
+    __device__ __forceinline__ void add_if(bool on, R a, R b, R c) {
+        x = on ? x + a : x;
+        y = on ? y + b : y;
+        z = on ? z + c : z;
+    }
 };
 template <>
 struct Acc3<float> {
@@ -94,49 +112,80 @@ struct Acc3<float> {
         kx.add(a); ky.add(b); kz.add(c);
         x = kx.s; y = ky.s; z = kz.s;
     }
+    __device__ __forceinline__ void add_if(bool on, float a, float b, float c) {
+        kx.add_if(on, a); ky.add_if(on, b); kz.add_if(on, c);
+        x = kx.s; y = ky.s; z = kz.s;
+    }
 };
 
 // one particle as the pair loops see it; the position is FP64 (FP64 path)
-// or fixed point (mixed path)
+// or fixed point (mixed path).  Its record holds the position, the index of
+// its entry in the fluid property table, the velocity and dRho; the table
+// gives h, c, m, rho0 (and nu) and the rest follows: rho = rho0 + dRho,
+// P = c^2 dRho, V = m / rho, gamma = nu rho.
 template <bool MIX>
 struct Nb {
     using R = typename Prec<MIX>::R;
     using PT = typename std::conditional<MIX, unsigned, double>::type;
     PT px, py, pz;
-    R h, vx, vy, vz, rho, P, c, V, gamma;
+    int id;
+    R vx, vy, vz, drho;
+    R h, c, m, rho0, nu;
+    R rho, P, V, gamma;
 };
 
-// the position record (with h): all the support test needs
+// record quads of slot j: two 16-byte quads (mixed) or two f64x4 (FP64)
+template <bool MIX>
+__device__ __forceinline__ const typename Prec<MIX>::R4* rec_of(const FluidSet& f, int cur, int j) {
+    return (const typename Prec<MIX>::R4*)f.rec[cur] + 2 * (int64_t)j;
+}
+template <bool MIX>
+__device__ __forceinline__ typename Prec<MIX>::R4* rec_w(const FluidSet& f, int cur, int j) {
+    return (typename Prec<MIX>::R4*)f.rec[cur] + 2 * (int64_t)j;
+}
+
+// the position quad (position and table index)
 template <bool MIX>
 __device__ __forceinline__ void load_pos(const FluidSet& f, int cur, int j, Nb<MIX>& o) {
     if constexpr (MIX) {
-        const uint4 a = ((const uint4*)f.p0[cur])[j];
-        o.px = a.x; o.py = a.y; o.pz = a.z; o.h = __uint_as_float(a.w);
+        const uint4 a = *(const uint4*)rec_of<MIX>(f, cur, j);
+        o.px = a.x; o.py = a.y; o.pz = a.z; o.id = (int)a.w;
     } else {
-        const double4 p = f.ph[j];
-        o.px = p.x; o.py = p.y; o.pz = p.z; o.h = p.w;
+        const double4 a = rec_of<MIX>(f, cur, j)[0];
+        o.px = a.x; o.py = a.y; o.pz = a.z; o.id = (int)a.w;
     }
 }
-// the state records
+// the state quad (velocity, dRho)
 template <bool MIX>
 __device__ __forceinline__ void load_state(const FluidSet& f, int cur, int j, Nb<MIX>& o) {
-    if constexpr (MIX) {
-        const float4 b = ((const float4*)f.pv[cur])[j];
-        const float4 c = ((const float4*)f.pq[cur])[j];
-        o.vx = b.x; o.vy = b.y; o.vz = b.z; o.P = b.w;
-        o.c = c.x; o.V = c.y; o.rho = c.z; o.gamma = c.w;
-    } else {
-        const double4 a = ((const double4*)f.pv[cur])[j];
-        const double4 b = ((const double4*)f.pq[cur])[j];
-        o.vx = a.x; o.vy = a.y; o.vz = a.z; o.rho = a.w;
-        o.P = b.x; o.c = b.y; o.V = b.z; o.gamma = b.w;
-    }
+    const auto b = rec_of<MIX>(f, cur, j)[1];
+    o.vx = b.x; o.vy = b.y; o.vz = b.z; o.drho = b.w;
 }
-template <bool MIX>
+// the particle's property-table entry: h, c, m, rho0 and, with NU, nu
+template <bool MIX, bool NU>
+__device__ __forceinline__ void load_props(const FluidSet& f, Nb<MIX>& o) {
+    const auto t = ((const typename Prec<MIX>::R4*)f.tab)[o.id];
+    o.h = t.x; o.c = t.y; o.m = t.z; o.rho0 = t.w;
+    if constexpr (NU) o.nu = ((const typename Prec<MIX>::R*)f.tabnu)[o.id];
+}
+template <class R> __device__ __forceinline__ R div_(R a, R b) { return a / b; }
+template <> __device__ __forceinline__ float div_<float>(float a, float b) {
+    return a * __builtin_amdgcn_rcpf(b);
+}
+template <bool MIX, bool NU>
+__device__ __forceinline__ void derive(Nb<MIX>& o) {
+    o.rho = o.rho0 + o.drho;
+    o.P = o.drho * o.c * o.c;
+    o.V = div_(o.m, o.rho);
+    if constexpr (NU) o.gamma = o.nu * o.rho;
+}
+template <bool MIX, bool NU>
 __device__ __forceinline__ Nb<MIX> load_nb(const FluidSet& f, int cur, int j) {
     Nb<MIX> o;
     load_pos<MIX>(f, cur, j, o);
     load_state<MIX>(f, cur, j, o);
+    load_props<MIX, NU>(f, o);
+    derive<MIX, NU>(o);
     return o;
 }
 
@@ -162,27 +211,27 @@ __device__ __forceinline__ unsigned fixq(double x, double origin, double inv_uni
     t = t < 0.0 ? 0.0 : (t > 4294967295.0 ? 4294967295.0 : t);
     return (unsigned)t;
 }
-__device__ __forceinline__ uint4 fix_pos(const FluidSet& f, const double4& p) {
-    return make_uint4(fixq(p.x, f.qo[0], f.qi[0]), fixq(p.y, f.qo[1], f.qi[1]),
-                      fixq(p.z, f.qo[2], f.qi[2]), __float_as_uint((float)p.w));
+
+// the position quad of slot s from its FP64 position
+template <bool MIX>
+__device__ __forceinline__ void store_pos(const FluidSet& f, int buf, int s, const double4& p) {
+    auto* r = rec_w<MIX>(f, buf, s);
+    const int id = f.tid[s];
+    if constexpr (MIX)
+        *(uint4*)r = make_uint4(fixq(p.x, f.qo[0], f.qi[0]), fixq(p.y, f.qo[1], f.qi[1]),
+                                fixq(p.z, f.qo[2], f.qi[2]), (unsigned)id);
+    else
+        r[0] = make_double4(p.x, p.y, p.z, (double)id);
 }
 
-// the pair records of particle s from its FP64 masters (p = position, h)
+// the pair record of particle s from its FP64 masters (p = position)
 template <bool MIX>
 __device__ __forceinline__ void write_records(const FluidSet& f, int buf, int s, const double4& p,
-                                              double vx, double vy, double vz, double rho0,
-                                              double drho, double c, double nu) {
-    const double rho = rho0 + drho;
-    const double h13 = p.w / 1.3;
-    const double m = h13 * h13 * h13 * rho0;
-    if constexpr (MIX) {
-        ((uint4*)f.p0[buf])[s] = fix_pos(f, p);
-        ((float4*)f.pv[buf])[s] = make_float4((float)vx, (float)vy, (float)vz, (float)(drho * c * c));
-        ((float4*)f.pq[buf])[s] = make_float4((float)c, (float)(m / rho), (float)rho, (float)(nu * rho));
-    } else {
-        ((double4*)f.pv[buf])[s] = make_double4(vx, vy, vz, rho);
-        ((double4*)f.pq[buf])[s] = make_double4(drho * c * c, c, m / rho, nu * rho);
-    }
+                                              double vx, double vy, double vz, double drho) {
+    using R = typename Prec<MIX>::R;
+    store_pos<MIX>(f, buf, s, p);
+    auto* r = rec_w<MIX>(f, buf, s);
+    r[1] = {(R)vx, (R)vy, (R)vz, (R)drho};
 }
 
 // the neighbour row of slot s: the exact walked row or the Verlet superset;
@@ -196,6 +245,58 @@ __device__ __forceinline__ Row nb_row(const Lists& L, int s) {
     if (c >= 0) return {L.nbr + s, c < L.K ? c : L.K};
     const int m = L.sup_cnt[s];
     return {L.sup + s, m < L.supK ? m : L.supK};
+}
+
+// What a pair loop reads of each candidate: its position only (POS), its
+// whole state (STATE), or that and its viscosity (STATE_NU).
+enum NbMode { POS = 0, STATE = 1, STATE_NU = 2 };
+
+// Visits the row of slot s in batches of U candidates.  The batch's indices
+// (column-major, so coalesced over the wavefront) are fetched one batch
+// ahead and its U records are gathered before any of them is used: a
+// wavefront keeps U record gathers in flight instead of walking one
+// dependent index -> record chain per candidate.  Every load is issued
+// unconditionally (indices past the row end re-read its last entry and are
+// clamped into [0, nf)) so the loads of a batch overlap; fn(q, ok) sees
+// every loaded candidate in row order, ok = false past the row end, and must
+// not branch on it (keep its contribution with add_if).  The table entries
+// of a batch are read once its position quads are in; a table shared by
+// many particles is served to the whole wavefront from one cache line.
+constexpr int kBatch = 4;
+template <bool MIX, int U, NbMode MODE, class Fn>
+__device__ __forceinline__ void for_row(const FluidArgs& a, int s, Fn&& fn) {
+    const Row row = nb_row(a.L, s);
+    const int n = row.n;
+    const int last = n > 0 ? n - 1 : 0;
+    const int64_t st = a.nf;
+    const int jmax = a.nf - 1;
+    auto index = [&](int k) {
+        // read once per pass: streamed past the caches the records live in
+        const int j = __builtin_nontemporal_load(row.p + (int64_t)min(k, last) * st);
+        return min(max(j, 0), jmax);
+    };
+    int jn[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) jn[u] = index(u);
+    for (int k = 0; k < n; k += U) {
+        Nb<MIX> q[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            load_pos<MIX>(a.f, a.cur, jn[u], q[u]);
+            if constexpr (MODE != POS) load_state<MIX>(a.f, a.cur, jn[u], q[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) jn[u] = index(k + U + u);
+        if constexpr (MODE != POS) {
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                load_props<MIX, MODE == STATE_NU>(a.f, q[u]);
+                derive<MIX, MODE == STATE_NU>(q[u]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) fn(q[u], k + u < n);
+    }
 }
 
 template <class R>
@@ -281,8 +382,7 @@ __global__ void __launch_bounds__(kBS) k_records(FluidSet f, int n, int buf) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const double4 v = f.vr[s], p = f.ph[s];
-    const double2 cn = f.cn[s];
-    write_records<MIX>(f, buf, s, p, v.x, v.y, v.z, v.w, f.drho[buf][s], cn.x, cn.y);
+    write_records<MIX>(f, buf, s, p, v.x, v.y, v.z, f.drho[buf][s]);
 }
 
 // ------------------------------------------------------------- continuity
@@ -292,25 +392,19 @@ __global__ void __launch_bounds__(kMaxBS) k_continuity(FluidArgs a, double dt) {
     const int s = xcd_block() * blockDim.x + threadIdx.x;
     if (s >= a.nf) return;
     const FluidSet& f = a.f;
-    const Nb<MIX> o = load_nb<MIX>(f, a.cur, s);
+    const Nb<MIX> o = load_nb<MIX, false>(f, a.cur, s);
     const double4 ps = f.ph[s];
     Acc1<R> acc;
-    const Row row = nb_row(a.L, s);
-    for (int k = 0; k < row.n; k++) {
-        const int j = row.p[(int64_t)k * a.nf];
-        Nb<MIX> q;
-        load_pos<MIX>(f, a.cur, j, q);
+    for_row<MIX, kBatch, STATE>(a, s, [&](const Nb<MIX>& q, bool ok) {
         R rx, ry, rz;
         rel<MIX>(f, o, q, rx, ry, rz);
         const R r = dist<R>(rx * rx + ry * ry + rz * rz);
         const R F = grad_factor<R>(r, R(0.5) * (o.h + q.h));
-        if (F == R(0)) continue;
-        load_state<MIX>(f, a.cur, j, q);
         const R dvr = (o.vx - q.vx) * rx + (o.vy - q.vy) * ry + (o.vz - q.vz) * rz;
         const R c = R(0.5) * (o.c + q.c);
-        acc.add(R(2) * o.rho * q.V * F * (q.rho * dvr + (o.P - q.P) * r * rcp_(c)) *
-                rcp_(o.rho + q.rho));
-    }
+        acc.add_if(ok && F != R(0), R(2) * o.rho * q.V * F * (q.rho * dvr + (o.P - q.P) * r * rcp_(c)) *
+                                  rcp_(o.rho + q.rho));
+    });
     const double4 vm = f.vr[s];
     if (fs_valid(a.L, a.nf)) {
         const R rho0 = (R)vm.w;
@@ -331,8 +425,7 @@ __global__ void __launch_bounds__(kMaxBS) k_continuity(FluidArgs a, double dt) {
     }
     const double nd = f.drho[a.cur][s] + dt * (double)acc.get();
     f.drho[1 - a.cur][s] = nd;
-    const double2 cn = f.cn[s];
-    write_records<MIX>(f, 1 - a.cur, s, ps, vm.x, vm.y, vm.z, vm.w, nd, cn.x, cn.y);
+    write_records<MIX>(f, 1 - a.cur, s, ps, vm.x, vm.y, vm.z, nd);
 }
 
 // --------------------------------------------------------------- momentum
@@ -342,7 +435,7 @@ __global__ void __launch_bounds__(kMaxBS) k_momentum(FluidArgs a, int write_fs) 
     const int s = xcd_block() * blockDim.x + threadIdx.x;
     if (s >= a.nf) return;
     const FluidSet& f = a.f;
-    const Nb<MIX> o = load_nb<MIX>(f, a.cur, s);
+    const Nb<MIX> o = load_nb<MIX, true>(f, a.cur, s);
     const double4 ps = f.ph[s];
     const R two_over_rho = R(2) * rcp_(o.rho);
     // artificial viscosity (SPHKernelDEMContactModel.h:74-77), opt-in: the
@@ -351,19 +444,13 @@ __global__ void __launch_bounds__(kMaxBS) k_momentum(FluidArgs a, int write_fs) 
     const bool art = (a.flags & DSPH_FLAG_ART_VISCOSITY) != 0;
     const R alpha = art ? (R)f.alpha[s] : R(0);
     Acc3<R> acc;
-    const Row row = nb_row(a.L, s);
-    for (int k = 0; k < row.n; k++) {
-        const int j = row.p[(int64_t)k * a.nf];
-        Nb<MIX> q;
-        load_pos<MIX>(f, a.cur, j, q);
+    for_row<MIX, kBatch, STATE_NU>(a, s, [&](const Nb<MIX>& q, bool ok) {
         R rx, ry, rz;
         rel<MIX>(f, o, q, rx, ry, rz);
         const R r2 = rx * rx + ry * ry + rz * rz;
         const R ir = r2 > R(0) ? rsq_(r2) : R(0);
         const R r = r2 * ir;
         const R F = grad_factor<R>(r, R(0.5) * (o.h + q.h));
-        if (F == R(0)) continue;
-        load_state<MIX>(f, a.cur, j, q);
         const R c = R(0.5) * (o.c + q.c);
         const R dvx = o.vx - q.vx, dvy = o.vy - q.vy, dvz = o.vz - q.vz;
         // U_L - U_R = -(v_i - v_j) . e_ij
@@ -379,8 +466,9 @@ __global__ void __launch_bounds__(kMaxBS) k_momentum(FluidArgs a, int write_fs) 
             const R dvr = dvx * rx + dvy * ry + dvz * rz;
             sr += q.V * q.rho * alpha * hb * c * dvr * rcp_(rb * r2 + R(0.01) * hb * hb) * F;
         }
-        acc.add(pre * gbar * dvx + sr * rx, pre * gbar * dvy + sr * ry, pre * gbar * dvz + sr * rz);
-    }
+        acc.add_if(ok && F != R(0), pre * gbar * dvx + sr * rx, pre * gbar * dvy + sr * ry,
+                   pre * gbar * dvz + sr * rz);
+    });
     if (fs_valid(a.L, a.nf)) {
         const double4 vm = f.vr[s];
         const double h13 = ps.w / 1.3;
@@ -405,7 +493,7 @@ __global__ void __launch_bounds__(kMaxBS) k_fs_force(FluidArgs a) {
     if (s >= a.nf || !fs_valid(a.L, a.nf)) return;
     const int b0 = a.L.fsoff[s], e = a.L.fsoff[s + 1];
     if (b0 == e) return;
-    const Nb<MIX> o = load_nb<MIX>(a.f, a.cur, s);
+    const Nb<MIX> o = load_nb<MIX, true>(a.f, a.cur, s);
     const double4 ps = a.f.ph[s];
     const double4 vm = a.f.vr[s];
     const double h13 = ps.w / 1.3;
@@ -431,16 +519,16 @@ __global__ void __launch_bounds__(kMaxBS) k_density_reinit(FluidArgs a, int i_st
     const R sigma = R(kWC) * ih * ih * ih;
     Acc1<R> sw;
     sw.add(sigma);  // W(0, h)
-    const Row row = nb_row(a.L, s);
-    for (int k = 0; k < row.n; k++) {
-        const double4 q = f.ph[row.p[(int64_t)k * a.nf]];
-        const R rx = (R)(p.x - q.x), ry = (R)(p.y - q.y), rz = (R)(p.z - q.z);
+    Nb<MIX> o;
+    load_pos<MIX>(f, a.cur, s, o);
+    for_row<MIX, kBatch, POS>(a, s, [&](const Nb<MIX>& q, bool ok) {
+        R rx, ry, rz;
+        rel<MIX>(f, o, q, rx, ry, rz);
         const R qq = dist<R>(rx * rx + ry * ry + rz * rz) * ih;
-        if (qq >= R(2)) continue;
         const R t = R(1) - R(0.5) * qq;
         const R t2 = t * t;
-        sw.add(sigma * t2 * t2 * (R(1) + R(2) * qq));
-    }
+        sw.add_if(ok && qq < R(2), sigma * t2 * t2 * (R(1) + R(2) * qq));
+    });
     if (fs_valid(a.L, a.nf)) {
         const double h13 = p.w / 1.3;
         const R invV0 = rcp_((R)(h13 * h13 * h13));
@@ -466,8 +554,7 @@ __global__ void __launch_bounds__(kMaxBS) k_density_reinit(FluidArgs a, int i_st
         const double r1 = rho0 * sumw / f.sumw0[s];
         const double nd = (rs > r1 ? rs : r1) - rho0;
         f.drho[a.cur][s] = nd;
-        const double2 cn = f.cn[s];
-        write_records<MIX>(f, a.cur, s, p, vm.x, vm.y, vm.z, rho0, nd, cn.x, cn.y);
+        write_records<MIX>(f, a.cur, s, p, vm.x, vm.y, vm.z, nd);
     }
 }
 
@@ -482,9 +569,7 @@ __global__ void k_drift(FluidSet f, int n, double dt, int cur) {
     p.y += dt * v.y;
     p.z += dt * v.z;
     f.ph[s] = p;
-    if constexpr (MIX) {
-        ((uint4*)f.p0[cur])[s] = fix_pos(f, p);
-    }
+    store_pos<MIX>(f, cur, s, p);
 }
 
 // v += dt a (fluidVelocityIntegrate) then x += dt/2 v (fluidPositionIntegrate)
@@ -504,15 +589,12 @@ __global__ void k_kick_drift(FluidSet f, int n, double dt, int cur) {
     p.y += hd * v.y;
     p.z += hd * v.z;
     f.ph[s] = p;
-    using R4 = typename Prec<MIX>::R4;
     using R = typename Prec<MIX>::R;
-    R4* pv = (R4*)f.pv[cur];
-    R4 r = pv[s];
-    r.x = (R)v.x; r.y = (R)v.y; r.z = (R)v.z;
-    pv[s] = r;
-    if constexpr (MIX) {
-        ((uint4*)f.p0[cur])[s] = fix_pos(f, p);
-    }
+    auto* rec = rec_w<MIX>(f, cur, s);
+    rec[1].x = (R)v.x;
+    rec[1].y = (R)v.y;
+    rec[1].z = (R)v.z;
+    store_pos<MIX>(f, cur, s, p);
 }
 
 }  // namespace

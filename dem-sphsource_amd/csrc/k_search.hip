@@ -76,36 +76,6 @@ __device__ __forceinline__ bool within(double px, double py, double pz, double q
     return R - sqrt(d2) >= 0.0;
 }
 
-// squared distance from p to cell h's box, enlarged by pad (conservative)
-__device__ __forceinline__ double box_dist2(const GridP& g, int h, double px, double py,
-                                            double pz) {
-    const int gxy = g.gx * g.gy;
-    const int cz = h / gxy, rem = h - cz * gxy;
-    const int cy = rem / g.gx, cx = rem - cy * g.gx;
-    auto axis = [&](double p, double mn, double cs, int c) {
-        double lo = mn + c * cs - g.pad, hi = mn + (c + 1) * cs + g.pad;
-        double d = lo - p;
-        double e = p - hi;
-        d = d > e ? d : e;
-        return d > 0 ? d : 0.0;
-    };
-    double ax = axis(px, g.minx, g.csx, cx), ay = axis(py, g.miny, g.csy, cy),
-           az = axis(pz, g.minz, g.csz, cz);
-    return ax * ax + ay * ay + az * az;
-}
-
-// does the reference stencil of a particle at (x,y,z) include the spare cell?
-__device__ __forceinline__ bool stencil_has_spare(const GridP& g, double x, double y, double z) {
-    int ax, ay, az;
-    grid_pos(g, x, y, z, ax, ay, az);
-    const int spare = g.ncell - 1;
-    for (int zz = -1; zz <= 1; zz++)
-        for (int yy = -1; yy <= 1; yy++)
-            for (int xx = -1; xx <= 1; xx++)
-                if ((az + zz) * g.gy * g.gx + (ay + yy) * g.gx + (ax + xx) == spare) return true;
-    return false;
-}
-
 __device__ __forceinline__ void wave_max_atomic(unsigned long long* addr, unsigned long long v) {
 #pragma unroll
     for (int d = 32; d > 0; d >>= 1) {
@@ -232,9 +202,8 @@ __global__ void k_reorder(FluidSet src, FluidSet dst, const int* __restrict__ pe
     dst.scene[t] = src.scene[old];
     dst.acc[t] = src.acc[old];
     dst.orig[t] = o;
-    ((R4*)dst.pv[cur])[t] = ((const R4*)src.pv[cur])[old];
-    ((R4*)dst.pq[cur])[t] = ((const R4*)src.pq[cur])[old];
-    if (src.p0[cur]) ((uint4*)dst.p0[cur])[t] = ((const uint4*)src.p0[cur])[old];
+    for (int u = 0; u < 2; u++) ((R4*)dst.rec[cur])[2 * t + u] = ((const R4*)src.rec[cur])[2 * old + u];
+    dst.tid[t] = src.tid[old];
     slot_of[o] = t;
 }
 
@@ -253,9 +222,8 @@ __global__ void k_copyback(FluidSet dst, FluidSet src, int n, int cur,
     dst.scene[t] = src.scene[t];
     dst.acc[t] = src.acc[t];
     dst.orig[t] = src.orig[t];
-    ((R4*)dst.pv[cur])[t] = ((const R4*)src.pv[cur])[t];
-    ((R4*)dst.pq[cur])[t] = ((const R4*)src.pq[cur])[t];
-    if (src.p0[cur]) ((uint4*)dst.p0[cur])[t] = ((const uint4*)src.p0[cur])[t];
+    for (int u = 0; u < 2; u++) ((R4*)dst.rec[cur])[2 * t + u] = ((const R4*)src.rec[cur])[2 * t + u];
+    dst.tid[t] = src.tid[t];
 }
 
 // slot of every sorted position, and the reference hash of every slot
@@ -277,147 +245,175 @@ __global__ void k_store_xb(const double4* __restrict__ p, double4* __restrict__ 
 }
 
 // ------------------------------------------------------- cell blocks
-// every cell holding fluid is cut into blocks of <= bmem consecutive sorted
-// particles (a tunable <= 64), one wavefront each; blocks are listed in cell
-// rank order
-__global__ void k_blk_count(const int* __restrict__ cs, const int* __restrict__ ce,
-                            const unsigned* __restrict__ cinv, int ncell, int bmem,
-                            int* __restrict__ nb, const unsigned long long* __restrict__ ctl) {
-    if (ctl[CTL_REBUILD] == 0) return;
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= ncell) return;
-    const int c = (int)cinv[r];
-    const int st = cs[c];
-    nb[r] = st >= 0 ? (ce[c] - st + bmem - 1) / bmem : 0;
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) v = min(v, __shfl_xor(v, d, 64));
+    return __builtin_amdgcn_readfirstlane(v);
 }
-__global__ void k_blk_fill(const int* __restrict__ cs, const int* __restrict__ ce,
-                           const unsigned* __restrict__ cinv, int ncell, int bmem,
-                           const int* __restrict__ off, int2* __restrict__ blocks,
-                           const unsigned long long* __restrict__ ctl) {
-    if (ctl[CTL_REBUILD] == 0) return;
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= ncell) return;
-    const int c = (int)cinv[r];
-    const int st = cs[c];
-    if (st < 0) return;
-    const int n = ce[c] - st;
-    const int o = off[r];
-    for (int b = 0; b * bmem < n; b++) blocks[o + b] = make_int2(c, st + bmem * b);
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) v = max(v, __shfl_xor(v, d, 64));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+// distance from p to the slab [lo, lo + w] along one axis
+__device__ __forceinline__ float slab_dist(float p, float lo, float w) {
+    return fmaxf(fmaxf(lo - p, p - (lo + w)), 0.0f);
 }
 
-__device__ __forceinline__ double readlane_d(double v, int k) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)b, k);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-
-// Verlet superset build (setFluidInteractionsKernel's loop, radius + skin),
-// one wavefront per cell block.  Spare-cell particles and spare-cell
-// candidates are left out: they are walked exactly every step.  A candidate
-// cell is read 64 members at a time with one coalesced load per lane and
-// then broadcast lane by lane (readlane), so the scan over the members has
-// no memory latency in its loop.  Membership is d^2 <= (R (1 + 1e-9))^2 with
-// R = 2 max(h_i, h_j) + skin (or 2 h_i + skin for solids): a superset only
-// has to contain every pair that can come within range before the next
-// rebuild, so it errs on the inclusive side.
-__global__ void __launch_bounds__(kBS) k_build(SearchArgs a, const int2* __restrict__ blocks,
-                                               const int* __restrict__ nblk_p) {
+// Verlet superset build (setFluidInteractionsKernel's loop, radius + skin).
+// One wavefront per chunk of <= 64 consecutive sorted particles of one scene
+// (Morton cell order, so a chunk covers a few neighbouring cells), one lane
+// per particle.  The wavefront walks the cells of the chunk's bounding box
+// widened by the superset stencil half-width, skipping every z slab, y row
+// and cell that no lane can reach.  The members of a cell in reach are
+// staged 64 at a time through the wavefront's LDS slice (one coalesced load
+// per lane) and then read back by every lane at the same address, so the
+// membership scan is a broadcast LDS read and a few FP32 operations per
+// candidate, with no global-memory latency in its loop.  Geometry is FP32
+// relative to a corner of the box (a few cells away: ~1e-8 m rounding);
+// membership is d^2 <= R^2 (1 + 2e-4) with R = 2 max(h_i, h_j) + skin (2 h_i
+// + skin for solids), evaluated as max((2 h_i + skin)^2, (2 h_j + skin)^2).
+// A superset only has to contain every pair that can come within range
+// before the next rebuild, so the margin errs on the inclusive side.
+// Spare-cell particles and spare-cell candidates are left out: they are
+// walked exactly every step.
+__global__ void __launch_bounds__(kBS) k_build(SearchArgs a, const int2* __restrict__ chunks,
+                                               int nchunk) {
     if (a.ctl[CTL_REBUILD] == 0) return;
+    __shared__ float4 lq[kBS];  // candidate (x, y, z, reach^2), one 64-slot slice per wavefront
+    __shared__ int lj[kBS];
     const int lane = threadIdx.x & 63;
+    const int wbase = threadIdx.x & ~63;
     const int wv = xcd_block() * (kBS / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nwv = gridDim.x * (kBS / 64);
-    const int nblk = *nblk_p;
-    const double grow = (1.0 + 1e-9) * (1.0 + 1e-9);
     const int supK = a.L.supK, fsupK = a.L.fsupK;
+    const float grow = 1.0f + 2e-4f;
     bool ovf = false;
-    for (int b = wv; b < nblk; b += nwv) {
-        const int2 blk = blocks[b];
-        const int cell = __builtin_amdgcn_readfirstlane(blk.x);
-        const int t0 = __builtin_amdgcn_readfirstlane(blk.y);
-        const int nmem = min(a.bmem, a.fce[cell] - t0);
-        const bool act = lane < nmem;
-        const int s = a.fmember[t0 + (act ? lane : 0)];
-        // the block's cell, and so its scene, is the same for every lane
+    for (int b = wv; b < nchunk; b += nwv) {
+        const int2 ch = chunks[b];
+        const int t0 = __builtin_amdgcn_readfirstlane(ch.x);
+        const int t1 = __builtin_amdgcn_readfirstlane(ch.y);
+        bool act = t0 + lane < t1;
+        const int s = a.fmember[act ? t0 + lane : t0];
+        // a chunk never spans two scenes
         const int sc = __builtin_amdgcn_readfirstlane(fluid_scene(a.G, a.fmember[t0]));
         const GridP& g = grid_at(a.G, sc);
-        const int lc = cell - g.cell0;  // the cell in its scene's own indices
         const int spare = g.ncell - 1;
-        const double skin = g.skin;
-        if (lc == spare) {  // walked exactly while out of the domain
-            if (act) {
-                a.L.sup_cnt[s] = 0;
-                a.L.fsup_cnt[s] = 0;
-            }
-            continue;
+        const int lc = (int)a.hslot[s] - g.cell0;  // the cell in its scene's own indices
+        if (act && lc == spare) {  // walked exactly while out of the domain
+            a.L.sup_cnt[s] = 0;
+            a.L.fsup_cnt[s] = 0;
+            act = false;
         }
-        const double4 p = a.ph[s];
-        const double hi = p.w;
-        const double Rf = (2.0 * fmax(hi, g.hmax) + skin) * (1.0 + 1e-9);
-        const double Rs = (2.0 * hi + skin) * (1.0 + 1e-9);
-        const double Rs2 = (2.0 * hi + skin) * (2.0 * hi + skin) * grow;
-        int c = 0, cs = 0;
         const int gxy = g.gx * g.gy;
         const int cz = lc / gxy, rem = lc - cz * gxy;
         const int cy = rem / g.gx, cx = rem - cy * g.gx;
+        const int x0 = wave_min(act ? cx : 0x7fffffff);
+        if (x0 == 0x7fffffff) continue;  // no lane in a real cell
         const int w = g.sw;
-        for (int zz = -w; zz <= w; zz++)
-            for (int yy = -w; yy <= w; yy++)
-                for (int xx = -w; xx <= w; xx++) {
-                    const int hl = (cz + zz) * gxy + (cy + yy) * g.gx + (cx + xx);
-                    if (hl < 0 || hl >= spare) continue;
-                    const int h = g.cell0 + hl;
-                    const double bd2 = box_dist2(g, hl, p.x, p.y, p.z);
-                    if (__ballot(act && bd2 <= Rf * Rf) != 0) {
+        const int X0 = max(x0 - w, 0), X1 = min(wave_max(act ? cx : -1) + w, g.gx - 1);
+        const int Y0 = max(wave_min(act ? cy : 0x7fffffff) - w, 0);
+        const int Y1 = min(wave_max(act ? cy : -1) + w, g.gy - 1);
+        const int Z0 = max(wave_min(act ? cz : 0x7fffffff) - w, 0);
+        const int Z1 = min(wave_max(act ? cz : -1) + w, g.gz - 1);
+        // the box corner all FP32 coordinates of this chunk are taken from
+        const double ox = g.minx + X0 * g.csx, oy = g.miny + Y0 * g.csy, oz = g.minz + Z0 * g.csz;
+        const double4 p = a.ph[s];
+        const float px = (float)(p.x - ox), py = (float)(p.y - oy), pz = (float)(p.z - oz);
+        const float skin = (float)g.skin;
+        const float Ri = 2.0f * (float)p.w + skin;
+        const float Ai = Ri * Ri * grow;  // own reach^2: the solid test, and i's side of max
+        const float Rf = 2.0f * fmaxf((float)p.w, (float)g.hmax) + skin;  // any fluid pair of i
+        const float cullf = act ? Rf * Rf * grow : -1.0f, culls = act ? Ai : -1.0f;
+        const float csx = (float)g.csx, csy = (float)g.csy, csz = (float)g.csz;
+        const float pad = (float)g.pad;
+        int c = 0, cs = 0;
+        for (int zc = Z0; zc <= Z1; zc++) {
+            const float dz = slab_dist(pz, (zc - Z0) * csz - pad, csz + 2.0f * pad);
+            const float dz2 = dz * dz;
+            if (__ballot(dz2 <= cullf) == 0) continue;
+            for (int yc = Y0; yc <= Y1; yc++) {
+                const float dy = slab_dist(py, (yc - Y0) * csy - pad, csy + 2.0f * pad);
+                const float dzy2 = dz2 + dy * dy;
+                if (__ballot(dzy2 <= cullf) == 0) continue;
+                for (int xc = X0; xc <= X1; xc++) {
+                    const float dx = slab_dist(px, (xc - X0) * csx - pad, csx + 2.0f * pad);
+                    const float d2 = dzy2 + dx * dx;
+                    const int h = g.cell0 + zc * gxy + yc * g.gx + xc;
+                    if (__ballot(d2 <= cullf) != 0) {
                         const int st0 = __builtin_amdgcn_readfirstlane(a.fcs[h]);
                         const int en = __builtin_amdgcn_readfirstlane(a.fce[h]);
                         for (int base = st0; st0 >= 0 && base < en; base += 64) {
                             const int u = base + lane;
-                            int jl = -1;
-                            double4 ql = make_double4(0, 0, 0, 0);
                             if (u < en) {
-                                jl = a.fmember[u];
-                                ql = a.ph[jl];
+                                const int j = a.fmember[u];
+                                const double4 q = a.ph[j];
+                                const float Rj = 2.0f * (float)q.w + skin;
+                                lq[wbase + lane] = make_float4((float)(q.x - ox), (float)(q.y - oy),
+                                                               (float)(q.z - oz), Rj * Rj * grow);
+                                lj[wbase + lane] = j;
                             }
+                            __builtin_amdgcn_wave_barrier();
                             const int nb = min(64, en - base);
-                            for (int k = 0; k < nb; k++) {
-                                const int j = __builtin_amdgcn_readlane(jl, k);
-                                const double qx = readlane_d(ql.x, k), qy = readlane_d(ql.y, k),
-                                             qz = readlane_d(ql.z, k), qh = readlane_d(ql.w, k);
-                                const double dx = p.x - qx, dy = p.y - qy, dz = p.z - qz;
-                                const double R = 2.0 * fmax(hi, qh) + skin;
-                                if (act && j != s && dx * dx + dy * dy + dz * dz <= R * R * grow) {
-                                    if (c < supK) a.L.sup[(int64_t)c * a.nf + s] = j;
-                                    c++;
+                            for (int k = 0; k < nb; k += 4) {
+                                // four broadcast reads in flight; slots past nb
+                                // (within the slice) are read but not used
+                                float4 q[4];
+                                int jj[4];
+#pragma unroll
+                                for (int v = 0; v < 4; v++) {
+                                    q[v] = lq[wbase + ((k + v) & 63)];
+                                    jj[v] = lj[wbase + ((k + v) & 63)];
+                                }
+#pragma unroll
+                                for (int v = 0; v < 4; v++) {
+                                    const float ex = px - q[v].x, ey = py - q[v].y, ez = pz - q[v].z;
+                                    const float e2 = ex * ex + ey * ey + ez * ez;
+                                    const bool in = act & (k + v < nb) & (jj[v] != s) &
+                                                    (e2 <= (Ai > q[v].w ? Ai : q[v].w));
+                                    if (in && c < supK) a.L.sup[(int64_t)c * a.nf + s] = jj[v];
+                                    c += in;
                                 }
                             }
+                            __builtin_amdgcn_wave_barrier();
                         }
                     }
-                    if (__ballot(act && bd2 <= Rs * Rs) != 0) {
+                    if (__ballot(d2 <= culls) != 0) {
                         const int st1 = __builtin_amdgcn_readfirstlane(a.scs[h]);
                         const int en = __builtin_amdgcn_readfirstlane(a.sce[h]);
                         for (int base = st1; st1 >= 0 && base < en; base += 64) {
                             const int u = base + lane;
-                            int al = -1;
-                            double4 ql = make_double4(0, 0, 0, 0);
                             if (u < en) {
-                                al = a.sperm[u];
-                                ql = a.spr[al];
+                                const int ai = a.sperm[u];
+                                const double4 q = a.spr[ai];
+                                lq[wbase + lane] = make_float4((float)(q.x - ox), (float)(q.y - oy),
+                                                               (float)(q.z - oz), 0.0f);
+                                lj[wbase + lane] = ai;
                             }
+                            __builtin_amdgcn_wave_barrier();
                             const int nb = min(64, en - base);
-                            for (int k = 0; k < nb; k++) {
-                                const int ai = __builtin_amdgcn_readlane(al, k);
-                                const double qx = readlane_d(ql.x, k), qy = readlane_d(ql.y, k),
-                                             qz = readlane_d(ql.z, k);
-                                const double dx = p.x - qx, dy = p.y - qy, dz = p.z - qz;
-                                if (act && dx * dx + dy * dy + dz * dz <= Rs2) {
-                                    if (cs < fsupK) a.L.fsup[(int64_t)cs * a.nf + s] = ai;
-                                    cs++;
+                            for (int k = 0; k < nb; k += 4) {
+                                float4 q[4];
+                                int aa[4];
+#pragma unroll
+                                for (int v = 0; v < 4; v++) {
+                                    q[v] = lq[wbase + ((k + v) & 63)];
+                                    aa[v] = lj[wbase + ((k + v) & 63)];
+                                }
+#pragma unroll
+                                for (int v = 0; v < 4; v++) {
+                                    const float ex = px - q[v].x, ey = py - q[v].y, ez = pz - q[v].z;
+                                    const bool in = (k + v < nb) & (ex * ex + ey * ey + ez * ez <= culls);
+                                    if (in && cs < fsupK) a.L.fsup[(int64_t)cs * a.nf + s] = aa[v];
+                                    cs += in;
                                 }
                             }
+                            __builtin_amdgcn_wave_barrier();
                         }
                     }
                 }
+            }
+        }
         if (act) {
             a.L.sup_cnt[s] = c;
             a.L.fsup_cnt[s] = cs;
@@ -556,7 +552,7 @@ __global__ void __launch_bounds__(kBS) k_filter(SearchArgs a) {
         const GridP& g = grid_at(a.G, fluid_scene(a.G, s));
         const unsigned spare = (unsigned)(g.cell0 + g.ncell - 1);  // the scene's spare cell
         const bool walk = a.ctl[CTL_WALK_ALL] != 0 || g.walk_only || a.hslot[s] == spare ||
-                          stencil_has_spare(g, p.x, p.y, p.z);
+                          a.cspare[a.hslot[s]] != 0;
         int64_t w = FILL ? (int64_t)a.L.fsoff[s] : 0;
         if (walk) {
             lane_walk<FILL>(a, p, s, c, cs, w);
@@ -876,24 +872,17 @@ void member_slots(hipStream_t s, const int* perm, const int* slot_of, const unsi
                            cinv, member, hslot, n);
 }
 
-static int wave_grid(int nf) {
-    // about one wavefront per cell block (4 per workgroup); waves stride over
-    // the block list, which lives on the device
-    int b = (nf + 127) / 128;
-    return b < 1 ? 1 : (b > 16384 ? 16384 : b);
+static int chunk_grid(int nchunk) {
+    // one wavefront per chunk, 4 per workgroup; beyond the cap the waves
+    // stride over the chunk list
+    const int b = (nchunk + 3) / 4;
+    return b < 1 ? 1 : (b > 65535 ? 65535 : b);
 }
 
-void fluid_superset(hipStream_t s, const SearchArgs& a, const unsigned* cinv, int* nb, int* off,
-                    int2* blocks, int* scan_tmp, double4* fxb, double4* sxb) {
+void fluid_superset(hipStream_t s, const SearchArgs& a, const int2* chunks, int nchunk, double4* fxb,
+                    double4* sxb) {
     if (a.nf <= 0) return;
-    const int nc = a.ncell;
-    hipLaunchKernelGGL(k_blk_count, dim3(blocks_for(nc)), dim3(kBS), 0, s, a.fcs, a.fce, cinv, nc,
-                       a.bmem, nb, (const unsigned long long*)a.ctl);
-    exclusive_scan(s, nb, off, nc, scan_tmp);
-    hipLaunchKernelGGL(k_blk_fill, dim3(blocks_for(nc)), dim3(kBS), 0, s, a.fcs, a.fce, cinv, nc,
-                       a.bmem, (const int*)off, blocks, (const unsigned long long*)a.ctl);
-    hipLaunchKernelGGL(k_build, dim3(wave_grid(a.nf)), dim3(kBS), 0, s, a, (const int2*)blocks,
-                       (const int*)(off + nc));
+    hipLaunchKernelGGL(k_build, dim3(chunk_grid(nchunk)), dim3(kBS), 0, s, a, chunks, nchunk);
     hipLaunchKernelGGL(k_store_xb, dim3(blocks_for(a.nf)), dim3(kBS), 0, s, a.ph, fxb, a.nf,
                        (const unsigned long long*)a.ctl);
     if (a.ns > 0)
