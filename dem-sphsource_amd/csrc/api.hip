@@ -380,9 +380,13 @@ SearchArgs search_args(dsph_ctx* c) {
     return a;
 }
 
-// neighborSearch (neighborSearch.cu:314-381): reference grid of both species
-// every step; on fluid steps the Verlet superset (rebuilt when needed, with
-// the sorted-order reorder) and the exact lists; DEM contacts every step.
+// neighborSearch (neighborSearch.cu:314-381): the reference hashes of both
+// species every step; on fluid steps the Verlet superset (rebuilt when
+// needed, with the sorted-order reorder) and the exact lists; DEM contacts
+// every step.  The cell-sorted grids are only read by a superset build and
+// by exact stencil walks, so the sorts run on the steps that have one (a
+// device-side decision, CTL_SORT); the reference grid of any step is
+// reconstructed from its hashes on read-back.
 void neighbor_search(dsph_ctx* c, int i_step, int gap, bool save_history) {
     hipStream_t s = c->st;
     long long* stw = (long long*)c->status;
@@ -390,24 +394,27 @@ void neighbor_search(dsph_ctx* c, int i_step, int gap, bool save_history) {
     {
         FamTimer t(c, F_GRID);
         const Grids G = grids_view(c);
-        hash_fluid(s, G, c->fset[0].ph, c->fset[0].orig, c->nf, c->crank, c->fkey, c->frank, stw);
+        hash_fluid(s, G, c->fset[0].ph, c->fset[0].orig, c->nf, c->crank, c->cspare, c->fkey, c->frank,
+                   c->hslot, c->ctl, stw);
         hash_solid(s, G, c->so.pr, c->ns, c->skey, stw);
-        // ranges of an empty species are filled with -1 as well
-        if (c->nf > 0)
-            radix_sort_iota(s, c->frank, c->nf, c->key_bits, c->fkey_sorted, c->fperm, c->radix_tmp);
-        cell_spans(s, c->fkey_sorted, c->nf, c->fcs, c->fce, c->ncell, c->cinv);
-        if (c->ns > 0)
-            radix_sort_iota(s, c->skey, c->ns, c->key_bits, c->skey_sorted, c->sperm, c->radix_tmp);
-        cell_spans(s, c->skey_sorted, c->ns, c->scs, c->sce, c->ncell, nullptr);
         if (fluid_step) {
+            bool walk_only = false;
+            for (const GridP& g : c->grids) walk_only = walk_only || g.walk_only;
             rebuild_decision(s, G, c->fset[0].ph, c->fxb, c->nf, c->so.pr, c->sxb, c->ns,
-                             c->skin_h <= 0.0, c->ctl, stw);
+                             c->skin_h <= 0.0, walk_only, c->ctl, stw);
+            const Gate gate = c->ctl + CTL_SORT;
+            radix_sort_iota(s, c->frank, c->nf, c->key_bits, c->fkey_sorted, c->fperm, c->radix_tmp, gate);
+            cell_spans(s, c->fkey_sorted, c->nf, c->fcs, c->fce, c->ncell, c->cinv, gate);
+            if (c->ns > 0)
+                radix_sort_iota(s, c->skey, c->ns, c->key_bits, c->skey_sorted, c->sperm, c->radix_tmp,
+                                gate);
+            cell_spans(s, c->skey_sorted, c->ns, c->scs, c->sce, c->ncell, nullptr, gate);
             if (c->reorder_period > 0)
                 reorder_fluid(s, c->fset[0], c->fset[1], c->fperm, c->fslot, c->nf, c->mixed(),
                               c->dcur, c->ctl);
+            member_slots(s, c->fperm, c->fslot, c->fkey_sorted, c->cinv, c->fmember, c->hslot, c->nf,
+                         gate);
         }
-        if (c->nf > 0)
-            member_slots(s, c->fperm, c->fslot, c->fkey_sorted, c->cinv, c->fmember, c->hslot, c->nf);
     }
     if (fluid_step) {
         const SearchArgs sa = search_args(c);
@@ -1562,8 +1569,13 @@ int dsph_initialize(dsph_ctx* c, int compute_normals) {
             h2d(c, c->B.Fn, z.data(), nb);
             h2d(c, c->B.Tt, z.data(), nb);
         }
-        if (compute_normals)
+        if (compute_normals && c->ns > 0) {
+            // the solid grid of the initial positions (the gated search sorts
+            // only on fluid steps)
+            radix_sort_iota(c->st, c->skey, c->ns, c->key_bits, c->skey_sorted, c->sperm, c->radix_tmp);
+            cell_spans(c->st, c->skey_sorted, c->ns, c->scs, c->sce, c->ncell, nullptr);
             solid_normals(c->st, grids_view(c), c->so, c->ns, c->scs, c->sce, c->sperm);
+        }
         HIPCHK(hipStreamSynchronize(c->st));
         invalidate_graphs(c);
         c->initialized = true;
@@ -2053,19 +2065,12 @@ int dsph_download_grid(dsph_ctx* c, int species, int32_t* hash, int32_t* perm, i
     API_GUARD(c);
     if (species != 0 && species != 1) return DSPH_E_INVALID;
     try {
+        // the device sorts only when a build or an exact walk needs it (and
+        // the fluid by Morton cell rank); the reference's arrays of the last
+        // search (stable sort by hash) are rebuilt from its hashes
         const int n = species == 0 ? c->nf : c->ns;
-        if (species == 1) {
-            if (hash) d2h(c, (unsigned*)hash, c->skey, n);
-            if (perm) d2h(c, perm, c->sperm, n);
-            if (start) d2h(c, start, c->scs, c->ncell);
-            if (end) d2h(c, end, c->sce, c->ncell);
-            HIPCHK(hipStreamSynchronize(c->st));
-            return DSPH_OK;
-        }
-        // the fluid is sorted by Morton cell rank on the device; the
-        // reference's arrays (stable sort by hash) are rebuilt from the hashes
         std::vector<unsigned> key(n);
-        d2h(c, key.data(), c->fkey, n);
+        d2h(c, key.data(), species == 0 ? c->fkey : c->skey, n);
         HIPCHK(hipStreamSynchronize(c->st));
         std::vector<int> cnt((size_t)c->ncell + 1, 0), p(n);
         for (int i = 0; i < n; i++) cnt[key[i] + 1]++;

@@ -49,6 +49,8 @@ enum {
     CTL_REBUILD = 3,  // this search rebuilds the superset
     CTL_WALK_ALL = 4, // some particle is out of the domain: walk the stencil exactly
     CTL_OUT_AT_BUILD = 5,  // the last build had particles out of the domain
+    CTL_NEAR_SPARE = 6,    // some fluid particle's stencil reaches the spare cell
+    CTL_SORT = 7,          // this search sorts the grids (rebuild or exact walks)
     CTL_WORDS = 8
 };
 
@@ -213,19 +215,22 @@ struct Bonds {
 
 // ---- launch wrappers (implemented in the .hip files) -----------------------
 // generic
-void fill_int(hipStream_t s, int* p, int64_t n, int v);
+// gate: when non-null, the kernels do nothing unless *gate != 0 (a device
+// flag, so the decision can be taken inside a captured step)
+using Gate = const unsigned long long*;
+void fill_int(hipStream_t s, int* p, int64_t n, int v, Gate gate = nullptr);
 void copy_int(hipStream_t s, int* dst, const int* src, int64_t n);
 // exclusive scan of n ints into out[0..n] (out[n] = total); tmp >= scan_tmp_size(n)
 int64_t scan_tmp_size(int64_t n);
-void exclusive_scan(hipStream_t s, const int* in, int* out, int64_t n, int* tmp);
+void exclusive_scan(hipStream_t s, const int* in, int* out, int64_t n, int* tmp, Gate gate = nullptr);
 // stable LSD radix sort of (key, index) pairs; keys < 2^bits; vals out = perm
 int64_t radix_tmp_size(int64_t n);
 void radix_sort_iota(hipStream_t s, const unsigned* keys_in, int64_t n, int bits,
-                     unsigned* keys_out, int* perm_out, unsigned* tmp);
+                     unsigned* keys_out, int* perm_out, unsigned* tmp, Gate gate = nullptr);
 // start/end of each cell over sorted keys; with inv, the keys are cell
 // ranks and inv maps a rank to its cell
 void cell_spans(hipStream_t s, const unsigned* sorted, int64_t n, int* start, int* end,
-                int ncell, const unsigned* inv);
+                int ncell, const unsigned* inv, Gate gate = nullptr);
 
 // search (k_search.hip)
 struct SearchArgs {
@@ -249,19 +254,23 @@ struct SearchArgs {
     unsigned long long* ctl;
     unsigned long long* st;
 };
+// also the reference hash of every slot, and CTL_NEAR_SPARE when a particle's
+// cell has a stencil reaching the spare cell
 void hash_fluid(hipStream_t s, const Grids& G, const double4* ph, const int* orig, int n,
-                const unsigned* crank, unsigned* key_orig, unsigned* rank_orig, long long* status);
+                const unsigned* crank, const unsigned char* cspare, unsigned* key_orig,
+                unsigned* rank_orig, unsigned* hslot, unsigned long long* ctl, long long* status);
 void hash_solid(hipStream_t s, const Grids& G, const double4* pr, int n, unsigned* key,
                 long long* status);
 // displacements since the last build in units of each particle's scene skin;
 // always != 0 rebuilds at every search
+// sets CTL_REBUILD, and CTL_SORT = rebuild, exact walks or always_sort
 void rebuild_decision(hipStream_t s, const Grids& G, const double4* fph, const double4* fxb, int nf,
-                      const double4* spr, const double4* sxb, int ns, int always,
+                      const double4* spr, const double4* sxb, int ns, int always, int always_sort,
                       unsigned long long* ctl, long long* status);
 void reorder_fluid(hipStream_t s, const FluidSet& f, const FluidSet& scratch, const int* perm,
                    int* slot_of, int n, bool mixed, int cur, const unsigned long long* ctl);
 void member_slots(hipStream_t s, const int* perm, const int* slot_of, const unsigned* sorted_rank,
-                  const unsigned* cinv, int* member, unsigned* hslot, int n);
+                  const unsigned* cinv, int* member, unsigned* hslot, int n, Gate gate);
 // chunks: (first, end) sorted positions of <= 64 consecutive particles of
 // one scene, one wavefront each
 void fluid_superset(hipStream_t s, const SearchArgs& a, const int2* chunks, int nchunk, double4* fxb,

@@ -92,10 +92,12 @@ __global__ void k_status_reset(unsigned long long* st) {
 }
 
 __global__ void k_hash_fluid(Grids G, const double4* __restrict__ ph, const int* __restrict__ orig,
-                             int n, const unsigned* __restrict__ crank, unsigned* __restrict__ key,
-                             unsigned* __restrict__ rank, unsigned long long* st) {
+                             int n, const unsigned* __restrict__ crank,
+                             const unsigned char* __restrict__ cspare, unsigned* __restrict__ key,
+                             unsigned* __restrict__ rank, unsigned* __restrict__ hslot,
+                             unsigned long long* ctl, unsigned long long* st) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned long long out = 0;
+    bool out = false, near = false;
     if (s < n) {
         const GridP& g = grid_at(G, fluid_scene(G, s));
         double4 p = ph[s];
@@ -104,10 +106,13 @@ __global__ void k_hash_fluid(Grids G, const double4* __restrict__ ph, const int*
         const int o = orig[s];
         key[o] = (unsigned)h;
         rank[o] = crank[h];
+        hslot[s] = (unsigned)h;
         out = hl == g.ncell - 1;
+        near = cspare[h] != 0;
     }
-    unsigned long long m = __ballot(out != 0);
+    const unsigned long long m = __ballot(out);
     if ((threadIdx.x & 63) == 0 && m) atomicAdd(&st[ST_OUT_OF_DOMAIN], (unsigned long long)__popcll(m));
+    if ((threadIdx.x & 63) == 0 && __ballot(near) != 0) atomicOr(&ctl[CTL_NEAR_SPARE], 1ull);
 }
 
 __global__ void k_hash_solid(Grids G, const double4* __restrict__ pr, int n,
@@ -164,7 +169,8 @@ __global__ void __launch_bounds__(kBS) k_disp(Grids G, bool fluid, const double4
 
 // rebuild when two particles may have closed the skin: d_f + max(d_f, d_s)
 // >= skin, with displacements in skin units (each scene its own skin)
-__global__ void k_decide(unsigned long long* ctl, unsigned long long* st, int always) {
+__global__ void k_decide(unsigned long long* ctl, unsigned long long* st, int always,
+                         int always_sort) {
     const double df = __longlong_as_double((long long)ctl[CTL_DISP_F]);
     const double ds = __longlong_as_double((long long)ctl[CTL_DISP_S]);
     const bool out = st[ST_OUT_OF_DOMAIN] != 0;
@@ -176,6 +182,9 @@ __global__ void k_decide(unsigned long long* ctl, unsigned long long* st, int al
         (ctl[CTL_FORCE] != 0 || out || stale || always || move) ? 1ull : 0ull;
     ctl[CTL_REBUILD] = r;
     ctl[CTL_WALK_ALL] = out ? 1ull : 0ull;
+    // the grids are sorted for a build and for exact stencil walks
+    ctl[CTL_SORT] = (r || ctl[CTL_NEAR_SPARE] != 0 || always_sort) ? 1ull : 0ull;
+    ctl[CTL_NEAR_SPARE] = 0;
     if (r) ctl[CTL_OUT_AT_BUILD] = out ? 1ull : 0ull;
     ctl[CTL_FORCE] = 0;
     ctl[CTL_DISP_F] = 0;
@@ -226,10 +235,12 @@ __global__ void k_copyback(FluidSet dst, FluidSet src, int n, int cur,
     dst.tid[t] = src.tid[t];
 }
 
-// slot of every sorted position, and the reference hash of every slot
+// slot of every sorted position and, after a reorder, the reference hash of
+// every slot in its new place (without one the hash kernel's are current)
 __global__ void k_member(const int* __restrict__ perm, const int* __restrict__ slot_of,
                          const unsigned* __restrict__ sorted_rank, const unsigned* __restrict__ cinv,
-                         int* __restrict__ member, unsigned* __restrict__ hslot, int n) {
+                         int* __restrict__ member, unsigned* __restrict__ hslot, int n, Gate gate) {
+    if (*gate == 0) return;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int s = slot_of[perm[t]];
@@ -560,20 +571,37 @@ __global__ void __launch_bounds__(kBS) k_filter(SearchArgs a) {
             const double hi = p.w;
             const int n2 = min(a.L.fsup_cnt[s], a.L.fsupK);
             const int* __restrict__ fsup = a.L.fsup + s;
-            for (int k = 0; k < n2; k++) {
-                const int ai = fsup[(int64_t)k * a.nf];
-                if (a.skey[ai] == spare) continue;
-                const double4 q = a.spr[ai];
-                if (!within(p.x, p.y, p.z, q.x, q.y, q.z, 2.0 * hi)) continue;
-                if (FILL) {
-                    if (w < a.L.fs_cap) {
-                        a.L.fs_i[w] = s;
-                        a.L.fs_a[w] = ai;
-                        a.L.fsF[w] = make_double4(0, 0, 0, 0);
+            const int last = n2 > 0 ? n2 - 1 : 0;
+            // candidates in batches of 8: indices, then hashes and positions,
+            // all in flight together (entries past the row end re-read its
+            // last one and are not used)
+            constexpr int U = 8;
+            for (int k = 0; k < n2; k += U) {
+                int ai[U];
+                unsigned key[U];
+                double4 q[U];
+#pragma unroll
+                for (int u = 0; u < U; u++)
+                    ai[u] = __builtin_nontemporal_load(fsup + (int64_t)min(k + u, last) * a.nf);
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    key[u] = a.skey[ai[u]];
+                    q[u] = a.spr[ai[u]];
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const bool in = (k + u < n2) & (key[u] != spare) &
+                                    within(p.x, p.y, p.z, q[u].x, q[u].y, q[u].z, 2.0 * hi);
+                    if (FILL) {
+                        if (in && w < a.L.fs_cap) {
+                            a.L.fs_i[w] = s;
+                            a.L.fs_a[w] = ai[u];
+                            a.L.fsF[w] = make_double4(0, 0, 0, 0);
+                        }
+                        w += in;
+                    } else {
+                        cs += in;
                     }
-                    w++;
-                } else {
-                    cs++;
                 }
             }
         }
@@ -825,11 +853,12 @@ int cap_grid(int64_t n, int cap) {
 
 // ---------------------------------------------------------------- wrappers
 void hash_fluid(hipStream_t s, const Grids& G, const double4* ph, const int* orig, int n,
-                const unsigned* crank, unsigned* key, unsigned* rank, long long* status) {
+                const unsigned* crank, const unsigned char* cspare, unsigned* key, unsigned* rank,
+                unsigned* hslot, unsigned long long* ctl, long long* status) {
     hipLaunchKernelGGL(k_status_reset, dim3(1), dim3(ST_WORDS), 0, s, (unsigned long long*)status);
     if (n > 0)
         hipLaunchKernelGGL(k_hash_fluid, dim3(blocks_for(n)), dim3(kBS), 0, s, G, ph, orig, n, crank,
-                           key, rank, (unsigned long long*)status);
+                           cspare, key, rank, hslot, ctl, (unsigned long long*)status);
 }
 
 void hash_solid(hipStream_t s, const Grids& G, const double4* pr, int n, unsigned* key,
@@ -840,7 +869,7 @@ void hash_solid(hipStream_t s, const Grids& G, const double4* pr, int n, unsigne
 }
 
 void rebuild_decision(hipStream_t s, const Grids& G, const double4* fph, const double4* fxb, int nf,
-                      const double4* spr, const double4* sxb, int ns, int always,
+                      const double4* spr, const double4* sxb, int ns, int always, int always_sort,
                       unsigned long long* ctl, long long* status) {
     if (nf > 0)
         hipLaunchKernelGGL(k_disp, dim3(std::min(blocks_for(nf), 1024)), dim3(kBS), 0, s, G, true, fph,
@@ -848,7 +877,8 @@ void rebuild_decision(hipStream_t s, const Grids& G, const double4* fph, const d
     if (ns > 0)
         hipLaunchKernelGGL(k_disp, dim3(std::min(blocks_for(ns), 1024)), dim3(kBS), 0, s, G, false, spr,
                            sxb, ns, ctl + CTL_DISP_S);
-    hipLaunchKernelGGL(k_decide, dim3(1), dim3(1), 0, s, ctl, (unsigned long long*)status, always);
+    hipLaunchKernelGGL(k_decide, dim3(1), dim3(1), 0, s, ctl, (unsigned long long*)status, always,
+                       always_sort);
 }
 
 void reorder_fluid(hipStream_t s, const FluidSet& f, const FluidSet& scratch, const int* perm,
@@ -866,10 +896,10 @@ void reorder_fluid(hipStream_t s, const FluidSet& f, const FluidSet& scratch, co
 }
 
 void member_slots(hipStream_t s, const int* perm, const int* slot_of, const unsigned* sorted_rank,
-                  const unsigned* cinv, int* member, unsigned* hslot, int n) {
+                  const unsigned* cinv, int* member, unsigned* hslot, int n, Gate gate) {
     if (n > 0)
         hipLaunchKernelGGL(k_member, dim3(blocks_for(n)), dim3(kBS), 0, s, perm, slot_of, sorted_rank,
-                           cinv, member, hslot, n);
+                           cinv, member, hslot, n, gate);
 }
 
 static int chunk_grid(int nchunk) {

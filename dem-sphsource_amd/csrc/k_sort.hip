@@ -24,7 +24,8 @@ namespace {
 constexpr int kBS = 256;        // threads per block
 constexpr int kTile = 1024;     // keys per radix tile / scan block
 
-__global__ void k_fill_int(int* __restrict__ p, int64_t n, int v) {
+__global__ void k_fill_int(int* __restrict__ p, int64_t n, int v, Gate gate) {
+    if (gate && *gate == 0) return;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
         p[i] = v;
@@ -53,7 +54,8 @@ __device__ __forceinline__ int block_inclusive_scan(int v, int* lds_wave) {
 }
 
 __global__ void __launch_bounds__(kBS) k_scan_reduce(const int* __restrict__ in, int64_t n,
-                                                     int* __restrict__ bsum) {
+                                                     int* __restrict__ bsum, Gate gate) {
+    if (gate && *gate == 0) return;
     __shared__ int lw[4];
     int64_t base = (int64_t)blockIdx.x * kTile + threadIdx.x * 4;
     int s = 0;
@@ -68,7 +70,8 @@ __global__ void __launch_bounds__(kBS) k_scan_reduce(const int* __restrict__ in,
 // grand total to out[n]
 __global__ void __launch_bounds__(kBS) k_scan_final(const int* __restrict__ in, int64_t n,
                                                     const int* __restrict__ boff,
-                                                    int* __restrict__ out) {
+                                                    int* __restrict__ out, Gate gate) {
+    if (gate && *gate == 0) return;
     __shared__ int lw[4];
     int64_t base = (int64_t)blockIdx.x * kTile + threadIdx.x * 4;
     int v[4];
@@ -89,7 +92,9 @@ __global__ void __launch_bounds__(kBS) k_scan_final(const int* __restrict__ in, 
 }
 
 __global__ void __launch_bounds__(kBS) k_rs_hist(const unsigned* __restrict__ kin, int64_t n,
-                                                 int shift, int* __restrict__ hist, int ntiles) {
+                                                 int shift, int* __restrict__ hist, int ntiles,
+                                                 Gate gate) {
+    if (gate && *gate == 0) return;
     __shared__ int h[256];
     const int t = threadIdx.x;
     h[t] = 0;
@@ -108,7 +113,8 @@ __global__ void __launch_bounds__(kBS) k_rs_scatter(const unsigned* __restrict__
                                                     const int* __restrict__ vin, int64_t n,
                                                     int shift, const int* __restrict__ off,
                                                     int ntiles, unsigned* __restrict__ kout,
-                                                    int* __restrict__ vout) {
+                                                    int* __restrict__ vout, Gate gate) {
+    if (gate && *gate == 0) return;
     __shared__ int run[256];
     __shared__ int wcnt[4][256];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -146,7 +152,8 @@ __global__ void __launch_bounds__(kBS) k_rs_scatter(const unsigned* __restrict__
 
 // findStartAndEnd without the end[] race of the reference (last key -> n)
 __global__ void k_spans(const unsigned* __restrict__ sorted, int64_t n, int* __restrict__ start,
-                        int* __restrict__ end, const unsigned* __restrict__ inv) {
+                        int* __restrict__ end, const unsigned* __restrict__ inv, Gate gate) {
+    if (gate && *gate == 0) return;
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     const unsigned k = sorted[i];
@@ -157,11 +164,11 @@ __global__ void k_spans(const unsigned* __restrict__ sorted, int64_t n, int* __r
 
 }  // namespace
 
-void fill_int(hipStream_t s, int* p, int64_t n, int v) {
+void fill_int(hipStream_t s, int* p, int64_t n, int v, Gate gate) {
     if (n <= 0) return;
     int b = blocks_for(n);
     if (b > 4096) b = 4096;
-    hipLaunchKernelGGL(k_fill_int, dim3(b), dim3(kBS), 0, s, p, n, v);
+    hipLaunchKernelGGL(k_fill_int, dim3(b), dim3(kBS), 0, s, p, n, v, gate);
 }
 
 void copy_int(hipStream_t s, int* d, const int* src, int64_t n) {
@@ -181,21 +188,22 @@ int64_t scan_tmp_size(int64_t n) {
     return tot + 2;
 }
 
-void exclusive_scan(hipStream_t s, const int* in, int* out, int64_t n, int* tmp) {
+void exclusive_scan(hipStream_t s, const int* in, int* out, int64_t n, int* tmp, Gate gate) {
     if (n <= 0) {
-        fill_int(s, out, 1, 0);
+        fill_int(s, out, 1, 0, gate);
         return;
     }
     int64_t nb = (n + kTile - 1) / kTile;
     if (nb == 1) {
-        hipLaunchKernelGGL(k_scan_final, dim3(1), dim3(kBS), 0, s, in, n, (const int*)nullptr, out);
+        hipLaunchKernelGGL(k_scan_final, dim3(1), dim3(kBS), 0, s, in, n, (const int*)nullptr, out, gate);
         return;
     }
     int* bsum = tmp;
     int* boff = tmp + nb + 1;
-    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(kBS), 0, s, in, n, bsum);
-    exclusive_scan(s, bsum, boff, nb, tmp + 2 * nb + 2);
-    hipLaunchKernelGGL(k_scan_final, dim3((unsigned)nb), dim3(kBS), 0, s, in, n, (const int*)boff, out);
+    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(kBS), 0, s, in, n, bsum, gate);
+    exclusive_scan(s, bsum, boff, nb, tmp + 2 * nb + 2, gate);
+    hipLaunchKernelGGL(k_scan_final, dim3((unsigned)nb), dim3(kBS), 0, s, in, n, (const int*)boff, out,
+                       gate);
 }
 
 int64_t radix_tmp_size(int64_t n) {
@@ -206,7 +214,7 @@ int64_t radix_tmp_size(int64_t n) {
 }
 
 void radix_sort_iota(hipStream_t s, const unsigned* keys_in, int64_t n, int bits,
-                     unsigned* keys_out, int* perm_out, unsigned* tmp) {
+                     unsigned* keys_out, int* perm_out, unsigned* tmp, Gate gate) {
     if (n <= 0) return;
     const int64_t nt = (n + kTile - 1) / kTile;
     unsigned* kb[2] = {tmp, tmp + n};
@@ -221,19 +229,20 @@ void radix_sort_iota(hipStream_t s, const unsigned* keys_in, int64_t n, int bits
         const int* vin = p == 0 ? nullptr : vb[(p - 1) & 1];
         unsigned* kout = p == passes - 1 ? keys_out : kb[p & 1];
         int* vout = p == passes - 1 ? perm_out : vb[p & 1];
-        hipLaunchKernelGGL(k_rs_hist, dim3((unsigned)nt), dim3(kBS), 0, s, kin, n, 8 * p, hist, (int)nt);
-        exclusive_scan(s, hist, hoff, 256 * nt, stmp);
+        hipLaunchKernelGGL(k_rs_hist, dim3((unsigned)nt), dim3(kBS), 0, s, kin, n, 8 * p, hist, (int)nt,
+                           gate);
+        exclusive_scan(s, hist, hoff, 256 * nt, stmp, gate);
         hipLaunchKernelGGL(k_rs_scatter, dim3((unsigned)nt), dim3(kBS), 0, s, kin, vin, n, 8 * p,
-                           (const int*)hoff, (int)nt, kout, vout);
+                           (const int*)hoff, (int)nt, kout, vout, gate);
     }
 }
 
 void cell_spans(hipStream_t s, const unsigned* sorted, int64_t n, int* start, int* end, int ncell,
-                const unsigned* inv) {
-    fill_int(s, start, ncell, -1);
-    fill_int(s, end, ncell, -1);
+                const unsigned* inv, Gate gate) {
+    fill_int(s, start, ncell, -1, gate);
+    fill_int(s, end, ncell, -1, gate);
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_spans, dim3(blocks_for(n)), dim3(kBS), 0, s, sorted, n, start, end, inv);
+    hipLaunchKernelGGL(k_spans, dim3(blocks_for(n)), dim3(kBS), 0, s, sorted, n, start, end, inv, gate);
 }
 
 }  // namespace dsph
