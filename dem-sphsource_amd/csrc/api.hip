@@ -96,6 +96,10 @@ struct dsph_ctx {
     double skin_h = 0.3;        // superset skin / max fluid h
     int tune_bs = 256;          // workgroup of the fluid pair loops
     int tune_bmem = 64;         // particles per superset build chunk (one wavefront)
+    int tune_group = 1;         // particles sharing one superset row
+    bool fluid_uniform = false; // every fluid particle has the same (h, c, rho0, nu)
+    UniformFluid ufluid;
+    int sub_bits = 9;           // fluid sort key: sub-cell Morton bits below the cell rank
     // active (non-dummy) solid grid for the DEM contact search
     unsigned* akey = nullptr;
     unsigned* akey_sorted = nullptr;
@@ -218,6 +222,20 @@ void alloc_sup(dsph_ctx* c, int K, int KS) {
     c->L.fsup = dalloc<int>(c, (size_t)KS * std::max(c->nf, 1));
 }
 
+// more superset capacity with the current rows kept: the ELL is
+// column-major with stride nf, so the first supK columns copy as one block
+void grow_sup_keep(dsph_ctx* c, int K) {
+    if (K <= c->L.supK) return;
+    const size_t nf = (size_t)std::max(c->nf, 1);
+    int* fresh = dalloc<int>(c, (size_t)K * nf);
+    HIPCHK(hipMemcpyAsync(fresh, c->L.sup, sizeof(int) * (size_t)c->L.supK * nf, hipMemcpyDeviceToDevice,
+                          c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    dfree(c, c->L.sup);
+    c->L.sup = fresh;
+    c->L.supK = K;
+}
+
 void alloc_fs(dsph_ctx* c, int64_t cap) {
     dfree(c, c->L.fs_i);
     dfree(c, c->L.fs_a);
@@ -270,6 +288,8 @@ void invalidate_graphs(dsph_ctx* c) {
 FluidArgs fluid_args(dsph_ctx* c) {
     FluidArgs a;
     a.f = c->fset[0];
+    a.uni = c->fluid_uniform;
+    a.u = c->ufluid;
     a.nf = c->nf;
     a.bs = c->tune_bs;
     a.cur = c->dcur;
@@ -359,6 +379,7 @@ SearchArgs search_args(dsph_ctx* c) {
     SearchArgs a;
     a.G = grids_view(c);
     a.ncell = c->ncell;
+    a.group = c->tune_group;
     a.nf = c->nf;
     a.ns = c->ns;
     a.ph = c->fset[0].ph;
@@ -394,8 +415,8 @@ void neighbor_search(dsph_ctx* c, int i_step, int gap, bool save_history) {
     {
         FamTimer t(c, F_GRID);
         const Grids G = grids_view(c);
-        hash_fluid(s, G, c->fset[0].ph, c->fset[0].orig, c->nf, c->crank, c->cspare, c->fkey, c->frank,
-                   c->hslot, c->ctl, stw);
+        hash_fluid(s, G, c->fset[0].ph, c->fset[0].orig, c->nf, c->sub_bits, c->crank, c->cspare, c->fkey,
+                   c->frank, c->hslot, c->ctl, stw);
         hash_solid(s, G, c->so.pr, c->ns, c->skey, stw);
         if (fluid_step) {
             bool walk_only = false;
@@ -403,8 +424,9 @@ void neighbor_search(dsph_ctx* c, int i_step, int gap, bool save_history) {
             rebuild_decision(s, G, c->fset[0].ph, c->fxb, c->nf, c->so.pr, c->sxb, c->ns,
                              c->skin_h <= 0.0, walk_only, c->ctl, stw);
             const Gate gate = c->ctl + CTL_SORT;
-            radix_sort_iota(s, c->frank, c->nf, c->key_bits, c->fkey_sorted, c->fperm, c->radix_tmp, gate);
-            cell_spans(s, c->fkey_sorted, c->nf, c->fcs, c->fce, c->ncell, c->cinv, gate);
+            radix_sort_iota(s, c->frank, c->nf, c->key_bits + c->sub_bits, c->fkey_sorted, c->fperm,
+                            c->radix_tmp, gate);
+            cell_spans(s, c->fkey_sorted, c->nf, c->fcs, c->fce, c->ncell, c->cinv, gate, c->sub_bits);
             if (c->ns > 0)
                 radix_sort_iota(s, c->skey, c->ns, c->key_bits, c->skey_sorted, c->sperm, c->radix_tmp,
                                 gate);
@@ -412,8 +434,8 @@ void neighbor_search(dsph_ctx* c, int i_step, int gap, bool save_history) {
             if (c->reorder_period > 0)
                 reorder_fluid(s, c->fset[0], c->fset[1], c->fperm, c->fslot, c->nf, c->mixed(),
                               c->dcur, c->ctl);
-            member_slots(s, c->fperm, c->fslot, c->fkey_sorted, c->cinv, c->fmember, c->hslot, c->nf,
-                         gate);
+            member_slots(s, c->fperm, c->fslot, c->fkey_sorted, c->sub_bits, c->cinv, c->fmember, c->hslot,
+                         c->nf, gate);
         }
     }
     if (fluid_step) {
@@ -456,8 +478,21 @@ void search_checked(dsph_ctx* c, int i_step, int gap) {
     for (int attempt = 0; attempt < 8; attempt++) {
         unsigned long long w[ST_WORDS];
         read_status(c, w);
-        if (!(w[ST_FF_OVERFLOW] | w[ST_FS_OVERFLOW] | w[ST_SS_OVERFLOW] | w[ST_SUP_OVERFLOW])) return;
-        if (w[ST_SUP_OVERFLOW]) alloc_sup(c, c->L.supK * 3 / 2 + 16, c->L.fsupK * 3 / 2 + 16);
+        const bool ovf = w[ST_FF_OVERFLOW] | w[ST_FS_OVERFLOW] | w[ST_SS_OVERFLOW] | w[ST_SUP_OVERFLOW];
+        if (!ovf) {
+            // superset rows change only at builds, which graph-replayed steps
+            // cannot grow: rows above 3/4 of the capacity get room for half
+            // as much again (the rows are kept, no new build)
+            if (w[ST_SUP_MAX] * 4 > (unsigned long long)c->L.supK * 3) {
+                grow_sup_keep(c, (int)(w[ST_SUP_MAX] * 3 / 2 + 16));
+                invalidate_graphs(c);
+            }
+            return;
+        }
+        if (w[ST_SUP_OVERFLOW])
+            alloc_sup(c, (int)std::max<unsigned long long>(w[ST_SUP_MAX] * 3 / 2 + 16,
+                                                           (unsigned long long)c->L.supK * 3 / 2 + 16),
+                      c->L.fsupK * 3 / 2 + 16);
         if (w[ST_FF_OVERFLOW])
             alloc_ff(c, (int)std::max<unsigned long long>(w[ST_FF_MAX] + 16, (unsigned long long)c->L.K * 2));
         if (w[ST_FS_OVERFLOW]) alloc_fs(c, (int64_t)(w[ST_FS_TOTAL] * 3 / 2 + 1024));
@@ -1030,6 +1065,16 @@ int dsph_set_tuning(dsph_ctx* c, int32_t key, int32_t value) {
     if (key == DSPH_TUNE_PAIR_BLOCK) {
         if (value < 64 || value > 512 || value % 64) return DSPH_E_INVALID;
         c->tune_bs = value;
+    } else if (key == DSPH_TUNE_GROUP) {
+        if (value != 1 && value != 2 && value != 4 && value != 8) return DSPH_E_INVALID;
+        c->tune_group = value;
+        if (c->uploaded) {
+            try {
+                force_rebuild(c);
+            } catch (const ApiError& e) {
+                return fail(c, e);
+            }
+        }
     } else if (key == DSPH_TUNE_BUILD_BLOCK) {
         if (value < 1 || value > 64) return DSPH_E_INVALID;
         c->tune_bmem = value;
@@ -1054,6 +1099,8 @@ int dsph_get_tuning(dsph_ctx* c, int32_t key, int32_t* value) {
         *value = c->tune_bs;
     else if (key == DSPH_TUNE_BUILD_BLOCK)
         *value = c->tune_bmem;
+    else if (key == DSPH_TUNE_GROUP)
+        *value = c->tune_group;
     else
         return DSPH_E_INVALID;
     return DSPH_OK;
@@ -1115,6 +1162,7 @@ int dsph_upload_batch(dsph_ctx* c, const dsph_scene* scenes, int32_t n_scenes) {
         superset_stencil(c);  // also copies the grids to the device
         c->key_bits = 1;
         while ((1LL << c->key_bits) < c->ncell) c->key_bits++;
+        c->sub_bits = std::max(0, std::min(9, (32 - c->key_bits) / 3 * 3));
         c->sscene = dalloc<int>(c, std::max(ns, 1));
         if (ns) h2d(c, c->sscene, M.sscene.data(), ns);
         c->dwoff = dalloc<int>(c, n_scenes + 1);
@@ -1200,6 +1248,16 @@ int dsph_upload_batch(dsph_ctx* c, const dsph_scene* scenes, int32_t n_scenes) {
                 tid[i] = uniq.emplace(key, (int)uniq.size()).first->second;
             }
             const size_t ntab = std::max<size_t>(uniq.size(), 1);
+            c->fluid_uniform = uniq.size() == 1;
+            if (c->fluid_uniform) {
+                const auto& k = uniq.begin()->first;
+                const double h13 = k[0] / 1.3;
+                c->ufluid.h = k[0];
+                c->ufluid.c = k[1];
+                c->ufluid.m = h13 * h13 * h13 * k[2];
+                c->ufluid.rho0 = k[2];
+                c->ufluid.nu = k[3];
+            }
             std::vector<double> t4(4 * ntab, 0.0), tn(ntab, 0.0);
             for (const auto& kv : uniq) {
                 const double h = kv.first[0], h13 = h / 1.3;
@@ -1451,6 +1509,13 @@ int dsph_upload_batch(dsph_ctx* c, const dsph_scene* scenes, int32_t n_scenes) {
         L.fsup_cnt = dalloc<int>(c, nf);
         fill_int(c->st, L.sup_cnt, nf, 0);
         fill_int(c->st, L.fsup_cnt, nf, 0);
+        L.grow = dalloc<int>(c, nf);
+        {
+            std::vector<int> self(nf);
+            for (int i = 0; i < nf; i++) self[i] = i;
+            h2d(c, L.grow, self.data(), nf);
+            HIPCHK(hipStreamSynchronize(c->st));
+        }
         alloc_fs(c, std::max<int64_t>(4096, 8LL * nf));
         L.fsr_cnt = dalloc<int>(c, ns);
         L.fsr_off = dalloc<int>(c, (size_t)ns + 1);

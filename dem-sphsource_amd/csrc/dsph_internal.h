@@ -38,6 +38,7 @@ enum {
     ST_SS_TOTAL = 6,
     ST_SUP_OVERFLOW = 7,
     ST_REBUILDS = 8,
+    ST_SUP_MAX = 9,        // longest (group) superset row of this search's build
     ST_WORDS = 12
 };
 
@@ -178,6 +179,10 @@ struct Lists {
     int supK = 0;
     int* sup = nullptr;
     int* sup_cnt = nullptr;
+    // superset rows are shared by groups of consecutive sorted particles
+    // (the union of their supersets): grow[slot] = the slot whose row and
+    // count hold the group's row
+    int* grow = nullptr;
     int fsupK = 0;
     int* fsup = nullptr;
     int* fsup_cnt = nullptr;
@@ -230,12 +235,13 @@ void radix_sort_iota(hipStream_t s, const unsigned* keys_in, int64_t n, int bits
 // start/end of each cell over sorted keys; with inv, the keys are cell
 // ranks and inv maps a rank to its cell
 void cell_spans(hipStream_t s, const unsigned* sorted, int64_t n, int* start, int* end,
-                int ncell, const unsigned* inv, Gate gate = nullptr);
+                int ncell, const unsigned* inv, Gate gate = nullptr, int shift = 0);
 
 // search (k_search.hip)
 struct SearchArgs {
     Grids G;
     int ncell;              // cells of all scenes
+    int group;              // particles sharing one superset row (1, 2, 4 or 8)
     int nf, ns;
     const double4* ph;      // fluid position + h, slot order
     const int* fcs;         // fluid cell ranges over sorted positions
@@ -256,7 +262,10 @@ struct SearchArgs {
 };
 // also the reference hash of every slot, and CTL_NEAR_SPARE when a particle's
 // cell has a stencil reaching the spare cell
-void hash_fluid(hipStream_t s, const Grids& G, const double4* ph, const int* orig, int n,
+// The fluid sort key: the cell's Morton rank above the Morton code of the
+// particle's sub-cell (sub_bits / 3 bits per axis), so a cell's members are
+// contiguous and themselves in z-order.
+void hash_fluid(hipStream_t s, const Grids& G, const double4* ph, const int* orig, int n, int sub_bits,
                 const unsigned* crank, const unsigned char* cspare, unsigned* key_orig,
                 unsigned* rank_orig, unsigned* hslot, unsigned long long* ctl, long long* status);
 void hash_solid(hipStream_t s, const Grids& G, const double4* pr, int n, unsigned* key,
@@ -270,7 +279,7 @@ void rebuild_decision(hipStream_t s, const Grids& G, const double4* fph, const d
 void reorder_fluid(hipStream_t s, const FluidSet& f, const FluidSet& scratch, const int* perm,
                    int* slot_of, int n, bool mixed, int cur, const unsigned long long* ctl);
 void member_slots(hipStream_t s, const int* perm, const int* slot_of, const unsigned* sorted_rank,
-                  const unsigned* cinv, int* member, unsigned* hslot, int n, Gate gate);
+                  int sub_bits, const unsigned* cinv, int* member, unsigned* hslot, int n, Gate gate);
 // chunks: (first, end) sorted positions of <= 64 consecutive particles of
 // one scene, one wavefront each
 void fluid_superset(hipStream_t s, const SearchArgs& a, const int2* chunks, int nchunk, double4* fxb,
@@ -294,8 +303,14 @@ void solid_normals(hipStream_t s, const Grids& G, const Solids& so, int ns, cons
                    const int* sce, const int* sperm);
 
 // fluid (k_fluid.hip)
+// the fluid properties when every fluid particle shares them
+struct UniformFluid {
+    double h = 0, c = 0, m = 0, rho0 = 0, nu = 0;
+};
 struct FluidArgs {
     FluidSet f;
+    int uni;           // all fluid particles share one (h, c, rho0, nu): u
+    UniformFluid u;
     int nf;
     int bs;   // workgroup size of the pair loops (tunable, <= 512)
     int cur;  // current drho buffer

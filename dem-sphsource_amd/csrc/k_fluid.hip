@@ -14,7 +14,7 @@
 // step (spare-cell cases) are read from the exact list instead (cnt >= 0).
 // Two precisions from one template:
 //   MIX = false  FP64 throughout (the reference's arithmetic);
-//   MIX = true   pair loops in FP32 with Kahan-compensated neighbour sums;
+//   MIX = true   pair loops in FP32, neighbour sums in FP64;
 //                the FP64 masters (position, velocity, dRho) are updated in
 //                FP64 and the FP32 records refreshed.
 //
@@ -54,67 +54,36 @@ constexpr double kWC = 21.0 / (16.0 * kPi);
 template <bool MIX> struct Prec { using R = double; using R4 = double4; };
 template <> struct Prec<true> { using R = float; using R4 = float4; };
 
-// Kahan-compensated FP32 sum.  add_if(on, x) is add(x) when on and no-op
-// otherwise, without a branch: the pair loops compute every candidate's
-// term and keep it only inside the kernel support, so the compiler cannot
-// sink a candidate's record loads behind its support test.
-struct KahanF {
-    float s = 0.f, c = 0.f;
-    __device__ __forceinline__ void add(float x) {
-        float y = x - c;
-        float t = s + y;
-        c = (t - s) - y;
-        s = t;
-    }
-    __device__ __forceinline__ void add_if(bool on, float x) {
-        const float y = x - c;
-        const float t = s + y;
-        const float cn = (t - s) - y;
-        s = on ? t : s;
-        c = on ? cn : c;
-    }
-};
-
 template <class R> __device__ __forceinline__ R rcp_(R x) { return R(1) / x; }
 template <> __device__ __forceinline__ float rcp_<float>(float x) { return __builtin_amdgcn_rcpf(x); }
 template <class R> __device__ __forceinline__ R rsq_(R x) { return R(1) / sqrt(x); }
 template <> __device__ __forceinline__ float rsq_<float>(float x) { return __builtin_amdgcn_rsqf(x); }
 
+// Neighbour sums.  FP64 path: plain FP64 sums.  Mixed path: FP32 terms
+// summed in FP64 (a convert and an FP64 add per term, no compensation
+// needed).  add_if(on, x) adds x when on and +0 otherwise, without a branch:
+// the pair loops compute every candidate's term and keep it only inside the
+// kernel support, so the compiler cannot sink a candidate's record loads
+// behind its support test; +0 leaves an FP64 sum unchanged.
 template <class R>
 struct Acc1 {
-    R s = 0;
-    __device__ __forceinline__ void add(R x) { s += x; }
-    __device__ __forceinline__ void add_if(bool on, R x) { s = on ? s + x : s; }
-    __device__ __forceinline__ R get() const { return s; }
-};
-template <>
-struct Acc1<float> {
-    KahanF k;
-    __device__ __forceinline__ void add(float x) { k.add(x); }
-    __device__ __forceinline__ void add_if(bool on, float x) { k.add_if(on, x); }
-    __device__ __forceinline__ float get() const { return k.s; }
+    double s = 0.0;
+    __device__ __forceinline__ void add(R x) { s += (double)x; }
+    __device__ __forceinline__ void add_if(bool on, R x) { s += (double)(on ? x : R(0)); }
+    __device__ __forceinline__ double get() const { return s; }
 };
 template <class R>
 struct Acc3 {
-    R x = 0, y = 0, z = 0;
-    __device__ __forceinline__ void add(R a, R b, R c) { x += a; y += b; z += c; }
+    double x = 0.0, y = 0.0, z = 0.0;
+    __device__ __forceinline__ void add(R a, R b, R c) {
+        x += (double)a;
+        y += (double)b;
+        z += (double)c;
+    }
     __device__ __forceinline__ void add_if(bool on, R a, R b, R c) {
-        x = on ? x + a : x;
-        y = on ? y + b : y;
-        z = on ? z + c : z;
-    }
-};
-template <>
-struct Acc3<float> {
-    KahanF kx, ky, kz;
-    float x = 0, y = 0, z = 0;
-    __device__ __forceinline__ void add(float a, float b, float c) {
-        kx.add(a); ky.add(b); kz.add(c);
-        x = kx.s; y = ky.s; z = kz.s;
-    }
-    __device__ __forceinline__ void add_if(bool on, float a, float b, float c) {
-        kx.add_if(on, a); ky.add_if(on, b); kz.add_if(on, c);
-        x = kx.s; y = ky.s; z = kz.s;
+        x += (double)(on ? a : R(0));
+        y += (double)(on ? b : R(0));
+        z += (double)(on ? c : R(0));
     }
 };
 
@@ -131,7 +100,7 @@ struct Nb {
     int id;
     R vx, vy, vz, drho;
     R h, c, m, rho0, nu;
-    R rho, P, V, gamma;
+    R rho, P;
 };
 
 // record quads of slot j: two 16-byte quads (mixed) or two f64x4 (FP64)
@@ -161,31 +130,32 @@ __device__ __forceinline__ void load_state(const FluidSet& f, int cur, int j, Nb
     const auto b = rec_of<MIX>(f, cur, j)[1];
     o.vx = b.x; o.vy = b.y; o.vz = b.z; o.drho = b.w;
 }
-// the particle's property-table entry: h, c, m, rho0 and, with NU, nu
-template <bool MIX, bool NU>
-__device__ __forceinline__ void load_props(const FluidSet& f, Nb<MIX>& o) {
-    const auto t = ((const typename Prec<MIX>::R4*)f.tab)[o.id];
-    o.h = t.x; o.c = t.y; o.m = t.z; o.rho0 = t.w;
-    if constexpr (NU) o.nu = ((const typename Prec<MIX>::R*)f.tabnu)[o.id];
+// the particle's property-table entry: h, c, m, rho0 and, with NU, nu.
+// UNI: every fluid particle has the same entry, held by the launch (no load)
+template <bool MIX, bool NU, bool UNI>
+__device__ __forceinline__ void load_props(const FluidArgs& a, Nb<MIX>& o) {
+    using R = typename Prec<MIX>::R;
+    if constexpr (UNI) {
+        o.h = (R)a.u.h; o.c = (R)a.u.c; o.m = (R)a.u.m; o.rho0 = (R)a.u.rho0;
+        if constexpr (NU) o.nu = (R)a.u.nu;
+    } else {
+        const auto t = ((const typename Prec<MIX>::R4*)a.f.tab)[o.id];
+        o.h = t.x; o.c = t.y; o.m = t.z; o.rho0 = t.w;
+        if constexpr (NU) o.nu = ((const R*)a.f.tabnu)[o.id];
+    }
 }
-template <class R> __device__ __forceinline__ R div_(R a, R b) { return a / b; }
-template <> __device__ __forceinline__ float div_<float>(float a, float b) {
-    return a * __builtin_amdgcn_rcpf(b);
-}
-template <bool MIX, bool NU>
+template <bool MIX>
 __device__ __forceinline__ void derive(Nb<MIX>& o) {
     o.rho = o.rho0 + o.drho;
     o.P = o.drho * o.c * o.c;
-    o.V = div_(o.m, o.rho);
-    if constexpr (NU) o.gamma = o.nu * o.rho;
 }
-template <bool MIX, bool NU>
-__device__ __forceinline__ Nb<MIX> load_nb(const FluidSet& f, int cur, int j) {
+template <bool MIX, bool UNI = false>
+__device__ __forceinline__ Nb<MIX> load_nb(const FluidArgs& a, int cur, int j) {
     Nb<MIX> o;
-    load_pos<MIX>(f, cur, j, o);
-    load_state<MIX>(f, cur, j, o);
-    load_props<MIX, NU>(f, o);
-    derive<MIX, NU>(o);
+    load_pos<MIX>(a.f, cur, j, o);
+    load_state<MIX>(a.f, cur, j, o);
+    load_props<MIX, true, UNI>(a, o);
+    derive<MIX>(o);
     return o;
 }
 
@@ -243,8 +213,9 @@ struct Row {
 __device__ __forceinline__ Row nb_row(const Lists& L, int s) {
     const int c = L.cnt[s];
     if (c >= 0) return {L.nbr + s, c < L.K ? c : L.K};
-    const int m = L.sup_cnt[s];
-    return {L.sup + s, m < L.supK ? m : L.supK};
+    const int r = L.grow[s];  // superset rows are shared by a group of slots
+    const int m = L.sup_cnt[r];
+    return {L.sup + r, m < L.supK ? m : L.supK};
 }
 
 // What a pair loop reads of each candidate: its position only (POS), its
@@ -258,12 +229,13 @@ enum NbMode { POS = 0, STATE = 1, STATE_NU = 2 };
 // dependent index -> record chain per candidate.  Every load is issued
 // unconditionally (indices past the row end re-read its last entry and are
 // clamped into [0, nf)) so the loads of a batch overlap; fn(q, ok) sees
-// every loaded candidate in row order, ok = false past the row end, and must
-// not branch on it (keep its contribution with add_if).  The table entries
+// every loaded candidate in row order, ok = false past the row end and for
+// the particle itself (a shared row holds its group), and must not branch on
+// it (keep its contribution with add_if).  The table entries
 // of a batch are read once its position quads are in; a table shared by
 // many particles is served to the whole wavefront from one cache line.
 constexpr int kBatch = 4;
-template <bool MIX, int U, NbMode MODE, class Fn>
+template <bool MIX, int U, NbMode MODE, bool UNI, class Fn>
 __device__ __forceinline__ void for_row(const FluidArgs& a, int s, Fn&& fn) {
     const Row row = nb_row(a.L, s);
     const int n = row.n;
@@ -280,22 +252,24 @@ __device__ __forceinline__ void for_row(const FluidArgs& a, int s, Fn&& fn) {
     for (int u = 0; u < U; u++) jn[u] = index(u);
     for (int k = 0; k < n; k += U) {
         Nb<MIX> q[U];
+        bool ok[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
             load_pos<MIX>(a.f, a.cur, jn[u], q[u]);
             if constexpr (MODE != POS) load_state<MIX>(a.f, a.cur, jn[u], q[u]);
+            ok[u] = (k + u < n) & (jn[u] != s);
         }
 #pragma unroll
         for (int u = 0; u < U; u++) jn[u] = index(k + U + u);
         if constexpr (MODE != POS) {
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                load_props<MIX, MODE == STATE_NU>(a.f, q[u]);
-                derive<MIX, MODE == STATE_NU>(q[u]);
+                load_props<MIX, MODE == STATE_NU, UNI>(a, q[u]);
+                derive<MIX>(q[u]);
             }
         }
 #pragma unroll
-        for (int u = 0; u < U; u++) fn(q[u], k + u < n);
+        for (int u = 0; u < U; u++) fn(q[u], ok[u]);
     }
 }
 
@@ -304,14 +278,17 @@ __device__ __forceinline__ R dist(R r2) {
     return r2 > R(0) ? r2 * rsq_(r2) : R(0);
 }
 
-// F(r,h) of grad W = F r_ij; 0 outside [1e-10, 2h)
+// F(r,h) of grad W = F r_ij; 0 outside [1e-10, 2h); ih = 1/h
 template <class R>
-__device__ __forceinline__ R grad_factor(R r, R h) {
-    const R ih = rcp_(h);
+__device__ __forceinline__ R grad_factor_ih(R r, R h, R ih) {
     const R t = R(1) - R(0.5) * r * ih;
     const R ih2 = ih * ih;
     const R f = R(kGradC) * t * t * t * (ih2 * ih2 * ih);
     return (r >= R(1e-10) && r < R(2) * h) ? f : R(0);
+}
+template <class R>
+__device__ __forceinline__ R grad_factor(R r, R h) {
+    return grad_factor_ih<R>(r, h, rcp_(h));
 }
 
 // solid-side values of a fluid–solid pair (r_i - r_a subtracted in FP64)
@@ -367,7 +344,7 @@ __device__ __forceinline__ bool fs_term(const Nb<MIX>& o, typename Prec<MIX>::R 
     beta = beta < o.c ? beta : o.c;
     const R Ps = (o.rho * Pa + rhoa * o.P + o.rho * rhoa * beta * dU) * rcp_(o.rho + rhoa);
     const R pre = R(2) * rcp_(o.rho) * Va * F;
-    const R g2 = R(2) * o.gamma;
+    const R g2 = R(2) * o.nu * o.rho;
     tx = pre * (g2 * (o.vx - sp.vx) - Ps * sp.rx);
     ty = pre * (g2 * (o.vy - sp.vy) - Ps * sp.ry);
     tz = pre * (g2 * (o.vz - sp.vz) - Ps * sp.rz);
@@ -386,24 +363,36 @@ __global__ void __launch_bounds__(kBS) k_records(FluidSet f, int n, int buf) {
 }
 
 // ------------------------------------------------------------- continuity
-template <bool MIX>
+// The pair sums are the reference's algebra with the reciprocals merged:
+//   continuity  2 rho_i V_j F (rho_j dvr + dP r / c) / (rho_i + rho_j)
+//             = 2 rho_i m_j F (rho_j c dvr + dP r) / (c rho_j (rho_i + rho_j)),
+//   momentum    (2 / rho_i) V_j F [gbar dv - Ps r] with Ps = N / (rho_i + rho_j)
+//             = common [(rho_i + rho_j) gbar dv - N r],
+//               common = (2 / rho_i) m_j F / (rho_j (rho_i + rho_j)),
+// (V_j = m_j / rho_j), so a pair costs one reciprocal besides 1/h and the
+// distance.  UNI: all fluid shares one (h, c, m, rho0, nu): 1/h and c are
+// constants and (rho_i + rho_j) gbar = 2 nu rho_i rho_j.
+template <bool MIX, bool UNI>
 __global__ void __launch_bounds__(kMaxBS) k_continuity(FluidArgs a, double dt) {
     using R = typename Prec<MIX>::R;
     const int s = xcd_block() * blockDim.x + threadIdx.x;
     if (s >= a.nf) return;
     const FluidSet& f = a.f;
-    const Nb<MIX> o = load_nb<MIX, false>(f, a.cur, s);
+    const Nb<MIX> o = load_nb<MIX, UNI>(a, a.cur, s);
     const double4 ps = f.ph[s];
+    const R uih = UNI ? R(1) / o.h : R(0);
+    const R two_rho_i = R(2) * o.rho;
     Acc1<R> acc;
-    for_row<MIX, kBatch, STATE>(a, s, [&](const Nb<MIX>& q, bool ok) {
+    for_row<MIX, kBatch, STATE, UNI>(a, s, [&](const Nb<MIX>& q, bool ok) {
         R rx, ry, rz;
         rel<MIX>(f, o, q, rx, ry, rz);
         const R r = dist<R>(rx * rx + ry * ry + rz * rz);
-        const R F = grad_factor<R>(r, R(0.5) * (o.h + q.h));
+        const R hb = UNI ? o.h : R(0.5) * (o.h + q.h);
+        const R F = grad_factor_ih<R>(r, hb, UNI ? uih : rcp_(hb));
         const R dvr = (o.vx - q.vx) * rx + (o.vy - q.vy) * ry + (o.vz - q.vz) * rz;
-        const R c = R(0.5) * (o.c + q.c);
-        acc.add_if(ok && F != R(0), R(2) * o.rho * q.V * F * (q.rho * dvr + (o.P - q.P) * r * rcp_(c)) *
-                                  rcp_(o.rho + q.rho));
+        const R c = UNI ? o.c : R(0.5) * (o.c + q.c);
+        const R num = q.rho * c * dvr + (o.P - q.P) * r;
+        acc.add_if(ok && F != R(0), two_rho_i * q.m * F * num * rcp_(c * q.rho * (o.rho + q.rho)));
     });
     const double4 vm = f.vr[s];
     if (fs_valid(a.L, a.nf)) {
@@ -423,51 +412,62 @@ __global__ void __launch_bounds__(kMaxBS) k_continuity(FluidArgs a, double dt) {
                     rcp_(o.rho + rhoa));
         }
     }
-    const double nd = f.drho[a.cur][s] + dt * (double)acc.get();
+    const double nd = f.drho[a.cur][s] + dt * acc.get();
     f.drho[1 - a.cur][s] = nd;
     write_records<MIX>(f, 1 - a.cur, s, ps, vm.x, vm.y, vm.z, nd);
 }
 
 // --------------------------------------------------------------- momentum
-template <bool MIX>
+template <bool MIX, bool UNI>
 __global__ void __launch_bounds__(kMaxBS) k_momentum(FluidArgs a, int write_fs) {
     using R = typename Prec<MIX>::R;
     const int s = xcd_block() * blockDim.x + threadIdx.x;
     if (s >= a.nf) return;
     const FluidSet& f = a.f;
-    const Nb<MIX> o = load_nb<MIX, true>(f, a.cur, s);
+    const Nb<MIX> o = load_nb<MIX, UNI>(a, a.cur, s);
     const double4 ps = f.ph[s];
+    const R uih = UNI ? R(1) / o.h : R(0);
     const R two_over_rho = R(2) * rcp_(o.rho);
+    const R gam_i = o.nu * o.rho;
+    const R two_nu_rho_i = R(2) * o.nu * o.rho;
     // artificial viscosity (SPHKernelDEMContactModel.h:74-77), opt-in: the
     // particle's alpha times m_j h_ij c_ij (v_ij . r_ij) / (rho_ij r^2 +
     // 0.01 h_ij^2) grad W_ij, oriented to remove kinetic energy
     const bool art = (a.flags & DSPH_FLAG_ART_VISCOSITY) != 0;
     const R alpha = art ? (R)f.alpha[s] : R(0);
     Acc3<R> acc;
-    for_row<MIX, kBatch, STATE_NU>(a, s, [&](const Nb<MIX>& q, bool ok) {
+    for_row<MIX, kBatch, STATE_NU, UNI>(a, s, [&](const Nb<MIX>& q, bool ok) {
         R rx, ry, rz;
         rel<MIX>(f, o, q, rx, ry, rz);
         const R r2 = rx * rx + ry * ry + rz * rz;
         const R ir = r2 > R(0) ? rsq_(r2) : R(0);
         const R r = r2 * ir;
-        const R F = grad_factor<R>(r, R(0.5) * (o.h + q.h));
-        const R c = R(0.5) * (o.c + q.c);
+        const R hb = UNI ? o.h : R(0.5) * (o.h + q.h);
+        const R F = grad_factor_ih<R>(r, hb, UNI ? uih : rcp_(hb));
+        const R c = UNI ? o.c : R(0.5) * (o.c + q.c);
         const R dvx = o.vx - q.vx, dvy = o.vy - q.vy, dvz = o.vz - q.vz;
         // U_L - U_R = -(v_i - v_j) . e_ij
-        const R dU = -(dvx * rx + dvy * ry + dvz * rz) * ir;
+        const R dvr = dvx * rx + dvy * ry + dvz * rz;
+        const R dU = -dvr * ir;
         R beta = R(3) * (dU > R(0) ? dU : R(0));
         beta = beta < c ? beta : c;
-        const R Ps = (o.rho * q.P + q.rho * o.P + o.rho * q.rho * beta * dU) * rcp_(o.rho + q.rho);
-        const R gbar = R(2) * o.gamma * q.gamma * rcp_(o.gamma + q.gamma);
-        const R pre = two_over_rho * q.V * F;
-        R sr = -pre * Ps;  // the radial factor of the pair term
-        if (art) {
-            const R hb = R(0.5) * (o.h + q.h), rb = R(0.5) * (o.rho + q.rho);
-            const R dvr = dvx * rx + dvy * ry + dvz * rz;
-            sr += q.V * q.rho * alpha * hb * c * dvr * rcp_(rb * r2 + R(0.01) * hb * hb) * F;
+        const R rsum = o.rho + q.rho;
+        const R common = two_over_rho * q.m * F * rcp_(q.rho * rsum);
+        // N = rho_i P_j + rho_j P_i + rho_i rho_j beta dU
+        R sr = -common * (o.rho * q.P + q.rho * o.P + o.rho * q.rho * beta * dU);
+        // (rho_i + rho_j) gbar, gbar = 2 gamma_i gamma_j / (gamma_i + gamma_j)
+        R vis;
+        if constexpr (UNI) {
+            vis = common * two_nu_rho_i * q.rho;
+        } else {
+            const R gam_j = q.nu * q.rho;
+            vis = common * rsum * R(2) * gam_i * gam_j * rcp_(gam_i + gam_j);
         }
-        acc.add_if(ok && F != R(0), pre * gbar * dvx + sr * rx, pre * gbar * dvy + sr * ry,
-                   pre * gbar * dvz + sr * rz);
+        if (art) {
+            const R rb = R(0.5) * rsum;
+            sr += q.m * alpha * hb * c * dvr * rcp_(rb * r2 + R(0.01) * hb * hb) * F;
+        }
+        acc.add_if(ok && F != R(0), vis * dvx + sr * rx, vis * dvy + sr * ry, vis * dvz + sr * rz);
     });
     if (fs_valid(a.L, a.nf)) {
         const double4 vm = f.vr[s];
@@ -482,7 +482,7 @@ __global__ void __launch_bounds__(kMaxBS) k_momentum(FluidArgs a, int write_fs) 
             if (write_fs) a.L.fsF[k] = make_double4(mi * tx, mi * ty, mi * tz, 0.0);
         }
     }
-    f.acc[s] = make_double4((double)acc.x + a.gx, (double)acc.y + a.gy, (double)acc.z + a.gz, 0.0);
+    f.acc[s] = make_double4(acc.x + a.gx, acc.y + a.gy, acc.z + a.gz, 0.0);
 }
 
 // calFluid2SolidForce (integrate.cu:282-330) on steps without a fluid update
@@ -493,7 +493,7 @@ __global__ void __launch_bounds__(kMaxBS) k_fs_force(FluidArgs a) {
     if (s >= a.nf || !fs_valid(a.L, a.nf)) return;
     const int b0 = a.L.fsoff[s], e = a.L.fsoff[s + 1];
     if (b0 == e) return;
-    const Nb<MIX> o = load_nb<MIX, true>(a.f, a.cur, s);
+    const Nb<MIX> o = load_nb<MIX>(a, a.cur, s);
     const double4 ps = a.f.ph[s];
     const double4 vm = a.f.vr[s];
     const double h13 = ps.w / 1.3;
@@ -521,7 +521,7 @@ __global__ void __launch_bounds__(kMaxBS) k_density_reinit(FluidArgs a, int i_st
     sw.add(sigma);  // W(0, h)
     Nb<MIX> o;
     load_pos<MIX>(f, a.cur, s, o);
-    for_row<MIX, kBatch, POS>(a, s, [&](const Nb<MIX>& q, bool ok) {
+    for_row<MIX, kBatch, POS, false>(a, s, [&](const Nb<MIX>& q, bool ok) {
         R rx, ry, rz;
         rel<MIX>(f, o, q, rx, ry, rz);
         const R qq = dist<R>(rx * rx + ry * ry + rz * rz) * ih;
@@ -616,8 +616,21 @@ void density_reinit(hipStream_t s, const FluidArgs& a, int i_step, bool mixed) {
     if (a.nf > 0) DSPH_LAUNCH_BS(k_density_reinit, a.nf, a.bs, s, a, i_step);
 }
 
+#define DSPH_LAUNCH_UNI(kern, n, bs, s, ...)                                                     \
+    do {                                                                                         \
+        const dim3 g(blocks_for(n, bs)), b(bs);                                                  \
+        if (mixed && a.uni)                                                                      \
+            hipLaunchKernelGGL((kern<true, true>), g, b, 0, s, __VA_ARGS__);                      \
+        else if (mixed)                                                                          \
+            hipLaunchKernelGGL((kern<true, false>), g, b, 0, s, __VA_ARGS__);                     \
+        else if (a.uni)                                                                          \
+            hipLaunchKernelGGL((kern<false, true>), g, b, 0, s, __VA_ARGS__);                     \
+        else                                                                                     \
+            hipLaunchKernelGGL((kern<false, false>), g, b, 0, s, __VA_ARGS__);                    \
+    } while (0)
+
 void continuity(hipStream_t s, const FluidArgs& a, double dt, bool mixed) {
-    if (a.nf > 0) DSPH_LAUNCH_BS(k_continuity, a.nf, a.bs, s, a, dt);
+    if (a.nf > 0) DSPH_LAUNCH_UNI(k_continuity, a.nf, a.bs, s, a, dt);
 }
 
 void fluid_drift(hipStream_t s, const FluidArgs& a, double dt, bool mixed) {
@@ -625,7 +638,7 @@ void fluid_drift(hipStream_t s, const FluidArgs& a, double dt, bool mixed) {
 }
 
 void momentum(hipStream_t s, const FluidArgs& a, bool mixed, bool write_fs_force) {
-    if (a.nf > 0) DSPH_LAUNCH_BS(k_momentum, a.nf, a.bs, s, a, (int)write_fs_force);
+    if (a.nf > 0) DSPH_LAUNCH_UNI(k_momentum, a.nf, a.bs, s, a, (int)write_fs_force);
 }
 
 void fluid_solid_force(hipStream_t s, const FluidArgs& a, bool mixed) {

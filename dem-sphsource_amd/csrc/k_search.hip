@@ -88,11 +88,28 @@ __device__ __forceinline__ void wave_max_atomic(unsigned long long* addr, unsign
 __global__ void k_status_reset(unsigned long long* st) {
     int i = threadIdx.x;
     // overflow flags are sticky until read; counters are per search
-    if (i == ST_OUT_OF_DOMAIN || i == ST_FF_MAX || i == ST_FS_TOTAL || i == ST_SS_TOTAL) st[i] = 0;
+    if (i == ST_OUT_OF_DOMAIN || i == ST_FF_MAX || i == ST_FS_TOTAL || i == ST_SS_TOTAL || i == ST_SUP_MAX)
+        st[i] = 0;
+}
+
+// Morton code of a point's sub-cell, b bits per axis: (x - min) / cs is the
+// same quotient the cell index truncates, so the sub-cell is its fraction
+__device__ __forceinline__ unsigned sub_code(const GridP& g, double x, double y, double z, int b) {
+    const double n = (double)(1 << b);
+    const double tx = (x - g.minx) / g.csx, ty = (y - g.miny) / g.csy, tz = (z - g.minz) / g.csz;
+    const int mx = (1 << b) - 1;
+    const int qx = min(max((int)((tx - floor(tx)) * n), 0), mx);
+    const int qy = min(max((int)((ty - floor(ty)) * n), 0), mx);
+    const int qz = min(max((int)((tz - floor(tz)) * n), 0), mx);
+    unsigned code = 0;
+    for (int k = 0; k < b; k++)
+        code |= (((qx >> k) & 1u) << (3 * k)) | (((qy >> k) & 1u) << (3 * k + 1)) |
+                (((qz >> k) & 1u) << (3 * k + 2));
+    return code;
 }
 
 __global__ void k_hash_fluid(Grids G, const double4* __restrict__ ph, const int* __restrict__ orig,
-                             int n, const unsigned* __restrict__ crank,
+                             int n, int sub_bits, const unsigned* __restrict__ crank,
                              const unsigned char* __restrict__ cspare, unsigned* __restrict__ key,
                              unsigned* __restrict__ rank, unsigned* __restrict__ hslot,
                              unsigned long long* ctl, unsigned long long* st) {
@@ -105,9 +122,10 @@ __global__ void k_hash_fluid(Grids G, const double4* __restrict__ ph, const int*
         const int h = g.cell0 + hl;
         const int o = orig[s];
         key[o] = (unsigned)h;
-        rank[o] = crank[h];
-        hslot[s] = (unsigned)h;
         out = hl == g.ncell - 1;
+        const unsigned sub = out ? 0u : sub_code(g, p.x, p.y, p.z, sub_bits / 3);
+        rank[o] = (crank[h] << sub_bits) | sub;
+        hslot[s] = (unsigned)h;
         near = cspare[h] != 0;
     }
     const unsigned long long m = __ballot(out);
@@ -238,14 +256,15 @@ __global__ void k_copyback(FluidSet dst, FluidSet src, int n, int cur,
 // slot of every sorted position and, after a reorder, the reference hash of
 // every slot in its new place (without one the hash kernel's are current)
 __global__ void k_member(const int* __restrict__ perm, const int* __restrict__ slot_of,
-                         const unsigned* __restrict__ sorted_rank, const unsigned* __restrict__ cinv,
-                         int* __restrict__ member, unsigned* __restrict__ hslot, int n, Gate gate) {
+                         const unsigned* __restrict__ sorted_rank, int sub_bits,
+                         const unsigned* __restrict__ cinv, int* __restrict__ member,
+                         unsigned* __restrict__ hslot, int n, Gate gate) {
     if (*gate == 0) return;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int s = slot_of[perm[t]];
     member[t] = s;
-    hslot[s] = cinv[sorted_rank[t]];
+    hslot[s] = cinv[sorted_rank[t] >> sub_bits];
 }
 
 __global__ void k_store_xb(const double4* __restrict__ p, double4* __restrict__ xb, int n,
@@ -300,6 +319,7 @@ __global__ void __launch_bounds__(kBS) k_build(SearchArgs a, const int2* __restr
     const int supK = a.L.supK, fsupK = a.L.fsupK;
     const float grow = 1.0f + 2e-4f;
     bool ovf = false;
+    int cmax = 0;
     for (int b = wv; b < nchunk; b += nwv) {
         const int2 ch = chunks[b];
         const int t0 = __builtin_amdgcn_readfirstlane(ch.x);
@@ -314,8 +334,20 @@ __global__ void __launch_bounds__(kBS) k_build(SearchArgs a, const int2* __restr
         if (act && lc == spare) {  // walked exactly while out of the domain
             a.L.sup_cnt[s] = 0;
             a.L.fsup_cnt[s] = 0;
+            a.L.grow[s] = s;
             act = false;
         }
+        // Groups of G consecutive lanes share one row, the union of their
+        // supersets, kept by the group's first active lane (the writer): the
+        // lanes of a group then read the same candidates in the same order,
+        // so their record loads coalesce.  A lane's own row is a subset.
+        const int G = a.group;
+        const int gl = lane & ~(G - 1);
+        const unsigned long long gmask = (G == 64 ? ~0ull : ((1ull << G) - 1)) << gl;
+        const unsigned long long actm = __ballot(act) & gmask;
+        const int wl = actm ? __ffsll((long long)actm) - 1 : lane;
+        const bool writer = act && lane == wl;
+        const int ws = __shfl(s, wl, 64);
         const int gxy = g.gx * g.gy;
         const int cz = lc / gxy, rem = lc - cz * gxy;
         const int cy = rem / g.gx, cx = rem - cy * g.gx;
@@ -382,8 +414,9 @@ __global__ void __launch_bounds__(kBS) k_build(SearchArgs a, const int2* __restr
                                     const float e2 = ex * ex + ey * ey + ez * ez;
                                     const bool in = act & (k + v < nb) & (jj[v] != s) &
                                                     (e2 <= (Ai > q[v].w ? Ai : q[v].w));
-                                    if (in && c < supK) a.L.sup[(int64_t)c * a.nf + s] = jj[v];
-                                    c += in;
+                                    const bool gin = (__ballot(in) & gmask) != 0;
+                                    if (writer && gin && c < supK) a.L.sup[(int64_t)c * a.nf + s] = jj[v];
+                                    c += gin;
                                 }
                             }
                             __builtin_amdgcn_wave_barrier();
@@ -428,10 +461,13 @@ __global__ void __launch_bounds__(kBS) k_build(SearchArgs a, const int2* __restr
         if (act) {
             a.L.sup_cnt[s] = c;
             a.L.fsup_cnt[s] = cs;
+            a.L.grow[s] = ws;
         }
         ovf = ovf || c > supK || cs > fsupK;
+        cmax = max(cmax, c);
     }
     if (ovf) atomicOr(&a.st[ST_SUP_OVERFLOW], 1ull);
+    wave_max_atomic(&a.st[ST_SUP_MAX], (unsigned long long)cmax);
 }
 
 // number of the 27 linear stencil offsets around cell (ax, ay, az) that give
@@ -635,9 +671,11 @@ __global__ void __launch_bounds__(kBS) k_exact_rows(SearchArgs a, int* __restric
         c = cw;
     } else {
         const double4 p = a.fsp[s];
-        const int n = min(a.L.sup_cnt[s], a.L.supK);
+        const int r = a.L.grow[s];  // the group's row
+        const int n = min(a.L.sup_cnt[r], a.L.supK);
         for (int k = 0; k < n; k++) {
-            const int j = a.L.sup[(int64_t)k * a.nf + s];
+            const int j = a.L.sup[(int64_t)k * a.nf + r];
+            if (j == s) continue;
             const double4 q = a.fsp[j];
             if (within(p.x, p.y, p.z, q.x, q.y, q.z, 2.0 * fmax(p.w, q.w)))
                 out[(int64_t)(c++) * a.nf + s] = j;
@@ -852,13 +890,13 @@ int cap_grid(int64_t n, int cap) {
 }  // namespace
 
 // ---------------------------------------------------------------- wrappers
-void hash_fluid(hipStream_t s, const Grids& G, const double4* ph, const int* orig, int n,
+void hash_fluid(hipStream_t s, const Grids& G, const double4* ph, const int* orig, int n, int sub_bits,
                 const unsigned* crank, const unsigned char* cspare, unsigned* key, unsigned* rank,
                 unsigned* hslot, unsigned long long* ctl, long long* status) {
     hipLaunchKernelGGL(k_status_reset, dim3(1), dim3(ST_WORDS), 0, s, (unsigned long long*)status);
     if (n > 0)
-        hipLaunchKernelGGL(k_hash_fluid, dim3(blocks_for(n)), dim3(kBS), 0, s, G, ph, orig, n, crank,
-                           cspare, key, rank, hslot, ctl, (unsigned long long*)status);
+        hipLaunchKernelGGL(k_hash_fluid, dim3(blocks_for(n)), dim3(kBS), 0, s, G, ph, orig, n, sub_bits,
+                           crank, cspare, key, rank, hslot, ctl, (unsigned long long*)status);
 }
 
 void hash_solid(hipStream_t s, const Grids& G, const double4* pr, int n, unsigned* key,
@@ -896,10 +934,10 @@ void reorder_fluid(hipStream_t s, const FluidSet& f, const FluidSet& scratch, co
 }
 
 void member_slots(hipStream_t s, const int* perm, const int* slot_of, const unsigned* sorted_rank,
-                  const unsigned* cinv, int* member, unsigned* hslot, int n, Gate gate) {
+                  int sub_bits, const unsigned* cinv, int* member, unsigned* hslot, int n, Gate gate) {
     if (n > 0)
         hipLaunchKernelGGL(k_member, dim3(blocks_for(n)), dim3(kBS), 0, s, perm, slot_of, sorted_rank,
-                           cinv, member, hslot, n, gate);
+                           sub_bits, cinv, member, hslot, n, gate);
 }
 
 static int chunk_grid(int nchunk) {

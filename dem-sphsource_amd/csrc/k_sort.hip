@@ -152,14 +152,15 @@ __global__ void __launch_bounds__(kBS) k_rs_scatter(const unsigned* __restrict__
 
 // findStartAndEnd without the end[] race of the reference (last key -> n)
 __global__ void k_spans(const unsigned* __restrict__ sorted, int64_t n, int* __restrict__ start,
-                        int* __restrict__ end, const unsigned* __restrict__ inv, Gate gate) {
+                        int* __restrict__ end, const unsigned* __restrict__ inv, Gate gate,
+                        int shift) {
     if (gate && *gate == 0) return;
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const unsigned k = sorted[i];
+    const unsigned k = sorted[i] >> shift;
     const unsigned cell = inv ? inv[k] : k;
-    if (i == 0 || sorted[i - 1] != k) start[cell] = (int)i;
-    if (i == n - 1 || sorted[i + 1] != k) end[cell] = (int)(i + 1);
+    if (i == 0 || (sorted[i - 1] >> shift) != k) start[cell] = (int)i;
+    if (i == n - 1 || (sorted[i + 1] >> shift) != k) end[cell] = (int)(i + 1);
 }
 
 }  // namespace
@@ -238,11 +239,12 @@ void radix_sort_iota(hipStream_t s, const unsigned* keys_in, int64_t n, int bits
 }
 
 void cell_spans(hipStream_t s, const unsigned* sorted, int64_t n, int* start, int* end, int ncell,
-                const unsigned* inv, Gate gate) {
+                const unsigned* inv, Gate gate, int shift) {
     fill_int(s, start, ncell, -1, gate);
     fill_int(s, end, ncell, -1, gate);
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_spans, dim3(blocks_for(n)), dim3(kBS), 0, s, sorted, n, start, end, inv, gate);
+    hipLaunchKernelGGL(k_spans, dim3(blocks_for(n)), dim3(kBS), 0, s, sorted, n, start, end, inv, gate,
+                       shift);
 }
 
 }  // namespace dsph

@@ -24,7 +24,7 @@ LIB_PATH = os.path.join(HERE, "libdsph.so")
 
 PREC_FP64, PREC_MIXED = 0, 1
 FLAG_WALL_CONTINUE, FLAG_CLUMP_QUATERNION, FLAG_ART_VISCOSITY = 1, 2, 4
-TUNE_PAIR_BLOCK, TUNE_BUILD_BLOCK = 1, 2
+TUNE_PAIR_BLOCK, TUNE_BUILD_BLOCK, TUNE_GROUP = 1, 2, 3
 LIST_FLUID_FLUID, LIST_FLUID_SOLID, LIST_SOLID_SOLID, LIST_BOND = 0, 1, 2, 3
 ST_FF_OVERFLOW, ST_FS_OVERFLOW, ST_SS_OVERFLOW, ST_OUT_OF_DOMAIN = 0, 1, 2, 3
 ST_FF_MAX, ST_FS_TOTAL, ST_SS_TOTAL, ST_SUP_OVERFLOW, ST_REBUILDS = 4, 5, 6, 7, 8
@@ -192,6 +192,8 @@ class Solver:
             self.set_tuning(TUNE_PAIR_BLOCK, cfg["pair_block"])
         if "build_block" in cfg:
             self.set_tuning(TUNE_BUILD_BLOCK, cfg["build_block"])
+        if "group" in cfg:
+            self.set_tuning(TUNE_GROUP, cfg["group"])
         if "skin" in cfg:
             self.set_skin(cfg["skin"])
 
@@ -358,6 +360,7 @@ TUNING_SPACE = {
     "skin": [0.1, 0.2, 0.3, 0.5],               # Verlet skin / h: superset size vs rebuilds
     "pair_block": [64, 128, 256, 512],          # workgroup of the fluid pair loops
     "build_block": [32, 48, 64],                # particles per wavefront in the superset build
+    "group": [1, 2, 4, 8],                      # particles sharing one superset row
 }
 
 
@@ -382,7 +385,7 @@ def autotune(scene, precision=PREC_MIXED, flags=0, trial_steps=20, warmup=5, spa
         g.upload(scene)
         dt = scene.dt
     g.initialize()
-    cfg = {"skin": 0.3, "pair_block": 256, "build_block": 64}
+    cfg = {"skin": 0.3, "pair_block": 256, "build_block": 64, "group": 1}
     trials = []
     step = [1]
 
@@ -412,7 +415,7 @@ def autotune(scene, precision=PREC_MIXED, flags=0, trial_steps=20, warmup=5, spa
             log(f"autotune {c} -> {ms:.4f} ms/step")
         return ms
 
-    for key in ("skin", "pair_block", "build_block"):
+    for key in ("group", "skin", "pair_block", "build_block"):
         best = None
         for v in space[key]:
             ms = measure(dict(cfg, **{key: v}))

@@ -386,6 +386,7 @@ void dataBase::buildDeviceData() {
     check(dsph_set_flags(dev, flags_));
     check(dsph_set_tuning(dev, DSPH_TUNE_PAIR_BLOCK, tunePair_));
     check(dsph_set_tuning(dev, DSPH_TUNE_BUILD_BLOCK, tuneBuild_));
+    check(dsph_set_tuning(dev, DSPH_TUNE_GROUP, tuneGroup_));
     check(dsph_set_skin(dev, tuneSkin_));
     uploadContactModels();
     dsph_scene s{};
@@ -500,9 +501,10 @@ void dataBase::autotune() {
     buildDeviceData();
     check(dsph_initialize(dev, 1));
     int step = 1;
-    auto measure = [&](int pair, int build, double skin) {
+    auto measure = [&](int pair, int build, double skin, int group) {
         check(dsph_set_tuning(dev, DSPH_TUNE_PAIR_BLOCK, pair));
         check(dsph_set_tuning(dev, DSPH_TUNE_BUILD_BLOCK, build));
+        check(dsph_set_tuning(dev, DSPH_TUNE_GROUP, group));
         check(dsph_set_skin(dev, skin));
         // eager warm-up steps grow any list the new skin overflows
         for (int k = 0; k < warm; k++, step++) check(dsph_step(dev, step, dt, gap, 1));
@@ -522,25 +524,30 @@ void dataBase::autotune() {
         check(r.first);
         return r.second;
     };
-    int pair = 256, build = 64;
+    int pair = 256, build = 64, group = 1;
     double skin = 0.3, best = 1e300;
+    for (int v : {1, 2, 4, 8}) {
+        const double ms = measure(pair, build, skin, v);
+        if (ms < best) best = ms, group = v;
+    }
+    best = 1e300;
     for (double v : {0.1, 0.2, 0.3, 0.5}) {
-        const double ms = measure(pair, build, v);
+        const double ms = measure(pair, build, v, group);
         if (ms < best) best = ms, skin = v;
     }
     best = 1e300;
     for (int v : {64, 128, 256, 512}) {
-        const double ms = measure(v, build, skin);
+        const double ms = measure(v, build, skin, group);
         if (ms < best) best = ms, pair = v;
     }
     best = 1e300;
     for (int v : {32, 48, 64}) {
-        const double ms = measure(pair, v, skin);
+        const double ms = measure(pair, v, skin, group);
         if (ms < best) best = ms, build = v;
     }
-    std::cout << "Autotuned: pair-loop workgroup " << pair << ", build block " << build << ", skin "
-              << skin << " h (" << best << " ms/step)" << std::endl;
-    setTuning(pair, build, skin);
+    std::cout << "Autotuned: row group " << group << ", pair-loop workgroup " << pair << ", build block "
+              << build << ", skin " << skin << " h (" << best << " ms/step)" << std::endl;
+    setTuning(pair, build, skin, group);
     dsph_destroy(dev);
     dev = nullptr;
 }

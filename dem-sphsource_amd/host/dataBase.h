@@ -76,10 +76,11 @@ protected:
     // pick launch shapes and skin now (used by solverBase::initialize)
     void autotune();
     // the configuration the next device context gets
-    void setTuning(int pairBlock, int buildBlock, double skin) {
+    void setTuning(int pairBlock, int buildBlock, double skin, int group = 1) {
         tunePair_ = pairBlock;
         tuneBuild_ = buildBlock;
         tuneSkin_ = skin;
+        tuneGroup_ = group;
     }
 
     struct simulationParameter {
@@ -119,7 +120,7 @@ private:
     void uploadContactModels();
     int precision_ = DSPH_PREC_FP64;
     unsigned flags_ = 0;
-    int tunePair_ = 256, tuneBuild_ = 64;
+    int tunePair_ = 256, tuneBuild_ = 64, tuneGroup_ = 1;
     double tuneSkin_ = 0.3;
 
 protected:

@@ -74,6 +74,7 @@ extern "C" {
 #define DSPH_STATUS_SS_TOTAL      6 /* solid-solid contacts found            */
 #define DSPH_STATUS_SUP_OVERFLOW  7 /* a fluid Verlet superset row overflowed */
 #define DSPH_STATUS_REBUILDS      8 /* superset rebuilds since upload        */
+#define DSPH_STATUS_SUP_MAX       9 /* longest superset row of the last build */
 #define DSPH_STATUS_WORDS        12
 
 /*
@@ -198,11 +199,17 @@ int dsph_set_skin(dsph_ctx* ctx, double skin_over_h);
  *   DSPH_TUNE_PAIR_BLOCK   workgroup size of the fluid pair loops, a
  *                          multiple of 64 in [64, 512] (default 256);
  *   DSPH_TUNE_BUILD_BLOCK  particles per wavefront in the Verlet superset
- *                          build, [1, 64] (default 64).
+ *                          build, [1, 64] (default 64);
+ *   DSPH_TUNE_GROUP        consecutive (cell- and sub-cell-sorted) fluid
+ *                          particles that share one superset row, the union
+ *                          of their supersets: 1, 2, 4 or 8 (default 1).  A
+ *                          shared row costs more candidates per particle but
+ *                          lets the group's gathers coalesce.
  * The Verlet skin (dsph_set_skin) and the reorder period complete the
  * autotuner's search space (dsph.py autotune, dataBase::autotune).        */
 #define DSPH_TUNE_PAIR_BLOCK   1
 #define DSPH_TUNE_BUILD_BLOCK  2
+#define DSPH_TUNE_GROUP        3
 int dsph_set_tuning(dsph_ctx* ctx, int32_t key, int32_t value);
 int dsph_get_tuning(dsph_ctx* ctx, int32_t key, int32_t* value);
 

@@ -598,11 +598,12 @@ def test_batch_matches_scenes_run_alone(prec):
 def test_launch_shapes_do_not_change_results(prec):
     sc = SCENES["coupled_small"]
     runs = []
-    for pair, build in ((256, 64), (64, 32), (512, 17)):
+    for pair, build, group in ((256, 64, 1), (64, 32, 4), (512, 17, 8), (128, 64, 2)):
         g = dsph.Solver(prec)
         g.upload(sc)
         g.set_tuning(dsph.TUNE_PAIR_BLOCK, pair)
         g.set_tuning(dsph.TUNE_BUILD_BLOCK, build)
+        g.set_tuning(dsph.TUNE_GROUP, group)
         assert g.get_tuning(dsph.TUNE_PAIR_BLOCK) == pair
         g.initialize()
         g.run(15, sc.dt, graphs=True)
