@@ -408,7 +408,10 @@ SearchArgs search_args(dsph_ctx* c) {
 // by exact stencil walks, so the sorts run on the steps that have one (a
 // device-side decision, CTL_SORT); the reference grid of any step is
 // reconstructed from its hashes on read-back.
-void neighbor_search(dsph_ctx* c, int i_step, int gap, bool save_history) {
+// zero_fs: the new fluid-solid pairs start with zero force, as the
+// reference's fresh list does; a full step skips it (its fluid phase writes
+// every pair's force before anything reads them)
+void neighbor_search(dsph_ctx* c, int i_step, int gap, bool save_history, bool zero_fs = true) {
     hipStream_t s = c->st;
     long long* stw = (long long*)c->status;
     const bool fluid_step = i_step % gap == 0 && c->nf > 0;
@@ -421,8 +424,8 @@ void neighbor_search(dsph_ctx* c, int i_step, int gap, bool save_history) {
         if (fluid_step) {
             bool walk_only = false;
             for (const GridP& g : c->grids) walk_only = walk_only || g.walk_only;
-            rebuild_decision(s, G, c->fset[0].ph, c->fxb, c->nf, c->so.pr, c->sxb, c->ns,
-                             c->skin_h <= 0.0, walk_only, c->ctl, stw);
+            rebuild_decision(s, G, c->fset[0].ph, c->fxb, c->nf, c->so.pr, c->sxb, c->so.live,
+                             c->so.nlive, c->skin_h <= 0.0, walk_only, c->ctl, stw);
             const Gate gate = c->ctl + CTL_SORT;
             radix_sort_iota(s, c->frank, c->nf, c->key_bits + c->sub_bits, c->fkey_sorted, c->fperm,
                             c->radix_tmp, gate);
@@ -445,7 +448,7 @@ void neighbor_search(dsph_ctx* c, int i_step, int gap, bool save_history) {
             fluid_superset(s, sa, c->fchunks, c->nfchunks, c->fxb, c->sxb);
         }
         FamTimer t(c, F_FSLIST);
-        fluid_filter(s, sa, c->scan_tmp);
+        fluid_filter(s, sa, c->scan_tmp, zero_fs);
         if (c->ns > 0)
             reverse_movable(s, c->L.fs_a, c->L.fsoff + c->nf, c->L.fs_cap, c->so.vel, c->ns,
                             c->L.fsr_cnt, c->L.fsr_off, c->L.fsr_idx, c->scan_tmp);
@@ -573,7 +576,7 @@ void fluid_integrate(dsph_ctx* c, double dt, int i_step, int gap) {
 }
 
 void step_body(dsph_ctx* c, int i_step, double dt, int gap) {
-    neighbor_search(c, i_step, gap, true);
+    neighbor_search(c, i_step, gap, true, false);
     solid_before(c, dt);
     fluid_integrate(c, dt, i_step, gap);
     contacts(c, dt);
@@ -1349,7 +1352,7 @@ int dsph_upload_batch(dsph_ctx* c, const dsph_scene* scenes, int32_t n_scenes) {
         // ---- solids
         std::vector<double4> spr(ns), svel(ns), sw(ns), snrm(ns), zero4(std::max({ns, nc, 1}), make_double4(0, 0, 0, 0));
         std::vector<double> seff(ns);
-        std::vector<int> smat(ns), sclus(ns), sclump(ns), sdum(ns), active;
+        std::vector<int> smat(ns), sclus(ns), sclump(ns), sdum(ns), active, live;
         c->h_spos.assign(3 * (size_t)ns, 0);
         c->h_srad.assign(ns, 0);
         c->h_smat.assign(ns, 0);
@@ -1368,6 +1371,9 @@ int dsph_upload_batch(dsph_ctx* c, const dsph_scene* scenes, int32_t n_scenes) {
             sclump[i] = s->solid_clump ? s->solid_clump[i] : -1;
             sdum[i] = s->solid_dummy ? s->solid_dummy[i] : 0;
             if (sdum[i] != 1) active.push_back(i);
+            if (sdum[i] != 1 || svel[i].w > 0.0 || svel[i].x != 0.0 || svel[i].y != 0.0 || svel[i].z != 0.0 ||
+                sw[i].x != 0.0 || sw[i].y != 0.0 || sw[i].z != 0.0 || sclump[i] >= 0)
+                live.push_back(i);
             for (int k = 0; k < 3; k++) c->h_spos[3 * i + k] = p[k];
             c->h_srad[i] = s->solid_radius[i];
             c->h_smat[i] = smat[i];
@@ -1387,6 +1393,8 @@ int dsph_upload_batch(dsph_ctx* c, const dsph_scene* scenes, int32_t n_scenes) {
         so.dummy = dalloc<int>(c, ns);
         so.active = dalloc<int>(c, active.size());
         so.nactive = (int)active.size();
+        so.live = dalloc<int>(c, live.size());
+        so.nlive = (int)live.size();
         h2d(c, so.pr, spr.data(), ns);
         h2d(c, so.vel, svel.data(), ns);
         h2d(c, so.acc, zero4.data(), ns);
@@ -1399,6 +1407,7 @@ int dsph_upload_batch(dsph_ctx* c, const dsph_scene* scenes, int32_t n_scenes) {
         h2d(c, so.clump, sclump.data(), ns);
         h2d(c, so.dummy, sdum.data(), ns);
         h2d(c, so.active, active.data(), active.size());
+        h2d(c, so.live, live.data(), live.size());
         if (s->solid_normal) {
             for (int i = 0; i < ns; i++)
                 snrm[i] = make_double4(s->solid_normal[3 * i], s->solid_normal[3 * i + 1], s->solid_normal[3 * i + 2], 0);

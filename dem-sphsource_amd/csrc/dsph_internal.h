@@ -145,6 +145,10 @@ struct Solids {
     int* dummy = nullptr;
     int* active = nullptr;  // indices of non-dummy solids
     int nactive = 0;
+    // the solids whose state the DEM phases can change: non-dummies, and
+    // any dummy with mass, a velocity or a clump (wall dummies are static)
+    int* live = nullptr;
+    int nlive = 0;
 };
 
 struct Clumps {
@@ -274,7 +278,8 @@ void hash_solid(hipStream_t s, const Grids& G, const double4* pr, int n, unsigne
 // always != 0 rebuilds at every search
 // sets CTL_REBUILD, and CTL_SORT = rebuild, exact walks or always_sort
 void rebuild_decision(hipStream_t s, const Grids& G, const double4* fph, const double4* fxb, int nf,
-                      const double4* spr, const double4* sxb, int ns, int always, int always_sort,
+                      const double4* spr, const double4* sxb, const int* slive, int nslive, int always,
+                      int always_sort,
                       unsigned long long* ctl, long long* status);
 void reorder_fluid(hipStream_t s, const FluidSet& f, const FluidSet& scratch, const int* perm,
                    int* slot_of, int n, bool mixed, int cur, const unsigned long long* ctl);
@@ -287,7 +292,7 @@ void fluid_superset(hipStream_t s, const SearchArgs& a, const int2* chunks, int 
 // exact fluid neighbour rows of all particles for read-back (ELL, stride nf,
 // capacity max(K, supK)); out_cnt = -1 marks a walked row that overflowed
 void fluid_exact_rows(hipStream_t s, const SearchArgs& a, int* out, int* out_cnt);
-void fluid_filter(hipStream_t s, const SearchArgs& a, int* scan_tmp);
+void fluid_filter(hipStream_t s, const SearchArgs& a, int* scan_tmp, bool zero_fs);
 void reverse_movable(hipStream_t s, const int* pair_key, const int* pair_off_total, int64_t cap,
                      const double4* key_vel, int nkeys, int* cnt, int* off, int* idx,
                      int* scan_tmp);

@@ -114,9 +114,12 @@ __global__ void k_bonds(DemArgs a, double dt) {
     a.L.ssT[kc] = d4(v3(T0) + (Tt * n + Tb));
 }
 
+// the per-solid DEM kernels run over the live solids only: wall dummies
+// never move, take no contacts and keep zero acceleration and torque
 __global__ void k_solid_sum(DemArgs a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.ns) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.so.nlive) return;
+    const int i = a.so.live[t];
     const double4 pri = a.so.pr[i];
     const d3 pi = v3(pri);
     d3 F{0, 0, 0}, T{0, 0, 0};
@@ -153,8 +156,9 @@ __global__ void k_solid_sum(DemArgs a) {
 }
 
 __global__ void k_walls(DemArgs a, double dt) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.ns) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.so.nlive) return;
+    const int i = a.so.live[t];
     const double4 v4 = a.so.vel[i];
     if (a.so.dummy[i] == 1 || v4.w < 1.e-10) return;
     const bool cont = (a.flags & DSPH_FLAG_WALL_CONTINUE) != 0;
@@ -208,8 +212,9 @@ __global__ void k_clump_sum(DemArgs a) {
 }
 
 __global__ void k_solid_kick(DemArgs a, double dt) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.ns) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.so.nlive) return;
+    const int i = a.so.live[t];
     if (a.so.clump[i] >= 0) return;
     double4 v = a.so.vel[i];
     const double4 ac = a.so.acc[i];
@@ -220,17 +225,18 @@ __global__ void k_solid_kick(DemArgs a, double dt) {
     const double r = a.so.pr[i].w;
     if (v.w < 1.e-10 || r < 1.e-10) return;
     const double I = 0.4 * r * r / v.w;
-    const double4 t = a.so.tq[i];
+    const double4 tq = a.so.tq[i];
     double4 w = a.so.w[i];
-    w.x += t.x / I * dt;
-    w.y += t.y / I * dt;
-    w.z += t.z / I * dt;
+    w.x += tq.x / I * dt;
+    w.y += tq.y / I * dt;
+    w.z += tq.z / I * dt;
     a.so.w[i] = w;
 }
 
 __global__ void k_solid_drift(DemArgs a, double dt) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.ns) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.so.nlive) return;
+    const int i = a.so.live[t];
     double4 p = a.so.pr[i];
     const double4 v = a.so.vel[i];
     p.x += dt * v.x;
@@ -321,19 +327,19 @@ void dem_bonds(hipStream_t s, const DemArgs& a, double dt) {
     if (a.b.n > 0) hipLaunchKernelGGL(k_bonds, dim3(blocks_for(a.b.n)), dim3(kBS), 0, s, a, dt);
 }
 void dem_solid_sum(hipStream_t s, const DemArgs& a) {
-    if (a.ns > 0) hipLaunchKernelGGL(k_solid_sum, dim3(blocks_for(a.ns)), dim3(kBS), 0, s, a);
+    if (a.so.nlive > 0) hipLaunchKernelGGL(k_solid_sum, dim3(blocks_for(a.so.nlive)), dim3(kBS), 0, s, a);
 }
 void dem_walls(hipStream_t s, const DemArgs& a, double dt) {
-    if (a.ns > 0 && a.nw > 0) hipLaunchKernelGGL(k_walls, dim3(blocks_for(a.ns)), dim3(kBS), 0, s, a, dt);
+    if (a.so.nlive > 0 && a.nw > 0) hipLaunchKernelGGL(k_walls, dim3(blocks_for(a.so.nlive)), dim3(kBS), 0, s, a, dt);
 }
 void dem_clump_sum(hipStream_t s, const DemArgs& a) {
     if (a.nc > 0) hipLaunchKernelGGL(k_clump_sum, dim3(blocks_for(a.nc)), dim3(kBS), 0, s, a);
 }
 void dem_solid_kick(hipStream_t s, const DemArgs& a, double dt) {
-    if (a.ns > 0) hipLaunchKernelGGL(k_solid_kick, dim3(blocks_for(a.ns)), dim3(kBS), 0, s, a, dt);
+    if (a.so.nlive > 0) hipLaunchKernelGGL(k_solid_kick, dim3(blocks_for(a.so.nlive)), dim3(kBS), 0, s, a, dt);
 }
 void dem_solid_drift(hipStream_t s, const DemArgs& a, double dt) {
-    if (a.ns > 0) hipLaunchKernelGGL(k_solid_drift, dim3(blocks_for(a.ns)), dim3(kBS), 0, s, a, dt);
+    if (a.so.nlive > 0) hipLaunchKernelGGL(k_solid_drift, dim3(blocks_for(a.so.nlive)), dim3(kBS), 0, s, a, dt);
 }
 void dem_clump_kick(hipStream_t s, const DemArgs& a, double dt) {
     if (a.nc > 0) hipLaunchKernelGGL(k_clump_kick, dim3(blocks_for(a.nc)), dim3(kBS), 0, s, a, dt);

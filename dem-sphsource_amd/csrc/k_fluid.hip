@@ -330,7 +330,6 @@ __device__ __forceinline__ bool fs_term(const Nb<MIX>& o, typename Prec<MIX>::R 
     using R = typename Prec<MIX>::R;
     const R r = dist<R>(sp.rx * sp.rx + sp.ry * sp.ry + sp.rz * sp.rz);
     const R F = grad_factor<R>(r, R(0.5) * (o.h + R(2.6) * sp.rad));
-    if (F == R(0)) return false;
     const R d2 = R(2) * sp.rad;
     const R Va = d2 * d2 * d2;
     R Pa, rhoa;
@@ -345,13 +344,35 @@ __device__ __forceinline__ bool fs_term(const Nb<MIX>& o, typename Prec<MIX>::R 
     const R Ps = (o.rho * Pa + rhoa * o.P + o.rho * rhoa * beta * dU) * rcp_(o.rho + rhoa);
     const R pre = R(2) * rcp_(o.rho) * Va * F;
     const R g2 = R(2) * o.nu * o.rho;
-    tx = pre * (g2 * (o.vx - sp.vx) - Ps * sp.rx);
-    ty = pre * (g2 * (o.vy - sp.vy) - Ps * sp.ry);
-    tz = pre * (g2 * (o.vz - sp.vz) - Ps * sp.rz);
-    return true;
+    // outside the support (F = 0) the pair's force is zero
+    const bool on = F != R(0);
+    tx = on ? pre * (g2 * (o.vx - sp.vx) - Ps * sp.rx) : R(0);
+    ty = on ? pre * (g2 * (o.vy - sp.vy) - Ps * sp.ry) : R(0);
+    tz = on ? pre * (g2 * (o.vz - sp.vz) - Ps * sp.rz) : R(0);
+    return on;
 }
 
 __device__ __forceinline__ bool fs_valid(const Lists& L, int nf) { return L.fsoff[nf] <= L.fs_cap; }
+
+// Visits slot s's fluid-solid pairs [fsoff[s], fsoff[s+1]) in batches of U:
+// the batch's solid indices, then all their records, are in flight together
+// (entries past the segment end re-read its last pair and are not used).
+// fn(sp, k, ok) must not branch on ok (keep contributions with add_if).
+template <class R, int U, class Fn>
+__device__ __forceinline__ void for_fs(const FluidArgs& a, int s, double px, double py, double pz,
+                                       Fn&& fn) {
+    const int b = a.L.fsoff[s], e = a.L.fsoff[s + 1];
+    for (int k = b; k < e; k += U) {
+        int ai[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) ai[u] = a.L.fs_a[min(k + u, e - 1)];
+        SolidP<R> sp[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) sp[u] = load_solid<R>(a.so, ai[u], px, py, pz);
+#pragma unroll
+        for (int u = 0; u < U; u++) fn(sp[u], k + u, k + u < e);
+    }
+}
 
 // --------------------------------------------------------------- records
 template <bool MIX>
@@ -397,20 +418,18 @@ __global__ void __launch_bounds__(kMaxBS) k_continuity(FluidArgs a, double dt) {
     const double4 vm = f.vr[s];
     if (fs_valid(a.L, a.nf)) {
         const R rho0 = (R)vm.w;
-        const int e = a.L.fsoff[s + 1];
-        for (int k = a.L.fsoff[s]; k < e; k++) {
-            const SolidP<R> sp = load_solid<R>(a.so, a.L.fs_a[k], ps.x, ps.y, ps.z);
+        for_fs<R, 4>(a, s, ps.x, ps.y, ps.z, [&](const SolidP<R>& sp, int, bool ok) {
             const R r = dist<R>(sp.rx * sp.rx + sp.ry * sp.ry + sp.rz * sp.rz);
             const R F = grad_factor<R>(r, R(0.5) * (o.h + R(2.6) * sp.rad));
-            if (F == R(0)) continue;
             const R d2 = R(2) * sp.rad;
             const R Va = d2 * d2 * d2;
             R Pa, rhoa;
             wall_state<R>(o.P, o.rho, o.c, rho0, sp, (R)a.gx, (R)a.gy, (R)a.gz, Pa, rhoa);
             const R dvr = (o.vx - sp.vx) * sp.rx + (o.vy - sp.vy) * sp.ry + (o.vz - sp.vz) * sp.rz;
-            acc.add(R(2) * o.rho * Va * F * (R(2) * rhoa * dvr + (o.P - Pa) * r * rcp_(o.c)) *
-                    rcp_(o.rho + rhoa));
-        }
+            acc.add_if(ok && F != R(0), R(2) * o.rho * Va * F *
+                                            (R(2) * rhoa * dvr + (o.P - Pa) * r * rcp_(o.c)) *
+                                            rcp_(o.rho + rhoa));
+        });
     }
     const double nd = f.drho[a.cur][s] + dt * acc.get();
     f.drho[1 - a.cur][s] = nd;
@@ -473,14 +492,12 @@ __global__ void __launch_bounds__(kMaxBS) k_momentum(FluidArgs a, int write_fs) 
         const double4 vm = f.vr[s];
         const double h13 = ps.w / 1.3;
         const double mi = h13 * h13 * h13 * vm.w;
-        const int e = a.L.fsoff[s + 1];
-        for (int k = a.L.fsoff[s]; k < e; k++) {
-            const SolidP<R> sp = load_solid<R>(a.so, a.L.fs_a[k], ps.x, ps.y, ps.z);
-            R tx = 0, ty = 0, tz = 0;
+        for_fs<R, 4>(a, s, ps.x, ps.y, ps.z, [&](const SolidP<R>& sp, int k, bool ok) {
+            R tx, ty, tz;
             const bool on = fs_term<MIX>(o, (R)vm.w, sp, (R)a.gx, (R)a.gy, (R)a.gz, tx, ty, tz);
-            if (on) acc.add(tx, ty, tz);
-            if (write_fs) a.L.fsF[k] = make_double4(mi * tx, mi * ty, mi * tz, 0.0);
-        }
+            acc.add_if(ok && on, tx, ty, tz);
+            if (write_fs && ok) a.L.fsF[k] = make_double4(mi * tx, mi * ty, mi * tz, 0.0);
+        });
     }
     f.acc[s] = make_double4(acc.x + a.gx, acc.y + a.gy, acc.z + a.gz, 0.0);
 }

@@ -158,12 +158,16 @@ __device__ __forceinline__ unsigned long long dbits(double d) {
 // grid-stride over the particles, one atomic per workgroup (thousands of
 // per-wavefront atomics on one word serialise in L2)
 // displacement in units of the particle's scene skin
+// largest displacement since the last build, in units of the scene's skin,
+// over p[s] for s < n, or over p[list[t]] for t < n
 __global__ void __launch_bounds__(kBS) k_disp(Grids G, bool fluid, const double4* __restrict__ p,
-                                              const double4* __restrict__ xb, int n,
+                                              const double4* __restrict__ xb,
+                                              const int* __restrict__ list, int n,
                                               unsigned long long* slot) {
     __shared__ unsigned long long wmax[kBS / 64];
     unsigned long long m = 0;
-    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        const int s = list ? list[t] : t;
         const double4 a = p[s], b = xb[s];
         const double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
         const double skin = grid_at(G, fluid ? fluid_scene(G, s) : solid_scene(G, s)).skin;
@@ -591,7 +595,7 @@ __device__ void lane_walk(const SearchArgs& a, const double4 p, int s, int& c, i
 // skip candidates outside the kernel support themselves.  Fluid–solid pairs
 // are filtered exactly for every row (COUNT, then FILL into the CSR).
 template <bool FILL>
-__global__ void __launch_bounds__(kBS) k_filter(SearchArgs a) {
+__global__ void __launch_bounds__(kBS) k_filter(SearchArgs a, int zero_fs) {
     const int s = xcd_block() * blockDim.x + threadIdx.x;
     int c = 0, cs = 0;
     if (s < a.nf) {
@@ -632,7 +636,7 @@ __global__ void __launch_bounds__(kBS) k_filter(SearchArgs a) {
                         if (in && w < a.L.fs_cap) {
                             a.L.fs_i[w] = s;
                             a.L.fs_a[w] = ai[u];
-                            a.L.fsF[w] = make_double4(0, 0, 0, 0);
+                            if (zero_fs) a.L.fsF[w] = make_double4(0, 0, 0, 0);
                         }
                         w += in;
                     } else {
@@ -907,14 +911,15 @@ void hash_solid(hipStream_t s, const Grids& G, const double4* pr, int n, unsigne
 }
 
 void rebuild_decision(hipStream_t s, const Grids& G, const double4* fph, const double4* fxb, int nf,
-                      const double4* spr, const double4* sxb, int ns, int always, int always_sort,
-                      unsigned long long* ctl, long long* status) {
+                      const double4* spr, const double4* sxb, const int* slive, int nslive, int always,
+                      int always_sort, unsigned long long* ctl, long long* status) {
     if (nf > 0)
         hipLaunchKernelGGL(k_disp, dim3(std::min(blocks_for(nf), 1024)), dim3(kBS), 0, s, G, true, fph,
-                           fxb, nf, ctl + CTL_DISP_F);
-    if (ns > 0)
-        hipLaunchKernelGGL(k_disp, dim3(std::min(blocks_for(ns), 1024)), dim3(kBS), 0, s, G, false, spr,
-                           sxb, ns, ctl + CTL_DISP_S);
+                           fxb, (const int*)nullptr, nf, ctl + CTL_DISP_F);
+    // static solids never move: only the live ones are checked
+    if (nslive > 0)
+        hipLaunchKernelGGL(k_disp, dim3(std::min(blocks_for(nslive), 1024)), dim3(kBS), 0, s, G, false,
+                           spr, sxb, slive, nslive, ctl + CTL_DISP_S);
     hipLaunchKernelGGL(k_decide, dim3(1), dim3(1), 0, s, ctl, (unsigned long long*)status, always,
                        always_sort);
 }
@@ -963,13 +968,13 @@ void fluid_exact_rows(hipStream_t s, const SearchArgs& a, int* out, int* out_cnt
         hipLaunchKernelGGL(k_exact_rows, dim3(blocks_for(a.nf)), dim3(kBS), 0, s, a, out, out_cnt);
 }
 
-void fluid_filter(hipStream_t s, const SearchArgs& a, int* scan_tmp) {
+void fluid_filter(hipStream_t s, const SearchArgs& a, int* scan_tmp, bool zero_fs) {
     if (a.nf <= 0) return;
-    hipLaunchKernelGGL(k_filter<false>, dim3(blocks_for(a.nf)), dim3(kBS), 0, s, a);
+    hipLaunchKernelGGL(k_filter<false>, dim3(blocks_for(a.nf)), dim3(kBS), 0, s, a, 0);
     exclusive_scan(s, a.L.fscnt, a.L.fsoff, a.nf, scan_tmp);
     hipLaunchKernelGGL(k_fs_status, dim3(1), dim3(1), 0, s, (const int*)a.L.fsoff, a.nf, a.L.fs_cap,
                        a.st);
-    hipLaunchKernelGGL(k_filter<true>, dim3(blocks_for(a.nf)), dim3(kBS), 0, s, a);
+    hipLaunchKernelGGL(k_filter<true>, dim3(blocks_for(a.nf)), dim3(kBS), 0, s, a, (int)zero_fs);
 }
 
 static void reverse_build(hipStream_t s, const int* key, const int* total_p, int64_t cap,
