@@ -89,6 +89,12 @@ struct dsph_ctx {
     int nfchunks = 0;
     unsigned* hslot = nullptr;  // reference hash per fluid slot
     unsigned char* cspare = nullptr;  // per cell: its stencil reaches the spare cell
+    // the static solids' grid (wall dummies never move: sorted once, at
+    // upload); with a moving dummy every search sorts all solids instead
+    int* dcs = nullptr;
+    int* dce = nullptr;
+    int* dperm = nullptr;
+    bool all_solid_sort = false;
     double4* fxb = nullptr;     // fluid positions at the last superset build
     double4* fsp = nullptr;     // fluid positions at the last fluid search
     double4* sxb = nullptr;     // solid positions at the last superset build
@@ -389,9 +395,20 @@ SearchArgs search_args(dsph_ctx* c) {
     a.forig = c->fset[0].orig;
     a.hslot = c->hslot;
     a.cspare = c->cspare;
-    a.scs = c->scs;
-    a.sce = c->sce;
-    a.sperm = c->sperm;
+    if (c->all_solid_sort) {
+        a.scs = c->scs;
+        a.sce = c->sce;
+        a.sperm = c->sperm;
+        a.acs = a.ace = a.aperm = a.active = nullptr;
+    } else {
+        a.scs = c->dcs;
+        a.sce = c->dce;
+        a.sperm = c->dperm;
+        a.acs = c->acs;
+        a.ace = c->ace;
+        a.aperm = c->aperm;
+        a.active = c->so.active;
+    }
     a.skey = c->skey;
     a.spr = c->so.pr;
     a.fsp = c->fsp;
@@ -430,16 +447,25 @@ void neighbor_search(dsph_ctx* c, int i_step, int gap, bool save_history, bool z
             radix_sort_iota(s, c->frank, c->nf, c->key_bits + c->sub_bits, c->fkey_sorted, c->fperm,
                             c->radix_tmp, gate);
             cell_spans(s, c->fkey_sorted, c->nf, c->fcs, c->fce, c->ncell, c->cinv, gate, c->sub_bits);
-            if (c->ns > 0)
-                radix_sort_iota(s, c->skey, c->ns, c->key_bits, c->skey_sorted, c->sperm, c->radix_tmp,
-                                gate);
-            cell_spans(s, c->skey_sorted, c->ns, c->scs, c->sce, c->ncell, nullptr, gate);
+            if (c->all_solid_sort) {
+                if (c->ns > 0)
+                    radix_sort_iota(s, c->skey, c->ns, c->key_bits, c->skey_sorted, c->sperm, c->radix_tmp,
+                                    gate);
+                cell_spans(s, c->skey_sorted, c->ns, c->scs, c->sce, c->ncell, nullptr, gate);
+            }
             if (c->reorder_period > 0)
                 reorder_fluid(s, c->fset[0], c->fset[1], c->fperm, c->fslot, c->nf, c->mixed(),
                               c->dcur, c->ctl);
             member_slots(s, c->fperm, c->fslot, c->fkey_sorted, c->sub_bits, c->cinv, c->fmember, c->hslot,
                          c->nf, gate);
         }
+    }
+    if (c->ns > 0) {
+        // the active solids' grid: the DEM search, and the fluid's view of
+        // the moving solids
+        FamTimer t(c, F_SSEARCH);
+        active_grid(s, c->so, c->skey, c->ncell, c->key_bits, c->akey, c->akey_sorted, c->aperm,
+                    c->acs, c->ace, c->radix_tmp);
     }
     if (fluid_step) {
         const SearchArgs sa = search_args(c);
@@ -456,8 +482,6 @@ void neighbor_search(dsph_ctx* c, int i_step, int gap, bool save_history, bool z
     if (c->ns > 0) {
         FamTimer t(c, F_SSEARCH);
         if (save_history) solid_save_history(s, c->L, c->ns);
-        active_grid(s, c->so, c->skey, c->ncell, c->key_bits, c->akey, c->akey_sorted, c->aperm,
-                    c->acs, c->ace, c->radix_tmp);
         solid_search(s, grids_view(c), c->so, c->ns, c->acs, c->ace, c->aperm, c->L, c->scan_tmp, stw);
         reverse_solid(s, c->L, c->ns, c->scan_tmp);
     }
@@ -601,6 +625,18 @@ void require(bool cond, const std::string& msg) {
 }  // namespace
 
 namespace {
+// calculatePointHash (neighborSearch.cu:19-36) on the host, with the
+// device's operations (k_search.hip point_hash): the scene's own cell, or
+// its spare cell outside [min, max) or at the rounding edge of max
+int host_point_hash(const GridP& g, double x, double y, double z) {
+    if (!(g.minx <= x && x < g.maxx && g.miny <= y && y < g.maxy && g.minz <= z && z < g.maxz))
+        return g.ncell - 1;
+    const int ix = (int)((x - g.minx) / g.csx), iy = (int)((y - g.miny) / g.csy),
+              iz = (int)((z - g.minz) / g.csz);
+    const int h = iz * g.gy * g.gx + iy * g.gx + ix;
+    return (h >= 0 && h < g.ncell - 1) ? h : g.ncell - 1;
+}
+
 // Slot order of the fluid: cells in Morton (z-order) order of their (x, y, z)
 // indices, members of a cell in ascending index.  A cell's members stay
 // contiguous, as the reference's hash sort keeps them, while spatially close
@@ -1165,7 +1201,7 @@ int dsph_upload_batch(dsph_ctx* c, const dsph_scene* scenes, int32_t n_scenes) {
         superset_stencil(c);  // also copies the grids to the device
         c->key_bits = 1;
         while ((1LL << c->key_bits) < c->ncell) c->key_bits++;
-        c->sub_bits = std::max(0, std::min(9, (32 - c->key_bits) / 3 * 3));
+        c->sub_bits = std::max(0, std::min(6, (32 - c->key_bits) / 3 * 3));
         c->sscene = dalloc<int>(c, std::max(ns, 1));
         if (ns) h2d(c, c->sscene, M.sscene.data(), ns);
         c->dwoff = dalloc<int>(c, n_scenes + 1);
@@ -1423,6 +1459,43 @@ int dsph_upload_batch(dsph_ctx* c, const dsph_scene* scenes, int32_t n_scenes) {
         c->aperm = dalloc<int>(c, active.size());
         c->acs = dalloc<int>(c, c->ncell);
         c->ace = dalloc<int>(c, c->ncell);
+        fill_int(c->st, c->acs, c->ncell, -1);
+        fill_int(c->st, c->ace, c->ncell, -1);
+        fill_int(c->st, (int*)c->akey_sorted, (int64_t)active.size(), 0);
+        {
+            // the static solids' grid, sorted once by their reference hash
+            // (a stable counting sort: ascending solid index within a cell)
+            c->all_solid_sort = false;
+            for (int i = 0; i < ns; i++)
+                if (sdum[i] == 1 && (svel[i].w > 0.0 || svel[i].x != 0.0 || svel[i].y != 0.0 ||
+                                     svel[i].z != 0.0 || sw[i].x != 0.0 || sw[i].y != 0.0 || sw[i].z != 0.0 ||
+                                     sclump[i] >= 0))
+                    c->all_solid_sort = true;
+            std::vector<int> key(ns, -1), cnt((size_t)c->ncell + 1, 0), perm;
+            for (int i = 0; i < ns; i++) {
+                if (sdum[i] != 1) continue;
+                const GridP& g = c->grids[c->h_sscene[i]];
+                key[i] = g.cell0 + host_point_hash(g, spr[i].x, spr[i].y, spr[i].z);
+                cnt[key[i] + 1]++;
+            }
+            for (int h = 0; h < c->ncell; h++) cnt[h + 1] += cnt[h];
+            std::vector<int> cs(c->ncell, -1), ce(c->ncell, -1), at(cnt.begin(), cnt.end() - 1);
+            perm.resize(cnt[c->ncell]);
+            for (int i = 0; i < ns; i++)
+                if (key[i] >= 0) perm[at[key[i]]++] = i;
+            for (int h = 0; h < c->ncell; h++)
+                if (cnt[h + 1] > cnt[h]) {
+                    cs[h] = cnt[h];
+                    ce[h] = cnt[h + 1];
+                }
+            c->dcs = dalloc<int>(c, c->ncell);
+            c->dce = dalloc<int>(c, c->ncell);
+            c->dperm = dalloc<int>(c, std::max<size_t>(perm.size(), 1));
+            h2d(c, c->dcs, cs.data(), c->ncell);
+            h2d(c, c->dce, ce.data(), c->ncell);
+            h2d(c, c->dperm, perm.data(), perm.size());
+            HIPCHK(hipStreamSynchronize(c->st));
+        }
 
         // ---- clumps
         std::vector<double4> cpos(nc), cvel(nc), cw(nc), cq(nc);

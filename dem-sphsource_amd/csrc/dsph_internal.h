@@ -254,9 +254,17 @@ struct SearchArgs {
     const int* forig;       // caller index of each fluid slot
     const unsigned* hslot;  // reference hash of each fluid slot
     const unsigned char* cspare;  // per cell: its 27-offset stencil reaches the spare cell
-    const int* scs;         // solid cell ranges
+    // the solids the fluid sees, as two grids: the static solids (wall
+    // dummies, sorted once) and the active grid of the non-dummies (sorted
+    // every step for the DEM search); without the second (acs null) the
+    // first holds every solid
+    const int* scs;         // cell ranges and solid indices of the first grid
     const int* sce;
     const int* sperm;
+    const int* acs;         // active grid: cell ranges over positions whose
+    const int* ace;         // solid is active[aperm[position]]
+    const int* aperm;
+    const int* active;
     const unsigned* skey;   // reference hash of each solid
     const double4* spr;
     double4* fsp;           // fluid positions of the last fluid search, slot order
@@ -297,6 +305,9 @@ void reverse_movable(hipStream_t s, const int* pair_key, const int* pair_off_tot
                      const double4* key_vel, int nkeys, int* cnt, int* off, int* idx,
                      int* scan_tmp);
 void solid_save_history(hipStream_t s, Lists& L, int ns);
+// the active grid; up to kSmallSort solids in one workgroup (LDS bitonic
+// sort, resetting only the cells the previous sort filled)
+constexpr int kSmallSort = 4096;
 void active_grid(hipStream_t s, const Solids& so, const unsigned* skey, int ncell, int key_bits,
                  unsigned* akey, unsigned* akey_sorted, int* aperm, int* acs, int* ace,
                  unsigned* radix_tmp);

@@ -427,36 +427,41 @@ __global__ void __launch_bounds__(kBS) k_build(SearchArgs a, const int2* __restr
                         }
                     }
                     if (__ballot(d2 <= culls) != 0) {
-                        const int st1 = __builtin_amdgcn_readfirstlane(a.scs[h]);
-                        const int en = __builtin_amdgcn_readfirstlane(a.sce[h]);
-                        for (int base = st1; st1 >= 0 && base < en; base += 64) {
-                            const int u = base + lane;
-                            if (u < en) {
-                                const int ai = a.sperm[u];
-                                const double4 q = a.spr[ai];
-                                lq[wbase + lane] = make_float4((float)(q.x - ox), (float)(q.y - oy),
-                                                               (float)(q.z - oz), 0.0f);
-                                lj[wbase + lane] = ai;
-                            }
-                            __builtin_amdgcn_wave_barrier();
-                            const int nb = min(64, en - base);
-                            for (int k = 0; k < nb; k += 4) {
-                                float4 q[4];
-                                int aa[4];
-#pragma unroll
-                                for (int v = 0; v < 4; v++) {
-                                    q[v] = lq[wbase + ((k + v) & 63)];
-                                    aa[v] = lj[wbase + ((k + v) & 63)];
+                        // the cell's static solids, then its active ones
+                        for (int grid = 0; grid < (a.acs ? 2 : 1); grid++) {
+                            const int* gcs = grid ? a.acs : a.scs;
+                            const int* gce = grid ? a.ace : a.sce;
+                            const int st1 = __builtin_amdgcn_readfirstlane(gcs[h]);
+                            const int en = __builtin_amdgcn_readfirstlane(gce[h]);
+                            for (int base = st1; st1 >= 0 && base < en; base += 64) {
+                                const int u = base + lane;
+                                if (u < en) {
+                                    const int ai = grid ? a.active[a.aperm[u]] : a.sperm[u];
+                                    const double4 q = a.spr[ai];
+                                    lq[wbase + lane] = make_float4((float)(q.x - ox), (float)(q.y - oy),
+                                                                   (float)(q.z - oz), 0.0f);
+                                    lj[wbase + lane] = ai;
                                 }
+                                __builtin_amdgcn_wave_barrier();
+                                const int nb = min(64, en - base);
+                                for (int k = 0; k < nb; k += 4) {
+                                    float4 q[4];
+                                    int aa[4];
 #pragma unroll
-                                for (int v = 0; v < 4; v++) {
-                                    const float ex = px - q[v].x, ey = py - q[v].y, ez = pz - q[v].z;
-                                    const bool in = (k + v < nb) & (ex * ex + ey * ey + ez * ez <= culls);
-                                    if (in && cs < fsupK) a.L.fsup[(int64_t)cs * a.nf + s] = aa[v];
-                                    cs += in;
+                                    for (int v = 0; v < 4; v++) {
+                                        q[v] = lq[wbase + ((k + v) & 63)];
+                                        aa[v] = lj[wbase + ((k + v) & 63)];
+                                    }
+#pragma unroll
+                                    for (int v = 0; v < 4; v++) {
+                                        const float ex = px - q[v].x, ey = py - q[v].y, ez = pz - q[v].z;
+                                        const bool in = (k + v < nb) & (ex * ex + ey * ey + ez * ez <= culls);
+                                        if (in && cs < fsupK) a.L.fsup[(int64_t)cs * a.nf + s] = aa[v];
+                                        cs += in;
+                                    }
                                 }
+                                __builtin_amdgcn_wave_barrier();
                             }
-                            __builtin_amdgcn_wave_barrier();
                         }
                     }
                 }
@@ -569,10 +574,12 @@ __device__ void lane_walk(const SearchArgs& a, const double4 p, int s, int& c, i
             for (int xx = -1; xx <= 1; xx++) {
                 const int h = (az + zz) * g.gy * g.gx + (ay + yy) * g.gx + (ax + xx);
                 if (h < 0 || h >= g.ncell) continue;
-                const int st1 = a.scs[g.cell0 + h];
+                for (int grid = 0; grid < (a.acs ? 2 : 1); grid++) {
+                const int st1 = (grid ? a.acs : a.scs)[g.cell0 + h];
                 if (st1 < 0) continue;
-                for (int t = st1; t < a.sce[g.cell0 + h]; t++) {
-                    const int ai = a.sperm[t];
+                const int en1 = (grid ? a.ace : a.sce)[g.cell0 + h];
+                for (int t = st1; t < en1; t++) {
+                    const int ai = grid ? a.active[a.aperm[t]] : a.sperm[t];
                     const double4 q = a.spr[ai];
                     if (!within(p.x, p.y, p.z, q.x, q.y, q.z, 2.0 * hi)) continue;
                     if (FILL) {
@@ -585,6 +592,7 @@ __device__ void lane_walk(const SearchArgs& a, const double4 p, int s, int& c, i
                     } else {
                         cs++;
                     }
+                }
                 }
             }
 }
@@ -754,6 +762,49 @@ __global__ void k_ss_save(const int* __restrict__ ssoff, int ns, const int* __re
         h_j[i] = ss_j[i];
 #pragma unroll
         for (int k = 0; k < 9; k++) hS[9 * i + k] = ssS[9 * i + k];
+    }
+}
+
+// The active grid of up to kSmallSort solids in one workgroup: a stable
+// (key, position) bitonic sort in LDS, then the cell ranges.  The cells the
+// previous sort filled (its sorted keys, still in `sorted`) are reset first,
+// so the ncell-long range arrays stay -1 elsewhere without a fill.
+__global__ void __launch_bounds__(1024) k_active_small(const int* __restrict__ active, int n,
+                                                       const unsigned* __restrict__ skey,
+                                                       unsigned* __restrict__ sorted, int* __restrict__ perm,
+                                                       int* __restrict__ cs, int* __restrict__ ce) {
+    __shared__ unsigned long long v[kSmallSort];
+    const int T = blockDim.x;
+    for (int t = threadIdx.x; t < n; t += T) {
+        const unsigned k = sorted[t];
+        cs[k] = -1;
+        ce[k] = -1;
+    }
+    int N2 = 1;
+    while (N2 < n) N2 <<= 1;
+    for (int t = threadIdx.x; t < N2; t += T)
+        v[t] = t < n ? ((unsigned long long)skey[active[t]] << 32) | (unsigned)t : ~0ull;
+    __syncthreads();
+    for (int k = 2; k <= N2; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < N2; i += T) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const unsigned long long x = v[i], y = v[l];
+                    if (((i & k) == 0) == (x > y)) {
+                        v[i] = y;
+                        v[l] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    for (int t = threadIdx.x; t < n; t += T) {
+        const unsigned k = (unsigned)(v[t] >> 32);
+        sorted[t] = k;
+        perm[t] = (int)(unsigned)v[t];
+        if (t == 0 || (unsigned)(v[t - 1] >> 32) != k) cs[k] = t;
+        if (t == n - 1 || (unsigned)(v[t + 1] >> 32) != k) ce[k] = t + 1;
     }
 }
 
@@ -1009,6 +1060,11 @@ void active_grid(hipStream_t s, const Solids& so, const unsigned* skey, int ncel
                  unsigned* akey, unsigned* akey_sorted, int* aperm, int* acs, int* ace,
                  unsigned* radix_tmp) {
     const int n = so.nactive;
+    if (n > 0 && n <= kSmallSort) {
+        hipLaunchKernelGGL(k_active_small, dim3(1), dim3(1024), 0, s, (const int*)so.active, n, skey,
+                           akey_sorted, aperm, acs, ace);
+        return;
+    }
     if (n > 0) {
         hipLaunchKernelGGL(k_active_keys, dim3(blocks_for(n)), dim3(kBS), 0, s, (const int*)so.active, n,
                            skey, akey);
