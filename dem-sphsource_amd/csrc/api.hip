@@ -216,7 +216,7 @@ struct FamTimer {
 void alloc_ff(dsph_ctx* c, int K) {
     dfree(c, c->L.nbr);
     c->L.K = K;
-    c->L.nbr = dalloc<int>(c, (size_t)K * std::max(c->nf, 1));
+    c->L.nbr = dalloc<int>(c, (size_t)(K + kRowPad) * std::max(c->nf, 1));
 }
 
 void alloc_sup(dsph_ctx* c, int K, int KS) {
@@ -224,7 +224,7 @@ void alloc_sup(dsph_ctx* c, int K, int KS) {
     dfree(c, c->L.fsup);
     c->L.supK = K;
     c->L.fsupK = KS;
-    c->L.sup = dalloc<int>(c, (size_t)K * std::max(c->nf, 1));
+    c->L.sup = dalloc<int>(c, (size_t)(K + kRowPad) * std::max(c->nf, 1));
     c->L.fsup = dalloc<int>(c, (size_t)KS * std::max(c->nf, 1));
 }
 
@@ -233,7 +233,7 @@ void alloc_sup(dsph_ctx* c, int K, int KS) {
 void grow_sup_keep(dsph_ctx* c, int K) {
     if (K <= c->L.supK) return;
     const size_t nf = (size_t)std::max(c->nf, 1);
-    int* fresh = dalloc<int>(c, (size_t)K * nf);
+    int* fresh = dalloc<int>(c, (size_t)(K + kRowPad) * nf);
     HIPCHK(hipMemcpyAsync(fresh, c->L.sup, sizeof(int) * (size_t)c->L.supK * nf, hipMemcpyDeviceToDevice,
                           c->st));
     HIPCHK(hipStreamSynchronize(c->st));
@@ -1243,6 +1243,9 @@ int dsph_upload_batch(dsph_ctx* c, const dsph_scene* scenes, int32_t n_scenes) {
             f.scene = dalloc<int>(c, std::max(nf, 1));
             f.tid = dalloc<int>(c, nf);
             const size_t rec = 2 * (mx ? sizeof(float4) : sizeof(double4));
+            // the pair loops address the record array with 32-bit byte offsets
+            if ((size_t)nf * rec > 0x7fffffffull)
+                throw ApiError{DSPH_E_INVALID, "too many fluid particles for one context"};
             for (int b = 0; b < 2; b++) f.rec[b] = dalloc<char>(c, rec * std::max(nf, 1));
             // fixed-point span of the mixed records: the domains of all
             // scenes plus one such extent on each side of every axis
@@ -1255,9 +1258,13 @@ int dsph_upload_batch(dsph_ctx* c, const dsph_scene* scenes, int32_t n_scenes) {
                 hi[1] = std::max(hi[1], gg.maxy);
                 hi[2] = std::max(hi[2], gg.maxz);
             }
+            // one unit on every axis (the pair loops work in position
+            // units), set by the longest extent
+            double Lmax = 0.0;
+            for (int a = 0; a < 3; a++) Lmax = std::max(Lmax, hi[a] - lo[a]);
+            const double unit = 3.0 * std::max(Lmax, 1e-300) / 4294967296.0;
             for (int a = 0; a < 3; a++) {
                 const double L = hi[a] - lo[a];
-                const double unit = 3.0 * L / 4294967296.0;
                 f.qo[a] = lo[a] - L;
                 f.qi[a] = 1.0 / unit;
                 f.qr[a] = (float)unit;

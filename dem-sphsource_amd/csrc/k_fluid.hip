@@ -19,17 +19,19 @@
 //                FP64 and the FP32 records refreshed.
 //
 // Neighbour records (ping-ponged with dRho so the Jacobi continuity never
-// overwrites what neighbours read):
-//   FP64: ph (x, y, z, h) + pv (v, rho) + pq (P, c, V = m/rho, gamma = nu rho),
-//         three f64x4;
-//   MIX:  p0 (x, y, z as 32-bit fixed point, h) + pv (v, P) + pq (c, V, rho,
-//         gamma), three 16-byte records.
-// A candidate's position record (with h) decides the kernel support; the
-// other two records are read only for pairs inside it.  The fixed-point map spans the domain
-//         plus one domain length on each side; r_i - r_j is the exact integer
-//         difference (|d| < 2^24 units converts to FP32 exactly) times the
-//         unit, so pair geometry never suffers the cancellation of absolute
-//         FP32 coordinates.
+// overwrites what neighbours read), two quads per slot, interleaved:
+//   FP64: (x, y, z, table index) + (v, dRho), two f64x4 (64 B);
+//   MIX:  (x, y, z as 32-bit fixed point, table index) + (v, dRho) as f32x4
+//         (32 B).
+// The per-particle constants (h, c, m, rho0, nu) live in a small property
+// table indexed by the table index (one entry when the scene's fluid is
+// uniform, and then not read at all); rho = rho0 + dRho and P = c^2 dRho are
+// derived in registers.  Every candidate's records are loaded and its term
+// computed, then kept only inside the kernel support (no data-dependent
+// branch around the loads).  The fixed-point map spans the domain plus one
+// domain length on each side; r_i - r_j is the exact integer difference
+// (|d| < 2^24 units converts to FP32 exactly) times the unit, so pair
+// geometry never suffers the cancellation of absolute FP32 coordinates.
 //
 // The Riemann algebra is rearranged (same quantities, fewer operations):
 //   grad W_ij = F(r,h) r_ij with F = -105/(16 pi) t^3 / h^5, t = 1 - r/(2h)
@@ -61,10 +63,11 @@ template <> __device__ __forceinline__ float rsq_<float>(float x) { return __bui
 
 // Neighbour sums.  FP64 path: plain FP64 sums.  Mixed path: FP32 terms
 // summed in FP64 (a convert and an FP64 add per term, no compensation
-// needed).  add_if(on, x) adds x when on and +0 otherwise, without a branch:
-// the pair loops compute every candidate's term and keep it only inside the
-// kernel support, so the compiler cannot sink a candidate's record loads
-// behind its support test; +0 leaves an FP64 sum unchanged.
+// needed), so a sum does not depend on how rows are batched or shared.
+// add_if(on, x) adds x when on and +0 otherwise, without a branch: the pair
+// loops compute every candidate's term and keep it only inside the kernel
+// support, so the compiler cannot sink a candidate's record loads behind its
+// support test; +0 leaves an FP64 sum unchanged.
 template <class R>
 struct Acc1 {
     double s = 0.0;
@@ -103,34 +106,73 @@ struct Nb {
     R rho, P;
 };
 
-// record quads of slot j: two 16-byte quads (mixed) or two f64x4 (FP64)
+// the record quads of slot j, interleaved: the position quad, then the
+// state quad (16 bytes each in the mixed path, f64x4 in the FP64 path), so
+// one cache line serves both gathers of a candidate
 template <bool MIX>
-__device__ __forceinline__ const typename Prec<MIX>::R4* rec_of(const FluidSet& f, int cur, int j) {
-    return (const typename Prec<MIX>::R4*)f.rec[cur] + 2 * (int64_t)j;
+__device__ __forceinline__ typename Prec<MIX>::R4* pos_w(const FluidSet& f, int cur, int j) {
+    return (typename Prec<MIX>::R4*)f.rec[cur] + 2 * (int64_t)j;
 }
 template <bool MIX>
-__device__ __forceinline__ typename Prec<MIX>::R4* rec_w(const FluidSet& f, int cur, int j) {
-    return (typename Prec<MIX>::R4*)f.rec[cur] + 2 * (int64_t)j;
+__device__ __forceinline__ typename Prec<MIX>::R4* state_w(const FluidSet& f, int cur, int j) {
+    return (typename Prec<MIX>::R4*)f.rec[cur] + 2 * (int64_t)j + 1;
 }
 
 // the position quad (position and table index)
 template <bool MIX>
 __device__ __forceinline__ void load_pos(const FluidSet& f, int cur, int j, Nb<MIX>& o) {
     if constexpr (MIX) {
-        const uint4 a = *(const uint4*)rec_of<MIX>(f, cur, j);
+        const uint4 a = *(const uint4*)pos_w<MIX>(f, cur, j);
         o.px = a.x; o.py = a.y; o.pz = a.z; o.id = (int)a.w;
     } else {
-        const double4 a = rec_of<MIX>(f, cur, j)[0];
+        const double4 a = *pos_w<MIX>(f, cur, j);
         o.px = a.x; o.py = a.y; o.pz = a.z; o.id = (int)a.w;
     }
 }
 // the state quad (velocity, dRho)
 template <bool MIX>
 __device__ __forceinline__ void load_state(const FluidSet& f, int cur, int j, Nb<MIX>& o) {
-    const auto b = rec_of<MIX>(f, cur, j)[1];
+    const auto b = *state_w<MIX>(f, cur, j);
     o.vx = b.x; o.vy = b.y; o.vz = b.z; o.drho = b.w;
 }
-// the particle's property-table entry: h, c, m, rho0 and, with NU, nu.
+
+// Neighbour records are gathered through a buffer resource over the record
+// array: the per-lane address is a 32-bit byte offset (no 64-bit address
+// arithmetic per candidate) and an offset past the array reads zeros
+// instead of faulting, so candidate indices need no clamping.
+constexpr int kRsrcWord3 = 0x00020000;  // gfx9 raw buffer access
+template <bool MIX> constexpr unsigned kRecBytes = MIX ? 32u : 64u;
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+using Rsrc = __amdgpu_buffer_rsrc_t;
+template <bool MIX>
+__device__ __forceinline__ Rsrc rec_rsrc(const FluidSet& f, int cur, int nf) {
+    return __builtin_amdgcn_make_buffer_rsrc(f.rec[cur], (short)0, (int)((unsigned)nf * kRecBytes<MIX>),
+                                             kRsrcWord3);
+}
+template <bool MIX, bool STATE>
+__device__ __forceinline__ void load_rec(Rsrc rs, int j, Nb<MIX>& o) {
+    const unsigned off = (unsigned)j * kRecBytes<MIX>;
+    if constexpr (MIX) {
+        const u32x4 a = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
+        o.px = a.x; o.py = a.y; o.pz = a.z; o.id = (int)a.w;
+        if constexpr (STATE) {
+            const f32x4 b = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(rs, off + 16, 0, 0));
+            o.vx = b.x; o.vy = b.y; o.vz = b.z; o.drho = b.w;
+        }
+    } else {
+        const f64x2 a0 = __builtin_bit_cast(f64x2, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
+        const f64x2 a1 = __builtin_bit_cast(f64x2, __builtin_amdgcn_raw_buffer_load_b128(rs, off + 16, 0, 0));
+        o.px = a0.x; o.py = a0.y; o.pz = a1.x; o.id = (int)a1.y;
+        if constexpr (STATE) {
+            const f64x2 b0 = __builtin_bit_cast(f64x2, __builtin_amdgcn_raw_buffer_load_b128(rs, off + 32, 0, 0));
+            const f64x2 b1 = __builtin_bit_cast(f64x2, __builtin_amdgcn_raw_buffer_load_b128(rs, off + 48, 0, 0));
+            o.vx = b0.x; o.vy = b0.y; o.vz = b1.x; o.drho = b1.y;
+        }
+    }
+}
+
 // UNI: every fluid particle has the same entry, held by the launch (no load)
 template <bool MIX, bool NU, bool UNI>
 __device__ __forceinline__ void load_props(const FluidArgs& a, Nb<MIX>& o) {
@@ -159,21 +201,30 @@ __device__ __forceinline__ Nb<MIX> load_nb(const FluidArgs& a, int cur, int j) {
     return o;
 }
 
-// r_i - r_j in the pair precision
+// r_i - r_j in position units: the exact integer difference of the fixed
+// point coordinates (mixed; one unit = FluidSet::qr metres on every axis) or
+// the FP64 difference in metres
 template <bool MIX>
-__device__ __forceinline__ void rel(const FluidSet& f, const Nb<MIX>& o, const Nb<MIX>& q,
-                                    typename Prec<MIX>::R& rx, typename Prec<MIX>::R& ry,
-                                    typename Prec<MIX>::R& rz) {
+__device__ __forceinline__ void rel(const Nb<MIX>& o, const Nb<MIX>& q, typename Prec<MIX>::R& rx,
+                                    typename Prec<MIX>::R& ry, typename Prec<MIX>::R& rz) {
     if constexpr (MIX) {
-        rx = (float)(int)(o.px - q.px) * f.qr[0];
-        ry = (float)(int)(o.py - q.py) * f.qr[1];
-        rz = (float)(int)(o.pz - q.pz) * f.qr[2];
+        rx = (float)(int)(o.px - q.px);
+        ry = (float)(int)(o.py - q.py);
+        rz = (float)(int)(o.pz - q.pz);
     } else {
         rx = o.px - q.px;
         ry = o.py - q.py;
         rz = o.pz - q.pz;
     }
 }
+// metres per position unit
+template <bool MIX>
+__device__ __forceinline__ typename Prec<MIX>::R pos_unit(const FluidSet& f) {
+    if constexpr (MIX) return f.qr[0];
+    else return 1.0;
+}
+template <class R> __device__ __forceinline__ R sqrt_(R x) { return sqrt(x); }
+template <> __device__ __forceinline__ float sqrt_<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 // 32-bit fixed point of one coordinate (clamped to the mapped span)
 __device__ __forceinline__ unsigned fixq(double x, double origin, double inv_unit) {
@@ -185,7 +236,7 @@ __device__ __forceinline__ unsigned fixq(double x, double origin, double inv_uni
 // the position quad of slot s from its FP64 position
 template <bool MIX>
 __device__ __forceinline__ void store_pos(const FluidSet& f, int buf, int s, const double4& p) {
-    auto* r = rec_w<MIX>(f, buf, s);
+    auto* r = pos_w<MIX>(f, buf, s);
     const int id = f.tid[s];
     if constexpr (MIX)
         *(uint4*)r = make_uint4(fixq(p.x, f.qo[0], f.qi[0]), fixq(p.y, f.qo[1], f.qi[1]),
@@ -200,8 +251,7 @@ __device__ __forceinline__ void write_records(const FluidSet& f, int buf, int s,
                                               double vx, double vy, double vz, double drho) {
     using R = typename Prec<MIX>::R;
     store_pos<MIX>(f, buf, s, p);
-    auto* r = rec_w<MIX>(f, buf, s);
-    r[1] = {(R)vx, (R)vy, (R)vz, (R)drho};
+    *state_w<MIX>(f, buf, s) = {(R)vx, (R)vy, (R)vz, (R)drho};
 }
 
 // the neighbour row of slot s: the exact walked row or the Verlet superset;
@@ -227,40 +277,42 @@ enum NbMode { POS = 0, STATE = 1, STATE_NU = 2 };
 // ahead and its U records are gathered before any of them is used: a
 // wavefront keeps U record gathers in flight instead of walking one
 // dependent index -> record chain per candidate.  Every load is issued
-// unconditionally (indices past the row end re-read its last entry and are
-// clamped into [0, nf)) so the loads of a batch overlap; fn(q, ok) sees
-// every loaded candidate in row order, ok = false past the row end and for
-// the particle itself (a shared row holds its group), and must not branch on
-// it (keep its contribution with add_if).  The table entries
-// of a batch are read once its position quads are in; a table shared by
-// many particles is served to the whole wavefront from one cache line.
+// unconditionally so the loads of a batch overlap: the index reads run up
+// to 2U - 1 columns past the row end, into the kRowPad columns every row
+// array carries, and the record gathers are buffer loads, safe for any
+// index.  fn(q, ok) sees every loaded candidate in row order, ok = false
+// past the row end, and must not branch on it (keep its contribution with
+// add_if).  The particle itself (a shared row holds its group) is at
+// distance 0, outside the support tests of continuity and momentum; in POS
+// mode ok is also false for it.
+// The table entries of a batch are read once its position quads are in; a
+// table shared by many particles is served to the whole wavefront from one
+// cache line.
 constexpr int kBatch = 4;
+static_assert(2 * kBatch <= kRowPad, "row arrays need 2 kBatch pad columns");
 template <bool MIX, int U, NbMode MODE, bool UNI, class Fn>
-__device__ __forceinline__ void for_row(const FluidArgs& a, int s, Fn&& fn) {
+__device__ __forceinline__ void for_row(const FluidArgs& a, Rsrc rs, int s, Fn&& fn) {
     const Row row = nb_row(a.L, s);
     const int n = row.n;
-    const int last = n > 0 ? n - 1 : 0;
     const int64_t st = a.nf;
-    const int jmax = a.nf - 1;
-    auto index = [&](int k) {
-        // read once per pass: streamed past the caches the records live in
-        const int j = __builtin_nontemporal_load(row.p + (int64_t)min(k, last) * st);
-        return min(max(j, 0), jmax);
-    };
+    const int* ip = row.p;
     int jn[U];
 #pragma unroll
-    for (int u = 0; u < U; u++) jn[u] = index(u);
+    for (int u = 0; u < U; u++) jn[u] = __builtin_nontemporal_load(ip + u * st);
     for (int k = 0; k < n; k += U) {
+        ip += U * st;
         Nb<MIX> q[U];
         bool ok[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            load_pos<MIX>(a.f, a.cur, jn[u], q[u]);
-            if constexpr (MODE != POS) load_state<MIX>(a.f, a.cur, jn[u], q[u]);
-            ok[u] = (k + u < n) & (jn[u] != s);
+            load_rec<MIX, MODE != POS>(rs, jn[u], q[u]);
+            // POS (the Shepard sum) counts coincident particles, so it
+            // tells the particle itself apart by index
+            ok[u] = (k + u < n) & (MODE != POS || jn[u] != s);
         }
+        // read once per pass: streamed past the caches the records live in
 #pragma unroll
-        for (int u = 0; u < U; u++) jn[u] = index(k + U + u);
+        for (int u = 0; u < U; u++) jn[u] = __builtin_nontemporal_load(ip + u * st);
         if constexpr (MODE != POS) {
 #pragma unroll
             for (int u = 0; u < U; u++) {
@@ -391,8 +443,12 @@ __global__ void __launch_bounds__(kBS) k_records(FluidSet f, int n, int buf) {
 //             = common [(rho_i + rho_j) gbar dv - N r],
 //               common = (2 / rho_i) m_j F / (rho_j (rho_i + rho_j)),
 // (V_j = m_j / rho_j), so a pair costs one reciprocal besides 1/h and the
-// distance.  UNI: all fluid shares one (h, c, m, rho0, nu): 1/h and c are
-// constants and (rho_i + rho_j) gbar = 2 nu rho_i rho_j.
+// distance.  Pair geometry is in position units (r = u r_u with u metres per
+// unit): t = 1 - r/(2h) = 1 + (-u / (2h)) r_u, and u goes into per-particle
+// constants with the constant of F.  UNI: all fluid shares one (h, c, m,
+// rho0, nu), so 1/h, c and m are constants, dP / c = c (dRho_i - dRho_j) and
+// (rho_i + rho_j) gbar = 2 nu rho_i rho_j.  A candidate counts when
+// r >= 1e-10 m (r_u^2 >= r2min) and t > 0 (r < 2h).
 template <bool MIX, bool UNI>
 __global__ void __launch_bounds__(kMaxBS) k_continuity(FluidArgs a, double dt) {
     using R = typename Prec<MIX>::R;
@@ -401,20 +457,41 @@ __global__ void __launch_bounds__(kMaxBS) k_continuity(FluidArgs a, double dt) {
     const FluidSet& f = a.f;
     const Nb<MIX> o = load_nb<MIX, UNI>(a, a.cur, s);
     const double4 ps = f.ph[s];
-    const R uih = UNI ? R(1) / o.h : R(0);
-    const R two_rho_i = R(2) * o.rho;
+    const Rsrc rs = rec_rsrc<MIX>(f, a.cur, a.nf);
+    const R u = pos_unit<MIX>(f);
+    const R r2min = R(1e-20) / (u * u);
+    // UNI: term = K t^3 [rho_j dvr_u + (c dRho_i - c dRho_j) r_u] / (rho_j (rho_i + rho_j))
+    const R ih = UNI ? R(1) / o.h : R(1);
+    const R ih2 = ih * ih;
+    const R nh = R(-0.5) * u * ih;
+    const R K = R(2) * o.rho * o.m * R(kGradC) * (ih2 * ih2 * ih) * u;
+    const R cd_i = o.c * o.drho;
     Acc1<R> acc;
-    for_row<MIX, kBatch, STATE, UNI>(a, s, [&](const Nb<MIX>& q, bool ok) {
-        R rx, ry, rz;
-        rel<MIX>(f, o, q, rx, ry, rz);
-        const R r = dist<R>(rx * rx + ry * ry + rz * rz);
-        const R hb = UNI ? o.h : R(0.5) * (o.h + q.h);
-        const R F = grad_factor_ih<R>(r, hb, UNI ? uih : rcp_(hb));
-        const R dvr = (o.vx - q.vx) * rx + (o.vy - q.vy) * ry + (o.vz - q.vz) * rz;
-        const R c = UNI ? o.c : R(0.5) * (o.c + q.c);
-        const R num = q.rho * c * dvr + (o.P - q.P) * r;
-        acc.add_if(ok && F != R(0), two_rho_i * q.m * F * num * rcp_(c * q.rho * (o.rho + q.rho)));
-    });
+    for_row<MIX, kBatch, STATE, UNI>(
+        a, rs, s,
+        [&](const Nb<MIX>& q, bool ok) {
+            R rx, ry, rz;
+            rel<MIX>(o, q, rx, ry, rz);
+            const R r2 = rx * rx + ry * ry + rz * rz;
+            const R r = sqrt_(r2);
+            const R dvr = (o.vx - q.vx) * rx + (o.vy - q.vy) * ry + (o.vz - q.vz) * rz;
+            R t, term;
+            if constexpr (UNI) {
+                t = R(1) + nh * r;
+                const R num = q.rho * dvr + (cd_i - o.c * q.drho) * r;
+                term = K * (t * t * t) * num * rcp_(q.rho * (o.rho + q.rho));
+            } else {
+                const R hb = R(0.5) * (o.h + q.h);
+                const R ihb = rcp_(hb);
+                t = R(1) - R(0.5) * u * ihb * r;
+                const R ihb2 = ihb * ihb;
+                const R F = R(kGradC) * (t * t * t) * (ihb2 * ihb2 * ihb);
+                const R c = R(0.5) * (o.c + q.c);
+                const R num = q.rho * c * dvr + (o.P - q.P) * r;
+                term = R(2) * o.rho * q.m * u * F * num * rcp_(c * q.rho * (o.rho + q.rho));
+            }
+            acc.add_if(ok & (r2 >= r2min) & (t > R(0)), term);
+        });
     const double4 vm = f.vr[s];
     if (fs_valid(a.L, a.nf)) {
         const R rho0 = (R)vm.w;
@@ -445,49 +522,83 @@ __global__ void __launch_bounds__(kMaxBS) k_momentum(FluidArgs a, int write_fs) 
     const FluidSet& f = a.f;
     const Nb<MIX> o = load_nb<MIX, UNI>(a, a.cur, s);
     const double4 ps = f.ph[s];
-    const R uih = UNI ? R(1) / o.h : R(0);
+    const Rsrc rs = rec_rsrc<MIX>(f, a.cur, a.nf);
+    const R u = pos_unit<MIX>(f);
+    const R r2min = R(1e-20) / (u * u);
     const R two_over_rho = R(2) * rcp_(o.rho);
     const R gam_i = o.nu * o.rho;
-    const R two_nu_rho_i = R(2) * o.nu * o.rho;
+    // UNI: common = Km t^3 / (rho_j (rho_i + rho_j)), and the pair's force is
+    // common (G rho_j dv - uN r_u) with uN = u N =
+    //   rho_j (u P_i + u rho_i beta dU) + (u rho_i c^2) dRho_j
+    const R ih = UNI ? R(1) / o.h : R(1);
+    const R ih2 = ih * ih;
+    const R nh = R(-0.5) * u * ih;
+    const R Km = two_over_rho * o.m * R(kGradC) * (ih2 * ih2 * ih);
+    const R G = R(2) * o.nu * o.rho;
+    const R uP = u * o.P;
+    const R uB = u * o.rho * o.c * o.c;
+    const R uE = u * o.rho;
     // artificial viscosity (SPHKernelDEMContactModel.h:74-77), opt-in: the
     // particle's alpha times m_j h_ij c_ij (v_ij . r_ij) / (rho_ij r^2 +
     // 0.01 h_ij^2) grad W_ij, oriented to remove kinetic energy
     const bool art = (a.flags & DSPH_FLAG_ART_VISCOSITY) != 0;
     const R alpha = art ? (R)f.alpha[s] : R(0);
     Acc3<R> acc;
-    for_row<MIX, kBatch, STATE_NU, UNI>(a, s, [&](const Nb<MIX>& q, bool ok) {
-        R rx, ry, rz;
-        rel<MIX>(f, o, q, rx, ry, rz);
-        const R r2 = rx * rx + ry * ry + rz * rz;
-        const R ir = r2 > R(0) ? rsq_(r2) : R(0);
-        const R r = r2 * ir;
-        const R hb = UNI ? o.h : R(0.5) * (o.h + q.h);
-        const R F = grad_factor_ih<R>(r, hb, UNI ? uih : rcp_(hb));
-        const R c = UNI ? o.c : R(0.5) * (o.c + q.c);
-        const R dvx = o.vx - q.vx, dvy = o.vy - q.vy, dvz = o.vz - q.vz;
-        // U_L - U_R = -(v_i - v_j) . e_ij
-        const R dvr = dvx * rx + dvy * ry + dvz * rz;
-        const R dU = -dvr * ir;
-        R beta = R(3) * (dU > R(0) ? dU : R(0));
-        beta = beta < c ? beta : c;
-        const R rsum = o.rho + q.rho;
-        const R common = two_over_rho * q.m * F * rcp_(q.rho * rsum);
-        // N = rho_i P_j + rho_j P_i + rho_i rho_j beta dU
-        R sr = -common * (o.rho * q.P + q.rho * o.P + o.rho * q.rho * beta * dU);
-        // (rho_i + rho_j) gbar, gbar = 2 gamma_i gamma_j / (gamma_i + gamma_j)
-        R vis;
-        if constexpr (UNI) {
-            vis = common * two_nu_rho_i * q.rho;
-        } else {
-            const R gam_j = q.nu * q.rho;
-            vis = common * rsum * R(2) * gam_i * gam_j * rcp_(gam_i + gam_j);
-        }
-        if (art) {
-            const R rb = R(0.5) * rsum;
-            sr += q.m * alpha * hb * c * dvr * rcp_(rb * r2 + R(0.01) * hb * hb) * F;
-        }
-        acc.add_if(ok && F != R(0), vis * dvx + sr * rx, vis * dvy + sr * ry, vis * dvz + sr * rz);
-    });
+    // the artificial-viscosity switch is taken once, outside the loop
+    auto rows = [&](auto art_tag) {
+        constexpr bool ART = decltype(art_tag)::value;
+        for_row<MIX, kBatch, STATE_NU, UNI>(
+            a, rs, s,
+            [&](const Nb<MIX>& q, bool ok) {
+                R rx, ry, rz;
+                rel<MIX>(o, q, rx, ry, rz);
+                const R r2 = rx * rx + ry * ry + rz * rz;
+                const R ir = rsq_(r2);
+                const R r = r2 * ir;
+                const R dvx = o.vx - q.vx, dvy = o.vy - q.vy, dvz = o.vz - q.vz;
+                const R dvr = dvx * rx + dvy * ry + dvz * rz;
+                // U_L - U_R = -(v_i - v_j) . e_ij (no unit)
+                const R dU = -dvr * ir;
+                R t, cv, cn, hb, c;
+                if constexpr (UNI) {
+                    hb = o.h;
+                    c = o.c;
+                    t = R(1) + nh * r;
+                    const R beta = fmin(R(3) * fmax(dU, R(0)), c);
+                    const R common = Km * (t * t * t) * rcp_(q.rho * (o.rho + q.rho));
+                    cv = common * G * q.rho;
+                    cn = common * (q.rho * (uP + uE * beta * dU) + uB * q.drho);
+                } else {
+                    hb = R(0.5) * (o.h + q.h);
+                    c = R(0.5) * (o.c + q.c);
+                    const R ihb = rcp_(hb);
+                    t = R(1) - R(0.5) * u * ihb * r;
+                    const R ihb2 = ihb * ihb;
+                    const R F = R(kGradC) * (t * t * t) * (ihb2 * ihb2 * ihb);
+                    const R beta = fmin(R(3) * fmax(dU, R(0)), c);
+                    const R rsum = o.rho + q.rho;
+                    const R common = two_over_rho * q.m * F * rcp_(q.rho * rsum);
+                    // (rho_i + rho_j) gbar, gbar = 2 gamma_i gamma_j / (gamma_i + gamma_j)
+                    const R gam_j = q.nu * q.rho;
+                    cv = common * rsum * R(2) * gam_i * gam_j * rcp_(gam_i + gam_j);
+                    // N = rho_i P_j + rho_j P_i + rho_i rho_j beta dU
+                    cn = common * u * (o.rho * q.P + q.rho * o.P + o.rho * q.rho * beta * dU);
+                }
+                if constexpr (ART) {
+                    // in metres: dvr u, r^2 u^2, grad W = F u r_u
+                    const R ihb = rcp_(hb);
+                    const R ihb2 = ihb * ihb;
+                    const R F = R(kGradC) * (t * t * t) * (ihb2 * ihb2 * ihb);
+                    const R rb = R(0.5) * (o.rho + q.rho);
+                    cn -= u * q.m * alpha * hb * c * (dvr * u) * rcp_(rb * (r2 * u * u) + R(0.01) * hb * hb) *
+                          F;
+                }
+                acc.add_if(ok & (r2 >= r2min) & (t > R(0)), cv * dvx - cn * rx, cv * dvy - cn * ry,
+                           cv * dvz - cn * rz);
+            });
+    };
+    if (art) rows(std::true_type{});
+    else rows(std::false_type{});
     if (fs_valid(a.L, a.nf)) {
         const double4 vm = f.vr[s];
         const double h13 = ps.w / 1.3;
@@ -538,14 +649,18 @@ __global__ void __launch_bounds__(kMaxBS) k_density_reinit(FluidArgs a, int i_st
     sw.add(sigma);  // W(0, h)
     Nb<MIX> o;
     load_pos<MIX>(f, a.cur, s, o);
-    for_row<MIX, kBatch, POS, false>(a, s, [&](const Nb<MIX>& q, bool ok) {
-        R rx, ry, rz;
-        rel<MIX>(f, o, q, rx, ry, rz);
-        const R qq = dist<R>(rx * rx + ry * ry + rz * rz) * ih;
-        const R t = R(1) - R(0.5) * qq;
-        const R t2 = t * t;
-        sw.add_if(ok && qq < R(2), sigma * t2 * t2 * (R(1) + R(2) * qq));
-    });
+    const Rsrc rs = rec_rsrc<MIX>(f, a.cur, a.nf);
+    const R ihu = pos_unit<MIX>(f) * ih;  // 1/h per position unit
+    for_row<MIX, kBatch, POS, false>(
+        a, rs, s,
+        [&](const Nb<MIX>& q, bool ok) {
+            R rx, ry, rz;
+            rel<MIX>(o, q, rx, ry, rz);
+            const R qq = sqrt_(rx * rx + ry * ry + rz * rz) * ihu;
+            const R t = R(1) - R(0.5) * qq;
+            const R t2 = t * t;
+            sw.add_if(ok & (qq < R(2)), sigma * t2 * t2 * (R(1) + R(2) * qq));
+        });
     if (fs_valid(a.L, a.nf)) {
         const double h13 = p.w / 1.3;
         const R invV0 = rcp_((R)(h13 * h13 * h13));
@@ -607,10 +722,10 @@ __global__ void k_kick_drift(FluidSet f, int n, double dt, int cur) {
     p.z += hd * v.z;
     f.ph[s] = p;
     using R = typename Prec<MIX>::R;
-    auto* rec = rec_w<MIX>(f, cur, s);
-    rec[1].x = (R)v.x;
-    rec[1].y = (R)v.y;
-    rec[1].z = (R)v.z;
+    auto* st = state_w<MIX>(f, cur, s);
+    st->x = (R)v.x;
+    st->y = (R)v.y;
+    st->z = (R)v.z;
     store_pos<MIX>(f, cur, s, p);
 }
 
