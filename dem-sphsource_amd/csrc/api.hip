@@ -91,6 +91,12 @@ struct dsph_ctx {
     unsigned char* cspare = nullptr;  // per cell: its stencil reaches the spare cell
     // the static solids' grid (wall dummies never move: sorted once, at
     // upload); with a moving dummy every search sorts all solids instead
+    // the solids' hashes are set for all of them (the static ones keep
+    // theirs); static_out: static solids out of the domain
+    bool skey_valid = false;
+    unsigned long long* dispmax = nullptr;  // per-workgroup fluid displacement maxima
+    int* rank_acc = nullptr;                // active-grid rank accumulators (zero between sorts)
+    unsigned long long static_out = 0;
     int* dcs = nullptr;
     int* dce = nullptr;
     int* dperm = nullptr;
@@ -360,6 +366,7 @@ void build_chunks(dsph_ctx* c) {
 void superset_stencil(dsph_ctx* c) {
     for (GridP& g : c->grids) {
         g.skin = c->skin_h * g.hmax;
+        g.iskin2 = g.skin > 0.0 ? 1.0 / (g.skin * g.skin) : 1e300;
         const double R = (2.0 + c->skin_h) * g.hmax * (1.0 + 1e-6);
         const double cmin = std::min({g.csx, g.csy, g.csz});
         int w = cmin > 0 ? (int)std::ceil(R / cmin) : 1;
@@ -436,13 +443,17 @@ void neighbor_search(dsph_ctx* c, int i_step, int gap, bool save_history, bool z
         FamTimer t(c, F_GRID);
         const Grids G = grids_view(c);
         hash_fluid(s, G, c->fset[0].ph, c->fset[0].orig, c->nf, c->sub_bits, c->crank, c->cspare, c->fkey,
-                   c->frank, c->hslot, c->ctl, stw);
-        hash_solid(s, G, c->so.pr, c->ns, c->skey, stw);
+                   c->frank, c->hslot, fluid_step ? c->fxb : nullptr, c->dispmax, c->ctl, stw);
+        bool walk_only = false;
+        for (const GridP& g : c->grids) walk_only = walk_only || g.walk_only;
+        // only the live solids move: after one full pass the others' hashes
+        // stay valid (static_out of them are out of the domain)
+        const bool full = !c->skey_valid;
+        if (full) hash_solid(s, G, c->so.pr, c->ns, c->skey, stw);
+        hash_live_decide(s, G, c->so.pr, c->so.live, full ? 0 : c->so.nlive, c->skey, full ? 0 : c->static_out,
+                         fluid_step, c->sxb, c->dispmax, c->nf, c->skin_h <= 0.0, walk_only, c->ctl, stw);
+        c->skey_valid = true;
         if (fluid_step) {
-            bool walk_only = false;
-            for (const GridP& g : c->grids) walk_only = walk_only || g.walk_only;
-            rebuild_decision(s, G, c->fset[0].ph, c->fxb, c->nf, c->so.pr, c->sxb, c->so.live,
-                             c->so.nlive, c->skin_h <= 0.0, walk_only, c->ctl, stw);
             const Gate gate = c->ctl + CTL_SORT;
             radix_sort_iota(s, c->frank, c->nf, c->key_bits + c->sub_bits, c->fkey_sorted, c->fperm,
                             c->radix_tmp, gate);
@@ -465,7 +476,7 @@ void neighbor_search(dsph_ctx* c, int i_step, int gap, bool save_history, bool z
         // the moving solids
         FamTimer t(c, F_SSEARCH);
         active_grid(s, c->so, c->skey, c->ncell, c->key_bits, c->akey, c->akey_sorted, c->aperm,
-                    c->acs, c->ace, c->radix_tmp);
+                    c->acs, c->ace, c->radix_tmp, c->rank_acc);
     }
     if (fluid_step) {
         const SearchArgs sa = search_args(c);
@@ -476,14 +487,14 @@ void neighbor_search(dsph_ctx* c, int i_step, int gap, bool save_history, bool z
         FamTimer t(c, F_FSLIST);
         fluid_filter(s, sa, c->scan_tmp, zero_fs);
         if (c->ns > 0)
-            reverse_movable(s, c->L.fs_a, c->L.fsoff + c->nf, c->L.fs_cap, c->so.vel, c->ns,
-                            c->L.fsr_cnt, c->L.fsr_off, c->L.fsr_idx, c->scan_tmp);
+            reverse_movable(s, c->L.fs_a, c->L.fsoff + c->nf, c->L.fs_cap, c->so.vel, c->so, c->ns, c->L,
+                            c->scan_tmp);
     }
     if (c->ns > 0) {
         FamTimer t(c, F_SSEARCH);
-        if (save_history) solid_save_history(s, c->L, c->ns);
+        if (save_history) solid_save_history(s, c->so, c->L, c->ns);
         solid_search(s, grids_view(c), c->so, c->ns, c->acs, c->ace, c->aperm, c->L, c->scan_tmp, stw);
-        reverse_solid(s, c->L, c->ns, c->scan_tmp);
+        reverse_solid(s, c->so, c->L, c->ns, c->scan_tmp);
     }
 }
 
@@ -1387,6 +1398,7 @@ int dsph_upload_batch(dsph_ctx* c, const dsph_scene* scenes, int32_t n_scenes) {
         h2d(c, c->fxb, ph.data(), nf);
         c->fsp = dalloc<double4>(c, nf);
         h2d(c, c->fsp, ph.data(), nf);
+        c->dispmax = dalloc<unsigned long long>(c, std::max(1, blocks_for(std::max(nf, 1))));
         c->ctl = dalloc<unsigned long long>(c, CTL_WORDS);
         HIPCHK(hipMemsetAsync(c->ctl, 0, CTL_WORDS * sizeof(unsigned long long), c->st));
         c->scs = dalloc<int>(c, c->ncell);
@@ -1457,6 +1469,18 @@ int dsph_upload_batch(dsph_ctx* c, const dsph_scene* scenes, int32_t n_scenes) {
             h2d(c, so.nrm, snrm.data(), ns);
         }
         c->skey = dalloc<unsigned>(c, ns);
+        c->skey_valid = false;
+        {
+            // static solids outside the domain (their hashes never change)
+            std::vector<char> is_live(ns, 0);
+            for (int i : live) is_live[i] = 1;
+            c->static_out = 0;
+            for (int i = 0; i < ns; i++) {
+                if (is_live[i]) continue;
+                const GridP& g = c->grids[c->h_sscene[i]];
+                if (host_point_hash(g, spr[i].x, spr[i].y, spr[i].z) == g.ncell - 1) c->static_out++;
+            }
+        }
         c->skey_sorted = dalloc<unsigned>(c, ns);
         c->sperm = dalloc<int>(c, ns);
         c->sxb = dalloc<double4>(c, ns);
@@ -1464,6 +1488,8 @@ int dsph_upload_batch(dsph_ctx* c, const dsph_scene* scenes, int32_t n_scenes) {
         c->akey = dalloc<unsigned>(c, active.size());
         c->akey_sorted = dalloc<unsigned>(c, active.size());
         c->aperm = dalloc<int>(c, active.size());
+        c->rank_acc = dalloc<int>(c, 3 * std::max<size_t>(active.size(), 1));
+        fill_int(c->st, c->rank_acc, 3 * (int64_t)std::max<size_t>(active.size(), 1), 0);
         c->acs = dalloc<int>(c, c->ncell);
         c->ace = dalloc<int>(c, c->ncell);
         fill_int(c->st, c->acs, c->ncell, -1);
@@ -1596,6 +1622,7 @@ int dsph_upload_batch(dsph_ctx* c, const dsph_scene* scenes, int32_t n_scenes) {
         alloc_sup(c, nf > 0 ? 160 : 1, nf > 0 ? 96 : 1);
         L.sup_cnt = dalloc<int>(c, nf);
         L.fsup_cnt = dalloc<int>(c, nf);
+        L.fsm = dalloc<unsigned long long>(c, 2 * (size_t)std::max(nf, 1));
         fill_int(c->st, L.sup_cnt, nf, 0);
         fill_int(c->st, L.fsup_cnt, nf, 0);
         L.grow = dalloc<int>(c, nf);
@@ -1606,18 +1633,26 @@ int dsph_upload_batch(dsph_ctx* c, const dsph_scene* scenes, int32_t n_scenes) {
             HIPCHK(hipStreamSynchronize(c->st));
         }
         alloc_fs(c, std::max<int64_t>(4096, 8LL * nf));
+        // solids without pairs keep count 0 and offset 0 throughout
         L.fsr_cnt = dalloc<int>(c, ns);
         L.fsr_off = dalloc<int>(c, (size_t)ns + 1);
+        fill_int(c->st, L.fsr_cnt, ns, 0);
         fill_int(c->st, L.fsr_off, (int64_t)ns + 1, 0);
         L.sscnt = dalloc<int>(c, ns);
         L.ssoff = dalloc<int>(c, (size_t)ns + 1);
         L.hoff = dalloc<int>(c, (size_t)ns + 1);
+        L.hcnt = dalloc<int>(c, ns);
         L.ssr_cnt = dalloc<int>(c, ns);
         L.ssr_off = dalloc<int>(c, (size_t)ns + 1);
+        L.rev_tmp = dalloc<int>(c, ns);
+        L.rev_cur = dalloc<int>(c, ns);
         fill_int(c->st, L.ssoff, (int64_t)ns + 1, 0);
         fill_int(c->st, L.sscnt, ns, 0);  // dummies are never searched: their count stays 0
         fill_int(c->st, L.hoff, (int64_t)ns + 1, 0);
+        fill_int(c->st, L.hcnt, ns, 0);
+        fill_int(c->st, L.ssr_cnt, ns, 0);
         fill_int(c->st, L.ssr_off, (int64_t)ns + 1, 0);
+        fill_int(c->st, L.rev_tmp, ns, 0);  // zero between reverse builds
         alloc_ss(c, std::max<int64_t>(4096, 6LL * (int64_t)active.size() + 4096), false);
 
         // ---- scratch

@@ -52,7 +52,8 @@ enum {
     CTL_OUT_AT_BUILD = 5,  // the last build had particles out of the domain
     CTL_NEAR_SPARE = 6,    // some fluid particle's stencil reaches the spare cell
     CTL_SORT = 7,          // this search sorts the grids (rebuild or exact walks)
-    CTL_WORDS = 8
+    CTL_TICKET = 8,        // blocks of the solid hash done (the last one decides)
+    CTL_WORDS = 9
 };
 
 // the reference's spatialGrid (deviceStructs.h:976-1014) of one scene.  A
@@ -73,6 +74,7 @@ struct GridP {
     double pad;     // conservative enlargement of cell boxes for pruning
     double hmax;    // largest fluid smoothing length of the scene
     double skin;    // Verlet skin of the scene's fluid superset (absolute)
+    double iskin2;  // 1 / skin^2 (1e300 without a skin)
 };
 
 // the grids of all scenes and the scene of every particle
@@ -195,6 +197,9 @@ struct Lists {
     int fsupK = 0;
     int* fsup = nullptr;
     int* fsup_cnt = nullptr;
+    // members of the fluid–solid list among the first 128 candidates of
+    // each fluid slot's solid superset row (two words per slot)
+    unsigned long long* fsm = nullptr;
     // solid–solid CSR by solid index (the reference's own list)
     int64_t ss_cap = 0;
     int* sscnt = nullptr;
@@ -204,11 +209,19 @@ struct Lists {
     double4* ssF = nullptr;
     double4* ssT = nullptr;
     double* ssS = nullptr;  // 9 per contact: sliding, rolling, torsion
+    // Offsets of the solid-keyed CSRs (ssoff, hoff, ssr_off, fsr_off) are
+    // kept only for solids that can have pairs (the active ones; movable
+    // ones for fsr), plus the total at index NS: a segment is
+    // [off[i], off[i] + cnt[i]).
     // previous step's list (spring history)
     int* hoff = nullptr;    // NS + 1
+    int* hcnt = nullptr;    // NS
     int* h_j = nullptr;
     double* hS = nullptr;
     // reverse solid–solid: CSR by pointing solid
+    // reverse-CSR scratch: per-solid counts (zero between builds) and cursors
+    int* rev_tmp = nullptr;
+    int* rev_cur = nullptr;
     int* ssr_cnt = nullptr;
     int* ssr_off = nullptr;
     int* ssr_idx = nullptr;
@@ -237,6 +250,14 @@ void copy_int(hipStream_t s, int* dst, const int* src, int64_t n);
 // exclusive scan of n ints into out[0..n] (out[n] = total); tmp >= scan_tmp_size(n)
 int64_t scan_tmp_size(int64_t n);
 void exclusive_scan(hipStream_t s, const int* in, int* out, int64_t n, int* tmp, Gate gate = nullptr);
+// exclusive scan over a list of indices (ascending; all others count 0), in
+// one workgroup: off[keys[r]] = sum of the counts of keys[0..r), off[n_all]
+// = total; with cnt_out the counts move from cnt (left zero) to cnt_out;
+// with cursor, cursor[key] = off[key]
+void scan_keys(hipStream_t s, const int* keys, int nkeys, int* cnt, int* cnt_out, int* off, int* cursor,
+               int n_all);
+// lists at most this long are scanned by scan_keys (one workgroup)
+constexpr int kKeyScanMax = 16384;
 // stable LSD radix sort of (key, index) pairs; keys < 2^bits; vals out = perm
 int64_t radix_tmp_size(int64_t n);
 void radix_sort_iota(hipStream_t s, const unsigned* keys_in, int64_t n, int bits,
@@ -278,21 +299,30 @@ struct SearchArgs {
     unsigned long long* st;
 };
 // also the reference hash of every slot, and CTL_NEAR_SPARE when a particle's
-// cell has a stencil reaching the spare cell
+// cell has a stencil reaching the spare cell; with fxb, the largest squared
+// displacement since the last build (in skin^2 units) of each workgroup of
+// 256 particles into bmax (blocks_for(n) words)
 // The fluid sort key: the cell's Morton rank above the Morton code of the
 // particle's sub-cell (sub_bits / 3 bits per axis), so a cell's members are
 // contiguous and themselves in z-order.
 void hash_fluid(hipStream_t s, const Grids& G, const double4* ph, const int* orig, int n, int sub_bits,
                 const unsigned* crank, const unsigned char* cspare, unsigned* key_orig,
-                unsigned* rank_orig, unsigned* hslot, unsigned long long* ctl, long long* status);
+                unsigned* rank_orig, unsigned* hslot, const double4* fxb, unsigned long long* bmax,
+                unsigned long long* ctl, long long* status);
+// the reference hash of every solid (and the out-of-domain count)
 void hash_solid(hipStream_t s, const Grids& G, const double4* pr, int n, unsigned* key,
                 long long* status);
-// displacements since the last build in units of each particle's scene skin;
-// always != 0 rebuilds at every search
-// sets CTL_REBUILD, and CTL_SORT = rebuild, exact walks or always_sort
-void rebuild_decision(hipStream_t s, const Grids& G, const double4* fph, const double4* fxb, int nf,
-                      const double4* spr, const double4* sxb, const int* slive, int nslive, int always,
-                      int always_sort,
+// The per-step solid hash: only the live solids move, so only their hashes
+// are recomputed (the others' were set at upload; static_out of them are
+// out of the domain).  With decide, also the live solids' displacement
+// since the last build, and then the rebuild decision, taken by the last
+// workgroup to finish once every displacement and out-of-domain count is
+// in: CTL_REBUILD when d_f + max(d_f, d_s) >= skin (displacements in units
+// of each scene's skin) or always != 0, and CTL_SORT = rebuild, exact walks
+// or always_sort.
+void hash_live_decide(hipStream_t s, const Grids& G, const double4* pr, const int* live, int nlive,
+                      unsigned* key, unsigned long long static_out, bool decide, const double4* sxb,
+                      const unsigned long long* fbmax, int nf, int always, int always_sort,
                       unsigned long long* ctl, long long* status);
 void reorder_fluid(hipStream_t s, const FluidSet& f, const FluidSet& scratch, const int* perm,
                    int* slot_of, int n, bool mixed, int cur, const unsigned long long* ctl);
@@ -306,19 +336,20 @@ void fluid_superset(hipStream_t s, const SearchArgs& a, const int2* chunks, int 
 // capacity max(K, supK)); out_cnt = -1 marks a walked row that overflowed
 void fluid_exact_rows(hipStream_t s, const SearchArgs& a, int* out, int* out_cnt);
 void fluid_filter(hipStream_t s, const SearchArgs& a, int* scan_tmp, bool zero_fs);
+// fluid–solid pairs by movable solid (fsr_*)
 void reverse_movable(hipStream_t s, const int* pair_key, const int* pair_off_total, int64_t cap,
-                     const double4* key_vel, int nkeys, int* cnt, int* off, int* idx,
-                     int* scan_tmp);
-void solid_save_history(hipStream_t s, Lists& L, int ns);
-// the active grid; up to kSmallSort solids in one workgroup (LDS bitonic
-// sort, resetting only the cells the previous sort filled)
+                     const double4* key_vel, const Solids& so, int ns, Lists& L, int* scan_tmp);
+void solid_save_history(hipStream_t s, const Solids& so, Lists& L, int ns);
+// the active grid; up to kSmallSort solids by counting ranks (resetting
+// only the cells the previous sort filled; rank_acc: 3 zeroed ints per
+// active solid, zero again afterwards)
 constexpr int kSmallSort = 4096;
 void active_grid(hipStream_t s, const Solids& so, const unsigned* skey, int ncell, int key_bits,
                  unsigned* akey, unsigned* akey_sorted, int* aperm, int* acs, int* ace,
-                 unsigned* radix_tmp);
+                 unsigned* radix_tmp, int* rank_acc);
 void solid_search(hipStream_t s, const Grids& G, const Solids& so, int ns, const int* acs,
                   const int* ace, const int* aperm, Lists& L, int* scan_tmp, long long* status);
-void reverse_solid(hipStream_t s, Lists& L, int ns, int* scan_tmp);
+void reverse_solid(hipStream_t s, const Solids& so, Lists& L, int ns, int* scan_tmp);
 // wall normals with the smoothing length of each solid's scene (grid hmax)
 void solid_normals(hipStream_t s, const Grids& G, const Solids& so, int ns, const int* scs,
                    const int* sce, const int* sperm);

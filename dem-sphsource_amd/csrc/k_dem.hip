@@ -81,7 +81,8 @@ __global__ void k_bonds(DemArgs a, double dt) {
     }
     const int i = a.b.i[b], j = a.b.j[b];
     int kc = -1;
-    const int64_t e = a.L.ssoff[i + 1] < a.L.ss_cap ? a.L.ssoff[i + 1] : a.L.ss_cap;
+    const int64_t e1 = (int64_t)a.L.ssoff[i] + a.L.sscnt[i];
+    const int64_t e = e1 < a.L.ss_cap ? e1 : a.L.ss_cap;
     for (int64_t k = a.L.ssoff[i]; k < e; k++)
         if (a.L.ss_j[k] == j) {
             kc = (int)k;
@@ -125,7 +126,9 @@ __global__ void k_solid_sum(DemArgs a) {
     d3 F{0, 0, 0}, T{0, 0, 0};
     const bool ok = a.L.ssoff[a.ns] <= a.L.ss_cap;
     if (ok) {
-        for (int k = a.L.ssoff[i]; k < a.L.ssoff[i + 1]; k++) {
+        // segments are [off[i], off[i] + cnt[i]): offsets are kept only
+        // for solids that can have pairs
+        for (int k = a.L.ssoff[i], e = k + a.L.sscnt[i]; k < e; k++) {
             const double4 prj = a.so.pr[a.L.ss_j[k]];
             d3 n;
             double delta;
@@ -134,7 +137,7 @@ __global__ void k_solid_sum(DemArgs a) {
             F += f;
             T += v3(a.L.ssT[k]) + cross(rc - pi, f);
         }
-        for (int k = a.L.ssr_off[i]; k < a.L.ssr_off[i + 1]; k++) {
+        for (int k = a.L.ssr_off[i], e = k + a.L.ssr_cnt[i]; k < e; k++) {
             const int k1 = a.L.ssr_idx[k];
             const double4 prj = a.so.pr[a.L.ss_i[k1]];
             d3 n;
@@ -149,7 +152,7 @@ __global__ void k_solid_sum(DemArgs a) {
     const double4 v = a.so.vel[i];
     // fluid forces only matter where 1/m > 0 (F * 0 = 0 otherwise)
     if (v.w > 0.0 && a.L.fsoff && a.L.fsr_off)
-        for (int k = a.L.fsr_off[i]; k < a.L.fsr_off[i + 1]; k++) F -= v3(a.L.fsF[a.L.fsr_idx[k]]);
+        for (int k = a.L.fsr_off[i], e = k + a.L.fsr_cnt[i]; k < e; k++) F -= v3(a.L.fsF[a.L.fsr_idx[k]]);
     const double gon = (v.w > 0.0 && a.so.clump[i] < 0) ? 1.0 : 0.0;
     a.so.acc[i] = d4(F * v.w + d3{a.gx, a.gy, a.gz} * gon);
     a.so.tq[i] = d4(T);

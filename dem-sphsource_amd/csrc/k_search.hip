@@ -108,13 +108,46 @@ __device__ __forceinline__ unsigned sub_code(const GridP& g, double x, double y,
     return code;
 }
 
-__global__ void k_hash_fluid(Grids G, const double4* __restrict__ ph, const int* __restrict__ orig,
-                             int n, int sub_bits, const unsigned* __restrict__ crank,
-                             const unsigned char* __restrict__ cspare, unsigned* __restrict__ key,
-                             unsigned* __restrict__ rank, unsigned* __restrict__ hslot,
-                             unsigned long long* ctl, unsigned long long* st) {
+__device__ __forceinline__ unsigned long long dbits(double d) {
+    return (unsigned long long)__double_as_longlong(d);
+}
+
+// squared displacement of p since the build position b, in units of the
+// scene's skin^2 (a NaN position forces a rebuild)
+__device__ __forceinline__ double disp2(const GridP& g, const double4& p, const double4& b) {
+    const double dx = p.x - b.x, dy = p.y - b.y, dz = p.z - b.z;
+    const double d2 = (dx * dx + dy * dy + dz * dz) * g.iskin2;
+    return d2 == d2 ? d2 : 1e300;
+}
+
+// largest value of a workgroup (positive doubles order like their bit
+// patterns), valid in thread 0
+__device__ __forceinline__ unsigned long long block_max(unsigned long long m) {
+    __shared__ unsigned long long wmax[kBS / 64];
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        const unsigned long long u = __shfl_xor(m, d, 64);
+        m = u > m ? u : m;
+    }
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = wmax[w] > m ? wmax[w] : m;
+    return m;
+}
+
+__global__ void __launch_bounds__(kBS) k_hash_fluid(Grids G, const double4* __restrict__ ph,
+                                                    const int* __restrict__ orig, int n, int sub_bits,
+                                                    const unsigned* __restrict__ crank,
+                                                    const unsigned char* __restrict__ cspare,
+                                                    unsigned* __restrict__ key, unsigned* __restrict__ rank,
+                                                    unsigned* __restrict__ hslot,
+                                                    const double4* __restrict__ xb,
+                                                    unsigned long long* __restrict__ bmax,
+                                                    unsigned long long* ctl, unsigned long long* st) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     bool out = false, near = false;
+    double d2 = 0.0;
     if (s < n) {
         const GridP& g = grid_at(G, fluid_scene(G, s));
         double4 p = ph[s];
@@ -127,10 +160,17 @@ __global__ void k_hash_fluid(Grids G, const double4* __restrict__ ph, const int*
         rank[o] = (crank[h] << sub_bits) | sub;
         hslot[s] = (unsigned)h;
         near = cspare[h] != 0;
+        if (xb) d2 = disp2(g, p, xb[s]);
     }
     const unsigned long long m = __ballot(out);
     if ((threadIdx.x & 63) == 0 && m) atomicAdd(&st[ST_OUT_OF_DOMAIN], (unsigned long long)__popcll(m));
     if ((threadIdx.x & 63) == 0 && __ballot(near) != 0) atomicOr(&ctl[CTL_NEAR_SPARE], 1ull);
+    // one maximum per workgroup, reduced by the decision (no contended
+    // atomic on one word)
+    if (xb) {
+        const unsigned long long m2 = block_max(dbits(d2));
+        if (threadIdx.x == 0) bmax[blockIdx.x] = m2;
+    }
 }
 
 __global__ void k_hash_solid(Grids G, const double4* __restrict__ pr, int n,
@@ -149,52 +189,11 @@ __global__ void k_hash_solid(Grids G, const double4* __restrict__ pr, int n,
 }
 
 // --------------------------------------------------- rebuild decision
-__device__ __forceinline__ unsigned long long dbits(double d) {
-    return (unsigned long long)__double_as_longlong(d);
-}
-
-// largest displacement since the last build (positive doubles order like
-// their bit patterns)
-// grid-stride over the particles, one atomic per workgroup (thousands of
-// per-wavefront atomics on one word serialise in L2)
-// displacement in units of the particle's scene skin
-// largest displacement since the last build, in units of the scene's skin,
-// over p[s] for s < n, or over p[list[t]] for t < n
-__global__ void __launch_bounds__(kBS) k_disp(Grids G, bool fluid, const double4* __restrict__ p,
-                                              const double4* __restrict__ xb,
-                                              const int* __restrict__ list, int n,
-                                              unsigned long long* slot) {
-    __shared__ unsigned long long wmax[kBS / 64];
-    unsigned long long m = 0;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
-        const int s = list ? list[t] : t;
-        const double4 a = p[s], b = xb[s];
-        const double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
-        const double skin = grid_at(G, fluid ? fluid_scene(G, s) : solid_scene(G, s)).skin;
-        double d = sqrt(dx * dx + dy * dy + dz * dz) / (skin > 0.0 ? skin : 1e-300);
-        if (!(d == d)) d = 1e300;  // NaN forces a rebuild
-        const unsigned long long v = dbits(d);
-        m = v > m ? v : m;
-    }
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) {
-        const unsigned long long u = __shfl_xor(m, d, 64);
-        m = u > m ? u : m;
-    }
-    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 1; w < kBS / 64; w++) m = wmax[w] > m ? wmax[w] : m;
-        if (m) atomicMax(slot, m);
-    }
-}
-
 // rebuild when two particles may have closed the skin: d_f + max(d_f, d_s)
 // >= skin, with displacements in skin units (each scene its own skin)
-__global__ void k_decide(unsigned long long* ctl, unsigned long long* st, int always,
-                         int always_sort) {
-    const double df = __longlong_as_double((long long)ctl[CTL_DISP_F]);
-    const double ds = __longlong_as_double((long long)ctl[CTL_DISP_S]);
+__device__ void decide_rebuild(unsigned long long* ctl, unsigned long long* st, int always, int always_sort) {
+    const double df = sqrt(__longlong_as_double((long long)ctl[CTL_DISP_F]));
+    const double ds = sqrt(__longlong_as_double((long long)ctl[CTL_DISP_S]));
     const bool out = st[ST_OUT_OF_DOMAIN] != 0;
     const bool move = df + (df > ds ? df : ds) >= 1.0 - 1e-9;
     // out-of-domain particles are missing from a superset built while they
@@ -212,6 +211,53 @@ __global__ void k_decide(unsigned long long* ctl, unsigned long long* st, int al
     ctl[CTL_DISP_F] = 0;
     ctl[CTL_DISP_S] = 0;
     st[ST_REBUILDS] += r;
+}
+
+// the live solids' hashes (and, with decide, their displacements and then
+// the rebuild decision in the last workgroup to finish)
+__global__ void __launch_bounds__(kBS) k_hash_live(Grids G, const double4* __restrict__ pr,
+                                                   const int* __restrict__ live, int n,
+                                                   unsigned* __restrict__ key, unsigned long long static_out,
+                                                   int decide, const double4* __restrict__ xb,
+                                                   const unsigned long long* __restrict__ fbmax, int nfb,
+                                                   int always, int always_sort, unsigned long long* ctl,
+                                                   unsigned long long* st) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    bool out = false;
+    double d2 = 0.0;
+    if (t < n) {
+        const int i = live[t];
+        const GridP& g = grid_at(G, solid_scene(G, i));
+        const double4 p = pr[i];
+        const int hl = point_hash(g, p.x, p.y, p.z);
+        key[i] = (unsigned)(g.cell0 + hl);
+        out = hl == g.ncell - 1;
+        if (decide) d2 = disp2(g, p, xb[i]);
+    }
+    const unsigned long long m = __ballot(out);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(&st[ST_OUT_OF_DOMAIN], (unsigned long long)__popcll(m));
+    if (t == 0 && static_out) atomicAdd(&st[ST_OUT_OF_DOMAIN], static_out);
+    if (!decide) return;
+    const unsigned long long ms = block_max(dbits(d2));
+    if (threadIdx.x == 0 && ms) atomicMax(&ctl[CTL_DISP_S], ms);
+    // every thread's counts and maxima are visible device-wide before this
+    // workgroup takes its ticket; the last ticket sees all of them
+    __threadfence();
+    __syncthreads();
+    __shared__ bool last;
+    if (threadIdx.x == 0) last = atomicAdd(&ctl[CTL_TICKET], 1ull) == (unsigned long long)gridDim.x - 1;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    // the fluid's largest displacement from the hash kernel's workgroup maxima
+    unsigned long long mf = 0;
+    for (int b = threadIdx.x; b < nfb; b += blockDim.x) mf = fbmax[b] > mf ? fbmax[b] : mf;
+    mf = block_max(mf);
+    if (threadIdx.x == 0) {
+        ctl[CTL_TICKET] = 0;
+        if (mf > ctl[CTL_DISP_F]) ctl[CTL_DISP_F] = mf;
+        decide_rebuild(ctl, st, always, always_sort);
+    }
 }
 
 // --------------------------------------------- sorted-order reorder
@@ -619,12 +665,40 @@ __global__ void __launch_bounds__(kBS) k_filter(SearchArgs a, int zero_fs) {
             const double hi = p.w;
             const int n2 = min(a.L.fsup_cnt[s], a.L.fsupK);
             const int* __restrict__ fsup = a.L.fsup + s;
+            unsigned long long* __restrict__ mask = a.L.fsm + 2 * (int64_t)s;
+            // COUNT tests every candidate and records the members among the
+            // first 128 in a bit mask; FILL writes those from the mask (the
+            // row's indices read in batches, all in flight together) and
+            // tests only candidates past the first 128 again
+            const int k0 = FILL ? min(n2, 128) : 0;
+            if (FILL) {
+                const unsigned long long m[2] = {mask[0], mask[1]};
+                constexpr int V = 8;
+                for (int k = 0; k < k0; k += V) {
+                    int ai[V];
+#pragma unroll
+                    for (int u = 0; u < V; u++)
+                        ai[u] = __builtin_nontemporal_load(fsup + (int64_t)min(k + u, k0 - 1) * a.nf);
+#pragma unroll
+                    for (int u = 0; u < V; u++) {
+                        const int kk = k + u;
+                        const bool in = kk < k0 && ((m[kk >> 6] >> (kk & 63)) & 1ull);
+                        if (in && w < a.L.fs_cap) {
+                            a.L.fs_i[w] = s;
+                            a.L.fs_a[w] = ai[u];
+                            if (zero_fs) a.L.fsF[w] = make_double4(0, 0, 0, 0);
+                        }
+                        w += in;
+                    }
+                }
+            }
+            unsigned long long m0 = 0, m1 = 0;
             const int last = n2 > 0 ? n2 - 1 : 0;
             // candidates in batches of 8: indices, then hashes and positions,
             // all in flight together (entries past the row end re-read its
             // last one and are not used)
             constexpr int U = 8;
-            for (int k = 0; k < n2; k += U) {
+            for (int k = k0; k < n2; k += U) {
                 int ai[U];
                 unsigned key[U];
                 double4 q[U];
@@ -649,8 +723,16 @@ __global__ void __launch_bounds__(kBS) k_filter(SearchArgs a, int zero_fs) {
                         w += in;
                     } else {
                         cs += in;
+                        const int kk = k + u;
+                        const unsigned long long bit = in ? 1ull << (kk & 63) : 0ull;
+                        if (kk < 64) m0 |= bit;
+                        else if (kk < 128) m1 |= bit;
                     }
                 }
+            }
+            if (!FILL) {
+                mask[0] = m0;
+                mask[1] = m1;
             }
         }
         if (!FILL) {
@@ -718,23 +800,29 @@ __global__ void k_rev_count(const int* __restrict__ key, const int* __restrict__
         atomicAdd(&cnt[a], 1);
     }
 }
+// cursor[a] starts at the segment's first slot
 __global__ void k_rev_fill(const int* __restrict__ key, const int* __restrict__ total_p, int64_t cap,
-                           const double4* __restrict__ kvel, const int* __restrict__ off,
-                           int* __restrict__ cursor, int* __restrict__ idx) {
+                           const double4* __restrict__ kvel, int* __restrict__ cursor,
+                           int* __restrict__ idx) {
     int64_t tot = *total_p;
     if (tot > cap) tot = cap;
     for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < tot;
          p += (int64_t)gridDim.x * blockDim.x) {
         int a = key[p];
         if (kvel && !(kvel[a].w > 0.0)) continue;
-        int slot = off[a] + atomicAdd(&cursor[a], 1);
-        idx[slot] = (int)p;
+        // every counted key has a cursor inside [0, cap); the test only
+        // keeps a key outside the scanned list from writing anywhere
+        const int slot = atomicAdd(&cursor[a], 1);
+        if (slot >= 0 && slot < cap) idx[slot] = (int)p;
     }
 }
-__global__ void k_rev_sort(const int* __restrict__ off, int n, int* __restrict__ idx) {
-    int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= n) return;
-    int b = off[a], e = off[a + 1];
+// segment of every listed key (all keys without a list) in ascending pair order
+__global__ void k_rev_sort(const int* __restrict__ keys, int n, const int* __restrict__ off,
+                           const int* __restrict__ cnt, int* __restrict__ idx) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int a = keys ? keys[t] : t;
+    const int b = off[a], e = b + cnt[a];
     for (int i = b + 1; i < e; i++) {  // insertion sort: segments are short
         int v = idx[i];
         int k = i - 1;
@@ -745,18 +833,33 @@ __global__ void k_rev_sort(const int* __restrict__ off, int n, int* __restrict__
         idx[k + 1] = v;
     }
 }
+// dense variant of the keyed scan's bookkeeping: counts move from tmp to
+// cnt (tmp left zero) and cursor = off
+__global__ void k_rev_settle(int n, int* __restrict__ tmp, int* __restrict__ cnt, const int* __restrict__ off,
+                             int* __restrict__ cursor) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        cnt[i] = tmp[i];
+        tmp[i] = 0;
+        cursor[i] = off[i];
+    }
+}
 
 // ------------------------------------------------------- DEM contacts
 // interactionSolid2Solid::save (deviceStructs.h:693-706): keep the previous
 // list's offsets, partners and springs for the history lookup
-__global__ void k_ss_save(const int* __restrict__ ssoff, int ns, const int* __restrict__ ss_j,
+__global__ void k_ss_save(const int* __restrict__ active, int nactive, const int* __restrict__ ssoff,
+                          const int* __restrict__ sscnt, int ns, const int* __restrict__ ss_j,
                           const double* __restrict__ ssS, int64_t cap, int* __restrict__ hoff,
-                          int* __restrict__ h_j, double* __restrict__ hS) {
+                          int* __restrict__ hcnt, int* __restrict__ h_j, double* __restrict__ hS) {
     int64_t tot = ssoff[ns];
     if (tot > cap) tot = cap;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i <= ns;
-         i += (int64_t)gridDim.x * blockDim.x)
-        hoff[i] = ssoff[i];
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nactive;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int a = active[t];
+        hoff[a] = ssoff[a];
+        hcnt[a] = sscnt[a];
+    }
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < tot;
          i += (int64_t)gridDim.x * blockDim.x) {
         h_j[i] = ss_j[i];
@@ -765,53 +868,116 @@ __global__ void k_ss_save(const int* __restrict__ ssoff, int ns, const int* __re
     }
 }
 
-// The active grid of up to kSmallSort solids in one workgroup: a stable
-// (key, position) bitonic sort in LDS, then the cell ranges.  The cells the
-// previous sort filled (its sorted keys, still in `sorted`) are reset first,
-// so the ncell-long range arrays stay -1 elsewhere without a fill.
-__global__ void __launch_bounds__(1024) k_active_small(const int* __restrict__ active, int n,
-                                                       const unsigned* __restrict__ skey,
-                                                       unsigned* __restrict__ sorted, int* __restrict__ perm,
-                                                       int* __restrict__ cs, int* __restrict__ ce) {
-    __shared__ unsigned long long v[kSmallSort];
-    const int T = blockDim.x;
-    for (int t = threadIdx.x; t < n; t += T) {
-        const unsigned k = sorted[t];
+
+// The active grid of up to kSmallSort solids by counting: with the unique
+// 64-bit keys K_i = (cell_i << 32) | i, the sorted position of solid i is
+// #{j : K_j < K_i}, and its cell's range is [#{j : K_j < cell_i << 32},
+// #{j : K_j < (cell_i + 1) << 32}).  Pass 1 counts these over tiles of j in
+// parallel (one workgroup per (i tile, j tile), counts added atomically
+// into zeroed accumulators) and resets the cells the previous sort filled;
+// pass 2 places every solid, writes its cell's range (all members write the
+// same values) and zeroes the accumulators again.
+constexpr int kRankI = 64, kRankJ = 256;
+__device__ __forceinline__ unsigned long long akey64(const int* active, const unsigned* skey, int t) {
+    return ((unsigned long long)skey[active[t]] << 32) | (unsigned)t;
+}
+__global__ void __launch_bounds__(kRankI) k_active_rank(const int* __restrict__ active, int n,
+                                                        const unsigned* __restrict__ skey,
+                                                        const unsigned* __restrict__ sorted_prev,
+                                                        int* __restrict__ cs, int* __restrict__ ce,
+                                                        int* __restrict__ acc) {
+    __shared__ unsigned long long kj[kRankJ];
+    const int ni = (n + kRankI - 1) / kRankI;
+    const int ti = blockIdx.x % ni, tj = blockIdx.x / ni;
+    const int i = ti * kRankI + threadIdx.x;
+    const int j0 = tj * kRankJ;
+    // reset the previous sort's cells (once, by the first column of tiles)
+    if (tj == 0 && i < n) {
+        const unsigned k = sorted_prev[i];
         cs[k] = -1;
         ce[k] = -1;
     }
-    int N2 = 1;
-    while (N2 < n) N2 <<= 1;
-    for (int t = threadIdx.x; t < N2; t += T)
-        v[t] = t < n ? ((unsigned long long)skey[active[t]] << 32) | (unsigned)t : ~0ull;
+    for (int t = threadIdx.x; t < kRankJ; t += kRankI)
+        kj[t] = j0 + t < n ? akey64(active, skey, j0 + t) : ~0ull;
     __syncthreads();
-    for (int k = 2; k <= N2; k <<= 1)
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = threadIdx.x; i < N2; i += T) {
-                const int l = i ^ j;
-                if (l > i) {
-                    const unsigned long long x = v[i], y = v[l];
-                    if (((i & k) == 0) == (x > y)) {
-                        v[i] = y;
-                        v[l] = x;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    for (int t = threadIdx.x; t < n; t += T) {
-        const unsigned k = (unsigned)(v[t] >> 32);
-        sorted[t] = k;
-        perm[t] = (int)(unsigned)v[t];
-        if (t == 0 || (unsigned)(v[t - 1] >> 32) != k) cs[k] = t;
-        if (t == n - 1 || (unsigned)(v[t + 1] >> 32) != k) ce[k] = t + 1;
+    if (i >= n) return;
+    const unsigned long long ki = akey64(active, skey, i);
+    const unsigned long long lo = ki & 0xffffffff00000000ull, hi = lo + (1ull << 32);
+    int r = 0, cl = 0, ch = 0;
+    const int m = min(kRankJ, n - j0);
+    for (int t = 0; t < m; t++) {
+        const unsigned long long x = kj[t];
+        r += x < ki;
+        cl += x < lo;
+        ch += x < hi;
     }
+    atomicAdd(&acc[3 * i], r);
+    atomicAdd(&acc[3 * i + 1], cl);
+    atomicAdd(&acc[3 * i + 2], ch);
+}
+__global__ void __launch_bounds__(kBS) k_active_place(const int* __restrict__ active, int n,
+                                                      const unsigned* __restrict__ skey,
+                                                      unsigned* __restrict__ sorted, int* __restrict__ perm,
+                                                      int* __restrict__ cs, int* __restrict__ ce,
+                                                      int* __restrict__ acc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned k = skey[active[i]];
+    const int r = acc[3 * i];
+    sorted[r] = k;
+    perm[r] = i;
+    cs[k] = acc[3 * i + 1];
+    ce[k] = acc[3 * i + 2];
+    acc[3 * i] = 0;
+    acc[3 * i + 1] = 0;
+    acc[3 * i + 2] = 0;
 }
 
 __global__ void k_active_keys(const int* __restrict__ active, int n, const unsigned* __restrict__ skey,
                               unsigned* __restrict__ akey) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) akey[k] = skey[active[k]];
+}
+
+// Pass 0 of the solid–solid search with 32 lanes per active solid: lane
+// c < 27 counts the contacts in stencil offset c (each visit of a repeated
+// cell counted, as the serial walk does), then the 32 lanes add up.
+__global__ void __launch_bounds__(kBS) k_ss_count(Grids G, Solids so, const int* __restrict__ acs,
+                                                  const int* __restrict__ ace, const int* __restrict__ aperm,
+                                                  int* __restrict__ sscnt) {
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k0 = gt >> 5, c = gt & 31;
+    int n = 0;
+    if (k0 < so.nactive && c < 27) {
+        const int a = so.active[k0];
+        const double4 p = so.pr[a];
+        const double ra = so.eff[a];
+        const int ca = so.clump[a];
+        const bool fixa = so.vel[a].w < 1.e-10;
+        const GridP& g = grid_at(G, solid_scene(G, a));
+        int ax, ay, az;
+        grid_pos(g, p.x, p.y, p.z, ax, ay, az);
+        const int zz = c / 9 - 1, yy = (c / 3) % 3 - 1, xx = c % 3 - 1;
+        const int hl = (az + zz) * g.gy * g.gx + (ay + yy) * g.gx + (ax + xx);
+        if (hl >= 0 && hl < g.ncell) {
+            const int h = g.cell0 + hl;
+            const int st0 = acs[h];
+            if (st0 >= 0) {
+                const int en = ace[h];
+                for (int t = st0; t < en; t++) {
+                    const int b = so.active[aperm[t]];
+                    if (b <= a) continue;
+                    if (ca >= 0 && ca == so.clump[b]) continue;
+                    if (fixa && so.vel[b].w < 1.e-10) continue;
+                    const double4 q = so.pr[b];
+                    n += within(p.x, p.y, p.z, q.x, q.y, q.z, so.eff[b] + ra) ? 1 : 0;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int d = 16; d > 0; d >>= 1) n += __shfl_xor(n, d, 64);
+    if (c == 0 && k0 < so.nactive) sscnt[so.active[k0]] = n;
 }
 
 // setSolid2SolidInteractionsKernel (neighborSearch.cu:222-312) over the
@@ -825,7 +991,8 @@ __global__ void __launch_bounds__(kBS) k_ss_search(
     const int* __restrict__ aperm, int* __restrict__ sscnt, const int* __restrict__ ssoff,
     int64_t cap, int* __restrict__ ss_i, int* __restrict__ ss_j, double4* __restrict__ ssF,
     double4* __restrict__ ssT, double* __restrict__ ssS, const int* __restrict__ hoff,
-    const int* __restrict__ h_j, const double* __restrict__ hS, unsigned long long* st) {
+    const int* __restrict__ hcnt, const int* __restrict__ h_j, const double* __restrict__ hS,
+    unsigned long long* st) {
     const int k0 = blockIdx.x * blockDim.x + threadIdx.x;
     if (FILL && k0 == 0) {
         long long tot = ssoff[ns];
@@ -847,7 +1014,7 @@ __global__ void __launch_bounds__(kBS) k_ss_search(
     int hb = 0, he = 0;
     if (FILL) {
         hb = hoff[a];
-        he = hoff[a + 1];
+        he = hb + hcnt[a];
     }
     for (int zz = -1; zz <= 1; zz++)
         for (int yy = -1; yy <= 1; yy++)
@@ -947,11 +1114,12 @@ int cap_grid(int64_t n, int cap) {
 // ---------------------------------------------------------------- wrappers
 void hash_fluid(hipStream_t s, const Grids& G, const double4* ph, const int* orig, int n, int sub_bits,
                 const unsigned* crank, const unsigned char* cspare, unsigned* key, unsigned* rank,
-                unsigned* hslot, unsigned long long* ctl, long long* status) {
+                unsigned* hslot, const double4* fxb, unsigned long long* bmax, unsigned long long* ctl,
+                long long* status) {
     hipLaunchKernelGGL(k_status_reset, dim3(1), dim3(ST_WORDS), 0, s, (unsigned long long*)status);
     if (n > 0)
         hipLaunchKernelGGL(k_hash_fluid, dim3(blocks_for(n)), dim3(kBS), 0, s, G, ph, orig, n, sub_bits,
-                           crank, cspare, key, rank, hslot, ctl, (unsigned long long*)status);
+                           crank, cspare, key, rank, hslot, fxb, bmax, ctl, (unsigned long long*)status);
 }
 
 void hash_solid(hipStream_t s, const Grids& G, const double4* pr, int n, unsigned* key,
@@ -961,18 +1129,16 @@ void hash_solid(hipStream_t s, const Grids& G, const double4* pr, int n, unsigne
                            (unsigned long long*)status);
 }
 
-void rebuild_decision(hipStream_t s, const Grids& G, const double4* fph, const double4* fxb, int nf,
-                      const double4* spr, const double4* sxb, const int* slive, int nslive, int always,
-                      int always_sort, unsigned long long* ctl, long long* status) {
-    if (nf > 0)
-        hipLaunchKernelGGL(k_disp, dim3(std::min(blocks_for(nf), 1024)), dim3(kBS), 0, s, G, true, fph,
-                           fxb, (const int*)nullptr, nf, ctl + CTL_DISP_F);
-    // static solids never move: only the live ones are checked
-    if (nslive > 0)
-        hipLaunchKernelGGL(k_disp, dim3(std::min(blocks_for(nslive), 1024)), dim3(kBS), 0, s, G, false,
-                           spr, sxb, slive, nslive, ctl + CTL_DISP_S);
-    hipLaunchKernelGGL(k_decide, dim3(1), dim3(1), 0, s, ctl, (unsigned long long*)status, always,
-                       always_sort);
+void hash_live_decide(hipStream_t s, const Grids& G, const double4* pr, const int* live, int nlive,
+                      unsigned* key, unsigned long long static_out, bool decide, const double4* sxb,
+                      const unsigned long long* fbmax, int nf, int always, int always_sort,
+                      unsigned long long* ctl, long long* status) {
+    if (nlive <= 0 && !decide && static_out == 0) return;
+    // one workgroup at least, so that a decision is taken without solids
+    const int nb = std::max(1, blocks_for(nlive));
+    hipLaunchKernelGGL(k_hash_live, dim3(nb), dim3(kBS), 0, s, G, pr, live, nlive, key, static_out,
+                       (int)decide, sxb, fbmax, nf > 0 ? blocks_for(nf) : 0, always, always_sort, ctl,
+                       (unsigned long long*)status);
 }
 
 void reorder_fluid(hipStream_t s, const FluidSet& f, const FluidSet& scratch, const int* perm,
@@ -1028,41 +1194,57 @@ void fluid_filter(hipStream_t s, const SearchArgs& a, int* scan_tmp, bool zero_f
     hipLaunchKernelGGL(k_filter<true>, dim3(blocks_for(a.nf)), dim3(kBS), 0, s, a, (int)zero_fs);
 }
 
+// Reverse CSR of pairs by their key (a solid): count into tmp (zero on
+// entry and on exit), scan, place with per-key cursors, then sort each
+// segment ascending.  Only the listed keys can have pairs: a short list is
+// scanned by one workgroup over the list, a long one over all n_all keys.
 static void reverse_build(hipStream_t s, const int* key, const int* total_p, int64_t cap,
-                          const double4* kvel, int nkeys, int* cnt, int* off, int* idx,
-                          int* scan_tmp) {
-    if (nkeys <= 0) return;
-    fill_int(s, cnt, nkeys, 0);
+                          const double4* kvel, const int* keys, int nkeys, int n_all, int* cnt, int* off,
+                          int* idx, int* tmp, int* cursor, int* scan_tmp) {
+    if (n_all <= 0) return;
     const int gb = cap_grid(cap, 2048);
-    if (cap > 0) hipLaunchKernelGGL(k_rev_count, dim3(gb), dim3(kBS), 0, s, key, total_p, cap, kvel, cnt);
-    exclusive_scan(s, cnt, off, nkeys, scan_tmp);
-    fill_int(s, cnt, nkeys, 0);
+    if (cap > 0) hipLaunchKernelGGL(k_rev_count, dim3(gb), dim3(kBS), 0, s, key, total_p, cap, kvel, tmp);
+    const bool sparse = nkeys <= kKeyScanMax;
+    if (sparse) {
+        scan_keys(s, keys, nkeys, tmp, cnt, off, cursor, n_all);
+    } else {
+        exclusive_scan(s, tmp, off, n_all, scan_tmp);
+        hipLaunchKernelGGL(k_rev_settle, dim3(cap_grid(n_all, 2048)), dim3(kBS), 0, s, n_all, tmp, cnt,
+                           (const int*)off, cursor);
+    }
     if (cap > 0)
-        hipLaunchKernelGGL(k_rev_fill, dim3(gb), dim3(kBS), 0, s, key, total_p, cap, kvel,
-                           (const int*)off, cnt, idx);
-    hipLaunchKernelGGL(k_rev_sort, dim3(blocks_for(nkeys)), dim3(kBS), 0, s, (const int*)off, nkeys, idx);
+        hipLaunchKernelGGL(k_rev_fill, dim3(gb), dim3(kBS), 0, s, key, total_p, cap, kvel, cursor, idx);
+    const int ns = sparse ? nkeys : n_all;
+    if (ns > 0)
+        hipLaunchKernelGGL(k_rev_sort, dim3(blocks_for(ns)), dim3(kBS), 0, s, sparse ? keys : nullptr, ns,
+                           (const int*)off, (const int*)cnt, idx);
 }
 
 void reverse_movable(hipStream_t s, const int* pair_key, const int* pair_total, int64_t cap,
-                     const double4* key_vel, int nkeys, int* cnt, int* off, int* idx,
-                     int* scan_tmp) {
-    reverse_build(s, pair_key, pair_total, cap, key_vel, nkeys, cnt, off, idx, scan_tmp);
+                     const double4* key_vel, const Solids& so, int ns, Lists& L, int* scan_tmp) {
+    // movable solids (1/m > 0) are live
+    reverse_build(s, pair_key, pair_total, cap, key_vel, so.live, so.nlive, ns, L.fsr_cnt, L.fsr_off,
+                  L.fsr_idx, L.rev_tmp, L.rev_cur, scan_tmp);
 }
 
-void solid_save_history(hipStream_t s, Lists& L, int ns) {
+void solid_save_history(hipStream_t s, const Solids& so, Lists& L, int ns) {
     if (ns <= 0) return;
-    hipLaunchKernelGGL(k_ss_save, dim3(cap_grid(L.ss_cap > ns + 1 ? L.ss_cap : ns + 1, 2048)), dim3(kBS),
-                       0, s, (const int*)L.ssoff, ns, (const int*)L.ss_j, (const double*)L.ssS, L.ss_cap,
-                       L.hoff, L.h_j, L.hS);
+    const int64_t n = L.ss_cap > so.nactive ? L.ss_cap : so.nactive;
+    hipLaunchKernelGGL(k_ss_save, dim3(cap_grid(n > 0 ? n : 1, 2048)), dim3(kBS), 0, s, (const int*)so.active,
+                       so.nactive, (const int*)L.ssoff, (const int*)L.sscnt, ns, (const int*)L.ss_j,
+                       (const double*)L.ssS, L.ss_cap, L.hoff, L.hcnt, L.h_j, L.hS);
 }
 
 void active_grid(hipStream_t s, const Solids& so, const unsigned* skey, int ncell, int key_bits,
                  unsigned* akey, unsigned* akey_sorted, int* aperm, int* acs, int* ace,
-                 unsigned* radix_tmp) {
+                 unsigned* radix_tmp, int* rank_acc) {
     const int n = so.nactive;
     if (n > 0 && n <= kSmallSort) {
-        hipLaunchKernelGGL(k_active_small, dim3(1), dim3(1024), 0, s, (const int*)so.active, n, skey,
-                           akey_sorted, aperm, acs, ace);
+        const int ni = (n + kRankI - 1) / kRankI, nj = (n + kRankJ - 1) / kRankJ;
+        hipLaunchKernelGGL(k_active_rank, dim3(ni * nj), dim3(kRankI), 0, s, (const int*)so.active, n, skey,
+                           (const unsigned*)akey_sorted, acs, ace, rank_acc);
+        hipLaunchKernelGGL(k_active_place, dim3(blocks_for(n)), dim3(kBS), 0, s, (const int*)so.active, n,
+                           skey, akey_sorted, aperm, acs, ace, rank_acc);
         return;
     }
     if (n > 0) {
@@ -1077,20 +1259,25 @@ void solid_search(hipStream_t s, const Grids& g, const Solids& so, int ns, const
                   const int* ace, const int* aperm, Lists& L, int* scan_tmp, long long* status) {
     if (ns <= 0) return;
     const int nb = blocks_for(so.nactive > 0 ? so.nactive : 1);
-    hipLaunchKernelGGL(k_ss_search<false>, dim3(nb), dim3(kBS), 0, s, g, so, ns, acs, ace, aperm, L.sscnt,
-                       (const int*)L.ssoff, L.ss_cap, L.ss_i, L.ss_j, L.ssF, L.ssT, L.ssS,
-                       (const int*)L.hoff, (const int*)L.h_j, (const double*)L.hS,
-                       (unsigned long long*)status);
-    exclusive_scan(s, L.sscnt, L.ssoff, ns, scan_tmp);
+    if (so.nactive > 0)
+        hipLaunchKernelGGL(k_ss_count, dim3(blocks_for(32 * (int64_t)so.nactive)), dim3(kBS), 0, s, g, so, acs,
+                           ace, aperm, L.sscnt);
+    // only active solids have contacts: offsets are kept for them (and the
+    // total at ns)
+    if (so.nactive <= kKeyScanMax)
+        scan_keys(s, so.active, so.nactive, L.sscnt, nullptr, L.ssoff, nullptr, ns);
+    else
+        exclusive_scan(s, L.sscnt, L.ssoff, ns, scan_tmp);
     hipLaunchKernelGGL(k_ss_search<true>, dim3(nb), dim3(kBS), 0, s, g, so, ns, acs, ace, aperm, L.sscnt,
                        (const int*)L.ssoff, L.ss_cap, L.ss_i, L.ss_j, L.ssF, L.ssT, L.ssS,
-                       (const int*)L.hoff, (const int*)L.h_j, (const double*)L.hS,
+                       (const int*)L.hoff, (const int*)L.hcnt, (const int*)L.h_j, (const double*)L.hS,
                        (unsigned long long*)status);
 }
 
-void reverse_solid(hipStream_t s, Lists& L, int ns, int* scan_tmp) {
-    reverse_build(s, L.ss_j, L.ssoff + ns, L.ss_cap, nullptr, ns, L.ssr_cnt, L.ssr_off, L.ssr_idx,
-                  scan_tmp);
+void reverse_solid(hipStream_t s, const Solids& so, Lists& L, int ns, int* scan_tmp) {
+    // partners are active solids
+    reverse_build(s, L.ss_j, L.ssoff + ns, L.ss_cap, nullptr, so.active, so.nactive, ns, L.ssr_cnt, L.ssr_off,
+                  L.ssr_idx, L.rev_tmp, L.rev_cur, scan_tmp);
 }
 
 void solid_normals(hipStream_t s, const Grids& G, const Solids& so, int ns, const int* scs,

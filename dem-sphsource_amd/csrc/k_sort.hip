@@ -37,7 +37,8 @@ __global__ void k_copy_int(int* __restrict__ d, const int* __restrict__ s, int64
         d[i] = s[i];
 }
 
-// inclusive scan of one value per thread across a 256-thread block
+// inclusive scan of one value per thread across a workgroup (lds_wave: one
+// int per wavefront)
 __device__ __forceinline__ int block_inclusive_scan(int v, int* lds_wave) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
@@ -163,6 +164,46 @@ __global__ void k_spans(const unsigned* __restrict__ sorted, int64_t n, int* __r
     if (i == n - 1 || (sorted[i + 1] >> shift) != k) end[cell] = (int)(i + 1);
 }
 
+// Exclusive scan over the listed indices only, in one workgroup of
+// kKeyBS threads walking the list in tiles: keys ascending, every unlisted
+// index has count 0.  off[keys[r]] = sum of counts of keys[0..r),
+// off[n_all] = the total.  With cnt_out the counts are moved: read from cnt,
+// written to cnt_out and cleared in cnt (so cnt is all zero again for the
+// next count); with cursor, cursor[key] = off[key].
+constexpr int kKeyBS = 1024;
+__global__ void __launch_bounds__(kKeyBS) k_scan_keys(const int* __restrict__ keys, int nkeys,
+                                                      int* __restrict__ cnt, int* __restrict__ cnt_out,
+                                                      int* __restrict__ off, int* __restrict__ cursor,
+                                                      int n_all) {
+    __shared__ int lds[kKeyBS / 64];
+    __shared__ int total;
+    int carry = 0;
+    for (int base = 0; base < nkeys; base += kKeyBS) {
+        const int r = base + (int)threadIdx.x;
+        int key = -1, v = 0;
+        if (r < nkeys) {
+            key = keys[r];
+            v = cnt[key];
+        }
+        const int inc = block_inclusive_scan(v, lds);
+        const int excl = carry + inc - v;
+        if (r < nkeys) {
+            off[key] = excl;
+            if (cnt_out) {
+                cnt_out[key] = v;
+                cnt[key] = 0;
+            }
+            if (cursor) cursor[key] = excl;
+        }
+        // every thread learns the tile total from the last thread's sum
+        if (threadIdx.x == kKeyBS - 1) total = inc;
+        __syncthreads();
+        carry += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) off[n_all] = carry;
+}
+
 }  // namespace
 
 void fill_int(hipStream_t s, int* p, int64_t n, int v, Gate gate) {
@@ -187,6 +228,12 @@ int64_t scan_tmp_size(int64_t n) {
         n = nb;
     }
     return tot + 2;
+}
+
+void scan_keys(hipStream_t s, const int* keys, int nkeys, int* cnt, int* cnt_out, int* off, int* cursor,
+               int n_all) {
+    hipLaunchKernelGGL(k_scan_keys, dim3(1), dim3(kKeyBS), 0, s, keys, nkeys, cnt, cnt_out, off, cursor,
+                       n_all);
 }
 
 void exclusive_scan(hipStream_t s, const int* in, int* out, int64_t n, int* tmp, Gate gate) {
