@@ -548,6 +548,10 @@ void search_checked(dsph_ctx* c, int i_step, int gap) {
 void solid_before(dsph_ctx* c, double dt) {
     FamTimer t(c, F_DEM);
     DemArgs a = dem_args(c);
+    if (c->nc == 0) {
+        dem_solid_kick_drift(c->st, a, 0.5 * dt, true);
+        return;
+    }
     dem_solid_kick(c->st, a, 0.5 * dt);
     dem_clump_kick(c->st, a, 0.5 * dt);
     dem_clump_drift(c->st, a, 0.5 * dt);
@@ -557,6 +561,10 @@ void solid_before(dsph_ctx* c, double dt) {
 void solid_after(dsph_ctx* c, double dt) {
     FamTimer t(c, F_DEM);
     DemArgs a = dem_args(c);
+    if (c->nc == 0) {
+        dem_solid_kick_drift(c->st, a, 0.5 * dt, false);
+        return;
+    }
     dem_solid_drift(c->st, a, 0.5 * dt);
     dem_clump_drift(c->st, a, 0.5 * dt);
     dem_clump_kick(c->st, a, 0.5 * dt);
@@ -568,8 +576,7 @@ void contacts(dsph_ctx* c, double dt) {
     DemArgs a = dem_args(c);
     dem_contacts(c->st, a, dt);
     dem_bonds(c->st, a, dt);
-    dem_solid_sum(c->st, a);
-    if (c->nw > 0) dem_walls(c->st, a, dt);
+    dem_solid_sum(c->st, a, dt);
     dem_clump_sum(c->st, a);
 }
 

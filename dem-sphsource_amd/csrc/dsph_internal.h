@@ -404,8 +404,10 @@ struct DemArgs {
 };
 void dem_contacts(hipStream_t s, const DemArgs& a, double dt);
 void dem_bonds(hipStream_t s, const DemArgs& a, double dt);
-void dem_solid_sum(hipStream_t s, const DemArgs& a);
-void dem_walls(hipStream_t s, const DemArgs& a, double dt);
+// contact sums of the live solids, then their infinite walls
+void dem_solid_sum(hipStream_t s, const DemArgs& a, double dt);
+// kick and drift of the live solids in one pass (scenes without clumps)
+void dem_solid_kick_drift(hipStream_t s, const DemArgs& a, double dt, bool kick_first);
 void dem_clump_sum(hipStream_t s, const DemArgs& a);
 void dem_solid_kick(hipStream_t s, const DemArgs& a, double dt);
 void dem_solid_drift(hipStream_t s, const DemArgs& a, double dt);

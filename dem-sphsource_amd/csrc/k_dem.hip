@@ -117,7 +117,40 @@ __global__ void k_bonds(DemArgs a, double dt) {
 
 // the per-solid DEM kernels run over the live solids only: wall dummies
 // never move, take no contacts and keep zero acceleration and torque
-__global__ void k_solid_sum(DemArgs a) {
+// infinite walls on live solid i (after its contact sums): adds the wall
+// forces and torques to acc and tq
+__device__ void wall_forces(const DemArgs& a, int i, double dt, d3& acc, d3& tq) {
+    const double4 v4 = a.so.vel[i];
+    if (a.so.dummy[i] == 1 || v4.w < 1.e-10) return;
+    const bool cont = (a.flags & DSPH_FLAG_WALL_CONTINUE) != 0;
+    const double4 pr = a.so.pr[i];
+    const d3 pi = v3(pr), vi = v3(v4), wi = v3(a.so.w[i]);
+    // the walls of the solid's own scene, in their upload order
+    const int sc = a.nscene == 1 ? 0 : a.sscene[i];
+    const int k0 = a.nscene == 1 ? 0 : a.woff[sc], k1 = a.nscene == 1 ? a.nw : a.woff[sc + 1];
+    for (int k = k0; k < k1; k++) {
+        double* h = a.whist + 9 * ((int64_t)i * a.nwmax + (k - k0));
+        const d3 n = v3(a.wn[k]);
+        const double delta = pr.w - dot(pi - v3(a.wpt[k]), n);
+        if (delta < 0.) {
+            for (int m = 0; m < 9; m++) h[m] = 0.0;
+            if (cont) continue;
+            break;
+        }
+        const double m = 1. / v4.w;
+        const d3 rc = pi - (pr.w - delta) * n;
+        const d3 vc = vi + cross(wi, rc - pi);
+        d3 F{0, 0, 0}, T{0, 0, 0};
+        Springs sp = ld_springs(h);
+        const int c = combined_index(a.tb.nmat, a.so.mat[i], a.wmat[k]);
+        contact_model(a.tb, c, F, T, sp, vc, wi, n, delta, m, pr.w, dt);
+        st_springs(h, sp);
+        acc += F * v4.w;
+        tq += T + cross(rc - pi, F);
+    }
+}
+
+__global__ void k_solid_sum(DemArgs a, double dt) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.so.nlive) return;
     const int i = a.so.live[t];
@@ -154,45 +187,10 @@ __global__ void k_solid_sum(DemArgs a) {
     if (v.w > 0.0 && a.L.fsoff && a.L.fsr_off)
         for (int k = a.L.fsr_off[i], e = k + a.L.fsr_cnt[i]; k < e; k++) F -= v3(a.L.fsF[a.L.fsr_idx[k]]);
     const double gon = (v.w > 0.0 && a.so.clump[i] < 0) ? 1.0 : 0.0;
-    a.so.acc[i] = d4(F * v.w + d3{a.gx, a.gy, a.gz} * gon);
-    a.so.tq[i] = d4(T);
-}
-
-__global__ void k_walls(DemArgs a, double dt) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.so.nlive) return;
-    const int i = a.so.live[t];
-    const double4 v4 = a.so.vel[i];
-    if (a.so.dummy[i] == 1 || v4.w < 1.e-10) return;
-    const bool cont = (a.flags & DSPH_FLAG_WALL_CONTINUE) != 0;
-    const double4 pr = a.so.pr[i];
-    const d3 pi = v3(pr), vi = v3(v4), wi = v3(a.so.w[i]);
-    d3 acc = v3(a.so.acc[i]), tq = v3(a.so.tq[i]);
-    // the walls of the solid's own scene, in their upload order
-    const int sc = a.nscene == 1 ? 0 : a.sscene[i];
-    const int k0 = a.nscene == 1 ? 0 : a.woff[sc], k1 = a.nscene == 1 ? a.nw : a.woff[sc + 1];
-    for (int k = k0; k < k1; k++) {
-        double* h = a.whist + 9 * ((int64_t)i * a.nwmax + (k - k0));
-        const d3 n = v3(a.wn[k]);
-        const double delta = pr.w - dot(pi - v3(a.wpt[k]), n);
-        if (delta < 0.) {
-            for (int m = 0; m < 9; m++) h[m] = 0.0;
-            if (cont) continue;
-            break;
-        }
-        const double m = 1. / v4.w;
-        const d3 rc = pi - (pr.w - delta) * n;
-        const d3 vc = vi + cross(wi, rc - pi);
-        d3 F{0, 0, 0}, T{0, 0, 0};
-        Springs sp = ld_springs(h);
-        const int c = combined_index(a.tb.nmat, a.so.mat[i], a.wmat[k]);
-        contact_model(a.tb, c, F, T, sp, vc, wi, n, delta, m, pr.w, dt);
-        st_springs(h, sp);
-        acc += F * v4.w;
-        tq += T + cross(rc - pi, F);
-    }
+    d3 acc = F * v.w + d3{a.gx, a.gy, a.gz} * gon;
+    if (a.nw > 0) wall_forces(a, i, dt, acc, T);
     a.so.acc[i] = d4(acc);
-    a.so.tq[i] = d4(tq);
+    a.so.tq[i] = d4(T);
 }
 
 __global__ void k_clump_sum(DemArgs a) {
@@ -214,10 +212,7 @@ __global__ void k_clump_sum(DemArgs a) {
     a.c.tq[c] = d4(T);
 }
 
-__global__ void k_solid_kick(DemArgs a, double dt) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.so.nlive) return;
-    const int i = a.so.live[t];
+__device__ __forceinline__ void solid_kick_one(const DemArgs& a, int i, double dt) {
     if (a.so.clump[i] >= 0) return;
     double4 v = a.so.vel[i];
     const double4 ac = a.so.acc[i];
@@ -235,17 +230,34 @@ __global__ void k_solid_kick(DemArgs a, double dt) {
     w.z += tq.z / I * dt;
     a.so.w[i] = w;
 }
-
-__global__ void k_solid_drift(DemArgs a, double dt) {
+__global__ void k_solid_kick(DemArgs a, double dt) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.so.nlive) return;
-    const int i = a.so.live[t];
+    solid_kick_one(a, a.so.live[t], dt);
+}
+
+__device__ __forceinline__ void solid_drift_one(const DemArgs& a, int i, double dt) {
     double4 p = a.so.pr[i];
     const double4 v = a.so.vel[i];
     p.x += dt * v.x;
     p.y += dt * v.y;
     p.z += dt * v.z;
     a.so.pr[i] = p;
+}
+__global__ void k_solid_drift(DemArgs a, double dt) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.so.nlive) return;
+    solid_drift_one(a, a.so.live[t], dt);
+}
+// without clumps a phase's kick and drift touch only the solid's own state:
+// one kernel, kick first (before contact) or drift first (after contact)
+__global__ void k_solid_kick_drift(DemArgs a, double dt, int kick_first) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.so.nlive) return;
+    const int i = a.so.live[t];
+    if (!kick_first) solid_drift_one(a, i, dt);
+    solid_kick_one(a, i, dt);
+    if (kick_first) solid_drift_one(a, i, dt);
 }
 
 __global__ void k_clump_kick(DemArgs a, double dt) {
@@ -329,11 +341,13 @@ void dem_contacts(hipStream_t s, const DemArgs& a, double dt) {
 void dem_bonds(hipStream_t s, const DemArgs& a, double dt) {
     if (a.b.n > 0) hipLaunchKernelGGL(k_bonds, dim3(blocks_for(a.b.n)), dim3(kBS), 0, s, a, dt);
 }
-void dem_solid_sum(hipStream_t s, const DemArgs& a) {
-    if (a.so.nlive > 0) hipLaunchKernelGGL(k_solid_sum, dim3(blocks_for(a.so.nlive)), dim3(kBS), 0, s, a);
+void dem_solid_sum(hipStream_t s, const DemArgs& a, double dt) {
+    if (a.so.nlive > 0) hipLaunchKernelGGL(k_solid_sum, dim3(blocks_for(a.so.nlive)), dim3(kBS), 0, s, a, dt);
 }
-void dem_walls(hipStream_t s, const DemArgs& a, double dt) {
-    if (a.so.nlive > 0 && a.nw > 0) hipLaunchKernelGGL(k_walls, dim3(blocks_for(a.so.nlive)), dim3(kBS), 0, s, a, dt);
+void dem_solid_kick_drift(hipStream_t s, const DemArgs& a, double dt, bool kick_first) {
+    if (a.so.nlive > 0)
+        hipLaunchKernelGGL(k_solid_kick_drift, dim3(blocks_for(a.so.nlive)), dim3(kBS), 0, s, a, dt,
+                           (int)kick_first);
 }
 void dem_clump_sum(hipStream_t s, const DemArgs& a) {
     if (a.nc > 0) hipLaunchKernelGGL(k_clump_sum, dim3(blocks_for(a.nc)), dim3(kBS), 0, s, a);
