@@ -96,6 +96,8 @@ struct dsph_ctx {
     bool skey_valid = false;
     unsigned long long* dispmax = nullptr;  // per-workgroup fluid displacement maxima
     int* rank_acc = nullptr;                // active-grid rank accumulators (zero between sorts)
+    unsigned long long* lb_ctl = nullptr;   // fluid–solid list scan: epoch << 32 | ticket
+    unsigned long long* lb_tiles = nullptr; // and its tile words
     unsigned long long static_out = 0;
     int* dcs = nullptr;
     int* dce = nullptr;
@@ -419,6 +421,8 @@ SearchArgs search_args(dsph_ctx* c) {
     a.skey = c->skey;
     a.spr = c->so.pr;
     a.fsp = c->fsp;
+    a.lb_ctl = c->lb_ctl;
+    a.lb_tiles = c->lb_tiles;
     a.L = c->L;
     a.ctl = c->ctl;
     a.st = c->status;
@@ -485,7 +489,7 @@ void neighbor_search(dsph_ctx* c, int i_step, int gap, bool save_history, bool z
             fluid_superset(s, sa, c->fchunks, c->nfchunks, c->fxb, c->sxb);
         }
         FamTimer t(c, F_FSLIST);
-        fluid_filter(s, sa, c->scan_tmp, zero_fs);
+        fluid_filter(s, sa, zero_fs);
         if (c->ns > 0)
             reverse_movable(s, c->L.fs_a, c->L.fsoff + c->nf, c->L.fs_cap, c->so.vel, c->so, c->ns, c->L,
                             c->scan_tmp);
@@ -516,6 +520,7 @@ void search_checked(dsph_ctx* c, int i_step, int gap) {
     for (int attempt = 0; attempt < 8; attempt++) {
         unsigned long long w[ST_WORDS];
         read_status(c, w);
+        if (w[ST_LOOKBACK]) throw ApiError{DSPH_E_HIP, "internal error: a list scan was abandoned"};
         const bool ovf = w[ST_FF_OVERFLOW] | w[ST_FS_OVERFLOW] | w[ST_SS_OVERFLOW] | w[ST_SUP_OVERFLOW];
         if (!ovf) {
             // superset rows change only at builds, which graph-replayed steps
@@ -1406,6 +1411,16 @@ int dsph_upload_batch(dsph_ctx* c, const dsph_scene* scenes, int32_t n_scenes) {
         c->fsp = dalloc<double4>(c, nf);
         h2d(c, c->fsp, ph.data(), nf);
         c->dispmax = dalloc<unsigned long long>(c, std::max(1, blocks_for(std::max(nf, 1))));
+        {
+            // tickets start at 0, the epoch at 1 (tile words start at epoch 0)
+            const unsigned long long lb0 = 1ull << 32;
+            c->lb_ctl = dalloc<unsigned long long>(c, 1);
+            h2d(c, c->lb_ctl, &lb0, 1);
+            const size_t nt = (size_t)std::max(1, blocks_for(std::max(nf, 1)));
+            c->lb_tiles = dalloc<unsigned long long>(c, nt);
+            HIPCHK(hipMemsetAsync(c->lb_tiles, 0, nt * sizeof(unsigned long long), c->st));
+            HIPCHK(hipStreamSynchronize(c->st));
+        }
         c->ctl = dalloc<unsigned long long>(c, CTL_WORDS);
         HIPCHK(hipMemsetAsync(c->ctl, 0, CTL_WORDS * sizeof(unsigned long long), c->st));
         c->scs = dalloc<int>(c, c->ncell);
@@ -1629,7 +1644,6 @@ int dsph_upload_batch(dsph_ctx* c, const dsph_scene* scenes, int32_t n_scenes) {
         alloc_sup(c, nf > 0 ? 160 : 1, nf > 0 ? 96 : 1);
         L.sup_cnt = dalloc<int>(c, nf);
         L.fsup_cnt = dalloc<int>(c, nf);
-        L.fsm = dalloc<unsigned long long>(c, 2 * (size_t)std::max(nf, 1));
         fill_int(c->st, L.sup_cnt, nf, 0);
         fill_int(c->st, L.fsup_cnt, nf, 0);
         L.grow = dalloc<int>(c, nf);
@@ -1896,6 +1910,7 @@ int dsph_step_n(dsph_ctx* c, int32_t i0, int32_t n, double dt, int32_t gap) {
         }
         unsigned long long w[ST_WORDS];
         read_status(c, w);
+        if (w[ST_LOOKBACK]) throw ApiError{DSPH_E_HIP, "internal error: a list scan was abandoned"};
         if (w[ST_FF_OVERFLOW] | w[ST_FS_OVERFLOW] | w[ST_SS_OVERFLOW] | w[ST_SUP_OVERFLOW]) {
             c->err = "neighbor list overflow during dsph_step_n; enlarge capacities (dsph_set_capacity) and rerun";
             return DSPH_E_CAPACITY;

@@ -39,6 +39,7 @@ enum {
     ST_SUP_OVERFLOW = 7,
     ST_REBUILDS = 8,
     ST_SUP_MAX = 9,        // longest (group) superset row of this search's build
+    ST_LOOKBACK = 10,      // a single-pass scan gave up waiting (sticky; an internal error)
     ST_WORDS = 12
 };
 
@@ -197,9 +198,6 @@ struct Lists {
     int fsupK = 0;
     int* fsup = nullptr;
     int* fsup_cnt = nullptr;
-    // members of the fluid–solid list among the first 128 candidates of
-    // each fluid slot's solid superset row (two words per slot)
-    unsigned long long* fsm = nullptr;
     // solid–solid CSR by solid index (the reference's own list)
     int64_t ss_cap = 0;
     int* sscnt = nullptr;
@@ -297,6 +295,10 @@ struct SearchArgs {
     Lists L;
     unsigned long long* ctl;
     unsigned long long* st;
+    // single-pass tile scan of the fluid–solid list: (epoch << 32 | next
+    // ticket) and one word per tile of 256 slots
+    unsigned long long* lb_ctl;
+    unsigned long long* lb_tiles;
 };
 // also the reference hash of every slot, and CTL_NEAR_SPARE when a particle's
 // cell has a stencil reaching the spare cell; with fxb, the largest squared
@@ -335,7 +337,7 @@ void fluid_superset(hipStream_t s, const SearchArgs& a, const int2* chunks, int 
 // exact fluid neighbour rows of all particles for read-back (ELL, stride nf,
 // capacity max(K, supK)); out_cnt = -1 marks a walked row that overflowed
 void fluid_exact_rows(hipStream_t s, const SearchArgs& a, int* out, int* out_cnt);
-void fluid_filter(hipStream_t s, const SearchArgs& a, int* scan_tmp, bool zero_fs);
+void fluid_filter(hipStream_t s, const SearchArgs& a, bool zero_fs);
 // fluid–solid pairs by movable solid (fsr_*)
 void reverse_movable(hipStream_t s, const int* pair_key, const int* pair_off_total, int64_t cap,
                      const double4* key_vel, const Solids& so, int ns, Lists& L, int* scan_tmp);

@@ -76,6 +76,23 @@ __device__ __forceinline__ bool within(double px, double py, double pz, double q
     return R - sqrt(d2) >= 0.0;
 }
 
+// inclusive scan of one int per thread across the workgroup (lds: one int
+// per wavefront)
+__device__ __forceinline__ int block_scan_int(int v, int* lds) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int u = __shfl_up(v, d, 64);
+        if (lane >= d) v += u;
+    }
+    if (lane == 63) lds[w] = v;
+    __syncthreads();
+    int add = 0;
+    for (int k = 0; k < w; k++) add += lds[k];
+    __syncthreads();
+    return v + add;
+}
+
 __device__ __forceinline__ void wave_max_atomic(unsigned long long* addr, unsigned long long v) {
 #pragma unroll
     for (int d = 32; d > 0; d >>= 1) {
@@ -560,27 +577,33 @@ __device__ void ff_walk(const SearchArgs& a, const double4 p, int s, int& c) {
     const double hi = p.w;
     int ax, ay, az;
     grid_pos(g, p.x, p.y, p.z, ax, ay, az);
-    int cand[55];
-    int nc = 0;
-    auto push = [&](int h, bool dedup) {
-        if (h < 0 || h >= g.ncell) return;
-        if (dedup)
-            for (int k = 0; k < nc; k++)
-                if (cand[k] == h) return;
-        cand[nc++] = h;
+    // candidate k of the walk (in order): the 27 stencil cells, the spare
+    // cell, and for a particle in the spare cell the 27 cells whose stencils
+    // reach it; -1 outside the grid.  Generated on the fly (no per-lane
+    // array): a candidate equal to an earlier one is skipped where the
+    // reference's walk would not repeat it (stencil cells only on grids
+    // whose offsets collide, the others always).
+    auto gen = [&](int k) -> int {
+        int h;
+        if (k < 27) {
+            h = (az + k / 9 - 1) * g.gy * g.gx + (ay + (k / 3) % 3 - 1) * g.gx + (ax + k % 3 - 1);
+        } else if (k == 27) {
+            h = spare;
+        } else {
+            if (hs != spare) return -1;
+            const int q = k - 28;
+            h = spare - ((q / 9 - 1) * g.gy * g.gx + ((q / 3) % 3 - 1) * g.gx + (q % 3 - 1));
+        }
+        return (h < 0 || h >= g.ncell) ? -1 : h;
     };
-    for (int zz = -1; zz <= 1; zz++)
-        for (int yy = -1; yy <= 1; yy++)
-            for (int xx = -1; xx <= 1; xx++)
-                push((az + zz) * g.gy * g.gx + (ay + yy) * g.gx + (ax + xx), g.collide != 0);
-    push(spare, true);
-    if (hs == spare)
-        for (int zz = -1; zz <= 1; zz++)
-            for (int yy = -1; yy <= 1; yy++)
-                for (int xx = -1; xx <= 1; xx++)
-                    push(spare - (zz * g.gy * g.gx + yy * g.gx + xx), true);
-    for (int k = 0; k < nc; k++) {
-        const int h = cand[k];
+    for (int k = 0; k < 55; k++) {
+        const int h = gen(k);
+        if (h < 0) continue;
+        if (k >= 27 || g.collide != 0) {
+            bool seen = false;
+            for (int k2 = 0; k2 < k && !seen; k2++) seen = gen(k2) == h;
+            if (seen) continue;
+        }
         const int st0 = a.fcs[c0 + h];
         if (st0 < 0) continue;
         const int mh = stencil_count(g, ax, ay, az, h);  // offsets of i's stencil reaching h
@@ -643,68 +666,115 @@ __device__ void lane_walk(const SearchArgs& a, const double4 p, int s, int& c, i
             }
 }
 
-// The per-step list pass.  Rows that must follow the reference stencil this
-// step (walk) get their exact fluid neighbours in the ELL list, cnt = count;
-// every other row keeps its superset for the pair loops (cnt = -1), which
-// skip candidates outside the kernel support themselves.  Fluid–solid pairs
-// are filtered exactly for every row (COUNT, then FILL into the CSR).
-template <bool FILL>
-__global__ void __launch_bounds__(kBS) k_filter(SearchArgs a, int zero_fs) {
-    const int s = xcd_block() * blockDim.x + threadIdx.x;
-    int c = 0, cs = 0;
-    if (s < a.nf) {
-        const double4 p = a.ph[s];
-        const GridP& g = grid_at(a.G, fluid_scene(a.G, s));
-        const unsigned spare = (unsigned)(g.cell0 + g.ncell - 1);  // the scene's spare cell
-        const bool walk = a.ctl[CTL_WALK_ALL] != 0 || g.walk_only || a.hslot[s] == spare ||
-                          a.cspare[a.hslot[s]] != 0;
-        int64_t w = FILL ? (int64_t)a.L.fsoff[s] : 0;
-        if (walk) {
-            lane_walk<FILL>(a, p, s, c, cs, w);
-        } else {
-            const double hi = p.w;
-            const int n2 = min(a.L.fsup_cnt[s], a.L.fsupK);
-            const int* __restrict__ fsup = a.L.fsup + s;
-            unsigned long long* __restrict__ mask = a.L.fsm + 2 * (int64_t)s;
-            // COUNT tests every candidate and records the members among the
-            // first 128 in a bit mask; FILL writes those from the mask (the
-            // row's indices read in batches, all in flight together) and
-            // tests only candidates past the first 128 again
-            const int k0 = FILL ? min(n2, 128) : 0;
-            if (FILL) {
-                const unsigned long long m[2] = {mask[0], mask[1]};
-                constexpr int V = 8;
-                for (int k = 0; k < k0; k += V) {
-                    int ai[V];
-#pragma unroll
-                    for (int u = 0; u < V; u++)
-                        ai[u] = __builtin_nontemporal_load(fsup + (int64_t)min(k + u, k0 - 1) * a.nf);
-#pragma unroll
-                    for (int u = 0; u < V; u++) {
-                        const int kk = k + u;
-                        const bool in = kk < k0 && ((m[kk >> 6] >> (kk & 63)) & 1ull);
-                        if (in && w < a.L.fs_cap) {
-                            a.L.fs_i[w] = s;
-                            a.L.fs_a[w] = ai[u];
-                            if (zero_fs) a.L.fsF[w] = make_double4(0, 0, 0, 0);
-                        }
-                        w += in;
-                    }
-                }
+// Decoupled look-back over tiles taken in ticket order: tile t publishes
+// its count (AGG), then walks back over its predecessors' words, adding
+// counts until it meets an inclusive prefix (PRE), and publishes its own
+// inclusive prefix.  A word is (epoch << 32) | flag | value; words of an
+// earlier launch carry an older epoch and are waited for.  Tiles take their
+// tickets in dispatch order, so every predecessor has started and will
+// publish; a wait that outlasts kSpinMax polls is abandoned with
+// ST_LOOKBACK set (the list is then unusable and the host reports it).
+// The words are read and written with atomic read-modify-writes, which are
+// coherent across the XCDs' L2 caches without any cache maintenance; the
+// values a tile needs travel inside the words themselves.
+constexpr unsigned long long kLbAgg = 1ull << 30, kLbPre = 2ull << 30, kLbVal = (1ull << 30) - 1;
+constexpr long kSpinMax = 1l << 25;
+__device__ __forceinline__ unsigned long long lb_read(unsigned long long* p) { return atomicAdd(p, 0ull); }
+__device__ __forceinline__ void lb_write(unsigned long long* p, unsigned long long v) { atomicExch(p, v); }
+// Exclusive prefix of tile t, computed by one whole wavefront of the tile:
+// lane l reads the word of predecessor top - l, so a window of 64
+// predecessors costs one round of loads.  The window's counts up to the
+// nearest inclusive prefix are summed; without one in the window, all 64
+// counts are added and the window moves back.  Index -1 acts as an
+// inclusive prefix 0.
+__device__ long long tile_prefix(unsigned long long* flags, int t, unsigned epoch, long long total,
+                                 unsigned long long* st) {
+    const int lane = threadIdx.x & 63;
+    const unsigned long long e = (unsigned long long)epoch << 32;
+    const unsigned long long tv = (unsigned long long)min(total, (long long)kLbVal);
+    if (lane == 0) lb_write(&flags[t], e | (t == 0 ? kLbPre : kLbAgg) | tv);
+    if (t == 0) return 0;
+    long long prefix = 0;
+    long spins = 0;
+    for (int top = t - 1;;) {
+        const int p = top - lane;
+        const unsigned long long f = p >= 0 ? lb_read(&flags[p]) : (e | kLbPre);
+        const bool ready = (f >> 32) == epoch;
+        const unsigned long long pre = __ballot(ready && (f & kLbPre) != 0);
+        const unsigned long long rdy = __ballot(ready);
+        const int first = pre ? __builtin_ctzll(pre) : 64;  // lane of the nearest prefix
+        const unsigned long long need = first >= 63 ? ~0ull : ((2ull << first) - 1);
+        if ((rdy & need) != need) {
+            if (++spins > kSpinMax) {
+                if (lane == 0) atomicOr(&st[ST_LOOKBACK], 1ull);
+                break;
             }
-            unsigned long long m0 = 0, m1 = 0;
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        long long v = lane <= first ? (long long)(f & kLbVal) : 0;
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+        prefix += v;
+        if (first < 64) break;
+        top -= 64;
+    }
+    if (lane == 0) lb_write(&flags[t], e | kLbPre | (unsigned long long)min(prefix + total, (long long)kLbVal));
+    return prefix;
+}
+
+// The per-step list pass, in one kernel.  Rows that must follow the
+// reference stencil this step (walk) get their exact fluid neighbours in
+// the ELL list, cnt = count; every other row keeps its superset for the
+// pair loops (cnt = -1), which skip candidates outside the kernel support
+// themselves.  Fluid–solid pairs are filtered exactly for every row: each
+// workgroup takes the next tile of slots by ticket, tests its candidates
+// (the members among a row's first 128 kept as a bit mask in registers),
+// scans the counts, gets the tile's offset by look-back and writes the CSR:
+// offsets, then pairs.
+__global__ void __launch_bounds__(kBS) k_filter(SearchArgs a, int zero_fs) {
+    __shared__ int tile_sh;
+    __shared__ unsigned epoch_sh;
+    __shared__ long long base_sh;
+    __shared__ int lds[kBS / 64];
+    if (threadIdx.x == 0) {
+        // ticket and epoch in one word, taken in one atomic step
+        const unsigned long long tk = atomicAdd(a.lb_ctl, 1ull);
+        epoch_sh = (unsigned)(tk >> 32);
+        tile_sh = (int)(unsigned)tk;
+    }
+    __syncthreads();
+    const int tile = tile_sh;
+    const unsigned epoch = epoch_sh;
+    const int s = tile * blockDim.x + threadIdx.x;
+    int c = 0, cs = 0;
+    bool walk = false;
+    double4 p = make_double4(0, 0, 0, 0);
+    unsigned long long m0 = 0, m1 = 0;
+    int n2 = 0;
+    unsigned spare = 0;
+    if (s < a.nf) {
+        p = a.ph[s];
+        const GridP& g = grid_at(a.G, fluid_scene(a.G, s));
+        spare = (unsigned)(g.cell0 + g.ncell - 1);  // the scene's spare cell
+        walk = a.ctl[CTL_WALK_ALL] != 0 || g.walk_only || a.hslot[s] == spare || a.cspare[a.hslot[s]] != 0;
+        if (walk) {
+            int64_t w0 = 0;
+            lane_walk<false>(a, p, s, c, cs, w0);
+        } else {
+            n2 = min(a.L.fsup_cnt[s], a.L.fsupK);
+            const int* __restrict__ fsup = a.L.fsup + s;
             const int last = n2 > 0 ? n2 - 1 : 0;
             // candidates in batches of 8: indices, then hashes and positions,
             // all in flight together (entries past the row end re-read its
             // last one and are not used)
             constexpr int U = 8;
-            for (int k = k0; k < n2; k += U) {
+            for (int k = 0; k < n2; k += U) {
                 int ai[U];
                 unsigned key[U];
                 double4 q[U];
 #pragma unroll
-                for (int u = 0; u < U; u++)
-                    ai[u] = __builtin_nontemporal_load(fsup + (int64_t)min(k + u, last) * a.nf);
+                for (int u = 0; u < U; u++) ai[u] = fsup[(int64_t)min(k + u, last) * a.nf];
 #pragma unroll
                 for (int u = 0; u < U; u++) {
                     key[u] = a.skey[ai[u]];
@@ -713,37 +783,78 @@ __global__ void __launch_bounds__(kBS) k_filter(SearchArgs a, int zero_fs) {
 #pragma unroll
                 for (int u = 0; u < U; u++) {
                     const bool in = (k + u < n2) & (key[u] != spare) &
-                                    within(p.x, p.y, p.z, q[u].x, q[u].y, q[u].z, 2.0 * hi);
-                    if (FILL) {
-                        if (in && w < a.L.fs_cap) {
-                            a.L.fs_i[w] = s;
-                            a.L.fs_a[w] = ai[u];
-                            if (zero_fs) a.L.fsF[w] = make_double4(0, 0, 0, 0);
-                        }
-                        w += in;
-                    } else {
-                        cs += in;
-                        const int kk = k + u;
-                        const unsigned long long bit = in ? 1ull << (kk & 63) : 0ull;
-                        if (kk < 64) m0 |= bit;
-                        else if (kk < 128) m1 |= bit;
-                    }
+                                    within(p.x, p.y, p.z, q[u].x, q[u].y, q[u].z, 2.0 * p.w);
+                    cs += in;
+                    const int kk = k + u;
+                    const unsigned long long bit = in ? 1ull << (kk & 63) : 0ull;
+                    if (kk < 64) m0 |= bit;
+                    else if (kk < 128) m1 |= bit;
                 }
             }
-            if (!FILL) {
-                mask[0] = m0;
-                mask[1] = m1;
-            }
         }
-        if (!FILL) {
-            a.L.cnt[s] = walk ? c : -1;
-            a.L.fscnt[s] = cs;
-            a.fsp[s] = p;
+        a.L.cnt[s] = walk ? c : -1;
+        a.L.fscnt[s] = cs;
+        a.fsp[s] = p;
+    }
+    wave_max_atomic(&a.st[ST_FF_MAX], (unsigned long long)c);
+    if (c > a.L.K) atomicOr(&a.st[ST_FF_OVERFLOW], 1ull);
+    // the tile's offsets
+    const int inc = block_scan_int(cs, lds);
+    if (threadIdx.x >= blockDim.x - 64) {
+        // the last wavefront looks back; its lane 63 holds the tile's total
+        const long long pre = tile_prefix(a.lb_tiles, tile, epoch, __shfl(inc, 63, 64), a.st);
+        if (threadIdx.x == blockDim.x - 1) base_sh = pre;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && tile == (int)gridDim.x - 1)
+        // the last ticket is taken: the next launch counts from 0 in a new epoch
+        atomicExch(a.lb_ctl, (unsigned long long)(epoch + 1u) << 32);
+    if (s >= a.nf) return;
+    int64_t w = base_sh + inc - cs;
+    a.L.fsoff[s] = (int)w;
+    if (s == a.nf - 1) {
+        const int64_t tot = w + cs;
+        a.L.fsoff[a.nf] = (int)tot;
+        a.st[ST_FS_TOTAL] = (unsigned long long)tot;
+        if (tot > a.L.fs_cap) atomicOr(&a.st[ST_FS_OVERFLOW], 1ull);
+    }
+    if (cs == 0) return;
+    if (walk) {
+        int c2 = 0, cs2 = 0;
+        lane_walk<true>(a, p, s, c2, cs2, w);
+        return;
+    }
+    const int* __restrict__ fsup = a.L.fsup + s;
+    const int k0 = min(n2, 128);
+    constexpr int V = 8;
+    for (int k = 0; k < k0; k += V) {
+        int ai[V];
+#pragma unroll
+        for (int u = 0; u < V; u++) ai[u] = fsup[(int64_t)min(k + u, k0 - 1) * a.nf];
+#pragma unroll
+        for (int u = 0; u < V; u++) {
+            const int kk = k + u;
+            const bool in = kk < k0 && (((kk < 64 ? m0 : m1) >> (kk & 63)) & 1ull);
+            if (in && w < a.L.fs_cap) {
+                a.L.fs_i[w] = s;
+                a.L.fs_a[w] = ai[u];
+                if (zero_fs) a.L.fsF[w] = make_double4(0, 0, 0, 0);
+            }
+            w += in;
         }
     }
-    if (!FILL) {
-        wave_max_atomic(&a.st[ST_FF_MAX], (unsigned long long)c);
-        if (c > a.L.K) atomicOr(&a.st[ST_FF_OVERFLOW], 1ull);
+    // members past the first 128 candidates are tested again
+    for (int k = k0; k < n2; k++) {
+        const int ai = fsup[(int64_t)k * a.nf];
+        const double4 q = a.spr[ai];
+        if (a.skey[ai] != spare && within(p.x, p.y, p.z, q.x, q.y, q.z, 2.0 * p.w)) {
+            if (w < a.L.fs_cap) {
+                a.L.fs_i[w] = s;
+                a.L.fs_a[w] = ai;
+                if (zero_fs) a.L.fsF[w] = make_double4(0, 0, 0, 0);
+            }
+            w++;
+        }
     }
 }
 
@@ -778,12 +889,6 @@ __global__ void __launch_bounds__(kBS) k_exact_rows(SearchArgs a, int* __restric
     out_cnt[s] = c;
 }
 
-__global__ void k_fs_status(const int* __restrict__ fsoff, int nf, int64_t cap,
-                            unsigned long long* st) {
-    const long long tot = fsoff[nf];
-    st[ST_FS_TOTAL] = (unsigned long long)tot;
-    if (tot > cap) st[ST_FS_OVERFLOW] |= 1ull;
-}
 
 // ------------------------------------------------ reverse (transposed) lists
 // reverse CSR of a pair list by its second index (optionally only keys with
@@ -1185,13 +1290,9 @@ void fluid_exact_rows(hipStream_t s, const SearchArgs& a, int* out, int* out_cnt
         hipLaunchKernelGGL(k_exact_rows, dim3(blocks_for(a.nf)), dim3(kBS), 0, s, a, out, out_cnt);
 }
 
-void fluid_filter(hipStream_t s, const SearchArgs& a, int* scan_tmp, bool zero_fs) {
+void fluid_filter(hipStream_t s, const SearchArgs& a, bool zero_fs) {
     if (a.nf <= 0) return;
-    hipLaunchKernelGGL(k_filter<false>, dim3(blocks_for(a.nf)), dim3(kBS), 0, s, a, 0);
-    exclusive_scan(s, a.L.fscnt, a.L.fsoff, a.nf, scan_tmp);
-    hipLaunchKernelGGL(k_fs_status, dim3(1), dim3(1), 0, s, (const int*)a.L.fsoff, a.nf, a.L.fs_cap,
-                       a.st);
-    hipLaunchKernelGGL(k_filter<true>, dim3(blocks_for(a.nf)), dim3(kBS), 0, s, a, (int)zero_fs);
+    hipLaunchKernelGGL(k_filter, dim3(blocks_for(a.nf)), dim3(kBS), 0, s, a, (int)zero_fs);
 }
 
 // Reverse CSR of pairs by their key (a solid): count into tmp (zero on
