@@ -421,6 +421,10 @@ SearchArgs search_args(dsph_ctx* c) {
     a.skey = c->skey;
     a.spr = c->so.pr;
     a.fsp = c->fsp;
+    a.fxb = c->fxb;
+    a.sxb = c->sxb;
+    a.slive = c->so.live;
+    a.nslive = c->so.nlive;
     a.lb_ctl = c->lb_ctl;
     a.lb_tiles = c->lb_tiles;
     a.L = c->L;
@@ -461,18 +465,21 @@ void neighbor_search(dsph_ctx* c, int i_step, int gap, bool save_history, bool z
             const Gate gate = c->ctl + CTL_SORT;
             radix_sort_iota(s, c->frank, c->nf, c->key_bits + c->sub_bits, c->fkey_sorted, c->fperm,
                             c->radix_tmp, gate);
-            cell_spans(s, c->fkey_sorted, c->nf, c->fcs, c->fce, c->ncell, c->cinv, gate, c->sub_bits);
             if (c->all_solid_sort) {
                 if (c->ns > 0)
                     radix_sort_iota(s, c->skey, c->ns, c->key_bits, c->skey_sorted, c->sperm, c->radix_tmp,
                                     gate);
                 cell_spans(s, c->skey_sorted, c->ns, c->scs, c->sce, c->ncell, nullptr, gate);
             }
-            if (c->reorder_period > 0)
+            const bool reorder = c->reorder_period > 0;
+            if (reorder)
                 reorder_fluid(s, c->fset[0], c->fset[1], c->fperm, c->fslot, c->nf, c->mixed(),
                               c->dcur, c->ctl);
-            member_slots(s, c->fperm, c->fslot, c->fkey_sorted, c->sub_bits, c->cinv, c->fmember, c->hslot,
-                         c->nf, gate);
+            // cell ranges, slots of the sorted positions, and the copy-back
+            // of a reorder, in one pass
+            after_sort(s, c->fset[0], c->fset[1], reorder, c->mixed(), c->dcur, c->fkey_sorted, c->nf,
+                       c->sub_bits, c->crank, c->ncell, c->fcs, c->fce, c->fperm, c->fslot, c->cinv,
+                       c->fmember, c->hslot, c->ctl);
         }
     }
     if (c->ns > 0) {
@@ -486,7 +493,7 @@ void neighbor_search(dsph_ctx* c, int i_step, int gap, bool save_history, bool z
         const SearchArgs sa = search_args(c);
         {
             FamTimer t(c, F_FSEARCH);
-            fluid_superset(s, sa, c->fchunks, c->nfchunks, c->fxb, c->sxb);
+            fluid_superset(s, sa, c->fchunks, c->nfchunks);
         }
         FamTimer t(c, F_FSLIST);
         fluid_filter(s, sa, zero_fs);

@@ -292,6 +292,10 @@ struct SearchArgs {
     const unsigned* skey;   // reference hash of each solid
     const double4* spr;
     double4* fsp;           // fluid positions of the last fluid search, slot order
+    double4* fxb;           // positions at the last build: fluid slots
+    double4* sxb;           // and solids (the live ones change)
+    const int* slive;
+    int nslive;
     Lists L;
     unsigned long long* ctl;
     unsigned long long* st;
@@ -328,12 +332,18 @@ void hash_live_decide(hipStream_t s, const Grids& G, const double4* pr, const in
                       unsigned long long* ctl, long long* status);
 void reorder_fluid(hipStream_t s, const FluidSet& f, const FluidSet& scratch, const int* perm,
                    int* slot_of, int n, bool mixed, int cur, const unsigned long long* ctl);
-void member_slots(hipStream_t s, const int* perm, const int* slot_of, const unsigned* sorted_rank,
-                  int sub_bits, const unsigned* cinv, int* member, unsigned* hslot, int n, Gate gate);
+// after a sort (CTL_SORT): the cell ranges of the sorted ranks (empty cells
+// get empty ranges), member[t] = slot of sorted position t and its hash;
+// with reorder, after a rebuild (CTL_REBUILD) the reordered set is copied
+// back from scratch
+void after_sort(hipStream_t s, const FluidSet& f, const FluidSet& scratch, bool reorder, bool mixed, int cur,
+                const unsigned* sorted, int n, int sub_bits, const unsigned* crank, int ncell, int* start,
+                int* end, const int* perm, const int* slot_of, const unsigned* cinv, int* member,
+                unsigned* hslot, const unsigned long long* ctl);
 // chunks: (first, end) sorted positions of <= 64 consecutive particles of
-// one scene, one wavefront each
-void fluid_superset(hipStream_t s, const SearchArgs& a, const int2* chunks, int nchunk, double4* fxb,
-                    double4* sxb);
+// one scene, one wavefront each; also stores the build positions (fxb, and
+// sxb of the live solids)
+void fluid_superset(hipStream_t s, const SearchArgs& a, const int2* chunks, int nchunk);
 // exact fluid neighbour rows of all particles for read-back (ELL, stride nf,
 // capacity max(K, supK)); out_cnt = -1 marks a walked row that overflowed
 void fluid_exact_rows(hipStream_t s, const SearchArgs& a, int* out, int* out_cnt);

@@ -301,44 +301,57 @@ __global__ void k_reorder(FluidSet src, FluidSet dst, const int* __restrict__ pe
     slot_of[o] = t;
 }
 
+// first sorted position whose cell rank (key >> sub_bits) is >= r
+__device__ __forceinline__ int rank_lower_bound(const unsigned* __restrict__ sorted, int n, unsigned r,
+                                                int sub_bits) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((sorted[mid] >> sub_bits) < r) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// After a sort (CTL_SORT): the range of every cell in the sorted order by
+// binary search over the sorted cell ranks (an empty cell gets an empty
+// range, so no reset is needed), and the slot of every sorted position with
+// its hash.  After a rebuild's reorder (CTL_REBUILD), the reordered set is
+// copied back in the same pass.
 template <class R4>
-__global__ void k_copyback(FluidSet dst, FluidSet src, int n, int cur,
-                           const unsigned long long* __restrict__ ctl) {
-    if (ctl[CTL_REBUILD] == 0) return;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    dst.ph[t] = src.ph[t];
-    dst.vr[t] = src.vr[t];
-    dst.drho[cur][t] = src.drho[cur][t];
-    dst.cn[t] = src.cn[t];
-    dst.sumw0[t] = src.sumw0[t];
-    dst.alpha[t] = src.alpha[t];
-    dst.scene[t] = src.scene[t];
-    dst.acc[t] = src.acc[t];
-    dst.orig[t] = src.orig[t];
-    for (int u = 0; u < 2; u++) ((R4*)dst.rec[cur])[2 * t + u] = ((const R4*)src.rec[cur])[2 * t + u];
-    dst.tid[t] = src.tid[t];
-}
-
-// slot of every sorted position and, after a reorder, the reference hash of
-// every slot in its new place (without one the hash kernel's are current)
-__global__ void k_member(const int* __restrict__ perm, const int* __restrict__ slot_of,
-                         const unsigned* __restrict__ sorted_rank, int sub_bits,
-                         const unsigned* __restrict__ cinv, int* __restrict__ member,
-                         unsigned* __restrict__ hslot, int n, Gate gate) {
-    if (*gate == 0) return;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const int s = slot_of[perm[t]];
-    member[t] = s;
-    hslot[s] = cinv[sorted_rank[t] >> sub_bits];
-}
-
-__global__ void k_store_xb(const double4* __restrict__ p, double4* __restrict__ xb, int n,
-                           const unsigned long long* __restrict__ ctl) {
-    if (ctl[CTL_REBUILD] == 0) return;
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < n) xb[s] = p[s];
+__global__ void k_after_sort(FluidSet dst, FluidSet src, int n, int cur, int reorder,
+                             const unsigned* __restrict__ sorted, int sub_bits,
+                             const unsigned* __restrict__ crank, int ncell, int* __restrict__ start,
+                             int* __restrict__ end, const int* __restrict__ perm,
+                             const int* __restrict__ slot_of, const unsigned* __restrict__ cinv,
+                             int* __restrict__ member, unsigned* __restrict__ hslot,
+                             const unsigned long long* __restrict__ ctl) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ctl[CTL_SORT] != 0) {
+        if (t < ncell) {
+            const unsigned r = crank[t];
+            start[t] = rank_lower_bound(sorted, n, r, sub_bits);
+            end[t] = rank_lower_bound(sorted, n, r + 1, sub_bits);
+        }
+        if (t < n) {
+            const int s = slot_of[perm[t]];
+            member[t] = s;
+            hslot[s] = cinv[sorted[t] >> sub_bits];
+        }
+    }
+    if (reorder && ctl[CTL_REBUILD] != 0 && t < n) {
+        dst.ph[t] = src.ph[t];
+        dst.vr[t] = src.vr[t];
+        dst.drho[cur][t] = src.drho[cur][t];
+        dst.cn[t] = src.cn[t];
+        dst.sumw0[t] = src.sumw0[t];
+        dst.alpha[t] = src.alpha[t];
+        dst.scene[t] = src.scene[t];
+        dst.acc[t] = src.acc[t];
+        dst.orig[t] = src.orig[t];
+        for (int u = 0; u < 2; u++) ((R4*)dst.rec[cur])[2 * t + u] = ((const R4*)src.rec[cur])[2 * t + u];
+        dst.tid[t] = src.tid[t];
+    }
 }
 
 // ------------------------------------------------------- cell blocks
@@ -377,6 +390,14 @@ __device__ __forceinline__ float slab_dist(float p, float lo, float w) {
 __global__ void __launch_bounds__(kBS) k_build(SearchArgs a, const int2* __restrict__ chunks,
                                                int nchunk) {
     if (a.ctl[CTL_REBUILD] == 0) return;
+    // the positions of this build: the reference of the next displacement
+    // tests (fluid slots, and the solids that can move)
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < a.nf; t += gridDim.x * blockDim.x)
+        a.fxb[t] = a.ph[t];
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < a.nslive; t += gridDim.x * blockDim.x) {
+        const int i = a.slive[t];
+        a.sxb[i] = a.spr[i];
+    }
     __shared__ float4 lq[kBS];  // candidate (x, y, z, reach^2), one 64-slot slice per wavefront
     __shared__ int lj[kBS];
     const int lane = threadIdx.x & 63;
@@ -1249,22 +1270,28 @@ void hash_live_decide(hipStream_t s, const Grids& G, const double4* pr, const in
 void reorder_fluid(hipStream_t s, const FluidSet& f, const FluidSet& scratch, const int* perm,
                    int* slot_of, int n, bool mixed, int cur, const unsigned long long* ctl) {
     if (n <= 0) return;
-    if (mixed) {
+    if (mixed)
         hipLaunchKernelGGL(k_reorder<float4>, dim3(blocks_for(n)), dim3(kBS), 0, s, f, scratch, perm,
                            slot_of, n, cur, ctl);
-        hipLaunchKernelGGL(k_copyback<float4>, dim3(blocks_for(n)), dim3(kBS), 0, s, f, scratch, n, cur, ctl);
-    } else {
+    else
         hipLaunchKernelGGL(k_reorder<double4>, dim3(blocks_for(n)), dim3(kBS), 0, s, f, scratch, perm,
                            slot_of, n, cur, ctl);
-        hipLaunchKernelGGL(k_copyback<double4>, dim3(blocks_for(n)), dim3(kBS), 0, s, f, scratch, n, cur, ctl);
-    }
 }
 
-void member_slots(hipStream_t s, const int* perm, const int* slot_of, const unsigned* sorted_rank,
-                  int sub_bits, const unsigned* cinv, int* member, unsigned* hslot, int n, Gate gate) {
-    if (n > 0)
-        hipLaunchKernelGGL(k_member, dim3(blocks_for(n)), dim3(kBS), 0, s, perm, slot_of, sorted_rank,
-                           sub_bits, cinv, member, hslot, n, gate);
+void after_sort(hipStream_t s, const FluidSet& f, const FluidSet& scratch, bool reorder, bool mixed, int cur,
+                const unsigned* sorted, int n, int sub_bits, const unsigned* crank, int ncell, int* start,
+                int* end, const int* perm, const int* slot_of, const unsigned* cinv, int* member,
+                unsigned* hslot, const unsigned long long* ctl) {
+    const int m = std::max(n, ncell);
+    if (m <= 0) return;
+    if (mixed)
+        hipLaunchKernelGGL(k_after_sort<float4>, dim3(blocks_for(m)), dim3(kBS), 0, s, f, scratch, n, cur,
+                           (int)reorder, sorted, sub_bits, crank, ncell, start, end, perm, slot_of, cinv, member,
+                           hslot, ctl);
+    else
+        hipLaunchKernelGGL(k_after_sort<double4>, dim3(blocks_for(m)), dim3(kBS), 0, s, f, scratch, n, cur,
+                           (int)reorder, sorted, sub_bits, crank, ncell, start, end, perm, slot_of, cinv, member,
+                           hslot, ctl);
 }
 
 static int chunk_grid(int nchunk) {
@@ -1274,15 +1301,9 @@ static int chunk_grid(int nchunk) {
     return b < 1 ? 1 : (b > 65535 ? 65535 : b);
 }
 
-void fluid_superset(hipStream_t s, const SearchArgs& a, const int2* chunks, int nchunk, double4* fxb,
-                    double4* sxb) {
+void fluid_superset(hipStream_t s, const SearchArgs& a, const int2* chunks, int nchunk) {
     if (a.nf <= 0) return;
     hipLaunchKernelGGL(k_build, dim3(chunk_grid(nchunk)), dim3(kBS), 0, s, a, chunks, nchunk);
-    hipLaunchKernelGGL(k_store_xb, dim3(blocks_for(a.nf)), dim3(kBS), 0, s, a.ph, fxb, a.nf,
-                       (const unsigned long long*)a.ctl);
-    if (a.ns > 0)
-        hipLaunchKernelGGL(k_store_xb, dim3(blocks_for(a.ns)), dim3(kBS), 0, s, a.spr, sxb, a.ns,
-                           (const unsigned long long*)a.ctl);
 }
 
 void fluid_exact_rows(hipStream_t s, const SearchArgs& a, int* out, int* out_cnt) {
