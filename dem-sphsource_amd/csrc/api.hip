@@ -96,6 +96,8 @@ struct dsph_ctx {
     bool skey_valid = false;
     unsigned long long* dispmax = nullptr;  // per-workgroup fluid displacement maxima
     int* rank_acc = nullptr;                // active-grid rank accumulators (zero between sorts)
+    int* cs_cnt = nullptr;                  // fluid sort: bucket counts (zero between sorts)
+    int* cs_off = nullptr;                  // and bucket offsets
     unsigned long long* lb_ctl = nullptr;   // fluid–solid list scan: epoch << 32 | ticket
     unsigned long long* lb_tiles = nullptr; // and its tile words
     unsigned long long static_out = 0;
@@ -463,8 +465,10 @@ void neighbor_search(dsph_ctx* c, int i_step, int gap, bool save_history, bool z
         c->skey_valid = true;
         if (fluid_step) {
             const Gate gate = c->ctl + CTL_SORT;
-            radix_sort_iota(s, c->frank, c->nf, c->key_bits + c->sub_bits, c->fkey_sorted, c->fperm,
-                            c->radix_tmp, gate);
+            // by cell rank (the bucket), then sub-cell code and index; the
+            // cell ranges come with it
+            counting_sort(s, c->frank, c->nf, c->sub_bits, c->ncell, c->cs_cnt, c->cs_off, c->scan_tmp,
+                          c->radix_tmp, c->fkey_sorted, c->fperm, c->cinv, c->fcs, c->fce, gate);
             if (c->all_solid_sort) {
                 if (c->ns > 0)
                     radix_sort_iota(s, c->skey, c->ns, c->key_bits, c->skey_sorted, c->sperm, c->radix_tmp,
@@ -475,11 +479,10 @@ void neighbor_search(dsph_ctx* c, int i_step, int gap, bool save_history, bool z
             if (reorder)
                 reorder_fluid(s, c->fset[0], c->fset[1], c->fperm, c->fslot, c->nf, c->mixed(),
                               c->dcur, c->ctl);
-            // cell ranges, slots of the sorted positions, and the copy-back
-            // of a reorder, in one pass
+            // slots of the sorted positions and the copy-back of a reorder,
+            // in one pass
             after_sort(s, c->fset[0], c->fset[1], reorder, c->mixed(), c->dcur, c->fkey_sorted, c->nf,
-                       c->sub_bits, c->crank, c->ncell, c->fcs, c->fce, c->fperm, c->fslot, c->cinv,
-                       c->fmember, c->hslot, c->ctl);
+                       c->sub_bits, c->fperm, c->fslot, c->cinv, c->fmember, c->hslot, c->ctl);
         }
     }
     if (c->ns > 0) {
@@ -1403,6 +1406,9 @@ int dsph_upload_batch(dsph_ctx* c, const dsph_scene* scenes, int32_t n_scenes) {
             }
             c->cspare = dalloc<unsigned char>(c, c->ncell);
             h2d(c, c->cspare, sp.data(), c->ncell);
+            c->cs_cnt = dalloc<int>(c, c->ncell);
+            c->cs_off = dalloc<int>(c, (size_t)c->ncell + 1);
+            fill_int(c->st, c->cs_cnt, c->ncell, 0);
             c->crank = dalloc<unsigned>(c, c->ncell);
             c->cinv = dalloc<unsigned>(c, c->ncell);
             h2d(c, c->crank, rank.data(), c->ncell);

@@ -256,6 +256,13 @@ void scan_keys(hipStream_t s, const int* keys, int nkeys, int* cnt, int* cnt_out
                int n_all);
 // lists at most this long are scanned by scan_keys (one workgroup)
 constexpr int kKeyScanMax = 16384;
+// stable sort of (key, index) pairs by counting into buckets key >> shift
+// (< nbuckets), then ranking within each bucket; cnt: nbuckets zeroed ints,
+// zero again afterwards; off: nbuckets + 1; tmp: 2 n words.  With
+// start/end, the range of bucket b is stored under inv[b] (or b).
+void counting_sort(hipStream_t s, const unsigned* keys, int n, int shift, int nbuckets, int* cnt, int* off,
+                   int* scan_tmp, unsigned* tmp, unsigned* keys_out, int* perm_out, const unsigned* inv,
+                   int* start, int* end, Gate gate = nullptr);
 // stable LSD radix sort of (key, index) pairs; keys < 2^bits; vals out = perm
 int64_t radix_tmp_size(int64_t n);
 void radix_sort_iota(hipStream_t s, const unsigned* keys_in, int64_t n, int bits,
@@ -332,14 +339,12 @@ void hash_live_decide(hipStream_t s, const Grids& G, const double4* pr, const in
                       unsigned long long* ctl, long long* status);
 void reorder_fluid(hipStream_t s, const FluidSet& f, const FluidSet& scratch, const int* perm,
                    int* slot_of, int n, bool mixed, int cur, const unsigned long long* ctl);
-// after a sort (CTL_SORT): the cell ranges of the sorted ranks (empty cells
-// get empty ranges), member[t] = slot of sorted position t and its hash;
-// with reorder, after a rebuild (CTL_REBUILD) the reordered set is copied
-// back from scratch
+// after a sort (CTL_SORT): member[t] = slot of sorted position t and its
+// hash; with reorder, after a rebuild (CTL_REBUILD) the reordered set is
+// copied back from scratch
 void after_sort(hipStream_t s, const FluidSet& f, const FluidSet& scratch, bool reorder, bool mixed, int cur,
-                const unsigned* sorted, int n, int sub_bits, const unsigned* crank, int ncell, int* start,
-                int* end, const int* perm, const int* slot_of, const unsigned* cinv, int* member,
-                unsigned* hslot, const unsigned long long* ctl);
+                const unsigned* sorted, int n, int sub_bits, const int* perm, const int* slot_of,
+                const unsigned* cinv, int* member, unsigned* hslot, const unsigned long long* ctl);
 // chunks: (first, end) sorted positions of <= 64 consecutive particles of
 // one scene, one wavefront each; also stores the build positions (fxb, and
 // sxb of the live solids)

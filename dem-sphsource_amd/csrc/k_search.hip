@@ -301,38 +301,17 @@ __global__ void k_reorder(FluidSet src, FluidSet dst, const int* __restrict__ pe
     slot_of[o] = t;
 }
 
-// first sorted position whose cell rank (key >> sub_bits) is >= r
-__device__ __forceinline__ int rank_lower_bound(const unsigned* __restrict__ sorted, int n, unsigned r,
-                                                int sub_bits) {
-    int lo = 0, hi = n;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if ((sorted[mid] >> sub_bits) < r) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
-
-// After a sort (CTL_SORT): the range of every cell in the sorted order by
-// binary search over the sorted cell ranks (an empty cell gets an empty
-// range, so no reset is needed), and the slot of every sorted position with
-// its hash.  After a rebuild's reorder (CTL_REBUILD), the reordered set is
+// After a sort (CTL_SORT): the slot of every sorted position with its
+// hash.  After a rebuild's reorder (CTL_REBUILD), the reordered set is
 // copied back in the same pass.
 template <class R4>
 __global__ void k_after_sort(FluidSet dst, FluidSet src, int n, int cur, int reorder,
-                             const unsigned* __restrict__ sorted, int sub_bits,
-                             const unsigned* __restrict__ crank, int ncell, int* __restrict__ start,
-                             int* __restrict__ end, const int* __restrict__ perm,
+                             const unsigned* __restrict__ sorted, int sub_bits, const int* __restrict__ perm,
                              const int* __restrict__ slot_of, const unsigned* __restrict__ cinv,
                              int* __restrict__ member, unsigned* __restrict__ hslot,
                              const unsigned long long* __restrict__ ctl) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (ctl[CTL_SORT] != 0) {
-        if (t < ncell) {
-            const unsigned r = crank[t];
-            start[t] = rank_lower_bound(sorted, n, r, sub_bits);
-            end[t] = rank_lower_bound(sorted, n, r + 1, sub_bits);
-        }
         if (t < n) {
             const int s = slot_of[perm[t]];
             member[t] = s;
@@ -1279,19 +1258,15 @@ void reorder_fluid(hipStream_t s, const FluidSet& f, const FluidSet& scratch, co
 }
 
 void after_sort(hipStream_t s, const FluidSet& f, const FluidSet& scratch, bool reorder, bool mixed, int cur,
-                const unsigned* sorted, int n, int sub_bits, const unsigned* crank, int ncell, int* start,
-                int* end, const int* perm, const int* slot_of, const unsigned* cinv, int* member,
-                unsigned* hslot, const unsigned long long* ctl) {
-    const int m = std::max(n, ncell);
-    if (m <= 0) return;
+                const unsigned* sorted, int n, int sub_bits, const int* perm, const int* slot_of,
+                const unsigned* cinv, int* member, unsigned* hslot, const unsigned long long* ctl) {
+    if (n <= 0) return;
     if (mixed)
-        hipLaunchKernelGGL(k_after_sort<float4>, dim3(blocks_for(m)), dim3(kBS), 0, s, f, scratch, n, cur,
-                           (int)reorder, sorted, sub_bits, crank, ncell, start, end, perm, slot_of, cinv, member,
-                           hslot, ctl);
+        hipLaunchKernelGGL(k_after_sort<float4>, dim3(blocks_for(n)), dim3(kBS), 0, s, f, scratch, n, cur,
+                           (int)reorder, sorted, sub_bits, perm, slot_of, cinv, member, hslot, ctl);
     else
-        hipLaunchKernelGGL(k_after_sort<double4>, dim3(blocks_for(m)), dim3(kBS), 0, s, f, scratch, n, cur,
-                           (int)reorder, sorted, sub_bits, crank, ncell, start, end, perm, slot_of, cinv, member,
-                           hslot, ctl);
+        hipLaunchKernelGGL(k_after_sort<double4>, dim3(blocks_for(n)), dim3(kBS), 0, s, f, scratch, n, cur,
+                           (int)reorder, sorted, sub_bits, perm, slot_of, cinv, member, hslot, ctl);
 }
 
 static int chunk_grid(int nchunk) {
