@@ -98,8 +98,7 @@ struct dsph_ctx {
     int* rank_acc = nullptr;                // active-grid rank accumulators (zero between sorts)
     int* cs_cnt = nullptr;                  // fluid sort: bucket counts (zero between sorts)
     int* cs_off = nullptr;                  // and bucket offsets
-    unsigned long long* lb_ctl = nullptr;   // fluid–solid list scan: epoch << 32 | ticket
-    unsigned long long* lb_tiles = nullptr; // and its tile words
+    int* tsum = nullptr;                    // fluid–solid count per tile of 256 slots
     unsigned long long static_out = 0;
     int* dcs = nullptr;
     int* dce = nullptr;
@@ -427,8 +426,7 @@ SearchArgs search_args(dsph_ctx* c) {
     a.sxb = c->sxb;
     a.slive = c->so.live;
     a.nslive = c->so.nlive;
-    a.lb_ctl = c->lb_ctl;
-    a.lb_tiles = c->lb_tiles;
+    a.tsum = c->tsum;
     a.L = c->L;
     a.ctl = c->ctl;
     a.st = c->status;
@@ -530,7 +528,6 @@ void search_checked(dsph_ctx* c, int i_step, int gap) {
     for (int attempt = 0; attempt < 8; attempt++) {
         unsigned long long w[ST_WORDS];
         read_status(c, w);
-        if (w[ST_LOOKBACK]) throw ApiError{DSPH_E_HIP, "internal error: a list scan was abandoned"};
         const bool ovf = w[ST_FF_OVERFLOW] | w[ST_FS_OVERFLOW] | w[ST_SS_OVERFLOW] | w[ST_SUP_OVERFLOW];
         if (!ovf) {
             // superset rows change only at builds, which graph-replayed steps
@@ -797,11 +794,17 @@ std::vector<int64_t> fs_reference_order(dsph_ctx* c, std::vector<int>& fluid_idx
     HIPCHK(hipStreamSynchronize(c->st));
     int64_t tot = off[nf];
     if (tot > c->L.fs_cap) throw ApiError{DSPH_E_CAPACITY, "fluid-solid list overflowed"};
+    // the offsets must rise from 0 to the total and the partners be solids
+    bool ok = tot >= 0 && off[0] == 0;
+    for (int s2 = 0; ok && s2 < nf; s2++) ok = off[s2] <= off[s2 + 1];
+    if (!ok) throw ApiError{DSPH_E_HIP, "internal error: inconsistent fluid-solid list"};
     std::vector<int> a(tot);
     std::vector<unsigned> skey(c->ns);
     d2h(c, a.data(), c->L.fs_a, tot);
     d2h(c, skey.data(), c->skey, c->ns);
     HIPCHK(hipStreamSynchronize(c->st));
+    for (int64_t k = 0; k < tot; k++)
+        if (a[k] < 0 || a[k] >= c->ns) throw ApiError{DSPH_E_HIP, "internal error: inconsistent fluid-solid list"};
     std::vector<int64_t> ids;
     ids.reserve(tot);
     fluid_idx.clear();
@@ -1424,16 +1427,7 @@ int dsph_upload_batch(dsph_ctx* c, const dsph_scene* scenes, int32_t n_scenes) {
         c->fsp = dalloc<double4>(c, nf);
         h2d(c, c->fsp, ph.data(), nf);
         c->dispmax = dalloc<unsigned long long>(c, std::max(1, blocks_for(std::max(nf, 1))));
-        {
-            // tickets start at 0, the epoch at 1 (tile words start at epoch 0)
-            const unsigned long long lb0 = 1ull << 32;
-            c->lb_ctl = dalloc<unsigned long long>(c, 1);
-            h2d(c, c->lb_ctl, &lb0, 1);
-            const size_t nt = (size_t)std::max(1, blocks_for(std::max(nf, 1)));
-            c->lb_tiles = dalloc<unsigned long long>(c, nt);
-            HIPCHK(hipMemsetAsync(c->lb_tiles, 0, nt * sizeof(unsigned long long), c->st));
-            HIPCHK(hipStreamSynchronize(c->st));
-        }
+        c->tsum = dalloc<int>(c, std::max(1, blocks_for(std::max(nf, 1))));
         c->ctl = dalloc<unsigned long long>(c, CTL_WORDS);
         HIPCHK(hipMemsetAsync(c->ctl, 0, CTL_WORDS * sizeof(unsigned long long), c->st));
         c->scs = dalloc<int>(c, c->ncell);
@@ -1657,6 +1651,7 @@ int dsph_upload_batch(dsph_ctx* c, const dsph_scene* scenes, int32_t n_scenes) {
         alloc_sup(c, nf > 0 ? 160 : 1, nf > 0 ? 96 : 1);
         L.sup_cnt = dalloc<int>(c, nf);
         L.fsup_cnt = dalloc<int>(c, nf);
+        L.fsm = dalloc<unsigned long long>(c, 2 * (size_t)std::max(nf, 1));
         fill_int(c->st, L.sup_cnt, nf, 0);
         fill_int(c->st, L.fsup_cnt, nf, 0);
         L.grow = dalloc<int>(c, nf);
@@ -1923,7 +1918,6 @@ int dsph_step_n(dsph_ctx* c, int32_t i0, int32_t n, double dt, int32_t gap) {
         }
         unsigned long long w[ST_WORDS];
         read_status(c, w);
-        if (w[ST_LOOKBACK]) throw ApiError{DSPH_E_HIP, "internal error: a list scan was abandoned"};
         if (w[ST_FF_OVERFLOW] | w[ST_FS_OVERFLOW] | w[ST_SS_OVERFLOW] | w[ST_SUP_OVERFLOW]) {
             c->err = "neighbor list overflow during dsph_step_n; enlarge capacities (dsph_set_capacity) and rerun";
             return DSPH_E_CAPACITY;

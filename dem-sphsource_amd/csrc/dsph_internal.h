@@ -39,7 +39,6 @@ enum {
     ST_SUP_OVERFLOW = 7,
     ST_REBUILDS = 8,
     ST_SUP_MAX = 9,        // longest (group) superset row of this search's build
-    ST_LOOKBACK = 10,      // a single-pass scan gave up waiting (sticky; an internal error)
     ST_WORDS = 12
 };
 
@@ -198,6 +197,9 @@ struct Lists {
     int fsupK = 0;
     int* fsup = nullptr;
     int* fsup_cnt = nullptr;
+    // members of the fluid–solid list among the first 128 candidates of
+    // each slot's solid superset row (two words per slot)
+    unsigned long long* fsm = nullptr;
     // solid–solid CSR by solid index (the reference's own list)
     int64_t ss_cap = 0;
     int* sscnt = nullptr;
@@ -306,10 +308,8 @@ struct SearchArgs {
     Lists L;
     unsigned long long* ctl;
     unsigned long long* st;
-    // single-pass tile scan of the fluid–solid list: (epoch << 32 | next
-    // ticket) and one word per tile of 256 slots
-    unsigned long long* lb_ctl;
-    unsigned long long* lb_tiles;
+    // fluid–solid count of each tile of 256 slots (the list pass)
+    int* tsum;
 };
 // also the reference hash of every slot, and CTL_NEAR_SPARE when a particle's
 // cell has a stencil reaching the spare cell; with fxb, the largest squared

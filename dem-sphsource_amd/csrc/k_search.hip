@@ -666,124 +666,52 @@ __device__ void lane_walk(const SearchArgs& a, const double4 p, int s, int& c, i
             }
 }
 
-// Decoupled look-back over tiles taken in ticket order: tile t publishes
-// its count (AGG), then walks back over its predecessors' words, adding
-// counts until it meets an inclusive prefix (PRE), and publishes its own
-// inclusive prefix.  A word is (epoch << 32) | flag | value; words of an
-// earlier launch carry an older epoch and are waited for.  Tiles take their
-// tickets in dispatch order, so every predecessor has started and will
-// publish; a wait that outlasts kSpinMax polls is abandoned with
-// ST_LOOKBACK set (the list is then unusable and the host reports it).
-// The words are read and written with atomic read-modify-writes, which are
-// coherent across the XCDs' L2 caches without any cache maintenance; the
-// values a tile needs travel inside the words themselves.
-constexpr unsigned long long kLbAgg = 1ull << 30, kLbPre = 2ull << 30, kLbVal = (1ull << 30) - 1;
-constexpr long kSpinMax = 1l << 25;
-__device__ __forceinline__ unsigned long long lb_read(unsigned long long* p) { return atomicAdd(p, 0ull); }
-__device__ __forceinline__ void lb_write(unsigned long long* p, unsigned long long v) { atomicExch(p, v); }
-// Exclusive prefix of tile t, computed by one whole wavefront of the tile:
-// lane l reads the word of predecessor top - l, so a window of 64
-// predecessors costs one round of loads.  The window's counts up to the
-// nearest inclusive prefix are summed; without one in the window, all 64
-// counts are added and the window moves back.  Index -1 acts as an
-// inclusive prefix 0.
-__device__ long long tile_prefix(unsigned long long* flags, int t, unsigned epoch, long long total,
-                                 unsigned long long* st) {
-    const int lane = threadIdx.x & 63;
-    const unsigned long long e = (unsigned long long)epoch << 32;
-    const unsigned long long tv = (unsigned long long)min(total, (long long)kLbVal);
-    if (lane == 0) lb_write(&flags[t], e | (t == 0 ? kLbPre : kLbAgg) | tv);
-    if (t == 0) return 0;
-    long long prefix = 0;
-    long spins = 0;
-    for (int top = t - 1;;) {
-        const int p = top - lane;
-        const unsigned long long f = p >= 0 ? lb_read(&flags[p]) : (e | kLbPre);
-        const bool ready = (f >> 32) == epoch;
-        const unsigned long long pre = __ballot(ready && (f & kLbPre) != 0);
-        const unsigned long long rdy = __ballot(ready);
-        const int first = pre ? __builtin_ctzll(pre) : 64;  // lane of the nearest prefix
-        const unsigned long long need = first >= 63 ? ~0ull : ((2ull << first) - 1);
-        if ((rdy & need) != need) {
-            if (++spins > kSpinMax) {
-                if (lane == 0) atomicOr(&st[ST_LOOKBACK], 1ull);
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-            continue;
-        }
-        long long v = lane <= first ? (long long)(f & kLbVal) : 0;
-#pragma unroll
-        for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
-        prefix += v;
-        if (first < 64) break;
-        top -= 64;
-    }
-    if (lane == 0) lb_write(&flags[t], e | kLbPre | (unsigned long long)min(prefix + total, (long long)kLbVal));
-    return prefix;
-}
-
-// The per-step list pass, in one kernel.  Rows that must follow the
+// The per-step list pass, in two kernels.  Rows that must follow the
 // reference stencil this step (walk) get their exact fluid neighbours in
-// the ELL list, cnt = count; every other row keeps its superset for the
-// pair loops (cnt = -1), which skip candidates outside the kernel support
-// themselves.  Fluid–solid pairs are filtered exactly for every row: each
-// workgroup takes the next tile of slots by ticket, tests its candidates
-// (the members among a row's first 128 kept as a bit mask in registers),
-// scans the counts, gets the tile's offset by look-back and writes the CSR:
-// offsets, then pairs.
-__global__ void __launch_bounds__(kBS) k_filter(SearchArgs a, int zero_fs) {
-    __shared__ int tile_sh;
-    __shared__ unsigned epoch_sh;
-    __shared__ long long base_sh;
+// the ELL list, cnt = count; every other row keeps its superset for the pair
+// loops (cnt = -1), which skip candidates outside the kernel support
+// themselves.  Fluid–solid pairs are filtered exactly for every row.
+// COUNT tests the candidates of each slot, records the members among its
+// row's first 128 in a bit mask, and leaves the count of its workgroup's
+// tile of slots in tsum.  FILL gives each tile its offset, the sum of the
+// earlier tiles' counts (read by the whole workgroup at once, no waiting on
+// other workgroups), scans the tile, and writes the CSR: offsets, then
+// pairs.
+__global__ void __launch_bounds__(kBS) k_filter_count(SearchArgs a) {
     __shared__ int lds[kBS / 64];
-    if (threadIdx.x == 0) {
-        // ticket and epoch in one word, taken in one atomic step
-        const unsigned long long tk = atomicAdd(a.lb_ctl, 1ull);
-        epoch_sh = (unsigned)(tk >> 32);
-        tile_sh = (int)(unsigned)tk;
-    }
-    __syncthreads();
-    const int tile = tile_sh;
-    const unsigned epoch = epoch_sh;
+    const int tile = xcd_block();
     const int s = tile * blockDim.x + threadIdx.x;
     int c = 0, cs = 0;
-    bool walk = false;
-    double4 p = make_double4(0, 0, 0, 0);
-    unsigned long long m0 = 0, m1 = 0;
-    int n2 = 0;
-    unsigned spare = 0;
     if (s < a.nf) {
-        p = a.ph[s];
+        const double4 p = a.ph[s];
         const GridP& g = grid_at(a.G, fluid_scene(a.G, s));
-        spare = (unsigned)(g.cell0 + g.ncell - 1);  // the scene's spare cell
-        walk = a.ctl[CTL_WALK_ALL] != 0 || g.walk_only || a.hslot[s] == spare || a.cspare[a.hslot[s]] != 0;
+        const unsigned spare = (unsigned)(g.cell0 + g.ncell - 1);  // the scene's spare cell
+        const bool walk =
+            a.ctl[CTL_WALK_ALL] != 0 || g.walk_only || a.hslot[s] == spare || a.cspare[a.hslot[s]] != 0;
+        unsigned long long m0 = 0, m1 = 0;
         if (walk) {
             int64_t w0 = 0;
             lane_walk<false>(a, p, s, c, cs, w0);
         } else {
-            n2 = min(a.L.fsup_cnt[s], a.L.fsupK);
+            const int n2 = min(a.L.fsup_cnt[s], a.L.fsupK);
             const int* __restrict__ fsup = a.L.fsup + s;
             const int last = n2 > 0 ? n2 - 1 : 0;
-            // candidates in batches of 8: indices, then hashes and positions,
-            // all in flight together (entries past the row end re-read its
-            // last one and are not used)
+            // candidates in batches of 8: indices, then positions, all in
+            // flight together (entries past the row end re-read its last one
+            // and are not used).  No candidate is in the spare cell here: a
+            // particle or solid out of the domain makes every row walked
+            // (CTL_WALK_ALL), so only the distance decides.
             constexpr int U = 8;
             for (int k = 0; k < n2; k += U) {
                 int ai[U];
-                unsigned key[U];
                 double4 q[U];
 #pragma unroll
                 for (int u = 0; u < U; u++) ai[u] = fsup[(int64_t)min(k + u, last) * a.nf];
 #pragma unroll
-                for (int u = 0; u < U; u++) {
-                    key[u] = a.skey[ai[u]];
-                    q[u] = a.spr[ai[u]];
-                }
+                for (int u = 0; u < U; u++) q[u] = a.spr[ai[u]];
 #pragma unroll
                 for (int u = 0; u < U; u++) {
-                    const bool in = (k + u < n2) & (key[u] != spare) &
-                                    within(p.x, p.y, p.z, q[u].x, q[u].y, q[u].z, 2.0 * p.w);
+                    const bool in = (k + u < n2) & within(p.x, p.y, p.z, q[u].x, q[u].y, q[u].z, 2.0 * p.w);
                     cs += in;
                     const int kk = k + u;
                     const unsigned long long bit = in ? 1ull << (kk & 63) : 0ull;
@@ -794,23 +722,33 @@ __global__ void __launch_bounds__(kBS) k_filter(SearchArgs a, int zero_fs) {
         }
         a.L.cnt[s] = walk ? c : -1;
         a.L.fscnt[s] = cs;
+        a.L.fsm[2 * (int64_t)s] = m0;
+        a.L.fsm[2 * (int64_t)s + 1] = m1;
         a.fsp[s] = p;
     }
     wave_max_atomic(&a.st[ST_FF_MAX], (unsigned long long)c);
     if (c > a.L.K) atomicOr(&a.st[ST_FF_OVERFLOW], 1ull);
-    // the tile's offsets
     const int inc = block_scan_int(cs, lds);
-    if (threadIdx.x >= blockDim.x - 64) {
-        // the last wavefront looks back; its lane 63 holds the tile's total
-        const long long pre = tile_prefix(a.lb_tiles, tile, epoch, __shfl(inc, 63, 64), a.st);
-        if (threadIdx.x == blockDim.x - 1) base_sh = pre;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0 && tile == (int)gridDim.x - 1)
-        // the last ticket is taken: the next launch counts from 0 in a new epoch
-        atomicExch(a.lb_ctl, (unsigned long long)(epoch + 1u) << 32);
+    if (threadIdx.x == blockDim.x - 1) a.tsum[tile] = inc;
+}
+
+__global__ void __launch_bounds__(kBS) k_filter_fill(SearchArgs a, int zero_fs) {
+    __shared__ int lds[kBS / 64];
+    __shared__ long long part[kBS / 64];
+    const int tile = xcd_block();
+    const int s = tile * blockDim.x + threadIdx.x;
+    // the tile's offset: the earlier tiles' counts, summed by the workgroup
+    long long pre = 0;
+    for (int b = threadIdx.x; b < tile; b += blockDim.x) pre += a.tsum[b];
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) pre += __shfl_xor(pre, d, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = pre;
+    const int cs = s < a.nf ? a.L.fscnt[s] : 0;
+    const int inc = block_scan_int(cs, lds);  // its barriers also publish part[]
+    long long base = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) base += part[w];
     if (s >= a.nf) return;
-    int64_t w = base_sh + inc - cs;
+    int64_t w = base + inc - cs;
     a.L.fsoff[s] = (int)w;
     if (s == a.nf - 1) {
         const int64_t tot = w + cs;
@@ -819,12 +757,15 @@ __global__ void __launch_bounds__(kBS) k_filter(SearchArgs a, int zero_fs) {
         if (tot > a.L.fs_cap) atomicOr(&a.st[ST_FS_OVERFLOW], 1ull);
     }
     if (cs == 0) return;
-    if (walk) {
+    if (a.L.cnt[s] >= 0) {  // a walked row: its pairs from the stencil walk
+        const double4 p = a.ph[s];
         int c2 = 0, cs2 = 0;
         lane_walk<true>(a, p, s, c2, cs2, w);
         return;
     }
+    const int n2 = min(a.L.fsup_cnt[s], a.L.fsupK);
     const int* __restrict__ fsup = a.L.fsup + s;
+    const unsigned long long m0 = a.L.fsm[2 * (int64_t)s], m1 = a.L.fsm[2 * (int64_t)s + 1];
     const int k0 = min(n2, 128);
     constexpr int V = 8;
     for (int k = 0; k < k0; k += V) {
@@ -844,16 +785,19 @@ __global__ void __launch_bounds__(kBS) k_filter(SearchArgs a, int zero_fs) {
         }
     }
     // members past the first 128 candidates are tested again
-    for (int k = k0; k < n2; k++) {
-        const int ai = fsup[(int64_t)k * a.nf];
-        const double4 q = a.spr[ai];
-        if (a.skey[ai] != spare && within(p.x, p.y, p.z, q.x, q.y, q.z, 2.0 * p.w)) {
-            if (w < a.L.fs_cap) {
-                a.L.fs_i[w] = s;
-                a.L.fs_a[w] = ai;
-                if (zero_fs) a.L.fsF[w] = make_double4(0, 0, 0, 0);
+    if (n2 > k0) {
+        const double4 p = a.ph[s];
+        for (int k = k0; k < n2; k++) {
+            const int ai = fsup[(int64_t)k * a.nf];
+            const double4 q = a.spr[ai];
+            if (within(p.x, p.y, p.z, q.x, q.y, q.z, 2.0 * p.w)) {
+                if (w < a.L.fs_cap) {
+                    a.L.fs_i[w] = s;
+                    a.L.fs_a[w] = ai;
+                    if (zero_fs) a.L.fsF[w] = make_double4(0, 0, 0, 0);
+                }
+                w++;
             }
-            w++;
         }
     }
 }
@@ -1288,7 +1232,8 @@ void fluid_exact_rows(hipStream_t s, const SearchArgs& a, int* out, int* out_cnt
 
 void fluid_filter(hipStream_t s, const SearchArgs& a, bool zero_fs) {
     if (a.nf <= 0) return;
-    hipLaunchKernelGGL(k_filter, dim3(blocks_for(a.nf)), dim3(kBS), 0, s, a, (int)zero_fs);
+    hipLaunchKernelGGL(k_filter_count, dim3(blocks_for(a.nf)), dim3(kBS), 0, s, a);
+    hipLaunchKernelGGL(k_filter_fill, dim3(blocks_for(a.nf)), dim3(kBS), 0, s, a, (int)zero_fs);
 }
 
 // Reverse CSR of pairs by their key (a solid): count into tmp (zero on
