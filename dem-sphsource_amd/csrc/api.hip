@@ -252,12 +252,10 @@ void grow_sup_keep(dsph_ctx* c, int K) {
 }
 
 void alloc_fs(dsph_ctx* c, int64_t cap) {
-    dfree(c, c->L.fs_i);
     dfree(c, c->L.fs_a);
     dfree(c, c->L.fsF);
     dfree(c, c->L.fsr_idx);
     c->L.fs_cap = cap;
-    c->L.fs_i = dalloc<int>(c, cap);
     c->L.fs_a = dalloc<int>(c, cap);
     c->L.fsF = dalloc<double4>(c, cap);
     c->L.fsr_idx = dalloc<int>(c, cap);

@@ -179,7 +179,6 @@ struct Lists {
     int64_t fs_cap = 0;
     int* fscnt = nullptr;
     int* fsoff = nullptr;  // NF + 1
-    int* fs_i = nullptr;   // fluid slot
     int* fs_a = nullptr;   // solid index
     double4* fsF = nullptr;
     // reverse fluid–solid for movable solids: CSR by solid

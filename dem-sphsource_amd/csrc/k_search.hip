@@ -653,7 +653,6 @@ __device__ void lane_walk(const SearchArgs& a, const double4 p, int s, int& c, i
                     if (!within(p.x, p.y, p.z, q.x, q.y, q.z, 2.0 * hi)) continue;
                     if (FILL) {
                         if (w < a.L.fs_cap) {
-                            a.L.fs_i[w] = s;
                             a.L.fs_a[w] = ai;
                             a.L.fsF[w] = make_double4(0, 0, 0, 0);
                         }
@@ -732,74 +731,98 @@ __global__ void __launch_bounds__(kBS) k_filter_count(SearchArgs a) {
     if (threadIdx.x == blockDim.x - 1) a.tsum[tile] = inc;
 }
 
+// Pairs of one wavefront's slots are contiguous in the list: a wavefront
+// whose pairs fit kFillBuf gathers them in LDS (each lane its own members
+// from its mask) and then stores them with consecutive lanes on consecutive
+// entries; the others, and walked rows, store lane by lane.
+constexpr int kFillBuf = 1024;
 __global__ void __launch_bounds__(kBS) k_filter_fill(SearchArgs a, int zero_fs) {
     __shared__ int lds[kBS / 64];
     __shared__ long long part[kBS / 64];
+    __shared__ int buf[kBS / 64][kFillBuf];
     const int tile = xcd_block();
     const int s = tile * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
     // the tile's offset: the earlier tiles' counts, summed by the workgroup
     long long pre = 0;
+#pragma unroll 8
     for (int b = threadIdx.x; b < tile; b += blockDim.x) pre += a.tsum[b];
 #pragma unroll
     for (int d = 32; d > 0; d >>= 1) pre += __shfl_xor(pre, d, 64);
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = pre;
-    const int cs = s < a.nf ? a.L.fscnt[s] : 0;
+    if (lane == 0) part[threadIdx.x >> 6] = pre;
+    const bool live = s < a.nf;
+    const int cs = live ? a.L.fscnt[s] : 0;
     const int inc = block_scan_int(cs, lds);  // its barriers also publish part[]
     long long base = 0;
     for (int w = 0; w < (int)(blockDim.x >> 6); w++) base += part[w];
-    if (s >= a.nf) return;
     int64_t w = base + inc - cs;
-    a.L.fsoff[s] = (int)w;
-    if (s == a.nf - 1) {
-        const int64_t tot = w + cs;
-        a.L.fsoff[a.nf] = (int)tot;
-        a.st[ST_FS_TOTAL] = (unsigned long long)tot;
-        if (tot > a.L.fs_cap) atomicOr(&a.st[ST_FS_OVERFLOW], 1ull);
+    if (live) {
+        a.L.fsoff[s] = (int)w;
+        if (s == a.nf - 1) {
+            const int64_t tot = w + cs;
+            a.L.fsoff[a.nf] = (int)tot;
+            a.st[ST_FS_TOTAL] = (unsigned long long)tot;
+            if (tot > a.L.fs_cap) atomicOr(&a.st[ST_FS_OVERFLOW], 1ull);
+        }
     }
-    if (cs == 0) return;
-    if (a.L.cnt[s] >= 0) {  // a walked row: its pairs from the stencil walk
+    const bool walked = live && cs > 0 && a.L.cnt[s] >= 0;
+    const int n2 = (live && cs > 0 && !walked) ? min(a.L.fsup_cnt[s], a.L.fsupK) : 0;
+    // the wavefront's pairs: [w0, w0 + T)
+    const int64_t w0 = __shfl(w, 0, 64);
+    const int64_t T = __shfl(w + cs, 63, 64) - w0;
+    const bool staged = T > 0 && T <= kFillBuf && __ballot(walked || n2 > 128) == 0;
+    if (walked) {  // its pairs from the stencil walk
         const double4 p = a.ph[s];
         int c2 = 0, cs2 = 0;
         lane_walk<true>(a, p, s, c2, cs2, w);
-        return;
     }
-    const int n2 = min(a.L.fsup_cnt[s], a.L.fsupK);
-    const int* __restrict__ fsup = a.L.fsup + s;
-    const unsigned long long m0 = a.L.fsm[2 * (int64_t)s], m1 = a.L.fsm[2 * (int64_t)s + 1];
-    const int k0 = min(n2, 128);
-    constexpr int V = 8;
-    for (int k = 0; k < k0; k += V) {
-        int ai[V];
+    int* const wb = buf[threadIdx.x >> 6];
+    if (n2 > 0) {
+        const int* __restrict__ fsup = a.L.fsup + s;
+        const unsigned long long m0 = a.L.fsm[2 * (int64_t)s], m1 = a.L.fsm[2 * (int64_t)s + 1];
+        const int k0 = min(n2, 128);
+        constexpr int V = 32;
+        for (int k = 0; k < k0; k += V) {
+            int ai[V];
 #pragma unroll
-        for (int u = 0; u < V; u++) ai[u] = fsup[(int64_t)min(k + u, k0 - 1) * a.nf];
+            for (int u = 0; u < V; u++) ai[u] = fsup[(int64_t)min(k + u, k0 - 1) * a.nf];
 #pragma unroll
-        for (int u = 0; u < V; u++) {
-            const int kk = k + u;
-            const bool in = kk < k0 && (((kk < 64 ? m0 : m1) >> (kk & 63)) & 1ull);
-            if (in && w < a.L.fs_cap) {
-                a.L.fs_i[w] = s;
-                a.L.fs_a[w] = ai[u];
-                if (zero_fs) a.L.fsF[w] = make_double4(0, 0, 0, 0);
-            }
-            w += in;
-        }
-    }
-    // members past the first 128 candidates are tested again
-    if (n2 > k0) {
-        const double4 p = a.ph[s];
-        for (int k = k0; k < n2; k++) {
-            const int ai = fsup[(int64_t)k * a.nf];
-            const double4 q = a.spr[ai];
-            if (within(p.x, p.y, p.z, q.x, q.y, q.z, 2.0 * p.w)) {
-                if (w < a.L.fs_cap) {
-                    a.L.fs_i[w] = s;
-                    a.L.fs_a[w] = ai;
-                    if (zero_fs) a.L.fsF[w] = make_double4(0, 0, 0, 0);
+            for (int u = 0; u < V; u++) {
+                const int kk = k + u;
+                const bool in = kk < k0 && (((kk < 64 ? m0 : m1) >> (kk & 63)) & 1ull);
+                if (in) {
+                    if (staged) wb[w - w0] = ai[u];
+                    else if (w < a.L.fs_cap) a.L.fs_a[w] = ai[u];
                 }
-                w++;
+                w += in;
+            }
+        }
+        // members past the first 128 candidates are tested again (never
+        // staged)
+        if (n2 > k0) {
+            const double4 p = a.ph[s];
+            for (int k = k0; k < n2; k++) {
+                const int ai = fsup[(int64_t)k * a.nf];
+                const double4 q = a.spr[ai];
+                if (within(p.x, p.y, p.z, q.x, q.y, q.z, 2.0 * p.w)) {
+                    if (w < a.L.fs_cap) a.L.fs_a[w] = ai;
+                    w++;
+                }
             }
         }
     }
+    if (staged) {
+        // the lanes' LDS writes are complete and visible to the wavefront
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int i = lane; i < T; i += 64)
+            if (w0 + i < a.L.fs_cap) a.L.fs_a[w0 + i] = wb[i];
+    }
+    // new pairs start with zero force where asked
+    if (zero_fs && T > 0)
+        for (int64_t i = w0 + lane; i < w0 + T; i += 64)
+            if (i < a.L.fs_cap) a.L.fsF[i] = make_double4(0, 0, 0, 0);
 }
 
 // Exact fluid neighbour rows of every particle for read-back, into a
