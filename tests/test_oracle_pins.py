@@ -523,3 +523,387 @@ def test_artificial_viscosity_slows_an_approaching_pair():
     a = _av_accel(sc, 0.1)
     assert a[0, 0] < 0 < a[1, 0]          # each pushed back along its approach
     assert np.abs(a[:, 1:]).max() < 1e-12 * abs(a[0, 0])
+
+
+# ------------------------------------------- fluid–wall terms and normals
+# The fluid–solid terms of continuity and momentum (integrate.cu:139-158,
+# 251-277) are a Riemann solve against a mirrored state: a wall particle a
+# of radius r_a acts on fluid particle i as a fluid particle at r_a would,
+# with h = 2.6 r_a, velocity 2 v_a - v_i, pressure P_a, density P_a/c^2 +
+# rho0_i and viscosity gamma_i, except that its volume is (2 r_a)^3 instead
+# of m/rho.  With the fluid particle on the wall normal through a
+# (e_ia = n_a) the two terms differ only by that volume ratio, so the
+# fluid–fluid term (pinned above) pins the wall term.
+_WALL_DX = 0.01
+_H = 1.3 * _WALL_DX
+_RHO0, _C = 1000.0, 20.0
+
+
+def _unit(v):
+    v = np.asarray(v, float)
+    return v / np.linalg.norm(v)
+
+
+def _wall_scene(n, d, v_i, v_a, drho_i, nu, g):
+    """Fluid particle i at distance d from wall particle a along the normal n."""
+    sc = scenes.empty_scene((0, 0, 0), (0.1, 0.1, 0.1))
+    sc.gravity = np.asarray(g, float)
+    p_i = np.array([0.05, 0.05, 0.05])
+    scenes.add_fluid(sc, [p_i], _H, _RHO0, _C, nu)
+    sc.fluid_vel = np.array([v_i], float)
+    sc.fluid_drho = np.array([drho_i])
+    scenes.add_dummy_wall(sc, [p_i - d * n], _H / 2.6)
+    sc.solid_vel = np.array([v_a], float)
+    sc.solid_normal = np.array([n], float)
+    return sc, p_i
+
+
+def _mirror_scene(n, d, v_i, v_a, drho_i, drho_j, nu, g):
+    """The same i with a fluid particle j in place of the wall particle."""
+    sc = scenes.empty_scene((0, 0, 0), (0.1, 0.1, 0.1))
+    sc.gravity = np.asarray(g, float)
+    p_i = np.array([0.05, 0.05, 0.05])
+    rho_i, rho_j = _RHO0 + drho_i, _RHO0 + drho_j
+    scenes.add_fluid(sc, [p_i], _H, _RHO0, _C, nu)
+    # gamma_j = nu_j rho_j equal to gamma_i
+    scenes.add_fluid(sc, [p_i - d * n], _H, _RHO0, _C, nu * rho_i / rho_j)
+    sc.fluid_vel = np.array([v_i, 2 * np.asarray(v_a) - np.asarray(v_i)], float)
+    sc.fluid_drho = np.array([drho_i, drho_j])
+    return sc
+
+
+def _fluid_step(sc, dt):
+    o = oracle.Oracle()
+    o.upload(sc)
+    o.initialize(compute_normals=False)
+    o.fluid_integrate(dt, 2, 1)      # no density re-init on step 2
+    f = o.fluid()
+    fs = o.fs_force() if sc.n_solid else None
+    o.close()
+    return f, fs
+
+
+_MIRROR_CASES = [
+    # normal, distance, v_i, v_a, dRho_i, nu, gravity
+    ((0.3, -0.2, 0.93), 0.8, (0.05, -0.1, 0.2), (0.01, 0.02, -0.03), 1.5, 1e-3, (0, 0, 0)),
+    ((0, 0, 1), 1.3, (0.0, 0.0, -0.3), (0.0, 0.0, 0.0), -0.7, 1e-6, (0, 0, 0)),
+    # gravity along n: the wall pressure takes its hydrostatic increment
+    ((0, 0, 1), 0.6, (0.1, 0.0, -0.2), (0.0, 0.05, 0.0), 0.4, 1e-3, (0, 0, 9.81)),
+]
+
+
+@pytest.mark.parametrize("case", _MIRROR_CASES)
+def test_wall_term_is_a_mirrored_fluid_particle(case):
+    n, d, v_i, v_a, drho_i, nu, g = case
+    n = _unit(n)
+    d *= _WALL_DX
+    rho_i = _RHO0 + drho_i
+    # P_a = P_i + rho_i max(0, g.n) (r_ia.n) (integrate.cu:152), as a density
+    drho_j = drho_i + rho_i * max(0.0, float(np.dot(g, n))) * d / _C ** 2
+    dt = 1e-10
+    fa, _ = _fluid_step(_wall_scene(n, d, v_i, v_a, drho_i, nu, g)[0], dt)
+    fb, _ = _fluid_step(_mirror_scene(n, d, v_i, v_a, drho_i, drho_j, nu, g), dt)
+    vol = (_RHO0 + drho_j) / _RHO0    # (2 r_a)^3 over the mirror particle's m/rho
+    rate_a = (fa["drho"][0] - drho_i) / dt
+    rate_b = (fb["drho"][0] - drho_i) / dt
+    assert abs(rate_b) > 1.0
+    assert rate_a == pytest.approx(vol * rate_b, rel=1e-6)
+    acc_a = fa["acc"][0] - g
+    acc_b = fb["acc"][0] - g
+    assert np.linalg.norm(acc_b) > 1.0
+    assert np.abs(acc_a - vol * acc_b).max() < 1e-7 * np.linalg.norm(acc_b)
+
+
+def test_wall_term_is_galilean_invariant():
+    # the mirror velocity 2 v_a - v_i moves with the frame
+    n = _unit((0.2, 0.1, 1.0))
+    u = np.array([0.3, -0.4, 0.25])
+    v_i, v_a = np.array([0.05, -0.1, 0.2]), np.array([0.01, 0.02, -0.03])
+    dt = 1e-10
+    f0, _ = _fluid_step(_wall_scene(n, 0.9 * _WALL_DX, v_i, v_a, 1.2, 1e-3, (0, 0, 0))[0], dt)
+    f1, _ = _fluid_step(_wall_scene(n, 0.9 * _WALL_DX, v_i + u, v_a + u, 1.2, 1e-3, (0, 0, 0))[0], dt)
+    r0, r1 = (f0["drho"][0] - 1.2) / dt, (f1["drho"][0] - 1.2) / dt
+    assert r1 == pytest.approx(r0, rel=1e-6)
+    assert np.abs(f1["acc"][0] - f0["acc"][0]).max() < 1e-7 * np.linalg.norm(f0["acc"][0])
+
+
+def test_wall_riemann_speeds_use_the_wall_normal():
+    # Off the normal axis, a velocity tangential to the wall has no normal
+    # component: U_L = U_R, no Riemann dissipation, and the acceleration is
+    # odd in that velocity around its value at rest (only the viscous term
+    # moves).  Speeds taken along e_ia instead would switch the one-sided
+    # dissipation on for one sign only.
+    n = np.array([0.0, 0.0, 1.0])
+    off = _unit((0.6, 0.0, 0.8))              # r_ia direction, not n
+    t = np.array([1.0, 0.0, 0.0])             # tangential, with t.e_ia != 0
+    dt = 1e-10
+
+    def acc(v):
+        sc, p_i = _wall_scene(n, 0.0, v, (0, 0, 0), 0.8, 1e-3, (0, 0, 0))
+        sc.solid_pos = np.array([p_i - 0.9 * _WALL_DX * off])
+        return _fluid_step(sc, dt)[0]["acc"][0]
+
+    a0, ap, am = acc(0 * t), acc(0.3 * t), acc(-0.3 * t)
+    assert np.linalg.norm(ap - a0) > 1e-6 * np.linalg.norm(a0)
+    assert np.abs((ap - a0) + (am - a0)).max() < 1e-7 * np.linalg.norm(ap - a0)
+
+
+def test_wall_hydrostatic_pressure_is_one_sided():
+    # max(0, (g - a_a).n_a): gravity pointing out of the wall adds nothing
+    n = _unit((0.1, 0.0, 1.0))
+    v_i, v_a = (0.05, 0.0, -0.2), (0, 0, 0)
+    dt = 1e-10
+    f0, _ = _fluid_step(_wall_scene(n, 0.7 * _WALL_DX, v_i, v_a, 0.5, 1e-3, (0, 0, 0))[0], dt)
+    g = np.array([0.0, 0.0, -9.81])           # g.n < 0
+    f1, _ = _fluid_step(_wall_scene(n, 0.7 * _WALL_DX, v_i, v_a, 0.5, 1e-3, g)[0], dt)
+    assert np.abs((f1["acc"][0] - g) - f0["acc"][0]).max() < 1e-7 * np.linalg.norm(f0["acc"][0])
+    assert (f1["drho"][0] - 0.5) == pytest.approx(f0["drho"][0] - 0.5, rel=1e-6)
+    f2, _ = _fluid_step(_wall_scene(n, 0.7 * _WALL_DX, v_i, v_a, 0.5, 1e-3, -g)[0], dt)
+    assert np.linalg.norm((f2["acc"][0] + g) - f0["acc"][0]) > 1e-3 * np.linalg.norm(f0["acc"][0])
+
+
+@pytest.mark.parametrize("g", [(0, 0, 0), (0, 0, -9.81), (0, 0, 9.81)])
+def test_fluid_to_solid_force_is_the_reaction_of_the_wall_term(g):
+    # calFluid2SolidForce (integrate.cu:282-330) is the wall term of
+    # momentum times m_i, per pair: summed over the walls of a fluid
+    # particle with no fluid neighbour it is m_i (a_i - g)
+    sc = scenes.empty_scene((0, 0, 0), (0.1, 0.1, 0.1))
+    sc.gravity = np.asarray(g, float)
+    p_i = np.array([0.05, 0.05, 0.05])
+    scenes.add_fluid(sc, [p_i], _H, _RHO0, _C, 1e-3)
+    sc.fluid_vel = np.array([[0.05, -0.1, -0.2]])
+    sc.fluid_drho = np.array([0.9])
+    r_a = _H / 2.6
+    grid = [(x, y) for x in (-1, 0, 1) for y in (-1, 0, 1)]
+    walls = [p_i + np.array([x * 2 * r_a, y * 2 * r_a, -0.7 * _WALL_DX]) for x, y in grid]
+    scenes.add_dummy_wall(sc, walls, r_a)
+    sc.solid_vel = np.tile([0.0, 0.02, 0.0], (len(walls), 1))
+    sc.solid_normal = np.tile(_unit((0.05, 0.0, 1.0)), (len(walls), 1))
+    f, fs = _fluid_step(sc, 1e-10)
+    assert fs.shape == (len(walls), 3)
+    m_i = (_H / 1.3) ** 3 * _RHO0
+    total = fs.sum(axis=0)
+    expect = m_i * (f["acc"][0] - np.asarray(g, float))
+    assert np.linalg.norm(expect) > 0
+    assert np.abs(total - expect).max() < 1e-9 * np.linalg.norm(expect)
+
+
+def _wall_slab(nx, ny, nz, r):
+    pts = np.array([[i, j, k] for k in range(nz) for j in range(ny) for i in range(nx)], float)
+    return 0.02 + 2 * r * pts
+
+
+def _normals(sc):
+    o = oracle.Oracle()
+    o.upload(sc)
+    o.initialize(compute_normals=True)
+    nrm = o.solid()["normal"]
+    o.close()
+    return nrm
+
+
+def test_wall_normals_point_out_of_a_slab():
+    # setSolidNormal (dataBase.cpp:348-413): normalize(-sum V_B grad W) over
+    # fixed neighbours within 2h, h the largest fluid h; a flat slab's top
+    # layer points up, its bottom layer down, its middle layer (symmetric
+    # neighbourhood) has no normal
+    r = 0.0025
+    sc = scenes.empty_scene((0, 0, 0), (0.2, 0.2, 0.2))
+    scenes.add_fluid(sc, [[0.15, 0.15, 0.15]], 1.3 * 2 * r, _RHO0, _C, 1e-6)
+    nx, nz = 15, 5
+    pts = _wall_slab(nx, nx, nz, r)
+    scenes.add_dummy_wall(sc, pts, r)
+    nrm = _normals(sc)
+    ijk = np.rint((pts - 0.02) / (2 * r)).astype(int)
+    reach = 3   # 2h / spacing = 2.6: lattice neighbours within 3 spacings
+    inner = np.all((ijk[:, :2] >= reach) & (ijk[:, :2] < nx - reach), axis=1)
+    top = inner & (ijk[:, 2] == nz - 1)
+    bot = inner & (ijk[:, 2] == 0)
+    mid = inner & (ijk[:, 2] == nz // 2)
+    assert top.sum() > 20
+    assert np.abs(nrm[top] - [0, 0, 1]).max() < 1e-12
+    assert np.abs(nrm[bot] - [0, 0, -1]).max() < 1e-12
+    assert np.all(nrm[mid] == 0.0)
+
+
+def test_wall_normals_bisect_the_edge_of_a_bar():
+    # a bar of square section along y: an edge particle's neighbourhood is
+    # symmetric under x <-> z, so its normal is (1, 0, 1)/sqrt(2) up to sign
+    r = 0.0025
+    sc = scenes.empty_scene((0, 0, 0), (0.2, 0.2, 0.2))
+    scenes.add_fluid(sc, [[0.15, 0.15, 0.15]], 1.3 * 2 * r, _RHO0, _C, 1e-6)
+    n = 6
+    pts = _wall_slab(n, 16, n, r)
+    scenes.add_dummy_wall(sc, pts, r)
+    nrm = _normals(sc)
+    ijk = np.rint((pts - 0.02) / (2 * r)).astype(int)
+    along = (ijk[:, 1] >= 3) & (ijk[:, 1] < 13)
+    for (ex, ez), sgn in (((n - 1, n - 1), (1, 1)), ((0, 0), (-1, -1)), ((n - 1, 0), (1, -1))):
+        sel = along & (ijk[:, 0] == ex) & (ijk[:, 2] == ez)
+        assert sel.sum() == 10
+        expect = np.array([sgn[0], 0, sgn[1]]) / math.sqrt(2)
+        assert np.abs(nrm[sel] - expect).max() < 1e-12
+
+
+def test_density_reinit_counts_a_wall_particle_as_its_volume_of_fluid():
+    # densityReinitialization (integrate.cu:3-53) adds W(r, h_i) (2 r_a)^3 / V0_i
+    # per wall particle, V0_i = (h_i/1.3)^3: walls of radius h/2.6 on the
+    # fluid lattice weigh exactly as the fluid particles they replace
+    dx = 0.01
+    h = 1.3 * dx
+    fluid = scenes.regular_packed_points((0.03, 0.03, 0.03), (6 * dx,) * 3, dx)
+    floor = scenes.regular_packed_points((0.03, 0.03, 0.0), (6 * dx, 6 * dx, 3 * dx), dx)
+
+    def sumw0(walls_as_fluid):
+        sc = scenes.empty_scene((-0.02, -0.02, -0.02), (0.14, 0.14, 0.14))
+        sc.gravity = np.zeros(3)
+        scenes.add_fluid(sc, fluid, h, _RHO0, _C, 1e-6)
+        if walls_as_fluid:
+            scenes.add_fluid(sc, floor, h, _RHO0, _C, 1e-6)
+        else:
+            scenes.add_dummy_wall(sc, floor, h / 2.6)
+        o = oracle.Oracle()
+        o.upload(sc)
+        o.initialize(compute_normals=False)
+        o.fluid_integrate(1e-9, 1, 1)    # step 1 stores sumW0
+        out = o.fluid()["sumw0"][:len(fluid)]
+        o.close()
+        return out
+
+    a, b = sumw0(False), sumw0(True)
+    near = fluid[:, 2] < 0.03 + 2 * h
+    assert near.sum() > 30
+    assert np.abs(a - b).max() < 1e-12 * b.max()
+
+
+def test_linear_contact_restitution_closed_form():
+    # LinearContact (SPHKernelDEMContactModel.h:214-268): F_n = k_n d n -
+    # 2 zeta sqrt(m* k_n) v_n, a damped oscillator whose contact ends at
+    # d = 0 after half a damped period: e = exp(-pi zeta / sqrt(1 - zeta^2))
+    kn, zeta = 1.0e5, 0.2
+    params = [kn, 0.0, 0.0, 0.0, zeta, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0]
+    n = np.array([0, 0, 1.0])
+    F, _, _ = oracle.contact(1, [0, 0, 0], [0, 0, 0], n, 3e-5, 0.01, 0.005, 1e-5, params)
+    assert F[2] == pytest.approx(kn * 3e-5, rel=1e-14)
+    e = _head_on_rebound(params, 0.5, 0.01, 2500.0, 1e-7, model=1)
+    assert e == pytest.approx(math.exp(-math.pi * zeta / math.sqrt(1 - zeta * zeta)), abs=2e-3)
+
+
+# --------------------------------------------- DEM sums and list history
+def _forces_now(sc, dt=1e-6):
+    """Solid accelerations and torques of the state as loaded, with the
+    positions they were computed at (one search, a half kick-drift, the
+    contact phase)."""
+    o = oracle.Oracle()
+    o.upload(sc)
+    o.initialize()
+    o.neighbor_search(1, 1)
+    o.solid_before_contact(dt)
+    o.contact_forces(dt)
+    s = o.solid()
+    o.close()
+    return s
+
+
+def test_dem_contact_sums_conserve_momentum_and_angular_momentum():
+    # calSolidForceTorque (integrate.cu:806-858): +F and +T + (r_c - r_i) x F
+    # on the pointed solid, the negatives on the pointing one, both about the
+    # same contact point, so sum F = 0 and sum (T + r x F) = 0 without
+    # gravity or walls
+    rng = np.random.default_rng(5)
+    r = 0.004
+    sc = scenes.empty_scene((-0.05, -0.05, -0.05), (0.1, 0.1, 0.1))
+    sc.gravity = np.zeros(3)
+    g = np.array([[i, j, k] for i in range(4) for j in range(4) for k in range(3)], float)
+    pos = 1.9 * r * g + rng.uniform(-0.1 * r, 0.1 * r, g.shape)
+    vel = rng.normal(scale=0.05, size=g.shape)
+    scenes.add_solid(sc, pos, r, 2500.0, vel=vel)
+    sc.solid_angvel = rng.normal(scale=5.0, size=g.shape)
+    sc.set_hertz(0, 0, 5.4945e6, 1.131e6, 0.5, 0.3, 0.2, 0.4, 0.1, 0.1)
+    s = _forces_now(sc)
+    m = 1.0 / np.asarray(sc.solid_inv_mass)
+    F = m[:, None] * s["acc"]
+    scale = np.abs(F).sum()
+    assert scale > 0 and np.abs(s["torque"]).sum() > 0
+    assert np.abs(F.sum(axis=0)).max() < 1e-12 * scale
+    L = s["torque"] + np.cross(s["pos"], F)
+    assert np.abs(L.sum(axis=0)).max() < 1e-12 * (np.abs(s["torque"]).sum() + np.abs(np.cross(s["pos"], F)).sum())
+
+
+def test_fluid_force_on_a_sphere_is_the_reaction_of_the_fluid():
+    # the fluid-solid pair forces enter the solid sum with a minus sign
+    # (integrate.cu:846-853): fluid plus sphere exchange no momentum
+    sc = scenes.empty_scene((0, 0, 0), (0.12, 0.12, 0.12))
+    sc.gravity = np.zeros(3)
+    dx = 0.005
+    c = np.array([0.06, 0.06, 0.06])
+    rs = 0.01
+    pts = scenes.regular_packed_points((0.03, 0.03, 0.03), (0.06, 0.06, 0.06), dx)
+    d = np.linalg.norm(pts - c, axis=1)
+    pts = pts[(d > rs + 0.3 * dx) & (d < rs + 3 * dx)]
+    scenes.add_fluid(sc, pts, 1.3 * dx, _RHO0, _C, 1e-3)
+    rng = np.random.default_rng(9)
+    sc.fluid_vel = rng.normal(scale=0.05, size=pts.shape)
+    sc.fluid_drho = rng.normal(scale=0.5, size=len(pts))
+    scenes.add_solid(sc, [c], rs, 1200.0, vel=(0.02, -0.01, 0.0))
+    o = oracle.Oracle()
+    o.upload(sc)
+    o.initialize()
+    dt = 1e-7
+    o.neighbor_search(2, 1)
+    o.solid_before_contact(dt)
+    o.fluid_integrate(dt, 2, 1)
+    o.contact_forces(dt)
+    fl, so, fs = o.fluid(), o.solid(), o.fs_force()
+    o.close()
+    assert len(fs) > 20
+    m_s = 1.0 / float(sc.solid_inv_mass[0])
+    m_f = (1.3 * dx / 1.3) ** 3 * _RHO0
+    total_fs = fs.sum(axis=0)
+    assert np.linalg.norm(total_fs) > 0
+    assert np.abs(m_s * so["acc"][0] + total_fs).max() < 1e-12 * np.abs(fs).sum()
+    P = m_f * fl["acc"].sum(axis=0) + m_s * so["acc"][0]
+    assert np.abs(P).max() < 1e-10 * m_f * np.abs(fl["acc"]).sum()
+
+
+def test_sliding_spring_history_carries_over_between_steps():
+    # interactionSolid2Solid::save and the history copy
+    # (neighborSearch.cu:287-301): a persisting contact keeps its springs,
+    # so a steady tangential slip accumulates v_t dt per step
+    r = 0.005
+    sc = scenes.empty_scene((-0.05, -0.05, -0.05), (0.1, 0.1, 0.1))
+    sc.gravity = np.zeros(3)
+    v = 0.01
+    scenes.add_solid(sc, [[0, 0, 0], [0, 0, 2 * r - 1e-5]], r, 1.0e9, vel=(v, 0, 0))
+    sc.solid_inv_mass[0] = 0.0                                    # fixed
+    sc.solid_vel[0] = 0.0
+    sc.set_hertz(0, 0, 5.4945e6, 1.131e6, 1.0, 0, 0, 1e9, 0, 0)    # no damping, no cap
+    o = _oracle_for(sc)
+    dt = 1e-6
+    k = 6
+    for s in range(1, k + 1):
+        o.step(s, dt)
+    sl = o.contacts()["sliding"]
+    o.close()
+    assert sl.shape == (1, 3)
+    assert np.linalg.norm(sl[0]) == pytest.approx(k * v * dt, rel=1e-3)
+
+
+def test_bonds_join_touching_spheres_of_one_cluster_only():
+    # addBondData (dataBase.cpp:270-346): a contact becomes a bond when the
+    # spheres touch physically (r_i + r_j + 1e-10 > d), share a cluster and
+    # their material pair has E_b > 1e-10
+    r = 0.005
+    sc = scenes.empty_scene((-0.05, -0.05, -0.05), (0.1, 0.1, 0.1))
+    sc.gravity = np.zeros(3)
+    scenes.add_cluster(sc, [[0, 0, 0], [2 * r, 0, 0], [4 * r, 0, 0], [6 * r + 1e-6, 0, 0]],
+                       r, 2500.0)
+    scenes.add_cluster(sc, [[-2 * r, 0, 0]], r, 2500.0)           # touches solid 0
+    sc.set_hertz(0, 0, 5.4945e6, 1.131e6, 0.5, 0, 0, 0.4, 0, 0)
+    sc.set_bonded(0, 0, 1e8, 2.0, 1.0, 1e6, 1e6, 0.3)
+    o = _oracle_for(sc)
+    a, b, _ = o.list(3)
+    cand, _, _ = o.list(2)
+    o.close()
+    assert len(cand) >= 4                   # every pair above is a contact candidate
+    assert sorted(zip(a.tolist(), b.tolist())) == [(0, 1), (1, 2)]
