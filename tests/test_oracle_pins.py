@@ -907,3 +907,71 @@ def test_bonds_join_touching_spheres_of_one_cluster_only():
     o.close()
     assert len(cand) >= 4                   # every pair above is a contact candidate
     assert sorted(zip(a.tolist(), b.tolist())) == [(0, 1), (1, 2)]
+
+
+# ------------------------------------------------------------------ clumps
+def _tetra_clump(sc, centroid, r, vel=(0, 0, 0), angvel=(0, 0, 0)):
+    off = scenes.tetra_offsets(2 * r)
+    m1 = 4 / 3 * math.pi * r ** 3 * 2500
+    I = 4 * 0.4 * m1 * r * r + 4 * m1 * float(np.sum(off[0] ** 2)) * 2 / 3
+    scenes.add_clump(sc, np.asarray(centroid) + off, np.full(4, r), np.asarray(centroid, float),
+                     4 * m1, (I, I, 1.3 * I, 0, 0, 0), vel=vel, angvel=angvel)
+    return 4 * m1
+
+
+def test_clump_pebbles_move_rigidly():
+    # clumpVelocityAngularVelocityIntegrate (integrate.cu:908-929): after the
+    # clump kick every pebble has v_c + w x (r_p - r_c) and w
+    sc = scenes.empty_scene((-0.1, -0.1, -0.1), (0.2, 0.2, 0.2))
+    sc.gravity = np.array([0.0, 0.0, -9.81])
+    _tetra_clump(sc, (0.01, -0.02, 0.03), 0.002, vel=(0.1, 0.0, -0.05), angvel=(3.0, -1.0, 2.0))
+    o = _oracle_for(sc)
+    for s in range(1, 21):
+        o.step(s, 1e-4)
+    c, p = o.clump(), o.solid()
+    o.close()
+    rel = p["pos"] - c["centroid"][0]
+    expect = c["vel"][0] + np.cross(c["angvel"][0], rel)
+    assert np.abs(p["vel"] - expect).max() < 1e-14
+    assert np.abs(p["angvel"] - c["angvel"][0]).max() == 0.0
+    # half kicks around the contact phase; the first one has no acceleration
+    # yet, so 19.5 steps of gravity
+    assert c["vel"][0, 2] == pytest.approx(-0.05 - 9.81 * 19.5 * 1e-4, rel=1e-12)
+
+
+def test_clump_force_and_torque_sum_the_pebble_contact():
+    # calClumpForceTorque (integrate.cu:860-881): the clump takes its
+    # pebbles' forces and their torques about the centroid.  One pebble
+    # touches a fixed sphere of its own radius, so the contact point is the
+    # midpoint of the two centres.
+    r = 0.002
+    sc = scenes.empty_scene((-0.1, -0.1, -0.1), (0.2, 0.2, 0.2))
+    sc.gravity = np.zeros(3)
+    ctr = np.array([0.0, 0.0, 0.0])
+    M = _tetra_clump(sc, ctr, r, vel=(0.02, 0.0, -0.1), angvel=(5.0, 2.0, -3.0))
+    pebbles = np.asarray(sc.solid_pos)[:4]
+    low = int(np.argmin(pebbles[:, 2]))
+    p_low = pebbles[low]
+    fixed = p_low - np.array([0.0, 0.3 * r, 1.99 * r])
+    fixed = p_low + (fixed - p_low) / np.linalg.norm(fixed - p_low) * (2 * r - 2e-6)
+    scenes.add_solid(sc, [fixed], r, 2500.0)
+    sc.solid_inv_mass[-1] = 0.0
+    sc.set_hertz(0, 0, 5.4945e6, 1.131e6, 0.5, 0.3, 0.2, 0.4, 0.1, 0.1)
+    o = oracle.Oracle()
+    o.upload(sc)
+    o.initialize()
+    o.neighbor_search(1, 1)
+    dt = 1e-7
+    o.solid_before_contact(dt)
+    o.contact_forces(dt)
+    c, so, ct = o.clump(), o.solid(), o.contacts()
+    a, b, _ = o.list(2)
+    o.close()
+    assert len(a) == 1 and {int(a[0]), int(b[0])} == {low, 4}
+    sgn = 1.0 if int(a[0]) == low else -1.0     # the force is on the pointed solid
+    F, T = sgn * ct["force"][0], sgn * ct["torque"][0]
+    mid = 0.5 * (so["pos"][low] + so["pos"][4])
+    assert np.linalg.norm(F) > 0 and np.linalg.norm(T) > 0
+    assert np.abs(M * c["acc"][0] - F).max() < 1e-12 * np.linalg.norm(F)
+    Tc = T + np.cross(mid - c["centroid"][0], F)
+    assert np.abs(c["torque"][0] - Tc).max() < 1e-12 * np.linalg.norm(Tc)
