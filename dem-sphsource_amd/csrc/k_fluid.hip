@@ -291,8 +291,7 @@ enum NbMode { POS = 0, STATE = 1, STATE_NU = 2 };
 constexpr int kBatch = 4;
 static_assert(2 * kBatch <= kRowPad, "row arrays need 2 kBatch pad columns");
 template <bool MIX, int U, NbMode MODE, bool UNI, class Fn>
-__device__ __forceinline__ void for_row(const FluidArgs& a, Rsrc rs, int s, Fn&& fn) {
-    const Row row = nb_row(a.L, s);
+__device__ __forceinline__ void for_row_each(const FluidArgs& a, Rsrc rs, int s, const Row& row, Fn&& fn) {
     const int n = row.n;
     const int64_t st = a.nf;
     const int* ip = row.p;
@@ -322,6 +321,111 @@ __device__ __forceinline__ void for_row(const FluidArgs& a, Rsrc rs, int s, Fn&&
         }
 #pragma unroll
         for (int u = 0; u < U; u++) fn(q[u], ok[u]);
+    }
+}
+
+// Lanes of a wavefront whose slots share one superset row (a group of 2 or
+// 4 consecutive slots, aligned in the wavefront) split each batch of four
+// candidates: with G lanes sharing, lane l of the group gathers candidates
+// k + l (+ G), and every candidate's record is then broadcast to the
+// group's lanes by a quad permutation (DPP), so a record crosses the
+// texture path once per group instead of once per lane.  Each lane still
+// visits candidates k, k + 1, k + 2, k + 3 in row order with the same
+// records, so the sums are bitwise those of for_row_each.
+typedef int i32x2 __attribute__((ext_vector_type(2)));
+template <int C>
+__device__ __forceinline__ int bc32(int x) {
+    return __builtin_amdgcn_update_dpp(0, x, C, 0xF, 0xF, false);
+}
+template <int C, class T>
+__device__ __forceinline__ T bc(T x) {
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, bc32<C>(__builtin_bit_cast(int, x)));
+    } else {
+        i32x2 v = __builtin_bit_cast(i32x2, x);
+        v.x = bc32<C>(v.x);
+        v.y = bc32<C>(v.y);
+        return __builtin_bit_cast(T, v);
+    }
+}
+// quad permutation that gives every lane of its group of G the value of the
+// group's lane m
+template <int G, int M>
+constexpr int kQuadFrom = G == 4 ? M * 85 : 160 + 85 * M;
+template <int C, bool MIX, bool STATE>
+__device__ __forceinline__ void bc_rec(const Nb<MIX>& q, Nb<MIX>& r) {
+    r.px = bc<C>(q.px); r.py = bc<C>(q.py); r.pz = bc<C>(q.pz); r.id = bc<C>(q.id);
+    if constexpr (STATE) {
+        r.vx = bc<C>(q.vx); r.vy = bc<C>(q.vy); r.vz = bc<C>(q.vz); r.drho = bc<C>(q.drho);
+    }
+}
+template <bool MIX, int G, NbMode MODE, bool UNI, class Fn>
+__device__ __forceinline__ void for_row_split(const FluidArgs& a, Rsrc rs, int s, const Row& row, Fn&& fn) {
+    constexpr int L = 4 / G;  // gathers per lane per batch
+    constexpr bool ST = MODE != POS;
+    const int n = row.n;
+    const int64_t st = a.nf;
+    const int* ip = row.p + (int64_t)(threadIdx.x & (G - 1)) * st;
+    int jn[L];
+#pragma unroll
+    for (int u = 0; u < L; u++) jn[u] = __builtin_nontemporal_load(ip + u * G * st);
+    for (int k = 0; k < n; k += 4) {
+        ip += 4 * st;
+        Nb<MIX> q[L];
+        int jq[L];
+#pragma unroll
+        for (int u = 0; u < L; u++) {
+            load_rec<MIX, ST>(rs, jn[u], q[u]);
+            jq[u] = jn[u];
+        }
+#pragma unroll
+        for (int u = 0; u < L; u++) jn[u] = __builtin_nontemporal_load(ip + u * G * st);
+        // candidate k + c: gathered by lane c % G of the group, as its
+        // (c / G)-th load
+        Nb<MIX> c[4];
+        bool ok[4];
+        bc_rec<kQuadFrom<G, 0>, MIX, ST>(q[0], c[0]);
+        bc_rec<kQuadFrom<G, 1 % G>, MIX, ST>(q[1 / G], c[1]);
+        bc_rec<kQuadFrom<G, 2 % G>, MIX, ST>(q[2 / G], c[2]);
+        bc_rec<kQuadFrom<G, 3 % G>, MIX, ST>(q[3 / G], c[3]);
+        if constexpr (MODE == POS) {
+            ok[0] = bc<kQuadFrom<G, 0>>(jq[0]) != s;
+            ok[1] = bc<kQuadFrom<G, 1 % G>>(jq[1 / G]) != s;
+            ok[2] = bc<kQuadFrom<G, 2 % G>>(jq[2 / G]) != s;
+            ok[3] = bc<kQuadFrom<G, 3 % G>>(jq[3 / G]) != s;
+        } else {
+            ok[0] = ok[1] = ok[2] = ok[3] = true;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) ok[u] = ok[u] & (k + u < n);
+        if constexpr (ST) {
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                load_props<MIX, MODE == STATE_NU, UNI>(a, c[u]);
+                derive<MIX>(c[u]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) fn(c[u], ok[u]);
+    }
+}
+
+// The row walk of slot s.  Wavefronts whose slots all share superset rows
+// in aligned groups of 4 (or 2) take the split walk; any other wavefront
+// (walked rows, unshared rows, a partial wavefront) walks every row itself.
+template <bool MIX, int U, NbMode MODE, bool UNI, class Fn>
+__device__ __forceinline__ void for_row(const FluidArgs& a, Rsrc rs, int s, Fn&& fn) {
+    static_assert(U == 4, "the split walk takes batches of four");
+    const Row row = nb_row(a.L, s);
+    // walked rows are never shared: a key no other lane has
+    const int key = a.L.cnt[s] >= 0 ? -1 - s : a.L.grow[s];
+    const bool full = __builtin_amdgcn_read_exec() == ~0ull;
+    if (full && __ballot(key != bc<kQuadFrom<4, 0>>(key)) == 0) {
+        for_row_split<MIX, 4, MODE, UNI>(a, rs, s, row, fn);
+    } else if (full && __ballot(key != bc<kQuadFrom<2, 0>>(key)) == 0) {
+        for_row_split<MIX, 2, MODE, UNI>(a, rs, s, row, fn);
+    } else {
+        for_row_each<MIX, U, MODE, UNI>(a, rs, s, row, fn);
     }
 }
 
