@@ -335,7 +335,7 @@ __device__ __forceinline__ void for_row_each(const FluidArgs& a, Rsrc rs, int s,
 typedef int i32x2 __attribute__((ext_vector_type(2)));
 template <int C>
 __device__ __forceinline__ int bc32(int x) {
-    return __builtin_amdgcn_update_dpp(0, x, C, 0xF, 0xF, false);
+    return __builtin_amdgcn_mov_dpp(x, C, 0xF, 0xF, false);
 }
 template <int C, class T>
 __device__ __forceinline__ T bc(T x) {
