@@ -168,7 +168,7 @@ struct Clumps {
 // Every fluid row array (nbr, sup) is allocated with kRowPad columns past its
 // capacity: the pair loops read row indices ahead of use, up to 2 batches
 // past a row's end, without clamping.
-constexpr int kRowPad = 8;
+constexpr int kRowPad = 16;
 
 struct Lists {
     // fluid–fluid ELL

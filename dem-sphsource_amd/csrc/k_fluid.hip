@@ -41,6 +41,7 @@
 //   2 V_i / m_i = 2 / rho_i.
 // Pairs closer than 1e-10 contribute nothing (grad W = 0 there upstream).
 #include <type_traits>
+#include <utility>
 
 #include "dsph_internal.h"
 
@@ -359,9 +360,40 @@ __device__ __forceinline__ void bc_rec(const Nb<MIX>& q, Nb<MIX>& r) {
         r.vx = bc<C>(q.vx); r.vy = bc<C>(q.vy); r.vz = bc<C>(q.vz); r.drho = bc<C>(q.drho);
     }
 }
-template <bool MIX, int G, NbMode MODE, bool UNI, class Fn>
+// candidate M of a batch (row order): gathered by lane M % G of the group
+// as its (M / G)-th load; broadcast to the group, its props read
+template <int M, int G, bool MIX, NbMode MODE, bool UNI>
+__device__ __forceinline__ void split_cand(const FluidArgs& a, const Nb<MIX>* q, const int* jq, int s, int k,
+                                           int n, Nb<MIX>& c, bool& ok) {
+    constexpr int C = kQuadFrom<G, M % G>;
+    bc_rec<C, MIX, MODE != POS>(q[M / G], c);
+    ok = k + M < n;
+    if constexpr (MODE == POS) ok = ok & (bc<C>(jq[M / G]) != s);
+    if constexpr (MODE != POS) {
+        load_props<MIX, MODE == STATE_NU, UNI>(a, c);
+        derive<MIX>(c);
+    }
+}
+template <int G, int H, bool MIX, NbMode MODE, bool UNI, class Fn, int... M>
+__device__ __forceinline__ void split_quarter(const FluidArgs& a, const Nb<MIX>* q, const int* jq, int s, int k,
+                                              int n, Fn&& fn, std::integer_sequence<int, M...>) {
+    Nb<MIX> c[4];
+    bool ok[4];
+    (split_cand<4 * H + M, G, MIX, MODE, UNI>(a, q, jq, s, k, n, c[M], ok[M]), ...);
+    (fn(c[M], ok[M]), ...);
+}
+template <int G, bool MIX, NbMode MODE, bool UNI, class Fn, int... H>
+__device__ __forceinline__ void split_batch(const FluidArgs& a, const Nb<MIX>* q, const int* jq, int s, int k,
+                                            int n, Fn&& fn, std::integer_sequence<int, H...>) {
+    (split_quarter<G, H, MIX, MODE, UNI>(a, q, jq, s, k, n, fn, std::make_integer_sequence<int, 4>{}), ...);
+}
+// B candidates per iteration (a multiple of 4): each lane gathers B / G of
+// them, and the next iteration's indices are read ahead, up to 2B - 1
+// columns past the batch start
+template <bool MIX, int G, int B, NbMode MODE, bool UNI, class Fn>
 __device__ __forceinline__ void for_row_split(const FluidArgs& a, Rsrc rs, int s, const Row& row, Fn&& fn) {
-    constexpr int L = 4 / G;  // gathers per lane per batch
+    static_assert(B % 4 == 0 && 2 * B - 1 <= kRowPad, "row arrays need 2B - 1 pad columns");
+    constexpr int L = B / G;  // gathers per lane per iteration
     constexpr bool ST = MODE != POS;
     const int n = row.n;
     const int64_t st = a.nf;
@@ -369,8 +401,8 @@ __device__ __forceinline__ void for_row_split(const FluidArgs& a, Rsrc rs, int s
     int jn[L];
 #pragma unroll
     for (int u = 0; u < L; u++) jn[u] = __builtin_nontemporal_load(ip + u * G * st);
-    for (int k = 0; k < n; k += 4) {
-        ip += 4 * st;
+    for (int k = 0; k < n; k += B) {
+        ip += B * st;
         Nb<MIX> q[L];
         int jq[L];
 #pragma unroll
@@ -380,36 +412,11 @@ __device__ __forceinline__ void for_row_split(const FluidArgs& a, Rsrc rs, int s
         }
 #pragma unroll
         for (int u = 0; u < L; u++) jn[u] = __builtin_nontemporal_load(ip + u * G * st);
-        // candidate k + c: gathered by lane c % G of the group, as its
-        // (c / G)-th load
-        Nb<MIX> c[4];
-        bool ok[4];
-        bc_rec<kQuadFrom<G, 0>, MIX, ST>(q[0], c[0]);
-        bc_rec<kQuadFrom<G, 1 % G>, MIX, ST>(q[1 / G], c[1]);
-        bc_rec<kQuadFrom<G, 2 % G>, MIX, ST>(q[2 / G], c[2]);
-        bc_rec<kQuadFrom<G, 3 % G>, MIX, ST>(q[3 / G], c[3]);
-        if constexpr (MODE == POS) {
-            ok[0] = bc<kQuadFrom<G, 0>>(jq[0]) != s;
-            ok[1] = bc<kQuadFrom<G, 1 % G>>(jq[1 / G]) != s;
-            ok[2] = bc<kQuadFrom<G, 2 % G>>(jq[2 / G]) != s;
-            ok[3] = bc<kQuadFrom<G, 3 % G>>(jq[3 / G]) != s;
-        } else {
-            ok[0] = ok[1] = ok[2] = ok[3] = true;
-        }
-#pragma unroll
-        for (int u = 0; u < 4; u++) ok[u] = ok[u] & (k + u < n);
-        if constexpr (ST) {
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                load_props<MIX, MODE == STATE_NU, UNI>(a, c[u]);
-                derive<MIX>(c[u]);
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < 4; u++) fn(c[u], ok[u]);
+        split_batch<G, MIX, MODE, UNI>(a, q, jq, s, k, n, fn, std::make_integer_sequence<int, B / 4>{});
     }
 }
 
+constexpr int kSplitBatch = 8;
 // The row walk of slot s.  Wavefronts whose slots all share superset rows
 // in aligned groups of 4 (or 2) take the split walk; any other wavefront
 // (walked rows, unshared rows, a partial wavefront) walks every row itself.
@@ -421,9 +428,9 @@ __device__ __forceinline__ void for_row(const FluidArgs& a, Rsrc rs, int s, Fn&&
     const int key = a.L.cnt[s] >= 0 ? -1 - s : a.L.grow[s];
     const bool full = __builtin_amdgcn_read_exec() == ~0ull;
     if (full && __ballot(key != bc<kQuadFrom<4, 0>>(key)) == 0) {
-        for_row_split<MIX, 4, MODE, UNI>(a, rs, s, row, fn);
+        for_row_split<MIX, 4, kSplitBatch, MODE, UNI>(a, rs, s, row, fn);
     } else if (full && __ballot(key != bc<kQuadFrom<2, 0>>(key)) == 0) {
-        for_row_split<MIX, 2, MODE, UNI>(a, rs, s, row, fn);
+        for_row_split<MIX, 2, kSplitBatch, MODE, UNI>(a, rs, s, row, fn);
     } else {
         for_row_each<MIX, U, MODE, UNI>(a, rs, s, row, fn);
     }
