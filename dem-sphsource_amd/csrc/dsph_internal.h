@@ -170,6 +170,20 @@ struct Clumps {
 // past a row's end, without clamping.
 constexpr int kRowPad = 16;
 
+// grow[slot] names the slot r whose storage holds the slot's superset row;
+// bits 28+ hold lg, the row's interleave: a group of 2^lg consecutive slots
+// r, r + 1, ... (all of one group) stores its shared row across their
+// storage, column c at (c >> lg) * nf + r + (c & (2^lg - 1)), so the
+// group's lanes read dense index lines.  lg = 0: column c at c * nf + r.
+// Slot counts stay below 2^26 (the pair loops' 32-bit record offsets).
+constexpr int kRowSlotBits = 28;
+__host__ __device__ __forceinline__ int row_slot(int g) { return g & ((1 << kRowSlotBits) - 1); }
+__host__ __device__ __forceinline__ int row_lg(int g) { return g >> kRowSlotBits; }
+__host__ __device__ __forceinline__ int64_t row_at(int g, int c, int nf) {
+    const int lg = row_lg(g);
+    return (int64_t)(c >> lg) * nf + row_slot(g) + (c & ((1 << lg) - 1));
+}
+
 struct Lists {
     // fluid–fluid ELL
     int K = 0;

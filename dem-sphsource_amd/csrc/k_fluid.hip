@@ -258,15 +258,17 @@ __device__ __forceinline__ void write_records(const FluidSet& f, int buf, int s,
 // the neighbour row of slot s: the exact walked row or the Verlet superset;
 // both are ELL column-major with stride nf
 struct Row {
-    const int* p;
+    const int* p;  // the row's storage slot
     int n;
+    int lg;        // interleave (row_at); walked rows have none
 };
 __device__ __forceinline__ Row nb_row(const Lists& L, int s) {
     const int c = L.cnt[s];
-    if (c >= 0) return {L.nbr + s, c < L.K ? c : L.K};
-    const int r = L.grow[s];  // superset rows are shared by a group of slots
+    if (c >= 0) return {L.nbr + s, c < L.K ? c : L.K, 0};
+    const int g = L.grow[s];  // superset rows are shared by a group of slots
+    const int r = row_slot(g);
     const int m = L.sup_cnt[r];
-    return {L.sup + r, m < L.supK ? m : L.supK};
+    return {L.sup + r, m < L.supK ? m : L.supK, row_lg(g)};
 }
 
 // What a pair loop reads of each candidate: its position only (POS), its
@@ -291,14 +293,18 @@ enum NbMode { POS = 0, STATE = 1, STATE_NU = 2 };
 // cache line.
 constexpr int kBatch = 4;
 static_assert(2 * kBatch <= kRowPad, "row arrays need 2 kBatch pad columns");
-template <bool MIX, int U, NbMode MODE, bool UNI, class Fn>
+// column c of a row at its storage slot p
+__device__ __forceinline__ int row_col(const int* p, int lg, int c, int64_t st) {
+    return __builtin_nontemporal_load(p + (int64_t)(c >> lg) * st + (c & ((1 << lg) - 1)));
+}
+template <bool MIX, int U, NbMode MODE, bool UNI, bool IL, class Fn>
 __device__ __forceinline__ void for_row_each(const FluidArgs& a, Rsrc rs, int s, const Row& row, Fn&& fn) {
     const int n = row.n;
     const int64_t st = a.nf;
     const int* ip = row.p;
     int jn[U];
 #pragma unroll
-    for (int u = 0; u < U; u++) jn[u] = __builtin_nontemporal_load(ip + u * st);
+    for (int u = 0; u < U; u++) jn[u] = IL ? row_col(row.p, row.lg, u, st) : __builtin_nontemporal_load(ip + u * st);
     for (int k = 0; k < n; k += U) {
         ip += U * st;
         Nb<MIX> q[U];
@@ -312,7 +318,8 @@ __device__ __forceinline__ void for_row_each(const FluidArgs& a, Rsrc rs, int s,
         }
         // read once per pass: streamed past the caches the records live in
 #pragma unroll
-        for (int u = 0; u < U; u++) jn[u] = __builtin_nontemporal_load(ip + u * st);
+        for (int u = 0; u < U; u++)
+            jn[u] = IL ? row_col(row.p, row.lg, k + U + u, st) : __builtin_nontemporal_load(ip + u * st);
         if constexpr (MODE != POS) {
 #pragma unroll
             for (int u = 0; u < U; u++) {
@@ -390,19 +397,23 @@ __device__ __forceinline__ void split_batch(const FluidArgs& a, const Nb<MIX>* q
 // B candidates per iteration (a multiple of 4): each lane gathers B / G of
 // them, and the next iteration's indices are read ahead, up to 2B - 1
 // columns past the batch start
-template <bool MIX, int G, int B, NbMode MODE, bool UNI, class Fn>
+// IL: the group's row is interleaved over its G slots (row_at), so lane l
+// reads the dense storage column of slot r + l
+template <bool MIX, int G, int B, NbMode MODE, bool UNI, bool IL, class Fn>
 __device__ __forceinline__ void for_row_split(const FluidArgs& a, Rsrc rs, int s, const Row& row, Fn&& fn) {
     static_assert(B % 4 == 0 && 2 * B - 1 <= kRowPad, "row arrays need 2B - 1 pad columns");
     constexpr int L = B / G;  // gathers per lane per iteration
     constexpr bool ST = MODE != POS;
+    constexpr int CS = IL ? 1 : G;  // storage columns between a lane's candidates
     const int n = row.n;
     const int64_t st = a.nf;
-    const int* ip = row.p + (int64_t)(threadIdx.x & (G - 1)) * st;
+    const int l = threadIdx.x & (G - 1);
+    const int* ip = IL ? row.p + l : row.p + (int64_t)l * st;
     int jn[L];
 #pragma unroll
-    for (int u = 0; u < L; u++) jn[u] = __builtin_nontemporal_load(ip + u * G * st);
+    for (int u = 0; u < L; u++) jn[u] = __builtin_nontemporal_load(ip + u * CS * st);
     for (int k = 0; k < n; k += B) {
-        ip += B * st;
+        ip += (B / G) * CS * st;
         Nb<MIX> q[L];
         int jq[L];
 #pragma unroll
@@ -411,7 +422,7 @@ __device__ __forceinline__ void for_row_split(const FluidArgs& a, Rsrc rs, int s
             jq[u] = jn[u];
         }
 #pragma unroll
-        for (int u = 0; u < L; u++) jn[u] = __builtin_nontemporal_load(ip + u * G * st);
+        for (int u = 0; u < L; u++) jn[u] = __builtin_nontemporal_load(ip + u * CS * st);
         split_batch<G, MIX, MODE, UNI>(a, q, jq, s, k, n, fn, std::make_integer_sequence<int, B / 4>{});
     }
 }
@@ -425,14 +436,22 @@ __device__ __forceinline__ void for_row(const FluidArgs& a, Rsrc rs, int s, Fn&&
     static_assert(U == 4, "the split walk takes batches of four");
     const Row row = nb_row(a.L, s);
     // walked rows are never shared: a key no other lane has
-    const int key = a.L.cnt[s] >= 0 ? -1 - s : a.L.grow[s];
+    const int key = a.L.cnt[s] >= 0 ? -1 - s : row_slot(a.L.grow[s]);
     const bool full = __builtin_amdgcn_read_exec() == ~0ull;
-    if (full && __ballot(key != bc<kQuadFrom<4, 0>>(key)) == 0) {
-        for_row_split<MIX, 4, kSplitBatch, MODE, UNI>(a, rs, s, row, fn);
-    } else if (full && __ballot(key != bc<kQuadFrom<2, 0>>(key)) == 0) {
-        for_row_split<MIX, 2, kSplitBatch, MODE, UNI>(a, rs, s, row, fn);
+    const bool q4 = full && __ballot(key != bc<kQuadFrom<4, 0>>(key)) == 0;
+    const bool q2 = full && __ballot(key != bc<kQuadFrom<2, 0>>(key)) == 0;
+    if (q4 && __ballot(row.lg != 2) == 0) {
+        for_row_split<MIX, 4, kSplitBatch, MODE, UNI, true>(a, rs, s, row, fn);
+    } else if (q2 && __ballot(row.lg != 1) == 0) {
+        for_row_split<MIX, 2, kSplitBatch, MODE, UNI, true>(a, rs, s, row, fn);
+    } else if (__ballot(row.lg != 0) != 0) {
+        for_row_each<MIX, U, MODE, UNI, true>(a, rs, s, row, fn);
+    } else if (q4) {
+        for_row_split<MIX, 4, kSplitBatch, MODE, UNI, false>(a, rs, s, row, fn);
+    } else if (q2) {
+        for_row_split<MIX, 2, kSplitBatch, MODE, UNI, false>(a, rs, s, row, fn);
     } else {
-        for_row_each<MIX, U, MODE, UNI>(a, rs, s, row, fn);
+        for_row_each<MIX, U, MODE, UNI, false>(a, rs, s, row, fn);
     }
 }
 

@@ -415,6 +415,11 @@ __global__ void __launch_bounds__(kBS) k_build(SearchArgs a, const int2* __restr
         const int wl = actm ? __ffsll((long long)actm) - 1 : lane;
         const bool writer = act && lane == wl;
         const int ws = __shfl(s, wl, 64);
+        // a complete group on consecutive slots interleaves its row over
+        // their storage (row_at)
+        const bool dense = G > 1 && actm == gmask &&
+                           (__ballot(act && s == ws + (lane - gl)) & gmask) == gmask;
+        const int gcode = dense ? ws | ((31 - __builtin_clz(G)) << kRowSlotBits) : ws;
         const int gxy = g.gx * g.gy;
         const int cz = lc / gxy, rem = lc - cz * gxy;
         const int cy = rem / g.gx, cx = rem - cy * g.gx;
@@ -482,7 +487,7 @@ __global__ void __launch_bounds__(kBS) k_build(SearchArgs a, const int2* __restr
                                     const bool in = act & (k + v < nb) & (jj[v] != s) &
                                                     (e2 <= (Ai > q[v].w ? Ai : q[v].w));
                                     const bool gin = (__ballot(in) & gmask) != 0;
-                                    if (writer && gin && c < supK) a.L.sup[(int64_t)c * a.nf + s] = jj[v];
+                                    if (writer && gin && c < supK) a.L.sup[row_at(gcode, c, a.nf)] = jj[v];
                                     c += gin;
                                 }
                             }
@@ -533,7 +538,7 @@ __global__ void __launch_bounds__(kBS) k_build(SearchArgs a, const int2* __restr
         if (act) {
             a.L.sup_cnt[s] = c;
             a.L.fsup_cnt[s] = cs;
-            a.L.grow[s] = ws;
+            a.L.grow[s] = gcode;
         }
         ovf = ovf || c > supK || cs > fsupK;
         cmax = max(cmax, c);
@@ -843,10 +848,10 @@ __global__ void __launch_bounds__(kBS) k_exact_rows(SearchArgs a, int* __restric
         c = cw;
     } else {
         const double4 p = a.fsp[s];
-        const int r = a.L.grow[s];  // the group's row
-        const int n = min(a.L.sup_cnt[r], a.L.supK);
+        const int g = a.L.grow[s];  // the group's row
+        const int n = min(a.L.sup_cnt[row_slot(g)], a.L.supK);
         for (int k = 0; k < n; k++) {
-            const int j = a.L.sup[(int64_t)k * a.nf + r];
+            const int j = a.L.sup[row_at(g, k, a.nf)];
             if (j == s) continue;
             const double4 q = a.fsp[j];
             if (within(p.x, p.y, p.z, q.x, q.y, q.z, 2.0 * fmax(p.w, q.w)))
