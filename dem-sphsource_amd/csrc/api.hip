@@ -600,14 +600,11 @@ void fluid_integrate(dsph_ctx* c, double dt, int i_step, int gap) {
     }
     if (i_step % gap == 0) {
         {
+            // with the half-step drift that follows it
             FamTimer t(c, F_CONT);
-            continuity(c->st, fluid_args(c), 0.5 * dtf, mx);
+            continuity(c->st, fluid_args(c), 0.5 * dtf, mx, 0.5 * dtf);
         }
         c->dcur ^= 1;
-        {
-            FamTimer t(c, F_FSTREAM);
-            fluid_drift(c->st, fluid_args(c), 0.5 * dtf, mx);
-        }
         {
             FamTimer t(c, F_MOM);
             momentum(c->st, fluid_args(c), mx, true);
@@ -618,7 +615,7 @@ void fluid_integrate(dsph_ctx* c, double dt, int i_step, int gap) {
         }
         {
             FamTimer t(c, F_CONT);
-            continuity(c->st, fluid_args(c), 0.5 * dtf, mx);
+            continuity(c->st, fluid_args(c), 0.5 * dtf, mx, 0.0);
         }
         c->dcur ^= 1;
     } else {

@@ -403,8 +403,8 @@ struct FluidArgs {
 };
 void fluid_records(hipStream_t s, const FluidArgs& a, int buf, bool mixed);
 void density_reinit(hipStream_t s, const FluidArgs& a, int i_step, bool mixed);
-void continuity(hipStream_t s, const FluidArgs& a, double dt, bool mixed);  // writes 1-cur
-void fluid_drift(hipStream_t s, const FluidArgs& a, double dt, bool mixed);
+// writes 1-cur; drift != 0 also moves each particle by drift v
+void continuity(hipStream_t s, const FluidArgs& a, double dt, bool mixed, double drift);
 void momentum(hipStream_t s, const FluidArgs& a, bool mixed, bool write_fs_force);
 void fluid_solid_force(hipStream_t s, const FluidArgs& a, bool mixed);
 void fluid_kick_drift(hipStream_t s, const FluidArgs& a, double dt, bool mixed);

@@ -580,7 +580,10 @@ __global__ void __launch_bounds__(kBS) k_records(FluidSet f, int n, int buf) {
 // (rho_i + rho_j) gbar = 2 nu rho_i rho_j.  A candidate counts when
 // r >= 1e-10 m (r_u^2 >= r2min) and t > 0 (r < 2h).
 template <bool MIX, bool UNI>
-__global__ void __launch_bounds__(kMaxBS) k_continuity(FluidArgs a, double dt) {
+// drift != 0: the fluid drift that follows the first continuity pass of a
+// step (fluidPositionIntegrate, integrate.cu:367-375), x += drift v, done
+// here for the slot's own position and written into its new records
+__global__ void __launch_bounds__(kMaxBS) k_continuity(FluidArgs a, double dt, double drift) {
     using R = typename Prec<MIX>::R;
     const int s = xcd_block() * blockDim.x + threadIdx.x;
     if (s >= a.nf) return;
@@ -640,7 +643,14 @@ __global__ void __launch_bounds__(kMaxBS) k_continuity(FluidArgs a, double dt) {
     }
     const double nd = f.drho[a.cur][s] + dt * acc.get();
     f.drho[1 - a.cur][s] = nd;
-    write_records<MIX>(f, 1 - a.cur, s, ps, vm.x, vm.y, vm.z, nd);
+    double4 p = ps;
+    if (drift != 0.0) {
+        p.x += drift * vm.x;
+        p.y += drift * vm.y;
+        p.z += drift * vm.z;
+        f.ph[s] = p;
+    }
+    write_records<MIX>(f, 1 - a.cur, s, p, vm.x, vm.y, vm.z, nd);
 }
 
 // --------------------------------------------------------------- momentum
@@ -821,19 +831,7 @@ __global__ void __launch_bounds__(kMaxBS) k_density_reinit(FluidArgs a, int i_st
 }
 
 // ------------------------------------------------------------ integration
-template <bool MIX>
-__global__ void k_drift(FluidSet f, int n, double dt, int cur) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    double4 p = f.ph[s];
-    const double4 v = f.vr[s];
-    p.x += dt * v.x;
-    p.y += dt * v.y;
-    p.z += dt * v.z;
-    f.ph[s] = p;
-    store_pos<MIX>(f, cur, s, p);
-}
-
+// (the drift after the first continuity pass is part of k_continuity)
 // v += dt a (fluidVelocityIntegrate) then x += dt/2 v (fluidPositionIntegrate)
 template <bool MIX>
 __global__ void k_kick_drift(FluidSet f, int n, double dt, int cur) {
@@ -891,13 +889,10 @@ void density_reinit(hipStream_t s, const FluidArgs& a, int i_step, bool mixed) {
             hipLaunchKernelGGL((kern<false, false>), g, b, 0, s, __VA_ARGS__);                    \
     } while (0)
 
-void continuity(hipStream_t s, const FluidArgs& a, double dt, bool mixed) {
-    if (a.nf > 0) DSPH_LAUNCH_UNI(k_continuity, a.nf, a.bs, s, a, dt);
+void continuity(hipStream_t s, const FluidArgs& a, double dt, bool mixed, double drift) {
+    if (a.nf > 0) DSPH_LAUNCH_UNI(k_continuity, a.nf, a.bs, s, a, dt, drift);
 }
 
-void fluid_drift(hipStream_t s, const FluidArgs& a, double dt, bool mixed) {
-    if (a.nf > 0) DSPH_LAUNCH(k_drift, a.nf, s, a.f, a.nf, dt, a.cur);
-}
 
 void momentum(hipStream_t s, const FluidArgs& a, bool mixed, bool write_fs_force) {
     if (a.nf > 0) DSPH_LAUNCH_UNI(k_momentum, a.nf, a.bs, s, a, (int)write_fs_force);
