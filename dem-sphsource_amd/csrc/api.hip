@@ -606,13 +606,11 @@ void fluid_integrate(dsph_ctx* c, double dt, int i_step, int gap) {
         }
         c->dcur ^= 1;
         {
+            // with the kick and drift that follow it, into the other buffer
             FamTimer t(c, F_MOM);
-            momentum(c->st, fluid_args(c), mx, true);
+            momentum(c->st, fluid_args(c), mx, true, dtf);
         }
-        {
-            FamTimer t(c, F_FSTREAM);
-            fluid_kick_drift(c->st, fluid_args(c), dtf, mx);
-        }
+        c->dcur ^= 1;
         {
             FamTimer t(c, F_CONT);
             continuity(c->st, fluid_args(c), 0.5 * dtf, mx, 0.0);

@@ -405,9 +405,9 @@ void fluid_records(hipStream_t s, const FluidArgs& a, int buf, bool mixed);
 void density_reinit(hipStream_t s, const FluidArgs& a, int i_step, bool mixed);
 // writes 1-cur; drift != 0 also moves each particle by drift v
 void continuity(hipStream_t s, const FluidArgs& a, double dt, bool mixed, double drift);
-void momentum(hipStream_t s, const FluidArgs& a, bool mixed, bool write_fs_force);
+// kick != 0 also kicks and drifts each particle into the other buffer (flip cur after)
+void momentum(hipStream_t s, const FluidArgs& a, bool mixed, bool write_fs_force, double kick);
 void fluid_solid_force(hipStream_t s, const FluidArgs& a, bool mixed);
-void fluid_kick_drift(hipStream_t s, const FluidArgs& a, double dt, bool mixed);
 
 // DEM (k_dem.hip)
 struct DemArgs {

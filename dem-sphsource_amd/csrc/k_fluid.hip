@@ -655,7 +655,12 @@ __global__ void __launch_bounds__(kMaxBS) k_continuity(FluidArgs a, double dt, d
 
 // --------------------------------------------------------------- momentum
 template <bool MIX, bool UNI>
-__global__ void __launch_bounds__(kMaxBS) k_momentum(FluidArgs a, int write_fs) {
+// kick != 0: the kick and drift that follow momentum (fluidVelocityIntegrate
+// and fluidPositionIntegrate, integrate.cu:355-375), v += kick a, then
+// x += kick/2 v, for the slot's own state; the new records go to the other
+// buffer with the slot's dRho (other slots still read this one), and the
+// caller flips the current buffer
+__global__ void __launch_bounds__(kMaxBS) k_momentum(FluidArgs a, int write_fs, double kick) {
     using R = typename Prec<MIX>::R;
     const int s = xcd_block() * blockDim.x + threadIdx.x;
     if (s >= a.nf) return;
@@ -750,7 +755,24 @@ __global__ void __launch_bounds__(kMaxBS) k_momentum(FluidArgs a, int write_fs) 
             if (write_fs && ok) a.L.fsF[k] = make_double4(mi * tx, mi * ty, mi * tz, 0.0);
         });
     }
-    f.acc[s] = make_double4(acc.x + a.gx, acc.y + a.gy, acc.z + a.gz, 0.0);
+    const double4 am = make_double4(acc.x + a.gx, acc.y + a.gy, acc.z + a.gz, 0.0);
+    f.acc[s] = am;
+    if (kick != 0.0) {
+        double4 v = f.vr[s];
+        v.x += kick * am.x;
+        v.y += kick * am.y;
+        v.z += kick * am.z;
+        f.vr[s] = v;
+        double4 p = ps;
+        const double hd = 0.5 * kick;
+        p.x += hd * v.x;
+        p.y += hd * v.y;
+        p.z += hd * v.z;
+        f.ph[s] = p;
+        const double d = f.drho[a.cur][s];
+        f.drho[1 - a.cur][s] = d;
+        write_records<MIX>(f, 1 - a.cur, s, p, v.x, v.y, v.z, d);
+    }
 }
 
 // calFluid2SolidForce (integrate.cu:282-330) on steps without a fluid update
@@ -830,33 +852,6 @@ __global__ void __launch_bounds__(kMaxBS) k_density_reinit(FluidArgs a, int i_st
     }
 }
 
-// ------------------------------------------------------------ integration
-// (the drift after the first continuity pass is part of k_continuity)
-// v += dt a (fluidVelocityIntegrate) then x += dt/2 v (fluidPositionIntegrate)
-template <bool MIX>
-__global__ void k_kick_drift(FluidSet f, int n, double dt, int cur) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    double4 v = f.vr[s];
-    const double4 ac = f.acc[s];
-    v.x += dt * ac.x;
-    v.y += dt * ac.y;
-    v.z += dt * ac.z;
-    f.vr[s] = v;
-    double4 p = f.ph[s];
-    const double hd = 0.5 * dt;
-    p.x += hd * v.x;
-    p.y += hd * v.y;
-    p.z += hd * v.z;
-    f.ph[s] = p;
-    using R = typename Prec<MIX>::R;
-    auto* st = state_w<MIX>(f, cur, s);
-    st->x = (R)v.x;
-    st->y = (R)v.y;
-    st->z = (R)v.z;
-    store_pos<MIX>(f, cur, s, p);
-}
-
 }  // namespace
 
 #define DSPH_LAUNCH_BS(kern, n, bs, s, ...)                                                 \
@@ -894,16 +889,13 @@ void continuity(hipStream_t s, const FluidArgs& a, double dt, bool mixed, double
 }
 
 
-void momentum(hipStream_t s, const FluidArgs& a, bool mixed, bool write_fs_force) {
-    if (a.nf > 0) DSPH_LAUNCH_UNI(k_momentum, a.nf, a.bs, s, a, (int)write_fs_force);
+void momentum(hipStream_t s, const FluidArgs& a, bool mixed, bool write_fs_force, double kick) {
+    if (a.nf > 0) DSPH_LAUNCH_UNI(k_momentum, a.nf, a.bs, s, a, (int)write_fs_force, kick);
 }
 
 void fluid_solid_force(hipStream_t s, const FluidArgs& a, bool mixed) {
     if (a.nf > 0) DSPH_LAUNCH_BS(k_fs_force, a.nf, a.bs, s, a);
 }
 
-void fluid_kick_drift(hipStream_t s, const FluidArgs& a, double dt, bool mixed) {
-    if (a.nf > 0) DSPH_LAUNCH(k_kick_drift, a.nf, s, a.f, a.nf, dt, a.cur);
-}
 
 }  // namespace dsph
