@@ -187,7 +187,7 @@ def tune(work, precision, flags, args):
     if args.no_autotune:
         return None
     import dsph
-    cfg, trials = dsph.autotune(work, precision, flags, trial_steps=10, warmup=3)
+    cfg, trials = dsph.autotune(work, precision, flags, trial_steps=30, warmup=5)
     print(f"# autotuned ({'mixed' if precision == dsph.PREC_MIXED else 'fp64'}): {cfg}",
           file=sys.stderr, flush=True)
     return cfg
