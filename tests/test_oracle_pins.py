@@ -434,6 +434,181 @@ def test_two_particle_symmetric_compression():
     assert f["drho"][1] == pytest.approx(f["drho"][0], rel=1e-12)
 
 
+# ------------------------------------------ the Riemann interface states
+# The pair terms of continuity and momentum (integrate.cu:100-101, 211-212)
+# are the Riemann-based WCSPH of Zhang, Hu & Adams (J. Comput. Phys. 340,
+# 2017): along the axis s = -e_ij from particle i (left state) to j (right
+# state) each pair solves the linearised (acoustic) Riemann problem with
+# impedance Z = rho c (c the pair's mean sound speed).  Its star state
+# follows from the two characteristic relations across the acoustic waves,
+#     p* + Z_L u* = p_L + Z_L u_L      (the left-going wave)
+#     p* - Z_R u* = p_R - Z_R u_R      (the right-going wave),
+# solved below as a 2x2 linear system (not a closed form).  Then
+#   continuity  drho_i/dt = 2 rho_i V_j (v_i - v*) . grad W_ij with
+#               v* = mean (rho-weighted) velocity with its s-component
+#               replaced by u* (the wave speed is the full c here);
+#   momentum    dv_i/dt = -2 (V_j / rho_i) p* grad W_ij (+ viscous), with
+#               p* solved at the low-dissipation wave speed
+#               beta = min(3 max(u_L - u_R, 0), c): the full acoustic
+#               p* in strong compression, the impedance-weighted pressure
+#               with no velocity term when the pair separates.
+def _acoustic_star(uL, uR, pL, pR, rhoL, rhoR, w):
+    """Star state (u*, p*) of the acoustic Riemann problem with impedances
+    Z = rho w.  The unknowns are p* and the momentum-flux jump w u*, so the
+    system stays regular as the wave speed w goes to 0 (u* is then
+    undetermined and returned as nan)."""
+    A = np.array([[1.0, rhoL], [1.0, -rhoR]])
+    b = np.array([pL + rhoL * w * uL, pR - rhoR * w * uR])
+    p, wu = np.linalg.solve(A, b)
+    return (wu / w if w > 0 else float("nan")), p
+
+
+def _riemann_pair(ri, rj, vi, vj, hi, hj, rho0i, rho0j, dri, drj, c, nu):
+    """A two-particle scene (no gravity, no walls) and the rates the
+    Riemann WCSPH assigns to particle 0 from the acoustic star state."""
+    sc = scenes.empty_scene((-0.2, -0.2, -0.2), (0.4, 0.4, 0.4))
+    sc.gravity = np.zeros(3)
+    scenes.add_fluid(sc, [ri], hi, rho0i, c, nu)
+    scenes.add_fluid(sc, [rj], hj, rho0j, c, nu)
+    sc.fluid_vel = np.array([vi, vj], float)
+    sc.fluid_drho = np.array([dri, drj], float)
+    ri, rj, vi, vj = (np.asarray(x, float) for x in (ri, rj, vi, vj))
+    rhoi, rhoj = rho0i + dri, rho0j + drj
+    Pi, Pj = c * c * dri, c * c * drj
+    Vj = (hj / 1.3) ** 3 * rho0j / rhoj
+    rij = ri - rj
+    s = -rij / np.linalg.norm(rij)
+    gW = oracle.grad_wendland(rij, 0.5 * (hi + hj))
+    uL, uR = float(vi @ s), float(vj @ s)
+    # continuity: the full acoustic wave speed
+    ustar, _ = _acoustic_star(uL, uR, Pi, Pj, rhoi, rhoj, c)
+    vbar = (rhoi * vi + rhoj * vj) / (rhoi + rhoj)
+    vstar = vbar - (vbar @ s) * s + ustar * s
+    drdt = 2 * rhoi * Vj * float((vi - vstar) @ gW)
+    # momentum: the low-dissipation wave speed
+    beta = min(3.0 * max(uL - uR, 0.0), c)
+    _, pstar = _acoustic_star(uL, uR, Pi, Pj, rhoi, rhoj, beta)
+    acc = -2.0 * Vj / rhoi * pstar * gW
+    return sc, drdt, acc, beta
+
+
+def _rates(sc, dt=1e-13):
+    """d rho/dt and the acceleration of every particle from one fluid step of
+    length dt: both continuity half steps and the momentum pass see the
+    initial state to O(dt)."""
+    o = oracle.Oracle()
+    o.upload(sc)
+    o.initialize(compute_normals=False)
+    o.fluid_integrate(dt, 2, 1)      # no density re-init on step 2
+    f = o.fluid()
+    o.close()
+    return (f["drho"] - np.asarray(sc.fluid_drho)) / dt, f["acc"]
+
+
+# Unequal densities (different rho0 and dRho), pressures, smoothing lengths
+# and approach speeds, with velocity components off the pair axis: strong
+# compression (beta = c), weak compression (0 < beta < c) and separation
+# (beta = 0).  nu is tiny: the viscous term (pinned separately) is
+# negligible here.
+_RIEMANN_CASES = {
+    # name: (r_j - r_i direction, v_i, v_j, h_i/dx, h_j/dx, rho0_i, rho0_j, dRho_i, dRho_j)
+    "strong_compression": ((1.0, 0.3, -0.2), (9.0, 1.0, -2.0), (-4.0, 0.5, 1.5),
+                           1.3, 1.2, 1000.0, 820.0, 3.5, -1.25),
+    "weak_compression": ((0.2, -1.0, 0.4), (0.3, -1.1, 0.7), (0.1, 0.9, -0.2),
+                         1.25, 1.35, 950.0, 1100.0, -2.0, 4.0),
+    "separation": ((-0.5, 0.1, 1.0), (0.4, -0.2, -0.9), (-0.3, 0.6, 1.1),
+                   1.3, 1.3, 1000.0, 1000.0, 1.75, -0.5),
+}
+
+
+@pytest.mark.parametrize("case", list(_RIEMANN_CASES))
+def test_riemann_pair_terms_match_the_acoustic_star_state(case):
+    d, vi, vj, hi, hj, rho0i, rho0j, dri, drj = _RIEMANN_CASES[case]
+    dx, c = 0.01, 20.0
+    ri = np.zeros(3)
+    rj = 1.1 * dx * _unit(d)
+    sc, drdt, acc, beta = _riemann_pair(ri, rj, vi, vj, hi * dx, hj * dx, rho0i, rho0j,
+                                        dri, drj, c, 1e-14)
+    {"strong_compression": lambda: beta == c,
+     "weak_compression": lambda: 0 < beta < c,
+     "separation": lambda: beta == 0.0}[case]() or pytest.fail(f"regime of {case}: beta={beta}")
+    rate, a = _rates(sc)
+    assert abs(drdt) > 1.0 and np.linalg.norm(acc) > 1.0
+    assert rate[0] == pytest.approx(drdt, rel=1e-6)
+    assert np.abs(a[0] - acc).max() < 1e-7 * np.linalg.norm(acc)
+
+
+def test_riemann_terms_swap_left_and_right():
+    # particle 1 sees the same pair with left and right exchanged: its
+    # rates are the acoustic star state of the mirrored problem
+    d, vi, vj, hi, hj, rho0i, rho0j, dri, drj = _RIEMANN_CASES["strong_compression"]
+    dx, c = 0.01, 20.0
+    rj = 1.1 * dx * _unit(d)
+    sc, _, _, _ = _riemann_pair(np.zeros(3), rj, vi, vj, hi * dx, hj * dx, rho0i, rho0j,
+                                dri, drj, c, 1e-14)
+    _, drdt1, acc1, _ = _riemann_pair(rj, np.zeros(3), vj, vi, hj * dx, hi * dx, rho0j, rho0i,
+                                      drj, dri, c, 1e-14)
+    rate, a = _rates(sc)
+    assert rate[1] == pytest.approx(drdt1, rel=1e-6)
+    assert np.abs(a[1] - acc1).max() < 1e-7 * np.linalg.norm(acc1)
+
+
+def test_viscous_term_is_the_laplacian_of_a_quadratic_shear():
+    # v_x = k z^2 in a lattice at rest (no pressure, no gravity): the viscous
+    # term approximates nu lap v = 2 nu k in the interior.  The velocities
+    # are small so the Riemann dissipation (quadratic in them) is negligible.
+    sc, n, dx = _periodic_free_lattice(n=16, dx=0.01)
+    nu, k = 1e-2, 1e-4
+    p = np.asarray(sc.fluid_pos)
+    z0 = 0.5 * n * dx
+    vel = np.zeros_like(p)
+    vel[:, 0] = k * (p[:, 2] - z0) ** 2
+    sc.fluid_vel = vel
+    sc.fluid_nu = np.full(sc.n_fluid, nu)
+    o = _oracle_for(sc)
+    o.fluid_integrate(1e-9, 2, 1)
+    a = o.fluid()["acc"]
+    lo, hi = 3 * dx, (n - 3) * dx
+    inner = np.all((p > lo) & (p < hi), axis=1)
+    assert inner.sum() > 500
+    ax = a[inner, 0]
+    assert np.mean(ax) == pytest.approx(2 * nu * k, rel=0.05)
+    assert np.std(ax) < 0.05 * 2 * nu * k
+    assert np.abs(a[inner, 1:]).max() < 1e-3 * 2 * nu * k
+
+
+def test_hydrostatic_column_settles_to_rho_g_depth():
+    # a water column at rest in a walled tank, released with no pressure:
+    # once the release has spread out, the time-averaged pressure c^2 dRho
+    # of each interior layer is rho0 g (depth below the free surface), to
+    # the SPH truncation error of the free surface (a few %)
+    dx, Hc = 0.01, 0.1
+    sc = scenes.dam_break(dx=dx, L=0.1, W=0.1, H=0.15, column=(0.1, 0.1, Hc), name="hydrostatic")
+    sc.fluid_nu = np.full(sc.n_fluid, 2e-3)
+    o = _oracle_for(sc)
+    c = float(sc.fluid_c[0])
+    o.run(600, sc.dt)
+    layers = (0.02, 0.04, 0.06, 0.08)
+    ratio = {lo: [] for lo in layers}
+    step = 601
+    for _ in range(40):               # 800 more steps, about 26 acoustic crossings
+        o.run(20, sc.dt, first=step)
+        step += 20
+        f = o.fluid()
+        P = c * c * f["drho"]
+        z = f["pos"][:, 2]
+        depth = np.max(z) + 0.5 * dx - z
+        xy = f["pos"][:, :2]
+        core = np.all((xy > 2 * dx) & (xy < 0.1 - 2 * dx), axis=1)
+        for lo in layers:
+            sel = core & (depth > lo) & (depth < lo + dx)
+            ratio[lo].append(np.mean(P[sel]) / (1000.0 * 9.81 * np.mean(depth[sel])))
+    o.close()
+    mean = {lo: float(np.mean(v)) for lo, v in ratio.items()}
+    for lo in layers:
+        assert mean[lo] == pytest.approx(1.0, abs=0.12), mean
+
+
 # ----------------------------------------------------- DEM integration
 def test_sphere_resting_on_plane_static_overlap():
     E, G = 5.4945e6, 1.131e6
