@@ -417,3 +417,161 @@ def ensemble(n_scenes=256, **kw) -> list:
     """The batched dam-break ensemble: n_scenes independent scenes (see
     ensemble_scene); run together they step with the smallest dt."""
     return [ensemble_scene(k, **kw) for k in range(n_scenes)]
+
+
+# --------------------------------------------------------------------------
+# the reference's own case driver (driver/cases/damBreak.cpp:14-58)
+# --------------------------------------------------------------------------
+def dam_break_case(dx=0.025, tank=(8.0, 0.7, 0.8), fluid=(3.5, 0.7, 0.4),
+                   clump_origin=(5.2, 0.275, 0.0), cube=0.15, E=0.3e9, cube_density=800.0,
+                   floor_point=None, name="dam_break_case") -> Scene:
+    """damBreak.cpp as written upstream, with its sizes as parameters (the
+    defaults are the reference's): a tank of three wall-dummy layers, a
+    water column in the corner (c = 30 m/s, nu = 1e-6), an infinite floor
+    plane of material 0, one clump of (cube / dx)^3 touching pebbles of
+    material 1 (the 0.15 m cube of 216 pebbles upstream) with the box's
+    inertia tensor, Hertz-Mindlin for material pairs (0,1) and (1,1), the
+    DEM time step 0.2 sqrt(m / k) and the fluid sub-cycling
+    gap = int(dt_f / dt) with dt_f = 0.6 h / (22 sqrt(0.4 g))."""
+    L, W, H = tank
+    o = np.array([-3 * dx] * 3)
+    size = np.array([L + 6 * dx, W + 6 * dx, H + 6 * dx])
+    sc = empty_scene(o, size, name=name)
+    h = 1.3 * dx
+    add_fluid(sc, regular_packed_points((0, 0, 0), fluid, dx), h, 1000.0, 30.0, 1.0e-6)
+    sp0 = regular_packed_points(o, size, dx)
+    keep = (sp0[:, 0] < 0) | (sp0[:, 1] < 0) | (sp0[:, 2] < 0) | (sp0[:, 0] > L) | (sp0[:, 1] > W)
+    r = 0.5 * dx
+    add_dummy_wall(sc, sp0[keep], r)
+    add_infinite_wall(sc, floor_point if floor_point is not None else (0.5 * L, 0.5 * W, 0.0),
+                      (0, 0, 1), 0)
+    sc.set_hertz(0, 1, E, 0.3 * E, 0.9, 0, 0, 0.35, 0, 0)
+    sc.set_hertz(1, 1, 0.5 * E, 0.15 * E, 0.9, 0, 0, 0.45, 0, 0)
+    pts = regular_packed_points(clump_origin, (cube, cube, cube), 2 * r)
+    mass = cube_density * cube ** 3
+    inertia = (mass / 12.0 * (cube * cube + cube * cube),) * 3 + (0.0, 0.0, 0.0)
+    centroid = np.asarray(clump_origin, float) + 0.5 * cube
+    add_clump(sc, pts, np.full(len(pts), r), centroid, mass, inertia, material=1)
+    k = 0.5 * E * math.pi * r
+    mij = 8 * r ** 3 * cube_density
+    sc.dt = 0.2 * math.sqrt(mij / k)
+    dt_f = 0.6 * 1.3 * dx / (22.0 * math.sqrt(0.4 * 9.8))
+    sc.gap = max(1, int(dt_f / sc.dt))
+    return sc
+
+
+def dam_break_case_small(name="dam_break_case_small") -> Scene:
+    """damBreak.cpp at its own particle spacing, pebble size, materials, time
+    step and sub-cycling (gap = 48), in a 1.0 x 0.35 x 0.4 m tank with a
+    0.3 m column and the 216-pebble clump 0.035 m from it, so the fluid
+    and the clump interact from the first step."""
+    return dam_break_case(tank=(1.0, 0.35, 0.4), fluid=(0.3, 0.35, 0.3),
+                          clump_origin=(0.31, 0.1, 0.0), name=name)
+
+
+def immersed_spheres(n_box=(12, 10, 10), dx=0.005, stride=3, r_frac=0.5, rho_s=2500.0,
+                     seed_stream=51, name="immersed_spheres") -> Scene:
+    """DEM spheres inside a fluid block: every stride-th lattice site of the
+    block's interior holds a sphere of radius r_frac dx instead of a fluid
+    particle, so each sphere has fluid within 2h on all sides.  Walls of
+    dummies around, tank planes for the spheres, Hertz-Mindlin contact."""
+    L, W, Hb = (n * dx for n in n_box)
+    o = np.array([-3 * dx] * 3)
+    size = np.array([L + 6 * dx, W + 6 * dx, 2 * Hb + 6 * dx])
+    sc = empty_scene(o, size, name=name)
+    pts = regular_packed_points((0, 0, 0), (L, W, Hb), dx)
+    ijk = np.rint(pts / dx - 0.5).astype(int)
+    inner = np.all((ijk >= 2) & (ijk < np.array(n_box) - 2), axis=1)
+    dem = inner & np.all(ijk % stride == 1, axis=1)
+    c0 = sound_speed(Hb)
+    add_fluid(sc, pts[~dem], 1.3 * dx, RHO0, c0, NU)
+    m = sc.n_fluid
+    sc.fluid_vel = uniform(ROOT_SEED, seed_stream, 3 * m, -0.05, 0.05).reshape(m, 3)
+    sc.fluid_drho = uniform(ROOT_SEED, seed_stream + 1, m, -0.5, 0.5)
+    sp0 = regular_packed_points(o, size, dx)
+    keep = (sp0[:, 0] < 0) | (sp0[:, 1] < 0) | (sp0[:, 2] < 0) | (sp0[:, 0] > L) | (sp0[:, 1] > W)
+    add_dummy_wall(sc, sp0[keep], 0.5 * dx)
+    r = r_frac * dx
+    add_solid(sc, pts[dem], r, rho_s, material=0)
+    Es, Gs = hertz_effective(1.0e7, 0.3)
+    sc.set_hertz(0, 0, Es, Gs, 0.5, 0.0, 0.0, 0.4, 0.0, 0.0)
+    add_tank_planes(sc, L, W)
+    k = 0.5 * 1.0e7 * math.pi * r
+    sc.dt = min(0.25 * 1.3 * dx / c0, 0.2 * math.sqrt(8 * r ** 3 * rho_s / k))
+    sc.gap = 1
+    return sc
+
+
+def dem_contact_mix(n_side=6, n_clump_side=3, r=0.002, rho_s=2500.0, overlap=0.02, seed_stream=61,
+                    name="dem_contact_mix") -> Scene:
+    """A DEM scene with every spring of the contact laws loaded from the
+    first step: a block of n_side^3 free spheres on a lattice that overlaps
+    neighbours by `overlap` of a diameter, with seeded random velocities,
+    spins and materials (about 30% material 1, the rest 0), resting on a
+    floor plane (material 1); beside it n_clump_side^3 tetrahedral 4-pebble
+    clumps (material 1) packed so neighbouring clumps' pebbles overlap by
+    the same fraction.  Material pairs: (0,0) Hertz-Mindlin with rolling and
+    torsion springs (kr/ks 0.5, kt/ks 0.3, mu_r 0.1, mu_t 0.05), (0,1)
+    Hertz-Mindlin, (1,1) the linear spring-dashpot law (LinearContact)."""
+    d = 2 * r * (1.0 - overlap)
+    span = max(n_side, 2 * n_clump_side) * d * 2
+    sc = empty_scene((-0.01, -0.01, -0.01), (span + 0.03, n_side * d + 0.02, span + 0.03), name=name)
+    ii, jj, kk = np.meshgrid(np.arange(n_side), np.arange(n_side), np.arange(n_side), indexing="ij")
+    pos = np.stack([ii.ravel() * d + r, jj.ravel() * d + r, kk.ravel() * d + r * (1 - overlap)], axis=1)
+    m = len(pos)
+    add_solid(sc, pos, r, rho_s, material=0)
+    sc.solid_material = (uniform(ROOT_SEED, seed_stream + 2, m) < 0.3).astype(np.int32)
+    sc.solid_vel = uniform(ROOT_SEED, seed_stream, 3 * m, -0.05, 0.05).reshape(m, 3)
+    sc.solid_angvel = uniform(ROOT_SEED, seed_stream + 1, 3 * m, -50.0, 50.0).reshape(m, 3)
+    # clumps: regular tetrahedra of touching pebbles of radius r, on a
+    # lattice whose pitch makes neighbouring clumps' pebbles overlap
+    edge = 2 * r
+    # the nearest pebbles of clumps adjacent along an axis are sqrt((p -
+    # sqrt(2) r)^2 + 2 r^2) apart at pitch p: set that to 2 r (1 - overlap)
+    pitch = r * (math.sqrt(2.0) + math.sqrt(4.0 * (1.0 - overlap) ** 2 - 2.0))
+    x0 = n_side * d + 2 * r
+    z0 = r * (math.sqrt(2.0) / 2.0 + 1.0 - overlap)
+    m_one = 4.0 / 3.0 * math.pi * r ** 3 * rho_s
+    d2 = float(np.sum(tetra_offsets(edge)[0] ** 2))
+    Ii = 4 * (0.4 * m_one * r * r) + 4 * m_one * d2 * 2.0 / 3.0
+    for c in range(n_clump_side ** 3):
+        a, b, e = c % n_clump_side, (c // n_clump_side) % n_clump_side, c // n_clump_side ** 2
+        cen = np.array([x0 + pitch * a + r, r + pitch * b, z0 + pitch * e])
+        add_clump(sc, cen + tetra_offsets(edge), np.full(4, r), cen, 4 * m_one, (Ii, Ii, Ii, 0, 0, 0),
+                  material=1)
+    add_infinite_wall(sc, (0, 0, 0), (0, 0, 1), 1)
+    Es, Gs = hertz_effective(1.0e7, 0.3)
+    sc.set_hertz(0, 0, Es, Gs, 0.5, 0.5, 0.3, 0.4, 0.1, 0.05)
+    sc.set_hertz(0, 1, Es, Gs, 0.6, 0.4, 0.2, 0.3, 0.08, 0.04)
+    # LinearContact (SPHKernelDEMContactModel.h:214-268): k_n, k_s, k_r, k_t,
+    # d_n, d_s, d_r, d_t, mu_s, mu_r, mu_t
+    kn = 2.0e3
+    sc.set_linear(1, 1, kn, 0.5 * kn, 0.1 * kn, 0.05 * kn, 0.02, 0.01, 0.005, 0.005, 0.35, 0.1, 0.05)
+    sc.dt = 1.0e-6
+    sc.gap = 1
+    return sc
+
+
+def bonded_bar(n=(10, 3, 3), r=0.002, rho_s=2500.0, pull=0.5, k_bond=1.0e8, sigma_t=1.0e6,
+               name="bonded_bar") -> Scene:
+    """A bar of touching bonded spheres (one cluster) whose halves move apart
+    at +-pull m/s along x, no gravity: the bonds across the middle load in
+    tension past sigma_t and break."""
+    nx, ny, nz = n
+    sc = empty_scene((-0.05, -0.05, -0.05), (0.1 + 2 * nx * r, 0.1, 0.1), name=name)
+    sc.gravity = np.zeros(3)
+    idx = np.arange(nx * ny * nz)
+    ix, iy, iz = idx % nx, (idx // nx) % ny, idx // (nx * ny)
+    pos = np.stack([2 * r * ix, 2 * r * iy, 2 * r * iz], axis=1).astype(float)
+    vel = np.zeros_like(pos)
+    vel[:, 0] = np.where(ix < nx // 2, -pull, pull)
+    add_cluster(sc, pos, r, rho_s, material=0)
+    sc.solid_vel = vel
+    Es, Gs = hertz_effective(1.0e7, 0.3)
+    sc.set_hertz(0, 0, Es, Gs, 0.5, 0.0, 0.0, 0.4, 0.0, 0.0)
+    Eb = k_bond * (2 * r) / (math.pi * r * r)
+    sc.set_bonded(0, 0, Eb, 1.0, 1.0, sigma_t, 0.0, 0.0)
+    m = 4.0 / 3.0 * math.pi * r ** 3 * rho_s
+    sc.dt = 0.2 * 2.0 / math.sqrt(k_bond / m) / 2.0
+    sc.gap = 1
+    return sc

@@ -1147,6 +1147,31 @@ int orc_solid_after_contact(orc_ctx* o, double dt) {
     return 0;
 }
 
+/* dataBase::addExternalForce (dataBase.cpp:179-195): a force on one solid
+ * between the contact phase and solidIntegrateAfterContact (the user hook
+ * of solverBase::update), as an acceleration increment F / m */
+int orc_add_external_force(orc_ctx* o, int32_t i, const double f[3]) {
+    if (i < 0 || i >= o->ns) return -1;
+    o->sacc[i] += mk(f[0], f[1], f[2]) * o->sinvm[i];
+    return 0;
+}
+
+/* dataBase::addGlobalDamping (dataBase.cpp:197-224): the solid's force and
+ * torque each reduced by C_d times their magnitude along its velocity and
+ * angular velocity (skipped for fixed solids and slower than 1e-10) */
+int orc_add_global_damping(orc_ctx* o, int32_t i, double cd) {
+    if (i < 0 || i >= o->ns) return -1;
+    const double im = o->sinvm[i];
+    if (im < 1.e-10) return 0;
+    V3 F = o->sacc[i] / im, T = o->storq[i];
+    const V3 v = o->svel[i], w = o->sw[i];
+    if (len(v) > 1.e-10) F += -cd * len(F) * unit(v);
+    if (len(w) > 1.e-10) T += -cd * len(T) * unit(w);
+    o->sacc[i] = F * im;
+    o->storq[i] = T;
+    return 0;
+}
+
 /* solverBase::update (solverBase.cpp:47-59) without hook and output */
 int orc_step(orc_ctx* o, int32_t i_step, double dt, int32_t gap) {
     orc_neighbor_search(o, i_step, gap);

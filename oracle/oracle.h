@@ -33,6 +33,10 @@ int      orc_fluid_integrate(orc_ctx* o, double dt, int32_t i_step, int32_t gap)
 int      orc_contact_forces(orc_ctx* o, double dt);
 int      orc_solid_after_contact(orc_ctx* o, double dt);
 int      orc_step(orc_ctx* o, int32_t i_step, double dt, int32_t gap);
+/* runtime mutation between the contact phase and solid_after_contact */
+int      orc_add_external_force(orc_ctx* o, int32_t solid_index,
+                                const double force[3]);
+int      orc_add_global_damping(orc_ctx* o, int32_t solid_index, double c_d);
 
 int orc_download_fluid(orc_ctx* o, double* pos, double* vel, double* acc,
                        double* drho, double* sumw0);

@@ -37,6 +37,7 @@ def _load():
         ("orc_fluid_integrate", [C.c_double, C.c_int32, C.c_int32]),
         ("orc_contact_forces", [C.c_double]), ("orc_solid_after_contact", [C.c_double]),
         ("orc_step", [C.c_int32, C.c_double, C.c_int32]),
+        ("orc_add_external_force", [C.c_int32, _dp]), ("orc_add_global_damping", [C.c_int32, C.c_double]),
         ("orc_download_fluid", [_dp] * 5), ("orc_download_solid", [_dp] * 6),
         ("orc_download_clump", [_dp] * 6),
         ("orc_download_fluid_solid_force", [_dp, C.c_int64, _i64p]),
@@ -135,6 +136,12 @@ class Oracle:
 
     def step(self, i_step, dt, gap=1):
         self.L.orc_step(self.h, i_step, dt, gap)
+
+    def add_external_force(self, idx, f):
+        self._ok(self.L.orc_add_external_force(self.h, idx, _p(vec(f))), "add_external_force")
+
+    def add_global_damping(self, idx, cd):
+        self._ok(self.L.orc_add_global_damping(self.h, idx, float(cd)), "add_global_damping")
 
     def run(self, n_steps, dt, gap=1, first=1):
         for s in range(first, first + n_steps):

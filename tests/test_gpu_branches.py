@@ -1,0 +1,252 @@
+"""Parity of the hot-path branches the dam-break and granular workloads leave
+idle, HIP path (through the C ABI) against the FP64 CPU oracle:
+
+- fluid sub-cycling, gap = int(dt_f / dt) > 1 (fluidIntegrate,
+  integrate.cu:385-421): the fluid→solid force on the DEM-only steps, the
+  density re-initialisation of step 1 on a non-fluid step, and the
+  reference's own case (driver/cases/damBreak.cpp:14-58: a 216-pebble clump
+  with its inertia tensor, two materials, an infinite floor);
+- DEM spheres immersed in fluid (fluid within 2h of every sphere);
+- rolling and torsion springs, the linear contact law, material pairs;
+- a parallel bond loaded past its tensile strength;
+- addExternalForce / addGlobalDamping (dataBase.cpp:179-224) between the
+  contact phase and the after-contact integration.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import dsph
+import oracle
+import refmodel
+import scenes
+
+pytestmark = pytest.mark.gpu
+
+TOL_FP64 = 1e-10
+TOL_MIXED = 1e-5
+
+
+def note(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def pair(sc, prec, flags=0):
+    g = dsph.Solver(prec)
+    g.set_flags(flags)
+    g.upload(sc)
+    g.initialize()
+    o = oracle.Oracle()
+    o.set_flags(flags)
+    o.upload(sc)
+    o.initialize()
+    return g, o
+
+
+def rel(a, b, floor=1e-300):
+    return refmodel.rel_err(np.asarray(a), np.asarray(b), floor=floor)
+
+
+# ------------------------------------------------------- gap > 1 sub-cycling
+CASE = scenes.dam_break_case_small()
+
+
+def test_case_scene_is_the_reference_case():
+    assert CASE.gap == 48                       # int(dt_f / dt) of damBreak.cpp
+    assert CASE.n_clump == 1
+    assert int(CASE.clump_pebble_end[0] - CASE.clump_pebble_start[0]) == 216
+    assert set(np.unique(CASE.solid_material[CASE.solid_dummy == 0])) == {1}
+    assert CASE.n_wall == 1
+
+
+@pytest.mark.parametrize("prec", [dsph.PREC_FP64, dsph.PREC_MIXED])
+def test_gap_subcycling_one_fluid_step_and_dem_steps(prec):
+    note(f"dam break case gap={CASE.gap} prec={prec}")
+    sc = CASE
+    g, o = pair(sc, prec)
+    tol = TOL_FP64 if prec == dsph.PREC_FP64 else TOL_MIXED
+    # step 1 is a DEM-only step with the density re-initialisation (sumW0);
+    # steps 2..47 DEM-only with the fluid->solid force; 48 a fluid step
+    for s in range(1, 2 * sc.gap + 2):
+        g.step(s, sc.dt, sc.gap)
+        o.step(s, sc.dt, sc.gap)
+        if s in (1, 2, sc.gap, sc.gap + 1):
+            fa, fb = g.fs_force(), o.fs_force()
+            assert fa.shape == fb.shape and fa.size > 0, s
+            assert rel(fa, fb) < tol, (s, rel(fa, fb))
+    gf, of = g.fluid(), o.fluid()
+    assert rel(gf["sumw0"], of["sumw0"]) < tol
+    for k in ("pos", "vel", "drho"):
+        assert rel(gf[k], of[k], 1e-3 if k != "pos" else 0.0) < 20 * tol, k
+    gc, oc = g.clump(), o.clump()
+    for k in ("centroid", "vel", "angvel"):
+        assert rel(gc[k], oc[k], 1e-9) < (1e-8 if prec == dsph.PREC_FP64 else 1e-4), k
+    # the clump moved: fluid pressure and the floor act on it
+    assert np.linalg.norm(gc["vel"][0]) > 1e-5
+
+
+def test_gap_subcycling_graph_replay_equals_eager():
+    sc = CASE
+    runs = []
+    for graphs in (False, True):
+        g = dsph.Solver(dsph.PREC_MIXED)
+        g.upload(sc)
+        g.initialize()
+        g.run(3 * sc.gap + 5, sc.dt, sc.gap, graphs=graphs)
+        f, s, c = g.fluid(), g.solid(), g.clump()
+        runs.append((f, s, c))
+        g.close()
+    for a, b in zip(runs[0], runs[1]):
+        for k in a:
+            assert np.array_equal(a[k], b[k]), k
+
+
+def test_gap_subcycling_integrals_over_five_fluid_steps():
+    # 250 DEM steps: five fluid steps, one of them (240) with the density
+    # re-initialisation, mixed precision against the FP64 oracle
+    sc = CASE
+    n = 250
+    g, o = pair(sc, dsph.PREC_MIXED)
+    g.run(n, sc.dt, sc.gap, graphs=True)
+    o.run(n, sc.dt, sc.gap)
+    gf, of = g.fluid(), o.fluid()
+    m = (np.asarray(sc.fluid_h) / 1.3) ** 3 * np.asarray(sc.fluid_rho0)
+    for key, fn in (("ke", lambda f: 0.5 * np.sum(m * np.sum(f["vel"] ** 2, axis=1))),
+                    ("mean_drho", lambda f: np.mean(np.abs(f["drho"]))),
+                    ("front", lambda f: np.max(f["pos"][:, 0]))):
+        a, b = fn(gf), fn(of)
+        assert abs(a - b) <= 0.01 * abs(b), (key, a, b)
+    gc, oc = g.clump(), o.clump()
+    assert rel(gc["centroid"], oc["centroid"]) < 1e-6
+    assert rel(gc["vel"], oc["vel"], 1e-6) < 0.01
+
+
+# ------------------------------------------------------------ immersed DEM
+IMMERSED = scenes.immersed_spheres()
+
+
+@pytest.mark.parametrize("prec", [dsph.PREC_FP64, dsph.PREC_MIXED])
+def test_immersed_spheres_one_step(prec):
+    note(f"immersed spheres prec={prec}")
+    sc = IMMERSED
+    g, o = pair(sc, prec)
+    mov = np.asarray(sc.solid_inv_mass) > 0
+    # every sphere has fluid within 2h
+    fl, sl, _ = o.list(dsph.LIST_FLUID_SOLID)
+    assert set(np.flatnonzero(mov)) <= set(sl.tolist())
+    g.step(1, sc.dt)
+    o.step(1, sc.dt)
+    tol = TOL_FP64 if prec == dsph.PREC_FP64 else TOL_MIXED
+    fa, fb = g.fs_force(), o.fs_force()
+    assert rel(fa, fb) < tol
+    ga, oa = g.solid(), o.solid()
+    for k in ("acc", "vel", "pos", "angvel", "torque"):
+        assert rel(ga[k][mov], oa[k][mov]) < tol, k
+    gf, of = g.fluid(), o.fluid()
+    for k in ("acc", "vel", "drho"):
+        assert rel(gf[k], of[k]) < tol, k
+
+
+@pytest.mark.parametrize("prec", [dsph.PREC_FP64, dsph.PREC_MIXED])
+def test_immersed_spheres_many_steps(prec):
+    sc = IMMERSED
+    g, o = pair(sc, prec)
+    n = 150
+    g.run(n, sc.dt, graphs=True)
+    o.run(n, sc.dt)
+    mov = np.asarray(sc.solid_inv_mass) > 0
+    ga, oa = g.solid(), o.solid()
+    m = 1.0 / np.asarray(sc.solid_inv_mass)[mov]
+    ke = lambda s: 0.5 * np.sum(m * np.sum(s["vel"][mov] ** 2, axis=1))
+    cz = lambda s: np.mean(s["pos"][mov, 2])
+    if prec == dsph.PREC_FP64:
+        assert rel(ga["pos"][mov], oa["pos"][mov]) < 1e-9
+        assert rel(ga["vel"][mov], oa["vel"][mov]) < 1e-6
+    assert abs(ke(ga) - ke(oa)) <= 0.01 * ke(oa)
+    assert abs(cz(ga) - cz(oa)) <= 0.01 * abs(cz(oa))
+
+
+# ---------------------------------------- springs, linear law, materials
+MIX = scenes.dem_contact_mix()
+
+
+def test_contact_mix_springs_linear_law_and_materials():
+    note("dem contact mix")
+    sc = MIX
+    g, o = pair(sc, dsph.PREC_FP64)
+    n = 300
+    for s in range(1, n + 1):
+        g.step(s, sc.dt)
+        o.step(s, sc.dt)
+        if s in (1, n):
+            ca, cb = g.contacts(), o.contacts()
+            ga_, gb_, _ = g.list(dsph.LIST_SOLID_SOLID)
+            oa_, ob_, _ = o.list(dsph.LIST_SOLID_SOLID)
+            assert np.array_equal(ga_, oa_) and np.array_equal(gb_, ob_), s
+            for k in ("force", "torque", "sliding", "rolling", "torsion"):
+                assert rel(ca[k], cb[k]) < 1e-8, (s, k)
+    # the scene loads every spring and every material pair
+    mat = np.asarray(sc.solid_material)
+    pairs = {(min(mat[i], mat[j]), max(mat[i], mat[j])) for i, j in zip(ga_, gb_)}
+    assert pairs >= {(0, 0), (0, 1), (1, 1)}
+    for k in ("sliding", "rolling", "torsion"):
+        assert np.abs(ca[k]).max() > 0, k
+    a, b = g.solid(), o.solid()
+    assert rel(a["pos"], b["pos"]) < 1e-10
+    for k in ("vel", "angvel"):
+        assert rel(a[k], b[k], 1e-3) < 1e-7, k
+    ca, cb = g.clump(), o.clump()
+    for k in ("centroid", "vel", "angvel"):
+        assert rel(ca[k], cb[k], 1e-3) < 1e-7, k
+
+
+# ------------------------------------------------------------- bond break
+def test_bond_loaded_past_tensile_strength_breaks():
+    sc = scenes.bonded_bar()
+    g, o = pair(sc, dsph.PREC_FP64)
+    on0 = g.bonds()["is_bonded"].copy()
+    assert on0.all() and len(on0) > 100
+    g.run(400, sc.dt)
+    o.run(400, sc.dt)
+    a, b = g.bonds(), o.bonds()
+    assert np.array_equal(a["is_bonded"], b["is_bonded"])
+    assert 0 < int((a["is_bonded"] == 0).sum()) < len(on0)
+    for k in ("normal_force", "torsion", "shear", "bending"):
+        assert rel(a[k], b[k], 1e-6) < 1e-8, k
+    sa, sb = g.solid(), o.solid()
+    assert rel(sa["pos"], sb["pos"]) < 1e-10
+
+
+# --------------------------------------- external force and global damping
+def test_external_force_and_global_damping_in_the_user_hook():
+    sc = MIX
+    g, o = pair(sc, dsph.PREC_FP64)
+    rng = np.random.default_rng(7)
+    n = 120
+    targets = rng.choice(sc.n_solid, 12, replace=False)
+    for s in range(1, n + 1):
+        for x in (g, o):
+            x.neighbor_search(s, 1)
+            x.solid_before_contact(sc.dt)
+            x.fluid_integrate(sc.dt, s, 1)
+            x.contact_forces(sc.dt)
+            # handleDataAfterCalculateContact (solverBase.cpp:55)
+            for k, i in enumerate(targets):
+                i = int(i)
+                if k % 2 == 0:
+                    x.add_external_force(i, [1e-4 * (k + 1), -2e-4, 3e-4 * np.sin(s)])
+                else:
+                    x.add_global_damping(i, 0.05 * k)
+            x.solid_after_contact(sc.dt)
+    a, b = g.solid(), o.solid()
+    assert rel(a["pos"], b["pos"]) < 1e-10
+    for k in ("vel", "angvel"):
+        assert rel(a[k], b[k], 1e-3) < 1e-7, k
+    # and the forces acted: the run without them differs
+    h = dsph.Solver(dsph.PREC_FP64)
+    h.upload(sc)
+    h.initialize()
+    h.run(n, sc.dt)
+    assert rel(h.solid()["vel"], a["vel"], 1e-3) > 1e-6

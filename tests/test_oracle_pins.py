@@ -1150,3 +1150,54 @@ def test_clump_force_and_torque_sum_the_pebble_contact():
     assert np.abs(M * c["acc"][0] - F).max() < 1e-12 * np.linalg.norm(F)
     Tc = T + np.cross(mid - c["centroid"][0], F)
     assert np.abs(c["torque"][0] - Tc).max() < 1e-12 * np.linalg.norm(Tc)
+
+
+# ------------------------------------- runtime mutation (dataBase.cpp:179-224)
+def _lone_sphere(v=(0.0, 0.0, 0.0), g=(0.0, 0.0, 0.0)):
+    sc = scenes.empty_scene((-1, -1, -1), (2, 2, 2))
+    sc.gravity = np.asarray(g, float)
+    scenes.add_solid(sc, [[0.0, 0.0, 0.0]], 0.01, 2500.0, material=0, vel=v)
+    return sc
+
+
+def _step_with_hook(o, s, dt, hook):
+    o.neighbor_search(s, 1)
+    o.solid_before_contact(dt)
+    o.fluid_integrate(dt, s, 1)
+    o.contact_forces(dt)
+    hook(o)
+    o.solid_after_contact(dt)
+
+
+def test_external_force_accelerates_a_free_sphere():
+    # a constant force F in the user hook of every step: the half kicks
+    # around it give v = (N - 1/2) F dt / m after N steps (the first
+    # step's opening half kick sees no force yet)
+    sc = _lone_sphere()
+    m = 1.0 / float(sc.solid_inv_mass[0])
+    F = np.array([2e-3, -1e-3, 5e-4])
+    dt, n = 1e-4, 25
+    o = _oracle_for(sc)
+    for s in range(1, n + 1):
+        _step_with_hook(o, s, dt, lambda x: x.add_external_force(0, F))
+    v = o.solid()["vel"][0]
+    assert np.allclose(v, (n - 0.5) * dt * F / m, rtol=1e-12, atol=0)
+
+
+def test_global_damping_opposes_the_velocity():
+    # gravity only, moving along x: F = m g, |F| = m |g|, and the damping
+    # takes C_d |F| along -v, a deceleration C_d |g| along x; one step
+    # (the opening half kick sees no acceleration yet)
+    g = 9.81
+    cd, dt = 0.2, 1e-3
+    sc = _lone_sphere(v=(1.0, 0.0, 0.0), g=(0, 0, -g))
+    o = _oracle_for(sc)
+    _step_with_hook(o, 1, dt, lambda x: x.add_global_damping(0, cd))
+    v = o.solid()["vel"][0]
+    assert v[0] == pytest.approx(1.0 - 0.5 * dt * cd * g, rel=1e-14)
+    assert v[2] == pytest.approx(-0.5 * dt * g, rel=1e-14)
+    # a fixed solid is left alone, and so is one at rest
+    sc2 = _lone_sphere(g=(0, 0, -g))
+    o2 = _oracle_for(sc2)
+    _step_with_hook(o2, 1, dt, lambda x: x.add_global_damping(0, cd))
+    assert o2.solid()["vel"][0][0] == 0.0
