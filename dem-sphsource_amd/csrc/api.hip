@@ -152,6 +152,10 @@ struct dsph_ctx {
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[F_NFAM];
     size_t ev_used[F_NFAM] = {0};
+    // device copy of the state taken before a graph-replayed run, restored
+    // when a list overflows during it (one buffer per state array)
+    std::vector<void*> backup;
+    std::vector<size_t> backup_bytes;
 
     bool mixed() const { return prec == DSPH_PREC_MIXED; }
 };
@@ -193,6 +197,8 @@ void free_all(dsph_ctx* c) {
     c->graphs.clear();
     for (void* p : c->allocs) (void)hipFree(p);
     c->allocs.clear();
+    c->backup.clear();
+    c->backup_bytes.clear();
 }
 
 int fail(dsph_ctx* c, const ApiError& e) {
@@ -295,6 +301,179 @@ void invalidate_graphs(dsph_ctx* c) {
         if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
     }
     c->graphs.clear();
+}
+
+// the bond arrays for nb bonds (replacing any earlier ones)
+void alloc_bonds(dsph_ctx* c, int nb) {
+    Bonds& B = c->B;
+    for (void* p : {(void*)B.i, (void*)B.j, (void*)B.on, (void*)B.nrm, (void*)B.pt, (void*)B.Fs, (void*)B.Tb,
+                    (void*)B.Fn, (void*)B.Tt})
+        dfree(c, p);
+    B = Bonds();
+    B.n = nb;
+    B.i = dalloc<int>(c, nb);
+    B.j = dalloc<int>(c, nb);
+    B.on = dalloc<int>(c, nb);
+    B.nrm = dalloc<double4>(c, nb);
+    B.pt = dalloc<double4>(c, nb);
+    B.Fs = dalloc<double4>(c, nb);
+    B.Tb = dalloc<double4>(c, nb);
+    B.Fn = dalloc<double>(c, nb);
+    B.Tt = dalloc<double>(c, nb);
+}
+
+// ------------------------------------------------------------------ state
+// Every device array a later step (or a read-back) reads from an earlier
+// one, in a fixed order: the fluid in its slot order with both record
+// buffers, the hashed grids and sort results, the Verlet supersets with
+// their build positions and control words, every list (the fluid-solid
+// list and its forces carry over the DEM-only steps of a sub-cycled run,
+// the contact list carries the spring history), solids, clumps, wall
+// springs, bonds, the contact tables and the status words.  Scratch whose
+// contents no step inherits (the reorder copy, sort and scan buffers,
+// per-workgroup maxima) and arrays fixed at upload are left out.
+struct StateArr {
+    void* p;
+    size_t bytes;
+};
+std::vector<StateArr> state_arrays(dsph_ctx* c) {
+    std::vector<StateArr> v;
+    auto add = [&](const void* p, size_t n, size_t elem) {
+        if (p && n) v.push_back({const_cast<void*>(p), n * elem});
+    };
+    const size_t nf = c->nf, ns = c->ns, nc = c->nc, nb = c->B.n, ncell = c->ncell;
+    const size_t na = c->so.nactive;
+    const FluidSet& f = c->fset[0];
+    const size_t rec = 2 * (c->mixed() ? sizeof(float4) : sizeof(double4));
+    add(f.ph, nf, 32);
+    add(f.vr, nf, 32);
+    add(f.drho[0], nf, 8);
+    add(f.drho[1], nf, 8);
+    add(f.cn, nf, 16);
+    add(f.alpha, nf, 8);
+    add(f.sumw0, nf, 8);
+    add(f.acc, nf, 32);
+    add(f.orig, nf, 4);
+    add(f.scene, nf, 4);
+    add(f.tid, nf, 4);
+    add(f.rec[0], nf, rec);
+    add(f.rec[1], nf, rec);
+    for (const void* p : {(const void*)c->fslot, (const void*)c->fmember, (const void*)c->fkey,
+                          (const void*)c->frank, (const void*)c->fkey_sorted, (const void*)c->fperm,
+                          (const void*)c->hslot})
+        add(p, nf, 4);
+    add(c->fcs, ncell, 4);
+    add(c->fce, ncell, 4);
+    add(c->fxb, nf, 32);
+    add(c->fsp, nf, 32);
+    add(c->sxb, ns, 32);
+    add(c->ctl, CTL_WORDS, 8);
+    add(c->status, ST_WORDS, 8);
+    add(c->akey, na, 4);
+    add(c->akey_sorted, na, 4);
+    add(c->aperm, na, 4);
+    add(c->acs, ncell, 4);
+    add(c->ace, ncell, 4);
+    const Solids& so = c->so;
+    for (const void* p : {(const void*)so.pr, (const void*)so.vel, (const void*)so.acc, (const void*)so.w,
+                          (const void*)so.tq, (const void*)so.nrm})
+        add(p, ns, 32);
+    add(c->skey, ns, 4);
+    add(c->skey_sorted, ns, 4);
+    add(c->sperm, ns, 4);
+    add(c->scs, ncell, 4);
+    add(c->sce, ncell, 4);
+    const Clumps& cl = c->cl;
+    for (const void* p : {(const void*)cl.pos, (const void*)cl.vel, (const void*)cl.acc, (const void*)cl.w,
+                          (const void*)cl.tq, (const void*)cl.q})
+        add(p, nc, 32);
+    add(c->whist, 9 * ns * (size_t)c->nwmax, 8);
+    add(c->tb.hertz, 8 * (size_t)c->ncomb, 8);
+    add(c->tb.linear, 11 * (size_t)c->ncomb, 8);
+    add(c->tb.bonded, 6 * (size_t)c->ncomb, 8);
+    const Lists& L = c->L;
+    add(L.cnt, nf, 4);
+    add(L.nbr, (size_t)(L.K + kRowPad) * nf, 4);
+    add(L.fscnt, nf, 4);
+    add(L.fsoff, nf + 1, 4);
+    add(L.fs_a, (size_t)L.fs_cap, 4);
+    add(L.fsF, (size_t)L.fs_cap, 32);
+    add(L.fsr_idx, (size_t)L.fs_cap, 4);
+    add(L.fsr_cnt, ns, 4);
+    add(L.fsr_off, ns + 1, 4);
+    add(L.sup, (size_t)(L.supK + kRowPad) * nf, 4);
+    add(L.sup_cnt, nf, 4);
+    add(L.grow, nf, 4);
+    add(L.fsup, (size_t)L.fsupK * nf, 4);
+    add(L.fsup_cnt, nf, 4);
+    add(L.fsm, 2 * nf, 8);
+    add(L.ss_i, (size_t)L.ss_cap, 4);
+    add(L.ss_j, (size_t)L.ss_cap, 4);
+    add(L.ssF, (size_t)L.ss_cap, 32);
+    add(L.ssT, (size_t)L.ss_cap, 32);
+    add(L.ssS, 9 * (size_t)L.ss_cap, 8);
+    add(L.ssr_idx, (size_t)L.ss_cap, 4);
+    add(L.sscnt, ns, 4);
+    add(L.ssoff, ns + 1, 4);
+    add(L.hoff, ns + 1, 4);
+    add(L.hcnt, ns, 4);
+    add(L.h_j, (size_t)L.ss_cap, 4);
+    add(L.hS, 9 * (size_t)L.ss_cap, 8);
+    add(L.ssr_cnt, ns, 4);
+    add(L.ssr_off, ns + 1, 4);
+    const Bonds& B = c->B;
+    add(B.i, nb, 4);
+    add(B.j, nb, 4);
+    add(B.on, nb, 4);
+    add(B.nrm, nb, 32);
+    add(B.pt, nb, 32);
+    add(B.Fs, nb, 32);
+    add(B.Tb, nb, 32);
+    add(B.Fn, nb, 8);
+    add(B.Tt, nb, 8);
+    return v;
+}
+
+// host-side state that goes with the device arrays of a snapshot
+struct HostState {
+    int dcur;
+    int skey_valid;
+};
+
+void free_backup(dsph_ctx* c) {
+    for (void* p : c->backup) dfree(c, p);
+    c->backup.clear();
+    c->backup_bytes.clear();
+}
+
+// device copy of the state (buffers reallocated when a capacity changed)
+HostState snapshot_device(dsph_ctx* c) {
+    const std::vector<StateArr> v = state_arrays(c);
+    bool same = v.size() == c->backup.size();
+    for (size_t k = 0; same && k < v.size(); k++) same = v[k].bytes == c->backup_bytes[k];
+    if (!same) {
+        HIPCHK(hipStreamSynchronize(c->st));
+        free_backup(c);
+        for (const StateArr& a : v) {
+            c->backup.push_back(dalloc<char>(c, a.bytes));
+            c->backup_bytes.push_back(a.bytes);
+        }
+    }
+    for (size_t k = 0; k < v.size(); k++)
+        HIPCHK(hipMemcpyAsync(c->backup[k], v[k].p, v[k].bytes, hipMemcpyDeviceToDevice, c->st));
+    return HostState{c->dcur, (int)c->skey_valid};
+}
+
+void restore_device(dsph_ctx* c, const HostState& h) {
+    const std::vector<StateArr> v = state_arrays(c);
+    if (v.size() != c->backup.size()) throw ApiError{DSPH_E_STATE, "internal error: state snapshot out of date"};
+    for (size_t k = 0; k < v.size(); k++) {
+        if (v[k].bytes != c->backup_bytes[k])
+            throw ApiError{DSPH_E_STATE, "internal error: state snapshot out of date"};
+        HIPCHK(hipMemcpyAsync(v[k].p, c->backup[k], v[k].bytes, hipMemcpyDeviceToDevice, c->st));
+    }
+    c->dcur = h.dcur;
+    c->skey_valid = h.skey_valid != 0;
 }
 
 // ----------------------------------------------------------------- phases
@@ -1756,16 +1935,7 @@ int dsph_initialize(dsph_ctx* c, int compute_normals) {
             }
         }
         const int nb = (int)bi.size();
-        c->B.n = nb;
-        c->B.i = dalloc<int>(c, nb);
-        c->B.j = dalloc<int>(c, nb);
-        c->B.on = dalloc<int>(c, nb);
-        c->B.nrm = dalloc<double4>(c, nb);
-        c->B.pt = dalloc<double4>(c, nb);
-        c->B.Fs = dalloc<double4>(c, nb);
-        c->B.Tb = dalloc<double4>(c, nb);
-        c->B.Fn = dalloc<double>(c, nb);
-        c->B.Tt = dalloc<double>(c, nb);
+        alloc_bonds(c, nb);
         if (nb) {
             std::vector<int> ones(nb, 1);
             std::vector<double4> z4(nb, make_double4(0, 0, 0, 0));
@@ -1881,6 +2051,9 @@ int dsph_step_n(dsph_ctx* c, int32_t i0, int32_t n, double dt, int32_t gap) {
             invalidate_graphs(c);
             c->graph_dt = dt;
         }
+        // the state before the run, to roll back to if a list overflows
+        // inside it (graph-replayed searches cannot grow anything)
+        const HostState before = snapshot_device(c);
         for (int k = 0; k < n; k++) {
             const int is = i0 + k;
             if (timing) {  // events cannot live inside a captured graph
@@ -1912,8 +2085,21 @@ int dsph_step_n(dsph_ctx* c, int32_t i0, int32_t n, double dt, int32_t gap) {
         unsigned long long w[ST_WORDS];
         read_status(c, w);
         if (w[ST_FF_OVERFLOW] | w[ST_FS_OVERFLOW] | w[ST_SS_OVERFLOW] | w[ST_SUP_OVERFLOW]) {
-            c->err = "neighbor list overflow during dsph_step_n; enlarge capacities (dsph_set_capacity) and rerun";
-            return DSPH_E_CAPACITY;
+            // some search of the run overflowed a list and every later
+            // kernel read cut-short rows: back to the state before the run
+            // (with its status words, so the flags are clear again), then
+            // the same steps eagerly, each search checked: a search that
+            // overflows grows its lists and runs again, as in dsph_step
+            restore_device(c, before);
+            for (int k = 0; k < n; k++) {
+                const int is = i0 + k;
+                search_checked(c, is, gap);
+                solid_before(c, dt);
+                fluid_integrate(c, dt, is, gap);
+                contacts(c, dt);
+                solid_after(c, dt);
+            }
+            HIPCHK(hipStreamSynchronize(c->st));
         }
         return DSPH_OK;
     } catch (const ApiError& e) {
@@ -1924,6 +2110,154 @@ int dsph_step_n(dsph_ctx* c, int32_t i0, int32_t n, double dt, int32_t gap) {
 int dsph_synchronize(dsph_ctx* c) {
     if (!c) return DSPH_E_INVALID;
     return hipStreamSynchronize(c->st) == hipSuccess ? DSPH_OK : DSPH_E_HIP;
+}
+
+// ------------------------------------------------------------ checkpoint
+namespace {
+// the checkpoint blob: this header, n_arrays byte counts (int64), then the
+// arrays of state_arrays() back to back
+struct StateBlobHeader {
+    char magic[8];
+    int32_t version;
+    int32_t prec;
+    int32_t nf, ns, nc, nw, nscene, nbond, ncell, ncomb, nactive, nwmax;
+    int32_t K, supK, fsupK, reorder_period;
+    int64_t fs_cap, ss_cap;
+    int32_t dcur, skey_valid, tune_bs, tune_bmem, tune_group, pad;
+    uint32_t flags, pad1;
+    double skin_h, gv[3];
+    int64_t n_arrays, payload;
+};
+constexpr char kStateMagic[8] = {'D', 'S', 'P', 'H', 'S', 'T', 'A', 'T'};
+constexpr int32_t kStateVersion = 1;
+
+StateBlobHeader state_header(dsph_ctx* c, const std::vector<StateArr>& v) {
+    StateBlobHeader h{};
+    std::memcpy(h.magic, kStateMagic, 8);
+    h.version = kStateVersion;
+    h.prec = c->prec;
+    h.nf = c->nf;
+    h.ns = c->ns;
+    h.nc = c->nc;
+    h.nw = c->nw;
+    h.nscene = c->nscene;
+    h.nbond = c->B.n;
+    h.ncell = c->ncell;
+    h.ncomb = c->ncomb;
+    h.nactive = c->so.nactive;
+    h.nwmax = c->nwmax;
+    h.K = c->L.K;
+    h.supK = c->L.supK;
+    h.fsupK = c->L.fsupK;
+    h.reorder_period = c->reorder_period;
+    h.fs_cap = c->L.fs_cap;
+    h.ss_cap = c->L.ss_cap;
+    h.dcur = c->dcur;
+    h.skey_valid = c->skey_valid ? 1 : 0;
+    h.tune_bs = c->tune_bs;
+    h.tune_bmem = c->tune_bmem;
+    h.tune_group = c->tune_group;
+    h.flags = c->flags;
+    h.skin_h = c->skin_h;
+    for (int k = 0; k < 3; k++) h.gv[k] = c->gv[k];
+    h.n_arrays = (int64_t)v.size();
+    int64_t tot = 0;
+    for (const StateArr& a : v) tot += (int64_t)a.bytes;
+    h.payload = tot;
+    return h;
+}
+
+int64_t blob_size(const StateBlobHeader& h) {
+    return (int64_t)sizeof(StateBlobHeader) + 8 * h.n_arrays + h.payload;
+}
+}  // namespace
+
+int dsph_save_state(dsph_ctx* c, void* buffer, int64_t cap, int64_t* size_out) {
+    API_GUARD(c);
+    if (!c->initialized) return DSPH_E_STATE;
+    try {
+        const std::vector<StateArr> v = state_arrays(c);
+        const StateBlobHeader h = state_header(c, v);
+        const int64_t need = blob_size(h);
+        if (size_out) *size_out = need;
+        if (!buffer) return DSPH_OK;
+        if (cap < need) throw ApiError{DSPH_E_INVALID, "state buffer too small: " + std::to_string(need) + " bytes needed"};
+        char* out = (char*)buffer;
+        std::memcpy(out, &h, sizeof(h));
+        int64_t* sizes = (int64_t*)(out + sizeof(h));
+        char* data = out + sizeof(h) + 8 * h.n_arrays;
+        for (size_t k = 0; k < v.size(); k++) {
+            sizes[k] = (int64_t)v[k].bytes;
+            HIPCHK(hipMemcpyAsync(data, v[k].p, v[k].bytes, hipMemcpyDeviceToHost, c->st));
+            data += v[k].bytes;
+        }
+        HIPCHK(hipStreamSynchronize(c->st));
+        return DSPH_OK;
+    } catch (const ApiError& e) {
+        return fail(c, e);
+    }
+}
+
+int dsph_load_state(dsph_ctx* c, const void* buffer, int64_t size) {
+    API_GUARD(c);
+    if (!buffer || size < (int64_t)sizeof(StateBlobHeader)) return DSPH_E_INVALID;
+    try {
+        StateBlobHeader h;
+        std::memcpy(&h, buffer, sizeof(h));
+        require(std::memcmp(h.magic, kStateMagic, 8) == 0 && h.version == kStateVersion,
+                "not a state blob of this library version");
+        require(size >= blob_size(h), "state blob truncated");
+        require(h.prec == c->prec, "state blob of the other precision");
+        require(h.nf == c->nf && h.ns == c->ns && h.nc == c->nc && h.nw == c->nw && h.nscene == c->nscene &&
+                    h.ncell == c->ncell && h.ncomb == c->ncomb && h.nactive == c->so.nactive &&
+                    h.nwmax == c->nwmax,
+                "state blob of another scene (particle, wall, cell or material counts differ)");
+        require(h.tune_bs >= 64 && h.tune_bs <= 512 && h.tune_bs % 64 == 0 && h.tune_bmem >= 1 &&
+                    h.tune_bmem <= 64 && (h.tune_group == 1 || h.tune_group == 2 || h.tune_group == 4 ||
+                                          h.tune_group == 8) && h.K >= 1 && h.supK >= 1 && h.fsupK >= 1 &&
+                    h.fs_cap >= 1 && h.ss_cap >= 1 && h.nbond >= 0 && h.skin_h >= 0.0,
+                "state blob header out of range");
+        HIPCHK(hipStreamSynchronize(c->st));
+        invalidate_graphs(c);
+        // what the saved run's results depend on besides its arrays
+        c->flags = h.flags;
+        for (int k = 0; k < 3; k++) c->gv[k] = h.gv[k];
+        c->reorder_period = h.reorder_period;
+        c->tune_bs = h.tune_bs;
+        c->tune_group = h.tune_group;
+        if (c->tune_bmem != h.tune_bmem) {
+            c->tune_bmem = h.tune_bmem;
+            build_chunks(c);
+        }
+        c->skin_h = h.skin_h;
+        superset_stencil(c);
+        // its capacities
+        if (c->L.K != h.K) alloc_ff(c, h.K);
+        if (c->L.supK != h.supK || c->L.fsupK != h.fsupK) alloc_sup(c, h.supK, h.fsupK);
+        if (c->L.fs_cap != h.fs_cap) alloc_fs(c, h.fs_cap);
+        if (c->L.ss_cap != h.ss_cap) alloc_ss(c, h.ss_cap, false);
+        if (c->B.n != h.nbond || !c->B.i) alloc_bonds(c, h.nbond);
+        const std::vector<StateArr> v = state_arrays(c);
+        require((int64_t)v.size() == h.n_arrays, "state blob does not match this context's arrays");
+        const char* in = (const char*)buffer;
+        const int64_t* sizes = (const int64_t*)(in + sizeof(h));
+        for (size_t k = 0; k < v.size(); k++)
+            require(sizes[k] == (int64_t)v[k].bytes, "state blob does not match this context's arrays");
+        const char* data = in + sizeof(h) + 8 * h.n_arrays;
+        for (size_t k = 0; k < v.size(); k++) {
+            HIPCHK(hipMemcpyAsync(v[k].p, data, v[k].bytes, hipMemcpyHostToDevice, c->st));
+            data += v[k].bytes;
+        }
+        c->dcur = h.dcur;
+        c->skey_valid = h.skey_valid != 0;
+        c->h_bonded.resize(6 * (size_t)c->ncomb);
+        d2h(c, c->h_bonded.data(), c->tb.bonded, c->h_bonded.size());
+        HIPCHK(hipStreamSynchronize(c->st));
+        c->initialized = true;
+        return DSPH_OK;
+    } catch (const ApiError& e) {
+        return fail(c, e);
+    }
 }
 
 int dsph_set_contact_models(dsph_ctx* c, const double* hertz, const double* linear,

@@ -57,6 +57,8 @@ _SIGS = {
     "dsph_step": [C.c_int32, C.c_double, C.c_int32, C.c_int],
     "dsph_step_n": [C.c_int32, C.c_int32, C.c_double, C.c_int32],
     "dsph_synchronize": [],
+    "dsph_save_state": [_vp, C.c_int64, _i64p],
+    "dsph_load_state": [_vp, C.c_int64],
     "dsph_set_contact_models": [_dp, _dp, _dp],
     "dsph_set_clump_velocity": [C.c_int32, _dp],
     "dsph_add_external_force": [C.c_int32, _dp],
@@ -244,6 +246,19 @@ class Solver:
 
     def synchronize(self):
         self._call("dsph_synchronize")
+
+    def save_state(self) -> bytes:
+        """The full state as a checkpoint blob (dsph_save_state)."""
+        n = C.c_int64(0)
+        self._call("dsph_save_state", None, 0, C.byref(n))
+        buf = C.create_string_buffer(n.value)
+        self._call("dsph_save_state", buf, n.value, C.byref(n))
+        return buf.raw[: n.value]
+
+    def load_state(self, blob: bytes):
+        """Continue a saved run (dsph_load_state): the scene must be uploaded."""
+        buf = C.create_string_buffer(bytes(blob), len(blob))
+        self._call("dsph_load_state", buf, len(blob))
 
     def set_contact_models(self, hertz=None, linear=None, bonded=None):
         """Replace contact tables during the run (field-major, as in Scene)."""

@@ -213,6 +213,30 @@ def sound_speed(column_height=0.4):
     return 10.0 * math.sqrt(2.0 * G * column_height)
 
 
+def tank_wall_points(origin, size, spacing, L, W) -> np.ndarray:
+    """The points of regular_packed_points(origin, size, spacing) outside
+    [0,L]x[0,W]x[0,inf) (the wall layers of an open tank), in the same
+    order, without materialising the whole lattice: the lattice's y-z slab
+    is filtered once and reused for every x inside [0, L]."""
+    origin = [float(v) for v in origin]
+    size = [float(v) for v in size]
+    num = [int((s + 1.e-10) / spacing) for s in size]
+    if min(num) == 0:
+        p = regular_packed_points(origin, size, spacing)
+        keep = (p[:, 0] < 0) | (p[:, 1] < 0) | (p[:, 2] < 0) | (p[:, 0] > L) | (p[:, 1] > W)
+        return p[keep]
+    ax = [origin[k] + (size[k] / float(num[k])) * (np.arange(1, num[k] + 1, dtype=np.float64) - 0.5)
+          for k in range(3)]
+    Y, Z = np.meshgrid(ax[1], ax[2], indexing="ij")
+    yz = np.stack([Y.ravel(), Z.ravel()], axis=1)
+    shell = yz[(yz[:, 0] < 0) | (yz[:, 1] < 0) | (yz[:, 0] > W)]
+    parts = []
+    for x in ax[0]:
+        q = yz if (x < 0 or x > L) else shell
+        parts.append(np.concatenate([np.full((len(q), 1), x), q], axis=1))
+    return np.concatenate(parts) if parts else np.zeros((0, 3))
+
+
 def dam_break(dx=0.02, L=1.6, W=0.6, H=0.6, column=(0.4, 0.4, 0.4), with_floor_walls=False,
               name="dam_break") -> Scene:
     """Plain 3-D dam break (damBreak.cpp:14-58 pattern): tank LxWxH, water
@@ -225,9 +249,7 @@ def dam_break(dx=0.02, L=1.6, W=0.6, H=0.6, column=(0.4, 0.4, 0.4), with_floor_w
     c0 = sound_speed(column[2])
     fp = regular_packed_points((0, 0, 0), column, dx)
     add_fluid(sc, fp, h, RHO0, c0, NU)
-    sp0 = regular_packed_points(o, size, dx)
-    keep = (sp0[:, 0] < 0) | (sp0[:, 1] < 0) | (sp0[:, 2] < 0) | (sp0[:, 0] > L) | (sp0[:, 1] > W)
-    add_dummy_wall(sc, sp0[keep], 0.5 * dx)
+    add_dummy_wall(sc, tank_wall_points(o, size, dx, L, W), 0.5 * dx)
     sc.dt = 0.25 * h / c0
     sc.gap = 1
     return sc
@@ -365,9 +387,7 @@ def random_fluid_box(n=2000, box=(0.1, 0.08, 0.06), dx=0.005, seed_stream=7, h_j
     sc.fluid_vel = uniform(ROOT_SEED, seed_stream + 2, 3 * m, -vel_scale, vel_scale).reshape(m, 3)
     sc.fluid_drho = uniform(ROOT_SEED, seed_stream + 3, m, -drho_scale, drho_scale)
     if with_walls:
-        sp0 = regular_packed_points(o, size, dx)
-        keep = (sp0[:, 0] < 0) | (sp0[:, 1] < 0) | (sp0[:, 2] < 0) | (sp0[:, 0] > L) | (sp0[:, 1] > W)
-        add_dummy_wall(sc, sp0[keep], 0.5 * dx)
+        add_dummy_wall(sc, tank_wall_points(o, size, dx, L, W), 0.5 * dx)
     sc.dt = 0.25 * 1.3 * dx / c0
     return sc
 
@@ -439,10 +459,8 @@ def dam_break_case(dx=0.025, tank=(8.0, 0.7, 0.8), fluid=(3.5, 0.7, 0.4),
     sc = empty_scene(o, size, name=name)
     h = 1.3 * dx
     add_fluid(sc, regular_packed_points((0, 0, 0), fluid, dx), h, 1000.0, 30.0, 1.0e-6)
-    sp0 = regular_packed_points(o, size, dx)
-    keep = (sp0[:, 0] < 0) | (sp0[:, 1] < 0) | (sp0[:, 2] < 0) | (sp0[:, 0] > L) | (sp0[:, 1] > W)
     r = 0.5 * dx
-    add_dummy_wall(sc, sp0[keep], r)
+    add_dummy_wall(sc, tank_wall_points(o, size, dx, L, W), r)
     add_infinite_wall(sc, floor_point if floor_point is not None else (0.5 * L, 0.5 * W, 0.0),
                       (0, 0, 1), 0)
     sc.set_hertz(0, 1, E, 0.3 * E, 0.9, 0, 0, 0.35, 0, 0)
@@ -488,9 +506,7 @@ def immersed_spheres(n_box=(12, 10, 10), dx=0.005, stride=3, r_frac=0.5, rho_s=2
     m = sc.n_fluid
     sc.fluid_vel = uniform(ROOT_SEED, seed_stream, 3 * m, -0.05, 0.05).reshape(m, 3)
     sc.fluid_drho = uniform(ROOT_SEED, seed_stream + 1, m, -0.5, 0.5)
-    sp0 = regular_packed_points(o, size, dx)
-    keep = (sp0[:, 0] < 0) | (sp0[:, 1] < 0) | (sp0[:, 2] < 0) | (sp0[:, 0] > L) | (sp0[:, 1] > W)
-    add_dummy_wall(sc, sp0[keep], 0.5 * dx)
+    add_dummy_wall(sc, tank_wall_points(o, size, dx, L, W), 0.5 * dx)
     r = r_frac * dx
     add_solid(sc, pts[dem], r, rho_s, material=0)
     Es, Gs = hertz_effective(1.0e7, 0.3)

@@ -40,7 +40,7 @@ extern "C" {
 
 /* ---- precision of the fluid pair loops ------------------------------- */
 #define DSPH_PREC_FP64   0  /* everything FP64 (the reference's arithmetic)   */
-#define DSPH_PREC_MIXED  1  /* FP32 pair loops with compensated sums,
+#define DSPH_PREC_MIXED  1  /* FP32 pair terms accumulated in FP64,
                                FP64 integration, springs and quaternions      */
 
 /* ---- behaviour flags (reference quirks) ------------------------------ */
@@ -240,12 +240,36 @@ int dsph_solid_after_contact(dsph_ctx* ctx, double dt);
 int dsph_step(dsph_ctx* ctx, int32_t i_step, double dt, int32_t gap,
               int check_capacity);
 /* n consecutive steps i_step0 .. i_step0+n-1 replayed from captured HIP
- * graphs (no host synchronisation inside); returns DSPH_E_CAPACITY if any
- * list overflowed during the run.  No reference counterpart (the reference
- * syncs three times per step, neighborSearch.cu:339-340,373). */
+ * graphs (no host synchronisation inside).  Replayed searches cannot grow
+ * a list: the call first copies the state on the device, and when a list
+ * overflowed at some step of the run it returns to that copy and runs the
+ * n steps again eagerly with checked searches (as dsph_step with
+ * check_capacity), which grow what overflows.  The copy costs one
+ * device-to-device pass over the state per call and device memory of the
+ * state's size.  No reference counterpart (the reference syncs three times
+ * per step, neighborSearch.cu:339-340,373). */
 int dsph_step_n(dsph_ctx* ctx, int32_t i_step0, int32_t n, double dt,
                 int32_t gap);
 int dsph_synchronize(dsph_ctx* ctx);
+
+/* ---- full-state checkpoint ---------------------------------------------
+ * No reference counterpart: the reference writes VTU frames only
+ * (solverBase.cpp:69-333) and keeps the spring history on the device
+ * (deviceStructs.h:506-509, 693-711), so a run cannot be resumed.  The blob
+ * holds every device array a later step reads: the fluid in its device
+ * order with sumW0 and both record buffers, the hashed grids, the Verlet
+ * supersets and their build positions, the fluid-solid and contact lists
+ * with their forces and spring history, solids, clumps, wall springs,
+ * bonds, the contact tables and the status words, plus the capacities,
+ * launch tuning, Verlet skin, flags and gravity of the run.
+ * dsph_save_state needs an initialised context; with buffer == NULL it
+ * only stores the blob size in *size_out.  dsph_load_state needs a context
+ * with the same scene uploaded (same precision and counts); it takes the
+ * blob's capacities, tuning, skin, flags and gravity and leaves the
+ * context initialised, so the next step continues the saved run bit for
+ * bit.  The blob is in this library's own layout (host byte order).       */
+int dsph_save_state(dsph_ctx* ctx, void* buffer, int64_t cap, int64_t* size_out);
+int dsph_load_state(dsph_ctx* ctx, const void* buffer, int64_t size);
 
 /* ---- runtime parameter changes ------------------------------------------
  * replaces: dataBase::setHertzianContactModel / setLinearContactModel /

@@ -250,3 +250,39 @@ def test_external_force_and_global_damping_in_the_user_hook():
     h.initialize()
     h.run(n, sc.dt)
     assert rel(h.solid()["vel"], a["vel"], 1e-3) > 1e-6
+
+
+# --------------------------------- the reference's case through the C++ API
+def test_cpp_dam_break_case_matches_python_path(tmp_path):
+    """host/cases/damBreak.cpp (the reference's case on the solverBase /
+    dataBase API, small tank) against the same scene from scenes.py stepped
+    through the Python binding: bit-identical after two fluid steps."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(dsph.LIB_PATH), "_build", "damBreak")
+    assert os.path.exists(exe), "build the package first (make in dem-sphsource_amd)"
+    out = tmp_path / "state.bin"
+    steps = 2 * CASE.gap + 3
+    note(f"c++ case driver damBreak small, {steps} steps")
+    r = subprocess.run([exe, "small", str(steps), str(dsph.PREC_FP64), str(out)], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = out.read_bytes()
+    nf, ns, nc, gap = (int(v) for v in np.frombuffer(raw[:16], np.int32))
+    assert (nf, ns, nc, gap) == (CASE.n_fluid, CASE.n_solid, CASE.n_clump, CASE.gap)
+    d = np.frombuffer(raw[16:], np.float64)
+    k = 0
+    arrays = []
+    for n in (nf, nf, ns, ns, nc, nc, nc):
+        arrays.append(d[k:k + 3 * n].reshape(n, 3))
+        k += 3 * n
+    g = dsph.Solver(dsph.PREC_FP64)
+    g.upload(CASE)
+    g.initialize()
+    for s in range(1, steps + 1):
+        g.step(s, CASE.dt, CASE.gap)
+    f, s_, c = g.fluid(), g.solid(), g.clump()
+    mine = [f["pos"], f["vel"], s_["pos"], s_["vel"], c["centroid"], c["vel"], c["angvel"]]
+    for a, b in zip(mine, arrays):
+        assert np.array_equal(a, b)
+    assert os.path.isdir(tmp_path / "damBreakOutput")

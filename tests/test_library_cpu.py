@@ -125,3 +125,15 @@ int main() {
     assert np.array_equal(got, pts)
     u = scenes.uniform(scenes.ROOT_SEED, 1001, 100, -0.001, 0.001)
     assert np.array_equal(np.frombuffer(raw[8 * n:], np.float64), u)
+
+
+@pytest.mark.parametrize("dx,L,W,H", [(0.02, 1.6, 0.6, 0.6), (0.0103, 1.2, 0.6, 0.6),
+                                      (0.025, 1.0, 0.35, 0.4), (0.005, 0.06, 0.05, 0.05)])
+def test_tank_wall_points_equal_the_filtered_lattice(dx, L, W, H):
+    """The shell generator gives the points (and order) of filtering the whole
+    getRegularPackedPoints lattice with the damBreak.cpp wall test."""
+    o = np.array([-3 * dx] * 3)
+    size = np.array([L + 6 * dx, W + 6 * dx, H + 6 * dx])
+    sp0 = scenes.regular_packed_points(o, size, dx)
+    keep = (sp0[:, 0] < 0) | (sp0[:, 1] < 0) | (sp0[:, 2] < 0) | (sp0[:, 0] > L) | (sp0[:, 1] > W)
+    assert np.array_equal(scenes.tank_wall_points(o, size, dx, L, W), sp0[keep])
