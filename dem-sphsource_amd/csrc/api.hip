@@ -156,6 +156,9 @@ struct dsph_ctx {
     // when a list overflows during it (one buffer per state array)
     std::vector<void*> backup;
     std::vector<size_t> backup_bytes;
+    // work buffer of dsph_hash_sort (grown on demand)
+    unsigned* hs_work = nullptr;
+    int64_t hs_words = 0;
 
     bool mixed() const { return prec == DSPH_PREC_MIXED; }
 };
@@ -199,6 +202,8 @@ void free_all(dsph_ctx* c) {
     c->allocs.clear();
     c->backup.clear();
     c->backup_bytes.clear();
+    c->hs_work = nullptr;
+    c->hs_words = 0;
 }
 
 int fail(dsph_ctx* c, const ApiError& e) {
@@ -1224,6 +1229,29 @@ void merge_scenes(const dsph_scene* sc, int k, Merged& m) {
     f.wall_material = m.wmat.data();
 }
 }  // namespace
+
+namespace dsph {
+int ctx_device(dsph_ctx* c) { return c->dev; }
+hipStream_t ctx_stream(dsph_ctx* c) { return c->st; }
+void ctx_set_error(dsph_ctx* c, const char* msg) { c->err = msg; }
+unsigned* ctx_work(dsph_ctx* c, int64_t words) {
+    if (c->hs_words >= words) return c->hs_work;
+    try {
+        if (c->hs_work) {
+            HIPCHK(hipStreamSynchronize(c->st));
+            dfree(c, c->hs_work);
+            c->hs_work = nullptr;
+            c->hs_words = 0;
+        }
+        c->hs_work = dalloc<unsigned>(c, (size_t)words);
+        c->hs_words = words;
+        return c->hs_work;
+    } catch (const ApiError& e) {
+        c->err = e.msg;
+        return nullptr;
+    }
+}
+}  // namespace dsph
 
 extern "C" {
 

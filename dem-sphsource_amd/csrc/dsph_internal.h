@@ -335,6 +335,12 @@ void hash_fluid(hipStream_t s, const Grids& G, const double4* ph, const int* ori
                 const unsigned* crank, const unsigned char* cspare, unsigned* key_orig,
                 unsigned* rank_orig, unsigned* hslot, const double4* fxb, unsigned long long* bmax,
                 unsigned long long* ctl, long long* status);
+// calculatePointHash of n caller points (interleaved xyz doubles) on one
+// grid (a scene-local grid: cell0 = 0), into key
+void hash_points(hipStream_t s, const GridP& g, const double* xyz, int n, unsigned* key);
+// the device address of the allocation-free entry points' scratch on the
+// current device (api_grid.hip); 0 on success
+int noalloc_scratch_setup(int device);
 // the reference hash of every solid (and the out-of-domain count)
 void hash_solid(hipStream_t s, const Grids& G, const double4* pr, int n, unsigned* key,
                 long long* status);

@@ -190,6 +190,15 @@ __global__ void __launch_bounds__(kBS) k_hash_fluid(Grids G, const double4* __re
     }
 }
 
+// calculatePointHash of caller points (interleaved xyz doubles) on one grid
+__global__ void __launch_bounds__(kBS) k_hash_points(GridP g, const double* __restrict__ xyz, int n,
+                                                     unsigned* __restrict__ key) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const double* p = xyz + 3 * (int64_t)s;
+    key[s] = (unsigned)point_hash(g, p[0], p[1], p[2]);
+}
+
 __global__ void k_hash_solid(Grids G, const double4* __restrict__ pr, int n,
                              unsigned* __restrict__ key, unsigned long long* st) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1197,6 +1206,10 @@ void hash_fluid(hipStream_t s, const Grids& G, const double4* ph, const int* ori
     if (n > 0)
         hipLaunchKernelGGL(k_hash_fluid, dim3(blocks_for(n)), dim3(kBS), 0, s, G, ph, orig, n, sub_bits,
                            crank, cspare, key, rank, hslot, fxb, bmax, ctl, (unsigned long long*)status);
+}
+
+void hash_points(hipStream_t s, const GridP& g, const double* xyz, int n, unsigned* key) {
+    if (n > 0) hipLaunchKernelGGL(k_hash_points, dim3(blocks_for(n)), dim3(kBS), 0, s, g, xyz, n, key);
 }
 
 void hash_solid(hipStream_t s, const Grids& G, const double4* pr, int n, unsigned* key,

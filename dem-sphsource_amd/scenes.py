@@ -175,7 +175,10 @@ def add_clump(sc: Scene, pos, radius, centroid, mass, inertia, material=0,
     vel = np.asarray(vel, float)
     angvel = np.asarray(angvel, float)
     start = sc.n_solid
-    volume = float(np.sum(np.power(2.0 * radius, 3)))
+    # summed in pebble order, as the reference's loop does (dataBase.cpp:138-142)
+    volume = 0.0
+    for v in np.power(2.0 * radius, 3):
+        volume += float(v)
     dens = mass / volume if volume > 0 else 0.0
     invm = np.where((dens > 0) & (radius > 0), 1.0 / (np.power(2.0 * radius, 3) * dens), 0.0)
     pv = vel + np.cross(angvel, pos - centroid)
@@ -466,12 +469,14 @@ def dam_break_case(dx=0.025, tank=(8.0, 0.7, 0.8), fluid=(3.5, 0.7, 0.4),
     sc.set_hertz(0, 1, E, 0.3 * E, 0.9, 0, 0, 0.35, 0, 0)
     sc.set_hertz(1, 1, 0.5 * E, 0.15 * E, 0.9, 0, 0, 0.45, 0, 0)
     pts = regular_packed_points(clump_origin, (cube, cube, cube), 2 * r)
-    mass = cube_density * cube ** 3
+    # products in the reference's evaluation order, so the mass and the
+    # time step round as they do there
+    mass = cube_density * cube * cube * cube
     inertia = (mass / 12.0 * (cube * cube + cube * cube),) * 3 + (0.0, 0.0, 0.0)
     centroid = np.asarray(clump_origin, float) + 0.5 * cube
     add_clump(sc, pts, np.full(len(pts), r), centroid, mass, inertia, material=1)
     k = 0.5 * E * math.pi * r
-    mij = 8 * r ** 3 * cube_density
+    mij = 8 * r * r * r * cube_density
     sc.dt = 0.2 * math.sqrt(mij / k)
     dt_f = 0.6 * 1.3 * dx / (22.0 * math.sqrt(0.4 * 9.8))
     sc.gap = max(1, int(dt_f / sc.dt))

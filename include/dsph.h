@@ -335,6 +335,28 @@ int dsph_download_grid(dsph_ctx* ctx, int species, int32_t* hash,
 
 int dsph_status(dsph_ctx* ctx, int64_t words[DSPH_STATUS_WORDS]);
 
+/* ---- the hashed grid of caller-held particles --------------------------
+ * replaces: updateGridCellStartEnd for one species (neighborSearch.cu:89-113)
+ * — calculatePointHash (:19-36), the stable key/value sort of the hashes
+ * with values 0..n-1 (sortKeyValuePairs, :3-9) and findStartAndEnd (:46-87)
+ * — on arrays the caller holds in device memory, on the context's device
+ * and stream (the context needs no scene).  The grid is the spatialGrid of
+ * dataBase::setSpatialGrids (dataBase.cpp:415-432): bounds origin ..
+ * origin + size, cells[] cells per axis, cell size size / cells, and a
+ * spare cell n_cells - 1 (n_cells = cells[0]*cells[1]*cells[2] + 1) for
+ * points outside the bounds.
+ *   pos                   device, [3n] xyz doubles
+ *   hash                  device out, [n]: each point's cell
+ *   perm                  device out, [n]: sorted position -> point index
+ *                         (ascending cell, ascending index within a cell)
+ *   cell_start, cell_end  device out, [n_cells]: the sorted positions
+ *                         [start, end) of each cell, -1 for empty cells
+ * Asynchronous: the results are complete after dsph_synchronize.  The work
+ * buffer (about 5 words per point) is the context's, grown on demand.    */
+int dsph_hash_sort(dsph_ctx* ctx, const double* pos, int32_t n, const double origin[3],
+                   const double size[3], const int32_t cells[3], int32_t* hash,
+                   int32_t* perm, int32_t* cell_start, int32_t* cell_end);
+
 /* profiling aid: average device time (ms) of the last dsph_step_n run per
  * kernel family measured with HIP events on the context stream; names is a
  * NUL-separated list.  Returns the number of entries written. */

@@ -283,6 +283,8 @@ def test_cpp_dam_break_case_matches_python_path(tmp_path):
         g.step(s, CASE.dt, CASE.gap)
     f, s_, c = g.fluid(), g.solid(), g.clump()
     mine = [f["pos"], f["vel"], s_["pos"], s_["vel"], c["centroid"], c["vel"], c["angvel"]]
-    for a, b in zip(mine, arrays):
-        assert np.array_equal(a, b)
+    names = ["fluid pos", "fluid vel", "solid pos", "solid vel", "clump centroid", "clump vel",
+             "clump angvel"]
+    for name, a, b in zip(names, mine, arrays):
+        assert np.array_equal(a, b), (name, float(np.max(np.abs(a - b))))
     assert os.path.isdir(tmp_path / "damBreakOutput")
