@@ -1273,6 +1273,9 @@ int dsph_create(int device_index, int precision, dsph_ctx** out) {
         delete c;
         return DSPH_E_HIP;
     }
+    // the address of the allocation-free entry points' scratch on this
+    // device (a failure here is reported by those entry points)
+    (void)noalloc_scratch_setup(device_index);
     *out = c;
     return DSPH_OK;
 }

@@ -5,8 +5,10 @@
 // (findStartAndEnd, :46-87), for callers that hold their own particles.
 //
 // The work buffer (sorted keys and the radix sort's ping-pong and histogram
-// buffers) is the context's own, grown on demand.
+// buffers) is the context's own, grown on demand.  dsph_hash_sort_noalloc
+// takes it from a fixed device array instead and allocates nothing.
 #include <algorithm>
+#include <mutex>
 #include <string>
 
 #include "../../include/dsph.h"
@@ -85,7 +87,72 @@ int check_args(dsph_ctx* c, const double* pos, int32_t n, const double* origin, 
     return DSPH_OK;
 }
 
+// ----------------------------------------------- allocation-free variant
+// The scratch of dsph_hash_sort_noalloc: the sorted keys and the radix
+// sort's buffers for up to kNoAllocMaxPoints points, a file-scope device
+// array that every GPU the library runs on holds (its own copy) until the
+// process ends.  Nothing in it needs to start at zero: each call writes the
+// parts it reads (every histogram entry, every scan block sum).  Calls that
+// overlap on one GPU share it and race.
+constexpr int32_t kNoAllocMaxPoints = 1 << 20;
+constexpr int64_t kScratchWords = 6 * (int64_t(1) << 20);  // 24 MiB
+static __device__ __align__(256) unsigned scratch[kScratchWords];
+
+// the address of `scratch` on each device, looked up once per device with
+// that device current; host code passes these addresses, never the name
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned* g_scratch_ptr[kMaxDevices] = {};
+
+unsigned* scratch_on(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+
 }  // namespace
+
+namespace dsph {
+// the scratch's address on `device`, which must be current; called by
+// dsph_create for its device and, failing that, by the entry point
+int noalloc_scratch_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return DSPH_E_NODEVICE;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return DSPH_OK;
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return DSPH_E_HIP;
+    g_scratch_ptr[device] = (unsigned*)p;
+    return DSPH_OK;
+}
+}  // namespace dsph
+
+extern "C" int dsph_hash_sort_noalloc(dsph_ctx* c, const double* pos, int32_t n, const double origin[3],
+                                      const double size[3], const int32_t cells[3], int32_t* hash,
+                                      int32_t* perm, int32_t* cell_start, int32_t* cell_end) {
+    GridP g;
+    const int rc = check_args(c, pos, n, origin, size, cells, hash, perm, cell_start, cell_end, g);
+    if (rc != DSPH_OK) return rc;
+    if (n > kNoAllocMaxPoints || work_words(std::max(n, 1)) > kScratchWords) {
+        ctx_set_error(c, "dsph_hash_sort_noalloc: at most 1048576 points (use dsph_hash_sort)");
+        return DSPH_E_INVALID;
+    }
+    const int dev = ctx_device(c);
+    if (hipSetDevice(dev) != hipSuccess) return DSPH_E_HIP;
+    unsigned* work = scratch_on(dev);
+    if (!work) {
+        if (noalloc_scratch_setup(dev) != DSPH_OK) {
+            ctx_set_error(c, "dsph_hash_sort_noalloc: no address for the device scratch");
+            return DSPH_E_HIP;
+        }
+        work = scratch_on(dev);
+    }
+    hash_sort_on(ctx_stream(c), g, pos, n, work, hash, perm, cell_start, cell_end);
+    if (hipGetLastError() != hipSuccess) {
+        ctx_set_error(c, "dsph_hash_sort_noalloc: kernel launch failed");
+        return DSPH_E_HIP;
+    }
+    return DSPH_OK;
+}
 
 extern "C" int dsph_hash_sort(dsph_ctx* c, const double* pos, int32_t n, const double origin[3],
                               const double size[3], const int32_t cells[3], int32_t* hash, int32_t* perm,

@@ -74,6 +74,7 @@ _SIGS = {
     "dsph_download_grid": [C.c_int, _ip, _ip, _ip, _ip],
     "dsph_status": [_i64p],
     "dsph_hash_sort": [_vp, C.c_int32, _dp, _dp, _ip, _vp, _vp, _vp, _vp],
+    "dsph_hash_sort_noalloc": [_vp, C.c_int32, _dp, _dp, _ip, _vp, _vp, _vp, _vp],
     "dsph_set_timing": [C.c_int],
     "dsph_kernel_times": [C.c_char_p, C.c_int64, _dp, C.c_int32],
 }
@@ -359,12 +360,14 @@ class Solver:
         self._call("dsph_status", _p(w, _i64p))
         return w
 
-    def hash_sort(self, pos_ptr, n, origin, size, cells, hash_ptr, perm_ptr, start_ptr, end_ptr):
-        """dsph_hash_sort on caller device arrays (raw device addresses)."""
+    def hash_sort(self, pos_ptr, n, origin, size, cells, hash_ptr, perm_ptr, start_ptr, end_ptr,
+                  noalloc=False):
+        """dsph_hash_sort (or dsph_hash_sort_noalloc) on caller device arrays
+        (raw device addresses)."""
         o, z = np.asarray(origin, float), np.asarray(size, float)
         g = np.asarray(cells, np.int32)
-        self._call("dsph_hash_sort", _vp(pos_ptr), int(n), _p(o), _p(z), _p(g, _ip), _vp(hash_ptr),
-                   _vp(perm_ptr), _vp(start_ptr), _vp(end_ptr))
+        self._call("dsph_hash_sort_noalloc" if noalloc else "dsph_hash_sort", _vp(pos_ptr), int(n), _p(o),
+                   _p(z), _p(g, _ip), _vp(hash_ptr), _vp(perm_ptr), _vp(start_ptr), _vp(end_ptr))
 
     def set_timing(self, on=True):
         self._call("dsph_set_timing", int(on))

@@ -356,6 +356,18 @@ int dsph_status(dsph_ctx* ctx, int64_t words[DSPH_STATUS_WORDS]);
 int dsph_hash_sort(dsph_ctx* ctx, const double* pos, int32_t n, const double origin[3],
                    const double size[3], const int32_t cells[3], int32_t* hash,
                    int32_t* perm, int32_t* cell_start, int32_t* cell_end);
+/* dsph_hash_sort for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * no reference counterpart.  Same arguments and results; the work buffer is
+ * a fixed 24 MiB device array of the library (each GPU it runs on holds its
+ * own copy until the process ends), so it takes at most 1,048,576 points
+ * (larger n: DSPH_E_INVALID) and calls no allocating routine.  Calls on one
+ * GPU share that array: calls that overlap on one GPU (two streams or
+ * contexts at once) race.                                                  */
+int dsph_hash_sort_noalloc(dsph_ctx* ctx, const double* pos, int32_t n,
+                           const double origin[3], const double size[3],
+                           const int32_t cells[3], int32_t* hash, int32_t* perm,
+                           int32_t* cell_start, int32_t* cell_end);
 
 /* profiling aid: average device time (ms) of the last dsph_step_n run per
  * kernel family measured with HIP events on the context stream; names is a
