@@ -89,6 +89,7 @@ struct dsph_ctx {
     int nfchunks = 0;
     unsigned* hslot = nullptr;  // reference hash per fluid slot
     unsigned char* cspare = nullptr;  // per cell: its stencil reaches the spare cell
+    int* near_out = nullptr;          // per cell: near an out-of-domain fluid particle (this step)
     // the static solids' grid (wall dummies never move: sorted once, at
     // upload); with a moving dummy every search sorts all solids instead
     // the solids' hashes are set for all of them (the static ones keep
@@ -96,8 +97,6 @@ struct dsph_ctx {
     bool skey_valid = false;
     unsigned long long* dispmax = nullptr;  // per-workgroup fluid displacement maxima
     int* rank_acc = nullptr;                // active-grid rank accumulators (zero between sorts)
-    int* cs_cnt = nullptr;                  // fluid sort: bucket counts (zero between sorts)
-    int* cs_off = nullptr;                  // and bucket offsets
     int* tsum = nullptr;                    // fluid–solid count per tile of 256 slots
     unsigned long long static_out = 0;
     int* dcs = nullptr;
@@ -369,6 +368,7 @@ std::vector<StateArr> state_arrays(dsph_ctx* c) {
         add(p, nf, 4);
     add(c->fcs, ncell, 4);
     add(c->fce, ncell, 4);
+    add(c->near_out, ncell, 4);
     add(c->fxb, nf, 32);
     add(c->fsp, nf, 32);
     add(c->sxb, ns, 32);
@@ -587,6 +587,7 @@ SearchArgs search_args(dsph_ctx* c) {
     a.forig = c->fset[0].orig;
     a.hslot = c->hslot;
     a.cspare = c->cspare;
+    a.near_out = c->near_out;
     if (c->all_solid_sort) {
         a.scs = c->scs;
         a.sce = c->sce;
@@ -633,7 +634,8 @@ void neighbor_search(dsph_ctx* c, int i_step, int gap, bool save_history, bool z
         FamTimer t(c, F_GRID);
         const Grids G = grids_view(c);
         hash_fluid(s, G, c->fset[0].ph, c->fset[0].orig, c->nf, c->sub_bits, c->crank, c->cspare, c->fkey,
-                   c->frank, c->hslot, fluid_step ? c->fxb : nullptr, c->dispmax, c->ctl, stw);
+                   c->frank, c->hslot, fluid_step ? c->fxb : nullptr, c->dispmax,
+                   fluid_step ? c->near_out : nullptr, c->ncell, c->ctl, stw);
         bool walk_only = false;
         for (const GridP& g : c->grids) walk_only = walk_only || g.walk_only;
         // only the live solids move: after one full pass the others' hashes
@@ -645,10 +647,13 @@ void neighbor_search(dsph_ctx* c, int i_step, int gap, bool save_history, bool z
         c->skey_valid = true;
         if (fluid_step) {
             const Gate gate = c->ctl + CTL_SORT;
-            // by cell rank (the bucket), then sub-cell code and index; the
-            // cell ranges come with it
-            counting_sort(s, c->frank, c->nf, c->sub_bits, c->ncell, c->cs_cnt, c->cs_off, c->scan_tmp,
-                          c->radix_tmp, c->fkey_sorted, c->fperm, c->cinv, c->fcs, c->fce, gate);
+            // by cell rank, then sub-cell code, then index: a stable LSD radix
+            // sort of the keys (linear in the particle count however full a
+            // cell or the spare cell is); the cell ranges come from the
+            // sorted keys
+            radix_sort_iota(s, c->frank, c->nf, c->key_bits + c->sub_bits, c->fkey_sorted, c->fperm,
+                            c->radix_tmp, gate);
+            cell_spans(s, c->fkey_sorted, c->nf, c->fcs, c->fce, c->ncell, c->cinv, gate, c->sub_bits);
             if (c->all_solid_sort) {
                 if (c->ns > 0)
                     radix_sort_iota(s, c->skey, c->ns, c->key_bits, c->skey_sorted, c->sperm, c->radix_tmp,
@@ -1612,9 +1617,8 @@ int dsph_upload_batch(dsph_ctx* c, const dsph_scene* scenes, int32_t n_scenes) {
             }
             c->cspare = dalloc<unsigned char>(c, c->ncell);
             h2d(c, c->cspare, sp.data(), c->ncell);
-            c->cs_cnt = dalloc<int>(c, c->ncell);
-            c->cs_off = dalloc<int>(c, (size_t)c->ncell + 1);
-            fill_int(c->st, c->cs_cnt, c->ncell, 0);
+            c->near_out = dalloc<int>(c, c->ncell);
+            fill_int(c->st, c->near_out, c->ncell, 0);
             c->crank = dalloc<unsigned>(c, c->ncell);
             c->cinv = dalloc<unsigned>(c, c->ncell);
             h2d(c, c->crank, rank.data(), c->ncell);

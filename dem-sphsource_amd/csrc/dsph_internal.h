@@ -48,12 +48,10 @@ enum {
     CTL_DISP_S = 1,   // max solid displacement (f64 bits)
     CTL_FORCE = 2,    // host request: rebuild at the next search
     CTL_REBUILD = 3,  // this search rebuilds the superset
-    CTL_WALK_ALL = 4, // some particle is out of the domain: walk the stencil exactly
-    CTL_OUT_AT_BUILD = 5,  // the last build had particles out of the domain
-    CTL_NEAR_SPARE = 6,    // some fluid particle's stencil reaches the spare cell
-    CTL_SORT = 7,          // this search sorts the grids (rebuild or exact walks)
-    CTL_TICKET = 8,        // blocks of the solid hash done (the last one decides)
-    CTL_WORDS = 9
+    CTL_NEAR_SPARE = 4,    // some fluid particle's stencil reaches the spare cell, or it is in it
+    CTL_SORT = 5,          // this search sorts the grids (rebuild or exact walks)
+    CTL_TICKET = 6,        // blocks of the solid hash done (the last one decides)
+    CTL_WORDS = 7
 };
 
 // the reference's spatialGrid (deviceStructs.h:976-1014) of one scene.  A
@@ -271,13 +269,6 @@ void scan_keys(hipStream_t s, const int* keys, int nkeys, int* cnt, int* cnt_out
                int n_all);
 // lists at most this long are scanned by scan_keys (one workgroup)
 constexpr int kKeyScanMax = 16384;
-// stable sort of (key, index) pairs by counting into buckets key >> shift
-// (< nbuckets), then ranking within each bucket; cnt: nbuckets zeroed ints,
-// zero again afterwards; off: nbuckets + 1; tmp: 2 n words.  With
-// start/end, the range of bucket b is stored under inv[b] (or b).
-void counting_sort(hipStream_t s, const unsigned* keys, int n, int shift, int nbuckets, int* cnt, int* off,
-                   int* scan_tmp, unsigned* tmp, unsigned* keys_out, int* perm_out, const unsigned* inv,
-                   int* start, int* end, Gate gate = nullptr);
 // stable LSD radix sort of (key, index) pairs; keys < 2^bits; vals out = perm
 int64_t radix_tmp_size(int64_t n);
 void radix_sort_iota(hipStream_t s, const unsigned* keys_in, int64_t n, int bits,
@@ -300,6 +291,7 @@ struct SearchArgs {
     const int* forig;       // caller index of each fluid slot
     const unsigned* hslot;  // reference hash of each fluid slot
     const unsigned char* cspare;  // per cell: its 27-offset stencil reaches the spare cell
+    const int* near_out;    // per cell: within one cell of an out-of-domain fluid particle (this step)
     // the solids the fluid sees, as two grids: the static solids (wall
     // dummies, sorted once) and the active grid of the non-dummies (sorted
     // every step for the DEM search); without the second (acs null) the
@@ -331,10 +323,14 @@ struct SearchArgs {
 // The fluid sort key: the cell's Morton rank above the Morton code of the
 // particle's sub-cell (sub_bits / 3 bits per axis), so a cell's members are
 // contiguous and themselves in z-order.
+// With near_out (fluid steps), the cells within one cell of each fluid
+// particle out of the domain are marked (the array is cleared first); a
+// particle whose build position was out of the domain and that is back in
+// it requests a rebuild.
 void hash_fluid(hipStream_t s, const Grids& G, const double4* ph, const int* orig, int n, int sub_bits,
                 const unsigned* crank, const unsigned char* cspare, unsigned* key_orig,
                 unsigned* rank_orig, unsigned* hslot, const double4* fxb, unsigned long long* bmax,
-                unsigned long long* ctl, long long* status);
+                int* near_out, int ncell, unsigned long long* ctl, long long* status);
 // calculatePointHash of n caller points (interleaved xyz doubles) on one
 // grid (a scene-local grid: cell0 = 0), into key
 void hash_points(hipStream_t s, const GridP& g, const double* xyz, int n, unsigned* key);

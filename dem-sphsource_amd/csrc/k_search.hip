@@ -153,6 +153,25 @@ __device__ __forceinline__ unsigned long long block_max(unsigned long long m) {
     return m;
 }
 
+// Cells within one cell of an out-of-domain particle's position (clamped to
+// the grid): every fluid particle within 2 max(h) <= one cell of it lies in
+// them, and only those can pair with it under the reference's asymmetric
+// spare-cell rule, so only their rows are walked on the reference stencil.
+__device__ __forceinline__ void mark_near(const GridP& g, const double4& p, int* __restrict__ near_out) {
+    const double tx = (p.x - g.minx) / g.csx, ty = (p.y - g.miny) / g.csy, tz = (p.z - g.minz) / g.csz;
+    if (!(tx == tx && ty == ty && tz == tz)) return;  // a NaN position pairs with nothing
+    const int cx = (int)fmin(fmax(floor(tx), 0.0), (double)(g.gx - 1));
+    const int cy = (int)fmin(fmax(floor(ty), 0.0), (double)(g.gy - 1));
+    const int cz = (int)fmin(fmax(floor(tz), 0.0), (double)(g.gz - 1));
+    for (int z = max(cz - 1, 0); z <= min(cz + 1, g.gz - 1); z++)
+        for (int y = max(cy - 1, 0); y <= min(cy + 1, g.gy - 1); y++)
+            for (int x = max(cx - 1, 0); x <= min(cx + 1, g.gx - 1); x++)
+                near_out[g.cell0 + (z * g.gy + y) * g.gx + x] = 1;
+}
+
+// near_out (fluid steps): marks the cells near out-of-domain particles; a
+// particle whose build position (xb) was out of the domain and that is back
+// in it is in no superset row and requests a rebuild
 __global__ void __launch_bounds__(kBS) k_hash_fluid(Grids G, const double4* __restrict__ ph,
                                                     const int* __restrict__ orig, int n, int sub_bits,
                                                     const unsigned* __restrict__ crank,
@@ -161,9 +180,10 @@ __global__ void __launch_bounds__(kBS) k_hash_fluid(Grids G, const double4* __re
                                                     unsigned* __restrict__ hslot,
                                                     const double4* __restrict__ xb,
                                                     unsigned long long* __restrict__ bmax,
+                                                    int* __restrict__ near_out,
                                                     unsigned long long* ctl, unsigned long long* st) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
-    bool out = false, near = false;
+    bool out = false, near = false, back = false;
     double d2 = 0.0;
     if (s < n) {
         const GridP& g = grid_at(G, fluid_scene(G, s));
@@ -177,11 +197,17 @@ __global__ void __launch_bounds__(kBS) k_hash_fluid(Grids G, const double4* __re
         rank[o] = (crank[h] << sub_bits) | sub;
         hslot[s] = (unsigned)h;
         near = cspare[h] != 0;
-        if (xb) d2 = disp2(g, p, xb[s]);
+        if (xb) {
+            const double4 b = xb[s];
+            d2 = disp2(g, p, b);
+            back = !out && point_hash(g, b.x, b.y, b.z) == g.ncell - 1;
+        }
+        if (out && near_out) mark_near(g, p, near_out);
     }
     const unsigned long long m = __ballot(out);
     if ((threadIdx.x & 63) == 0 && m) atomicAdd(&st[ST_OUT_OF_DOMAIN], (unsigned long long)__popcll(m));
     if ((threadIdx.x & 63) == 0 && __ballot(near) != 0) atomicOr(&ctl[CTL_NEAR_SPARE], 1ull);
+    if ((threadIdx.x & 63) == 0 && __ballot(back) != 0) atomicOr(&ctl[CTL_FORCE], 1ull);
     // one maximum per workgroup, reduced by the decision (no contended
     // atomic on one word)
     if (xb) {
@@ -220,19 +246,17 @@ __global__ void k_hash_solid(Grids G, const double4* __restrict__ pr, int n,
 __device__ void decide_rebuild(unsigned long long* ctl, unsigned long long* st, int always, int always_sort) {
     const double df = sqrt(__longlong_as_double((long long)ctl[CTL_DISP_F]));
     const double ds = sqrt(__longlong_as_double((long long)ctl[CTL_DISP_S]));
-    const bool out = st[ST_OUT_OF_DOMAIN] != 0;
     const bool move = df + (df > ds ? df : ds) >= 1.0 - 1e-9;
-    // out-of-domain particles are missing from a superset built while they
-    // are out: rebuild until a build sees every particle inside
-    const bool stale = ctl[CTL_OUT_AT_BUILD] != 0;
-    const unsigned long long r =
-        (ctl[CTL_FORCE] != 0 || out || stale || always || move) ? 1ull : 0ull;
+    // out-of-domain particles are in no superset row (a build finds
+    // particles by position): the rows near them are walked exactly
+    // instead, and one that comes back in requests a rebuild (CTL_FORCE)
+    const unsigned long long r = (ctl[CTL_FORCE] != 0 || always || move) ? 1ull : 0ull;
     ctl[CTL_REBUILD] = r;
-    ctl[CTL_WALK_ALL] = out ? 1ull : 0ull;
-    // the grids are sorted for a build and for exact stencil walks
+    // the grids are sorted for a build and for exact stencil walks (rows of
+    // cells whose stencil reaches the spare cell, and rows near particles
+    // in it: a fluid particle out of the domain raises CTL_NEAR_SPARE too)
     ctl[CTL_SORT] = (r || ctl[CTL_NEAR_SPARE] != 0 || always_sort) ? 1ull : 0ull;
     ctl[CTL_NEAR_SPARE] = 0;
-    if (r) ctl[CTL_OUT_AT_BUILD] = out ? 1ull : 0ull;
     ctl[CTL_FORCE] = 0;
     ctl[CTL_DISP_F] = 0;
     ctl[CTL_DISP_S] = 0;
@@ -249,7 +273,7 @@ __global__ void __launch_bounds__(kBS) k_hash_live(Grids G, const double4* __res
                                                    int always, int always_sort, unsigned long long* ctl,
                                                    unsigned long long* st) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    bool out = false;
+    bool out = false, back = false;
     double d2 = 0.0;
     if (t < n) {
         const int i = live[t];
@@ -258,11 +282,18 @@ __global__ void __launch_bounds__(kBS) k_hash_live(Grids G, const double4* __res
         const int hl = point_hash(g, p.x, p.y, p.z);
         key[i] = (unsigned)(g.cell0 + hl);
         out = hl == g.ncell - 1;
-        if (decide) d2 = disp2(g, p, xb[i]);
+        if (decide) {
+            // a solid out of the domain at the build is in no fluid row's
+            // solid superset: back in, it requests a rebuild
+            const double4 b = xb[i];
+            d2 = disp2(g, p, b);
+            back = !out && point_hash(g, b.x, b.y, b.z) == g.ncell - 1;
+        }
     }
     const unsigned long long m = __ballot(out);
     if ((threadIdx.x & 63) == 0 && m) atomicAdd(&st[ST_OUT_OF_DOMAIN], (unsigned long long)__popcll(m));
     if (t == 0 && static_out) atomicAdd(&st[ST_OUT_OF_DOMAIN], static_out);
+    if ((threadIdx.x & 63) == 0 && __ballot(back) != 0) atomicOr(&ctl[CTL_FORCE], 1ull);
     if (!decide) return;
     const unsigned long long ms = block_max(dbits(d2));
     if (threadIdx.x == 0 && ms) atomicMax(&ctl[CTL_DISP_S], ms);
@@ -699,8 +730,8 @@ __global__ void __launch_bounds__(kBS) k_filter_count(SearchArgs a) {
         const double4 p = a.ph[s];
         const GridP& g = grid_at(a.G, fluid_scene(a.G, s));
         const unsigned spare = (unsigned)(g.cell0 + g.ncell - 1);  // the scene's spare cell
-        const bool walk =
-            a.ctl[CTL_WALK_ALL] != 0 || g.walk_only || a.hslot[s] == spare || a.cspare[a.hslot[s]] != 0;
+        const unsigned hs = a.hslot[s];
+        const bool walk = g.walk_only || hs == spare || a.cspare[hs] != 0 || a.near_out[hs] != 0;
         unsigned long long m0 = 0, m1 = 0;
         if (walk) {
             int64_t w0 = 0;
@@ -711,20 +742,25 @@ __global__ void __launch_bounds__(kBS) k_filter_count(SearchArgs a) {
             const int last = n2 > 0 ? n2 - 1 : 0;
             // candidates in batches of 8: indices, then positions, all in
             // flight together (entries past the row end re-read its last one
-            // and are not used).  No candidate is in the spare cell here: a
-            // particle or solid out of the domain makes every row walked
-            // (CTL_WALK_ALL), so only the distance decides.
+            // and are not used).  A row not walked has no stencil cell that
+            // is the spare cell, so a solid that has left the domain since
+            // the build is not its pair; otherwise the distance decides.
             constexpr int U = 8;
             for (int k = 0; k < n2; k += U) {
                 int ai[U];
                 double4 q[U];
+                unsigned sk[U];
 #pragma unroll
                 for (int u = 0; u < U; u++) ai[u] = fsup[(int64_t)min(k + u, last) * a.nf];
 #pragma unroll
-                for (int u = 0; u < U; u++) q[u] = a.spr[ai[u]];
+                for (int u = 0; u < U; u++) {
+                    q[u] = a.spr[ai[u]];
+                    sk[u] = a.skey[ai[u]];
+                }
 #pragma unroll
                 for (int u = 0; u < U; u++) {
-                    const bool in = (k + u < n2) & within(p.x, p.y, p.z, q[u].x, q[u].y, q[u].z, 2.0 * p.w);
+                    const bool in = (k + u < n2) & (sk[u] != spare) &
+                                    within(p.x, p.y, p.z, q[u].x, q[u].y, q[u].z, 2.0 * p.w);
                     cs += in;
                     const int kk = k + u;
                     const unsigned long long bit = in ? 1ull << (kk & 63) : 0ull;
@@ -1200,12 +1236,14 @@ int cap_grid(int64_t n, int cap) {
 // ---------------------------------------------------------------- wrappers
 void hash_fluid(hipStream_t s, const Grids& G, const double4* ph, const int* orig, int n, int sub_bits,
                 const unsigned* crank, const unsigned char* cspare, unsigned* key, unsigned* rank,
-                unsigned* hslot, const double4* fxb, unsigned long long* bmax, unsigned long long* ctl,
-                long long* status) {
+                unsigned* hslot, const double4* fxb, unsigned long long* bmax, int* near_out, int ncell,
+                unsigned long long* ctl, long long* status) {
     hipLaunchKernelGGL(k_status_reset, dim3(1), dim3(ST_WORDS), 0, s, (unsigned long long*)status);
+    if (near_out) fill_int(s, near_out, ncell, 0);
     if (n > 0)
         hipLaunchKernelGGL(k_hash_fluid, dim3(blocks_for(n)), dim3(kBS), 0, s, G, ph, orig, n, sub_bits,
-                           crank, cspare, key, rank, hslot, fxb, bmax, ctl, (unsigned long long*)status);
+                           crank, cspare, key, rank, hslot, fxb, bmax, near_out, ctl,
+                           (unsigned long long*)status);
 }
 
 void hash_points(hipStream_t s, const GridP& g, const double* xyz, int n, unsigned* key) {

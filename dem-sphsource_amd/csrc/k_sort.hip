@@ -204,60 +204,6 @@ __global__ void __launch_bounds__(kKeyBS) k_scan_keys(const int* __restrict__ ke
     if (threadIdx.x == 0) off[n_all] = carry;
 }
 
-// Counting sort by bucket (the key's high bits, key >> shift): count, scan,
-// scatter into the buckets in any order (each bucket's slots handed out
-// from its end by decrementing its count, which leaves the counts zero for
-// the next sort), then every element moves to its place in its bucket: its
-// rank by (key, index) among the bucket's elements.  The result is the
-// stable sort of the keys, as a radix sort gives it, in a fixed number of
-// launches.
-__global__ void __launch_bounds__(kBS) k_cs_count(const unsigned* __restrict__ key, int n, int shift,
-                                                  int* __restrict__ cnt, Gate gate) {
-    if (gate && *gate == 0) return;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) atomicAdd(&cnt[key[t] >> shift], 1);
-}
-__global__ void __launch_bounds__(kBS) k_cs_scatter(const unsigned* __restrict__ key, int n, int shift,
-                                                    const int* __restrict__ off, int* __restrict__ cnt,
-                                                    unsigned* __restrict__ ktmp, int* __restrict__ vtmp,
-                                                    Gate gate) {
-    if (gate && *gate == 0) return;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const unsigned k = key[t];
-    const int b = (int)(k >> shift);
-    const int pos = off[b] + atomicSub(&cnt[b], 1) - 1;
-    ktmp[pos] = k;
-    vtmp[pos] = t;
-}
-// thread t < n: element t of the scattered order to its sorted place;
-// thread t < nb: with start/end, the range of bucket t under its cell inv[t]
-__global__ void __launch_bounds__(kBS) k_cs_place(const int* __restrict__ off, int nb, int n, int shift,
-                                                  const unsigned* __restrict__ ktmp, const int* __restrict__ vtmp,
-                                                  unsigned* __restrict__ kout, int* __restrict__ vout,
-                                                  const unsigned* __restrict__ inv, int* __restrict__ start,
-                                                  int* __restrict__ end, Gate gate) {
-    if (gate && *gate == 0) return;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (start && t < nb) {
-        const unsigned cell = inv ? inv[t] : (unsigned)t;
-        start[cell] = off[t];
-        end[cell] = off[t + 1];
-    }
-    if (t >= n) return;
-    const unsigned k = ktmp[t];
-    const int v = vtmp[t];
-    const int b = (int)(k >> shift);
-    const int lo = off[b], hi = off[b + 1];
-    int r = 0;
-    for (int q = lo; q < hi; q++) {
-        const unsigned kq = ktmp[q];
-        r += (kq < k) || (kq == k && vtmp[q] < v);
-    }
-    kout[lo + r] = k;
-    vout[lo + r] = v;
-}
-
 }  // namespace
 
 void fill_int(hipStream_t s, int* p, int64_t n, int v, Gate gate) {
@@ -306,21 +252,6 @@ void exclusive_scan(hipStream_t s, const int* in, int* out, int64_t n, int* tmp,
     exclusive_scan(s, bsum, boff, nb, tmp + 2 * nb + 2, gate);
     hipLaunchKernelGGL(k_scan_final, dim3((unsigned)nb), dim3(kBS), 0, s, in, n, (const int*)boff, out,
                        gate);
-}
-
-void counting_sort(hipStream_t s, const unsigned* keys, int n, int shift, int nbuckets, int* cnt, int* off,
-                   int* scan_tmp, unsigned* tmp, unsigned* keys_out, int* perm_out, const unsigned* inv,
-                   int* start, int* end, Gate gate) {
-    if (n <= 0) return;
-    unsigned* ktmp = tmp;
-    int* vtmp = (int*)(tmp + n);
-    hipLaunchKernelGGL(k_cs_count, dim3(blocks_for(n)), dim3(kBS), 0, s, keys, n, shift, cnt, gate);
-    exclusive_scan(s, cnt, off, nbuckets, scan_tmp, gate);
-    hipLaunchKernelGGL(k_cs_scatter, dim3(blocks_for(n)), dim3(kBS), 0, s, keys, n, shift, (const int*)off, cnt,
-                       ktmp, vtmp, gate);
-    hipLaunchKernelGGL(k_cs_place, dim3(blocks_for(std::max(n, nbuckets))), dim3(kBS), 0, s, (const int*)off,
-                       nbuckets, n, shift, (const unsigned*)ktmp, (const int*)vtmp, keys_out, perm_out, inv, start,
-                       end, gate);
 }
 
 int64_t radix_tmp_size(int64_t n) {

@@ -635,3 +635,47 @@ def test_autotune_returns_a_configuration_of_its_space():
     g.initialize()
     g.run(5, sc.dt, graphs=True)
     assert np.all(np.isfinite(g.fluid()["pos"]))
+
+
+@pytest.mark.parametrize("first", [False, True], ids=["last", "first"])
+def test_particle_staying_out_of_the_domain(first):
+    # one fluid particle above the open top, moving up: it stays in the
+    # spare cell.  Only the rows near it are walked on the reference
+    # stencil; the superset is not rebuilt every step for it, and the
+    # lists stay those of the oracle.  Then a second particle leaves
+    # through the top.
+    dx = 0.005
+    sc = scenes.empty_scene((-3 * dx,) * 3, (0.04 + 6 * dx, 0.03 + 6 * dx, 0.03 + 6 * dx))
+    top = -3 * dx + 0.03 + 6 * dx
+    pts = scenes.regular_packed_points((0, 0, top - 0.02 - 2 * dx), (0.04, 0.03, 0.02), dx)
+    up = [[0.012, 0.011, top + 0.4 * dx]]
+    leave = [[0.03, 0.02, top - 0.3 * dx]]      # just under the top, leaving
+    parts = [up, pts, leave] if first else [pts, leave, up]
+    for q in parts:
+        scenes.add_fluid(sc, q, 1.3 * dx, 1000.0, 20.0, 1e-6)
+    k_up = 0 if first else sc.n_fluid - 1
+    k_leave = sc.n_fluid - 1 if first else sc.n_fluid - 2
+    vel = np.zeros((sc.n_fluid, 3))
+    vel[k_up] = [0.0, 0.0, 0.5]
+    vel[k_leave] = [0.0, 0.0, 1.0]
+    sc.fluid_vel = vel
+    sc.gravity = np.zeros(3)
+    dt = 2e-4
+    g, o = make_pair(sc, dsph.PREC_FP64)
+    g.set_skin(0.3)
+    rb0 = int(g.status()[dsph.ST_REBUILDS])
+    outs = []
+    for s in range(1, 31):
+        g.step(s, dt)
+        o.step(s, dt)
+        outs.append(int(g.status()[dsph.ST_OUT_OF_DOMAIN]))
+        for kind in (dsph.LIST_FLUID_FLUID, dsph.LIST_FLUID_SOLID):
+            ga, gb, _ = g.list(kind)
+            oa, ob, _ = o.list(kind)
+            assert np.array_equal(ga, oa) and np.array_equal(gb, ob), (s, kind)
+        gf, of = g.fluid(), o.fluid()
+        for f, floor in (("pos", 0.0), ("vel", 1.0), ("drho", 1.0)):
+            assert refmodel.rel_err(gf[f], of[f], floor) < TOL_FP64, (s, f)
+    assert min(outs) >= 1 and max(outs) == 2
+    rebuilds = int(g.status()[dsph.ST_REBUILDS]) - rb0
+    assert rebuilds < 10, rebuilds
