@@ -1,20 +1,25 @@
 """Benchmark: particle-steps/s of the coupled DEM-SPH dam break on one MI355X.
 
 Workload: the coupled dam break at dx = 0.005 m — 512,000 fluid particles,
-452,376 wall dummy particles and 2,000 DEM spheres (Hertz–Mindlin contact,
+452,376 wall dummy particles and 2,000 DEM spheres (Hertz-Mindlin contact,
 tank planes), synthetic and deterministic.  One step = the whole per-timestep
 hot path (neighbour search, solid half-kick/drift, fluid integration with the
 Riemann WCSPH pair loops, DEM contact forces, solid half-drift/kick), all
 inputs resident in HBM.
 
-Prints one JSON line.  Besides the headline (mixed precision) it reports the
-all-FP64 GPU path on the same workload, the HBM roofline fraction of the
-dominant kernel (from HIP-event kernel times taken inside the timed region)
-and a CPU baseline (the FP64 OpenMP oracle on a bounded sample of steps).
+Prints one JSON line.  Timed: --steps steps replayed from HIP graphs after
+--warmup steps, for the mixed path (`value`) and, over the same step window,
+for the all-FP64 path.  Then the same window runs again from a checkpoint
+of its first step, eagerly, with HIP events around every kernel family on
+the context stream: those durations give the dominant kernel's roofline.
+A 1000-step window of both precisions (superset rebuilds amortised) and a
+CPU baseline (the FP64 OpenMP oracle on a bounded sample) are reported
+beside it.
 """
 import argparse
 import json
 import os
+import statistics
 import sys
 import time
 
@@ -26,22 +31,17 @@ import numpy as np  # noqa: E402
 METRIC = "particle-steps/s at 1M SPH+DEM; % HBM roofline; FP32-mixed vs FP64 speedup"
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 
-# Algorithmic bytes of the fluid pair loops per launch: every per-particle
-# field the kernel uses read once, every output written once, plus the 4-byte
-# neighbour index of each directed pair (neighbour fields served from cache
-# are not counted).  Field bytes per fluid particle:
-#   continuity: read pos+h (32, f64x4), v+rho0 (16 f32x4 | 32 f64x4),
-#               dRho (4 f32 | 8 f64), c,nu,m,h (16 f32x4 | c,nu 16 f64x2),
-#               list count (4) and fluid-solid offset (4);
-#               write dRho f64 (8) [+ f32 shadow (4)]
-#   momentum:   the same reads; write acceleration f64x4 (32) and the
-#               fluid->solid force f64x4 (32) per fluid-solid pair
-PAIR_BYTES = {
-    "continuity": {"mixed": (32 + 16 + 4 + 16 + 4 + 4, 8 + 4, 0),
-                   "fp64": (32 + 32 + 8 + 16 + 4 + 4, 8, 0)},
-    "momentum": {"mixed": (32 + 16 + 4 + 16 + 4 + 4, 32, 32),
-                 "fp64": (32 + 32 + 8 + 16 + 4 + 4, 32, 32)},
-}
+# Algorithmic bytes of the fluid pair loops per launch (the survey's
+# measurement convention): every per-particle field of the reference's
+# dataflow read once and every output written once, per fluid particle —
+#   continuity (K8, per pass): position, velocity, h, rho0, dRho, c in and
+#     dRho out: 88 B FP64, 44 B with the FP32 shadows of the mixed path;
+#   momentum (K9): those plus nu in and the acceleration out: 112 B / 56 B;
+# plus the 4-byte index of every directed fluid-fluid pair and every
+# fluid-solid pair the pass visits, and for momentum the 24-byte force of
+# every fluid-solid pair (calFluid2SolidForce's output, fused into it).
+FIELD_BYTES = {"continuity": {"mixed": 44, "fp64": 88}, "momentum": {"mixed": 56, "fp64": 112}}
+FS_FORCE_BYTES = 24
 
 
 def build_scene(dx):
@@ -49,46 +49,47 @@ def build_scene(dx):
     return scenes.coupled_dam_break(dx=dx)
 
 
-def run_gpu(scene, precision, steps, warmup, timing_steps, eager=False, flags=0, tuning=None):
-    """Times `steps` steps after `warmup` (HIP-graph replay unless eager), then
-    runs `timing_steps` more steps eagerly with per-family HIP events on the
-    context stream (events cannot live inside a captured graph; kernel
-    durations do not depend on how the kernels were launched).  `scene` may
-    be a list of scenes: they are uploaded as one batch and step with the
-    smallest of their time steps."""
+def new_solver(scene, precision, flags=0, tuning=None):
     import dsph
     g = dsph.Solver(precision)
     g.set_flags(flags)
     if isinstance(scene, list):
         g.upload_batch(scene)
-        dt = min(sc.dt for sc in scene)
     else:
         g.upload(scene)
-        dt = scene.dt
     if tuning:
         g.apply_tuning(tuning)
     g.initialize()
-    i = 1
+    return g
+
+
+def timed_window(scene, precision, steps, warmup, timing=True, flags=0, tuning=None):
+    """Graph-replayed steps warmup+1 .. warmup+steps on a fresh context:
+    (seconds, superset rebuilds in the window, kernel-family times).  The
+    kernel times come from the same window run again from a checkpoint of
+    its first step, eagerly, with HIP events on the context stream."""
+    import dsph
+    dt = min(sc.dt for sc in scene) if isinstance(scene, list) else scene.dt
+    g = new_solver(scene, precision, flags, tuning)
     if warmup:
-        g.run(warmup, dt, first=i, graphs=not eager)
-        i += warmup
+        g.run(warmup, dt, first=1, graphs=True)
+    blob = g.save_state() if timing else None
     g.synchronize()
-    rb0 = int(g.status()[8])
+    rb0 = int(g.status()[dsph.ST_REBUILDS])
     t0 = time.perf_counter()
-    # with graphs the run ends with a status read (device sync)
-    g.run(steps, dt, first=i, graphs=not eager)
+    g.run(steps, dt, first=warmup + 1, graphs=True)
     g.synchronize()
-    t1 = time.perf_counter()
-    g.rebuilds_timed = int(g.status()[8]) - rb0
-    i += steps
+    secs = time.perf_counter() - t0
+    rebuilds = int(g.status()[dsph.ST_REBUILDS]) - rb0
     times = {}
-    if timing_steps:
+    if timing:
+        g.load_state(blob)
         g.set_timing(True)
-        g.run(timing_steps, dt, first=i, graphs=False)
+        g.run(steps, dt, first=warmup + 1, graphs=False)
         g.synchronize()
         times = g.kernel_times()
         g.set_timing(False)
-    return g, t1 - t0, times
+    return g, secs, rebuilds, times
 
 
 def pair_counts(g):
@@ -104,8 +105,9 @@ def roofline(times, prec_key, nf, ff_directed, fs_pairs):
         if fam not in times or times[fam][1] == 0:
             continue
         tot_ms, launches = times[fam]
-        rd, wr, per_fs = PAIR_BYTES[fam][prec_key]
-        nbytes = nf * (rd + wr) + 4 * (ff_directed + fs_pairs) + per_fs * fs_pairs
+        nbytes = nf * FIELD_BYTES[fam][prec_key] + 4 * (ff_directed + fs_pairs)
+        if fam == "momentum":
+            nbytes += FS_FORCE_BYTES * fs_pairs
         avg_s = tot_ms / launches / 1e3
         gbs = nbytes / avg_s / 1e9
         cand = {"kernel": fam, "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
@@ -155,26 +157,43 @@ def copy_probe(mib):
                       "GBps": round(8 * n / dt / 1e9, 1)}), flush=True)
 
 
-def cpu_baseline(scene, budget_s, max_steps):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(scene, sample_steps, runs):
+    """The FP64 OpenMP oracle on the box's host cores: one warm-up step after
+    its initial search, then `runs` timed samples of `sample_steps` steps
+    each; the median rate."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     o = oracle.Oracle(threads=threads)
     o.upload(scene)
     o.initialize()
-    n = 0
-    t0 = time.perf_counter()
-    while n < max_steps:
-        o.step(n + 1, scene.dt)
-        n += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
-    dt = time.perf_counter() - t0
+    step = 1
+    o.step(step, scene.dt)
+    rates = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        for _ in range(sample_steps):
+            step += 1
+            o.step(step, scene.dt)
+        rates.append(scene.n_particles * sample_steps / (time.perf_counter() - t0))
     o.close()
-    return {"value": round(scene.n_particles * n / dt, 1), "unit": "particle-steps/s",
-            "cores": threads, "kind": "port",
-            "sample": f"FP64 OpenMP oracle, first {n} steps of the same workload after its "
-                      f"initial search ({dt:.1f} s)"}
+    med = statistics.median(rates)
+    return {"value": round(med, 1), "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "cpu": cpu_model(),
+            "sample": f"FP64 OpenMP oracle on the same workload: 1 warm-up step after the initial "
+                      f"search, then the median of {runs} runs of {sample_steps} steps "
+                      f"(rates {[round(r) for r in rates]})"}
 
 
 def tune(work, precision, flags, args):
@@ -186,8 +205,8 @@ def tune(work, precision, flags, args):
         return cfg.get("fp64" if precision == dsph.PREC_FP64 else "mixed", cfg)
     if args.no_autotune:
         return None
-    import dsph
-    cfg, trials = dsph.autotune(work, precision, flags, trial_steps=30, warmup=5)
+    cfg, trials = dsph.autotune(work, precision, flags, trial_steps=40, warmup=5,
+                                advance=args.tune_advance)
     print(f"# autotuned ({'mixed' if precision == dsph.PREC_MIXED else 'fp64'}): {cfg}",
           file=sys.stderr, flush=True)
     return cfg
@@ -214,23 +233,20 @@ def other_workload(args):
         work, scs = sc, [sc]
     npart = sum(sc.n_particles for sc in scs)
     print(f"# workload: {desc}", file=sys.stderr, flush=True)
-    tsteps = 0 if args.no_timing else args.timing_steps
     tun = tune(work, dsph.PREC_MIXED, flags, args)
-    g, secs, times = run_gpu(work, dsph.PREC_MIXED, args.steps, args.warmup, tsteps, args.eager, flags,
-                             tun)
+    g, secs, rb, times = timed_window(work, dsph.PREC_MIXED, args.steps, args.warmup,
+                                      not args.no_timing, flags, tun)
+    g.close()
     value = npart * args.steps / secs
     print(f"# mixed: {secs / args.steps * 1e3:.3f} ms/step  {value:.4g} particle-steps/s",
           file=sys.stderr, flush=True)
-    g.close()
     fp64 = None
-    if not args.no_fp64 and args.fp64_steps > 0:
+    if not args.no_fp64:
         tun64 = tune(work, dsph.PREC_FP64, flags, args)
-        g64, s64, _ = run_gpu(work, dsph.PREC_FP64, args.fp64_steps, args.warmup, 0, args.eager, flags,
-                              tun64)
-        fp64 = {"value": round(npart * args.fp64_steps / s64, 1),
-                "ms_per_step": round(s64 / args.fp64_steps * 1e3, 4), "steps": args.fp64_steps,
-                "tuning": tun64}
+        g64, s64, rb64, _ = timed_window(work, dsph.PREC_FP64, args.steps, args.warmup, False, flags, tun64)
         g64.close()
+        fp64 = {"value": round(npart * args.steps / s64, 1), "ms_per_step": round(s64 / args.steps * 1e3, 4),
+                "steps": args.steps, "superset_rebuilds": rb64, "tuning": tun64}
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
@@ -239,6 +255,7 @@ def other_workload(args):
         "data": "synthetic: seeded SplitMix64 scenes",
         "config": {"workload": desc, "particles": npart,
                    "precision": "mixed (f32 pair loops, f64 integration)", "tuning": tun},
+        "superset_rebuilds": rb,
         "fp64_path": fp64,
         "mixed_over_fp64": round(value / fp64["value"], 3) if fp64 else None,
         "kernel_ms": {k: [round(v[0] / max(v[1], 1), 4), v[1]] for k, v in times.items()},
@@ -250,19 +267,17 @@ def other_workload(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--dx", type=float, default=0.005)
-    ap.add_argument("--fp64-steps", type=int, default=1000)
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--cpu-max-steps", type=int, default=20)
+    ap.add_argument("--steady-steps", type=int, default=1000,
+                    help="also time this many steps per precision (superset rebuilds amortised); 0: off")
+    ap.add_argument("--cpu-sample-steps", type=int, default=4)
+    ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--no-timing", action="store_true")
-    ap.add_argument("--timing-steps", type=int, default=100,
-                    help="eager steps after the timed region that carry the per-kernel HIP events")
-    ap.add_argument("--eager", action="store_true",
-                    help="launch every step's kernels directly instead of replaying HIP graphs")
+    ap.add_argument("--no-timing", action="store_true",
+                    help="skip the eager re-run of the window with per-kernel HIP events")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration)")
     ap.add_argument("--workload", default="coupled", choices=["coupled", "ensemble", "granular"],
@@ -271,9 +286,11 @@ def main():
     ap.add_argument("--scenes", type=int, default=256, help="scenes of the ensemble workload")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes and skin instead of autotuning them")
+    ap.add_argument("--tune-advance", type=int, default=400,
+                    help="steps the autotuner runs before its trials, so they time the moving flow")
     ap.add_argument("--tuning", default=None, metavar="JSON",
                     help='fixed configuration instead of autotuning, e.g. \'{"mixed": {"skin": 0.2, '
-                         '"pair_block": 256, "build_block": 32}, "fp64": {...}}\' (profiling runs)')
+                         '"pair_block": 256, "build_block": 32, "group": 2}, "fp64": {...}}\'')
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -296,38 +313,58 @@ def main():
     print(f"# workload: {scene.n_fluid} fluid + {scene.n_solid} solid, dt={scene.dt:.3e}",
           file=sys.stderr, flush=True)
 
-    tsteps = 0 if args.no_timing else args.timing_steps
     tun = tune(scene, dsph.PREC_MIXED, 0, args)
-    g, secs, times = run_gpu(scene, dsph.PREC_MIXED, args.steps, args.warmup, tsteps, args.eager,
-                             tuning=tun)
+    g, secs, rebuilds, times = timed_window(scene, dsph.PREC_MIXED, args.steps, args.warmup,
+                                            not args.no_timing, tuning=tun)
     value = npart * args.steps / secs
-    print(f"# mixed: {secs / args.steps * 1e3:.3f} ms/step  {value:.4g} particle-steps/s",
-          file=sys.stderr, flush=True)
+    print(f"# mixed: {secs / args.steps * 1e3:.3f} ms/step  {value:.4g} particle-steps/s, "
+          f"{rebuilds} rebuilds", file=sys.stderr, flush=True)
     ff_directed, fs_pairs = pair_counts(g)
-    g_rebuilds = g.rebuilds_timed
     roof = roofline(times, "mixed", scene.n_fluid, ff_directed, fs_pairs) if times else None
     if roof:
         roof["traffic"] = traffic_from_profile(roof["kernel"], "mixed")
     g.close()
 
     fp64 = None
-    if not args.no_fp64 and args.fp64_steps > 0:
+    if not args.no_fp64:
         tun64 = tune(scene, dsph.PREC_FP64, 0, args)
-        g64, s64, t64 = run_gpu(scene, dsph.PREC_FP64, args.fp64_steps, args.warmup, tsteps,
-                                args.eager, tuning=tun64)
-        fp64 = {"value": round(npart * args.fp64_steps / s64, 1),
-                "ms_per_step": round(s64 / args.fp64_steps * 1e3, 4), "steps": args.fp64_steps,
-                "tuning": tun64, "superset_rebuilds": g64.rebuilds_timed}
+        g64, s64, rb64, t64 = timed_window(scene, dsph.PREC_FP64, args.steps, args.warmup,
+                                           not args.no_timing, tuning=tun64)
+        g64.close()
+        fp64 = {"value": round(npart * args.steps / s64, 1),
+                "ms_per_step": round(s64 / args.steps * 1e3, 4), "steps": args.steps,
+                "warmup": args.warmup, "superset_rebuilds": rb64, "tuning": tun64}
         r64 = roofline(t64, "fp64", scene.n_fluid, ff_directed, fs_pairs) if t64 else None
         if r64:
             fp64["roofline_frac"] = r64["frac"]
             fp64["dominant_kernel_ms"] = r64["avg_launch_ms"]
-        g64.close()
-        print(f"# fp64: {fp64['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+        print(f"# fp64: {fp64['ms_per_step']:.3f} ms/step, {rb64} rebuilds", file=sys.stderr, flush=True)
+
+    steady = None
+    if args.steady_steps > 0:
+        steady = {"steps": args.steps if args.steps >= args.steady_steps else args.steady_steps,
+                  "warmup": args.warmup}
+        legs = [("mixed", dsph.PREC_MIXED, tun)]
+        if fp64:
+            legs.append(("fp64", dsph.PREC_FP64, fp64["tuning"]))
+        for name, prec, tn in legs:
+            if args.steps >= args.steady_steps:
+                ms = secs / args.steps * 1e3 if name == "mixed" else fp64["ms_per_step"]
+                rb = rebuilds if name == "mixed" else fp64["superset_rebuilds"]
+            else:
+                gs, ss, rb, _ = timed_window(scene, prec, args.steady_steps, args.warmup, False, tuning=tn)
+                gs.close()
+                ms = ss / args.steady_steps * 1e3
+            steady[name] = {"ms_per_step": round(ms, 4), "value": round(npart / ms * 1e3, 1),
+                            "superset_rebuilds": rb}
+            print(f"# steady {name}: {ms:.3f} ms/step over {steady['steps']} steps, {rb} rebuilds",
+                  file=sys.stderr, flush=True)
+        if "fp64" in steady:
+            steady["mixed_over_fp64"] = round(steady["fp64"]["ms_per_step"] / steady["mixed"]["ms_per_step"], 3)
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(scene, args.cpu_seconds, args.cpu_max_steps)
+        cpu = cpu_baseline(scene, args.cpu_sample_steps, args.cpu_runs)
 
     out = {
         "metric": METRIC,
@@ -347,9 +384,10 @@ def main():
                    "particles": npart, "precision": "mixed (f32 pair loops, f64 integration)",
                    "tuning": tun},
         "moving_particle_steps_per_s": round(moving * args.steps / secs, 1),
-        "superset_rebuilds": g_rebuilds,
+        "superset_rebuilds": rebuilds,
         "fp64_path": fp64,
         "mixed_over_fp64": round(value / fp64["value"], 3) if fp64 else None,
+        "steady_state": steady,
         "roofline": roof,
         "kernel_ms": {k: [round(v[0] / max(v[1], 1), 4), v[1]] for k, v in times.items()},
         "pairs": {"fluid_fluid_directed": ff_directed, "fluid_solid": fs_pairs},

@@ -132,6 +132,16 @@ struct dsph_ctx {
     double4* wn = nullptr;
     int* wmat = nullptr;
     double* whist = nullptr;
+    // cylinder walls: (point, radius), (unit axis), (angular velocity)
+    double4* ypt = nullptr;
+    double4* yax = nullptr;
+    double4* yw = nullptr;
+    int* ymat = nullptr;
+    int ny = 0;
+    int* dyoff = nullptr;                 // device, cylinders of each scene
+    std::vector<int> yoff;
+    int nymax = 0;                        // most cylinders of one scene
+    double* yhist = nullptr;
     ContactTables tb{};
     int nmat = 0, ncomb = 0;
     Lists L;
@@ -141,10 +151,6 @@ struct dsph_ctx {
     unsigned long long* status = nullptr;
     int reorder_period = 1;
     double graph_dt = -1.0;  // time step baked into the captured graphs
-    // host copies used at initialisation (bond creation)
-    std::vector<double> h_spos, h_srad;
-    std::vector<int> h_smat, h_sclus;
-    std::vector<double> h_bonded;
     std::vector<void*> allocs;
     std::map<uint64_t, StepGraph> graphs;
     // timing
@@ -393,6 +399,7 @@ std::vector<StateArr> state_arrays(dsph_ctx* c) {
                           (const void*)cl.tq, (const void*)cl.q})
         add(p, nc, 32);
     add(c->whist, 9 * ns * (size_t)c->nwmax, 8);
+    add(c->yhist, 9 * ns * (size_t)c->nymax, 8);
     add(c->tb.hertz, 8 * (size_t)c->ncomb, 8);
     add(c->tb.linear, 11 * (size_t)c->ncomb, 8);
     add(c->tb.bonded, 6 * (size_t)c->ncomb, 8);
@@ -516,6 +523,14 @@ DemArgs dem_args(dsph_ctx* c) {
     a.nscene = c->nscene;
     a.nwmax = c->nwmax;
     a.whist = c->whist;
+    a.ypt = c->ypt;
+    a.yax = c->yax;
+    a.yw = c->yw;
+    a.ymat = c->ymat;
+    a.ny = c->ny;
+    a.yoff = c->dyoff;
+    a.nymax = c->nymax;
+    a.yhist = c->yhist;
     a.tb = c->tb;
     a.gx = c->gv[0];
     a.gy = c->gv[1];
@@ -1029,6 +1044,8 @@ struct Merged {
     std::vector<int> cstart, cend;
     std::vector<double> wpt, wn;
     std::vector<int> wmat;
+    std::vector<double> ypt, yax, yw, yrad;
+    std::vector<int> ymat, yoff;
     std::vector<int> fscene, sscene;           // caller order
     std::vector<int> foff, soff, coff, woff;   // nscene + 1 each
     std::vector<GridP> grids;
@@ -1055,6 +1072,16 @@ void validate_scene(const dsph_scene* s) {
                                 s->clump_pebble_start && s->clump_pebble_end),
             "clump arrays missing");
     require(s->n_wall == 0 || (s->wall_point && s->wall_normal), "wall arrays missing");
+    require(s->n_cylinder >= 0 && (s->n_cylinder == 0 || (s->cylinder_point && s->cylinder_axis &&
+                                                          s->cylinder_radius)),
+            "cylinder wall arrays missing");
+    for (int k = 0; k < s->n_cylinder; k++) {
+        const double* ax = s->cylinder_axis + 3 * k;
+        require(s->cylinder_radius[k] > 0.0 && ax[0] * ax[0] + ax[1] * ax[1] + ax[2] * ax[2] > 0.0,
+                "cylinder wall needs a positive radius and a non-zero axis");
+        const int m = s->cylinder_material ? s->cylinder_material[k] : 0;
+        require(m >= 0 && m < s->n_material, "cylinder wall material outside [0, n_material)");
+    }
     require(s->n_material >= 1, "n_material must be >= 1");
     for (int k = 0; k < 3; k++) require(s->domain_size[k] > 0, "domain size must be positive");
     const int nmat = s->n_material;
@@ -1135,6 +1162,7 @@ void merge_scenes(const dsph_scene* sc, int k, Merged& m) {
     m.soff = {0};
     m.coff = {0};
     m.woff = {0};
+    m.yoff = {0};
     int cell0 = 0, cluster_base = 0;
     for (int b = 0; b < k; b++) {
         const dsph_scene* s = sc + b;
@@ -1188,6 +1216,13 @@ void merge_scenes(const dsph_scene* sc, int k, Merged& m) {
         put(m.wpt, s->wall_point, nw, 3, 0.0);
         put(m.wn, s->wall_normal, nw, 3, 0.0);
         put(m.wmat, s->wall_material, nw, 1, 0);
+        const size_t ny = s->n_cylinder;
+        put(m.ypt, s->cylinder_point, ny, 3, 0.0);
+        put(m.yax, s->cylinder_axis, ny, 3, 0.0);
+        put(m.yw, s->cylinder_angvel, ny, 3, 0.0);
+        put(m.yrad, s->cylinder_radius, ny, 1, 0.0);
+        put(m.ymat, s->cylinder_material, ny, 1, 0);
+        m.yoff.push_back(m.yoff.back() + (int)ny);
         m.grids.push_back(scene_grid(s, cell0));
         require((long long)cell0 + m.grids.back().ncell < (1LL << 31) - 1, "batch has too many cells");
         cell0 += m.grids.back().ncell;
@@ -1232,6 +1267,12 @@ void merge_scenes(const dsph_scene* sc, int k, Merged& m) {
     f.wall_point = m.wpt.data();
     f.wall_normal = m.wn.data();
     f.wall_material = m.wmat.data();
+    f.n_cylinder = m.yoff.back();
+    f.cylinder_point = m.ypt.data();
+    f.cylinder_axis = m.yax.data();
+    f.cylinder_angvel = m.yw.data();
+    f.cylinder_radius = m.yrad.data();
+    f.cylinder_material = m.ymat.data();
 }
 }  // namespace
 
@@ -1644,10 +1685,6 @@ int dsph_upload_batch(dsph_ctx* c, const dsph_scene* scenes, int32_t n_scenes) {
         std::vector<double4> spr(ns), svel(ns), sw(ns), snrm(ns), zero4(std::max({ns, nc, 1}), make_double4(0, 0, 0, 0));
         std::vector<double> seff(ns);
         std::vector<int> smat(ns), sclus(ns), sclump(ns), sdum(ns), active, live;
-        c->h_spos.assign(3 * (size_t)ns, 0);
-        c->h_srad.assign(ns, 0);
-        c->h_smat.assign(ns, 0);
-        c->h_sclus.assign(ns, 0);
         for (int i = 0; i < ns; i++) {
             const double* p = s->solid_pos + 3 * i;
             spr[i] = make_double4(p[0], p[1], p[2], s->solid_radius[i]);
@@ -1665,10 +1702,6 @@ int dsph_upload_batch(dsph_ctx* c, const dsph_scene* scenes, int32_t n_scenes) {
             if (sdum[i] != 1 || svel[i].w > 0.0 || svel[i].x != 0.0 || svel[i].y != 0.0 || svel[i].z != 0.0 ||
                 sw[i].x != 0.0 || sw[i].y != 0.0 || sw[i].z != 0.0 || sclump[i] >= 0)
                 live.push_back(i);
-            for (int k = 0; k < 3; k++) c->h_spos[3 * i + k] = p[k];
-            c->h_srad[i] = s->solid_radius[i];
-            c->h_smat[i] = smat[i];
-            c->h_sclus[i] = sclus[i];
         }
         Solids& so = c->so;
         so.pr = dalloc<double4>(c, ns);
@@ -1825,6 +1858,39 @@ int dsph_upload_batch(dsph_ctx* c, const dsph_scene* scenes, int32_t n_scenes) {
         c->whist = dalloc<double>(c, nh);
         if (nh) HIPCHK(hipMemsetAsync(c->whist, 0, nh * sizeof(double), c->st));
 
+        // ---- cylinder walls (axes normalised) and their spring history
+        {
+            const int ny = c->ny = s->n_cylinder;
+            std::vector<double4> yp(ny), ya(ny), yv(ny);
+            std::vector<int> ym(ny);
+            for (int k = 0; k < ny; k++) {
+                const double* p = s->cylinder_point + 3 * k;
+                const double* ax = s->cylinder_axis + 3 * k;
+                const double* w = s->cylinder_angvel ? s->cylinder_angvel + 3 * k : nullptr;
+                const double l = std::sqrt(ax[0] * ax[0] + ax[1] * ax[1] + ax[2] * ax[2]);
+                yp[k] = make_double4(p[0], p[1], p[2], s->cylinder_radius[k]);
+                ya[k] = make_double4(ax[0] / l, ax[1] / l, ax[2] / l, 0);
+                yv[k] = make_double4(w ? w[0] : 0, w ? w[1] : 0, w ? w[2] : 0, 0);
+                ym[k] = s->cylinder_material ? s->cylinder_material[k] : 0;
+            }
+            c->ypt = dalloc<double4>(c, ny);
+            c->yax = dalloc<double4>(c, ny);
+            c->yw = dalloc<double4>(c, ny);
+            c->ymat = dalloc<int>(c, ny);
+            h2d(c, c->ypt, yp.data(), ny);
+            h2d(c, c->yax, ya.data(), ny);
+            h2d(c, c->yw, yv.data(), ny);
+            h2d(c, c->ymat, ym.data(), ny);
+            c->yoff = M.yoff;
+            c->dyoff = dalloc<int>(c, n_scenes + 1);
+            h2d(c, c->dyoff, M.yoff.data(), n_scenes + 1);
+            c->nymax = 0;
+            for (int b = 0; b < n_scenes; b++) c->nymax = std::max(c->nymax, M.yoff[b + 1] - M.yoff[b]);
+            const size_t nyh = 9 * (size_t)ns * c->nymax;
+            c->yhist = dalloc<double>(c, nyh);
+            if (nyh) HIPCHK(hipMemsetAsync(c->yhist, 0, nyh * sizeof(double), c->st));
+        }
+
         // ---- contact tables
         c->nmat = nmat;
         c->ncomb = nmat * (nmat + 1) / 2;
@@ -1838,7 +1904,6 @@ int dsph_upload_batch(dsph_ctx* c, const dsph_scene* scenes, int32_t n_scenes) {
         if (s->hertz) std::copy(s->hertz, s->hertz + 8 * ncb, hz.begin());
         if (s->linear) std::copy(s->linear, s->linear + 11 * ncb, li.begin());
         if (s->bonded) std::copy(s->bonded, s->bonded + 6 * ncb, bo.begin());
-        c->h_bonded = bo;
         double* dhz = dalloc<double>(c, hz.size());
         double* dli = dalloc<double>(c, li.size());
         double* dbo = dalloc<double>(c, bo.size());
@@ -1933,57 +1998,27 @@ int dsph_initialize(dsph_ctx* c, int compute_normals) {
     if (!c || !c->uploaded) return DSPH_E_STATE;
     try {
         search_checked(c, 0, 1);
-        // addBondData, initial branch (dataBase.cpp:270-308): bonds are the
-        // initial contacts in the same cluster that touch physically
-        int64_t tot = 0;
+        // addBondData, initial branch (dataBase.cpp:270-308), on the device:
+        // bonds are the initial contacts in one cluster that touch
+        // physically, in the contact list's order (flags, scan, fill)
         {
-            int t = 0;
-            d2h(c, &t, c->L.ssoff + c->ns, 1);
+            const int64_t cap = std::max<int64_t>(c->L.ss_cap, 1);
+            int* flag = dalloc<int>(c, (size_t)cap);
+            int* off = dalloc<int>(c, (size_t)cap + 1);
+            int* tmp = dalloc<int>(c, (size_t)scan_tmp_size(cap) + 64);
+            const DemArgs da = dem_args(c);
+            fill_int(c->st, flag, cap, 0);
+            dem_bond_flags(c->st, da, flag);
+            exclusive_scan(c->st, flag, off, cap, tmp);
+            int nb = 0;
+            d2h(c, &nb, off + cap, 1);
             HIPCHK(hipStreamSynchronize(c->st));
-            tot = t;
-        }
-        std::vector<int> si(tot), sj(tot);
-        d2h(c, si.data(), c->L.ss_i, tot);
-        d2h(c, sj.data(), c->L.ss_j, tot);
-        HIPCHK(hipStreamSynchronize(c->st));
-        std::vector<int> bi, bj;
-        std::vector<double4> bn, bpt;
-        const int ncb = c->ncomb;
-        for (int64_t k = 0; k < tot; k++) {
-            const int i = si[k], j = sj[k];
-            const double* pi = &c->h_spos[3 * (size_t)i];
-            const double* pj = &c->h_spos[3 * (size_t)j];
-            const double dx = pi[0] - pj[0], dy = pi[1] - pj[1], dz = pi[2] - pj[2];
-            const double d = std::sqrt(dx * dx + dy * dy + dz * dz);
-            const double ri = c->h_srad[i], rj = c->h_srad[j];
-            if (ri + rj + 1.e-10 <= d) continue;
-            const int cij = combined_index(c->nmat, c->h_smat[i], c->h_smat[j]);
-            const double Eb = c->h_bonded[1 * ncb + cij];
-            if (Eb > 1.e-10 && c->h_sclus[i] >= 0 && c->h_sclus[i] == c->h_sclus[j]) {
-                const double nx = dx / d, ny = dy / d, nz = dz / d;
-                const double delta = ri + rj - d;
-                const double a = rj - 0.5 * delta;
-                bi.push_back(i);
-                bj.push_back(j);
-                bn.push_back(make_double4(nx, ny, nz, 0));
-                bpt.push_back(make_double4(pj[0] + a * nx, pj[1] + a * ny, pj[2] + a * nz, 0));
-            }
-        }
-        const int nb = (int)bi.size();
-        alloc_bonds(c, nb);
-        if (nb) {
-            std::vector<int> ones(nb, 1);
-            std::vector<double4> z4(nb, make_double4(0, 0, 0, 0));
-            std::vector<double> z(nb, 0.0);
-            h2d(c, c->B.i, bi.data(), nb);
-            h2d(c, c->B.j, bj.data(), nb);
-            h2d(c, c->B.on, ones.data(), nb);
-            h2d(c, c->B.nrm, bn.data(), nb);
-            h2d(c, c->B.pt, bpt.data(), nb);
-            h2d(c, c->B.Fs, z4.data(), nb);
-            h2d(c, c->B.Tb, z4.data(), nb);
-            h2d(c, c->B.Fn, z.data(), nb);
-            h2d(c, c->B.Tt, z.data(), nb);
+            alloc_bonds(c, nb);
+            dem_bond_fill(c->st, dem_args(c), off, c->B);
+            HIPCHK(hipStreamSynchronize(c->st));
+            dfree(c, flag);
+            dfree(c, off);
+            dfree(c, tmp);
         }
         if (compute_normals && c->ns > 0) {
             // the solid grid of the initial positions (the gated search sorts
@@ -2158,7 +2193,7 @@ struct StateBlobHeader {
     int32_t nf, ns, nc, nw, nscene, nbond, ncell, ncomb, nactive, nwmax;
     int32_t K, supK, fsupK, reorder_period;
     int64_t fs_cap, ss_cap;
-    int32_t dcur, skey_valid, tune_bs, tune_bmem, tune_group, pad;
+    int32_t dcur, skey_valid, tune_bs, tune_bmem, tune_group, nymax;
     uint32_t flags, pad1;
     double skin_h, gv[3];
     int64_t n_arrays, payload;
@@ -2181,6 +2216,7 @@ StateBlobHeader state_header(dsph_ctx* c, const std::vector<StateArr>& v) {
     h.ncomb = c->ncomb;
     h.nactive = c->so.nactive;
     h.nwmax = c->nwmax;
+    h.nymax = c->nymax;
     h.K = c->L.K;
     h.supK = c->L.supK;
     h.fsupK = c->L.fsupK;
@@ -2245,7 +2281,7 @@ int dsph_load_state(dsph_ctx* c, const void* buffer, int64_t size) {
         require(h.prec == c->prec, "state blob of the other precision");
         require(h.nf == c->nf && h.ns == c->ns && h.nc == c->nc && h.nw == c->nw && h.nscene == c->nscene &&
                     h.ncell == c->ncell && h.ncomb == c->ncomb && h.nactive == c->so.nactive &&
-                    h.nwmax == c->nwmax,
+                    h.nwmax == c->nwmax && h.nymax == c->nymax,
                 "state blob of another scene (particle, wall, cell or material counts differ)");
         require(h.tune_bs >= 64 && h.tune_bs <= 512 && h.tune_bs % 64 == 0 && h.tune_bmem >= 1 &&
                     h.tune_bmem <= 64 && (h.tune_group == 1 || h.tune_group == 2 || h.tune_group == 4 ||
@@ -2285,8 +2321,6 @@ int dsph_load_state(dsph_ctx* c, const void* buffer, int64_t size) {
         }
         c->dcur = h.dcur;
         c->skey_valid = h.skey_valid != 0;
-        c->h_bonded.resize(6 * (size_t)c->ncomb);
-        d2h(c, c->h_bonded.data(), c->tb.bonded, c->h_bonded.size());
         HIPCHK(hipStreamSynchronize(c->st));
         c->initialized = true;
         return DSPH_OK;
@@ -2305,7 +2339,6 @@ int dsph_set_contact_models(dsph_ctx* c, const double* hertz, const double* line
         if (linear) h2d(c, const_cast<double*>(c->tb.linear), linear, 11 * (size_t)ncb);
         if (bonded) {
             h2d(c, const_cast<double*>(c->tb.bonded), bonded, 6 * (size_t)ncb);
-            c->h_bonded.assign(bonded, bonded + 6 * (size_t)ncb);
         }
         HIPCHK(hipStreamSynchronize(c->st));
         return DSPH_OK;

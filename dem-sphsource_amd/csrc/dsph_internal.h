@@ -430,13 +430,24 @@ struct DemArgs {
     int nscene;
     int nwmax;      // most walls of any scene
     double* whist;  // 9 * ns * nwmax
+    // cylinder walls, laid out like the infinite walls: (point, radius) and
+    // (unit axis, 0) and (angular velocity, 0) per wall, walls of scene k
+    // are [yoff[k], yoff[k+1]), 9 history doubles per solid and wall
+    const double4* ypt;
+    const double4* yax;
+    const double4* yw;
+    const int* ymat;
+    int ny;
+    const int* yoff;
+    int nymax;
+    double* yhist;  // 9 * ns * nymax
     ContactTables tb;
     double gx, gy, gz;
     uint32_t flags;
 };
 void dem_contacts(hipStream_t s, const DemArgs& a, double dt);
 void dem_bonds(hipStream_t s, const DemArgs& a, double dt);
-// contact sums of the live solids, then their infinite walls
+// contact sums of the live solids, then their infinite and cylinder walls
 void dem_solid_sum(hipStream_t s, const DemArgs& a, double dt);
 // kick and drift of the live solids in one pass (scenes without clumps)
 void dem_solid_kick_drift(hipStream_t s, const DemArgs& a, double dt, bool kick_first);
@@ -445,6 +456,11 @@ void dem_solid_kick(hipStream_t s, const DemArgs& a, double dt);
 void dem_solid_drift(hipStream_t s, const DemArgs& a, double dt);
 void dem_clump_kick(hipStream_t s, const DemArgs& a, double dt);
 void dem_clump_drift(hipStream_t s, const DemArgs& a, double dt);
+// bonds from the initial contact list (addBondData): flag[k] = contact k
+// becomes a bond (capacity ss_cap); then, with off the exclusive scan of the
+// flags, the bonds written at their offsets (b sized for the total)
+void dem_bond_flags(hipStream_t s, const DemArgs& a, int* flag);
+void dem_bond_fill(hipStream_t s, const DemArgs& a, const int* off, const Bonds& b);
 void dem_external_force(hipStream_t s, const DemArgs& a, int idx, double fx, double fy, double fz);
 void dem_global_damping(hipStream_t s, const DemArgs& a, int idx, double cd);
 

@@ -150,6 +150,58 @@ __device__ void wall_forces(const DemArgs& a, int i, double dt, d3& acc, d3& tq)
     }
 }
 
+// cylinder walls on live solid i (calCylinderWallForceTorque,
+// integrate.cu:612-715), after the infinite walls.  With d the centre's
+// distance from the axis, an inside contact overlaps by d + r - R and an
+// outside one (d >= R) by r + R - d; the contact point is R - delta/2 from
+// the axis.  The reference takes the outward normal for both (and never
+// launches the kernel: its cylinder walls are not copied to the device);
+// an inside contact here takes the inward one, so the wall holds the
+// sphere in.  The loop stops at the first wall out of reach, as the
+// infinite-wall loop does, unless DSPH_FLAG_WALL_CONTINUE.
+__device__ void cylinder_forces(const DemArgs& a, int i, double dt, d3& acc, d3& tq) {
+    const double4 v4 = a.so.vel[i];
+    if (a.so.dummy[i] == 1 || v4.w < 1.e-10) return;
+    const bool cont = (a.flags & DSPH_FLAG_WALL_CONTINUE) != 0;
+    const double4 pr = a.so.pr[i];
+    const d3 pi = v3(pr), vi = v3(v4), wi = v3(a.so.w[i]);
+    const double ri = pr.w;
+    const int sc = a.nscene == 1 ? 0 : a.sscene[i];
+    const int k0 = a.nscene == 1 ? 0 : a.yoff[sc], k1 = a.nscene == 1 ? a.ny : a.yoff[sc + 1];
+    for (int k = k0; k < k1; k++) {
+        double* h = a.yhist + 9 * ((int64_t)i * a.nymax + (k - k0));
+        const double4 p4 = a.ypt[k];
+        const double R = p4.w;
+        const d3 pk = v3(p4), ax = v3(a.yax[k]);
+        const d3 c0 = pk + ax * dot(pi - pk, ax);
+        const d3 out = unit(pi - c0);
+        const double d = vlen(pi - c0);
+        double delta = d + ri - R;
+        bool outside = false;
+        if (delta >= ri) {
+            delta = ri + R - d;
+            outside = true;
+        }
+        if (delta < 0.) {
+            for (int m = 0; m < 9; m++) h[m] = 0.0;
+            if (cont) continue;
+            break;
+        }
+        const d3 n = outside ? out : -out;
+        const d3 rc = c0 + out * (R - 0.5 * delta);
+        const d3 wk = v3(a.yw[k]);
+        const d3 vc = vi + cross(wi, rc - pi) - cross(wk, rc - pk);
+        const d3 wrel = wi - wk;
+        d3 F{0, 0, 0}, T{0, 0, 0};
+        Springs sp = ld_springs(h);
+        const int c = combined_index(a.tb.nmat, a.so.mat[i], a.ymat[k]);
+        contact_model(a.tb, c, F, T, sp, vc, wrel, n, delta, 1. / v4.w, ri, dt);
+        st_springs(h, sp);
+        acc += F * v4.w;
+        tq += T + cross(rc - pi, F);
+    }
+}
+
 __global__ void k_solid_sum(DemArgs a, double dt) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.so.nlive) return;
@@ -189,6 +241,7 @@ __global__ void k_solid_sum(DemArgs a, double dt) {
     const double gon = (v.w > 0.0 && a.so.clump[i] < 0) ? 1.0 : 0.0;
     d3 acc = F * v.w + d3{a.gx, a.gy, a.gz} * gon;
     if (a.nw > 0) wall_forces(a, i, dt, acc, T);
+    if (a.ny > 0) cylinder_forces(a, i, dt, acc, T);
     a.so.acc[i] = d4(acc);
     a.so.tq[i] = d4(T);
 }
@@ -327,6 +380,59 @@ __global__ void k_damping(DemArgs a, int i, double cd) {
     a.so.tq[i] = d4(T);
 }
 
+// dataBase::addBondData, initial branch (dataBase.cpp:270-308): a contact
+// of the initial list becomes a bond when both spheres touch physically
+// (r_i + r_j + 1e-10 > |x_i - x_j|), belong to one cluster and their
+// material pair has a bond modulus.  FLAG pass: one flag per contact;
+// FILL pass: the bonds at their scanned offsets, so they keep the contact
+// list's order.  FP64 without contraction, as the host loop computes it.
+__device__ __forceinline__ bool bond_candidate(const DemArgs& a, int64_t k, int& i, int& j) {
+#pragma clang fp contract(off)
+    i = a.L.ss_i[k];
+    j = a.L.ss_j[k];
+    const double4 pi = a.so.pr[i], pj = a.so.pr[j];
+    const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+    const double d = sqrt(dx * dx + dy * dy + dz * dz);
+    if (pi.w + pj.w + 1.e-10 <= d) return false;
+    const int ci = a.so.clus[i], cj = a.so.clus[j];
+    const int cij = combined_index(a.tb.nmat, a.so.mat[i], a.so.mat[j]);
+    const double Eb = a.tb.bonded[1 * a.tb.ncomb + cij];
+    return Eb > 1.e-10 && ci >= 0 && ci == cj;
+}
+
+__global__ void k_bond_flags(DemArgs a, const int* __restrict__ total, int* __restrict__ flag) {
+    const int64_t n = *total;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+        int i, j;
+        flag[k] = bond_candidate(a, k, i, j) ? 1 : 0;
+    }
+}
+
+__global__ void k_bond_fill(DemArgs a, const int* __restrict__ total, const int* __restrict__ off, Bonds b) {
+#pragma clang fp contract(off)
+    const int64_t n = *total;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+        int i, j;
+        if (!bond_candidate(a, k, i, j)) continue;
+        const int m = off[k];
+        const double4 pi = a.so.pr[i], pj = a.so.pr[j];
+        const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const double d = sqrt(dx * dx + dy * dy + dz * dz);
+        const double nx = dx / d, ny = dy / d, nz = dz / d;
+        const double delta = pi.w + pj.w - d;
+        const double arm = pj.w - 0.5 * delta;
+        b.i[m] = i;
+        b.j[m] = j;
+        b.on[m] = 1;
+        b.nrm[m] = make_double4(nx, ny, nz, 0.0);
+        b.pt[m] = make_double4(pj.x + arm * nx, pj.y + arm * ny, pj.z + arm * nz, 0.0);
+        b.Fs[m] = make_double4(0.0, 0.0, 0.0, 0.0);
+        b.Tb[m] = make_double4(0.0, 0.0, 0.0, 0.0);
+        b.Fn[m] = 0.0;
+        b.Tt[m] = 0.0;
+    }
+}
+
 int grid_stride_blocks(int64_t cap) {
     int b = blocks_for(cap);
     return b > 2048 ? 2048 : b;
@@ -364,6 +470,17 @@ void dem_clump_kick(hipStream_t s, const DemArgs& a, double dt) {
 void dem_clump_drift(hipStream_t s, const DemArgs& a, double dt) {
     if (a.nc > 0) hipLaunchKernelGGL(k_clump_drift, dim3(blocks_for(a.nc)), dim3(kBS), 0, s, a, dt);
 }
+void dem_bond_flags(hipStream_t s, const DemArgs& a, int* flag) {
+    if (a.ns > 0 && a.L.ss_cap > 0)
+        hipLaunchKernelGGL(k_bond_flags, dim3(grid_stride_blocks(a.L.ss_cap)), dim3(kBS), 0, s, a,
+                           (const int*)(a.L.ssoff + a.ns), flag);
+}
+void dem_bond_fill(hipStream_t s, const DemArgs& a, const int* off, const Bonds& b) {
+    if (a.ns > 0 && a.L.ss_cap > 0 && b.n > 0)
+        hipLaunchKernelGGL(k_bond_fill, dim3(grid_stride_blocks(a.L.ss_cap)), dim3(kBS), 0, s, a,
+                           (const int*)(a.L.ssoff + a.ns), off, b);
+}
+
 void dem_external_force(hipStream_t s, const DemArgs& a, int idx, double fx, double fy, double fz) {
     hipLaunchKernelGGL(k_ext_force, dim3(1), dim3(1), 0, s, a, idx, fx, fy, fz);
 }

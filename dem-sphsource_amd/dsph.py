@@ -391,16 +391,19 @@ TUNING_SPACE = {
 
 
 def autotune(scene, precision=PREC_MIXED, flags=0, trial_steps=20, warmup=5, space=None,
-             log=None):
+             log=None, advance=0):
     """Pick the fastest launch shapes and Verlet skin for a workload.
 
     Runs the workload (a Scene, or a list of Scenes uploaded as one batch) on
     a throwaway context and times `trial_steps` graph-replayed steps per
-    candidate, one coordinate of the search space at a time (skin, then the
-    pair-loop workgroup, then the build block), keeping the best value of
-    each before moving to the next.  Every candidate gives the same results;
-    only the time differs.  Returns (config, trials) where config feeds
-    Solver.apply_tuning and trials lists (config, ms per step)."""
+    candidate, one coordinate of the search space at a time (row group,
+    skin, pair-loop workgroup, build block), keeping the best value of each
+    before moving to the next.  With `advance`, the context first runs that
+    many steps and every trial starts from a checkpoint of that state, so
+    all candidates time the same window of the moving flow; otherwise the
+    trials follow each other from step 1.  Every candidate gives the same
+    results; only the time differs.  Returns (config, trials) where config
+    feeds Solver.apply_tuning and trials lists (config, ms per step)."""
     space = dict(TUNING_SPACE, **(space or {}))
     g = Solver(precision)
     g.set_flags(flags)
@@ -414,6 +417,11 @@ def autotune(scene, precision=PREC_MIXED, flags=0, trial_steps=20, warmup=5, spa
     cfg = {"skin": 0.3, "pair_block": 256, "build_block": 64, "group": 1}
     trials = []
     step = [1]
+    blob = None
+    if advance > 0:
+        g.run(advance, dt, first=1, graphs=True)
+        blob = g.save_state()
+        step[0] = advance + 1
 
     def timed(n):
         g.synchronize()
@@ -424,18 +432,19 @@ def autotune(scene, precision=PREC_MIXED, flags=0, trial_steps=20, warmup=5, spa
         return (time.perf_counter() - t0) / n * 1e3
 
     def measure(c):
+        if blob is not None:
+            g.load_state(blob)
+            step[0] = advance + 1
         g.apply_tuning(c)
-        # eager warm-up steps grow any list the new skin overflows
+        # eager warm-up steps grow any list the new skin overflows; then
+        # graph-replayed ones capture the step graphs (every combination of
+        # density re-initialisation and record buffer parity), so that no
+        # capture falls into the timed steps
         g.run(warmup, dt, first=step[0], graphs=False)
         step[0] += warmup
-        try:
-            ms = timed(trial_steps)
-        except DsphError as e:
-            if e.code != -3:
-                raise
-            g.run(1, dt, first=step[0], graphs=False)  # grow, then time again
-            step[0] += 1
-            ms = timed(trial_steps)
+        g.run(10, dt, first=step[0], graphs=True)
+        step[0] += 10
+        ms = timed(trial_steps)
         trials.append((dict(c), ms))
         if log:
             log(f"autotune {c} -> {ms:.4f} ms/step")

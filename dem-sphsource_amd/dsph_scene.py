@@ -39,6 +39,9 @@ class CScene(C.Structure):
         ("n_material", C.c_int32),
         ("hertz", _dp), ("linear", _dp), ("bonded", _dp),
         ("fluid_alpha", _dp),
+        ("n_cylinder", C.c_int32),
+        ("cylinder_point", _dp), ("cylinder_axis", _dp), ("cylinder_angvel", _dp),
+        ("cylinder_radius", _dp), ("cylinder_material", _ip),
     ]
 
 
@@ -107,6 +110,12 @@ class Scene:
     linear: np.ndarray | None = None
     bonded: np.ndarray | None = None
     fluid_alpha: np.ndarray | None = None  # artificial viscosity per fluid particle
+    # cylinder walls
+    cylinder_point: np.ndarray = field(default_factory=lambda: np.zeros((0, 3)))
+    cylinder_axis: np.ndarray = field(default_factory=lambda: np.zeros((0, 3)))
+    cylinder_angvel: np.ndarray = field(default_factory=lambda: np.zeros((0, 3)))
+    cylinder_radius: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    cylinder_material: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
     # driver parameters (not part of the C struct)
     dt: float = 1e-4
     gap: int = 1
@@ -142,6 +151,10 @@ class Scene:
     @property
     def n_wall(self) -> int:
         return int(np.asarray(self.wall_point).reshape(-1, 3).shape[0])
+
+    @property
+    def n_cylinder(self) -> int:
+        return int(np.asarray(self.cylinder_point).reshape(-1, 3).shape[0])
 
     @property
     def n_particles(self) -> int:
@@ -222,5 +235,11 @@ class Scene:
         s.linear = dp(self.linear)
         s.bonded = dp(self.bonded)
         s.fluid_alpha = dp(self.fluid_alpha)
+        s.n_cylinder = self.n_cylinder
+        s.cylinder_point = dp(self.cylinder_point)
+        s.cylinder_axis = dp(self.cylinder_axis)
+        s.cylinder_angvel = dp(self.cylinder_angvel)
+        s.cylinder_radius = dp(self.cylinder_radius)
+        s.cylinder_material = ip(self.cylinder_material)
         s._keep = keep  # keep arrays alive with the struct
         return s

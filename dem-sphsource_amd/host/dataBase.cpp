@@ -435,6 +435,13 @@ void dataBase::buildDeviceData() {
     s.wall_point = xyz(w.pointInPlane);
     s.wall_normal = xyz(w.normal);
     s.wall_material = w.materialID.data();
+    const HostCylinderWall& y = hos.cylinderWalls;
+    s.n_cylinder = y.num;
+    s.cylinder_point = xyz(y.pointInAxis);
+    s.cylinder_axis = xyz(y.axis);
+    s.cylinder_angvel = xyz(y.angularVelocity);
+    s.cylinder_radius = y.radius.data();
+    s.cylinder_material = y.materialID.data();
     s.n_material = hos.contactModels.nMaterial;
     s.hertz = tables_[0].data();
     s.linear = tables_[1].data();

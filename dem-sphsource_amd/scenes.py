@@ -204,6 +204,16 @@ def add_infinite_wall(sc: Scene, point, normal, material=0):
     sc.wall_material = np.concatenate([wm, [material]]).astype(np.int32)
 
 
+def add_cylinder_wall(sc: Scene, point, axis, angvel, radius, material=0):
+    """dataBase::addCylinderWall (core/dataBase.cpp:69-82), with the radius
+    kept (the reference drops it)."""
+    sc.cylinder_point = np.concatenate([sc.cylinder_point.reshape(-1, 3), np.asarray(point, float)[None]])
+    sc.cylinder_axis = np.concatenate([sc.cylinder_axis.reshape(-1, 3), np.asarray(axis, float)[None]])
+    sc.cylinder_angvel = np.concatenate([sc.cylinder_angvel.reshape(-1, 3), np.asarray(angvel, float)[None]])
+    sc.cylinder_radius = np.concatenate([sc.cylinder_radius, [float(radius)]])
+    sc.cylinder_material = np.concatenate([sc.cylinder_material, [material]]).astype(np.int32)
+
+
 # --------------------------------------------------------------------------
 # the workloads
 # --------------------------------------------------------------------------
@@ -594,5 +604,40 @@ def bonded_bar(n=(10, 3, 3), r=0.002, rho_s=2500.0, pull=0.5, k_bond=1.0e8, sigm
     sc.set_bonded(0, 0, Eb, 1.0, 1.0, sigma_t, 0.0, 0.0)
     m = 4.0 / 3.0 * math.pi * r ** 3 * rho_s
     sc.dt = 0.2 * 2.0 / math.sqrt(k_bond / m) / 2.0
+    sc.gap = 1
+    return sc
+
+
+def rotating_drum(R=0.03, L=0.03, r=0.002, rho_s=2500.0, omega=10.0, seed_stream=71,
+                  with_post=True, name="rotating_drum") -> Scene:
+    """DEM spheres in a horizontal drum (a cylinder wall of radius R about
+    the x axis, turning at omega) closed by two infinite planes at x = 0
+    and x = L; with_post adds a second, fixed cylinder wall of radius R/4
+    along y above the drum's axis that spheres thrown up meet from the
+    outside.  Spheres on a jittered lattice filling the drum's lower half,
+    the outermost pressing on its wall; Hertz-Mindlin with rolling
+    friction."""
+    sc = empty_scene((-0.01, -R - 0.01, -R - 0.01), (L + 0.02, 2 * R + 0.02, 2 * R + 0.02), name=name)
+    d = 2.05 * r
+    pts = []
+    for x in np.arange(r * 1.2, L - r, d):
+        for y in np.arange(-R + 0.98 * r, R - 0.98 * r, d):
+            for z in np.arange(-R + 0.98 * r, 0.0, d):
+                if y * y + z * z < (R - 0.95 * r) ** 2:   # the outer ones press on the drum
+                    pts.append((x, y, z))
+    pts = np.asarray(pts, float)
+    m = len(pts)
+    pts = pts + uniform(ROOT_SEED, seed_stream, 3 * m, -0.01 * r, 0.01 * r).reshape(m, 3)
+    add_solid(sc, pts, r, rho_s, material=0)
+    Es, Gs = hertz_effective(1.0e7, 0.3)
+    sc.set_hertz(0, 0, Es, Gs, 0.5, 0.3, 0.2, 0.4, 0.05, 0.05)
+    sc.set_hertz(0, 1, Es, Gs, 0.6, 0.3, 0.2, 0.5, 0.05, 0.05)
+    add_infinite_wall(sc, (0, 0, 0), (1, 0, 0), 1)
+    add_infinite_wall(sc, (L, 0, 0), (-1, 0, 0), 1)
+    add_cylinder_wall(sc, (0, 0, 0), (1, 0, 0), (omega, 0, 0), R, 1)
+    if with_post:
+        add_cylinder_wall(sc, (0.5 * L, 0, 0.4 * R), (0, 1, 0), (0, 0, 0), 0.25 * R, 1)
+    k = 0.5 * 1.0e7 * math.pi * r
+    sc.dt = 0.2 * math.sqrt(8 * r ** 3 * rho_s / k)
     sc.gap = 1
     return sc

@@ -142,6 +142,20 @@ typedef struct dsph_scene {
      * particle being updated: alpha is uniform within a scene.  No
      * reference counterpart (the reference's helper takes one alpha). */
     const double* fluid_alpha;  /* [n_fluid] */
+
+    /* HostCylinderWall (hostStructs.h:176-200), filled by
+     * dataBase::addCylinderWall: a point on the axis, the axis direction
+     * (normalised on upload), the wall's angular velocity and radius.  The
+     * reference never copies these walls to the device (dataBase.cpp:
+     * 434-449) and never stores the radius (:69-82); here they act on the
+     * solids as calCylinderWallForceTorque (integrate.cu:612-715) describes,
+     * after the infinite walls.                                            */
+    int32_t n_cylinder;
+    const double* cylinder_point;    /* [3n] */
+    const double* cylinder_axis;     /* [3n] */
+    const double* cylinder_angvel;   /* [3n] optional (0) */
+    const double* cylinder_radius;   /* [n]  */
+    const int32_t* cylinder_material;/* [n]  optional (0) */
 } dsph_scene;
 
 typedef struct dsph_ctx dsph_ctx;

@@ -356,6 +356,12 @@ struct orc_ctx {
     std::vector<V3> wpt, wn;
     std::vector<int> wmat;
     std::vector<Springs> wallHist; /* [ns * nw] */
+    /* cylinder walls (hostStructs.h:176-200) */
+    int ny = 0;
+    std::vector<V3> ypt, yax, yw;
+    std::vector<double> yrad;
+    std::vector<int> ymat;
+    std::vector<Springs> cylHist; /* [ns * ny] */
     /* contact tables */
     int nmat = 0;
     std::vector<double> hertz, linear, bonded; /* field-major */
@@ -628,6 +634,16 @@ int orc_upload_scene(orc_ctx* o, const dsph_scene* s) {
         o->wmat[i] = s->wall_material ? s->wall_material[i] : 0;
     }
     o->wallHist.assign((size_t)o->ns * o->nw, Springs{mk(0, 0, 0), mk(0, 0, 0), mk(0, 0, 0)});
+    n = o->ny = s->n_cylinder;
+    o->ypt.resize(n); o->yax.resize(n); o->yw.resize(n); o->yrad.resize(n); o->ymat.resize(n);
+    for (int i = 0; i < n; i++) {
+        o->ypt[i] = ld3(s->cylinder_point, i);
+        o->yax[i] = unit(ld3(s->cylinder_axis, i));
+        o->yw[i] = s->cylinder_angvel ? ld3(s->cylinder_angvel, i) : mk(0, 0, 0);
+        o->yrad[i] = s->cylinder_radius[i];
+        o->ymat[i] = s->cylinder_material ? s->cylinder_material[i] : 0;
+    }
+    o->cylHist.assign((size_t)o->ns * o->ny, Springs{mk(0, 0, 0), mk(0, 0, 0), mk(0, 0, 0)});
     o->nmat = s->n_material;
     o->ncomb = o->nmat * (o->nmat + 1) / 2;
     o->hertz.assign(8 * o->ncomb, 0);
@@ -1057,6 +1073,56 @@ static void wall_forces(orc_ctx* o, double dt) {
     }
 }
 
+/* calCylinderWallForceTorque (integrate.cu:612-715): sphere against the
+ * wall of radius R around an axis through p along a.  With d the centre's
+ * distance from the axis, an inside contact overlaps by d + r - R and an
+ * outside one (d >= R) by r + R - d; the contact point is R - delta/2 from
+ * the axis.  The reference takes the outward normal for both; an inside
+ * contact here takes the inward one, so the wall holds the sphere in (the
+ * reference never launches this kernel: its cylinder walls are not copied
+ * to the device).  The loop stops at the first wall out of reach, as the
+ * infinite-wall loop does, unless DSPH_FLAG_WALL_CONTINUE. */
+static void cylinder_forces(orc_ctx* o, double dt) {
+    int ny = o->ny;
+    bool cont = (o->flags & DSPH_FLAG_WALL_CONTINUE) != 0;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < o->ns; i++) {
+        if (o->sdummy[i] == 1 || o->sinvm[i] < 1.e-10) continue;
+        double ri = o->srad[i];
+        V3 pi = o->spos[i], vi = o->svel[i], wi = o->sw[i];
+        for (int k = 0; k < ny; k++) {
+            Springs& h = o->cylHist[(size_t)i * ny + k];
+            double R = o->yrad[k];
+            V3 a = o->yax[k], pk = o->ypt[k];
+            V3 c0 = pk + a * dot(pi - pk, a);
+            V3 out = unit(pi - c0);
+            double d = len(pi - c0);
+            double delta = d + ri - R;
+            bool outside = false;
+            if (delta >= ri) {
+                delta = ri + R - d;
+                outside = true;
+            }
+            if (delta < 0.) {
+                h = Springs{mk(0, 0, 0), mk(0, 0, 0), mk(0, 0, 0)};
+                if (cont) continue;
+                break;
+            }
+            V3 n = outside ? out : -out;
+            V3 rc = c0 + out * (R - 0.5 * delta);
+            V3 wk = o->yw[k];
+            V3 vc = vi + cross(wi, rc - pi) - cross(wk, rc - pk);
+            V3 wrel = wi - wk;
+            double m = 1. / o->sinvm[i];
+            V3 F = mk(0, 0, 0), T = mk(0, 0, 0);
+            int cij = combined_index(o->nmat, o->smat[i], o->ymat[k]);
+            run_contact_model(o, cij, F, T, h, vc, wrel, n, delta, m, ri, dt);
+            o->sacc[i] += F * o->sinvm[i];
+            o->storq[i] += T + cross(rc - pi, F);
+        }
+    }
+}
+
 /* calClumpForceTorque (integrate.cu:860-881) */
 static void clump_force_sum(orc_ctx* o) {
 #pragma omp parallel for schedule(static)
@@ -1080,6 +1146,7 @@ int orc_contact_forces(orc_ctx* o, double dt) {
     solid_bonds(o, dt);
     solid_force_sum(o);
     if (o->nw > 0) wall_forces(o, dt);
+    if (o->ny > 0) cylinder_forces(o, dt);
     clump_force_sum(o);
     return 0;
 }

@@ -288,3 +288,24 @@ def test_cpp_dam_break_case_matches_python_path(tmp_path):
     for name, a, b in zip(names, mine, arrays):
         assert np.array_equal(a, b), (name, float(np.max(np.abs(a - b))))
     assert os.path.isdir(tmp_path / "damBreakOutput")
+
+
+# ----------------------------------------------------------- cylinder walls
+def test_rotating_drum_cylinder_walls_match_the_oracle():
+    """Spheres in a turning drum (a cylinder wall, inside contacts) with a
+    fixed cylinder post above its axis (outside contacts), closed by two
+    planes: DEM state against the oracle after 300 steps."""
+    sc = scenes.rotating_drum()
+    assert sc.n_cylinder == 2
+    g, o = pair(sc, dsph.PREC_FP64)
+    n = 300
+    g.run(n, sc.dt, graphs=True)
+    o.run(n, sc.dt)
+    a, b = g.solid(), o.solid()
+    assert np.abs(b["angvel"]).max() > 1.0          # the drum's friction spun them
+    assert rel(a["pos"], b["pos"]) < 1e-10
+    for k in ("vel", "angvel"):
+        assert rel(a[k], b[k], 1e-3) < 1e-7, k
+    ca, cb = g.contacts(), o.contacts()
+    for k in ("force", "sliding", "rolling"):
+        assert rel(ca[k], cb[k]) < 1e-7, k

@@ -1201,3 +1201,66 @@ def test_global_damping_opposes_the_velocity():
     o2 = _oracle_for(sc2)
     _step_with_hook(o2, 1, dt, lambda x: x.add_global_damping(0, cd))
     assert o2.solid()["vel"][0][0] == 0.0
+
+
+# ------------------------------------------------ cylinder walls (K16)
+def _settle_against_cylinder(inside, g_vec, pos, axis, R):
+    E, G = 5.4945e6, 1.131e6
+    r, rho = 0.01, 2500.0
+    sc = scenes.empty_scene((-0.2, -0.2, -0.2), (0.4, 0.4, 0.4))
+    sc.gravity = np.asarray(g_vec, float)
+    scenes.add_solid(sc, [pos], r, rho)
+    sc.set_hertz(0, 0, E, G, 0.05, 0, 0, 0.4, 0, 0)
+    scenes.add_cylinder_wall(sc, (0, 0, 0), axis, (0, 0, 0), R, 0)
+    o = _oracle_for(sc)
+    for s in range(1, 6001):
+        o.step(s, 2e-5)
+    m = 4 / 3 * math.pi * r ** 3 * rho
+    d_expect = (m * np.linalg.norm(g_vec) / (4 / 3 * E * math.sqrt(r))) ** (2 / 3)
+    return o.solid()["pos"][0], r, d_expect
+
+
+def test_sphere_resting_inside_a_pipe_static_overlap():
+    # a horizontal pipe (axis x) of radius R holds a sphere at its bottom:
+    # the contact law takes the sphere's radius, so the overlap is that of
+    # a sphere on a plane, and the sphere sits R - r + delta below the axis
+    R, r = 0.05, 0.01
+    p, r, d = _settle_against_cylinder(True, (0, 0, -9.81), (0.0, 0.0, -(R - r) * 0.999), (2, 0, 0), R)
+    assert abs(p[0]) < 1e-12 and abs(p[1]) < 1e-12
+    assert -p[2] - (R - r) == pytest.approx(d, rel=2e-3)
+
+
+def test_sphere_pressed_on_the_outside_of_a_cylinder():
+    # gravity along -x presses a sphere against the outside of a vertical
+    # cylinder (axis z) at x = R + r: the overlap is the plane's again
+    R, r = 0.03, 0.01
+    p, r, d = _settle_against_cylinder(False, (-9.81, 0, 0), ((R + r) * 1.001, 0.0, 0.0), (0, 0, 1), R)
+    assert R + r - p[0] == pytest.approx(d, rel=2e-3)
+    assert abs(p[1]) < 1e-12 and abs(p[2]) < 1e-12
+
+
+def test_rotating_pipe_drags_a_resting_sphere_along_its_wall():
+    # the pipe turns about its axis: friction at the contact pushes the
+    # sphere the way the wall moves there (and spins it the other way)
+    R, r, rho = 0.05, 0.01, 2500.0
+    sc = scenes.empty_scene((-0.2, -0.2, -0.2), (0.4, 0.4, 0.4))
+    scenes.add_solid(sc, [[0.0, 0.0, -(R - r)]], r, rho)
+    sc.set_hertz(0, 0, 5.4945e6, 1.131e6, 0.5, 0, 0, 0.4, 0, 0)
+    w = np.array([5.0, 0.0, 0.0])                     # about the pipe axis x
+    scenes.add_cylinder_wall(sc, (0, 0, 0), (1, 0, 0), w, R, 0)
+    o = _oracle_for(sc)
+    for s in range(1, 201):
+        o.step(s, 2e-5)
+    v = o.solid()["vel"][0]
+    wall_v = np.cross(w, [0.0, 0.0, -R])               # the wall's velocity under the sphere
+    assert wall_v[1] > 0 and v[1] > 0
+    assert o.solid()["angvel"][0][0] > 0               # rolls with the wall's sense of turn
+    # a pipe at rest leaves it at rest sideways
+    sc2 = scenes.empty_scene((-0.2, -0.2, -0.2), (0.4, 0.4, 0.4))
+    scenes.add_solid(sc2, [[0.0, 0.0, -(R - r)]], r, rho)
+    sc2.hertz = sc.hertz
+    scenes.add_cylinder_wall(sc2, (0, 0, 0), (1, 0, 0), (0, 0, 0), R, 0)
+    o2 = _oracle_for(sc2)
+    for s in range(1, 201):
+        o2.step(s, 2e-5)
+    assert abs(o2.solid()["vel"][0][1]) < 1e-12
