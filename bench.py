@@ -63,7 +63,7 @@ def new_solver(scene, precision, flags=0, tuning=None):
     return g
 
 
-def timed_window(scene, precision, steps, warmup, timing=True, flags=0, tuning=None):
+def timed_window(scene, precision, steps, warmup, timing=True, flags=0, tuning=None, graphs=True):
     """Graph-replayed steps warmup+1 .. warmup+steps on a fresh context:
     (seconds, superset rebuilds in the window, kernel-family times).  The
     kernel times come from the same window run again from a checkpoint of
@@ -72,12 +72,12 @@ def timed_window(scene, precision, steps, warmup, timing=True, flags=0, tuning=N
     dt = min(sc.dt for sc in scene) if isinstance(scene, list) else scene.dt
     g = new_solver(scene, precision, flags, tuning)
     if warmup:
-        g.run(warmup, dt, first=1, graphs=True)
+        g.run(warmup, dt, first=1, graphs=graphs)
     blob = g.save_state() if timing else None
     g.synchronize()
     rb0 = int(g.status()[dsph.ST_REBUILDS])
     t0 = time.perf_counter()
-    g.run(steps, dt, first=warmup + 1, graphs=True)
+    g.run(steps, dt, first=warmup + 1, graphs=graphs)
     g.synchronize()
     secs = time.perf_counter() - t0
     rebuilds = int(g.status()[dsph.ST_REBUILDS]) - rb0
@@ -264,6 +264,25 @@ def other_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def fp64_only(scene, args):
+    """The all-FP64 leg alone (a profiler run keeps one context per process):
+    a JSON line with its step time, kernel-family times and roofline."""
+    import dsph
+    tun64 = tune(scene, dsph.PREC_FP64, 0, args)
+    g, secs, rb, times = timed_window(scene, dsph.PREC_FP64, args.steps, args.warmup, not args.no_timing,
+                                      tuning=tun64, graphs=not args.eager)
+    ff_directed, fs_pairs = pair_counts(g)
+    g.close()
+    roof = roofline(times, "fp64", scene.n_fluid, ff_directed, fs_pairs) if times else None
+    if roof:
+        roof["traffic"] = traffic_from_profile(roof["kernel"], "fp64")
+    print(json.dumps({"fp64_only": True, "ms_per_step": round(secs / args.steps * 1e3, 4),
+                      "steps": args.steps, "warmup": args.warmup, "superset_rebuilds": rb,
+                      "tuning": tun64, "roofline": roof,
+                      "kernel_ms": {k: [round(v[0] / max(v[1], 1), 4), v[1]] for k, v in times.items()}}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -278,6 +297,11 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-timing", action="store_true",
                     help="skip the eager re-run of the window with per-kernel HIP events")
+    ap.add_argument("--eager", action="store_true",
+                    help="launch every step's kernels directly instead of replaying HIP graphs "
+                         "(profiler counter runs)")
+    ap.add_argument("--fp64-only", action="store_true",
+                    help="only the all-FP64 leg, one context per process (profiler runs)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration)")
     ap.add_argument("--workload", default="coupled", choices=["coupled", "ensemble", "granular"],
@@ -313,9 +337,13 @@ def main():
     print(f"# workload: {scene.n_fluid} fluid + {scene.n_solid} solid, dt={scene.dt:.3e}",
           file=sys.stderr, flush=True)
 
+    if args.fp64_only:
+        fp64_only(scene, args)
+        return
+
     tun = tune(scene, dsph.PREC_MIXED, 0, args)
     g, secs, rebuilds, times = timed_window(scene, dsph.PREC_MIXED, args.steps, args.warmup,
-                                            not args.no_timing, tuning=tun)
+                                            not args.no_timing, tuning=tun, graphs=not args.eager)
     value = npart * args.steps / secs
     print(f"# mixed: {secs / args.steps * 1e3:.3f} ms/step  {value:.4g} particle-steps/s, "
           f"{rebuilds} rebuilds", file=sys.stderr, flush=True)
@@ -329,7 +357,7 @@ def main():
     if not args.no_fp64:
         tun64 = tune(scene, dsph.PREC_FP64, 0, args)
         g64, s64, rb64, t64 = timed_window(scene, dsph.PREC_FP64, args.steps, args.warmup,
-                                           not args.no_timing, tuning=tun64)
+                                           not args.no_timing, tuning=tun64, graphs=not args.eager)
         g64.close()
         fp64 = {"value": round(npart * args.steps / s64, 1),
                 "ms_per_step": round(s64 / args.steps * 1e3, 4), "steps": args.steps,
