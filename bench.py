@@ -99,6 +99,16 @@ def pair_counts(g):
     return 2 * len(a), len(fs)
 
 
+def candidates(g, ff_directed):
+    """Candidates per fluid particle the pair loops evaluate (its group's
+    superset row) against its neighbours in the fluid-fluid list."""
+    st = g.superset_stats()
+    n = max(st["particles"], 1)
+    return {"walked_per_particle": round(st["walked"] / n, 2),
+            "neighbours_per_particle": round(ff_directed / n, 2),
+            "stored_per_particle": round(st["stored"] / n, 2), "longest_row": st["longest"]}
+
+
 def roofline(times, prec_key, nf, ff_directed, fs_pairs):
     best = None
     for fam in ("continuity", "momentum"):
@@ -348,6 +358,7 @@ def main():
     print(f"# mixed: {secs / args.steps * 1e3:.3f} ms/step  {value:.4g} particle-steps/s, "
           f"{rebuilds} rebuilds", file=sys.stderr, flush=True)
     ff_directed, fs_pairs = pair_counts(g)
+    cand = candidates(g, ff_directed)
     roof = roofline(times, "mixed", scene.n_fluid, ff_directed, fs_pairs) if times else None
     if roof:
         roof["traffic"] = traffic_from_profile(roof["kernel"], "mixed")
@@ -419,6 +430,7 @@ def main():
         "roofline": roof,
         "kernel_ms": {k: [round(v[0] / max(v[1], 1), 4), v[1]] for k, v in times.items()},
         "pairs": {"fluid_fluid_directed": ff_directed, "fluid_solid": fs_pairs},
+        "candidates": cand,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

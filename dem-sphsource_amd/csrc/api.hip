@@ -722,6 +722,15 @@ void read_status(dsph_ctx* c, unsigned long long* w) {
     HIPCHK(hipStreamSynchronize(c->st));
 }
 
+// a search that met a NaN/inf position (the hash kernels count them): the
+// state cannot recover from it, so the step reports it instead of going on
+void check_finite(const unsigned long long* w) {
+    if (w[ST_NONFINITE] == 0) return;
+    throw ApiError{DSPH_E_NONFINITE, std::to_string(w[ST_NONFINITE]) +
+                                         " particle(s) with a NaN or infinite position; reload a "
+                                         "saved state or upload the scene again"};
+}
+
 // search with the reference's grow-on-demand semantics (setNum realloc,
 // deviceStructs.h:483-491, 668-691): on overflow the lists grow and the
 // search is repeated without a second reorder or history save
@@ -730,6 +739,7 @@ void search_checked(dsph_ctx* c, int i_step, int gap) {
     for (int attempt = 0; attempt < 8; attempt++) {
         unsigned long long w[ST_WORDS];
         read_status(c, w);
+        check_finite(w);
         const bool ovf = w[ST_FF_OVERFLOW] | w[ST_FS_OVERFLOW] | w[ST_SS_OVERFLOW] | w[ST_SUP_OVERFLOW];
         if (!ovf) {
             // superset rows change only at builds, which graph-replayed steps
@@ -1101,7 +1111,21 @@ void validate_scene(const dsph_scene* s) {
         require(s->clump_pebble_start[k] >= 0 && s->clump_pebble_end[k] <= s->n_solid &&
                     s->clump_pebble_start[k] <= s->clump_pebble_end[k],
                 "clump pebble range out of bounds");
-    for (int i = 0; i < s->n_fluid; i++) require(s->fluid_h[i] > 0, "fluid smoothing length must be > 0");
+    for (int i = 0; i < s->n_fluid; i++)
+        require(s->fluid_h[i] > 0 && std::isfinite(s->fluid_h[i]), "fluid smoothing length must be finite and > 0");
+    // positions and sizes build the grid: NaN or inf there is rejected here
+    // (a position that turns non-finite while stepping is reported by the
+    // step instead, DSPH_E_NONFINITE)
+    auto finite3 = [](const double* p, int n) {
+        for (int64_t i = 0; i < 3 * (int64_t)n; i++)
+            if (!std::isfinite(p[i])) return false;
+        return true;
+    };
+    require(s->n_fluid == 0 || finite3(s->fluid_pos, s->n_fluid), "a fluid position is not finite");
+    require(s->n_solid == 0 || finite3(s->solid_pos, s->n_solid), "a solid position is not finite");
+    for (int i = 0; i < s->n_solid; i++)
+        require(std::isfinite(s->solid_radius[i]) && std::isfinite(s->solid_eff_radius[i]),
+                "a solid radius is not finite");
 }
 
 // the grid of one scene: cell = 2 max(all effective radii), G = int(L / cell)
@@ -2154,6 +2178,9 @@ int dsph_step_n(dsph_ctx* c, int32_t i0, int32_t n, double dt, int32_t gap) {
         }
         unsigned long long w[ST_WORDS];
         read_status(c, w);
+        // a NaN/inf position met by the last search: reported, not rolled
+        // back (the steps before it would only run into it again)
+        check_finite(w);
         if (w[ST_FF_OVERFLOW] | w[ST_FS_OVERFLOW] | w[ST_SS_OVERFLOW] | w[ST_SUP_OVERFLOW]) {
             // some search of the run overflowed a list and every later
             // kernel read cut-short rows: back to the state before the run
@@ -2712,6 +2739,43 @@ int dsph_status(dsph_ctx* c, int64_t words[DSPH_STATUS_WORDS]) {
         unsigned long long w[ST_WORDS];
         read_status(c, w);
         for (int k = 0; k < ST_WORDS; k++) words[k] = (int64_t)w[k];
+        return DSPH_OK;
+    } catch (const ApiError& e) {
+        return fail(c, e);
+    }
+}
+
+int dsph_superset_stats(dsph_ctx* c, int64_t out[4]) {
+    API_GUARD(c);
+    if (!out) return DSPH_E_INVALID;
+    if (!c->initialized) return DSPH_E_STATE;
+    try {
+        const int nf = c->nf;
+        std::vector<int> cnt(nf), grow(nf), sup(nf);
+        if (nf > 0) {
+            HIPCHK(hipMemcpyAsync(cnt.data(), c->L.cnt, nf * sizeof(int), hipMemcpyDeviceToHost, c->st));
+            HIPCHK(hipMemcpyAsync(grow.data(), c->L.grow, nf * sizeof(int), hipMemcpyDeviceToHost, c->st));
+            HIPCHK(hipMemcpyAsync(sup.data(), c->L.sup_cnt, nf * sizeof(int), hipMemcpyDeviceToHost, c->st));
+        }
+        HIPCHK(hipStreamSynchronize(c->st));
+        int64_t walked = 0, stored = 0, longest = 0;
+        for (int s = 0; s < nf; s++) {
+            int64_t m;
+            if (cnt[s] >= 0) {  // walked on the reference stencil this step
+                m = std::min(cnt[s], c->L.K);
+                stored += m;
+            } else {
+                const int r = row_slot(grow[s]);
+                m = std::min(sup[r], c->L.supK);
+                if (r == s) stored += m;  // the group's row, counted once
+            }
+            walked += m;
+            longest = std::max(longest, m);
+        }
+        out[0] = nf;
+        out[1] = walked;
+        out[2] = stored;
+        out[3] = longest;
         return DSPH_OK;
     } catch (const ApiError& e) {
         return fail(c, e);

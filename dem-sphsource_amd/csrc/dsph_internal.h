@@ -39,6 +39,7 @@ enum {
     ST_SUP_OVERFLOW = 7,
     ST_REBUILDS = 8,
     ST_SUP_MAX = 9,        // longest (group) superset row of this search's build
+    ST_NONFINITE = 10,     // fluid + live solids with a NaN/inf position at this search
     ST_WORDS = 12
 };
 

@@ -105,7 +105,8 @@ __device__ __forceinline__ void wave_max_atomic(unsigned long long* addr, unsign
 __global__ void k_status_reset(unsigned long long* st) {
     int i = threadIdx.x;
     // overflow flags are sticky until read; counters are per search
-    if (i == ST_OUT_OF_DOMAIN || i == ST_FF_MAX || i == ST_FS_TOTAL || i == ST_SS_TOTAL || i == ST_SUP_MAX)
+    if (i == ST_OUT_OF_DOMAIN || i == ST_FF_MAX || i == ST_FS_TOTAL || i == ST_SS_TOTAL || i == ST_SUP_MAX ||
+        i == ST_NONFINITE)
         st[i] = 0;
 }
 
@@ -183,11 +184,14 @@ __global__ void __launch_bounds__(kBS) k_hash_fluid(Grids G, const double4* __re
                                                     int* __restrict__ near_out,
                                                     unsigned long long* ctl, unsigned long long* st) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
-    bool out = false, near = false, back = false;
+    bool out = false, near = false, back = false, bad = false;
     double d2 = 0.0;
     if (s < n) {
         const GridP& g = grid_at(G, fluid_scene(G, s));
         double4 p = ph[s];
+        // a NaN/inf position is outside every domain: the spare cell, whose
+        // walks skip it (no distance test holds), and it is counted
+        bad = !(isfinite(p.x) && isfinite(p.y) && isfinite(p.z));
         const int hl = point_hash(g, p.x, p.y, p.z);
         const int h = g.cell0 + hl;
         const int o = orig[s];
@@ -206,6 +210,8 @@ __global__ void __launch_bounds__(kBS) k_hash_fluid(Grids G, const double4* __re
     }
     const unsigned long long m = __ballot(out);
     if ((threadIdx.x & 63) == 0 && m) atomicAdd(&st[ST_OUT_OF_DOMAIN], (unsigned long long)__popcll(m));
+    const unsigned long long mb = __ballot(bad);
+    if ((threadIdx.x & 63) == 0 && mb) atomicAdd(&st[ST_NONFINITE], (unsigned long long)__popcll(mb));
     if ((threadIdx.x & 63) == 0 && __ballot(near) != 0) atomicOr(&ctl[CTL_NEAR_SPARE], 1ull);
     if ((threadIdx.x & 63) == 0 && __ballot(back) != 0) atomicOr(&ctl[CTL_FORCE], 1ull);
     // one maximum per workgroup, reduced by the decision (no contended
@@ -229,15 +235,19 @@ __global__ void k_hash_solid(Grids G, const double4* __restrict__ pr, int n,
                              unsigned* __restrict__ key, unsigned long long* st) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long out = 0;
+    bool bad = false;
     if (s < n) {
         const GridP& g = grid_at(G, solid_scene(G, s));
         double4 p = pr[s];
+        bad = !(isfinite(p.x) && isfinite(p.y) && isfinite(p.z));
         const int hl = point_hash(g, p.x, p.y, p.z);
         key[s] = (unsigned)(g.cell0 + hl);
         out = hl == g.ncell - 1;
     }
     unsigned long long m = __ballot(out != 0);
     if ((threadIdx.x & 63) == 0 && m) atomicAdd(&st[ST_OUT_OF_DOMAIN], (unsigned long long)__popcll(m));
+    const unsigned long long mb = __ballot(bad);
+    if ((threadIdx.x & 63) == 0 && mb) atomicAdd(&st[ST_NONFINITE], (unsigned long long)__popcll(mb));
 }
 
 // --------------------------------------------------- rebuild decision
@@ -273,12 +283,13 @@ __global__ void __launch_bounds__(kBS) k_hash_live(Grids G, const double4* __res
                                                    int always, int always_sort, unsigned long long* ctl,
                                                    unsigned long long* st) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    bool out = false, back = false;
+    bool out = false, back = false, bad = false;
     double d2 = 0.0;
     if (t < n) {
         const int i = live[t];
         const GridP& g = grid_at(G, solid_scene(G, i));
         const double4 p = pr[i];
+        bad = !(isfinite(p.x) && isfinite(p.y) && isfinite(p.z));
         const int hl = point_hash(g, p.x, p.y, p.z);
         key[i] = (unsigned)(g.cell0 + hl);
         out = hl == g.ncell - 1;
@@ -292,6 +303,8 @@ __global__ void __launch_bounds__(kBS) k_hash_live(Grids G, const double4* __res
     }
     const unsigned long long m = __ballot(out);
     if ((threadIdx.x & 63) == 0 && m) atomicAdd(&st[ST_OUT_OF_DOMAIN], (unsigned long long)__popcll(m));
+    const unsigned long long mb = __ballot(bad);
+    if ((threadIdx.x & 63) == 0 && mb) atomicAdd(&st[ST_NONFINITE], (unsigned long long)__popcll(mb));
     if (t == 0 && static_out) atomicAdd(&st[ST_OUT_OF_DOMAIN], static_out);
     if ((threadIdx.x & 63) == 0 && __ballot(back) != 0) atomicOr(&ctl[CTL_FORCE], 1ull);
     if (!decide) return;

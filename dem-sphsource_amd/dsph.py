@@ -28,7 +28,9 @@ TUNE_PAIR_BLOCK, TUNE_BUILD_BLOCK, TUNE_GROUP = 1, 2, 3
 LIST_FLUID_FLUID, LIST_FLUID_SOLID, LIST_SOLID_SOLID, LIST_BOND = 0, 1, 2, 3
 ST_FF_OVERFLOW, ST_FS_OVERFLOW, ST_SS_OVERFLOW, ST_OUT_OF_DOMAIN = 0, 1, 2, 3
 ST_FF_MAX, ST_FS_TOTAL, ST_SS_TOTAL, ST_SUP_OVERFLOW, ST_REBUILDS = 4, 5, 6, 7, 8
+ST_SUP_MAX, ST_NONFINITE = 9, 10
 STATUS_WORDS = 12
+E_INVALID, E_HIP, E_CAPACITY, E_STATE, E_NOMEM, E_NODEVICE, E_NONFINITE = -1, -2, -3, -4, -5, -6, -7
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -73,6 +75,7 @@ _SIGS = {
     "dsph_grid_info": [_ip, _dp, _ip],
     "dsph_download_grid": [C.c_int, _ip, _ip, _ip, _ip],
     "dsph_status": [_i64p],
+    "dsph_superset_stats": [_i64p],
     "dsph_hash_sort": [_vp, C.c_int32, _dp, _dp, _ip, _vp, _vp, _vp, _vp],
     "dsph_hash_sort_noalloc": [_vp, C.c_int32, _dp, _dp, _ip, _vp, _vp, _vp, _vp],
     "dsph_set_timing": [C.c_int],
@@ -359,6 +362,13 @@ class Solver:
         w = np.zeros(STATUS_WORDS, np.int64)
         self._call("dsph_status", _p(w, _i64p))
         return w
+
+    def superset_stats(self):
+        """Candidates the pair loops walk (rows of the last search):
+        {"particles", "walked", "stored", "longest"}."""
+        w = np.zeros(4, np.int64)
+        self._call("dsph_superset_stats", _p(w, _i64p))
+        return dict(particles=int(w[0]), walked=int(w[1]), stored=int(w[2]), longest=int(w[3]))
 
     def hash_sort(self, pos_ptr, n, origin, size, cells, hash_ptr, perm_ptr, start_ptr, end_ptr,
                   noalloc=False):

@@ -30,6 +30,8 @@ extern "C" {
 #endif
 
 /* ---- status codes ---------------------------------------------------- */
+/* Scenes with a NaN/inf position, radius or smoothing length are rejected
+ * at upload (DSPH_E_INVALID).                                              */
 #define DSPH_OK             0
 #define DSPH_E_INVALID    (-1)  /* bad argument / inconsistent scene       */
 #define DSPH_E_HIP        (-2)  /* HIP runtime error                         */
@@ -37,6 +39,8 @@ extern "C" {
 #define DSPH_E_STATE      (-4)  /* call out of order (e.g. step before init) */
 #define DSPH_E_NOMEM      (-5)  /* device or host allocation failed          */
 #define DSPH_E_NODEVICE   (-6)  /* no HIP device / bad device index          */
+#define DSPH_E_NONFINITE  (-7)  /* a particle position became NaN or inf
+                                   (DSPH_STATUS_NONFINITE)                   */
 
 /* ---- precision of the fluid pair loops ------------------------------- */
 #define DSPH_PREC_FP64   0  /* everything FP64 (the reference's arithmetic)   */
@@ -75,6 +79,10 @@ extern "C" {
 #define DSPH_STATUS_SUP_OVERFLOW  7 /* a fluid Verlet superset row overflowed */
 #define DSPH_STATUS_REBUILDS      8 /* superset rebuilds since upload        */
 #define DSPH_STATUS_SUP_MAX       9 /* longest superset row of the last build */
+#define DSPH_STATUS_NONFINITE    10 /* fluid particles + movable solids with a
+                                       NaN/inf position at the last search
+                                       (such a position never recovers: the
+                                       state stays poisoned until reloaded) */
 #define DSPH_STATUS_WORDS        12
 
 /*
@@ -250,7 +258,8 @@ int dsph_solid_after_contact(dsph_ctx* ctx, double dt);
 /* solverBase::update without the user hook and output
  * (solverBase.cpp:47-59): the five phases in order.  check_capacity != 0
  * synchronises after the search and grows lists that overflowed, re-running
- * the search (the reference's setNum realloc, deviceStructs.h:483-491).   */
+ * the search (the reference's setNum realloc, deviceStructs.h:483-491); it
+ * returns DSPH_E_NONFINITE when that search met a NaN/inf position.        */
 int dsph_step(dsph_ctx* ctx, int32_t i_step, double dt, int32_t gap,
               int check_capacity);
 /* n consecutive steps i_step0 .. i_step0+n-1 replayed from captured HIP
@@ -260,8 +269,10 @@ int dsph_step(dsph_ctx* ctx, int32_t i_step, double dt, int32_t gap,
  * n steps again eagerly with checked searches (as dsph_step with
  * check_capacity), which grow what overflows.  The copy costs one
  * device-to-device pass over the state per call and device memory of the
- * state's size.  No reference counterpart (the reference syncs three times
- * per step, neighborSearch.cu:339-340,373). */
+ * state's size.  A NaN/inf position met by the last search of the run is
+ * returned as DSPH_E_NONFINITE (nothing is rolled back).  No reference
+ * counterpart (the reference syncs three times per step,
+ * neighborSearch.cu:339-340,373). */
 int dsph_step_n(dsph_ctx* ctx, int32_t i_step0, int32_t n, double dt,
                 int32_t gap);
 int dsph_synchronize(dsph_ctx* ctx);
@@ -348,6 +359,16 @@ int dsph_download_grid(dsph_ctx* ctx, int species, int32_t* hash,
                        int32_t* perm, int32_t* cell_start, int32_t* cell_end);
 
 int dsph_status(dsph_ctx* ctx, int64_t words[DSPH_STATUS_WORDS]);
+
+/* Candidates the fluid pair loops walk, from the rows of the last search
+ * (synchronous, diagnostic): out[0] fluid particles, out[1] candidates summed
+ * over particles (a particle walks its group's whole superset row, or its
+ * exact row when walked on the reference stencil), out[2] row entries
+ * stored (a shared row once), out[3] the longest row.  Dividing out[1] by
+ * out[0] gives the candidates per particle that the pair loops evaluate,
+ * against the neighbours per particle of the fluid-fluid list.  No
+ * reference counterpart (the reference walks its 27 cells every pass). */
+int dsph_superset_stats(dsph_ctx* ctx, int64_t out[4]);
 
 /* ---- the hashed grid of caller-held particles --------------------------
  * replaces: updateGridCellStartEnd for one species (neighborSearch.cu:89-113)

@@ -309,3 +309,77 @@ def test_rotating_drum_cylinder_walls_match_the_oracle():
     ca, cb = g.contacts(), o.contacts()
     for k in ("force", "sliding", "rolling"):
         assert rel(ca[k], cb[k]) < 1e-7, k
+
+
+# ------------------------------------------------------------ NaN injection
+# A NaN injected from the host spreads through the pair sums into positions;
+# the hash kernels count non-finite positions and the step reports them.
+# Every kernel stays in bounds for such a particle: it hashes to the spare
+# cell, and no distance test against it holds.
+@pytest.mark.parametrize("prec", [dsph.PREC_FP64, dsph.PREC_MIXED])
+def test_nan_fluid_velocity_is_reported_by_the_step(prec):
+    sc = scenes.immersed_spheres()
+    k = sc.n_fluid // 2
+    sc.fluid_vel[k, 0] = np.nan
+    note(f"NaN fluid velocity at particle {k}, prec={prec}")
+    g = dsph.Solver(prec)
+    g.upload(sc)
+    g.initialize()
+    code = None
+    for s in range(1, 6):
+        try:
+            g.step(s, sc.dt)
+        except dsph.DsphError as e:
+            code, at = e.code, s
+            assert "NaN or infinite position" in str(e)
+            break
+    assert code == dsph.E_NONFINITE
+    # the first drift puts the NaN into the particle's position (step 1);
+    # the search of step 2 finds it
+    assert at == 2
+    assert g.status()[dsph.ST_NONFINITE] >= 1
+    # graph replay reports it too, after the run
+    with pytest.raises(dsph.DsphError) as e:
+        g.step_n(at + 1, 3, sc.dt)
+    assert e.value.code == dsph.E_NONFINITE
+    # the clean scene uploaded again clears it
+    g.upload(scenes.immersed_spheres())
+    g.initialize()
+    g.step(1, sc.dt)
+    g.step(2, sc.dt)
+    assert g.status()[dsph.ST_NONFINITE] == 0
+
+
+def test_nan_external_force_on_a_sphere_is_reported():
+    sc = scenes.dem_contact_mix()
+    g = dsph.Solver(dsph.PREC_FP64)
+    g.upload(sc)
+    g.initialize()
+    g.step(1, sc.dt)
+    i = int(np.flatnonzero(np.asarray(sc.solid_inv_mass) > 0)[0])
+    g.neighbor_search(2, 1)
+    g.solid_before_contact(sc.dt)
+    g.fluid_integrate(sc.dt, 2, 1)
+    g.contact_forces(sc.dt)
+    g.add_external_force(i, [np.nan, 0.0, 0.0])
+    g.solid_after_contact(sc.dt)
+    # after-contact kick then before-contact drift: the position of step 3
+    with pytest.raises(dsph.DsphError) as e:
+        for s in (3, 4):
+            g.step(s, sc.dt)
+    assert e.value.code == dsph.E_NONFINITE
+    assert g.status()[dsph.ST_NONFINITE] >= 1
+
+
+def test_non_finite_positions_are_rejected_at_upload():
+    g = dsph.Solver(dsph.PREC_MIXED)
+    for field, val in (("fluid_pos", np.nan), ("solid_pos", np.inf), ("solid_radius", np.nan)):
+        sc = scenes.immersed_spheres()
+        arr = getattr(sc, field)
+        arr.reshape(-1)[arr.size // 2] = val
+        with pytest.raises(dsph.DsphError) as e:
+            g.upload(sc)
+        assert e.value.code == dsph.E_INVALID and "not finite" in str(e.value), field
+    g.upload(scenes.immersed_spheres())
+    g.initialize()
+    g.step(1, scenes.immersed_spheres().dt)
