@@ -402,23 +402,55 @@ __device__ __forceinline__ float slab_dist(float p, float lo, float w) {
     return fmaxf(fmaxf(lo - p, p - (lo + w)), 0.0f);
 }
 
+__device__ __forceinline__ float wave_minf(float v) {
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) v = fminf(v, __shfl_xor(v, d, 64));
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ float wave_maxf(float v) {
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, 64));
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+// lanes below this one in a wave mask
+__device__ __forceinline__ int lanes_below(unsigned long long m) {
+    return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+// A wavefront's queue of staged candidates in LDS: up to 2 x 64 entries of
+// (x, y, z relative to the box corner, reach^2) and the candidate's index.
+// Candidates are appended in walk order, and the membership scan takes them
+// 64 at a time, so a scan runs on full batches however few candidates each
+// cell contributes.
+struct LdsQueue {
+    float4* q;
+    int* j;
+    int n;  // wave-uniform
+};
+constexpr int kQueue = 128;
+
 // Verlet superset build (setFluidInteractionsKernel's loop, radius + skin).
 // One wavefront per chunk of <= 64 consecutive sorted particles of one scene
 // (Morton cell order, so a chunk covers a few neighbouring cells), one lane
 // per particle.  The wavefront walks the cells of the chunk's bounding box
 // widened by the superset stencil half-width, skipping every z slab, y row
 // and cell that no lane can reach.  The members of a cell in reach are
-// staged 64 at a time through the wavefront's LDS slice (one coalesced load
-// per lane) and then read back by every lane at the same address, so the
-// membership scan is a broadcast LDS read and a few FP32 operations per
-// candidate, with no global-memory latency in its loop.  Geometry is FP32
-// relative to a corner of the box (a few cells away: ~1e-8 m rounding);
-// membership is d^2 <= R^2 (1 + 2e-4) with R = 2 max(h_i, h_j) + skin (2 h_i
-// + skin for solids), evaluated as max((2 h_i + skin)^2, (2 h_j + skin)^2).
-// A superset only has to contain every pair that can come within range
-// before the next rebuild, so the margin errs on the inclusive side.
-// Spare-cell particles and spare-cell candidates are left out: they are
-// walked exactly every step.
+// loaded 64 at a time (one coalesced load per lane), and only those inside
+// the lanes' bounding box widened by the largest reach enter the
+// wavefront's LDS queue: on the reference grid of inflated cells (cell = 2
+// max effective radius, DEM spheres included) that box holds about a third
+// of the cells' members.  The queue is then read back by every lane at the
+// same address, so the membership scan is a broadcast LDS read and a few
+// FP32 operations per candidate, with no global-memory latency in its loop.
+// Culling by the box removes only candidates no lane can reach, and keeps
+// the walk order, so the rows are those of a scan over every member.
+// Geometry is FP32 relative to a corner of the box (a few cells away: ~1e-8
+// m rounding); membership is d^2 <= R^2 (1 + 2e-4) with R = 2 max(h_i, h_j)
+// + skin (2 h_i + skin for solids), evaluated as max((2 h_i + skin)^2,
+// (2 h_j + skin)^2).  A superset only has to contain every pair that can
+// come within range before the next rebuild, so the margin errs on the
+// inclusive side.  Spare-cell particles and spare-cell candidates are left
+// out: they are walked exactly every step.
 __global__ void __launch_bounds__(kBS) k_build(SearchArgs a, const int2* __restrict__ chunks,
                                                int nchunk) {
     if (a.ctl[CTL_REBUILD] == 0) return;
@@ -430,10 +462,11 @@ __global__ void __launch_bounds__(kBS) k_build(SearchArgs a, const int2* __restr
         const int i = a.slive[t];
         a.sxb[i] = a.spr[i];
     }
-    __shared__ float4 lq[kBS];  // candidate (x, y, z, reach^2), one 64-slot slice per wavefront
-    __shared__ int lj[kBS];
+    // per wavefront: a fluid queue and a solid queue
+    __shared__ float4 lq[2 * kQueue * (kBS / 64)];
+    __shared__ int lj[2 * kQueue * (kBS / 64)];
     const int lane = threadIdx.x & 63;
-    const int wbase = threadIdx.x & ~63;
+    const int wq = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * 2 * kQueue;
     const int wv = xcd_block() * (kBS / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nwv = gridDim.x * (kBS / 64);
     const int supK = a.L.supK, fsupK = a.L.fsupK;
@@ -495,7 +528,85 @@ __global__ void __launch_bounds__(kBS) k_build(SearchArgs a, const int2* __restr
         const float cullf = act ? Rf * Rf * grow : -1.0f, culls = act ? Ai : -1.0f;
         const float csx = (float)g.csx, csy = (float)g.csy, csz = (float)g.csz;
         const float pad = (float)g.pad;
+        // the lanes' bounding box widened by the largest reach of each kind
+        // (a margin of 1e-3 over the scan's 1 + 2e-4: a culled candidate is
+        // out of every lane's reach)
+        const float big = 3.0e38f;
+        const float bx0 = wave_minf(act ? px : big), bx1 = wave_maxf(act ? px : -big);
+        const float by0 = wave_minf(act ? py : big), by1 = wave_maxf(act ? py : -big);
+        const float bz0 = wave_minf(act ? pz : big), bz1 = wave_maxf(act ? pz : -big);
+        const float mf = 1.001f * wave_maxf(act ? Rf : 0.0f), ms = 1.001f * wave_maxf(act ? Ri : 0.0f);
+        auto in_box = [&](float x, float y, float z, float m) {
+            return x >= bx0 - m && x <= bx1 + m && y >= by0 - m && y <= by1 + m && z >= bz0 - m && z <= bz1 + m;
+        };
+        LdsQueue qf{lq + wq, lj + wq, 0}, qs{lq + wq + kQueue, lj + wq + kQueue, 0};
         int c = 0, cs = 0;
+        // membership scan of the first nb (<= 64) queued fluid candidates
+        auto scan_fluid = [&](int nb) {
+            for (int k = 0; k < nb; k += 4) {
+                // four broadcast reads in flight; entries past nb (within
+                // the queue) are read but not used
+                float4 q[4];
+                int jj[4];
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    q[v] = qf.q[k + v];
+                    jj[v] = qf.j[k + v];
+                }
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const float ex = px - q[v].x, ey = py - q[v].y, ez = pz - q[v].z;
+                    const float e2 = ex * ex + ey * ey + ez * ez;
+                    const bool in = act & (k + v < nb) & (jj[v] != s) & (e2 <= (Ai > q[v].w ? Ai : q[v].w));
+                    const bool gin = (__ballot(in) & gmask) != 0;
+                    if (writer && gin && c < supK) a.L.sup[row_at(gcode, c, a.nf)] = jj[v];
+                    c += gin;
+                }
+            }
+        };
+        auto scan_solid = [&](int nb) {
+            for (int k = 0; k < nb; k += 4) {
+                float4 q[4];
+                int aa[4];
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    q[v] = qs.q[k + v];
+                    aa[v] = qs.j[k + v];
+                }
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const float ex = px - q[v].x, ey = py - q[v].y, ez = pz - q[v].z;
+                    const bool in = (k + v < nb) & (ex * ex + ey * ey + ez * ez <= culls);
+                    if (in && cs < fsupK) a.L.fsup[(int64_t)cs * a.nf + s] = aa[v];
+                    cs += in;
+                }
+            }
+        };
+        // appends this lane's candidate (keep) in lane order; a full batch
+        // of 64 is scanned and the rest moved to the queue's front
+        auto push = [&](LdsQueue& Q, bool keep, float4 v, int j, auto&& scan) {
+            const unsigned long long km = __ballot(keep);
+            if (keep) {
+                const int at = Q.n + lanes_below(km);
+                Q.q[at] = v;
+                Q.j[at] = j;
+            }
+            Q.n += __popcll(km);
+            __builtin_amdgcn_wave_barrier();
+            if (Q.n >= 64) {
+                scan(64);
+                __builtin_amdgcn_wave_barrier();
+                const float4 mv = Q.q[64 + (lane & 63)];
+                const int mj = Q.j[64 + (lane & 63)];
+                __builtin_amdgcn_wave_barrier();
+                if (lane < Q.n - 64) {
+                    Q.q[lane] = mv;
+                    Q.j[lane] = mj;
+                }
+                Q.n -= 64;
+                __builtin_amdgcn_wave_barrier();
+            }
+        };
         for (int zc = Z0; zc <= Z1; zc++) {
             const float dz = slab_dist(pz, (zc - Z0) * csz - pad, csz + 2.0f * pad);
             const float dz2 = dz * dz;
@@ -513,38 +624,18 @@ __global__ void __launch_bounds__(kBS) k_build(SearchArgs a, const int2* __restr
                         const int en = __builtin_amdgcn_readfirstlane(a.fce[h]);
                         for (int base = st0; st0 >= 0 && base < en; base += 64) {
                             const int u = base + lane;
+                            bool keep = false;
+                            float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                            int j = 0;
                             if (u < en) {
-                                const int j = a.fmember[u];
+                                j = a.fmember[u];
                                 const double4 q = a.ph[j];
                                 const float Rj = 2.0f * (float)q.w + skin;
-                                lq[wbase + lane] = make_float4((float)(q.x - ox), (float)(q.y - oy),
-                                                               (float)(q.z - oz), Rj * Rj * grow);
-                                lj[wbase + lane] = j;
+                                v = make_float4((float)(q.x - ox), (float)(q.y - oy), (float)(q.z - oz),
+                                                Rj * Rj * grow);
+                                keep = in_box(v.x, v.y, v.z, mf);
                             }
-                            __builtin_amdgcn_wave_barrier();
-                            const int nb = min(64, en - base);
-                            for (int k = 0; k < nb; k += 4) {
-                                // four broadcast reads in flight; slots past nb
-                                // (within the slice) are read but not used
-                                float4 q[4];
-                                int jj[4];
-#pragma unroll
-                                for (int v = 0; v < 4; v++) {
-                                    q[v] = lq[wbase + ((k + v) & 63)];
-                                    jj[v] = lj[wbase + ((k + v) & 63)];
-                                }
-#pragma unroll
-                                for (int v = 0; v < 4; v++) {
-                                    const float ex = px - q[v].x, ey = py - q[v].y, ez = pz - q[v].z;
-                                    const float e2 = ex * ex + ey * ey + ez * ez;
-                                    const bool in = act & (k + v < nb) & (jj[v] != s) &
-                                                    (e2 <= (Ai > q[v].w ? Ai : q[v].w));
-                                    const bool gin = (__ballot(in) & gmask) != 0;
-                                    if (writer && gin && c < supK) a.L.sup[row_at(gcode, c, a.nf)] = jj[v];
-                                    c += gin;
-                                }
-                            }
-                            __builtin_amdgcn_wave_barrier();
+                            push(qf, keep, v, j, scan_fluid);
                         }
                     }
                     if (__ballot(d2 <= culls) != 0) {
@@ -556,38 +647,26 @@ __global__ void __launch_bounds__(kBS) k_build(SearchArgs a, const int2* __restr
                             const int en = __builtin_amdgcn_readfirstlane(gce[h]);
                             for (int base = st1; st1 >= 0 && base < en; base += 64) {
                                 const int u = base + lane;
+                                bool keep = false;
+                                float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                                int ai = 0;
                                 if (u < en) {
-                                    const int ai = grid ? a.active[a.aperm[u]] : a.sperm[u];
+                                    ai = grid ? a.active[a.aperm[u]] : a.sperm[u];
                                     const double4 q = a.spr[ai];
-                                    lq[wbase + lane] = make_float4((float)(q.x - ox), (float)(q.y - oy),
-                                                                   (float)(q.z - oz), 0.0f);
-                                    lj[wbase + lane] = ai;
+                                    v = make_float4((float)(q.x - ox), (float)(q.y - oy), (float)(q.z - oz), 0.0f);
+                                    keep = in_box(v.x, v.y, v.z, ms);
                                 }
-                                __builtin_amdgcn_wave_barrier();
-                                const int nb = min(64, en - base);
-                                for (int k = 0; k < nb; k += 4) {
-                                    float4 q[4];
-                                    int aa[4];
-#pragma unroll
-                                    for (int v = 0; v < 4; v++) {
-                                        q[v] = lq[wbase + ((k + v) & 63)];
-                                        aa[v] = lj[wbase + ((k + v) & 63)];
-                                    }
-#pragma unroll
-                                    for (int v = 0; v < 4; v++) {
-                                        const float ex = px - q[v].x, ey = py - q[v].y, ez = pz - q[v].z;
-                                        const bool in = (k + v < nb) & (ex * ex + ey * ey + ez * ez <= culls);
-                                        if (in && cs < fsupK) a.L.fsup[(int64_t)cs * a.nf + s] = aa[v];
-                                        cs += in;
-                                    }
-                                }
-                                __builtin_amdgcn_wave_barrier();
+                                push(qs, keep, v, ai, scan_solid);
                             }
                         }
                     }
                 }
             }
         }
+        // what is left in the queues
+        if (qf.n > 0) scan_fluid(qf.n);
+        if (qs.n > 0) scan_solid(qs.n);
+        __builtin_amdgcn_wave_barrier();
         if (act) {
             a.L.sup_cnt[s] = c;
             a.L.fsup_cnt[s] = cs;
