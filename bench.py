@@ -359,6 +359,7 @@ def main():
           f"{rebuilds} rebuilds", file=sys.stderr, flush=True)
     ff_directed, fs_pairs = pair_counts(g)
     cand = candidates(g, ff_directed)
+    tiles = g.tile_stats()
     roof = roofline(times, "mixed", scene.n_fluid, ff_directed, fs_pairs) if times else None
     if roof:
         roof["traffic"] = traffic_from_profile(roof["kernel"], "mixed")
@@ -431,6 +432,7 @@ def main():
         "kernel_ms": {k: [round(v[0] / max(v[1], 1), 4), v[1]] for k, v in times.items()},
         "pairs": {"fluid_fluid_directed": ff_directed, "fluid_solid": fs_pairs},
         "candidates": cand,
+        "tiles": tiles,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -111,6 +111,8 @@ struct dsph_ctx {
     int tune_bs = 256;          // workgroup of the fluid pair loops
     int tune_bmem = 64;         // particles per superset build chunk (one wavefront)
     int tune_group = 1;         // particles sharing one superset row
+    int tune_tile = 1;          // mixed pair loops over LDS tiles (1) or gathering from HBM (0)
+    Tiles T;                    // the tiles while they are on (tiles_on)
     bool fluid_uniform = false; // every fluid particle has the same (h, c, rho0, nu)
     UniformFluid ufluid;
     int sub_bits = 9;           // fluid sort key: sub-cell Morton bits below the cell rank
@@ -181,10 +183,13 @@ T* dalloc(dsph_ctx* c, size_t n) {
     return (T*)p;
 }
 
+// frees an allocation of this context (a pointer it does not hold, e.g.
+// one already released with the whole scene, is left alone)
 void dfree(dsph_ctx* c, void* p) {
     if (!p) return;
     auto it = std::find(c->allocs.begin(), c->allocs.end(), p);
-    if (it != c->allocs.end()) c->allocs.erase(it);
+    if (it == c->allocs.end()) return;
+    c->allocs.erase(it);
     (void)hipFree(p);
 }
 
@@ -205,6 +210,7 @@ void free_all(dsph_ctx* c) {
     c->graphs.clear();
     for (void* p : c->allocs) (void)hipFree(p);
     c->allocs.clear();
+    c->T = Tiles{};
     c->backup.clear();
     c->backup_bytes.clear();
     c->hs_work = nullptr;
@@ -244,6 +250,54 @@ void alloc_ff(dsph_ctx* c, int K) {
     c->L.nbr = dalloc<int>(c, (size_t)(K + kRowPad) * std::max(c->nf, 1));
 }
 
+// ------------------------------------------------------------------ tiles
+// The pair-loop tiles run for the mixed precision with the reorder on (a
+// tile must be a block of consecutive slots that is compact in space).
+bool tiles_on(const dsph_ctx* c) {
+    return c->tune_tile != 0 && c->mixed() && c->reorder_period > 0 && c->nf > 0;
+}
+
+void free_tiles(dsph_ctx* c) {
+    dfree(c, (void*)c->T.range);
+    dfree(c, c->T.uni);
+    dfree(c, c->T.ucnt);
+    dfree(c, c->T.row);
+    dfree(c, c->T.rcnt);
+    c->T = Tiles{};
+}
+
+// local rows of the tiles, (supK + kRowPad) columns like the superset
+void alloc_tile_rows(dsph_ctx* c) {
+    if (c->T.n == 0) return;
+    dfree(c, c->T.row);
+    c->T.row = dalloc<unsigned short>(c, (size_t)tile_row_cols(c->L.supK) * std::max(c->nf, 1));
+    fill_int(c->st, c->T.ucnt, c->T.n, -1);  // no tile is staged before the next build
+}
+
+// Runs of up to kTileSlots consecutive slots of one scene (scene k holds
+// the slots [foff[k], foff[k+1])).  No tile is built until the next
+// superset build (ucnt = -1: the tile walks its global rows).
+void alloc_tiles(dsph_ctx* c) {
+    free_tiles(c);
+    if (!tiles_on(c)) return;
+    std::vector<int2> r;
+    for (int k = 0; k < c->nscene; k++)
+        for (int t = c->foff[k]; t < c->foff[k + 1]; t += kTileSlots)
+            r.push_back(make_int2(t, std::min(t + kTileSlots, c->foff[k + 1])));
+    if (r.empty()) return;
+    Tiles& T = c->T;
+    T.n = (int)r.size();
+    int2* range = dalloc<int2>(c, r.size());
+    h2d(c, range, r.data(), r.size());
+    T.range = range;
+    T.uni = dalloc<int>(c, (size_t)T.n * kTileCap);
+    T.ucnt = dalloc<int>(c, T.n);
+    T.rcnt = dalloc<int>(c, c->nf);
+    fill_int(c->st, T.rcnt, c->nf, 0);
+    alloc_tile_rows(c);
+    HIPCHK(hipStreamSynchronize(c->st));
+}
+
 void alloc_sup(dsph_ctx* c, int K, int KS) {
     dfree(c, c->L.sup);
     dfree(c, c->L.fsup);
@@ -251,16 +305,28 @@ void alloc_sup(dsph_ctx* c, int K, int KS) {
     c->L.fsupK = KS;
     c->L.sup = dalloc<int>(c, (size_t)(K + kRowPad) * std::max(c->nf, 1));
     c->L.fsup = dalloc<int>(c, (size_t)KS * std::max(c->nf, 1));
+    alloc_tile_rows(c);
 }
 
 // more superset capacity with the current rows kept: the ELL is
 // column-major with stride nf, so the first supK columns copy as one block
+// (for the tiles' local rows also the pad columns, which hold positions
+// the pair loops read ahead)
 void grow_sup_keep(dsph_ctx* c, int K) {
     if (K <= c->L.supK) return;
     const size_t nf = (size_t)std::max(c->nf, 1);
     int* fresh = dalloc<int>(c, (size_t)(K + kRowPad) * nf);
     HIPCHK(hipMemcpyAsync(fresh, c->L.sup, sizeof(int) * (size_t)c->L.supK * nf, hipMemcpyDeviceToDevice,
                           c->st));
+    if (c->T.n > 0) {
+        // the quads of the old capacity are a prefix of the new array
+        unsigned short* rows = dalloc<unsigned short>(c, (size_t)tile_row_cols(K) * nf);
+        HIPCHK(hipMemcpyAsync(rows, c->T.row, sizeof(unsigned short) * (size_t)tile_row_cols(c->L.supK) * nf,
+                              hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        dfree(c, c->T.row);
+        c->T.row = rows;
+    }
     HIPCHK(hipStreamSynchronize(c->st));
     dfree(c, c->L.sup);
     c->L.sup = fresh;
@@ -418,6 +484,12 @@ std::vector<StateArr> state_arrays(dsph_ctx* c) {
     add(L.grow, nf, 4);
     add(L.fsup, (size_t)L.fsupK * nf, 4);
     add(L.fsup_cnt, nf, 4);
+    if (c->T.n > 0) {
+        add(c->T.uni, (size_t)c->T.n * kTileCap, 4);
+        add(c->T.ucnt, (size_t)c->T.n, 4);
+        add(c->T.row, (size_t)tile_row_cols(L.supK) * nf, 2);
+        add(c->T.rcnt, nf, 4);
+    }
     add(L.fsm, 2 * nf, 8);
     add(L.ss_i, (size_t)L.ss_cap, 4);
     add(L.ss_j, (size_t)L.ss_cap, 4);
@@ -503,6 +575,7 @@ FluidArgs fluid_args(dsph_ctx* c) {
     a.gy = c->gv[1];
     a.gz = c->gv[2];
     a.flags = c->flags;
+    a.T = c->T;
     return a;
 }
 
@@ -628,6 +701,7 @@ SearchArgs search_args(dsph_ctx* c) {
     a.L = c->L;
     a.ctl = c->ctl;
     a.st = c->status;
+    a.T = c->T;
     return a;
 }
 
@@ -697,6 +771,7 @@ void neighbor_search(dsph_ctx* c, int i_step, int gap, bool save_history, bool z
         {
             FamTimer t(c, F_FSEARCH);
             fluid_superset(s, sa, c->fchunks, c->nfchunks);
+            tile_rows(s, sa);
         }
         FamTimer t(c, F_FSLIST);
         fluid_filter(s, sa, zero_fs);
@@ -1376,6 +1451,15 @@ int dsph_set_reorder_period(dsph_ctx* c, int32_t p) {
     if (!c || p < 0) return DSPH_E_INVALID;
     c->reorder_period = p;
     invalidate_graphs(c);
+    if (c->uploaded && (c->T.n > 0) != tiles_on(c)) {
+        try {
+            HIPCHK(hipStreamSynchronize(c->st));
+            alloc_tiles(c);
+            force_rebuild(c);
+        } catch (const ApiError& e) {
+            return fail(c, e);
+        }
+    }
     return DSPH_OK;
 }
 
@@ -1434,6 +1518,18 @@ int dsph_set_tuning(dsph_ctx* c, int32_t key, int32_t value) {
                 return fail(c, e);
             }
         }
+    } else if (key == DSPH_TUNE_TILE) {
+        if (value != 0 && value != 1) return DSPH_E_INVALID;
+        c->tune_tile = value;
+        if (c->uploaded) {
+            try {
+                HIPCHK(hipStreamSynchronize(c->st));
+                alloc_tiles(c);
+                force_rebuild(c);
+            } catch (const ApiError& e) {
+                return fail(c, e);
+            }
+        }
     } else {
         return DSPH_E_INVALID;
     }
@@ -1449,6 +1545,8 @@ int dsph_get_tuning(dsph_ctx* c, int32_t key, int32_t* value) {
         *value = c->tune_bmem;
     else if (key == DSPH_TUNE_GROUP)
         *value = c->tune_group;
+    else if (key == DSPH_TUNE_TILE)
+        *value = c->tune_tile;
     else
         return DSPH_E_INVALID;
     return DSPH_OK;
@@ -1568,10 +1666,12 @@ int dsph_upload_batch(dsph_ctx* c, const dsph_scene* scenes, int32_t n_scenes) {
                 hi[2] = std::max(hi[2], gg.maxz);
             }
             // one unit on every axis (the pair loops work in position
-            // units), set by the longest extent
+            // units), set by the longest extent; at least 2e-10 m, so two
+            // records at different fixed-point positions are farther apart
+            // than the 1e-10 m below which a pair counts for nothing
             double Lmax = 0.0;
             for (int a = 0; a < 3; a++) Lmax = std::max(Lmax, hi[a] - lo[a]);
-            const double unit = 3.0 * std::max(Lmax, 1e-300) / 4294967296.0;
+            const double unit = std::max(3.0 * Lmax / 4294967296.0, 2e-10);
             for (int a = 0; a < 3; a++) {
                 const double L = hi[a] - lo[a];
                 f.qo[a] = lo[a] - L;
@@ -1957,6 +2057,7 @@ int dsph_upload_batch(dsph_ctx* c, const dsph_scene* scenes, int32_t n_scenes) {
             h2d(c, L.grow, self.data(), nf);
             HIPCHK(hipStreamSynchronize(c->st));
         }
+        alloc_tiles(c);
         alloc_fs(c, std::max<int64_t>(4096, 8LL * nf));
         // solids without pairs keep count 0 and offset 0 throughout
         L.fsr_cnt = dalloc<int>(c, ns);
@@ -2221,12 +2322,13 @@ struct StateBlobHeader {
     int32_t K, supK, fsupK, reorder_period;
     int64_t fs_cap, ss_cap;
     int32_t dcur, skey_valid, tune_bs, tune_bmem, tune_group, nymax;
-    uint32_t flags, pad1;
+    uint32_t flags;
+    int32_t tune_tile;
     double skin_h, gv[3];
     int64_t n_arrays, payload;
 };
 constexpr char kStateMagic[8] = {'D', 'S', 'P', 'H', 'S', 'T', 'A', 'T'};
-constexpr int32_t kStateVersion = 1;
+constexpr int32_t kStateVersion = 2;
 
 StateBlobHeader state_header(dsph_ctx* c, const std::vector<StateArr>& v) {
     StateBlobHeader h{};
@@ -2255,6 +2357,7 @@ StateBlobHeader state_header(dsph_ctx* c, const std::vector<StateArr>& v) {
     h.tune_bs = c->tune_bs;
     h.tune_bmem = c->tune_bmem;
     h.tune_group = c->tune_group;
+    h.tune_tile = c->tune_tile;
     h.flags = c->flags;
     h.skin_h = c->skin_h;
     for (int k = 0; k < 3; k++) h.gv[k] = c->gv[k];
@@ -2313,7 +2416,8 @@ int dsph_load_state(dsph_ctx* c, const void* buffer, int64_t size) {
         require(h.tune_bs >= 64 && h.tune_bs <= 512 && h.tune_bs % 64 == 0 && h.tune_bmem >= 1 &&
                     h.tune_bmem <= 64 && (h.tune_group == 1 || h.tune_group == 2 || h.tune_group == 4 ||
                                           h.tune_group == 8) && h.K >= 1 && h.supK >= 1 && h.fsupK >= 1 &&
-                    h.fs_cap >= 1 && h.ss_cap >= 1 && h.nbond >= 0 && h.skin_h >= 0.0,
+                    h.fs_cap >= 1 && h.ss_cap >= 1 && h.nbond >= 0 && h.skin_h >= 0.0 &&
+                    (h.tune_tile == 0 || h.tune_tile == 1),
                 "state blob header out of range");
         HIPCHK(hipStreamSynchronize(c->st));
         invalidate_graphs(c);
@@ -2329,6 +2433,8 @@ int dsph_load_state(dsph_ctx* c, const void* buffer, int64_t size) {
         }
         c->skin_h = h.skin_h;
         superset_stencil(c);
+        c->tune_tile = h.tune_tile;
+        if ((c->T.n > 0) != tiles_on(c)) alloc_tiles(c);
         // its capacities
         if (c->L.K != h.K) alloc_ff(c, h.K);
         if (c->L.supK != h.supK || c->L.fsupK != h.fsupK) alloc_sup(c, h.supK, h.fsupK);
@@ -2776,6 +2882,31 @@ int dsph_superset_stats(dsph_ctx* c, int64_t out[4]) {
         out[1] = walked;
         out[2] = stored;
         out[3] = longest;
+        return DSPH_OK;
+    } catch (const ApiError& e) {
+        return fail(c, e);
+    }
+}
+
+int dsph_tile_stats(dsph_ctx* c, int64_t out[4]) {
+    API_GUARD(c);
+    if (!out) return DSPH_E_INVALID;
+    if (!c->initialized) return DSPH_E_STATE;
+    try {
+        std::vector<int> u(c->T.n);
+        if (c->T.n > 0) d2h(c, u.data(), c->T.ucnt, c->T.n);
+        HIPCHK(hipStreamSynchronize(c->st));
+        int64_t staged = 0, total = 0, largest = 0;
+        for (int m : u) {
+            if (m < 0) continue;
+            staged++;
+            total += m;
+            largest = std::max<int64_t>(largest, m);
+        }
+        out[0] = c->T.n;
+        out[1] = staged;
+        out[2] = total;
+        out[3] = largest;
         return DSPH_OK;
     } catch (const ApiError& e) {
         return fail(c, e);

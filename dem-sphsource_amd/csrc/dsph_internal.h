@@ -239,6 +239,35 @@ struct Lists {
     int* ssr_idx = nullptr;
 };
 
+// Tiles of the mixed pair loops: runs of up to kTileSlots consecutive slots
+// of one scene.  Slots are in Morton order after every superset build, so a
+// tile is a compact block of particles, and the union of its rows'
+// candidates is a few times its size.  At every build each tile gets that
+// union (uni, ascending slots, at most kTileCap) and each of its rows the
+// same candidates in the same order, minus the particle itself, as indices
+// into the union (row, u16; rcnt entries), four consecutive entries of a
+// row in one 8-byte quad, quads ELL column-major: entry c of slot s at
+// ((c / 4) * nf + s) * 4 + c % 4.  The
+// pair loops stage the union's records in LDS once per tile and walk the
+// local rows there.  ucnt < 0: the union did not fit, and the tile walks its
+// global rows.
+constexpr int kTileSlots = 512;
+constexpr int kTileCap = 2432;  // 76 KiB of mixed records in LDS
+// entries per slot of the local rows: the superset capacity, the pad read
+// ahead, rounded up to whole quads
+__host__ __device__ constexpr int tile_row_cols(int supK) { return (supK + kRowPad + 3) / 4 * 4; }
+__host__ __device__ __forceinline__ int64_t tile_row_at(int c, int s, int nf) {
+    return ((int64_t)(c >> 2) * nf + s) * 4 + (c & 3);
+}
+struct Tiles {
+    int n = 0;                         // tiles (0: the tiled loops are off)
+    const int2* range = nullptr;       // [first slot, end slot) of each tile
+    int* uni = nullptr;                // n * kTileCap slots
+    int* ucnt = nullptr;               // n
+    unsigned short* row = nullptr;     // tile_row_cols(supK) * nf
+    int* rcnt = nullptr;               // nf
+};
+
 struct Bonds {
     int n = 0;
     int* i = nullptr;
@@ -316,6 +345,8 @@ struct SearchArgs {
     unsigned long long* st;
     // fluid–solid count of each tile of 256 slots (the list pass)
     int* tsum;
+    // the pair-loop tiles, rebuilt with the superset (n = 0: none)
+    Tiles T;
 };
 // also the reference hash of every slot, and CTL_NEAR_SPARE when a particle's
 // cell has a stencil reaching the spare cell; with fxb, the largest squared
@@ -365,6 +396,8 @@ void after_sort(hipStream_t s, const FluidSet& f, const FluidSet& scratch, bool 
 // one scene, one wavefront each; also stores the build positions (fxb, and
 // sxb of the live solids)
 void fluid_superset(hipStream_t s, const SearchArgs& a, const int2* chunks, int nchunk);
+// after a superset build (CTL_REBUILD): each tile's union and local rows
+void tile_rows(hipStream_t s, const SearchArgs& a);
 // exact fluid neighbour rows of all particles for read-back (ELL, stride nf,
 // capacity max(K, supK)); out_cnt = -1 marks a walked row that overflowed
 void fluid_exact_rows(hipStream_t s, const SearchArgs& a, int* out, int* out_cnt);
@@ -403,6 +436,7 @@ struct FluidArgs {
     Solids so;
     double gx, gy, gz;
     uint32_t flags;
+    Tiles T;  // n > 0: the mixed pair loops run tiled
 };
 void fluid_records(hipStream_t s, const FluidArgs& a, int buf, bool mixed);
 void density_reinit(hipStream_t s, const FluidArgs& a, int i_step, bool mixed);

@@ -293,6 +293,12 @@ enum NbMode { POS = 0, STATE = 1, STATE_NU = 2 };
 // cache line.
 constexpr int kBatch = 4;
 static_assert(2 * kBatch <= kRowPad, "row arrays need 2 kBatch pad columns");
+// The first batch's indices pass through an identity quad permutation that
+// the optimiser cannot see through.  Without it the loads before the loop
+// and the loads inside it (same kind, addresses one batch apart) are merged
+// into one load at the loop head, which every iteration then waits on
+// before its gathers: the index read is no longer a batch ahead.
+__device__ __forceinline__ int apart(int x) { return __builtin_amdgcn_mov_dpp(x, 0xE4, 0xF, 0xF, false); }
 // column c of a row at its storage slot p
 __device__ __forceinline__ int row_col(const int* p, int lg, int c, int64_t st) {
     return __builtin_nontemporal_load(p + (int64_t)(c >> lg) * st + (c & ((1 << lg) - 1)));
@@ -304,7 +310,8 @@ __device__ __forceinline__ void for_row_each(const FluidArgs& a, Rsrc rs, int s,
     const int* ip = row.p;
     int jn[U];
 #pragma unroll
-    for (int u = 0; u < U; u++) jn[u] = IL ? row_col(row.p, row.lg, u, st) : __builtin_nontemporal_load(ip + u * st);
+    for (int u = 0; u < U; u++)
+        jn[u] = apart(IL ? row_col(row.p, row.lg, u, st) : __builtin_nontemporal_load(ip + u * st));
     for (int k = 0; k < n; k += U) {
         ip += U * st;
         Nb<MIX> q[U];
@@ -411,7 +418,7 @@ __device__ __forceinline__ void for_row_split(const FluidArgs& a, Rsrc rs, int s
     const int* ip = IL ? row.p + l : row.p + (int64_t)l * st;
     int jn[L];
 #pragma unroll
-    for (int u = 0; u < L; u++) jn[u] = __builtin_nontemporal_load(ip + u * CS * st);
+    for (int u = 0; u < L; u++) jn[u] = apart(__builtin_nontemporal_load(ip + u * CS * st));
     for (int k = 0; k < n; k += B) {
         ip += (B / G) * CS * st;
         Nb<MIX> q[L];
@@ -455,6 +462,126 @@ __device__ __forceinline__ void for_row(const FluidArgs& a, Rsrc rs, int s, Fn&&
     }
 }
 
+// --------------------------------------------------------------- tiles
+// A tile's workgroup stages the records of its candidates' union in LDS
+// once (the position quads, and for continuity and momentum the state
+// quads), then every slot walks its local row there: indices into the union
+// (u16, ELL column-major), in the order of its global row, in batches of U
+// with the next batch's indices read ahead.  An LDS read replaces the
+// gather of the global walk.  Position and state quads sit in two arrays of
+// 16-byte entries, so consecutive union positions fall in consecutive bank
+// groups (a 32-byte stride would leave half of them unused and make the
+// lanes' reads collide).
+using TileRec = uint4;
+struct TileLds {
+    TileRec pos[kTileCap];
+    TileRec st[kTileCap];
+};
+template <bool STATE>
+__device__ __forceinline__ int stage_tile(const FluidArgs& a, int b, TileLds& L) {
+    const int m = a.T.ucnt[b];
+    const TileRec* rec = (const TileRec*)a.f.rec[a.cur];
+    const int* uni = a.T.uni + (int64_t)b * kTileCap;
+    for (int k = threadIdx.x; k < m; k += blockDim.x) {
+        const int j = uni[k];
+        L.pos[k] = rec[2 * (int64_t)j];
+        if constexpr (STATE) L.st[k] = rec[2 * (int64_t)j + 1];
+    }
+    __syncthreads();
+    return m;
+}
+template <bool STATE>
+__device__ __forceinline__ void load_tile_rec(const TileLds& L, int j, Nb<true>& o) {
+    const uint4 p = L.pos[j];
+    o.px = p.x; o.py = p.y; o.pz = p.z; o.id = (int)p.w;
+    if constexpr (STATE) {
+        const float4 v = __builtin_bit_cast(float4, L.st[j]);
+        o.vx = v.x; o.vy = v.y; o.vz = v.z; o.drho = v.w;
+    }
+}
+// one quad of a local row: four entries, one 8-byte load
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void row_quad(const u32x2* p, int* j) {
+    const u32x2 v = __builtin_nontemporal_load(p);
+    j[0] = (int)(v.x & 0xffffu);
+    j[1] = (int)(v.x >> 16);
+    j[2] = (int)(v.y & 0xffffu);
+    j[3] = (int)(v.y >> 16);
+}
+template <int U, NbMode MODE, bool UNI, class Fn>
+__device__ __forceinline__ void for_row_tile(const FluidArgs& a, const TileLds& lrec, int s, Fn&& fn) {
+    static_assert(U == 4, "a batch is one quad of the local row");
+    const int n = a.T.rcnt[s];
+    const int64_t st = a.nf;
+    const u32x2* ip = (const u32x2*)a.T.row + s;
+    int jn[U];
+    row_quad(ip, jn);
+#pragma unroll
+    for (int u = 0; u < U; u++) jn[u] = apart(jn[u]);
+    for (int k = 0; k < n; k += U) {
+        ip += st;
+        Nb<true> q[U];
+        bool ok[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            load_tile_rec<MODE != POS>(lrec, jn[u], q[u]);
+            ok[u] = k + u < n;
+        }
+        row_quad(ip, jn);
+        if constexpr (MODE != POS) {
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                load_props<true, MODE == STATE_NU, UNI>(a, q[u]);
+                derive<true>(q[u]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) fn(q[u], ok[u]);
+    }
+}
+
+// How a slot's candidates are visited: its global row (superset or exact),
+// records gathered from HBM; or, in a tiled launch, its local row in LDS,
+// with the global row, walked by the slot alone, for a row walked exactly
+// this step or a tile whose union did not fit.  run<MODE>(fn) as for_row.
+// kFsBatch: fluid-solid pairs per batch of the momentum pass
+template <bool MIX, bool UNI>
+struct GlobalWalk {
+    static constexpr int kFsBatch = 4;
+    const FluidArgs& a;
+    Rsrc rs;
+    int s;
+    template <NbMode MODE, class Fn>
+    __device__ __forceinline__ void run(Fn&& fn) const {
+        for_row<MIX, kBatch, MODE, UNI>(a, rs, s, fn);
+    }
+};
+// U: candidates per batch
+template <bool UNI, int U = kBatch>
+struct TileWalk {
+    static constexpr int kFsBatch = 2;
+    const FluidArgs& a;
+    Rsrc rs;
+    int s;
+    const TileLds& lrec;
+    bool tiled;
+    template <NbMode MODE, class Fn>
+    __device__ __forceinline__ void run(Fn&& fn) const {
+        if (tiled) {
+            for_row_tile<U, MODE, UNI>(a, lrec, s, fn);
+        } else {
+            const Row row = nb_row(a.L, s);
+            for_row_each<true, U, MODE, UNI, true>(a, rs, s, row, fn);
+        }
+    }
+};
+// the tile of this workgroup and the slot of this thread (-1: none)
+__device__ __forceinline__ int tile_slot(const FluidArgs& a, int b) {
+    const int2 tr = a.T.range[b];
+    const int s = tr.x + (int)threadIdx.x;
+    return s < tr.y ? s : -1;
+}
+
 template <class R>
 __device__ __forceinline__ R dist(R r2) {
     return r2 > R(0) ? r2 * rsq_(r2) : R(0);
@@ -471,6 +598,17 @@ __device__ __forceinline__ R grad_factor_ih(R r, R h, R ih) {
 template <class R>
 __device__ __forceinline__ R grad_factor(R r, R h) {
     return grad_factor_ih<R>(r, h, rcp_(h));
+}
+
+// The pair loops' kernel factor t = 1 - r/(2h) of a candidate, set to 0
+// where the candidate does not count (keep false: past the row end or
+// closer than 1e-10 m) and clamped at 0 outside the support.  Every pair
+// term carries t^3, so a candidate that does not count adds an exact zero
+// (+0 or -0, which leave an FP64 sum that starts at +0 unchanged) and the
+// sums need no select.
+template <class R>
+__device__ __forceinline__ R support_t(bool keep, R t) {
+    return keep ? fmax(t, R(0)) : R(0);
 }
 
 // solid-side values of a fluid–solid pair (r_i - r_a subtracted in FP64)
@@ -579,18 +717,16 @@ __global__ void __launch_bounds__(kBS) k_records(FluidSet f, int n, int buf) {
 // rho0, nu), so 1/h, c and m are constants, dP / c = c (dRho_i - dRho_j) and
 // (rho_i + rho_j) gbar = 2 nu rho_i rho_j.  A candidate counts when
 // r >= 1e-10 m (r_u^2 >= r2min) and t > 0 (r < 2h).
-template <bool MIX, bool UNI>
 // drift != 0: the fluid drift that follows the first continuity pass of a
 // step (fluidPositionIntegrate, integrate.cu:367-375), x += drift v, done
 // here for the slot's own position and written into its new records
-__global__ void __launch_bounds__(kMaxBS) k_continuity(FluidArgs a, double dt, double drift) {
+template <bool MIX, bool UNI, class Walk>
+__device__ __forceinline__ void continuity_slot(const FluidArgs& a, int s, double dt, double drift,
+                                                const Walk& walk) {
     using R = typename Prec<MIX>::R;
-    const int s = xcd_block() * blockDim.x + threadIdx.x;
-    if (s >= a.nf) return;
     const FluidSet& f = a.f;
     const Nb<MIX> o = load_nb<MIX, UNI>(a, a.cur, s);
     const double4 ps = f.ph[s];
-    const Rsrc rs = rec_rsrc<MIX>(f, a.cur, a.nf);
     const R u = pos_unit<MIX>(f);
     const R r2min = R(1e-20) / (u * u);
     // UNI: term = K t^3 [rho_j dvr_u + (c dRho_i - c dRho_j) r_u] / (rho_j (rho_i + rho_j))
@@ -600,30 +736,32 @@ __global__ void __launch_bounds__(kMaxBS) k_continuity(FluidArgs a, double dt, d
     const R K = R(2) * o.rho * o.m * R(kGradC) * (ih2 * ih2 * ih) * u;
     const R cd_i = o.c * o.drho;
     Acc1<R> acc;
-    for_row<MIX, kBatch, STATE, UNI>(
-        a, rs, s,
+    walk.template run<STATE>(
         [&](const Nb<MIX>& q, bool ok) {
             R rx, ry, rz;
             rel<MIX>(o, q, rx, ry, rz);
             const R r2 = rx * rx + ry * ry + rz * rz;
             const R r = sqrt_(r2);
             const R dvr = (o.vx - q.vx) * rx + (o.vy - q.vy) * ry + (o.vz - q.vz) * rz;
+            // mixed: a fixed-point unit is over 1e-10 m, so only coincident
+            // records are closer, and their term has dvr = r = 0
+            const bool keep = MIX ? ok : (ok & (r2 >= r2min));
             R t, term;
             if constexpr (UNI) {
-                t = R(1) + nh * r;
+                t = support_t(keep, R(1) + nh * r);
                 const R num = q.rho * dvr + (cd_i - o.c * q.drho) * r;
                 term = K * (t * t * t) * num * rcp_(q.rho * (o.rho + q.rho));
             } else {
                 const R hb = R(0.5) * (o.h + q.h);
                 const R ihb = rcp_(hb);
-                t = R(1) - R(0.5) * u * ihb * r;
+                t = support_t(keep, R(1) - R(0.5) * u * ihb * r);
                 const R ihb2 = ihb * ihb;
                 const R F = R(kGradC) * (t * t * t) * (ihb2 * ihb2 * ihb);
                 const R c = R(0.5) * (o.c + q.c);
                 const R num = q.rho * c * dvr + (o.P - q.P) * r;
                 term = R(2) * o.rho * q.m * u * F * num * rcp_(c * q.rho * (o.rho + q.rho));
             }
-            acc.add_if(ok & (r2 >= r2min) & (t > R(0)), term);
+            acc.add(term);
         });
     const double4 vm = f.vr[s];
     if (fs_valid(a.L, a.nf)) {
@@ -652,22 +790,36 @@ __global__ void __launch_bounds__(kMaxBS) k_continuity(FluidArgs a, double dt, d
     }
     write_records<MIX>(f, 1 - a.cur, s, p, vm.x, vm.y, vm.z, nd);
 }
+template <bool MIX, bool UNI>
+__global__ void __launch_bounds__(kMaxBS) k_continuity(FluidArgs a, double dt, double drift) {
+    const int s = xcd_block() * blockDim.x + threadIdx.x;
+    if (s >= a.nf) return;
+    continuity_slot<MIX, UNI>(a, s, dt, drift, GlobalWalk<MIX, UNI>{a, rec_rsrc<MIX>(a.f, a.cur, a.nf), s});
+}
+template <bool UNI>
+__global__ void __launch_bounds__(kTileSlots) k_continuity_tile(FluidArgs a, double dt, double drift) {
+    __shared__ TileLds lrec;
+    const int b = xcd_block();
+    const int m = stage_tile<true>(a, b, lrec);
+    const int s = tile_slot(a, b);
+    if (s < 0) return;
+    const bool tiled = m >= 0 && a.L.cnt[s] < 0;
+    continuity_slot<true, UNI>(a, s, dt, drift, TileWalk<UNI>{a, rec_rsrc<true>(a.f, a.cur, a.nf), s, lrec, tiled});
+}
 
 // --------------------------------------------------------------- momentum
-template <bool MIX, bool UNI>
 // kick != 0: the kick and drift that follow momentum (fluidVelocityIntegrate
 // and fluidPositionIntegrate, integrate.cu:355-375), v += kick a, then
 // x += kick/2 v, for the slot's own state; the new records go to the other
 // buffer with the slot's dRho (other slots still read this one), and the
 // caller flips the current buffer
-__global__ void __launch_bounds__(kMaxBS) k_momentum(FluidArgs a, int write_fs, double kick) {
+template <bool MIX, bool UNI, class Walk>
+__device__ __forceinline__ void momentum_slot(const FluidArgs& a, int s, int write_fs, double kick,
+                                              const Walk& walk) {
     using R = typename Prec<MIX>::R;
-    const int s = xcd_block() * blockDim.x + threadIdx.x;
-    if (s >= a.nf) return;
     const FluidSet& f = a.f;
     const Nb<MIX> o = load_nb<MIX, UNI>(a, a.cur, s);
     const double4 ps = f.ph[s];
-    const Rsrc rs = rec_rsrc<MIX>(f, a.cur, a.nf);
     const R u = pos_unit<MIX>(f);
     const R r2min = R(1e-20) / (u * u);
     const R two_over_rho = R(2) * rcp_(o.rho);
@@ -692,13 +844,14 @@ __global__ void __launch_bounds__(kMaxBS) k_momentum(FluidArgs a, int write_fs, 
     // the artificial-viscosity switch is taken once, outside the loop
     auto rows = [&](auto art_tag) {
         constexpr bool ART = decltype(art_tag)::value;
-        for_row<MIX, kBatch, STATE_NU, UNI>(
-            a, rs, s,
+        walk.template run<STATE_NU>(
             [&](const Nb<MIX>& q, bool ok) {
                 R rx, ry, rz;
                 rel<MIX>(o, q, rx, ry, rz);
                 const R r2 = rx * rx + ry * ry + rz * rz;
-                const R ir = rsq_(r2);
+                const bool keep = ok & (r2 >= r2min);
+                // finite for coincident records too (their t is 0)
+                const R ir = rsq_(fmax(r2, r2min));
                 const R r = r2 * ir;
                 const R dvx = o.vx - q.vx, dvy = o.vy - q.vy, dvz = o.vz - q.vz;
                 const R dvr = dvx * rx + dvy * ry + dvz * rz;
@@ -708,7 +861,7 @@ __global__ void __launch_bounds__(kMaxBS) k_momentum(FluidArgs a, int write_fs, 
                 if constexpr (UNI) {
                     hb = o.h;
                     c = o.c;
-                    t = R(1) + nh * r;
+                    t = support_t(keep, R(1) + nh * r);
                     const R beta = fmin(R(3) * fmax(dU, R(0)), c);
                     const R common = Km * (t * t * t) * rcp_(q.rho * (o.rho + q.rho));
                     cv = common * G * q.rho;
@@ -717,7 +870,7 @@ __global__ void __launch_bounds__(kMaxBS) k_momentum(FluidArgs a, int write_fs, 
                     hb = R(0.5) * (o.h + q.h);
                     c = R(0.5) * (o.c + q.c);
                     const R ihb = rcp_(hb);
-                    t = R(1) - R(0.5) * u * ihb * r;
+                    t = support_t(keep, R(1) - R(0.5) * u * ihb * r);
                     const R ihb2 = ihb * ihb;
                     const R F = R(kGradC) * (t * t * t) * (ihb2 * ihb2 * ihb);
                     const R beta = fmin(R(3) * fmax(dU, R(0)), c);
@@ -738,8 +891,7 @@ __global__ void __launch_bounds__(kMaxBS) k_momentum(FluidArgs a, int write_fs, 
                     cn -= u * q.m * alpha * hb * c * (dvr * u) * rcp_(rb * (r2 * u * u) + R(0.01) * hb * hb) *
                           F;
                 }
-                acc.add_if(ok & (r2 >= r2min) & (t > R(0)), cv * dvx - cn * rx, cv * dvy - cn * ry,
-                           cv * dvz - cn * rz);
+                acc.add(cv * dvx - cn * rx, cv * dvy - cn * ry, cv * dvz - cn * rz);
             });
     };
     if (art) rows(std::true_type{});
@@ -748,7 +900,7 @@ __global__ void __launch_bounds__(kMaxBS) k_momentum(FluidArgs a, int write_fs, 
         const double4 vm = f.vr[s];
         const double h13 = ps.w / 1.3;
         const double mi = h13 * h13 * h13 * vm.w;
-        for_fs<R, 4>(a, s, ps.x, ps.y, ps.z, [&](const SolidP<R>& sp, int k, bool ok) {
+        for_fs<R, Walk::kFsBatch>(a, s, ps.x, ps.y, ps.z, [&](const SolidP<R>& sp, int k, bool ok) {
             R tx, ty, tz;
             const bool on = fs_term<MIX>(o, (R)vm.w, sp, (R)a.gx, (R)a.gy, (R)a.gz, tx, ty, tz);
             acc.add_if(ok && on, tx, ty, tz);
@@ -774,6 +926,23 @@ __global__ void __launch_bounds__(kMaxBS) k_momentum(FluidArgs a, int write_fs, 
         write_records<MIX>(f, 1 - a.cur, s, p, v.x, v.y, v.z, d);
     }
 }
+template <bool MIX, bool UNI>
+__global__ void __launch_bounds__(kMaxBS) k_momentum(FluidArgs a, int write_fs, double kick) {
+    const int s = xcd_block() * blockDim.x + threadIdx.x;
+    if (s >= a.nf) return;
+    momentum_slot<MIX, UNI>(a, s, write_fs, kick, GlobalWalk<MIX, UNI>{a, rec_rsrc<MIX>(a.f, a.cur, a.nf), s});
+}
+template <bool UNI>
+__global__ void __launch_bounds__(kTileSlots) k_momentum_tile(FluidArgs a, int write_fs, double kick) {
+    __shared__ TileLds lrec;
+    const int b = xcd_block();
+    const int m = stage_tile<true>(a, b, lrec);
+    const int s = tile_slot(a, b);
+    if (s < 0) return;
+    const bool tiled = m >= 0 && a.L.cnt[s] < 0;
+    momentum_slot<true, UNI>(a, s, write_fs, kick,
+                             TileWalk<UNI>{a, rec_rsrc<true>(a.f, a.cur, a.nf), s, lrec, tiled});
+}
 
 // calFluid2SolidForce (integrate.cu:282-330) on steps without a fluid update
 template <bool MIX>
@@ -797,11 +966,9 @@ __global__ void __launch_bounds__(kMaxBS) k_fs_force(FluidArgs a) {
 }
 
 // -------------------------------------------------------- density re-init
-template <bool MIX>
-__global__ void __launch_bounds__(kMaxBS) k_density_reinit(FluidArgs a, int i_step) {
+template <bool MIX, class Walk>
+__device__ __forceinline__ void reinit_slot(const FluidArgs& a, int s, int i_step, const Walk& walk) {
     using R = typename Prec<MIX>::R;
-    const int s = xcd_block() * blockDim.x + threadIdx.x;
-    if (s >= a.nf) return;
     const FluidSet& f = a.f;
     const double4 p = f.ph[s];
     const R h = (R)p.w;
@@ -811,17 +978,15 @@ __global__ void __launch_bounds__(kMaxBS) k_density_reinit(FluidArgs a, int i_st
     sw.add(sigma);  // W(0, h)
     Nb<MIX> o;
     load_pos<MIX>(f, a.cur, s, o);
-    const Rsrc rs = rec_rsrc<MIX>(f, a.cur, a.nf);
     const R ihu = pos_unit<MIX>(f) * ih;  // 1/h per position unit
-    for_row<MIX, kBatch, POS, false>(
-        a, rs, s,
+    walk.template run<POS>(
         [&](const Nb<MIX>& q, bool ok) {
             R rx, ry, rz;
             rel<MIX>(o, q, rx, ry, rz);
             const R qq = sqrt_(rx * rx + ry * ry + rz * rz) * ihu;
-            const R t = R(1) - R(0.5) * qq;
+            const R t = support_t(ok, R(1) - R(0.5) * qq);
             const R t2 = t * t;
-            sw.add_if(ok & (qq < R(2)), sigma * t2 * t2 * (R(1) + R(2) * qq));
+            sw.add(sigma * t2 * t2 * (R(1) + R(2) * qq));
         });
     if (fs_valid(a.L, a.nf)) {
         const double h13 = p.w / 1.3;
@@ -851,6 +1016,21 @@ __global__ void __launch_bounds__(kMaxBS) k_density_reinit(FluidArgs a, int i_st
         write_records<MIX>(f, a.cur, s, p, vm.x, vm.y, vm.z, nd);
     }
 }
+template <bool MIX>
+__global__ void __launch_bounds__(kMaxBS) k_density_reinit(FluidArgs a, int i_step) {
+    const int s = xcd_block() * blockDim.x + threadIdx.x;
+    if (s >= a.nf) return;
+    reinit_slot<MIX>(a, s, i_step, GlobalWalk<MIX, false>{a, rec_rsrc<MIX>(a.f, a.cur, a.nf), s});
+}
+__global__ void __launch_bounds__(kTileSlots) k_density_reinit_tile(FluidArgs a, int i_step) {
+    __shared__ TileLds lrec;
+    const int b = xcd_block();
+    const int m = stage_tile<false>(a, b, lrec);
+    const int s = tile_slot(a, b);
+    if (s < 0) return;
+    const bool tiled = m >= 0 && a.L.cnt[s] < 0;
+    reinit_slot<true>(a, s, i_step, TileWalk<false>{a, rec_rsrc<true>(a.f, a.cur, a.nf), s, lrec, tiled});
+}
 
 }  // namespace
 
@@ -868,7 +1048,11 @@ void fluid_records(hipStream_t s, const FluidArgs& a, int buf, bool mixed) {
 }
 
 void density_reinit(hipStream_t s, const FluidArgs& a, int i_step, bool mixed) {
-    if (a.nf > 0) DSPH_LAUNCH_BS(k_density_reinit, a.nf, a.bs, s, a, i_step);
+    if (a.nf <= 0) return;
+    if (mixed && a.T.n > 0)
+        hipLaunchKernelGGL(k_density_reinit_tile, dim3(a.T.n), dim3(kTileSlots), 0, s, a, i_step);
+    else
+        DSPH_LAUNCH_BS(k_density_reinit, a.nf, a.bs, s, a, i_step);
 }
 
 #define DSPH_LAUNCH_UNI(kern, n, bs, s, ...)                                                     \
@@ -884,13 +1068,30 @@ void density_reinit(hipStream_t s, const FluidArgs& a, int i_step, bool mixed) {
             hipLaunchKernelGGL((kern<false, false>), g, b, 0, s, __VA_ARGS__);                    \
     } while (0)
 
+// the tiled launch: one workgroup of kTileSlots threads per tile
+#define DSPH_LAUNCH_TILE(kern, s, ...)                                                                \
+    do {                                                                                          \
+        if (a.uni)                                                                                \
+            hipLaunchKernelGGL((kern<true>), dim3(a.T.n), dim3(kTileSlots), 0, s, __VA_ARGS__);   \
+        else                                                                                      \
+            hipLaunchKernelGGL((kern<false>), dim3(a.T.n), dim3(kTileSlots), 0, s, __VA_ARGS__);  \
+    } while (0)
+
 void continuity(hipStream_t s, const FluidArgs& a, double dt, bool mixed, double drift) {
-    if (a.nf > 0) DSPH_LAUNCH_UNI(k_continuity, a.nf, a.bs, s, a, dt, drift);
+    if (a.nf <= 0) return;
+    if (mixed && a.T.n > 0)
+        DSPH_LAUNCH_TILE(k_continuity_tile, s, a, dt, drift);
+    else
+        DSPH_LAUNCH_UNI(k_continuity, a.nf, a.bs, s, a, dt, drift);
 }
 
 
 void momentum(hipStream_t s, const FluidArgs& a, bool mixed, bool write_fs_force, double kick) {
-    if (a.nf > 0) DSPH_LAUNCH_UNI(k_momentum, a.nf, a.bs, s, a, (int)write_fs_force, kick);
+    if (a.nf <= 0) return;
+    if (mixed && a.T.n > 0)
+        DSPH_LAUNCH_TILE(k_momentum_tile, s, a, (int)write_fs_force, kick);
+    else
+        DSPH_LAUNCH_UNI(k_momentum, a.nf, a.bs, s, a, (int)write_fs_force, kick);
 }
 
 void fluid_solid_force(hipStream_t s, const FluidArgs& a, bool mixed) {

@@ -1391,6 +1391,192 @@ static int chunk_grid(int nchunk) {
     return b < 1 ? 1 : (b > 65535 ? 65535 : b);
 }
 
+// ------------------------------------------------------- pair-loop tiles
+// The union of a tile's candidates and its local rows, after a superset
+// build.  One workgroup of kTileSlots threads per tile, one thread per slot.
+// Slots are grouped in pages of 256 consecutive slots, and the union is
+// kept as one 256-bit membership mask per page it touches, in an LDS hash
+// table keyed by page (open addressing, linear probing):
+//  1. every candidate of the tile's rows sets its bit (the particle itself
+//     is left out: its term is an exact zero in continuity and momentum,
+//     and the Shepard sum adds it apart);
+//  2. the pages are ranked by page number, and each page's members get
+//     their union positions from the counts of the pages before it and of
+//     the mask words before theirs: the union in ascending slots (the pair
+//     loops stage its records with coalesced loads) with no sort;
+//  3. each row is written again as union positions, in its own order: a
+//     page lookup and a population count per candidate;
+//  4. the entries the pair loops read past a row's end get position 0.
+// More than kTilePages pages, a union larger than kTileCap or a candidate
+// not found leave the tile on its global rows (ucnt = -1).
+constexpr int kTilePageBits = 8;                 // 256 slots per page
+constexpr int kTilePageHash = 256;               // hash entries (power of two)
+constexpr int kTilePages = kTilePageHash / 2;    // pages a tile may touch
+constexpr int kTileBatch = 32;
+// multiplicative (Fibonacci) hashing: the top 8 bits of key * 2^32 / phi
+__device__ __forceinline__ unsigned page_hash(unsigned key) { return (key * 2654435761u) >> 24; }
+static_assert(kTilePageHash == 256, "page_hash keeps 8 bits");
+
+__global__ void __launch_bounds__(kTileSlots) k_tile_build(SearchArgs a) {
+    if (a.ctl[CTL_REBUILD] == 0) return;
+    __shared__ unsigned pkey[kTilePageHash];        // page + 1; 0 = empty
+    __shared__ unsigned pbits[kTilePageHash][8];    // membership mask of the page's 256 slots
+    __shared__ unsigned short wpos[kTilePageHash][8];  // union position of the mask word's first member
+    __shared__ int prank[kTilePageHash + 1];        // pages by rank: members, then exclusive prefix
+    __shared__ int npage, bad;
+    const Tiles& T = a.T;
+    const int b = xcd_block();
+    const int2 tr = T.range[b];
+    const int tid = threadIdx.x;
+    for (int h = tid; h < kTilePageHash; h += blockDim.x) {
+        pkey[h] = 0u;
+#pragma unroll
+        for (int w = 0; w < 8; w++) pbits[h][w] = 0u;
+    }
+    if (tid == 0) {
+        npage = 0;
+        bad = 0;
+    }
+    __syncthreads();
+    const int s = tr.x + tid;
+    const bool act = s < tr.y;
+    int g = 0, n = 0;
+    if (act) {
+        g = a.L.grow[s];
+        n = min(a.L.sup_cnt[row_slot(g)], a.L.supK);
+    }
+    // the row is read kTileBatch candidates at a time, all loads of a batch
+    // in flight together (entries past the row end read as the particle
+    // itself, which is skipped)
+    constexpr int V = kTileBatch;
+    auto batch = [&](int c0, int* jj) {
+#pragma unroll
+        for (int v = 0; v < V; v++) jj[v] = c0 + v < n ? a.L.sup[row_at(g, c0 + v, a.nf)] : s;
+    };
+    // the hash entry of page p, inserted if absent (-1: the table is full).
+    // Most candidates' pages are in already: a plain read finds them, and
+    // only an empty entry is claimed with an atomic (the lanes of a wave
+    // would otherwise queue on the same entries)
+    auto page_slot = [&](unsigned p, bool insert) -> int {
+        const unsigned key = p + 1u;
+        unsigned h = page_hash(key);
+        for (int q = 0; q < kTilePageHash; q++) {
+            unsigned old = pkey[h];
+            if (insert && old == 0u) old = atomicCAS(&pkey[h], 0u, key);
+            if (old == key) return (int)h;
+            if (old == 0u) {
+                if (!insert) return -1;
+                atomicAdd(&npage, 1);
+                return (int)h;
+            }
+            h = (h + 1u) & (kTilePageHash - 1);
+        }
+        return -1;
+    };
+    // 1. membership
+    for (int c0 = 0; c0 < n; c0 += V) {
+        int jj[V];
+        batch(c0, jj);
+        for (int v = 0; v < V; v++) {
+            if (jj[v] == s) continue;
+            const unsigned j = (unsigned)jj[v];
+            const int h = page_slot(j >> kTilePageBits, true);
+            if (h < 0) {
+                bad = 1;
+                continue;
+            }
+            // most members are set already (a candidate is in many rows)
+            unsigned* word = &pbits[h][(j >> 5) & 7];
+            const unsigned bit = 1u << (j & 31);
+            if ((*(volatile unsigned*)word & bit) == 0u) atomicOr(word, bit);
+        }
+    }
+    __syncthreads();
+    if (bad || npage > kTilePages) {
+        if (tid == 0) T.ucnt[b] = -1;
+        return;
+    }
+    // 2. the rank of each page by page number, its member count by rank
+    for (int h = tid; h < kTilePageHash; h += blockDim.x) {
+        const unsigned k = pkey[h];
+        if (k != 0u) {
+            int r = 0;
+            for (int h2 = 0; h2 < kTilePageHash; h2++) {
+                const unsigned k2 = pkey[h2];
+                r += (k2 != 0u) & (k2 < k);
+            }
+            int cnt = 0;
+#pragma unroll
+            for (int w = 0; w < 8; w++) cnt += __popc(pbits[h][w]);
+            prank[r] = cnt;
+            wpos[h][0] = (unsigned short)r;  // the rank until the offsets are in
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {  // exclusive prefix over at most kTilePages ranks
+        int acc = 0;
+        for (int r = 0; r < npage; r++) {
+            const int c = prank[r];
+            prank[r] = acc;
+            acc += c;
+        }
+        prank[npage] = acc;
+    }
+    __syncthreads();
+    const int m = prank[npage];
+    if (m > kTileCap) {
+        if (tid == 0) T.ucnt[b] = -1;
+        return;
+    }
+    int* uni = T.uni + (int64_t)b * kTileCap;
+    for (int h = tid; h < kTilePageHash; h += blockDim.x) {
+        const unsigned k = pkey[h];
+        if (k == 0u) continue;
+        int at = prank[wpos[h][0]];
+        const unsigned base = (k - 1u) << kTilePageBits;
+#pragma unroll
+        for (int w = 0; w < 8; w++) {
+            wpos[h][w] = (unsigned short)at;
+            for (unsigned bits = pbits[h][w]; bits; bits &= bits - 1u)
+                uni[at++] = (int)(base + 32u * w + (unsigned)__builtin_ctz(bits));
+        }
+    }
+    __syncthreads();
+    // 3. the local rows
+    if (act) {
+        int w = 0;
+        for (int c0 = 0; c0 < n; c0 += V) {
+            int jj[V];
+            batch(c0, jj);
+            for (int v = 0; v < V; v++) {
+                if (jj[v] == s) continue;
+                const unsigned j = (unsigned)jj[v];
+                const int h = page_slot(j >> kTilePageBits, false);
+                int pos = 0;
+                if (h < 0) {
+                    bad = 1;
+                } else {
+                    const unsigned word = (j >> 5) & 7;
+                    pos = wpos[h][word] + __popc(pbits[h][word] & ((1u << (j & 31)) - 1u));
+                }
+                T.row[tile_row_at(w, s, a.nf)] = (unsigned short)pos;
+                w++;
+            }
+        }
+        // 4. the read-ahead entries, to the end of the next quad (w + 8 <=
+        // supK + kRowPad)
+        for (int c = w; c < ((w + 3) & ~3) + 4; c++) T.row[tile_row_at(c, s, a.nf)] = 0;
+        T.rcnt[s] = w;
+    }
+    __syncthreads();
+    if (tid == 0) T.ucnt[b] = bad ? -1 : m;
+}
+
+void tile_rows(hipStream_t s, const SearchArgs& a) {
+    if (a.nf > 0 && a.T.n > 0)
+        hipLaunchKernelGGL(k_tile_build, dim3(a.T.n), dim3(kTileSlots), 0, s, a);
+}
+
 void fluid_superset(hipStream_t s, const SearchArgs& a, const int2* chunks, int nchunk) {
     if (a.nf <= 0) return;
     hipLaunchKernelGGL(k_build, dim3(chunk_grid(nchunk)), dim3(kBS), 0, s, a, chunks, nchunk);

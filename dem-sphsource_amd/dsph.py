@@ -24,7 +24,7 @@ LIB_PATH = os.path.join(HERE, "libdsph.so")
 
 PREC_FP64, PREC_MIXED = 0, 1
 FLAG_WALL_CONTINUE, FLAG_CLUMP_QUATERNION, FLAG_ART_VISCOSITY = 1, 2, 4
-TUNE_PAIR_BLOCK, TUNE_BUILD_BLOCK, TUNE_GROUP = 1, 2, 3
+TUNE_PAIR_BLOCK, TUNE_BUILD_BLOCK, TUNE_GROUP, TUNE_TILE = 1, 2, 3, 4
 LIST_FLUID_FLUID, LIST_FLUID_SOLID, LIST_SOLID_SOLID, LIST_BOND = 0, 1, 2, 3
 ST_FF_OVERFLOW, ST_FS_OVERFLOW, ST_SS_OVERFLOW, ST_OUT_OF_DOMAIN = 0, 1, 2, 3
 ST_FF_MAX, ST_FS_TOTAL, ST_SS_TOTAL, ST_SUP_OVERFLOW, ST_REBUILDS = 4, 5, 6, 7, 8
@@ -76,6 +76,7 @@ _SIGS = {
     "dsph_download_grid": [C.c_int, _ip, _ip, _ip, _ip],
     "dsph_status": [_i64p],
     "dsph_superset_stats": [_i64p],
+    "dsph_tile_stats": [_i64p],
     "dsph_hash_sort": [_vp, C.c_int32, _dp, _dp, _ip, _vp, _vp, _vp, _vp],
     "dsph_hash_sort_noalloc": [_vp, C.c_int32, _dp, _dp, _ip, _vp, _vp, _vp, _vp],
     "dsph_set_timing": [C.c_int],
@@ -201,6 +202,8 @@ class Solver:
             self.set_tuning(TUNE_BUILD_BLOCK, cfg["build_block"])
         if "group" in cfg:
             self.set_tuning(TUNE_GROUP, cfg["group"])
+        if "tile" in cfg:
+            self.set_tuning(TUNE_TILE, cfg["tile"])
         if "skin" in cfg:
             self.set_skin(cfg["skin"])
 
@@ -370,6 +373,13 @@ class Solver:
         self._call("dsph_superset_stats", _p(w, _i64p))
         return dict(particles=int(w[0]), walked=int(w[1]), stored=int(w[2]), longest=int(w[3]))
 
+    def tile_stats(self):
+        """The pair-loop tiles of the last superset build: {"tiles",
+        "staged", "records", "largest"} (dsph_tile_stats)."""
+        w = np.zeros(4, np.int64)
+        self._call("dsph_tile_stats", _p(w, _i64p))
+        return dict(tiles=int(w[0]), staged=int(w[1]), records=int(w[2]), largest=int(w[3]))
+
     def hash_sort(self, pos_ptr, n, origin, size, cells, hash_ptr, perm_ptr, start_ptr, end_ptr,
                   noalloc=False):
         """dsph_hash_sort (or dsph_hash_sort_noalloc) on caller device arrays
@@ -397,6 +407,7 @@ TUNING_SPACE = {
     "pair_block": [64, 128, 256, 512],          # workgroup of the fluid pair loops
     "build_block": [32, 48, 64],                # particles per wavefront in the superset build
     "group": [1, 2, 4, 8],                      # particles sharing one superset row
+    "tile": [0, 1],                             # mixed pair loops over LDS tiles
 }
 
 
@@ -406,8 +417,8 @@ def autotune(scene, precision=PREC_MIXED, flags=0, trial_steps=20, warmup=5, spa
 
     Runs the workload (a Scene, or a list of Scenes uploaded as one batch) on
     a throwaway context and times `trial_steps` graph-replayed steps per
-    candidate, one coordinate of the search space at a time (row group,
-    skin, pair-loop workgroup, build block), keeping the best value of each
+    candidate, one coordinate of the search space at a time (LDS tiles, row
+    group, skin, pair-loop workgroup, build block), keeping the best value of each
     before moving to the next.  With `advance`, the context first runs that
     many steps and every trial starts from a checkpoint of that state, so
     all candidates time the same window of the moving flow; otherwise the
@@ -424,7 +435,7 @@ def autotune(scene, precision=PREC_MIXED, flags=0, trial_steps=20, warmup=5, spa
         g.upload(scene)
         dt = scene.dt
     g.initialize()
-    cfg = {"skin": 0.3, "pair_block": 256, "build_block": 64, "group": 1}
+    cfg = {"skin": 0.3, "pair_block": 256, "build_block": 64, "group": 1, "tile": 1}
     trials = []
     step = [1]
     blob = None
@@ -460,7 +471,7 @@ def autotune(scene, precision=PREC_MIXED, flags=0, trial_steps=20, warmup=5, spa
             log(f"autotune {c} -> {ms:.4f} ms/step")
         return ms
 
-    for key in ("group", "skin", "pair_block", "build_block"):
+    for key in ("tile", "group", "skin", "pair_block", "build_block"):
         best = None
         for v in space[key]:
             ms = measure(dict(cfg, **{key: v}))

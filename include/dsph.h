@@ -227,11 +227,20 @@ int dsph_set_skin(dsph_ctx* ctx, double skin_over_h);
  *                          of their supersets: 1, 2, 4 or 8 (default 1).  A
  *                          shared row costs more candidates per particle but
  *                          lets the group's gathers coalesce.
+ *   DSPH_TUNE_TILE         mixed precision only: 1 runs the pair loops over
+ *                          tiles of 1024 consecutive sorted particles, each
+ *                          staging its neighbours' records in LDS once and
+ *                          walking its rows there (default); 0 gathers every
+ *                          candidate's record from HBM.  Results are bitwise
+ *                          the same either way.  Needs the reorder (period
+ *                          > 0); a tile whose neighbours do not fit in LDS
+ *                          gathers from HBM.
  * The Verlet skin (dsph_set_skin) and the reorder period complete the
  * autotuner's search space (dsph.py autotune, dataBase::autotune).        */
 #define DSPH_TUNE_PAIR_BLOCK   1
 #define DSPH_TUNE_BUILD_BLOCK  2
 #define DSPH_TUNE_GROUP        3
+#define DSPH_TUNE_TILE         4
 int dsph_set_tuning(dsph_ctx* ctx, int32_t key, int32_t value);
 int dsph_get_tuning(dsph_ctx* ctx, int32_t key, int32_t* value);
 
@@ -369,6 +378,13 @@ int dsph_status(dsph_ctx* ctx, int64_t words[DSPH_STATUS_WORDS]);
  * against the neighbours per particle of the fluid-fluid list.  No
  * reference counterpart (the reference walks its 27 cells every pass). */
 int dsph_superset_stats(dsph_ctx* ctx, int64_t out[4]);
+
+/* The pair-loop tiles of the last superset build (DSPH_TUNE_TILE):
+ * out[0] tiles (0 when the tiles are off), out[1] tiles staged in LDS (the
+ * others' neighbours did not fit and they gather from HBM), out[2] records
+ * staged over those tiles, out[3] the largest tile union.  No reference
+ * counterpart. */
+int dsph_tile_stats(dsph_ctx* ctx, int64_t out[4]);
 
 /* ---- the hashed grid of caller-held particles --------------------------
  * replaces: updateGridCellStartEnd for one species (neighborSearch.cu:89-113)

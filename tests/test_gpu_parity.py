@@ -605,12 +605,13 @@ def test_batch_matches_scenes_run_alone(prec):
 def test_launch_shapes_do_not_change_results(prec):
     sc = SCENES["coupled_small"]
     runs = []
-    for pair, build, group in ((256, 64, 1), (64, 32, 4), (512, 17, 8), (128, 64, 2)):
+    for pair, build, group, tile in ((256, 64, 1, 0), (64, 32, 4, 1), (512, 17, 8, 0), (128, 64, 2, 1)):
         g = dsph.Solver(prec)
         g.upload(sc)
         g.set_tuning(dsph.TUNE_PAIR_BLOCK, pair)
         g.set_tuning(dsph.TUNE_BUILD_BLOCK, build)
         g.set_tuning(dsph.TUNE_GROUP, group)
+        g.set_tuning(dsph.TUNE_TILE, tile)
         assert g.get_tuning(dsph.TUNE_PAIR_BLOCK) == pair
         g.initialize()
         g.run(15, sc.dt, graphs=True)
@@ -620,6 +621,77 @@ def test_launch_shapes_do_not_change_results(prec):
         for k in f:
             assert np.array_equal(f[k], runs[0][0][k]), k
         assert all(np.array_equal(a, b) for a, b in zip(lst, runs[0][1]))
+
+
+# ------------------------------------------------------------- tiles
+def run_tiled_and_gathered(sc, steps, group=1, flags=0, gap=1, dt=None, batch=None, skin=None):
+    """The same mixed run with the pair loops over LDS tiles and gathering
+    from HBM (tile 1, then 0): the fields at the end of each.  Steps 1-5
+    eager, the rest replayed from HIP graphs."""
+    dt = dt or (min(x.dt for x in batch) if batch else sc.dt)
+    out = []
+    for tile in (1, 0):
+        g = dsph.Solver(dsph.PREC_MIXED)
+        g.set_flags(flags)
+        if batch:
+            g.upload_batch(batch)
+        else:
+            g.upload(sc)
+        g.set_tuning(dsph.TUNE_TILE, tile)
+        g.set_tuning(dsph.TUNE_GROUP, group)
+        assert g.get_tuning(dsph.TUNE_TILE) == tile
+        if skin is not None:
+            g.set_skin(skin)
+        g.initialize()
+        g.run(5, dt, gap, first=1, graphs=False)
+        g.run(steps - 5, dt, gap, first=6, graphs=True)
+        out.append(fields(g))
+        g.close()
+    return out
+
+
+@pytest.mark.parametrize("name,group", [("coupled_small", 1), ("coupled_small", 2), ("random_box", 1),
+                                        ("random_box", 4), ("dam_break_dx004", 1)])
+def test_tiled_pair_loops_equal_gathered_bitwise(name, group):
+    # the tiles change where a candidate's record is read (LDS instead of
+    # HBM), not which candidates are summed or in which order
+    note(f"tiles {name} group {group}")
+    a, b = run_tiled_and_gathered(SCENES[name], 25, group=group)
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_tiled_pair_loops_with_walked_rows_and_gap():
+    # rows walked on the reference stencil (a particle leaving the domain)
+    # take the global walk inside a tiled launch; the reference's case
+    # sub-cycles the fluid (gap > 1)
+    sc = scenes.dam_break_case_small()
+    note(f"tiles damBreak case: {sc.n_fluid} fluid, gap {sc.gap}")
+    a, b = run_tiled_and_gathered(sc, 2 * sc.gap + 4, gap=sc.gap)
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+    dx = 0.005
+    sc = scenes.empty_scene((-3 * dx,) * 3, (0.04 + 6 * dx, 0.03 + 6 * dx, 0.03 + 6 * dx))
+    top = -3 * dx + 0.03 + 6 * dx
+    scenes.add_fluid(sc, scenes.regular_packed_points((0, 0, top - 0.02 - 2 * dx), (0.04, 0.03, 0.02), dx),
+                     1.3 * dx, 1000.0, 20.0, 1e-6)
+    scenes.add_fluid(sc, [[0.03, 0.02, top - 0.3 * dx]], 1.3 * dx, 1000.0, 20.0, 1e-6)
+    vel = np.zeros((sc.n_fluid, 3))
+    vel[-1] = [0.0, 0.0, 1.0]
+    sc.fluid_vel = vel
+    sc.gravity = np.zeros(3)
+    note(f"tiles with a particle leaving the domain: {sc.n_fluid} fluid")
+    a, b = run_tiled_and_gathered(sc, 30, dt=2e-4, skin=0.3)
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_tiled_pair_loops_in_a_batch():
+    batch = batch_scenes()
+    note(f"tiles in a batch of {len(batch)} scenes")
+    a, b = run_tiled_and_gathered(None, 20, batch=batch)
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
 
 
 def test_autotune_returns_a_configuration_of_its_space():
