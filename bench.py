@@ -55,6 +55,7 @@ def new_solver(scene, precision, flags=0, tuning=None):
     g.set_flags(flags)
     if isinstance(scene, list):
         g.upload_batch(scene)
+        g.set_scene_time_steps([sc.dt for sc in scene])  # each scene at its own step
     else:
         g.upload(scene)
     if tuning:
@@ -69,7 +70,7 @@ def timed_window(scene, precision, steps, warmup, timing=True, flags=0, tuning=N
     kernel times come from the same window run again from a checkpoint of
     its first step, eagerly, with HIP events on the context stream."""
     import dsph
-    dt = min(sc.dt for sc in scene) if isinstance(scene, list) else scene.dt
+    dt = 0.0 if isinstance(scene, list) else scene.dt  # a batch steps each scene by its own
     g = new_solver(scene, precision, flags, tuning)
     if warmup:
         g.run(warmup, dt, first=1, graphs=graphs)

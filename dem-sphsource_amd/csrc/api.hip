@@ -153,6 +153,7 @@ struct dsph_ctx {
     unsigned long long* status = nullptr;
     int reorder_period = 1;
     double graph_dt = -1.0;  // time step baked into the captured graphs
+    double* sdt = nullptr;   // per-scene time steps of a batch (device), or none
     std::vector<void*> allocs;
     std::map<uint64_t, StepGraph> graphs;
     // timing
@@ -211,6 +212,7 @@ void free_all(dsph_ctx* c) {
     for (void* p : c->allocs) (void)hipFree(p);
     c->allocs.clear();
     c->T = Tiles{};
+    c->sdt = nullptr;
     c->backup.clear();
     c->backup_bytes.clear();
     c->hs_work = nullptr;
@@ -576,6 +578,7 @@ FluidArgs fluid_args(dsph_ctx* c) {
     a.gz = c->gv[2];
     a.flags = c->flags;
     a.T = c->T;
+    a.sdt = c->sdt;
     return a;
 }
 
@@ -609,6 +612,7 @@ DemArgs dem_args(dsph_ctx* c) {
     a.gy = c->gv[1];
     a.gz = c->gv[2];
     a.flags = c->flags;
+    a.sdt = c->sdt;
     return a;
 }
 
@@ -2176,9 +2180,46 @@ int dsph_neighbor_search(dsph_ctx* c, int32_t i_step, int32_t gap) {
     }
 }
 
+namespace {
+// The dt the phases compute with: the caller's, or with per-scene time
+// steps set (the caller then passes 0) 1, so every phase hands its kernels
+// the factor of each scene's own step
+double phase_dt(const dsph_ctx* c, double dt) {
+    if (!c->sdt) return dt;
+    if (dt != 0.0)
+        throw ApiError{DSPH_E_INVALID, "per-scene time steps are set: pass dt = 0 (or clear them)"};
+    return 1.0;
+}
+}  // namespace
+
+int dsph_set_scene_time_steps(dsph_ctx* c, const double* dt) {
+    API_GUARD(c);
+    if (!c->uploaded) return DSPH_E_STATE;
+    try {
+        HIPCHK(hipStreamSynchronize(c->st));
+        if (!dt) {
+            dfree(c, c->sdt);
+            c->sdt = nullptr;
+        } else {
+            require(c->nscene > 1, "per-scene time steps need a batch of scenes (dsph_upload_batch)");
+            for (int k = 0; k < c->nscene; k++)
+                require(std::isfinite(dt[k]) && dt[k] > 0.0, "scene time steps must be positive and finite");
+            if (!c->sdt) c->sdt = dalloc<double>(c, c->nscene);
+            h2d(c, c->sdt, dt, c->nscene);
+            HIPCHK(hipStreamSynchronize(c->st));
+        }
+        invalidate_graphs(c);
+        c->graph_dt = -1.0;
+        return DSPH_OK;
+    } catch (const ApiError& e) {
+        return fail(c, e);
+    }
+}
+
 int dsph_solid_before_contact(dsph_ctx* c, double dt) {
     API_GUARD(c);
     try {
+        dt = phase_dt(c, dt);
         solid_before(c, dt);
         return DSPH_OK;
     } catch (const ApiError& e) {
@@ -2190,6 +2231,7 @@ int dsph_fluid_integrate(dsph_ctx* c, double dt, int32_t i_step, int32_t gap) {
     API_GUARD(c);
     if (gap < 1) return DSPH_E_INVALID;
     try {
+        dt = phase_dt(c, dt);
         fluid_integrate(c, dt, i_step, gap);
         return DSPH_OK;
     } catch (const ApiError& e) {
@@ -2200,6 +2242,7 @@ int dsph_fluid_integrate(dsph_ctx* c, double dt, int32_t i_step, int32_t gap) {
 int dsph_contact_forces(dsph_ctx* c, double dt) {
     API_GUARD(c);
     try {
+        dt = phase_dt(c, dt);
         contacts(c, dt);
         return DSPH_OK;
     } catch (const ApiError& e) {
@@ -2210,6 +2253,7 @@ int dsph_contact_forces(dsph_ctx* c, double dt) {
 int dsph_solid_after_contact(dsph_ctx* c, double dt) {
     API_GUARD(c);
     try {
+        dt = phase_dt(c, dt);
         solid_after(c, dt);
         return DSPH_OK;
     } catch (const ApiError& e) {
@@ -2222,6 +2266,7 @@ int dsph_step(dsph_ctx* c, int32_t i_step, double dt, int32_t gap, int check_cap
     if (!c->initialized) return DSPH_E_STATE;
     if (gap < 1) return DSPH_E_INVALID;
     try {
+        dt = phase_dt(c, dt);
         if (check_capacity)
             search_checked(c, i_step, gap);
         else
@@ -2241,6 +2286,7 @@ int dsph_step_n(dsph_ctx* c, int32_t i0, int32_t n, double dt, int32_t gap) {
     if (!c->initialized) return DSPH_E_STATE;
     if (gap < 1 || n < 0) return DSPH_E_INVALID;
     try {
+        dt = phase_dt(c, dt);
         const bool timing = c->timing;
         if (dt != c->graph_dt) {
             invalidate_graphs(c);

@@ -94,6 +94,15 @@ __device__ __forceinline__ int solid_scene(const Grids& G, int i) {
     return G.nscene == 1 ? 0 : G.sscene[i];
 }
 
+// Per-scene time steps of a batch (dsph_set_scene_time_steps): with sdt
+// set, a phase's dt argument is its factor (1/2, 1, the fluid gap) and
+// particle i of scene scene[i] steps by that factor times sdt[scene[i]]
+// (the product of the one-scene run: factor and step commute exactly);
+// without, by dt itself.
+__device__ __forceinline__ double step_of(const double* sdt, const int* scene, int i, double dt) {
+    return sdt ? dt * sdt[scene[i]] : dt;
+}
+
 struct FluidSet {
     double4* ph = nullptr;              // x, y, z, h
     double4* vr = nullptr;              // vx, vy, vz, rho0 (FP64 master)
@@ -437,6 +446,7 @@ struct FluidArgs {
     double gx, gy, gz;
     uint32_t flags;
     Tiles T;  // n > 0: the mixed pair loops run tiled
+    const double* sdt = nullptr;  // per-scene time steps (step_of, scenes f.scene), or none
 };
 void fluid_records(hipStream_t s, const FluidArgs& a, int buf, bool mixed);
 void density_reinit(hipStream_t s, const FluidArgs& a, int i_step, bool mixed);
@@ -479,6 +489,7 @@ struct DemArgs {
     ContactTables tb;
     double gx, gy, gz;
     uint32_t flags;
+    const double* sdt = nullptr;  // per-scene time steps (step_of), or none
 };
 void dem_contacts(hipStream_t s, const DemArgs& a, double dt);
 void dem_bonds(hipStream_t s, const DemArgs& a, double dt);

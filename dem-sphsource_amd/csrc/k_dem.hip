@@ -48,6 +48,7 @@ __global__ void k_ss_contact(DemArgs a, double dt) {
     for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < tot;
          k += (int64_t)gridDim.x * blockDim.x) {
         const int i = a.L.ss_i[k], j = a.L.ss_j[k];
+        const double dti = step_of(a.sdt, a.sscene, i, dt);
         const double4 pri = a.so.pr[i], prj = a.so.pr[j];
         const d3 pi = v3(pri), pj = v3(prj);
         d3 n;
@@ -61,7 +62,7 @@ __global__ void k_ss_contact(DemArgs a, double dt) {
         d3 F{0, 0, 0}, T{0, 0, 0};
         Springs sp = ld_springs(a.L.ssS + 9 * k);
         const int c = combined_index(a.tb.nmat, a.so.mat[i], a.so.mat[j]);
-        contact_model(a.tb, c, F, T, sp, vc, wi - wj, n, delta, m, R, dt);
+        contact_model(a.tb, c, F, T, sp, vc, wi - wj, n, delta, m, R, dti);
         a.L.ssF[k] = d4(F);
         a.L.ssT[k] = d4(T);
         st_springs(a.L.ssS + 9 * k, sp);
@@ -102,8 +103,8 @@ __global__ void k_bonds(DemArgs a, double dt) {
     const int c = combined_index(a.tb.nmat, a.so.mat[i], a.so.mat[j]);
     double Fn = a.b.Fn[b], Tt = a.b.Tt[b];
     d3 Fs = v3(a.b.Fs[b]), Tb = v3(a.b.Tb[b]);
-    a.b.on[b] = parallel_bond(Fn, Tt, Fs, Tb, v3(a.b.nrm[b]), n, vc, wi, wj, pri.w, prj.w, dt,
-                              a.tb.bonded, a.tb.ncomb, c);
+    a.b.on[b] = parallel_bond(Fn, Tt, Fs, Tb, v3(a.b.nrm[b]), n, vc, wi, wj, pri.w, prj.w,
+                              step_of(a.sdt, a.sscene, i, dt), a.tb.bonded, a.tb.ncomb, c);
     a.b.Fn[b] = Fn;
     a.b.Tt[b] = Tt;
     a.b.Fs[b] = d4(Fs);
@@ -240,8 +241,9 @@ __global__ void k_solid_sum(DemArgs a, double dt) {
         for (int k = a.L.fsr_off[i], e = k + a.L.fsr_cnt[i]; k < e; k++) F -= v3(a.L.fsF[a.L.fsr_idx[k]]);
     const double gon = (v.w > 0.0 && a.so.clump[i] < 0) ? 1.0 : 0.0;
     d3 acc = F * v.w + d3{a.gx, a.gy, a.gz} * gon;
-    if (a.nw > 0) wall_forces(a, i, dt, acc, T);
-    if (a.ny > 0) cylinder_forces(a, i, dt, acc, T);
+    const double dti = step_of(a.sdt, a.sscene, i, dt);
+    if (a.nw > 0) wall_forces(a, i, dti, acc, T);
+    if (a.ny > 0) cylinder_forces(a, i, dti, acc, T);
     a.so.acc[i] = d4(acc);
     a.so.tq[i] = d4(T);
 }
@@ -267,6 +269,7 @@ __global__ void k_clump_sum(DemArgs a) {
 
 __device__ __forceinline__ void solid_kick_one(const DemArgs& a, int i, double dt) {
     if (a.so.clump[i] >= 0) return;
+    dt = step_of(a.sdt, a.sscene, i, dt);
     double4 v = a.so.vel[i];
     const double4 ac = a.so.acc[i];
     v.x += ac.x * dt;
@@ -290,6 +293,7 @@ __global__ void k_solid_kick(DemArgs a, double dt) {
 }
 
 __device__ __forceinline__ void solid_drift_one(const DemArgs& a, int i, double dt) {
+    dt = step_of(a.sdt, a.sscene, i, dt);
     double4 p = a.so.pr[i];
     const double4 v = a.so.vel[i];
     p.x += dt * v.x;
@@ -313,9 +317,15 @@ __global__ void k_solid_kick_drift(DemArgs a, double dt, int kick_first) {
     if (kick_first) solid_drift_one(a, i, dt);
 }
 
+// the scene of a clump is its pebbles' (a clump with none has no scene step)
+__device__ __forceinline__ double clump_step(const DemArgs& a, int c, double dt) {
+    return a.sdt && a.c.start[c] < a.c.end[c] ? step_of(a.sdt, a.sscene, a.c.start[c], dt) : dt;
+}
+
 __global__ void k_clump_kick(DemArgs a, double dt) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= a.nc) return;
+    dt = clump_step(a, c, dt);
     const d3 ac = v3(a.c.acc[c]);
     const d3 w0 = v3(a.c.w[c]);
     double4 cv = a.c.vel[c];
@@ -344,6 +354,7 @@ __global__ void k_clump_kick(DemArgs a, double dt) {
 __global__ void k_clump_drift(DemArgs a, double dt) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= a.nc) return;
+    dt = clump_step(a, c, dt);
     double4 p = a.c.pos[c];
     const double4 v = a.c.vel[c];
     p.x += dt * v.x;

@@ -499,10 +499,10 @@ __device__ __forceinline__ void load_tile_rec(const TileLds& L, int j, Nb<true>&
         o.vx = v.x; o.vy = v.y; o.vz = v.z; o.drho = v.w;
     }
 }
-// one quad of a local row: four entries, one 8-byte load
+// quads of a local row: four entries in one 8-byte load, kept two batches
+// ahead of use (the rows stream from HBM)
 typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ void row_quad(const u32x2* p, int* j) {
-    const u32x2 v = __builtin_nontemporal_load(p);
+__device__ __forceinline__ void quad_entries(u32x2 v, int* j) {
     j[0] = (int)(v.x & 0xffffu);
     j[1] = (int)(v.x >> 16);
     j[2] = (int)(v.y & 0xffffu);
@@ -514,11 +514,16 @@ __device__ __forceinline__ void for_row_tile(const FluidArgs& a, const TileLds& 
     const int n = a.T.rcnt[s];
     const int64_t st = a.nf;
     const u32x2* ip = (const u32x2*)a.T.row + s;
-    int jn[U];
-    row_quad(ip, jn);
-#pragma unroll
-    for (int u = 0; u < U; u++) jn[u] = apart(jn[u]);
+    u32x2 q0 = __builtin_nontemporal_load(ip), q1 = __builtin_nontemporal_load(ip + st);
+    q0.x = (unsigned)apart((int)q0.x);
+    q0.y = (unsigned)apart((int)q0.y);
+    q1.x = (unsigned)apart((int)q1.x);
+    q1.y = (unsigned)apart((int)q1.y);
     for (int k = 0; k < n; k += U) {
+        int jn[U];
+        quad_entries(q0, jn);
+        q0 = q1;
+        q1 = __builtin_nontemporal_load(ip + 2 * st);
         ip += st;
         Nb<true> q[U];
         bool ok[U];
@@ -527,7 +532,6 @@ __device__ __forceinline__ void for_row_tile(const FluidArgs& a, const TileLds& 
             load_tile_rec<MODE != POS>(lrec, jn[u], q[u]);
             ok[u] = k + u < n;
         }
-        row_quad(ip, jn);
         if constexpr (MODE != POS) {
 #pragma unroll
             for (int u = 0; u < U; u++) {
@@ -724,6 +728,11 @@ template <bool MIX, bool UNI, class Walk>
 __device__ __forceinline__ void continuity_slot(const FluidArgs& a, int s, double dt, double drift,
                                                 const Walk& walk) {
     using R = typename Prec<MIX>::R;
+    if (a.sdt) {  // factors of the scene's own step
+        const double k = a.sdt[a.f.scene[s]];
+        dt *= k;
+        drift *= k;
+    }
     const FluidSet& f = a.f;
     const Nb<MIX> o = load_nb<MIX, UNI>(a, a.cur, s);
     const double4 ps = f.ph[s];
@@ -817,6 +826,7 @@ template <bool MIX, bool UNI, class Walk>
 __device__ __forceinline__ void momentum_slot(const FluidArgs& a, int s, int write_fs, double kick,
                                               const Walk& walk) {
     using R = typename Prec<MIX>::R;
+    if (a.sdt) kick *= a.sdt[a.f.scene[s]];  // a factor of the scene's own step
     const FluidSet& f = a.f;
     const Nb<MIX> o = load_nb<MIX, UNI>(a, a.cur, s);
     const double4 ps = f.ph[s];

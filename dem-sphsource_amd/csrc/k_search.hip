@@ -1563,9 +1563,9 @@ __global__ void __launch_bounds__(kTileSlots) k_tile_build(SearchArgs a) {
                 w++;
             }
         }
-        // 4. the read-ahead entries, to the end of the next quad (w + 8 <=
-        // supK + kRowPad)
-        for (int c = w; c < ((w + 3) & ~3) + 4; c++) T.row[tile_row_at(c, s, a.nf)] = 0;
+        // 4. the read-ahead entries, to the end of the second quad after the
+        // row's last (w + 11 <= supK + kRowPad)
+        for (int c = w; c < ((w + 3) & ~3) + 8; c++) T.row[tile_row_at(c, s, a.nf)] = 0;
         T.rcnt[s] = w;
     }
     __syncthreads();

@@ -77,6 +77,7 @@ _SIGS = {
     "dsph_status": [_i64p],
     "dsph_superset_stats": [_i64p],
     "dsph_tile_stats": [_i64p],
+    "dsph_set_scene_time_steps": [_dp],
     "dsph_hash_sort": [_vp, C.c_int32, _dp, _dp, _ip, _vp, _vp, _vp, _vp],
     "dsph_hash_sort_noalloc": [_vp, C.c_int32, _dp, _dp, _ip, _vp, _vp, _vp, _vp],
     "dsph_set_timing": [C.c_int],
@@ -373,6 +374,16 @@ class Solver:
         self._call("dsph_superset_stats", _p(w, _i64p))
         return dict(particles=int(w[0]), walked=int(w[1]), stored=int(w[2]), longest=int(w[3]))
 
+    def set_scene_time_steps(self, dts):
+        """Per-scene time steps of a batch (dsph_set_scene_time_steps); the
+        step calls then take dt = 0.  None clears them."""
+        if dts is None:
+            self._call("dsph_set_scene_time_steps", None)
+            return
+        d = np.ascontiguousarray(dts, np.float64)
+        self._keep_sdt = d
+        self._call("dsph_set_scene_time_steps", _p(d))
+
     def tile_stats(self):
         """The pair-loop tiles of the last superset build: {"tiles",
         "staged", "records", "largest"} (dsph_tile_stats)."""
@@ -430,7 +441,8 @@ def autotune(scene, precision=PREC_MIXED, flags=0, trial_steps=20, warmup=5, spa
     g.set_flags(flags)
     if isinstance(scene, (list, tuple)):
         g.upload_batch(scene)
-        dt = min(sc.dt for sc in scene)
+        g.set_scene_time_steps([sc.dt for sc in scene])  # each scene at its own step
+        dt = 0.0
     else:
         g.upload(scene)
         dt = scene.dt

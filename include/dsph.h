@@ -228,7 +228,7 @@ int dsph_set_skin(dsph_ctx* ctx, double skin_over_h);
  *                          shared row costs more candidates per particle but
  *                          lets the group's gathers coalesce.
  *   DSPH_TUNE_TILE         mixed precision only: 1 runs the pair loops over
- *                          tiles of 1024 consecutive sorted particles, each
+ *                          tiles of 512 consecutive sorted particles, each
  *                          staging its neighbours' records in LDS once and
  *                          walking its rows there (default); 0 gathers every
  *                          candidate's record from HBM.  Results are bitwise
@@ -284,6 +284,15 @@ int dsph_step(dsph_ctx* ctx, int32_t i_step, double dt, int32_t gap,
  * neighborSearch.cu:339-340,373). */
 int dsph_step_n(dsph_ctx* ctx, int32_t i_step0, int32_t n, double dt,
                 int32_t gap);
+
+/* Per-scene time steps of a batch (dsph_upload_batch): scene k advances by
+ * dt[k] (n_scenes values, each > 0) per step, as it does run alone, instead
+ * of every scene by the one dt of the call.  While they are set, the dt
+ * argument of dsph_step, dsph_step_n and the phase functions must be 0
+ * (DSPH_E_INVALID otherwise); dt = NULL clears them.  A context of one
+ * scene returns DSPH_E_INVALID.  No reference counterpart (the reference
+ * runs one scene at a time). */
+int dsph_set_scene_time_steps(dsph_ctx* ctx, const double* dt);
 int dsph_synchronize(dsph_ctx* ctx);
 
 /* ---- full-state checkpoint ---------------------------------------------

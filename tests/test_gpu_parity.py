@@ -525,6 +525,71 @@ def batch_scenes():
     return scs
 
 
+@pytest.mark.parametrize("prec", [dsph.PREC_FP64, dsph.PREC_MIXED])
+def test_batch_with_scene_time_steps_matches_scenes_run_alone(prec):
+    """With per-scene time steps every scene of a batch advances by its own
+    dt, as it does alone: fields after 20 steps (10 eager, 10 replayed from
+    graphs) against the oracle running each scene alone at its dt."""
+    scs = batch_scenes()
+    dts = [sc.dt for sc in scs]
+    assert len(set(dts)) > 1
+    g = dsph.Solver(prec)
+    g.upload_batch(scs)
+    g.set_scene_time_steps(dts)
+    g.initialize()
+    info = g.batch_info()
+    fo, so_ = info["fluid"], info["solid"]
+    os_ = []
+    for sc in scs:
+        o = oracle.Oracle()
+        o.upload(sc)
+        o.initialize()
+        os_.append(o)
+    g.run(10, 0.0, first=1, graphs=False)
+    g.run(10, 0.0, first=11, graphs=True)
+    for o, sc in zip(os_, scs):
+        o.run(20, sc.dt)
+    # FP64: the one-step bar holds over the 20 steps; mixed: the one-step
+    # 1e-5 of FP32 pair terms, accumulated over 20 steps of moving fluid
+    tol = TOL_FP64 if prec == dsph.PREC_FP64 else 20 * TOL_MIXED
+    gf, gs = g.fluid(), g.solid()
+    floors = {"pos": 0.0, "vel": 1e-3, "drho": 1000.0, "angvel": 1e-3}
+    for k, o in enumerate(os_):
+        of, osd = o.fluid(), o.solid()
+        for key in ("pos", "vel", "drho"):
+            a = gf[key][fo[k]:fo[k + 1]]
+            if a.size:
+                assert refmodel.rel_err(a, of[key], floors[key]) < tol, (k, key)
+        for key in ("pos", "vel", "angvel"):
+            a = gs[key][so_[k]:so_[k + 1]]
+            if a.size:
+                assert refmodel.rel_err(a, osd[key], floors[key]) < tol, (k, key)
+    note(f"scene time steps {['%.3g' % d for d in dts]}: each scene matches its run alone")
+
+
+def test_scene_time_steps_reject_misuse():
+    scs = batch_scenes()
+    g = dsph.Solver(dsph.PREC_MIXED)
+    g.upload_batch(scs)
+    g.initialize()
+    for bad in ([1e-5] * (len(scs) - 1) + [0.0], [1e-5] * (len(scs) - 1) + [float("nan")]):
+        with pytest.raises(dsph.DsphError) as e:
+            g.set_scene_time_steps(bad)
+        assert e.value.code == dsph.E_INVALID
+    g.set_scene_time_steps([sc.dt for sc in scs])
+    with pytest.raises(dsph.DsphError) as e:
+        g.step(1, scs[0].dt)  # one dt for all while per-scene steps are set
+    assert e.value.code == dsph.E_INVALID
+    g.step(1, 0.0)
+    g.set_scene_time_steps(None)
+    g.step(2, min(sc.dt for sc in scs))
+    one = dsph.Solver(dsph.PREC_MIXED)
+    one.upload(scs[0])
+    with pytest.raises(dsph.DsphError) as e:
+        one.set_scene_time_steps([scs[0].dt])
+    assert e.value.code == dsph.E_INVALID
+
+
 def test_batch_rejects_different_contact_tables():
     scs = batch_scenes()
     scs[1].hertz = np.array(scs[1].hertz, float).copy()
