@@ -66,13 +66,14 @@ def make_cfg(name, threads):
 
 def make_cfg4(threads):
     scs = scenes.ensemble(256)
-    dt = W.ensemble_dt(scs)
-    d = dict(dt=dt, steps=W.CFG4_STEPS, subset=np.asarray(W.CFG4_SUBSET, np.int64))
+    dts = W.ensemble_dts(scs)
+    d = dict(steps=W.CFG4_STEPS, subset=np.asarray(W.CFG4_SUBSET, np.int64))
     for k in W.CFG4_SUBSET:
         sc = scs[k]
         lists, grids, fields, ints = run_oracle(sc, W.CFG4_STEPS, W.CFG4_CHECKPOINTS, W.FLAG_ART_VISCOSITY,
-                                                dt, threads, f"cfg4 scene {k}")
-        sub = W.pack(lists, grids, fields, ints, W.CFG4_CHECKPOINTS, dict(n_fluid=sc.n_fluid, n_solid=sc.n_solid))
+                                                dts[k], threads, f"cfg4 scene {k}")
+        sub = W.pack(lists, grids, fields, ints, W.CFG4_CHECKPOINTS,
+                     dict(n_fluid=sc.n_fluid, n_solid=sc.n_solid, dt=dts[k]))
         for key, v in sub.items():
             d[f"s{k}_{key}"] = v
     W.save("cfg4", d)

@@ -9,9 +9,9 @@ tests/golden/make_fixtures.py from the FP64 CPU oracle).
         clumps of 4, a 40,000-sphere bonded block), no fluid, 5000 steps
   cfg3  precision study, dx = 0.0016: 15,625,000 fluid + 4,281,966 wall +
         20,000 spheres (GPU FP64 against GPU mixed; no oracle fixture)
-  cfg4  256-scene dam-break ensemble, artificial viscosity on, one batch
-        stepping with the smallest time step; the oracle runs a seeded
-        subset of its scenes alone at that time step
+  cfg4  256-scene dam-break ensemble, artificial viscosity on, one batch,
+        each scene stepping with its own time step; the oracle runs a
+        seeded subset of its scenes alone
 
 A fixture holds, per configuration:
   - sha256 digests of the four neighbour / contact lists and of the fluid
@@ -42,8 +42,9 @@ def cfg3_scene():
                                     name="coupled_dam_break_16m")
 
 
-def ensemble_dt(scs):
-    return min(sc.dt for sc in scs)
+def ensemble_dts(scs):
+    """The time step of every scene of the ensemble: its own."""
+    return [sc.dt for sc in scs]
 
 
 CONFIGS = {
