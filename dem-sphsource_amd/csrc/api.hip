@@ -337,12 +337,18 @@ void grow_sup_keep(dsph_ctx* c, int K) {
 
 void alloc_fs(dsph_ctx* c, int64_t cap) {
     dfree(c, c->L.fs_a);
+    dfree(c, c->L.fs_own);
     dfree(c, c->L.fsF);
+    dfree(c, c->L.fsT);
     dfree(c, c->L.fsr_idx);
     c->L.fs_cap = cap;
     c->L.fs_a = dalloc<int>(c, cap);
+    c->L.fs_own = dalloc<int>(c, cap);
     c->L.fsF = dalloc<double4>(c, cap);
+    c->L.fsT = dalloc<double4>(c, cap);
     c->L.fsr_idx = dalloc<int>(c, cap);
+    // owners stay in range until a list is written
+    HIPCHK(hipMemsetAsync(c->L.fs_own, 0, (size_t)cap * sizeof(int), c->st));
 }
 
 // grows the solid–solid arrays, keeping the saved history (offsets are
@@ -477,6 +483,7 @@ std::vector<StateArr> state_arrays(dsph_ctx* c) {
     add(L.fscnt, nf, 4);
     add(L.fsoff, nf + 1, 4);
     add(L.fs_a, (size_t)L.fs_cap, 4);
+    add(L.fs_own, (size_t)L.fs_cap, 4);
     add(L.fsF, (size_t)L.fs_cap, 32);
     add(L.fsr_idx, (size_t)L.fs_cap, 4);
     add(L.fsr_cnt, ns, 4);
@@ -579,6 +586,7 @@ FluidArgs fluid_args(dsph_ctx* c) {
     a.flags = c->flags;
     a.T = c->T;
     a.sdt = c->sdt;
+    a.ns = c->ns;
     return a;
 }
 
@@ -2374,7 +2382,7 @@ struct StateBlobHeader {
     int64_t n_arrays, payload;
 };
 constexpr char kStateMagic[8] = {'D', 'S', 'P', 'H', 'S', 'T', 'A', 'T'};
-constexpr int32_t kStateVersion = 2;
+constexpr int32_t kStateVersion = 3;
 
 StateBlobHeader state_header(dsph_ctx* c, const std::vector<StateArr>& v) {
     StateBlobHeader h{};

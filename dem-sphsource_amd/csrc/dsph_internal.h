@@ -202,7 +202,11 @@ struct Lists {
     int* fscnt = nullptr;
     int* fsoff = nullptr;  // NF + 1
     int* fs_a = nullptr;   // solid index
+    int* fs_own = nullptr; // fluid slot of the pair
     double4* fsF = nullptr;
+    // the pairs' terms of the coming fluid pass (one R, or an R4, per pair),
+    // written by the pair kernel before the pass reads them
+    void* fsT = nullptr;
     // reverse fluid–solid for movable solids: CSR by solid
     int* fsr_cnt = nullptr;
     int* fsr_off = nullptr;  // NS + 1
@@ -447,6 +451,7 @@ struct FluidArgs {
     uint32_t flags;
     Tiles T;  // n > 0: the mixed pair loops run tiled
     const double* sdt = nullptr;  // per-scene time steps (step_of, scenes f.scene), or none
+    int ns = 0;  // solids (0: no fluid–solid pairs)
 };
 void fluid_records(hipStream_t s, const FluidArgs& a, int buf, bool mixed);
 void density_reinit(hipStream_t s, const FluidArgs& a, int i_step, bool mixed);

@@ -40,6 +40,7 @@
 //   the viscous term's (v_i - v_j)/|r| (e . grad W) = F (v_i - v_j);
 //   2 V_i / m_i = 2 / rho_i.
 // Pairs closer than 1e-10 contribute nothing (grad W = 0 there upstream).
+#include <algorithm>
 #include <type_traits>
 #include <utility>
 
@@ -548,10 +549,8 @@ __device__ __forceinline__ void for_row_tile(const FluidArgs& a, const TileLds& 
 // records gathered from HBM; or, in a tiled launch, its local row in LDS,
 // with the global row, walked by the slot alone, for a row walked exactly
 // this step or a tile whose union did not fit.  run<MODE>(fn) as for_row.
-// kFsBatch: fluid-solid pairs per batch of the momentum pass
 template <bool MIX, bool UNI>
 struct GlobalWalk {
-    static constexpr int kFsBatch = 4;
     const FluidArgs& a;
     Rsrc rs;
     int s;
@@ -563,7 +562,6 @@ struct GlobalWalk {
 // U: candidates per batch
 template <bool UNI, int U = kBatch>
 struct TileWalk {
-    static constexpr int kFsBatch = 2;
     const FluidArgs& a;
     Rsrc rs;
     int s;
@@ -678,23 +676,25 @@ __device__ __forceinline__ bool fs_term(const Nb<MIX>& o, typename Prec<MIX>::R 
 
 __device__ __forceinline__ bool fs_valid(const Lists& L, int nf) { return L.fsoff[nf] <= L.fs_cap; }
 
-// Visits slot s's fluid-solid pairs [fsoff[s], fsoff[s+1]) in batches of U:
-// the batch's solid indices, then all their records, are in flight together
-// (entries past the segment end re-read its last pair and are not used).
-// fn(sp, k, ok) must not branch on ok (keep contributions with add_if).
-template <class R, int U, class Fn>
-__device__ __forceinline__ void for_fs(const FluidArgs& a, int s, double px, double py, double pz,
-                                       Fn&& fn) {
+// Fluid–solid pairs are evaluated one thread per pair (k_fs_terms) before
+// each pass, not by the pair loops: pairs crowd at the walls (a wavefront
+// beside a wall holds tens of pairs per lane, one in the bulk none), so a
+// slot walking its own pairs would keep most lanes of the wall wavefronts
+// idle through a chain of dependent gathers.  The pass then adds its slot's
+// stored terms [fsoff[s], fsoff[s+1]) in list order, as the slot would have
+// added them itself; pairs outside the support store zeros.  Terms are read
+// in batches of four (reads past the segment end re-read its last term and
+// are not used); fn(term, ok) must not branch on ok.
+template <class T, class Fn>
+__device__ __forceinline__ void for_fs_terms(const FluidArgs& a, int s, Fn&& fn) {
     const int b = a.L.fsoff[s], e = a.L.fsoff[s + 1];
-    for (int k = b; k < e; k += U) {
-        int ai[U];
+    const T* __restrict__ t = (const T*)a.L.fsT;
+    for (int k = b; k < e; k += 4) {
+        T v[4];
 #pragma unroll
-        for (int u = 0; u < U; u++) ai[u] = a.L.fs_a[min(k + u, e - 1)];
-        SolidP<R> sp[U];
+        for (int u = 0; u < 4; u++) v[u] = t[min(k + u, e - 1)];
 #pragma unroll
-        for (int u = 0; u < U; u++) sp[u] = load_solid<R>(a.so, ai[u], px, py, pz);
-#pragma unroll
-        for (int u = 0; u < U; u++) fn(sp[u], k + u, k + u < e);
+        for (int u = 0; u < 4; u++) fn(v[u], k + u < e);
     }
 }
 
@@ -773,21 +773,7 @@ __device__ __forceinline__ void continuity_slot(const FluidArgs& a, int s, doubl
             acc.add(term);
         });
     const double4 vm = f.vr[s];
-    if (fs_valid(a.L, a.nf)) {
-        const R rho0 = (R)vm.w;
-        for_fs<R, 4>(a, s, ps.x, ps.y, ps.z, [&](const SolidP<R>& sp, int, bool ok) {
-            const R r = dist<R>(sp.rx * sp.rx + sp.ry * sp.ry + sp.rz * sp.rz);
-            const R F = grad_factor<R>(r, R(0.5) * (o.h + R(2.6) * sp.rad));
-            const R d2 = R(2) * sp.rad;
-            const R Va = d2 * d2 * d2;
-            R Pa, rhoa;
-            wall_state<R>(o.P, o.rho, o.c, rho0, sp, (R)a.gx, (R)a.gy, (R)a.gz, Pa, rhoa);
-            const R dvr = (o.vx - sp.vx) * sp.rx + (o.vy - sp.vy) * sp.ry + (o.vz - sp.vz) * sp.rz;
-            acc.add_if(ok && F != R(0), R(2) * o.rho * Va * F *
-                                            (R(2) * rhoa * dvr + (o.P - Pa) * r * rcp_(o.c)) *
-                                            rcp_(o.rho + rhoa));
-        });
-    }
+    if (fs_valid(a.L, a.nf)) for_fs_terms<R>(a, s, [&](R t, bool ok) { acc.add_if(ok, t); });
     const double nd = f.drho[a.cur][s] + dt * acc.get();
     f.drho[1 - a.cur][s] = nd;
     double4 p = ps;
@@ -823,7 +809,7 @@ __global__ void __launch_bounds__(kTileSlots) k_continuity_tile(FluidArgs a, dou
 // buffer with the slot's dRho (other slots still read this one), and the
 // caller flips the current buffer
 template <bool MIX, bool UNI, class Walk>
-__device__ __forceinline__ void momentum_slot(const FluidArgs& a, int s, int write_fs, double kick,
+__device__ __forceinline__ void momentum_slot(const FluidArgs& a, int s, double kick,
                                               const Walk& walk) {
     using R = typename Prec<MIX>::R;
     if (a.sdt) kick *= a.sdt[a.f.scene[s]];  // a factor of the scene's own step
@@ -906,17 +892,9 @@ __device__ __forceinline__ void momentum_slot(const FluidArgs& a, int s, int wri
     };
     if (art) rows(std::true_type{});
     else rows(std::false_type{});
-    if (fs_valid(a.L, a.nf)) {
-        const double4 vm = f.vr[s];
-        const double h13 = ps.w / 1.3;
-        const double mi = h13 * h13 * h13 * vm.w;
-        for_fs<R, Walk::kFsBatch>(a, s, ps.x, ps.y, ps.z, [&](const SolidP<R>& sp, int k, bool ok) {
-            R tx, ty, tz;
-            const bool on = fs_term<MIX>(o, (R)vm.w, sp, (R)a.gx, (R)a.gy, (R)a.gz, tx, ty, tz);
-            acc.add_if(ok && on, tx, ty, tz);
-            if (write_fs && ok) a.L.fsF[k] = make_double4(mi * tx, mi * ty, mi * tz, 0.0);
-        });
-    }
+    if (fs_valid(a.L, a.nf))
+        for_fs_terms<typename Prec<MIX>::R4>(
+            a, s, [&](const typename Prec<MIX>::R4& t, bool ok) { acc.add_if(ok, t.x, t.y, t.z); });
     const double4 am = make_double4(acc.x + a.gx, acc.y + a.gy, acc.z + a.gz, 0.0);
     f.acc[s] = am;
     if (kick != 0.0) {
@@ -937,41 +915,88 @@ __device__ __forceinline__ void momentum_slot(const FluidArgs& a, int s, int wri
     }
 }
 template <bool MIX, bool UNI>
-__global__ void __launch_bounds__(kMaxBS) k_momentum(FluidArgs a, int write_fs, double kick) {
+__global__ void __launch_bounds__(kMaxBS) k_momentum(FluidArgs a, double kick) {
     const int s = xcd_block() * blockDim.x + threadIdx.x;
     if (s >= a.nf) return;
-    momentum_slot<MIX, UNI>(a, s, write_fs, kick, GlobalWalk<MIX, UNI>{a, rec_rsrc<MIX>(a.f, a.cur, a.nf), s});
+    momentum_slot<MIX, UNI>(a, s, kick, GlobalWalk<MIX, UNI>{a, rec_rsrc<MIX>(a.f, a.cur, a.nf), s});
 }
 template <bool UNI>
-__global__ void __launch_bounds__(kTileSlots) k_momentum_tile(FluidArgs a, int write_fs, double kick) {
+__global__ void __launch_bounds__(kTileSlots) k_momentum_tile(FluidArgs a, double kick) {
     __shared__ TileLds lrec;
     const int b = xcd_block();
     const int m = stage_tile<true>(a, b, lrec);
     const int s = tile_slot(a, b);
     if (s < 0) return;
     const bool tiled = m >= 0 && a.L.cnt[s] < 0;
-    momentum_slot<true, UNI>(a, s, write_fs, kick,
+    momentum_slot<true, UNI>(a, s, kick,
                              TileWalk<UNI>{a, rec_rsrc<true>(a.f, a.cur, a.nf), s, lrec, tiled});
 }
 
-// calFluid2SolidForce (integrate.cu:282-330) on steps without a fluid update
-template <bool MIX>
-__global__ void __launch_bounds__(kMaxBS) k_fs_force(FluidArgs a) {
+// --------------------------------------------------- fluid–solid pair terms
+// One thread per pair k of the fluid–solid list (grid-stride over the
+// list's device-side length): the pair's term of the coming pass from its
+// fluid slot fs_own[k] and solid fs_a[k], stored in fsT[k]:
+//   FS_CONT     the continuity term (integrate.cu:103-161), one R;
+//   FS_MOM      the momentum term per unit fluid mass (integrate.cu:214-280,
+//               calFluid2SolidForce :282-330) as (x, y, z, 0), and with
+//               write_fs the pair's force on the solid, m_i times it, in fsF;
+//   FS_SHEPARD  the solid's part of the Shepard sum (integrate.cu:3-53).
+// The slot's state is that of the pass the terms are for (the same record
+// buffer cur, the FP64 master position before the pass moves it).
+enum FsMode { FS_CONT = 0, FS_MOM = 1, FS_SHEPARD = 2 };
+constexpr int kFsTermBlocks = 2048;  // grid cap: 8 workgroups per CU
+
+template <bool MIX, bool UNI, int MODE>
+__global__ void __launch_bounds__(kBS) k_fs_terms(FluidArgs a, int write_fs) {
     using R = typename Prec<MIX>::R;
-    const int s = xcd_block() * blockDim.x + threadIdx.x;
-    if (s >= a.nf || !fs_valid(a.L, a.nf)) return;
-    const int b0 = a.L.fsoff[s], e = a.L.fsoff[s + 1];
-    if (b0 == e) return;
-    const Nb<MIX> o = load_nb<MIX>(a, a.cur, s);
-    const double4 ps = a.f.ph[s];
-    const double4 vm = a.f.vr[s];
-    const double h13 = ps.w / 1.3;
-    const double mi = h13 * h13 * h13 * vm.w;
-    for (int k = b0; k < e; k++) {
-        const SolidP<R> sp = load_solid<R>(a.so, a.L.fs_a[k], ps.x, ps.y, ps.z);
-        R tx = 0, ty = 0, tz = 0;
-        fs_term<MIX>(o, (R)vm.w, sp, (R)a.gx, (R)a.gy, (R)a.gz, tx, ty, tz);
-        a.L.fsF[k] = make_double4(mi * tx, mi * ty, mi * tz, 0.0);
+    using R4 = typename Prec<MIX>::R4;
+    if (!fs_valid(a.L, a.nf)) return;
+    const FluidSet& f = a.f;
+    const int64_t tot = a.L.fsoff[a.nf];
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < tot; k += stride) {
+        const int s = a.L.fs_own[k];
+        const int ai = a.L.fs_a[k];
+        const double4 ps = f.ph[s];
+        if constexpr (MODE == FS_SHEPARD) {
+            const R h = (R)ps.w;
+            const R ih = rcp_(h);
+            const R sigma = R(kWC) * ih * ih * ih;
+            const double h13 = ps.w / 1.3;
+            const R invV0 = rcp_((R)(h13 * h13 * h13));
+            const double4 q = a.so.pr[ai];
+            const R rx = (R)(ps.x - q.x), ry = (R)(ps.y - q.y), rz = (R)(ps.z - q.z);
+            const R qq = dist<R>(rx * rx + ry * ry + rz * rz) * ih;
+            const R t = R(1) - R(0.5) * qq;
+            const R t2 = t * t;
+            const R d = R(2) * (R)q.w;
+            ((R*)a.L.fsT)[k] = qq < R(2) ? sigma * t2 * t2 * (R(1) + R(2) * qq) * (d * d * d) * invV0 : R(0);
+        } else {
+            const Nb<MIX> o = load_nb<MIX, UNI>(a, a.cur, s);
+            const double4 vm = f.vr[s];
+            const SolidP<R> sp = load_solid<R>(a.so, ai, ps.x, ps.y, ps.z);
+            if constexpr (MODE == FS_CONT) {
+                const R r = dist<R>(sp.rx * sp.rx + sp.ry * sp.ry + sp.rz * sp.rz);
+                const R F = grad_factor<R>(r, R(0.5) * (o.h + R(2.6) * sp.rad));
+                const R d2 = R(2) * sp.rad;
+                const R Va = d2 * d2 * d2;
+                R Pa, rhoa;
+                wall_state<R>(o.P, o.rho, o.c, (R)vm.w, sp, (R)a.gx, (R)a.gy, (R)a.gz, Pa, rhoa);
+                const R dvr = (o.vx - sp.vx) * sp.rx + (o.vy - sp.vy) * sp.ry + (o.vz - sp.vz) * sp.rz;
+                const R t = R(2) * o.rho * Va * F * (R(2) * rhoa * dvr + (o.P - Pa) * r * rcp_(o.c)) *
+                            rcp_(o.rho + rhoa);
+                ((R*)a.L.fsT)[k] = F != R(0) ? t : R(0);
+            } else {
+                R tx, ty, tz;
+                fs_term<MIX>(o, (R)vm.w, sp, (R)a.gx, (R)a.gy, (R)a.gz, tx, ty, tz);
+                ((R4*)a.L.fsT)[k] = R4{tx, ty, tz, R(0)};
+                if (write_fs) {
+                    const double h13 = ps.w / 1.3;
+                    const double mi = h13 * h13 * h13 * vm.w;
+                    a.L.fsF[k] = make_double4(mi * tx, mi * ty, mi * tz, 0.0);
+                }
+            }
+        }
     }
 }
 
@@ -998,21 +1023,7 @@ __device__ __forceinline__ void reinit_slot(const FluidArgs& a, int s, int i_ste
             const R t2 = t * t;
             sw.add(sigma * t2 * t2 * (R(1) + R(2) * qq));
         });
-    if (fs_valid(a.L, a.nf)) {
-        const double h13 = p.w / 1.3;
-        const R invV0 = rcp_((R)(h13 * h13 * h13));
-        const int e = a.L.fsoff[s + 1];
-        for (int k = a.L.fsoff[s]; k < e; k++) {
-            const double4 q = a.so.pr[a.L.fs_a[k]];
-            const R rx = (R)(p.x - q.x), ry = (R)(p.y - q.y), rz = (R)(p.z - q.z);
-            const R qq = dist<R>(rx * rx + ry * ry + rz * rz) * ih;
-            if (qq >= R(2)) continue;
-            const R t = R(1) - R(0.5) * qq;
-            const R t2 = t * t;
-            const R d = R(2) * (R)q.w;
-            sw.add(sigma * t2 * t2 * (R(1) + R(2) * qq) * (d * d * d) * invV0);
-        }
-    }
+    if (fs_valid(a.L, a.nf)) for_fs_terms<R>(a, s, [&](R t, bool ok) { sw.add_if(ok, t); });
     const double sumw = (double)sw.get();
     if (i_step == 1) {
         f.sumw0[s] = sumw;
@@ -1057,8 +1068,24 @@ void fluid_records(hipStream_t s, const FluidArgs& a, int buf, bool mixed) {
     if (a.nf > 0) DSPH_LAUNCH(k_records, a.nf, s, a.f, a.nf, buf);
 }
 
+// the fluid–solid pair terms of the coming pass (none without solids)
+template <int MODE>
+void fs_terms(hipStream_t s, const FluidArgs& a, bool mixed, int write_fs) {
+    if (a.nf <= 0 || a.ns <= 0 || a.L.fs_cap <= 0) return;
+    const dim3 g((unsigned)std::min<int64_t>((a.L.fs_cap + kBS - 1) / kBS, kFsTermBlocks)), b(kBS);
+    if (mixed && a.uni)
+        hipLaunchKernelGGL((k_fs_terms<true, true, MODE>), g, b, 0, s, a, write_fs);
+    else if (mixed)
+        hipLaunchKernelGGL((k_fs_terms<true, false, MODE>), g, b, 0, s, a, write_fs);
+    else if (a.uni)
+        hipLaunchKernelGGL((k_fs_terms<false, true, MODE>), g, b, 0, s, a, write_fs);
+    else
+        hipLaunchKernelGGL((k_fs_terms<false, false, MODE>), g, b, 0, s, a, write_fs);
+}
+
 void density_reinit(hipStream_t s, const FluidArgs& a, int i_step, bool mixed) {
     if (a.nf <= 0) return;
+    fs_terms<FS_SHEPARD>(s, a, mixed, 0);
     if (mixed && a.T.n > 0)
         hipLaunchKernelGGL(k_density_reinit_tile, dim3(a.T.n), dim3(kTileSlots), 0, s, a, i_step);
     else
@@ -1089,6 +1116,7 @@ void density_reinit(hipStream_t s, const FluidArgs& a, int i_step, bool mixed) {
 
 void continuity(hipStream_t s, const FluidArgs& a, double dt, bool mixed, double drift) {
     if (a.nf <= 0) return;
+    fs_terms<FS_CONT>(s, a, mixed, 0);
     if (mixed && a.T.n > 0)
         DSPH_LAUNCH_TILE(k_continuity_tile, s, a, dt, drift);
     else
@@ -1098,15 +1126,16 @@ void continuity(hipStream_t s, const FluidArgs& a, double dt, bool mixed, double
 
 void momentum(hipStream_t s, const FluidArgs& a, bool mixed, bool write_fs_force, double kick) {
     if (a.nf <= 0) return;
+    fs_terms<FS_MOM>(s, a, mixed, (int)write_fs_force);
     if (mixed && a.T.n > 0)
-        DSPH_LAUNCH_TILE(k_momentum_tile, s, a, (int)write_fs_force, kick);
+        DSPH_LAUNCH_TILE(k_momentum_tile, s, a, kick);
     else
-        DSPH_LAUNCH_UNI(k_momentum, a.nf, a.bs, s, a, (int)write_fs_force, kick);
+        DSPH_LAUNCH_UNI(k_momentum, a.nf, a.bs, s, a, kick);
 }
 
-void fluid_solid_force(hipStream_t s, const FluidArgs& a, bool mixed) {
-    if (a.nf > 0) DSPH_LAUNCH_BS(k_fs_force, a.nf, a.bs, s, a);
-}
+// calFluid2SolidForce (integrate.cu:282-330) on steps without a fluid update:
+// the pair forces alone
+void fluid_solid_force(hipStream_t s, const FluidArgs& a, bool mixed) { fs_terms<FS_MOM>(s, a, mixed, 1); }
 
 
 }  // namespace dsph

@@ -791,6 +791,7 @@ __device__ void lane_walk(const SearchArgs& a, const double4 p, int s, int& c, i
                     if (FILL) {
                         if (w < a.L.fs_cap) {
                             a.L.fs_a[w] = ai;
+                            a.L.fs_own[w] = s;
                             a.L.fsF[w] = make_double4(0, 0, 0, 0);
                         }
                         w++;
@@ -875,13 +876,15 @@ __global__ void __launch_bounds__(kBS) k_filter_count(SearchArgs a) {
 
 // Pairs of one wavefront's slots are contiguous in the list: a wavefront
 // whose pairs fit kFillBuf gathers them in LDS (each lane its own members
-// from its mask) and then stores them with consecutive lanes on consecutive
-// entries; the others, and walked rows, store lane by lane.
+// from its mask, with its lane number as the pair's owner) and then stores
+// them with consecutive lanes on consecutive entries; the others, and
+// walked rows, store lane by lane.  Each pair's fluid slot goes to fs_own.
 constexpr int kFillBuf = 1024;
 __global__ void __launch_bounds__(kBS) k_filter_fill(SearchArgs a, int zero_fs) {
     __shared__ int lds[kBS / 64];
     __shared__ long long part[kBS / 64];
     __shared__ int buf[kBS / 64][kFillBuf];
+    __shared__ unsigned char obuf[kBS / 64][kFillBuf];
     const int tile = xcd_block();
     const int s = tile * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
@@ -919,6 +922,7 @@ __global__ void __launch_bounds__(kBS) k_filter_fill(SearchArgs a, int zero_fs) 
         lane_walk<true>(a, p, s, c2, cs2, w);
     }
     int* const wb = buf[threadIdx.x >> 6];
+    unsigned char* const ob = obuf[threadIdx.x >> 6];
     if (n2 > 0) {
         const int* __restrict__ fsup = a.L.fsup + s;
         const unsigned long long m0 = a.L.fsm[2 * (int64_t)s], m1 = a.L.fsm[2 * (int64_t)s + 1];
@@ -933,8 +937,13 @@ __global__ void __launch_bounds__(kBS) k_filter_fill(SearchArgs a, int zero_fs) 
                 const int kk = k + u;
                 const bool in = kk < k0 && (((kk < 64 ? m0 : m1) >> (kk & 63)) & 1ull);
                 if (in) {
-                    if (staged) wb[w - w0] = ai[u];
-                    else if (w < a.L.fs_cap) a.L.fs_a[w] = ai[u];
+                    if (staged) {
+                        wb[w - w0] = ai[u];
+                        ob[w - w0] = (unsigned char)lane;
+                    } else if (w < a.L.fs_cap) {
+                        a.L.fs_a[w] = ai[u];
+                        a.L.fs_own[w] = s;
+                    }
                 }
                 w += in;
             }
@@ -947,7 +956,10 @@ __global__ void __launch_bounds__(kBS) k_filter_fill(SearchArgs a, int zero_fs) 
                 const int ai = fsup[(int64_t)k * a.nf];
                 const double4 q = a.spr[ai];
                 if (within(p.x, p.y, p.z, q.x, q.y, q.z, 2.0 * p.w)) {
-                    if (w < a.L.fs_cap) a.L.fs_a[w] = ai;
+                    if (w < a.L.fs_cap) {
+                        a.L.fs_a[w] = ai;
+                        a.L.fs_own[w] = s;
+                    }
                     w++;
                 }
             }
@@ -959,7 +971,10 @@ __global__ void __launch_bounds__(kBS) k_filter_fill(SearchArgs a, int zero_fs) 
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         for (int i = lane; i < T; i += 64)
-            if (w0 + i < a.L.fs_cap) a.L.fs_a[w0 + i] = wb[i];
+            if (w0 + i < a.L.fs_cap) {
+                a.L.fs_a[w0 + i] = wb[i];
+                a.L.fs_own[w0 + i] = s - lane + ob[i];
+            }
     }
     // new pairs start with zero force where asked
     if (zero_fs && T > 0)
