@@ -256,6 +256,19 @@ __device__ __forceinline__ void write_records(const FluidSet& f, int buf, int s,
     *state_w<MIX>(f, buf, s) = {(R)vx, (R)vy, (R)vz, (R)drho};
 }
 
+// the records of slot s from its records o of the other buffer with a new
+// dRho: the position quad as it is, the velocity as it is (o holds the
+// record's own FP32 / FP64 values)
+template <bool MIX>
+__device__ __forceinline__ void renew_records(const FluidSet& f, int buf, int s, const Nb<MIX>& o, double drho) {
+    using R = typename Prec<MIX>::R;
+    if constexpr (MIX)
+        *(uint4*)pos_w<MIX>(f, buf, s) = make_uint4(o.px, o.py, o.pz, (unsigned)o.id);
+    else
+        *pos_w<MIX>(f, buf, s) = make_double4(o.px, o.py, o.pz, (double)o.id);
+    *state_w<MIX>(f, buf, s) = {o.vx, o.vy, o.vz, (R)drho};
+}
+
 // the neighbour row of slot s: the exact walked row or the Verlet superset;
 // both are ELL column-major with stride nf
 struct Row {
@@ -501,7 +514,10 @@ __device__ __forceinline__ void load_tile_rec(const TileLds& L, int j, Nb<true>&
     }
 }
 // quads of a local row: four entries in one 8-byte load, kept two batches
-// ahead of use (the rows stream from HBM)
+// ahead of use (the rows stream from HBM).  The loop takes two batches per
+// iteration, each on a quad register of its own that is reloaded right
+// after its batch: a single register pair shifted down every batch would
+// wait on the newest load at the top of each batch, one batch ahead only.
 typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ void quad_entries(u32x2 v, int* j) {
     j[0] = (int)(v.x & 0xffffu);
@@ -509,23 +525,22 @@ __device__ __forceinline__ void quad_entries(u32x2 v, int* j) {
     j[2] = (int)(v.y & 0xffffu);
     j[3] = (int)(v.y >> 16);
 }
+__device__ __forceinline__ u32x2 apart2(u32x2 v) {
+    v.x = (unsigned)apart((int)v.x);
+    v.y = (unsigned)apart((int)v.y);
+    return v;
+}
+static_assert(kRowPad >= 15, "tile rows need three quads read ahead past the last");
 template <int U, NbMode MODE, bool UNI, class Fn>
 __device__ __forceinline__ void for_row_tile(const FluidArgs& a, const TileLds& lrec, int s, Fn&& fn) {
     static_assert(U == 4, "a batch is one quad of the local row");
     const int n = a.T.rcnt[s];
     const int64_t st = a.nf;
     const u32x2* ip = (const u32x2*)a.T.row + s;
-    u32x2 q0 = __builtin_nontemporal_load(ip), q1 = __builtin_nontemporal_load(ip + st);
-    q0.x = (unsigned)apart((int)q0.x);
-    q0.y = (unsigned)apart((int)q0.y);
-    q1.x = (unsigned)apart((int)q1.x);
-    q1.y = (unsigned)apart((int)q1.y);
-    for (int k = 0; k < n; k += U) {
+    u32x2 qa = apart2(__builtin_nontemporal_load(ip)), qb = apart2(__builtin_nontemporal_load(ip + st));
+    const auto batch = [&](u32x2 qd, int k) {
         int jn[U];
-        quad_entries(q0, jn);
-        q0 = q1;
-        q1 = __builtin_nontemporal_load(ip + 2 * st);
-        ip += st;
+        quad_entries(qd, jn);
         Nb<true> q[U];
         bool ok[U];
 #pragma unroll
@@ -542,6 +557,15 @@ __device__ __forceinline__ void for_row_tile(const FluidArgs& a, const TileLds& 
         }
 #pragma unroll
         for (int u = 0; u < U; u++) fn(q[u], ok[u]);
+    };
+    // quads up to three past the batch pair read ahead: the row carries zero
+    // entries to the end of the third quad after its last
+    for (int k = 0; k < n; k += 2 * U) {
+        batch(qa, k);
+        qa = __builtin_nontemporal_load(ip + 2 * st);
+        if (k + U < n) batch(qb, k + U);
+        qb = __builtin_nontemporal_load(ip + 3 * st);
+        ip += 2 * st;
     }
 }
 
@@ -735,7 +759,6 @@ __device__ __forceinline__ void continuity_slot(const FluidArgs& a, int s, doubl
     }
     const FluidSet& f = a.f;
     const Nb<MIX> o = load_nb<MIX, UNI>(a, a.cur, s);
-    const double4 ps = f.ph[s];
     const R u = pos_unit<MIX>(f);
     const R r2min = R(1e-20) / (u * u);
     // UNI: term = K t^3 [rho_j dvr_u + (c dRho_i - c dRho_j) r_u] / (rho_j (rho_i + rho_j))
@@ -772,18 +795,22 @@ __device__ __forceinline__ void continuity_slot(const FluidArgs& a, int s, doubl
             }
             acc.add(term);
         });
-    const double4 vm = f.vr[s];
     if (fs_valid(a.L, a.nf)) for_fs_terms<R>(a, s, [&](R t, bool ok) { acc.add_if(ok, t); });
     const double nd = f.drho[a.cur][s] + dt * acc.get();
     f.drho[1 - a.cur][s] = nd;
-    double4 p = ps;
     if (drift != 0.0) {
+        const double4 vm = f.vr[s];
+        double4 p = f.ph[s];
         p.x += drift * vm.x;
         p.y += drift * vm.y;
         p.z += drift * vm.z;
         f.ph[s] = p;
+        write_records<MIX>(f, 1 - a.cur, s, p, vm.x, vm.y, vm.z, nd);
+    } else {
+        // position and velocity are those the slot's records in buffer cur
+        // were written from: the new records differ in dRho only
+        renew_records<MIX>(f, 1 - a.cur, s, o, nd);
     }
-    write_records<MIX>(f, 1 - a.cur, s, p, vm.x, vm.y, vm.z, nd);
 }
 template <bool MIX, bool UNI>
 __global__ void __launch_bounds__(kMaxBS) k_continuity(FluidArgs a, double dt, double drift) {
@@ -897,22 +924,24 @@ __device__ __forceinline__ void momentum_slot(const FluidArgs& a, int s, double 
             a, s, [&](const typename Prec<MIX>::R4& t, bool ok) { acc.add_if(ok, t.x, t.y, t.z); });
     const double4 am = make_double4(acc.x + a.gx, acc.y + a.gy, acc.z + a.gz, 0.0);
     f.acc[s] = am;
+    // the other buffer gets the slot's records whatever the kick (the caller
+    // flips the current buffer)
+    double4 v = f.vr[s];
+    double4 p = ps;
     if (kick != 0.0) {
-        double4 v = f.vr[s];
         v.x += kick * am.x;
         v.y += kick * am.y;
         v.z += kick * am.z;
         f.vr[s] = v;
-        double4 p = ps;
         const double hd = 0.5 * kick;
         p.x += hd * v.x;
         p.y += hd * v.y;
         p.z += hd * v.z;
         f.ph[s] = p;
-        const double d = f.drho[a.cur][s];
-        f.drho[1 - a.cur][s] = d;
-        write_records<MIX>(f, 1 - a.cur, s, p, v.x, v.y, v.z, d);
     }
+    const double d = f.drho[a.cur][s];
+    f.drho[1 - a.cur][s] = d;
+    write_records<MIX>(f, 1 - a.cur, s, p, v.x, v.y, v.z, d);
 }
 template <bool MIX, bool UNI>
 __global__ void __launch_bounds__(kMaxBS) k_momentum(FluidArgs a, double kick) {

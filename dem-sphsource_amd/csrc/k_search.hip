@@ -1578,9 +1578,9 @@ __global__ void __launch_bounds__(kTileSlots) k_tile_build(SearchArgs a) {
                 w++;
             }
         }
-        // 4. the read-ahead entries, to the end of the second quad after the
-        // row's last (w + 11 <= supK + kRowPad)
-        for (int c = w; c < ((w + 3) & ~3) + 8; c++) T.row[tile_row_at(c, s, a.nf)] = 0;
+        // 4. the read-ahead entries, to the end of the third quad after the
+        // row's last (w + 15 <= supK + kRowPad)
+        for (int c = w; c < ((w + 3) & ~3) + 12; c++) T.row[tile_row_at(c, s, a.nf)] = 0;
         T.rcnt[s] = w;
     }
     __syncthreads();
