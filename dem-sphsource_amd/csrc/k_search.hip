@@ -879,7 +879,10 @@ __global__ void __launch_bounds__(kBS) k_filter_count(SearchArgs a) {
 // from its mask, with its lane number as the pair's owner) and then stores
 // them with consecutive lanes on consecutive entries; the others, and
 // walked rows, store lane by lane.  Each pair's fluid slot goes to fs_own.
-constexpr int kFillBuf = 1024;
+// 2048 pairs per wavefront holds the wavefronts beside a wall (a floor
+// wavefront has about 20 pairs per lane); the buffer sets the kernel's
+// occupancy (40 KiB per workgroup: 1024 pairs, 3072 and 4096 were slower).
+constexpr int kFillBuf = 2048;
 __global__ void __launch_bounds__(kBS) k_filter_fill(SearchArgs a, int zero_fs) {
     __shared__ int lds[kBS / 64];
     __shared__ long long part[kBS / 64];
