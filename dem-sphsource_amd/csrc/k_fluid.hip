@@ -5,8 +5,9 @@
 // solveMomentumConservationEquation (:163-280), calFluid2SolidForce
 // (:282-330), fluidVelocityIntegrate / fluidPositionIntegrate (:367-383).
 //
-// One thread per fluid particle in slot order walks its neighbour row and its
-// fluid–solid segment; there is no per-pair scratch and no atomics.  A row is
+// One thread per fluid particle in slot order walks its neighbour row, then
+// adds its fluid–solid terms, which a kernel with one thread per pair stored
+// just before (k_fs_terms); there are no atomics.  A row is
 // the particle's Verlet superset (all particles within 2h + skin at the last
 // rebuild, both directions): candidates outside the kernel support give
 // grad W = 0 and are skipped, so the contributing pairs and their order are
