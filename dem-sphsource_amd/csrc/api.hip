@@ -787,15 +787,18 @@ void neighbor_search(dsph_ctx* c, int i_step, int gap, bool save_history, bool z
         }
         FamTimer t(c, F_FSLIST);
         fluid_filter(s, sa, zero_fs);
-        if (c->ns > 0)
-            reverse_movable(s, c->L.fs_a, c->L.fsoff + c->nf, c->L.fs_cap, c->so.vel, c->so, c->ns, c->L,
-                            c->scan_tmp);
     }
     if (c->ns > 0) {
         FamTimer t(c, F_SSEARCH);
         if (save_history) solid_save_history(s, c->so, c->L, c->ns);
         solid_search(s, grids_view(c), c->so, c->ns, c->acs, c->ace, c->aperm, c->L, c->scan_tmp, stw);
-        reverse_solid(s, c->so, c->L, c->ns, c->scan_tmp);
+        // the reverse lists of the contacts and, on fluid steps, of the new
+        // fluid–solid pairs (by movable solid), built together
+        if (fluid_step)
+            reverse_both(s, c->L.fs_a, c->L.fsoff + c->nf, c->L.fs_cap, c->so.vel, c->so, c->L, c->ns,
+                         c->scan_tmp);
+        else
+            reverse_solid(s, c->so, c->L, c->ns, c->scan_tmp);
     }
 }
 
@@ -2084,6 +2087,8 @@ int dsph_upload_batch(dsph_ctx* c, const dsph_scene* scenes, int32_t n_scenes) {
         L.ssr_off = dalloc<int>(c, (size_t)ns + 1);
         L.rev_tmp = dalloc<int>(c, ns);
         L.rev_cur = dalloc<int>(c, ns);
+        L.rev_tmp2 = dalloc<int>(c, ns);
+        L.rev_cur2 = dalloc<int>(c, ns);
         fill_int(c->st, L.ssoff, (int64_t)ns + 1, 0);
         fill_int(c->st, L.sscnt, ns, 0);  // dummies are never searched: their count stays 0
         fill_int(c->st, L.hoff, (int64_t)ns + 1, 0);
@@ -2091,6 +2096,7 @@ int dsph_upload_batch(dsph_ctx* c, const dsph_scene* scenes, int32_t n_scenes) {
         fill_int(c->st, L.ssr_cnt, ns, 0);
         fill_int(c->st, L.ssr_off, (int64_t)ns + 1, 0);
         fill_int(c->st, L.rev_tmp, ns, 0);  // zero between reverse builds
+        fill_int(c->st, L.rev_tmp2, ns, 0);
         alloc_ss(c, std::max<int64_t>(4096, 6LL * (int64_t)active.size() + 4096), false);
 
         // ---- scratch

@@ -247,6 +247,8 @@ struct Lists {
     // reverse-CSR scratch: per-solid counts (zero between builds) and cursors
     int* rev_tmp = nullptr;
     int* rev_cur = nullptr;
+    int* rev_tmp2 = nullptr;  // the second list of a fused build (reverse_both)
+    int* rev_cur2 = nullptr;
     int* ssr_cnt = nullptr;
     int* ssr_off = nullptr;
     int* ssr_idx = nullptr;
@@ -429,6 +431,10 @@ void active_grid(hipStream_t s, const Solids& so, const unsigned* skey, int ncel
 void solid_search(hipStream_t s, const Grids& G, const Solids& so, int ns, const int* acs,
                   const int* ace, const int* aperm, Lists& L, int* scan_tmp, long long* status);
 void reverse_solid(hipStream_t s, const Solids& so, Lists& L, int ns, int* scan_tmp);
+// both of the above in one chain of four launches (the fluid–solid list as
+// reverse_movable takes it)
+void reverse_both(hipStream_t s, const int* pair_key, const int* pair_total, int64_t cap,
+                  const double4* key_vel, const Solids& so, Lists& L, int ns, int* scan_tmp);
 // wall normals with the smoothing length of each solid's scene (grid hmax)
 void solid_normals(hipStream_t s, const Grids& G, const Solids& so, int ns, const int* scs,
                    const int* sce, const int* sperm);

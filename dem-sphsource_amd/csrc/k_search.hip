@@ -1065,6 +1065,110 @@ __global__ void k_rev_sort(const int* __restrict__ keys, int n, const int* __res
         idx[k + 1] = v;
     }
 }
+// The reverse CSRs of two independent pair lists built together (the
+// fluid–solid pairs by movable solid and the solid–solid contacts by
+// partner): each kernel of the chain serves both lists, so the chain costs
+// four launches instead of eight.  Each list keeps its own counts and
+// cursors; its keys (keys[0, nkeys)) are the ones that can have pairs, in
+// the keyed (sparse) form of the single-list build, with the same results.
+struct RevList {
+    const int* key;          // pair -> key
+    const int* total;        // device-side pair count
+    int64_t cap;             // pairs stored (0: the list is left out)
+    const double4* kvel;     // given: only keys with kvel[k].w > 0
+    const int* keys;
+    int nkeys;
+    int n_all;               // keys in all (off[n_all] = the total)
+    int* cnt;
+    int* off;
+    int* idx;
+    int* tmp;                // zero on entry and on exit
+    int* cursor;
+};
+__device__ __forceinline__ int64_t rev_total(const RevList& R) {
+    const int64_t t = *R.total;
+    return t > R.cap ? R.cap : t;
+}
+__device__ __forceinline__ void rev_count_one(const RevList R) {
+    if (R.cap <= 0) return;
+    const int64_t tot = rev_total(R);
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < tot;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        const int a = R.key[p];
+        if (R.kvel && !(R.kvel[a].w > 0.0)) continue;
+        atomicAdd(&R.tmp[a], 1);
+    }
+}
+__global__ void k_rev_count2(RevList A, RevList B) {
+    rev_count_one(A);
+    rev_count_one(B);
+}
+// one workgroup per list: counts move from tmp to cnt (tmp left zero),
+// off and cursor the exclusive scan over the list's keys
+constexpr int kRevScanBS = 1024;
+__global__ void __launch_bounds__(kRevScanBS) k_rev_scan2(RevList A, RevList B) {
+    __shared__ int lds[kRevScanBS / 64];
+    __shared__ int total;
+    const RevList R = blockIdx.x == 0 ? A : B;
+    if (R.cap <= 0) return;
+    int carry = 0;
+    for (int base = 0; base < R.nkeys; base += kRevScanBS) {
+        const int r = base + (int)threadIdx.x;
+        int key = -1, v = 0;
+        if (r < R.nkeys) {
+            key = R.keys[r];
+            v = R.tmp[key];
+        }
+        const int inc = block_scan_int(v, lds);
+        const int excl = carry + inc - v;
+        if (r < R.nkeys) {
+            R.off[key] = excl;
+            R.cnt[key] = v;
+            R.tmp[key] = 0;
+            R.cursor[key] = excl;
+        }
+        if (threadIdx.x == kRevScanBS - 1) total = inc;
+        __syncthreads();
+        carry += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) R.off[R.n_all] = carry;
+}
+__device__ __forceinline__ void rev_fill_one(const RevList R) {
+    if (R.cap <= 0) return;
+    const int64_t tot = rev_total(R);
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < tot;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        const int a = R.key[p];
+        if (R.kvel && !(R.kvel[a].w > 0.0)) continue;
+        const int slot = atomicAdd(&R.cursor[a], 1);
+        if (slot >= 0 && slot < R.cap) R.idx[slot] = (int)p;
+    }
+}
+__global__ void k_rev_fill2(RevList A, RevList B) {
+    rev_fill_one(A);
+    rev_fill_one(B);
+}
+// each listed key's segment in ascending pair order (keys of A, then of B)
+__global__ void k_rev_sort2(RevList A, RevList B) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int na = A.cap > 0 ? A.nkeys : 0, nb = B.cap > 0 ? B.nkeys : 0;
+    if (t >= na + nb) return;
+    const RevList R = t < na ? A : B;
+    if (t >= na) t -= na;
+    const int a = R.keys[t];
+    const int b = R.off[a], e = b + R.cnt[a];
+    for (int i = b + 1; i < e; i++) {  // insertion sort: segments are short
+        const int v = R.idx[i];
+        int k = i - 1;
+        while (k >= b && R.idx[k] > v) {
+            R.idx[k + 1] = R.idx[k];
+            k--;
+        }
+        R.idx[k + 1] = v;
+    }
+}
+
 // dense variant of the keyed scan's bookkeeping: counts move from tmp to
 // cnt (tmp left zero) and cursor = off
 __global__ void k_rev_settle(int n, int* __restrict__ tmp, int* __restrict__ cnt, const int* __restrict__ off,
@@ -1695,6 +1799,26 @@ void reverse_solid(hipStream_t s, const Solids& so, Lists& L, int ns, int* scan_
     // partners are active solids
     reverse_build(s, L.ss_j, L.ssoff + ns, L.ss_cap, nullptr, so.active, so.nactive, ns, L.ssr_cnt, L.ssr_off,
                   L.ssr_idx, L.rev_tmp, L.rev_cur, scan_tmp);
+}
+
+void reverse_both(hipStream_t s, const int* pair_key, const int* pair_total, int64_t cap,
+                  const double4* key_vel, const Solids& so, Lists& L, int ns, int* scan_tmp) {
+    if (ns <= 0) return;
+    if (so.nlive > kKeyScanMax || so.nactive > kKeyScanMax) {
+        reverse_movable(s, pair_key, pair_total, cap, key_vel, so, ns, L, scan_tmp);
+        reverse_solid(s, so, L, ns, scan_tmp);
+        return;
+    }
+    const RevList A{pair_key, pair_total, cap, key_vel, so.live, so.nlive, ns,
+                    L.fsr_cnt, L.fsr_off, L.fsr_idx, L.rev_tmp2, L.rev_cur2};
+    const RevList B{L.ss_j, L.ssoff + ns, L.ss_cap, nullptr, so.active, so.nactive, ns,
+                    L.ssr_cnt, L.ssr_off, L.ssr_idx, L.rev_tmp, L.rev_cur};
+    const int gb = cap_grid(cap > L.ss_cap ? cap : L.ss_cap, 2048);
+    hipLaunchKernelGGL(k_rev_count2, dim3(gb), dim3(kBS), 0, s, A, B);
+    hipLaunchKernelGGL(k_rev_scan2, dim3(2), dim3(kRevScanBS), 0, s, A, B);
+    hipLaunchKernelGGL(k_rev_fill2, dim3(gb), dim3(kBS), 0, s, A, B);
+    const int nk = (cap > 0 ? so.nlive : 0) + (L.ss_cap > 0 ? so.nactive : 0);
+    if (nk > 0) hipLaunchKernelGGL(k_rev_sort2, dim3(blocks_for(nk)), dim3(kBS), 0, s, A, B);
 }
 
 void solid_normals(hipStream_t s, const Grids& G, const Solids& so, int ns, const int* scs,
