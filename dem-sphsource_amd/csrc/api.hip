@@ -1858,6 +1858,7 @@ int dsph_upload_batch(dsph_ctx* c, const dsph_scene* scenes, int32_t n_scenes) {
         so.nactive = (int)active.size();
         so.live = dalloc<int>(c, live.size());
         so.nlive = (int)live.size();
+        so.mrec = c->mixed() ? dalloc<uint4>(c, 2 * (size_t)ns) : nullptr;
         h2d(c, so.pr, spr.data(), ns);
         h2d(c, so.vel, svel.data(), ns);
         h2d(c, so.acc, zero4.data(), ns);
@@ -2111,6 +2112,7 @@ int dsph_upload_batch(dsph_ctx* c, const dsph_scene* scenes, int32_t n_scenes) {
             FluidArgs fa = fluid_args(c);
             fluid_records(c->st, fa, 0, mx);
             fluid_records(c->st, fa, 1, mx);
+            solid_records(c->st, c->so, c->ns, c->fset[0]);
         }
         force_rebuild(c);
         HIPCHK(hipStreamSynchronize(c->st));
@@ -2169,6 +2171,7 @@ int dsph_initialize(dsph_ctx* c, int compute_normals) {
             radix_sort_iota(c->st, c->skey, c->ns, c->key_bits, c->skey_sorted, c->sperm, c->radix_tmp);
             cell_spans(c->st, c->skey_sorted, c->ns, c->scs, c->sce, c->ncell, nullptr);
             solid_normals(c->st, grids_view(c), c->so, c->ns, c->scs, c->sce, c->sperm);
+            solid_records(c->st, c->so, c->ns, c->fset[0]);
         }
         HIPCHK(hipStreamSynchronize(c->st));
         invalidate_graphs(c);
@@ -2514,6 +2517,8 @@ int dsph_load_state(dsph_ctx* c, const void* buffer, int64_t size) {
         }
         c->dcur = h.dcur;
         c->skey_valid = h.skey_valid != 0;
+        // the solids' mixed records follow from the loaded state
+        solid_records(c->st, c->so, c->ns, c->fset[0]);
         HIPCHK(hipStreamSynchronize(c->st));
         c->initialized = true;
         return DSPH_OK;

@@ -159,6 +159,14 @@ struct Solids {
     // any dummy with mass, a velocity or a clump (wall dummies are static)
     int* live = nullptr;
     int nlive = 0;
+    // the mixed path's fluid–solid records, two 16-byte quads per solid:
+    // (x, y, z in the fluid's fixed-point map, radius as f32) and (normal
+    // as f32, flag); flag 0: a static solid with zero velocity and
+    // acceleration, whose record holds until the solids are uploaded, their
+    // normals computed or a state loaded; flag 1: the pair terms read the
+    // solid's FP64 state (solids the DEM phases move, and solids outside
+    // the fixed-point map)
+    uint4* mrec = nullptr;
 };
 
 struct Clumps {
@@ -466,6 +474,8 @@ void continuity(hipStream_t s, const FluidArgs& a, double dt, bool mixed, double
 // kick != 0 also kicks and drifts each particle into the other buffer (flip cur after)
 void momentum(hipStream_t s, const FluidArgs& a, bool mixed, bool write_fs_force, double kick);
 void fluid_solid_force(hipStream_t s, const FluidArgs& a, bool mixed);
+// the solids' mixed fluid–solid records (Solids::mrec) from their state
+void solid_records(hipStream_t s, const Solids& so, int ns, const FluidSet& f);
 
 // DEM (k_dem.hip)
 struct DemArgs {

@@ -976,6 +976,35 @@ __global__ void __launch_bounds__(kTileSlots) k_momentum_tile(FluidArgs a, doubl
 enum FsMode { FS_CONT = 0, FS_MOM = 1, FS_SHEPARD = 2 };
 constexpr int kFsTermBlocks = 2048;  // grid cap: 8 workgroups per CU
 
+// the solid side of pair (s, ai) for the fluid slot whose records are o: in
+// the mixed path a static solid comes from its record (r_i - r_a the exact
+// fixed-point difference, zero velocity and acceleration), every other
+// solid from its FP64 state (r_i - r_a subtracted in FP64)
+template <bool MIX>
+__device__ __forceinline__ SolidP<typename Prec<MIX>::R> fs_solid(const FluidArgs& a, int s, int ai,
+                                                                   const Nb<MIX>& o) {
+    using R = typename Prec<MIX>::R;
+    if constexpr (MIX) {
+        const uint4 q0 = a.so.mrec[2 * (int64_t)ai], q1 = a.so.mrec[2 * (int64_t)ai + 1];
+        if (q1.w == 0u) {
+            const R u = pos_unit<true>(a.f);
+            SolidP<R> sp;
+            sp.rx = u * (float)(int)(o.px - q0.x);
+            sp.ry = u * (float)(int)(o.py - q0.y);
+            sp.rz = u * (float)(int)(o.pz - q0.z);
+            sp.vx = sp.vy = sp.vz = R(0);
+            sp.ax = sp.ay = sp.az = R(0);
+            sp.nx = __uint_as_float(q1.x);
+            sp.ny = __uint_as_float(q1.y);
+            sp.nz = __uint_as_float(q1.z);
+            sp.rad = __uint_as_float(q0.w);
+            return sp;
+        }
+    }
+    const double4 ps = a.f.ph[s];
+    return load_solid<R>(a.so, ai, ps.x, ps.y, ps.z);
+}
+
 template <bool MIX, bool UNI, int MODE>
 __global__ void __launch_bounds__(kBS) k_fs_terms(FluidArgs a, int write_fs) {
     using R = typename Prec<MIX>::R;
@@ -987,47 +1016,71 @@ __global__ void __launch_bounds__(kBS) k_fs_terms(FluidArgs a, int write_fs) {
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < tot; k += stride) {
         const int s = a.L.fs_own[k];
         const int ai = a.L.fs_a[k];
-        const double4 ps = f.ph[s];
         if constexpr (MODE == FS_SHEPARD) {
-            const R h = (R)ps.w;
+            Nb<MIX> o;
+            load_pos<MIX>(f, a.cur, s, o);
+            const double hs = UNI ? a.u.h : f.ph[s].w;
+            const R h = (R)hs;
             const R ih = rcp_(h);
             const R sigma = R(kWC) * ih * ih * ih;
-            const double h13 = ps.w / 1.3;
+            const double h13 = hs / 1.3;
             const R invV0 = rcp_((R)(h13 * h13 * h13));
-            const double4 q = a.so.pr[ai];
-            const R rx = (R)(ps.x - q.x), ry = (R)(ps.y - q.y), rz = (R)(ps.z - q.z);
-            const R qq = dist<R>(rx * rx + ry * ry + rz * rz) * ih;
+            const SolidP<R> sp = fs_solid<MIX>(a, s, ai, o);
+            const R qq = dist<R>(sp.rx * sp.rx + sp.ry * sp.ry + sp.rz * sp.rz) * ih;
             const R t = R(1) - R(0.5) * qq;
             const R t2 = t * t;
-            const R d = R(2) * (R)q.w;
+            const R d = R(2) * sp.rad;
             ((R*)a.L.fsT)[k] = qq < R(2) ? sigma * t2 * t2 * (R(1) + R(2) * qq) * (d * d * d) * invV0 : R(0);
         } else {
             const Nb<MIX> o = load_nb<MIX, UNI>(a, a.cur, s);
-            const double4 vm = f.vr[s];
-            const SolidP<R> sp = load_solid<R>(a.so, ai, ps.x, ps.y, ps.z);
+            const SolidP<R> sp = fs_solid<MIX>(a, s, ai, o);
             if constexpr (MODE == FS_CONT) {
                 const R r = dist<R>(sp.rx * sp.rx + sp.ry * sp.ry + sp.rz * sp.rz);
                 const R F = grad_factor<R>(r, R(0.5) * (o.h + R(2.6) * sp.rad));
                 const R d2 = R(2) * sp.rad;
                 const R Va = d2 * d2 * d2;
                 R Pa, rhoa;
-                wall_state<R>(o.P, o.rho, o.c, (R)vm.w, sp, (R)a.gx, (R)a.gy, (R)a.gz, Pa, rhoa);
+                wall_state<R>(o.P, o.rho, o.c, o.rho0, sp, (R)a.gx, (R)a.gy, (R)a.gz, Pa, rhoa);
                 const R dvr = (o.vx - sp.vx) * sp.rx + (o.vy - sp.vy) * sp.ry + (o.vz - sp.vz) * sp.rz;
                 const R t = R(2) * o.rho * Va * F * (R(2) * rhoa * dvr + (o.P - Pa) * r * rcp_(o.c)) *
                             rcp_(o.rho + rhoa);
                 ((R*)a.L.fsT)[k] = F != R(0) ? t : R(0);
             } else {
                 R tx, ty, tz;
-                fs_term<MIX>(o, (R)vm.w, sp, (R)a.gx, (R)a.gy, (R)a.gz, tx, ty, tz);
+                fs_term<MIX>(o, o.rho0, sp, (R)a.gx, (R)a.gy, (R)a.gz, tx, ty, tz);
                 ((R4*)a.L.fsT)[k] = R4{tx, ty, tz, R(0)};
                 if (write_fs) {
-                    const double h13 = ps.w / 1.3;
-                    const double mi = h13 * h13 * h13 * vm.w;
+                    const double h13 = (UNI ? a.u.h : f.ph[s].w) / 1.3;
+                    const double mi = h13 * h13 * h13 * (UNI ? a.u.rho0 : f.vr[s].w);
                     a.L.fsF[k] = make_double4(mi * tx, mi * ty, mi * tz, 0.0);
                 }
             }
         }
     }
+}
+
+// Solids::mrec from the solids' state (mixed path).  A solid outside the
+// fixed-point map, or with a velocity or acceleration, is flagged; the
+// solids the DEM phases move are flagged by the second kernel.
+__global__ void __launch_bounds__(kBS) k_solid_mrec(Solids so, int ns, FluidSet f) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ns) return;
+    const double4 p = so.pr[i], v = so.vel[i], ac = so.acc[i], n = so.nrm[i];
+    bool in_map = true;
+    const double c[3] = {p.x, p.y, p.z};
+    for (int d = 0; d < 3; d++) {
+        const double t = (c[d] - f.qo[d]) * f.qi[d];
+        in_map = in_map && t >= 0.0 && t <= 4294967295.0;
+    }
+    const bool moving = v.x != 0.0 || v.y != 0.0 || v.z != 0.0 || ac.x != 0.0 || ac.y != 0.0 || ac.z != 0.0;
+    so.mrec[2 * (int64_t)i] = make_uint4(fixq(p.x, f.qo[0], f.qi[0]), fixq(p.y, f.qo[1], f.qi[1]),
+                                         fixq(p.z, f.qo[2], f.qi[2]), __float_as_uint((float)p.w));
+    so.mrec[2 * (int64_t)i + 1] = make_uint4(__float_as_uint((float)n.x), __float_as_uint((float)n.y),
+                                             __float_as_uint((float)n.z), (!in_map || moving) ? 1u : 0u);
+}
+__global__ void __launch_bounds__(kBS) k_solid_mrec_live(Solids so) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < so.nlive) so.mrec[2 * (int64_t)so.live[t] + 1].w = 1u;
 }
 
 // -------------------------------------------------------- density re-init
@@ -1161,6 +1214,12 @@ void momentum(hipStream_t s, const FluidArgs& a, bool mixed, bool write_fs_force
         DSPH_LAUNCH_TILE(k_momentum_tile, s, a, kick);
     else
         DSPH_LAUNCH_UNI(k_momentum, a.nf, a.bs, s, a, kick);
+}
+
+void solid_records(hipStream_t s, const Solids& so, int ns, const FluidSet& f) {
+    if (ns <= 0 || !so.mrec) return;
+    hipLaunchKernelGGL(k_solid_mrec, dim3(blocks_for(ns)), dim3(kBS), 0, s, so, ns, f);
+    if (so.nlive > 0) hipLaunchKernelGGL(k_solid_mrec_live, dim3(blocks_for(so.nlive)), dim3(kBS), 0, s, so);
 }
 
 // calFluid2SolidForce (integrate.cu:282-330) on steps without a fluid update:
