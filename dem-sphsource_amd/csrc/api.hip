@@ -71,6 +71,7 @@ struct dsph_ctx {
     int nwmax = 0;                        // most walls of one scene
     int key_bits = 1;
     double hmax = 0;  // largest fluid smoothing length of all scenes
+    double fix_lmax = 0;  // longest extent of the fixed-point map's domain box
     double gv[3] = {0, 0, -9.81};
     // fluid
     FluidSet fset[2];
@@ -255,8 +256,12 @@ void alloc_ff(dsph_ctx* c, int K) {
 // ------------------------------------------------------------------ tiles
 // The pair-loop tiles run for the mixed precision with the reorder on (a
 // tile must be a block of consecutive slots that is compact in space).
+// the tiled loops' sentinel record (fixed-point origin) lies at least the
+// domain box's longest extent from every particle in it, which must clear
+// every support (2 h_max) with room to spare
 bool tiles_on(const dsph_ctx* c) {
-    return c->tune_tile != 0 && c->mixed() && c->reorder_period > 0 && c->nf > 0;
+    return c->tune_tile != 0 && c->mixed() && c->reorder_period > 0 && c->nf > 0 &&
+           c->fix_lmax >= 4.0 * c->hmax;
 }
 
 void free_tiles(dsph_ctx* c) {
@@ -1686,6 +1691,7 @@ int dsph_upload_batch(dsph_ctx* c, const dsph_scene* scenes, int32_t n_scenes) {
             // than the 1e-10 m below which a pair counts for nothing
             double Lmax = 0.0;
             for (int a = 0; a < 3; a++) Lmax = std::max(Lmax, hi[a] - lo[a]);
+            c->fix_lmax = Lmax;
             const double unit = std::max(3.0 * Lmax / 4294967296.0, 2e-10);
             for (int a = 0; a < 3; a++) {
                 const double L = hi[a] - lo[a];

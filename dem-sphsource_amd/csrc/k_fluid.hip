@@ -487,10 +487,16 @@ __device__ __forceinline__ void for_row(const FluidArgs& a, Rsrc rs, int s, Fn&&
 // 16-byte entries, so consecutive union positions fall in consecutive bank
 // groups (a 32-byte stride would leave half of them unused and make the
 // lanes' reads collide).
+// Entry kTileCap is a sentinel that local rows carry past their end: a
+// record at fixed-point (0, 0, 0) at rest with dRho = 0.  Every particle of
+// a tiled row lies in the domain, which the fixed-point map places at least
+// 2^32 / 3 units from 0 on each axis, so the sentinel is far outside every
+// support: its t clamps to 0 and its term is an exact zero (finite
+// factors), and the walk needs no row-end test.
 using TileRec = uint4;
 struct TileLds {
-    TileRec pos[kTileCap];
-    TileRec st[kTileCap];
+    TileRec pos[kTileCap + 1];
+    TileRec st[kTileCap + 1];
 };
 template <bool STATE>
 __device__ __forceinline__ int stage_tile(const FluidArgs& a, int b, TileLds& L) {
@@ -501,6 +507,10 @@ __device__ __forceinline__ int stage_tile(const FluidArgs& a, int b, TileLds& L)
         const int j = uni[k];
         L.pos[k] = rec[2 * (int64_t)j];
         if constexpr (STATE) L.st[k] = rec[2 * (int64_t)j + 1];
+    }
+    if (threadIdx.x == 0) {
+        L.pos[kTileCap] = make_uint4(0u, 0u, 0u, 0u);
+        L.st[kTileCap] = make_uint4(0u, 0u, 0u, 0u);
     }
     __syncthreads();
     return m;
@@ -547,7 +557,7 @@ __device__ __forceinline__ void for_row_tile(const FluidArgs& a, const TileLds& 
 #pragma unroll
         for (int u = 0; u < U; u++) {
             load_tile_rec<MODE != POS>(lrec, jn[u], q[u]);
-            ok[u] = k + u < n;
+            ok[u] = true;  // entries past the row end are the sentinel
         }
         if constexpr (MODE != POS) {
 #pragma unroll

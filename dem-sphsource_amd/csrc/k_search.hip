@@ -102,12 +102,16 @@ __device__ __forceinline__ void wave_max_atomic(unsigned long long* addr, unsign
     if ((threadIdx.x & 63) == 0 && v) atomicMax(addr, v);
 }
 
-__global__ void k_status_reset(unsigned long long* st) {
-    int i = threadIdx.x;
+// the start of a search: the per-search status words, and on fluid steps
+// the near-out-of-domain cell flags (near_out), cleared in one launch
+__global__ void k_status_reset(unsigned long long* st, int* near_out, int ncell) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     // overflow flags are sticky until read; counters are per search
     if (i == ST_OUT_OF_DOMAIN || i == ST_FF_MAX || i == ST_FS_TOTAL || i == ST_SS_TOTAL || i == ST_SUP_MAX ||
         i == ST_NONFINITE)
         st[i] = 0;
+    if (near_out)
+        for (int c = i; c < ncell; c += gridDim.x * blockDim.x) near_out[c] = 0;
 }
 
 // Morton code of a point's sub-cell, b bits per axis: (x - min) / cs is the
@@ -1452,8 +1456,9 @@ void hash_fluid(hipStream_t s, const Grids& G, const double4* ph, const int* ori
                 const unsigned* crank, const unsigned char* cspare, unsigned* key, unsigned* rank,
                 unsigned* hslot, const double4* fxb, unsigned long long* bmax, int* near_out, int ncell,
                 unsigned long long* ctl, long long* status) {
-    hipLaunchKernelGGL(k_status_reset, dim3(1), dim3(ST_WORDS), 0, s, (unsigned long long*)status);
-    if (near_out) fill_int(s, near_out, ncell, 0);
+    static_assert(ST_WORDS <= kBS, "the first workgroup clears the status words");
+    const int nb = near_out ? std::min(std::max(blocks_for(ncell), 1), 4096) : 1;
+    hipLaunchKernelGGL(k_status_reset, dim3(nb), dim3(kBS), 0, s, (unsigned long long*)status, near_out, ncell);
     if (n > 0)
         hipLaunchKernelGGL(k_hash_fluid, dim3(blocks_for(n)), dim3(kBS), 0, s, G, ph, orig, n, sub_bits,
                            crank, cspare, key, rank, hslot, fxb, bmax, near_out, ctl,
@@ -1686,8 +1691,9 @@ __global__ void __launch_bounds__(kTileSlots) k_tile_build(SearchArgs a) {
             }
         }
         // 4. the read-ahead entries, to the end of the third quad after the
-        // row's last (w + 15 <= supK + kRowPad)
-        for (int c = w; c < ((w + 3) & ~3) + 12; c++) T.row[tile_row_at(c, s, a.nf)] = 0;
+        // row's last (w + 15 <= supK + kRowPad): the pair loops' sentinel
+        // entry kTileCap, which lies outside every support
+        for (int c = w; c < ((w + 3) & ~3) + 12; c++) T.row[tile_row_at(c, s, a.nf)] = (unsigned short)kTileCap;
         T.rcnt[s] = w;
     }
     __syncthreads();
