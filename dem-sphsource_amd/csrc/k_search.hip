@@ -1217,7 +1217,7 @@ __global__ void k_ss_save(const int* __restrict__ active, int nactive, const int
 // into zeroed accumulators) and resets the cells the previous sort filled;
 // pass 2 places every solid, writes its cell's range (all members write the
 // same values) and zeroes the accumulators again.
-constexpr int kRankI = 64, kRankJ = 256;
+constexpr int kRankI = 64, kRankJ = 64;
 __device__ __forceinline__ unsigned long long akey64(const int* active, const unsigned* skey, int t) {
     return ((unsigned long long)skey[active[t]] << 32) | (unsigned)t;
 }

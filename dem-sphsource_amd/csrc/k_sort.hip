@@ -92,6 +92,59 @@ __global__ void __launch_bounds__(kBS) k_scan_final(const int* __restrict__ in, 
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kBS - 1) out[n] = run;
 }
 
+// Exclusive scan of up to kScanOneMax ints in one workgroup; out[n] = the
+// total.  Each wavefront takes a contiguous segment (a multiple of 256
+// entries): it sums the segment with coalesced loads, the segment sums are
+// scanned across the workgroup, and the wavefront then scans its segment
+// 256 entries at a time (four consecutive entries per lane, a wavefront
+// scan of the lanes' sums, a running carry).  One launch instead of the
+// three of the tiled scan (a gated launch costs the same whether it works
+// or exits).
+constexpr int kScanOneBS = 1024;
+constexpr int64_t kScanOneMax = 262144;
+__global__ void __launch_bounds__(kScanOneBS) k_scan_one(const int* __restrict__ in, int64_t n,
+                                                         int* __restrict__ out, Gate gate) {
+    if (gate && *gate == 0) return;
+    constexpr int W = kScanOneBS / 64;
+    __shared__ int wsum[W];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t seg = ((n + W - 1) / W + 255) / 256 * 256;
+    const int64_t b = (int64_t)w * seg;
+    const int64_t e = b + seg < n ? b + seg : n;
+    int sum = 0;
+    for (int64_t i = b + lane; i < e; i += 64) sum += in[i];
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) sum += __shfl_xor(sum, d, 64);
+    if (lane == 0) wsum[w] = sum;
+    __syncthreads();
+    int carry = 0, total = 0;
+    for (int k = 0; k < W; k++) {
+        carry += k < w ? wsum[k] : 0;
+        total += wsum[k];
+    }
+    for (int64_t i0 = b; i0 < e; i0 += 256) {
+        const int64_t i = i0 + 4 * lane;
+        int v[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) v[k] = i + k < e ? in[i + k] : 0;
+        const int t = v[0] + v[1] + v[2] + v[3];
+        int inc = t;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int u = __shfl_up(inc, d, 64);
+            if (lane >= d) inc += u;
+        }
+        int run = carry + inc - t;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (i + k < e) out[i + k] = run;
+            run += v[k];
+        }
+        carry += __shfl(inc, 63, 64);
+    }
+    if (threadIdx.x == 0) out[n] = total;
+}
+
 __global__ void __launch_bounds__(kBS) k_rs_hist(const unsigned* __restrict__ kin, int64_t n,
                                                  int shift, int* __restrict__ hist, int ntiles,
                                                  Gate gate) {
@@ -279,7 +332,11 @@ void radix_sort_iota(hipStream_t s, const unsigned* keys_in, int64_t n, int bits
         int* vout = p == passes - 1 ? perm_out : vb[p & 1];
         hipLaunchKernelGGL(k_rs_hist, dim3((unsigned)nt), dim3(kBS), 0, s, kin, n, 8 * p, hist, (int)nt,
                            gate);
-        exclusive_scan(s, hist, hoff, 256 * nt, stmp, gate);
+        if (256 * nt <= kScanOneMax)
+            hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(kScanOneBS), 0, s, (const int*)hist, 256 * nt, hoff,
+                               gate);
+        else
+            exclusive_scan(s, hist, hoff, 256 * nt, stmp, gate);
         hipLaunchKernelGGL(k_rs_scatter, dim3((unsigned)nt), dim3(kBS), 0, s, kin, vin, n, 8 * p,
                            (const int*)hoff, (int)nt, kout, vout, gate);
     }
