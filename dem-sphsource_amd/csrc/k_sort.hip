@@ -31,6 +31,16 @@ __global__ void k_fill_int(int* __restrict__ p, int64_t n, int v, Gate gate) {
         p[i] = v;
 }
 
+// two arrays of n ints set to v in one launch
+__global__ void k_fill_int2(int* __restrict__ p, int* __restrict__ q, int64_t n, int v, Gate gate) {
+    if (gate && *gate == 0) return;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        p[i] = v;
+        q[i] = v;
+    }
+}
+
 __global__ void k_copy_int(int* __restrict__ d, const int* __restrict__ s, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
@@ -344,8 +354,11 @@ void radix_sort_iota(hipStream_t s, const unsigned* keys_in, int64_t n, int bits
 
 void cell_spans(hipStream_t s, const unsigned* sorted, int64_t n, int* start, int* end, int ncell,
                 const unsigned* inv, Gate gate, int shift) {
-    fill_int(s, start, ncell, -1, gate);
-    fill_int(s, end, ncell, -1, gate);
+    if (ncell > 0) {
+        int b = blocks_for(ncell);
+        if (b > 4096) b = 4096;
+        hipLaunchKernelGGL(k_fill_int2, dim3(b), dim3(kBS), 0, s, start, end, (int64_t)ncell, -1, gate);
+    }
     if (n <= 0) return;
     hipLaunchKernelGGL(k_spans, dim3(blocks_for(n)), dim3(kBS), 0, s, sorted, n, start, end, inv, gate,
                        shift);
