@@ -30,6 +30,8 @@
 // After a build the fluid particles are reordered into cell-sorted slots
 // (gather + copy-back, so device pointers never change and a captured HIP
 // graph stays valid).
+#include <type_traits>
+
 #include "dsph_internal.h"
 
 namespace dsph {
@@ -455,6 +457,10 @@ constexpr int kQueue = 128;
 // come within range before the next rebuild, so the margin errs on the
 // inclusive side.  Spare-cell particles and spare-cell candidates are left
 // out: they are walked exactly every step.
+// G1: rows of single slots (group 1, the tiled path's setting): a lane's
+// own test decides and the lane writes its own row, with its store offset
+// advanced per member (no ballot, no row_at); the rows are the same.
+template <bool G1>
 __global__ void __launch_bounds__(kBS) k_build(SearchArgs a, const int2* __restrict__ chunks,
                                                int nchunk) {
     if (a.ctl[CTL_REBUILD] == 0) return;
@@ -498,7 +504,7 @@ __global__ void __launch_bounds__(kBS) k_build(SearchArgs a, const int2* __restr
         // supersets, kept by the group's first active lane (the writer): the
         // lanes of a group then read the same candidates in the same order,
         // so their record loads coalesce.  A lane's own row is a subset.
-        const int G = a.group;
+        const int G = G1 ? 1 : a.group;
         const int gl = lane & ~(G - 1);
         const unsigned long long gmask = (G == 64 ? ~0ull : ((1ull << G) - 1)) << gl;
         const unsigned long long actm = __ballot(act) & gmask;
@@ -545,9 +551,12 @@ __global__ void __launch_bounds__(kBS) k_build(SearchArgs a, const int2* __restr
         };
         LdsQueue qf{lq + wq, lj + wq, 0}, qs{lq + wq + kQueue, lj + wq + kQueue, 0};
         int c = 0, cs = 0;
+        int64_t wo = s;  // G1: the store offset of the row's next column
         // membership scan of the first nb (<= 64) queued fluid candidates
-        auto scan_fluid = [&](int nb) {
-            for (int k = 0; k < nb; k += 4) {
+        // (full: nb = 64)
+        auto scan_fluid = [&](int nb, auto full) {
+            constexpr bool FULL = decltype(full)::value;
+            for (int k = 0; k < (FULL ? 64 : nb); k += 4) {
                 // four broadcast reads in flight; entries past nb (within
                 // the queue) are read but not used
                 float4 q[4];
@@ -561,15 +570,23 @@ __global__ void __launch_bounds__(kBS) k_build(SearchArgs a, const int2* __restr
                 for (int v = 0; v < 4; v++) {
                     const float ex = px - q[v].x, ey = py - q[v].y, ez = pz - q[v].z;
                     const float e2 = ex * ex + ey * ey + ez * ez;
-                    const bool in = act & (k + v < nb) & (jj[v] != s) & (e2 <= (Ai > q[v].w ? Ai : q[v].w));
-                    const bool gin = (__ballot(in) & gmask) != 0;
-                    if (writer && gin && c < supK) a.L.sup[row_at(gcode, c, a.nf)] = jj[v];
-                    c += gin;
+                    bool in = act & (jj[v] != s) & (e2 <= (Ai > q[v].w ? Ai : q[v].w));
+                    if constexpr (!FULL) in = in & (k + v < nb);
+                    if constexpr (G1) {
+                        if (in && c < supK) a.L.sup[wo] = jj[v];
+                        wo += in ? (int64_t)a.nf : 0;
+                        c += in;
+                    } else {
+                        const bool gin = (__ballot(in) & gmask) != 0;
+                        if (writer && gin && c < supK) a.L.sup[row_at(gcode, c, a.nf)] = jj[v];
+                        c += gin;
+                    }
                 }
             }
         };
-        auto scan_solid = [&](int nb) {
-            for (int k = 0; k < nb; k += 4) {
+        auto scan_solid = [&](int nb, auto full) {
+            constexpr bool FULL = decltype(full)::value;
+            for (int k = 0; k < (FULL ? 64 : nb); k += 4) {
                 float4 q[4];
                 int aa[4];
 #pragma unroll
@@ -580,7 +597,7 @@ __global__ void __launch_bounds__(kBS) k_build(SearchArgs a, const int2* __restr
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
                     const float ex = px - q[v].x, ey = py - q[v].y, ez = pz - q[v].z;
-                    const bool in = (k + v < nb) & (ex * ex + ey * ey + ez * ez <= culls);
+                    const bool in = (FULL || k + v < nb) & (ex * ex + ey * ey + ez * ez <= culls);
                     if (in && cs < fsupK) a.L.fsup[(int64_t)cs * a.nf + s] = aa[v];
                     cs += in;
                 }
@@ -598,7 +615,7 @@ __global__ void __launch_bounds__(kBS) k_build(SearchArgs a, const int2* __restr
             Q.n += __popcll(km);
             __builtin_amdgcn_wave_barrier();
             if (Q.n >= 64) {
-                scan(64);
+                scan(64, std::true_type{});
                 __builtin_amdgcn_wave_barrier();
                 const float4 mv = Q.q[64 + (lane & 63)];
                 const int mj = Q.j[64 + (lane & 63)];
@@ -668,8 +685,8 @@ __global__ void __launch_bounds__(kBS) k_build(SearchArgs a, const int2* __restr
             }
         }
         // what is left in the queues
-        if (qf.n > 0) scan_fluid(qf.n);
-        if (qs.n > 0) scan_solid(qs.n);
+        if (qf.n > 0) scan_fluid(qf.n, std::false_type{});
+        if (qs.n > 0) scan_solid(qs.n, std::false_type{});
         __builtin_amdgcn_wave_barrier();
         if (act) {
             a.L.sup_cnt[s] = c;
@@ -1707,7 +1724,10 @@ void tile_rows(hipStream_t s, const SearchArgs& a) {
 
 void fluid_superset(hipStream_t s, const SearchArgs& a, const int2* chunks, int nchunk) {
     if (a.nf <= 0) return;
-    hipLaunchKernelGGL(k_build, dim3(chunk_grid(nchunk)), dim3(kBS), 0, s, a, chunks, nchunk);
+    if (a.group == 1)
+        hipLaunchKernelGGL(k_build<true>, dim3(chunk_grid(nchunk)), dim3(kBS), 0, s, a, chunks, nchunk);
+    else
+        hipLaunchKernelGGL(k_build<false>, dim3(chunk_grid(nchunk)), dim3(kBS), 0, s, a, chunks, nchunk);
 }
 
 void fluid_exact_rows(hipStream_t s, const SearchArgs& a, int* out, int* out_cnt) {
