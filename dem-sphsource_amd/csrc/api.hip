@@ -168,6 +168,20 @@ struct dsph_ctx {
     // work buffer of dsph_hash_sort (grown on demand)
     unsigned* hs_work = nullptr;
     int64_t hs_words = 0;
+    // VTU frames (dsph_frame_begin / dsph_frame_write): one staging block on
+    // the device, and per slot its pinned host copy, kept across uploads
+    struct FrameSlot {
+        void* host = nullptr;  // pinned (hipHostMalloc)
+        size_t cap = 0;        // bytes allocated
+        size_t bytes = 0;      // bytes of the pending frame
+        hipEvent_t ev = nullptr;
+        bool pending = false;
+        int32_t step = 0;
+        double time = 0.0;
+        int64_t nf = 0, ns = 0, pcap = 0;
+    } frames[DSPH_FRAME_SLOTS];
+    void* frame_dev = nullptr;  // in allocs
+    size_t frame_dev_bytes = 0;
 
     bool mixed() const { return prec == DSPH_PREC_MIXED; }
 };
@@ -218,6 +232,9 @@ void free_all(dsph_ctx* c) {
     c->backup_bytes.clear();
     c->hs_work = nullptr;
     c->hs_words = 0;
+    c->frame_dev = nullptr;
+    c->frame_dev_bytes = 0;
+    for (auto& f : c->frames) f.pending = false;
 }
 
 int fail(dsph_ctx* c, const ApiError& e) {
@@ -1455,6 +1472,10 @@ int dsph_destroy(dsph_ctx* c) {
             (void)hipEventDestroy(e.first);
             (void)hipEventDestroy(e.second);
         }
+    for (auto& f : c->frames) {
+        if (f.host) (void)hipHostFree(f.host);
+        if (f.ev) (void)hipEventDestroy(f.ev);
+    }
     (void)hipStreamDestroy(c->st);
     delete c;
     return DSPH_OK;
@@ -2207,7 +2228,7 @@ namespace {
 // The dt the phases compute with: the caller's, or with per-scene time
 // steps set (the caller then passes 0) 1, so every phase hands its kernels
 // the factor of each scene's own step
-double phase_dt(const dsph_ctx* c, double dt) {
+static double phase_dt(const dsph_ctx* c, double dt) {
     if (!c->sdt) return dt;
     if (dt != 0.0)
         throw ApiError{DSPH_E_INVALID, "per-scene time steps are set: pass dt = 0 (or clear them)"};
@@ -2399,7 +2420,7 @@ struct StateBlobHeader {
 constexpr char kStateMagic[8] = {'D', 'S', 'P', 'H', 'S', 'T', 'A', 'T'};
 constexpr int32_t kStateVersion = 3;
 
-StateBlobHeader state_header(dsph_ctx* c, const std::vector<StateArr>& v) {
+static StateBlobHeader state_header(dsph_ctx* c, const std::vector<StateArr>& v) {
     StateBlobHeader h{};
     std::memcpy(h.magic, kStateMagic, 8);
     h.version = kStateVersion;
@@ -2437,7 +2458,7 @@ StateBlobHeader state_header(dsph_ctx* c, const std::vector<StateArr>& v) {
     return h;
 }
 
-int64_t blob_size(const StateBlobHeader& h) {
+static int64_t blob_size(const StateBlobHeader& h) {
     return (int64_t)sizeof(StateBlobHeader) + 8 * h.n_arrays + h.payload;
 }
 }  // namespace
@@ -2590,6 +2611,223 @@ int dsph_add_global_damping(dsph_ctx* c, int32_t idx, double cd) {
     } catch (const ApiError& e) {
         return fail(c, e);
     }
+}
+
+// ------------------------------------------------------------------ frames
+namespace {
+// byte layout of a frame's staging block: the arrays of FrameOut, each
+// 256-byte aligned, then the pair count
+struct FrameLayout {
+    size_t fpos, fh, fvel, frho, fp, spos, srad, sclus, sclump, svel, sw, snrm, mid, force, fid, sid, head,
+        bytes;
+};
+static FrameLayout frame_layout(int64_t nf, int64_t ns, int64_t pcap) {
+    FrameLayout L{};
+    size_t at = 0;
+    auto take = [&](size_t n) {
+        const size_t o = at;
+        at += (n + 255) / 256 * 256;
+        return o;
+    };
+    L.fpos = take(12 * nf);
+    L.fh = take(4 * nf);
+    L.fvel = take(12 * nf);
+    L.frho = take(4 * nf);
+    L.fp = take(4 * nf);
+    L.spos = take(12 * ns);
+    L.srad = take(4 * ns);
+    L.sclus = take(4 * ns);
+    L.sclump = take(4 * ns);
+    L.svel = take(12 * ns);
+    L.sw = take(12 * ns);
+    L.snrm = take(12 * ns);
+    L.mid = take(12 * pcap);
+    L.force = take(12 * pcap);
+    L.fid = take(4 * pcap);
+    L.sid = take(4 * pcap);
+    L.head = take(8);
+    L.bytes = at;
+    return L;
+}
+
+// One binary appended VTU file (raw encoding, UInt64 block headers) of n
+// vertex cells: the FieldData TIME and STEP, the points, the cells, and the
+// point arrays, in the reference's element order (Points, Cells, PointData).
+struct VtuArray {
+    const char* name;
+    const char* type;  // "Float32", "Int32"
+    int comps;
+    const void* data;
+    size_t bytes;
+};
+static bool write_vtu(const std::string& path, double time, int32_t step, int64_t n, const float* points,
+               const std::vector<VtuArray>& arrays, const char* scalars) {
+    std::vector<int32_t> conn((size_t)n), offs((size_t)n);
+    std::vector<uint8_t> types((size_t)n, 1);  // VTK_VERTEX
+    for (int64_t i = 0; i < n; i++) {
+        conn[i] = (int32_t)i;
+        offs[i] = (int32_t)(i + 1);
+    }
+    const float tf = (float)time;
+    struct Block {
+        const void* p;
+        size_t bytes;
+    };
+    std::vector<Block> blocks;
+    size_t off = 0;
+    auto next = [&](const void* p, size_t b) {
+        const size_t o = off;
+        blocks.push_back({p, b});
+        off += 8 + b;
+        return o;
+    };
+    std::string x;
+    x += "<?xml version=\"1.0\"?>\n<VTKFile type=\"UnstructuredGrid\" version=\"1.0\" "
+         "byte_order=\"LittleEndian\" header_type=\"UInt64\">\n  <UnstructuredGrid>\n";
+    x += "    <FieldData>\n";
+    x += "      <DataArray type=\"Float32\" Name=\"TIME\" NumberOfTuples=\"1\" format=\"appended\" offset=\"" +
+         std::to_string(next(&tf, 4)) + "\"/>\n";
+    x += "      <DataArray type=\"Int32\" Name=\"STEP\" NumberOfTuples=\"1\" format=\"appended\" offset=\"" +
+         std::to_string(next(&step, 4)) + "\"/>\n";
+    x += "    </FieldData>\n";
+    x += "    <Piece NumberOfPoints=\"" + std::to_string(n) + "\" NumberOfCells=\"" + std::to_string(n) + "\">\n";
+    x += "      <Points>\n        <DataArray type=\"Float32\" NumberOfComponents=\"3\" format=\"appended\" "
+         "offset=\"" + std::to_string(next(points, 12 * (size_t)n)) + "\"/>\n      </Points>\n";
+    x += "      <Cells>\n";
+    x += "        <DataArray type=\"Int32\" Name=\"connectivity\" format=\"appended\" offset=\"" +
+         std::to_string(next(conn.data(), 4 * (size_t)n)) + "\"/>\n";
+    x += "        <DataArray type=\"Int32\" Name=\"offsets\" format=\"appended\" offset=\"" +
+         std::to_string(next(offs.data(), 4 * (size_t)n)) + "\"/>\n";
+    x += "        <DataArray type=\"UInt8\" Name=\"types\" format=\"appended\" offset=\"" +
+         std::to_string(next(types.data(), (size_t)n)) + "\"/>\n";
+    x += "      </Cells>\n";
+    x += std::string("      <PointData Scalars=\"") + scalars + "\">\n";
+    for (const VtuArray& a : arrays) {
+        x += std::string("        <DataArray type=\"") + a.type + "\" Name=\"" + a.name + "\"";
+        if (a.comps > 1) x += " NumberOfComponents=\"" + std::to_string(a.comps) + "\"";
+        x += " format=\"appended\" offset=\"" + std::to_string(next(a.data, a.bytes)) + "\"/>\n";
+    }
+    x += "      </PointData>\n    </Piece>\n  </UnstructuredGrid>\n  <AppendedData encoding=\"raw\">\n   _";
+    FILE* fp = std::fopen(path.c_str(), "wb");
+    if (!fp) return false;
+    bool ok = std::fwrite(x.data(), 1, x.size(), fp) == x.size();
+    for (const Block& b : blocks) {
+        const uint64_t h = b.bytes;
+        ok = ok && std::fwrite(&h, 8, 1, fp) == 1;
+        if (b.bytes) ok = ok && std::fwrite(b.p, 1, b.bytes, fp) == b.bytes;
+    }
+    const char tail[] = "\n  </AppendedData>\n</VTKFile>\n";
+    ok = ok && std::fwrite(tail, 1, sizeof(tail) - 1, fp) == sizeof(tail) - 1;
+    return std::fclose(fp) == 0 && ok;
+}
+}  // namespace
+
+int dsph_frame_begin(dsph_ctx* c, int32_t slot, int32_t i_step, double time) {
+    API_GUARD(c);
+    if (slot < 0 || slot >= DSPH_FRAME_SLOTS) return DSPH_E_INVALID;
+    try {
+        auto& F = c->frames[slot];
+        const int64_t nf = c->nf, ns = c->ns;
+        const int64_t pcap = (nf > 0 && ns > 0) ? c->L.fs_cap : 0;
+        const FrameLayout L = frame_layout(nf, ns, pcap);
+        // this slot's previous copy has landed before its buffer is reused
+        if (F.ev) HIPCHK(hipEventSynchronize(F.ev));
+        if (c->frame_dev_bytes < L.bytes) {
+            // (a copy of the other slot may still read the old block)
+            HIPCHK(hipStreamSynchronize(c->st));
+            dfree(c, c->frame_dev);
+            c->frame_dev = dalloc<char>(c, L.bytes);
+            c->frame_dev_bytes = L.bytes;
+        }
+        if (F.cap < L.bytes) {
+            if (F.host) HIPCHK(hipHostFree(F.host));
+            F.host = nullptr;
+            F.cap = 0;
+            if (hipHostMalloc(&F.host, L.bytes, hipHostMallocDefault) != hipSuccess)
+                throw ApiError{DSPH_E_NOMEM, "pinned host memory for a frame"};
+            F.cap = L.bytes;
+        }
+        if (!F.ev) HIPCHK(hipEventCreateWithFlags(&F.ev, hipEventDisableTiming));
+        char* d = (char*)c->frame_dev;
+        FrameOut o;
+        o.fpos = (float*)(d + L.fpos);
+        o.fh = (float*)(d + L.fh);
+        o.fvel = (float*)(d + L.fvel);
+        o.frho = (float*)(d + L.frho);
+        o.fp = (float*)(d + L.fp);
+        o.spos = (float*)(d + L.spos);
+        o.srad = (float*)(d + L.srad);
+        o.sclus = (int*)(d + L.sclus);
+        o.sclump = (int*)(d + L.sclump);
+        o.svel = (float*)(d + L.svel);
+        o.sw = (float*)(d + L.sw);
+        o.snrm = (float*)(d + L.snrm);
+        o.mid = (float*)(d + L.mid);
+        o.force = (float*)(d + L.force);
+        o.fid = (int*)(d + L.fid);
+        o.sid = (int*)(d + L.sid);
+        o.head = (long long*)(d + L.head);
+        // the records of buffer dcur hold the fluid's current dRho
+        frame_pack(c->st, c->fset[0], (int)nf, c->dcur, c->so, (int)ns, c->L, pcap, o);
+        HIPCHK(hipMemcpyAsync(F.host, c->frame_dev, L.bytes, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipEventRecord(F.ev, c->st));
+        F.bytes = L.bytes;
+        F.step = i_step;
+        F.time = time;
+        F.nf = nf;
+        F.ns = ns;
+        F.pcap = pcap;
+        F.pending = true;
+        return DSPH_OK;
+    } catch (const ApiError& e) {
+        return fail(c, e);
+    }
+}
+
+int dsph_frame_write(dsph_ctx* c, int32_t slot, const char* dir, int32_t frame, uint32_t species) {
+    if (!c || !dir || slot < 0 || slot >= DSPH_FRAME_SLOTS || frame < 0) return DSPH_E_INVALID;
+    auto& F = c->frames[slot];
+    if (!F.pending) return DSPH_E_STATE;
+    if (hipEventSynchronize(F.ev) != hipSuccess) return DSPH_E_HIP;
+    const FrameLayout L = frame_layout(F.nf, F.ns, F.pcap);
+    const char* h = (const char*)F.host;
+    char num[16];
+    std::snprintf(num, sizeof(num), "%04d", (int)frame);
+    const std::string base = std::string(dir) + "/";
+    bool ok = true;
+    if (species & DSPH_FRAME_FLUID) {
+        const size_t n = (size_t)F.nf;
+        ok = ok && write_vtu(base + "fluid_" + num + ".vtu", F.time, F.step, F.nf, (const float*)(h + L.fpos),
+                             {{"smooth length", "Float32", 1, h + L.fh, 4 * n},
+                              {"velocity", "Float32", 3, h + L.fvel, 12 * n},
+                              {"density", "Float32", 1, h + L.frho, 4 * n},
+                              {"pressure", "Float32", 1, h + L.fp, 4 * n}},
+                             "smooth length");
+    }
+    if (species & DSPH_FRAME_SOLID) {
+        const size_t n = (size_t)F.ns;
+        ok = ok && write_vtu(base + "solid_" + num + ".vtu", F.time, F.step, F.ns, (const float*)(h + L.spos),
+                             {{"radius", "Float32", 1, h + L.srad, 4 * n},
+                              {"clusterID", "Int32", 1, h + L.sclus, 4 * n},
+                              {"clumpID", "Int32", 1, h + L.sclump, 4 * n},
+                              {"velocity", "Float32", 3, h + L.svel, 12 * n},
+                              {"angularVelocity", "Float32", 3, h + L.sw, 12 * n},
+                              {"normal", "Float32", 3, h + L.snrm, 12 * n}},
+                             "radius");
+    }
+    if (species & DSPH_FRAME_FLUID_SOLID) {
+        long long np = 0;
+        std::memcpy(&np, h + L.head, 8);
+        if (np < 0) np = 0;  // the list had overflowed: no pairs
+        const size_t n = (size_t)np;
+        ok = ok && write_vtu(base + "fluid2Solid_" + num + ".vtu", F.time, F.step, np, (const float*)(h + L.mid),
+                             {{"force", "Float32", 3, h + L.force, 12 * n},
+                              {"fluidID", "Int32", 1, h + L.fid, 4 * n},
+                              {"solidID", "Int32", 1, h + L.sid, 4 * n}},
+                             "force");
+    }
+    F.pending = false;
+    return ok ? DSPH_OK : DSPH_E_IO;
 }
 
 // ------------------------------------------------------------- read-back

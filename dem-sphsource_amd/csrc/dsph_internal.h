@@ -477,6 +477,20 @@ void fluid_solid_force(hipStream_t s, const FluidArgs& a, bool mixed);
 // the solids' mixed fluid–solid records (Solids::mrec) from their state
 void solid_records(hipStream_t s, const Solids& so, int ns, const FluidSet& f);
 
+// Frames (k_frame.hip): the arrays of one VTU frame in a device staging
+// block, FP32 (Int32 for indices), caller order
+struct FrameOut {
+    float *fpos, *fh, *fvel, *frho, *fp;                     // fluid: 3, 1, 3, 1, 1 per particle
+    float *spos, *srad;                                      // solid: 3, 1
+    int *sclus, *sclump;                                     //        1, 1
+    float *svel, *sw, *snrm;                                 //        3, 3, 3
+    float *mid, *force;                                      // pair:  3, 3
+    int *fid, *sid;                                          //        1, 1
+    long long* head;                                         // [0]: pairs packed (-1: list overflowed)
+};
+void frame_pack(hipStream_t s, const FluidSet& f, int nf, int cur, const Solids& so, int ns, const Lists& L,
+                int64_t pair_cap, const FrameOut& o);
+
 // DEM (k_dem.hip)
 struct DemArgs {
     Solids so;

@@ -287,7 +287,20 @@ def test_cpp_dam_break_case_matches_python_path(tmp_path):
              "clump angvel"]
     for name, a, b in zip(names, mine, arrays):
         assert np.array_equal(a, b), (name, float(np.max(np.abs(a - b))))
-    assert os.path.isdir(tmp_path / "damBreakOutput")
+    # the case's outputData() asks for fluid and solid frames: frame 0 after
+    # initialisation and frame 1 at the last step, written by the driver's
+    # writer threads; frame 1 holds the final state in FP32
+    from test_gpu_frames import read_vtu
+    d = tmp_path / "damBreakOutput"
+    assert sorted(os.listdir(d)) == ["fluid_0000.vtu", "fluid_0001.vtu", "solid_0000.vtu",
+                                     "solid_0001.vtu"]
+    fl, so = read_vtu(d / "fluid_0001.vtu"), read_vtu(d / "solid_0001.vtu")
+    assert fl["STEP"][0] == steps and so["STEP"][0] == steps
+    assert np.array_equal(fl["points"], arrays[0].astype(np.float32))
+    assert np.array_equal(fl["velocity"], arrays[1].astype(np.float32))
+    assert np.array_equal(so["points"], arrays[2].astype(np.float32))
+    assert np.array_equal(so["velocity"], arrays[3].astype(np.float32))
+    assert read_vtu(d / "fluid_0000.vtu")["STEP"][0] == 0
 
 
 # ----------------------------------------------------------- cylinder walls

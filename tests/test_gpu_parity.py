@@ -447,7 +447,7 @@ def test_cpp_case_driver_matches_python_path(tmp_path):
     steps = 12
     note("c++ case driver coupledDamBreak dx=0.04")
     r = subprocess.run([exe, "0.04", str(steps), "4", "3", "3", str(dsph.PREC_FP64), str(out)],
-                       capture_output=True, text=True, timeout=120)
+                       cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     raw = out.read_bytes()
     nf, ns = (int(v) for v in np.frombuffer(raw[:8], np.int32))
