@@ -12,9 +12,10 @@ Prints one JSON line.  Timed: --steps steps replayed from HIP graphs after
 for the all-FP64 path.  Then the same window runs again from a checkpoint
 of its first step, eagerly, with HIP events around every kernel family on
 the context stream: those durations give the dominant kernel's roofline.
-A 1000-step window of both precisions (superset rebuilds amortised) and a
-CPU baseline (the FP64 OpenMP oracle on a bounded sample) are reported
-beside it.
+A 1000-step window of both precisions (superset rebuilds amortised), a
+CPU baseline (the FP64 OpenMP oracle on a bounded sample) and the VTU frames
+leg (what writing frames costs the step loop, against the reference's
+synchronous ASCII output of the same frame) are reported beside it.
 """
 import argparse
 import json
@@ -207,6 +208,133 @@ def cpu_baseline(scene, sample_steps, runs):
                       f"(rates {[round(r) for r in rates]})"}
 
 
+def _dir_bytes(d):
+    return sum(os.path.getsize(os.path.join(d, f)) for f in os.listdir(d))
+
+
+def frames_leg(scene, tuning, args):
+    """VTU frames of the same workload, mixed precision.  The step loop
+    (chunks of --frame-every graph-replayed steps) runs twice from one
+    checkpoint: without frames, then with a frame of every species after
+    each chunk (dsph_frame_begin, and dsph_frame_write on a writer thread
+    per slot, as the C++ driver does): the difference is what a frame costs
+    the loop.  Then one frame timed alone, and the reference's way of
+    writing the same frame (solverBase.cpp:69-333: the FP64 state read back
+    synchronously, then every value printed as ASCII, by the oracle's
+    restatement of its writers) on the host."""
+    import shutil
+    import tempfile
+    import threading
+
+    import dsph
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    dt, every, nfr = scene.dt, args.frame_every, args.frames
+    g = new_solver(scene, dsph.PREC_MIXED, tuning=tuning)
+    g.run(args.warmup, dt, first=1, graphs=True)
+    blob = g.save_state()
+    first = args.warmup + 1
+
+    def loop(frames_dir):
+        # from the checkpoint (which drops the step graphs), one untimed
+        # chunk records the graphs; then nfr timed chunks, each followed by
+        # a frame when frames_dir is given: both loops do the same steps
+        g.load_state(blob)
+        g.run(every, dt, first=first, graphs=True)
+        g.synchronize()
+        writers, wtimes = [None, None], []
+
+        def write(slot, k):
+            t = time.perf_counter()
+            g.frame_write(slot, frames_dir, k)
+            wtimes.append(time.perf_counter() - t)
+
+        t0 = time.perf_counter()
+        for k in range(1, nfr + 1):
+            g.run(every, dt, first=first + k * every, graphs=True)
+            if frames_dir:
+                slot = k % dsph.FRAME_SLOTS
+                if writers[slot]:
+                    writers[slot].join()
+                step = first + (k + 1) * every - 1
+                g.frame_begin(slot, step, step * dt)
+                writers[slot] = threading.Thread(target=write, args=(slot, k))
+                writers[slot].start()
+        g.synchronize()
+        t_loop = time.perf_counter() - t0
+        for w in writers:
+            if w:
+                w.join()
+        return t_loop, wtimes
+
+    d = tempfile.mkdtemp(prefix="dsph_frames_")
+    try:
+        base, with_f, wtimes = [], [], []
+        for _ in range(2):  # alternated, the faster of each kept
+            base.append(loop(None)[0])
+            tf, wt = loop(d)
+            with_f.append(tf)
+            wtimes += wt
+        base, with_f = min(base), min(with_f)
+        frame_bytes = _dir_bytes(d) / nfr
+        shutil.rmtree(d)
+        os.mkdir(d)
+        # one frame alone: the begin call, then the write (copy wait, pair
+        # copy, files)
+        g.synchronize()
+        t0 = time.perf_counter()
+        g.frame_begin(0, first, first * dt)
+        t1 = time.perf_counter()
+        g.frame_write(0, d, 0)
+        t2 = time.perf_counter()
+        ns_pairs = len(g.list(dsph.LIST_FLUID_SOLID)[0])
+        d2h = 36 * scene.n_fluid + 60 * scene.n_solid + 32 * ns_pairs
+        # the reference's way: FP64 read-back of what its writers print,
+        # then the ASCII files
+        t3 = time.perf_counter()
+        fl, so = g.fluid(), g.solid()
+        fi, sj, _ = g.list(dsph.LIST_FLUID_SOLID)
+        F = g.fs_force()
+        t4 = time.perf_counter()
+        ns = scene.n_solid
+        clus = scene.solid_cluster if scene.solid_cluster is not None else -np.ones(ns)
+        clump = scene.solid_clump if scene.solid_clump is not None else -np.ones(ns)
+        ref = os.path.join(d, "ref")
+        os.mkdir(ref)
+        oracle.vtu_fluid_ascii(os.path.join(ref, "fluid_0000.vtu"), first * dt, first, fl["pos"], scene.fluid_h,
+                               fl["vel"], scene.fluid_rho0, fl["drho"], scene.fluid_c)
+        oracle.vtu_solid_ascii(os.path.join(ref, "solid_0000.vtu"), first * dt, first, so["pos"],
+                               scene.solid_radius, clus, clump, so["vel"], so["angvel"], so["normal"])
+        oracle.vtu_fluid_solid_ascii(os.path.join(ref, "fluid2Solid_0000.vtu"), first * dt, first, fi, sj,
+                                     fl["pos"], so["pos"], F)
+        t5 = time.perf_counter()
+        ascii_bytes = _dir_bytes(ref)
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+        g.close()
+    per_frame = (with_f - base) / nfr
+    out = {
+        "every_steps": every, "frames": nfr, "species": "fluid, solid, fluid2Solid",
+        "frame_bytes": int(frame_bytes), "d2h_bytes": d2h, "fluid_solid_pairs": ns_pairs,
+        "loop_ms_per_step_without": round(base / (nfr * every) * 1e3, 4),
+        "loop_ms_per_step_with": round(with_f / (nfr * every) * 1e3, 4),
+        "loop_ms_per_frame": round(per_frame * 1e3, 3),
+        "write_ms_on_thread": round(statistics.median(wtimes) * 1e3, 2),
+        "alone": {"begin_ms": round((t1 - t0) * 1e3, 3), "write_ms": round((t2 - t1) * 1e3, 2),
+                  "MB_per_s": round(frame_bytes / (t2 - t0) / 1e6, 1)},
+        "reference_way": {"readback_ms": round((t4 - t3) * 1e3, 2), "ascii_write_ms": round((t5 - t4) * 1e3, 1),
+                          "frame_bytes": int(ascii_bytes), "kind": "port",
+                          "note": "FP64 read-back through the C ABI, then the oracle's restatement of "
+                                  "the reference's ASCII writers on one host thread; the GPU waits"},
+        "files": "written to a temporary directory and removed",
+    }
+    print(f"# frames: {out['loop_ms_per_frame']} ms of step loop per frame (every {every} steps), "
+          f"write {out['write_ms_on_thread']} ms on the writer thread, {frame_bytes / 1e6:.1f} MB; "
+          f"reference way {out['reference_way']['readback_ms'] + out['reference_way']['ascii_write_ms']:.0f} ms, "
+          f"{ascii_bytes / 1e6:.1f} MB", file=sys.stderr, flush=True)
+    return out
+
+
 def tune(work, precision, flags, args):
     """The autotuner's choice for one precision (None when disabled; the
     --tuning configuration when given)."""
@@ -305,6 +433,11 @@ def main():
     ap.add_argument("--cpu-sample-steps", type=int, default=4)
     ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-frames", action="store_true", help="skip the VTU frames leg")
+    ap.add_argument("--frames-only", action="store_true",
+                    help="only the VTU frames leg, printed as its own JSON line (profiler runs)")
+    ap.add_argument("--frame-every", type=int, default=100, help="steps between frames in the frames leg")
+    ap.add_argument("--frames", type=int, default=4, help="frames in the frames leg")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-timing", action="store_true",
                     help="skip the eager re-run of the window with per-kernel HIP events")
@@ -350,6 +483,10 @@ def main():
 
     if args.fp64_only:
         fp64_only(scene, args)
+        return
+    if args.frames_only:
+        tn = tune(scene, dsph.PREC_MIXED, 0, args)
+        print(json.dumps({"frames": frames_leg(scene, tn, args)}), flush=True)
         return
 
     tun = tune(scene, dsph.PREC_MIXED, 0, args)
@@ -403,6 +540,8 @@ def main():
         if "fp64" in steady:
             steady["mixed_over_fp64"] = round(steady["fp64"]["ms_per_step"] / steady["mixed"]["ms_per_step"], 3)
 
+    frames = None if args.no_frames else frames_leg(scene, tun, args)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(scene, args.cpu_sample_steps, args.cpu_runs)
@@ -435,6 +574,7 @@ def main():
         "candidates": cand,
         "tiles": tiles,
         "cpu_baseline": cpu,
+        "frames": frames,
     }
     print(json.dumps(out), flush=True)
 

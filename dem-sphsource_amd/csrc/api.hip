@@ -168,20 +168,22 @@ struct dsph_ctx {
     // work buffer of dsph_hash_sort (grown on demand)
     unsigned* hs_work = nullptr;
     int64_t hs_words = 0;
-    // VTU frames (dsph_frame_begin / dsph_frame_write): one staging block on
-    // the device, and per slot its pinned host copy, kept across uploads
+    // VTU frames (dsph_frame_begin / dsph_frame_write): per slot a staging
+    // block on the device and its pinned host copy (kept across uploads),
+    // copied on a stream of their own
     struct FrameSlot {
+        void* dev = nullptr;   // in allocs
+        size_t dev_cap = 0;
         void* host = nullptr;  // pinned (hipHostMalloc)
         size_t cap = 0;        // bytes allocated
-        size_t bytes = 0;      // bytes of the pending frame
-        hipEvent_t ev = nullptr;
+        hipEvent_t packed = nullptr, copied = nullptr;
         bool pending = false;
+        uint32_t species = 0;
         int32_t step = 0;
         double time = 0.0;
         int64_t nf = 0, ns = 0, pcap = 0;
     } frames[DSPH_FRAME_SLOTS];
-    void* frame_dev = nullptr;  // in allocs
-    size_t frame_dev_bytes = 0;
+    hipStream_t cst = nullptr;  // the frames' copy stream (created at the first frame)
 
     bool mixed() const { return prec == DSPH_PREC_MIXED; }
 };
@@ -224,6 +226,7 @@ void free_all(dsph_ctx* c) {
         if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
     }
     c->graphs.clear();
+    if (c->cst) (void)hipStreamSynchronize(c->cst);  // frame copies read staging blocks
     for (void* p : c->allocs) (void)hipFree(p);
     c->allocs.clear();
     c->T = Tiles{};
@@ -232,9 +235,11 @@ void free_all(dsph_ctx* c) {
     c->backup_bytes.clear();
     c->hs_work = nullptr;
     c->hs_words = 0;
-    c->frame_dev = nullptr;
-    c->frame_dev_bytes = 0;
-    for (auto& f : c->frames) f.pending = false;
+    for (auto& f : c->frames) {
+        f.dev = nullptr;
+        f.dev_cap = 0;
+        f.pending = false;
+    }
 }
 
 int fail(dsph_ctx* c, const ApiError& e) {
@@ -1474,8 +1479,10 @@ int dsph_destroy(dsph_ctx* c) {
         }
     for (auto& f : c->frames) {
         if (f.host) (void)hipHostFree(f.host);
-        if (f.ev) (void)hipEventDestroy(f.ev);
+        if (f.packed) (void)hipEventDestroy(f.packed);
+        if (f.copied) (void)hipEventDestroy(f.copied);
     }
+    if (c->cst) (void)hipStreamDestroy(c->cst);
     (void)hipStreamDestroy(c->st);
     delete c;
     return DSPH_OK;
@@ -2615,13 +2622,14 @@ int dsph_add_global_damping(dsph_ctx* c, int32_t idx, double cd) {
 
 // ------------------------------------------------------------------ frames
 namespace {
-// byte layout of a frame's staging block: the arrays of FrameOut, each
-// 256-byte aligned, then the pair count
+// byte layout of a frame's staging block: the arrays of the species asked
+// for, each 256-byte aligned, the pair count, then the pair arrays (n pairs
+// packed consecutively in room for pcap); the part before the pairs is
+// copied by dsph_frame_begin, the 32 n bytes of pairs by dsph_frame_write
 struct FrameLayout {
-    size_t fpos, fh, fvel, frho, fp, spos, srad, sclus, sclump, svel, sw, snrm, mid, force, fid, sid, head,
-        bytes;
+    size_t fpos, fh, fvel, frho, fp, spos, srad, sclus, sclump, svel, sw, snrm, head, pairs, bytes;
 };
-static FrameLayout frame_layout(int64_t nf, int64_t ns, int64_t pcap) {
+static FrameLayout frame_layout(int64_t nf, int64_t ns, int64_t pcap, uint32_t species) {
     FrameLayout L{};
     size_t at = 0;
     auto take = [&](size_t n) {
@@ -2629,23 +2637,23 @@ static FrameLayout frame_layout(int64_t nf, int64_t ns, int64_t pcap) {
         at += (n + 255) / 256 * 256;
         return o;
     };
-    L.fpos = take(12 * nf);
-    L.fh = take(4 * nf);
-    L.fvel = take(12 * nf);
-    L.frho = take(4 * nf);
-    L.fp = take(4 * nf);
-    L.spos = take(12 * ns);
-    L.srad = take(4 * ns);
-    L.sclus = take(4 * ns);
-    L.sclump = take(4 * ns);
-    L.svel = take(12 * ns);
-    L.sw = take(12 * ns);
-    L.snrm = take(12 * ns);
-    L.mid = take(12 * pcap);
-    L.force = take(12 * pcap);
-    L.fid = take(4 * pcap);
-    L.sid = take(4 * pcap);
+    const size_t f = (species & DSPH_FRAME_FLUID) ? (size_t)nf : 0;
+    const size_t so = (species & DSPH_FRAME_SOLID) ? (size_t)ns : 0;
+    const size_t p = (species & DSPH_FRAME_FLUID_SOLID) ? (size_t)pcap : 0;
+    L.fpos = take(12 * f);
+    L.fh = take(4 * f);
+    L.fvel = take(12 * f);
+    L.frho = take(4 * f);
+    L.fp = take(4 * f);
+    L.spos = take(12 * so);
+    L.srad = take(4 * so);
+    L.sclus = take(4 * so);
+    L.sclump = take(4 * so);
+    L.svel = take(12 * so);
+    L.sw = take(12 * so);
+    L.snrm = take(12 * so);
     L.head = take(8);
+    L.pairs = take(32 * p);
     L.bytes = at;
     return L;
 }
@@ -2722,22 +2730,26 @@ static bool write_vtu(const std::string& path, double time, int32_t step, int64_
 }
 }  // namespace
 
-int dsph_frame_begin(dsph_ctx* c, int32_t slot, int32_t i_step, double time) {
+int dsph_frame_begin(dsph_ctx* c, int32_t slot, int32_t i_step, double time, uint32_t species) {
     API_GUARD(c);
-    if (slot < 0 || slot >= DSPH_FRAME_SLOTS) return DSPH_E_INVALID;
+    if (slot < 0 || slot >= DSPH_FRAME_SLOTS || species == 0 || (species & ~(uint32_t)DSPH_FRAME_ALL))
+        return DSPH_E_INVALID;
     try {
         auto& F = c->frames[slot];
         const int64_t nf = c->nf, ns = c->ns;
         const int64_t pcap = (nf > 0 && ns > 0) ? c->L.fs_cap : 0;
-        const FrameLayout L = frame_layout(nf, ns, pcap);
-        // this slot's previous copy has landed before its buffer is reused
-        if (F.ev) HIPCHK(hipEventSynchronize(F.ev));
-        if (c->frame_dev_bytes < L.bytes) {
-            // (a copy of the other slot may still read the old block)
-            HIPCHK(hipStreamSynchronize(c->st));
-            dfree(c, c->frame_dev);
-            c->frame_dev = dalloc<char>(c, L.bytes);
-            c->frame_dev_bytes = L.bytes;
+        const FrameLayout L = frame_layout(nf, ns, pcap, species);
+        // this slot's previous copy has landed before its blocks are reused
+        if (F.copied) HIPCHK(hipEventSynchronize(F.copied));
+        if (!c->cst) HIPCHK(hipStreamCreateWithFlags(&c->cst, hipStreamNonBlocking));
+        if (!F.packed) HIPCHK(hipEventCreateWithFlags(&F.packed, hipEventDisableTiming));
+        if (!F.copied) HIPCHK(hipEventCreateWithFlags(&F.copied, hipEventDisableTiming));
+        if (F.dev_cap < L.bytes) {
+            dfree(c, F.dev);
+            F.dev = nullptr;
+            F.dev_cap = 0;
+            F.dev = dalloc<char>(c, L.bytes);
+            F.dev_cap = L.bytes;
         }
         if (F.cap < L.bytes) {
             if (F.host) HIPCHK(hipHostFree(F.host));
@@ -2747,8 +2759,7 @@ int dsph_frame_begin(dsph_ctx* c, int32_t slot, int32_t i_step, double time) {
                 throw ApiError{DSPH_E_NOMEM, "pinned host memory for a frame"};
             F.cap = L.bytes;
         }
-        if (!F.ev) HIPCHK(hipEventCreateWithFlags(&F.ev, hipEventDisableTiming));
-        char* d = (char*)c->frame_dev;
+        char* d = (char*)F.dev;
         FrameOut o;
         o.fpos = (float*)(d + L.fpos);
         o.fh = (float*)(d + L.fh);
@@ -2762,16 +2773,17 @@ int dsph_frame_begin(dsph_ctx* c, int32_t slot, int32_t i_step, double time) {
         o.svel = (float*)(d + L.svel);
         o.sw = (float*)(d + L.sw);
         o.snrm = (float*)(d + L.snrm);
-        o.mid = (float*)(d + L.mid);
-        o.force = (float*)(d + L.force);
-        o.fid = (int*)(d + L.fid);
-        o.sid = (int*)(d + L.sid);
         o.head = (long long*)(d + L.head);
-        // the records of buffer dcur hold the fluid's current dRho
-        frame_pack(c->st, c->fset[0], (int)nf, c->dcur, c->so, (int)ns, c->L, pcap, o);
-        HIPCHK(hipMemcpyAsync(F.host, c->frame_dev, L.bytes, hipMemcpyDeviceToHost, c->st));
-        HIPCHK(hipEventRecord(F.ev, c->st));
-        F.bytes = L.bytes;
+        o.pairs = (float*)(d + L.pairs);
+        // the records of buffer dcur hold the fluid's current dRho; the
+        // packing is on the context stream, its copy on the copy stream, so
+        // the steps that follow run beside the copy
+        frame_pack(c->st, species, c->fset[0], (int)nf, c->dcur, c->so, (int)ns, c->L, pcap, o);
+        HIPCHK(hipEventRecord(F.packed, c->st));
+        HIPCHK(hipStreamWaitEvent(c->cst, F.packed, 0));
+        HIPCHK(hipMemcpyAsync(F.host, F.dev, L.pairs, hipMemcpyDeviceToHost, c->cst));
+        HIPCHK(hipEventRecord(F.copied, c->cst));
+        F.species = species;
         F.step = i_step;
         F.time = time;
         F.nf = nf;
@@ -2784,18 +2796,31 @@ int dsph_frame_begin(dsph_ctx* c, int32_t slot, int32_t i_step, double time) {
     }
 }
 
-int dsph_frame_write(dsph_ctx* c, int32_t slot, const char* dir, int32_t frame, uint32_t species) {
+int dsph_frame_write(dsph_ctx* c, int32_t slot, const char* dir, int32_t frame) {
     if (!c || !dir || slot < 0 || slot >= DSPH_FRAME_SLOTS || frame < 0) return DSPH_E_INVALID;
     auto& F = c->frames[slot];
     if (!F.pending) return DSPH_E_STATE;
-    if (hipEventSynchronize(F.ev) != hipSuccess) return DSPH_E_HIP;
-    const FrameLayout L = frame_layout(F.nf, F.ns, F.pcap);
-    const char* h = (const char*)F.host;
+    if (hipEventSynchronize(F.copied) != hipSuccess) return DSPH_E_HIP;
+    const FrameLayout L = frame_layout(F.nf, F.ns, F.pcap, F.species);
+    char* const h = (char*)F.host;
+    long long np = 0;
+    if (F.species & DSPH_FRAME_FLUID_SOLID) {
+        std::memcpy(&np, h + L.head, 8);
+        if (np < 0) np = 0;  // the list had overflowed: no pairs
+        if (np > 0) {
+            // the pairs the device packed, and no more (the slot's staging
+            // block is not packed again before this call returns)
+            if (hipMemcpyAsync(h + L.pairs, (const char*)F.dev + L.pairs, 32 * (size_t)np, hipMemcpyDeviceToHost,
+                               c->cst) != hipSuccess ||
+                hipEventRecord(F.copied, c->cst) != hipSuccess || hipEventSynchronize(F.copied) != hipSuccess)
+                return DSPH_E_HIP;
+        }
+    }
     char num[16];
     std::snprintf(num, sizeof(num), "%04d", (int)frame);
     const std::string base = std::string(dir) + "/";
     bool ok = true;
-    if (species & DSPH_FRAME_FLUID) {
+    if (F.species & DSPH_FRAME_FLUID) {
         const size_t n = (size_t)F.nf;
         ok = ok && write_vtu(base + "fluid_" + num + ".vtu", F.time, F.step, F.nf, (const float*)(h + L.fpos),
                              {{"smooth length", "Float32", 1, h + L.fh, 4 * n},
@@ -2804,7 +2829,7 @@ int dsph_frame_write(dsph_ctx* c, int32_t slot, const char* dir, int32_t frame, 
                               {"pressure", "Float32", 1, h + L.fp, 4 * n}},
                              "smooth length");
     }
-    if (species & DSPH_FRAME_SOLID) {
+    if (F.species & DSPH_FRAME_SOLID) {
         const size_t n = (size_t)F.ns;
         ok = ok && write_vtu(base + "solid_" + num + ".vtu", F.time, F.step, F.ns, (const float*)(h + L.spos),
                              {{"radius", "Float32", 1, h + L.srad, 4 * n},
@@ -2815,15 +2840,16 @@ int dsph_frame_write(dsph_ctx* c, int32_t slot, const char* dir, int32_t frame, 
                               {"normal", "Float32", 3, h + L.snrm, 12 * n}},
                              "radius");
     }
-    if (species & DSPH_FRAME_FLUID_SOLID) {
-        long long np = 0;
-        std::memcpy(&np, h + L.head, 8);
-        if (np < 0) np = 0;  // the list had overflowed: no pairs
+    if (F.species & DSPH_FRAME_FLUID_SOLID) {
         const size_t n = (size_t)np;
-        ok = ok && write_vtu(base + "fluid2Solid_" + num + ".vtu", F.time, F.step, np, (const float*)(h + L.mid),
-                             {{"force", "Float32", 3, h + L.force, 12 * n},
-                              {"fluidID", "Int32", 1, h + L.fid, 4 * n},
-                              {"solidID", "Int32", 1, h + L.sid, 4 * n}},
+        const char* mid = h + L.pairs;
+        const char* force = mid + 12 * n;
+        const char* fid = force + 12 * n;
+        const char* sid = fid + 4 * n;
+        ok = ok && write_vtu(base + "fluid2Solid_" + num + ".vtu", F.time, F.step, np, (const float*)mid,
+                             {{"force", "Float32", 3, force, 12 * n},
+                              {"fluidID", "Int32", 1, fid, 4 * n},
+                              {"solidID", "Int32", 1, sid, 4 * n}},
                              "force");
     }
     F.pending = false;

@@ -484,12 +484,13 @@ struct FrameOut {
     float *spos, *srad;                                      // solid: 3, 1
     int *sclus, *sclump;                                     //        1, 1
     float *svel, *sw, *snrm;                                 //        3, 3, 3
-    float *mid, *force;                                      // pair:  3, 3
-    int *fid, *sid;                                          //        1, 1
-    long long* head;                                         // [0]: pairs packed (-1: list overflowed)
+    float* pairs;                                            // pair:  3, 3, then (int) 1, 1, as
+                                                             //        consecutive arrays of the n pairs
+    long long* head;                                         // [0]: n, pairs packed (-1: list overflowed)
 };
-void frame_pack(hipStream_t s, const FluidSet& f, int nf, int cur, const Solids& so, int ns, const Lists& L,
-                int64_t pair_cap, const FrameOut& o);
+// packs the species (DSPH_FRAME_* bits) of the current state
+void frame_pack(hipStream_t s, unsigned species, const FluidSet& f, int nf, int cur, const Solids& so, int ns,
+                const Lists& L, int64_t pair_cap, const FrameOut& o);
 
 // DEM (k_dem.hip)
 struct DemArgs {

@@ -4,9 +4,9 @@
 // outputFluid2SolidVTU, solverBase.cpp:69-333) copies the whole device state
 // into its host mirrors and prints every value.  A frame here is packed by
 // three kernels into one staging block of FP32 arrays (Int32 for indices) in
-// the caller's order, which the host copies in one transfer: 36 B per fluid
-// particle, 60 B per solid and 32 B per fluid–solid pair, against 64–128 B
-// of FP64 state per particle.
+// the caller's order, which the host copies off the compute stream: 36 B per
+// fluid particle, 60 B per solid and 32 B per fluid–solid pair, against
+// 64–128 B of FP64 state per particle.
 #include "dsph_internal.h"
 
 namespace dsph {
@@ -56,38 +56,48 @@ __global__ void __launch_bounds__(kBS) k_frame_solid(Solids so, int ns, FrameOut
     o.snrm[k + 2] = (float)n.z;
 }
 
-// fluid–solid pairs of the current list (at most cap of them are packed;
-// head[0] = the pairs packed, or -1 when the list overflowed its storage)
+// fluid–solid pairs of the current list, packed as consecutive arrays of
+// the n pairs (midpoint[3n], force[3n], fluid index[n], solid index[n]) so
+// that the host copies exactly 32 n bytes; at most cap pairs fit (head[0] =
+// n, or -1 when the list overflowed its storage)
 __global__ void __launch_bounds__(kBS) k_frame_pairs(FluidSet f, Solids so, Lists L, int nf, int64_t cap,
                                                      FrameOut o) {
     const int64_t tot = nf > 0 && cap > 0 ? L.fsoff[nf] : 0;
     const int64_t n = tot <= cap ? tot : -1;
     if (blockIdx.x == 0 && threadIdx.x == 0) o.head[0] = n;
+    float* const mid = o.pairs;
+    float* const force = mid + 3 * (n > 0 ? n : 0);
+    int* const fid = (int*)(force + 3 * (n > 0 ? n : 0));
+    int* const sid = fid + (n > 0 ? n : 0);
     for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n;
          k += (int64_t)gridDim.x * blockDim.x) {
         const int s = L.fs_own[k], a = L.fs_a[k];
         const double4 p = f.ph[s], q = so.pr[a];
         const double4 F = L.fsF[k];
-        o.mid[3 * k] = (float)(0.5 * (p.x + q.x));
-        o.mid[3 * k + 1] = (float)(0.5 * (p.y + q.y));
-        o.mid[3 * k + 2] = (float)(0.5 * (p.z + q.z));
-        o.force[3 * k] = (float)F.x;
-        o.force[3 * k + 1] = (float)F.y;
-        o.force[3 * k + 2] = (float)F.z;
-        o.fid[k] = f.orig[s];
-        o.sid[k] = a;
+        mid[3 * k] = (float)(0.5 * (p.x + q.x));
+        mid[3 * k + 1] = (float)(0.5 * (p.y + q.y));
+        mid[3 * k + 2] = (float)(0.5 * (p.z + q.z));
+        force[3 * k] = (float)F.x;
+        force[3 * k + 1] = (float)F.y;
+        force[3 * k + 2] = (float)F.z;
+        fid[k] = f.orig[s];
+        sid[k] = a;
     }
 }
 
 }  // namespace
 
-void frame_pack(hipStream_t s, const FluidSet& f, int nf, int cur, const Solids& so, int ns, const Lists& L,
-                int64_t pair_cap, const FrameOut& o) {
-    if (nf > 0) hipLaunchKernelGGL(k_frame_fluid, dim3(blocks_for(nf)), dim3(kBS), 0, s, f, nf, cur, o);
-    if (ns > 0) hipLaunchKernelGGL(k_frame_solid, dim3(blocks_for(ns)), dim3(kBS), 0, s, so, ns, o);
-    const int64_t g = pair_cap > 0 ? (pair_cap + kBS - 1) / kBS : 1;
-    hipLaunchKernelGGL(k_frame_pairs, dim3((unsigned)(g < 4096 ? g : 4096)), dim3(kBS), 0, s, f, so, L, nf,
-                       pair_cap, o);
+void frame_pack(hipStream_t s, unsigned species, const FluidSet& f, int nf, int cur, const Solids& so, int ns,
+                const Lists& L, int64_t pair_cap, const FrameOut& o) {
+    if ((species & DSPH_FRAME_FLUID) && nf > 0)
+        hipLaunchKernelGGL(k_frame_fluid, dim3(blocks_for(nf)), dim3(kBS), 0, s, f, nf, cur, o);
+    if ((species & DSPH_FRAME_SOLID) && ns > 0)
+        hipLaunchKernelGGL(k_frame_solid, dim3(blocks_for(ns)), dim3(kBS), 0, s, so, ns, o);
+    if (species & DSPH_FRAME_FLUID_SOLID) {
+        const int64_t g = pair_cap > 0 ? (pair_cap + kBS - 1) / kBS : 1;
+        hipLaunchKernelGGL(k_frame_pairs, dim3((unsigned)(g < 4096 ? g : 4096)), dim3(kBS), 0, s, f, so, L, nf,
+                           pair_cap, o);
+    }
 }
 
 }  // namespace dsph

@@ -69,8 +69,8 @@ _SIGS = {
     "dsph_download_solid": [_dp] * 6,
     "dsph_download_clump": [_dp] * 6,
     "dsph_download_fluid_solid_force": [_dp, C.c_int64, _i64p],
-    "dsph_frame_begin": [C.c_int32, C.c_int32, C.c_double],
-    "dsph_frame_write": [C.c_int32, C.c_char_p, C.c_int32, C.c_uint32],
+    "dsph_frame_begin": [C.c_int32, C.c_int32, C.c_double, C.c_uint32],
+    "dsph_frame_write": [C.c_int32, C.c_char_p, C.c_int32],
     "dsph_download_contacts": [_dp] * 5 + [C.c_int64, _i64p],
     "dsph_download_bonds": [_dp] * 4 + [_ip, C.c_int64, _i64p],
     "dsph_download_list": [C.c_int, _ip, _ip, C.c_int64, _i64p, _ip],
@@ -89,6 +89,7 @@ _SIGS = {
 EXPORTS = ["dsph_create", "dsph_destroy", "dsph_last_error", "dsph_version"] + list(_SIGS)
 
 FRAME_FLUID, FRAME_SOLID, FRAME_FLUID_SOLID, FRAME_ALL = 1, 2, 4, 7
+FRAME_SLOTS = 2
 
 
 class DsphError(RuntimeError):
@@ -332,17 +333,17 @@ class Solver:
         return f[: n.value]
 
     # ---- frames (binary appended VTU, include/dsph.h)
-    def frame_begin(self, slot, i_step, time):
-        """Packs the current state into frame slot `slot` and queues its copy
-        to the host; returns at once (dsph_frame_begin)."""
-        self._call("dsph_frame_begin", slot, i_step, time)
+    def frame_begin(self, slot, i_step, time, species=FRAME_ALL):
+        """Packs the species of the current state into frame slot `slot` and
+        queues their copy to the host; returns at once (dsph_frame_begin)."""
+        self._call("dsph_frame_begin", slot, i_step, time, species)
 
-    def frame_write(self, slot, directory, frame, species=FRAME_ALL):
-        """Writes the frame of `slot` as fluid_NNNN.vtu, solid_NNNN.vtu and
-        fluid2Solid_NNNN.vtu into `directory` (dsph_frame_write; may run on
-        another thread while this solver steps)."""
-        rc = self.L.dsph_frame_write(self.h, slot, os.fsencode(directory), frame, species)
-        if rc < 0:
+    def frame_write(self, slot, directory, frame):
+        """Writes the frame of `slot` (fluid_NNNN.vtu, solid_NNNN.vtu,
+        fluid2Solid_NNNN.vtu, the species it was begun with) into
+        `directory` (dsph_frame_write; may run on another thread)."""
+        rc = self.L.dsph_frame_write(self.h, slot, os.fsencode(directory), frame)
+        if rc != 0:
             raise DsphError("dsph_frame_write", rc, "frame slot empty" if rc == -4 else "file not written")
 
     def contacts(self):

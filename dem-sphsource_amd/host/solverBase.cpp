@@ -65,8 +65,8 @@ void solverBase::update() {
 // The reference's writers (solverBase.cpp:69-333) each copy the device state
 // to the host mirrors and print one ASCII file while the GPU waits.  Here each
 // adds its species to the frame; when outputData() returns the frame is
-// packed on the device and its copy to pinned host memory queued
-// (dsph_frame_begin), and a writer thread writes the binary VTU files
+// packed on the device and its copy to pinned host memory queued beside the
+// steps (dsph_frame_begin), and a writer thread writes the binary VTU files
 // (dsph_frame_write) while the steps go on.  The two frame slots alternate;
 // a slot's previous writer is joined before the slot is used again.
 void solverBase::outputFluidVTU() { frameSpecies_ |= DSPH_FRAME_FLUID; }
@@ -77,13 +77,12 @@ void solverBase::flushFrame() {
     if (!frameSpecies_ || !dev) return;
     const int slot = simPara.iFrame % DSPH_FRAME_SLOTS;
     if (writers_[slot].joinable()) writers_[slot].join();
-    check(dsph_frame_begin(dev, slot, simPara.iStep, simPara.iStep * simPara.timeStep));
+    check(dsph_frame_begin(dev, slot, simPara.iStep, simPara.iStep * simPara.timeStep, frameSpecies_));
     dsph_ctx* const d = dev;
-    const uint32_t species = frameSpecies_;
     const int frame = simPara.iFrame;
     const std::string out = dir;
-    writers_[slot] = std::thread([d, slot, out, frame, species] {
-        const int rc = dsph_frame_write(d, slot, out.c_str(), frame, species);
+    writers_[slot] = std::thread([d, slot, out, frame] {
+        const int rc = dsph_frame_write(d, slot, out.c_str(), frame);
         if (rc != DSPH_OK) std::cerr << "frame " << frame << ": VTU files not written (code " << rc << ")\n";
     });
     frameSpecies_ = 0;

@@ -352,12 +352,14 @@ int dsph_download_fluid_solid_force(dsph_ctx* ctx, double* force, int64_t cap,
  * replaces: solverBase::outputFluidVTU, outputSolidVTU and
  * outputFluid2SolidVTU (solverBase.cpp:69-333), which copy the device state
  * to the host mirrors and format every value as ASCII while the GPU waits.
- * Here dsph_frame_begin packs the frame on the device (FP32 values, caller
- * order, the reference's arrays) and queues its copy into pinned host memory
- * on the context's stream, so it returns at once and the caller keeps
- * stepping; dsph_frame_write waits for that copy and writes the files as
- * binary appended VTU (raw encoding, UInt64 block headers), the reference's
- * names and arrays:
+ * dsph_frame_begin packs the species asked for on the device (FP32 values,
+ * caller order, the reference's arrays) on the context's stream and queues
+ * the copy of the fluid and solid arrays into pinned host memory on a copy
+ * stream of its own: it returns at once and the steps that follow run
+ * beside the copy.  dsph_frame_write waits for that copy, copies the
+ * fluid–solid pairs (exactly the pairs of the list, a count the device
+ * knows) and writes the files as binary appended VTU (raw encoding, UInt64
+ * block headers) with the reference's names and arrays:
  *   fluid_NNNN.vtu        points, "smooth length", "velocity", "density"
  *                         (rho0 + dRho), "pressure" (c^2 dRho);
  *   solid_NNNN.vtu        points, "radius", "clusterID", "clumpID",
@@ -367,24 +369,24 @@ int dsph_download_fluid_solid_force(dsph_ctx* ctx, double* force, int64_t cap,
  *                         "fluidID", "solidID" (caller indices), in the
  *                         device list's order (the reference writes its own
  *                         list order, which has no further meaning);
- * each with the FieldData TIME and STEP of dsph_frame_begin.  NNNN is
- * frame, four digits.  species: an OR of DSPH_FRAME_FLUID, DSPH_FRAME_SOLID,
- * DSPH_FRAME_FLUID_SOLID.  A frame lives in one of DSPH_FRAME_SLOTS slots:
- * dsph_frame_write reads only its slot's host copy and touches no other
- * context state, so it may run on another thread while the caller steps
- * (not on the slot dsph_frame_begin is filling, and not across
- * dsph_upload_* / dsph_destroy).  It reports by return code only
- * (dsph_last_error is not updated): DSPH_E_STATE when the slot holds no
- * frame, DSPH_E_IO when a file cannot be written.  No reference
- * counterpart for the split (the reference's output is synchronous).     */
+ * each with the FieldData TIME and STEP given to dsph_frame_begin.  NNNN is
+ * frame, four digits.  species: a nonzero OR of DSPH_FRAME_FLUID,
+ * DSPH_FRAME_SOLID, DSPH_FRAME_FLUID_SOLID.  A frame lives in one of
+ * DSPH_FRAME_SLOTS slots until it is written: dsph_frame_write uses only its
+ * slot, so it may run on another thread while the caller steps and begins
+ * frames in the other slot (not while dsph_frame_begin fills the same slot,
+ * and not across dsph_upload_* / dsph_destroy).  It reports by return code
+ * only (dsph_last_error is not updated): DSPH_E_STATE when the slot holds no
+ * frame, DSPH_E_IO when a file cannot be written.  No reference counterpart
+ * for the split (the reference's output is synchronous).                 */
 #define DSPH_FRAME_SLOTS       2
 #define DSPH_FRAME_FLUID       1
 #define DSPH_FRAME_SOLID       2
 #define DSPH_FRAME_FLUID_SOLID 4
 #define DSPH_FRAME_ALL         7
-int dsph_frame_begin(dsph_ctx* ctx, int32_t slot, int32_t i_step, double time);
-int dsph_frame_write(dsph_ctx* ctx, int32_t slot, const char* dir, int32_t frame,
+int dsph_frame_begin(dsph_ctx* ctx, int32_t slot, int32_t i_step, double time,
                      uint32_t species);
+int dsph_frame_write(dsph_ctx* ctx, int32_t slot, const char* dir, int32_t frame);
 /* solid2Solid per-contact force/torque/springs, aligned with the
  * DSPH_LIST_SOLID_SOLID order; any pointer may be NULL                    */
 int dsph_download_contacts(dsph_ctx* ctx, double* force, double* torque,

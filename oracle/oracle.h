@@ -88,6 +88,22 @@ void   orc_rotate_vector(const double v[3], const double angle[3],
                          double out[3]);
 int    orc_combined_index(int n_material, int a, int b);
 
+/* ASCII VTU frames of host arrays as the reference's writers print them
+ * (solverBase.cpp:69-333, vtu_ascii.cpp); 0, or -1 when the file cannot be
+ * written.  Vectors are [3n] x,y,z. */
+int orc_vtu_fluid_ascii(const char* path, double time, int32_t step, int64_t n,
+                        const double* pos, const double* h, const double* vel,
+                        const double* rho0, const double* drho, const double* c);
+int orc_vtu_solid_ascii(const char* path, double time, int32_t step, int64_t n,
+                        const double* pos, const double* radius,
+                        const int32_t* cluster, const int32_t* clump,
+                        const double* vel, const double* angvel,
+                        const double* normal);
+int orc_vtu_fluid_solid_ascii(const char* path, double time, int32_t step,
+                              int64_t n, const int32_t* fluid,
+                              const int32_t* solid, const double* fluid_pos,
+                              const double* solid_pos, const double* force);
+
 #ifdef __cplusplus
 }
 #endif

@@ -64,6 +64,12 @@ def _load():
     lib.orc_rotate_vector.argtypes = [_dp, _dp, _dp]
     lib.orc_combined_index.argtypes = [C.c_int, C.c_int, C.c_int]
     lib.orc_combined_index.restype = C.c_int
+    head = [C.c_char_p, C.c_double, C.c_int32, C.c_int64]
+    lib.orc_vtu_fluid_ascii.argtypes = head + [_dp] * 6
+    lib.orc_vtu_solid_ascii.argtypes = head + [_dp, _dp, _ip, _ip, _dp, _dp, _dp]
+    lib.orc_vtu_fluid_solid_ascii.argtypes = head + [_ip, _ip, _dp, _dp, _dp]
+    for f in (lib.orc_vtu_fluid_ascii, lib.orc_vtu_solid_ascii, lib.orc_vtu_fluid_solid_ascii):
+        f.restype = C.c_int
     return lib
 
 
@@ -83,6 +89,37 @@ def _p(a, t=_dp):
 
 def vec(a):
     return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _ivec(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _vtu_ok(rc, path):
+    if rc != 0:
+        raise OSError(f"cannot write {path}")
+
+
+# ASCII VTU frames of host arrays, as the reference's writers print them
+# (core/solverBase.cpp:69-333): see vtu_ascii.cpp
+def vtu_fluid_ascii(path, time, step, pos, h, vel, rho0, drho, c):
+    a = [vec(pos), vec(h), vec(vel), vec(rho0), vec(drho), vec(c)]
+    _vtu_ok(lib().orc_vtu_fluid_ascii(str(path).encode(), time, step, len(a[1]), *map(_p, a)), path)
+
+
+def vtu_solid_ascii(path, time, step, pos, radius, cluster, clump, vel, angvel, normal):
+    d = [vec(pos), vec(radius)]
+    i = [_ivec(cluster), _ivec(clump)]
+    v = [vec(vel), vec(angvel), vec(normal)]
+    _vtu_ok(lib().orc_vtu_solid_ascii(str(path).encode(), time, step, len(d[1]), _p(d[0]), _p(d[1]),
+                                      _p(i[0], _ip), _p(i[1], _ip), *map(_p, v)), path)
+
+
+def vtu_fluid_solid_ascii(path, time, step, fluid, solid, fluid_pos, solid_pos, force):
+    i = [_ivec(fluid), _ivec(solid)]
+    d = [vec(fluid_pos), vec(solid_pos), vec(force)]
+    _vtu_ok(lib().orc_vtu_fluid_solid_ascii(str(path).encode(), time, step, len(i[0]), _p(i[0], _ip),
+                                            _p(i[1], _ip), *map(_p, d)), path)
 
 
 class Oracle:

@@ -290,7 +290,7 @@ def test_cpp_dam_break_case_matches_python_path(tmp_path):
     # the case's outputData() asks for fluid and solid frames: frame 0 after
     # initialisation and frame 1 at the last step, written by the driver's
     # writer threads; frame 1 holds the final state in FP32
-    from test_gpu_frames import read_vtu
+    from vtu import read_vtu
     d = tmp_path / "damBreakOutput"
     assert sorted(os.listdir(d)) == ["fluid_0000.vtu", "fluid_0001.vtu", "solid_0000.vtu",
                                      "solid_0001.vtu"]
