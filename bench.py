@@ -280,15 +280,20 @@ def frames_leg(scene, tuning, args):
         shutil.rmtree(d)
         os.mkdir(d)
         # one frame alone: the begin call, then the write (copy wait, pair
-        # copy, files)
+        # copy, files); the packing kernels and the copy timed with HIP
+        # events on their streams
         g.synchronize()
+        g.set_timing(True)
         t0 = time.perf_counter()
         g.frame_begin(0, first, first * dt)
         t1 = time.perf_counter()
         g.frame_write(0, d, 0)
         t2 = time.perf_counter()
+        ft = g.kernel_times()
+        g.set_timing(False)
         ns_pairs = len(g.list(dsph.LIST_FLUID_SOLID)[0])
         d2h = 36 * scene.n_fluid + 60 * scene.n_solid + 32 * ns_pairs
+        d2h_begin = 36 * scene.n_fluid + 60 * scene.n_solid  # copied beside the steps
         # the reference's way: FP64 read-back of what its writers print,
         # then the ASCII files
         t3 = time.perf_counter()
@@ -313,6 +318,13 @@ def frames_leg(scene, tuning, args):
         shutil.rmtree(d, ignore_errors=True)
         g.close()
     per_frame = (with_f - base) / nfr
+    pack_ms, copy_ms = ft["frame_pack"][0], ft["frame_copy"][0]
+    # algorithmic bytes of the packing kernels: per fluid particle 84 B read
+    # (position+h, velocity+rho0, dRho, c, caller index) and 36 B written,
+    # per solid 136 B read (position+radius, velocity, angular velocity,
+    # normal, cluster, clump) and 60 B written, per pair 104 B read (two
+    # indices, both positions, the force) and 32 B written
+    pack_bytes = 120 * scene.n_fluid + 196 * scene.n_solid + 136 * ns_pairs
     out = {
         "every_steps": every, "frames": nfr, "species": "fluid, solid, fluid2Solid",
         "frame_bytes": int(frame_bytes), "d2h_bytes": d2h, "fluid_solid_pairs": ns_pairs,
@@ -321,7 +333,10 @@ def frames_leg(scene, tuning, args):
         "loop_ms_per_frame": round(per_frame * 1e3, 3),
         "write_ms_on_thread": round(statistics.median(wtimes) * 1e3, 2),
         "alone": {"begin_ms": round((t1 - t0) * 1e3, 3), "write_ms": round((t2 - t1) * 1e3, 2),
-                  "MB_per_s": round(frame_bytes / (t2 - t0) / 1e6, 1)},
+                  "MB_per_s": round(frame_bytes / (t2 - t0) / 1e6, 1),
+                  "pack_ms": round(pack_ms, 4), "pack_GB_per_s": round(pack_bytes / pack_ms / 1e6, 1),
+                  "pack_bytes": pack_bytes, "copy_ms": round(copy_ms, 4),
+                  "copy_GB_per_s": round(d2h_begin / copy_ms / 1e6, 1)},
         "reference_way": {"readback_ms": round((t4 - t3) * 1e3, 2), "ascii_write_ms": round((t5 - t4) * 1e3, 1),
                           "frame_bytes": int(ascii_bytes), "kind": "port",
                           "note": "FP64 read-back through the C ABI, then the oracle's restatement of "

@@ -43,10 +43,13 @@ struct StepGraph {
 };
 
 // kernel-family timers (HIP events on the context stream)
-enum Fam { F_GRID = 0, F_FSEARCH, F_FSLIST, F_SSEARCH, F_REINIT, F_CONT, F_MOM, F_FSTREAM, F_DEM, F_NFAM };
+// (the frame copy's on the frames' copy stream)
+enum Fam { F_GRID = 0, F_FSEARCH, F_FSLIST, F_SSEARCH, F_REINIT, F_CONT, F_MOM, F_FSTREAM, F_DEM, F_FPACK,
+           F_FCOPY, F_NFAM };
 const char* kFamNames[F_NFAM] = {"grid_hash_sort", "fluid_search",  "fluid_solid_list",
                                  "solid_search",   "density_reinit", "continuity",
-                                 "momentum",       "fluid_stream",  "dem"};
+                                 "momentum",       "fluid_stream",  "dem",
+                                 "frame_pack",     "frame_copy"};
 
 }  // namespace
 
@@ -251,7 +254,8 @@ int fail(dsph_ctx* c, const ApiError& e) {
 struct FamTimer {
     dsph_ctx* c;
     int f;
-    FamTimer(dsph_ctx* c_, int f_) : c(c_), f(f_) {
+    hipStream_t s;
+    FamTimer(dsph_ctx* c_, int f_, hipStream_t s_ = nullptr) : c(c_), f(f_), s(s_ ? s_ : c_->st) {
         if (!c->timing) return;
         if (c->ev_used[f] == c->ev[f].size()) {
             hipEvent_t a, b;
@@ -259,11 +263,11 @@ struct FamTimer {
             HIPCHK(hipEventCreate(&b));
             c->ev[f].push_back({a, b});
         }
-        HIPCHK(hipEventRecord(c->ev[f][c->ev_used[f]].first, c->st));
+        HIPCHK(hipEventRecord(c->ev[f][c->ev_used[f]].first, s));
     }
     ~FamTimer() {
         if (!c->timing) return;
-        (void)hipEventRecord(c->ev[f][c->ev_used[f]].second, c->st);
+        (void)hipEventRecord(c->ev[f][c->ev_used[f]].second, s);
         c->ev_used[f]++;
     }
 };
@@ -2778,10 +2782,16 @@ int dsph_frame_begin(dsph_ctx* c, int32_t slot, int32_t i_step, double time, uin
         // the records of buffer dcur hold the fluid's current dRho; the
         // packing is on the context stream, its copy on the copy stream, so
         // the steps that follow run beside the copy
-        frame_pack(c->st, species, c->fset[0], (int)nf, c->dcur, c->so, (int)ns, c->L, pcap, o);
+        {
+            FamTimer t(c, F_FPACK);
+            frame_pack(c->st, species, c->fset[0], (int)nf, c->dcur, c->so, (int)ns, c->L, pcap, o);
+        }
         HIPCHK(hipEventRecord(F.packed, c->st));
         HIPCHK(hipStreamWaitEvent(c->cst, F.packed, 0));
-        HIPCHK(hipMemcpyAsync(F.host, F.dev, L.pairs, hipMemcpyDeviceToHost, c->cst));
+        {
+            FamTimer t(c, F_FCOPY, c->cst);
+            HIPCHK(hipMemcpyAsync(F.host, F.dev, L.pairs, hipMemcpyDeviceToHost, c->cst));
+        }
         HIPCHK(hipEventRecord(F.copied, c->cst));
         F.species = species;
         F.step = i_step;
@@ -3258,6 +3268,7 @@ int dsph_set_timing(dsph_ctx* c, int enabled) {
 int dsph_kernel_times(dsph_ctx* c, char* names, int64_t names_cap, double* ms, int32_t cap) {
     if (!c) return DSPH_E_INVALID;
     if (hipStreamSynchronize(c->st) != hipSuccess) return DSPH_E_HIP;
+    if (c->cst && hipStreamSynchronize(c->cst) != hipSuccess) return DSPH_E_HIP;
     std::string all;
     int m = 0;
     for (int f = 0; f < F_NFAM && m < cap; f++) {

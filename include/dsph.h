@@ -469,9 +469,12 @@ int dsph_hash_sort_noalloc(dsph_ctx* ctx, const double* pos, int32_t n,
                            const int32_t cells[3], int32_t* hash, int32_t* perm,
                            int32_t* cell_start, int32_t* cell_end);
 
-/* profiling aid: average device time (ms) of the last dsph_step_n run per
- * kernel family measured with HIP events on the context stream; names is a
- * NUL-separated list.  Returns the number of entries written. */
+/* profiling aid: per kernel family, the total device time (ms) and the
+ * launch count since the last call or dsph_set_timing, measured with HIP
+ * events on the context stream (ms[2k], ms[2k+1]); the families include
+ * "frame_pack" (dsph_frame_begin's packing) and "frame_copy" (its copy, on
+ * the frames' copy stream).  names is a NUL-separated list.  Returns the
+ * number of entries written. */
 int dsph_kernel_times(dsph_ctx* ctx, char* names, int64_t names_cap,
                       double* ms, int32_t cap);
 int dsph_set_timing(dsph_ctx* ctx, int enabled);
