@@ -2429,7 +2429,7 @@ struct StateBlobHeader {
     int64_t n_arrays, payload;
 };
 constexpr char kStateMagic[8] = {'D', 'S', 'P', 'H', 'S', 'T', 'A', 'T'};
-constexpr int32_t kStateVersion = 3;
+constexpr int32_t kStateVersion = 4;
 
 static StateBlobHeader state_header(dsph_ctx* c, const std::vector<StateArr>& v) {
     StateBlobHeader h{};

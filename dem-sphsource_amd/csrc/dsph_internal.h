@@ -267,15 +267,22 @@ struct Lists {
 // tile is a compact block of particles, and the union of its rows'
 // candidates is a few times its size.  At every build each tile gets that
 // union (uni, ascending slots, at most kTileCap) and each of its rows the
-// same candidates in the same order, minus the particle itself, as indices
-// into the union (row, u16; rcnt entries), four consecutive entries of a
-// row in one 8-byte quad, quads ELL column-major: entry c of slot s at
-// ((c / 4) * nf + s) * 4 + c % 4.  The
-// pair loops stage the union's records in LDS once per tile and walk the
-// local rows there.  ucnt < 0: the union did not fit, and the tile walks its
-// global rows.
+// same candidates in the same order, minus the particle itself, as LDS
+// entries of the union (row, u16; rcnt entries), four consecutive entries of
+// a row in one 8-byte quad, quads ELL column-major: entry c of slot s at
+// ((c / 4) * nf + s) * 4 + c % 4.  The pair loops stage the union's records
+// in LDS once per tile and walk the local rows there.  ucnt < 0: the union
+// did not fit, and the tile walks its global rows.
 constexpr int kTileSlots = 512;
 constexpr int kTileCap = 2432;  // 76 KiB of mixed records in LDS
+// LDS entry of union position p: p with its bits 0-2 XOR bits 3-5, a
+// permutation within each aligned block of 64 entries (kTileCap, the
+// sentinel, maps to itself).  A 16-byte LDS read serves eight lanes per
+// pass, one per bank group (entry mod 8); neighbouring slots' rows step
+// through the union a few positions per slot, and a step of 2, 4 or 8 put
+// several lanes of a pass on one bank group, which the XOR spreads.
+static_assert(kTileCap % 64 == 0, "the swizzle permutes aligned blocks of 64 entries");
+__host__ __device__ constexpr int tile_swz(int p) { return p ^ ((p >> 3) & 7); }
 // entries per slot of the local rows: the superset capacity, the pad read
 // ahead, rounded up to whole quads
 __host__ __device__ constexpr int tile_row_cols(int supK) { return (supK + kRowPad + 3) / 4 * 4; }

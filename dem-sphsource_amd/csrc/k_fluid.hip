@@ -484,9 +484,9 @@ __device__ __forceinline__ void for_row(const FluidArgs& a, Rsrc rs, int s, Fn&&
 // (u16, ELL column-major), in the order of its global row, in batches of U
 // with the next batch's indices read ahead.  An LDS read replaces the
 // gather of the global walk.  Position and state quads sit in two arrays of
-// 16-byte entries, so consecutive union positions fall in consecutive bank
-// groups (a 32-byte stride would leave half of them unused and make the
-// lanes' reads collide).
+// 16-byte entries, so consecutive entries fall in consecutive bank groups (a
+// 32-byte stride would leave half of them unused and make the lanes' reads
+// collide); union position k sits at entry tile_swz(k).
 // Entry kTileCap is a sentinel that local rows carry past their end: a
 // record at fixed-point (0, 0, 0) at rest with dRho = 0.  Every particle of
 // a tiled row lies in the domain, which the fixed-point map places at least
@@ -504,9 +504,9 @@ __device__ __forceinline__ int stage_tile(const FluidArgs& a, int b, TileLds& L)
     const TileRec* rec = (const TileRec*)a.f.rec[a.cur];
     const int* uni = a.T.uni + (int64_t)b * kTileCap;
     for (int k = threadIdx.x; k < m; k += blockDim.x) {
-        const int j = uni[k];
-        L.pos[k] = rec[2 * (int64_t)j];
-        if constexpr (STATE) L.st[k] = rec[2 * (int64_t)j + 1];
+        const int j = uni[k], e = tile_swz(k);  // the local rows hold entries e
+        L.pos[e] = rec[2 * (int64_t)j];
+        if constexpr (STATE) L.st[e] = rec[2 * (int64_t)j + 1];
     }
     if (threadIdx.x == 0) {
         L.pos[kTileCap] = make_uint4(0u, 0u, 0u, 0u);

@@ -1703,7 +1703,7 @@ __global__ void __launch_bounds__(kTileSlots) k_tile_build(SearchArgs a) {
                     const unsigned word = (j >> 5) & 7;
                     pos = wpos[h][word] + __popc(pbits[h][word] & ((1u << (j & 31)) - 1u));
                 }
-                T.row[tile_row_at(w, s, a.nf)] = (unsigned short)pos;
+                T.row[tile_row_at(w, s, a.nf)] = (unsigned short)tile_swz(pos);
                 w++;
             }
         }
