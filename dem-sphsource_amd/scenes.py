@@ -533,6 +533,41 @@ def immersed_spheres(n_box=(12, 10, 10), dx=0.005, stride=3, r_frac=0.5, rho_s=2
     return sc
 
 
+def poured(dx=0.005, name="poured"):
+    """A scene and the particles added to it after the run started
+    (dataBase::addFluid / addCluster / addClump mid-run,
+    core/dataBase.cpp:113-117, 226-346).  The scene: spheres inside a fluid
+    block in a dummy-walled tank (immersed_spheres), a bonded cluster of
+    eight touching spheres above the fluid and two touching spheres sliding
+    past each other.  The addition: a fluid block
+    above the fluid, a bonded cluster of four touching spheres against the
+    first cluster, and a clump of two pebbles.  Returns (scene, addition)."""
+    base = immersed_spheres(n_box=(8, 6, 6), dx=dx, name=name)
+    r = 0.5 * dx
+    d = 2.0 * r * 0.999  # touching neighbours in a cluster
+    g = np.array([[i, j, k] for k in range(2) for j in range(2) for i in range(2)], float)
+    add_cluster(base, np.array([0.005, 0.01, 0.042]) + d * g, r, 2500.0, material=0)
+    # two unbonded touching spheres sliding past each other (a contact with
+    # tangential springs)
+    n0 = base.n_solid
+    add_solid(base, [[0.03, 0.005, 0.042], [0.03 + d, 0.005, 0.042]], r, 2500.0, material=0)
+    base.solid_vel[n0:] = [[0.0, 0.01, 0.0], [0.0, -0.01, 0.0]]
+    # a bond stiffness of 1e4 N/m: stable at the scene's step
+    Eb = 1.0e4 * (2 * r) / (math.pi * r * r)
+    base.set_bonded(0, 0, Eb, 1.0, 1.0, 1.0e6, 0.0, 0.0)
+    extra = empty_scene(base.domain_origin, base.domain_size, n_material=base.n_material)
+    pts = regular_packed_points((0.02, 0.005, 0.045), (3 * dx, 3 * dx, 2 * dx), dx)
+    add_fluid(extra, pts, 1.3 * dx, RHO0, sound_speed(6 * dx), NU)
+    g4 = np.array([[0, j, k] for k in range(2) for j in range(2)], float)
+    add_cluster(extra, np.array([0.005 + 2 * d, 0.01, 0.042]) + d * g4, r, 2500.0, material=0)
+    extra.solid_cluster = extra.solid_cluster + int(np.max(base.solid_cluster)) + 1
+    peb = np.array([[0.03, 0.02, 0.06], [0.03 + 2 * r, 0.02, 0.06]])
+    m = 2 * 4.0 / 3.0 * math.pi * r ** 3 * 2500.0
+    I = 0.4 * m * r * r + 0.5 * m * r * r
+    add_clump(extra, peb, np.full(2, r), peb.mean(axis=0), m, (I, I, I, 0.0, 0.0, 0.0), material=0)
+    return base, extra
+
+
 def dem_contact_mix(n_side=6, n_clump_side=3, r=0.002, rho_s=2500.0, overlap=0.02, seed_stream=61,
                     name="dem_contact_mix") -> Scene:
     """A DEM scene with every spring of the contact laws loaded from the

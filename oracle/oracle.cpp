@@ -734,28 +734,151 @@ int orc_initialize(orc_ctx* o, int compute_normals) {
     return 0;
 }
 
+/* dataBase::addFluid / addSolid / addCluster / addClump called after the run
+ * started (dataBase.cpp:113-117, 226-268, 270-346).  The particles of e go
+ * after the existing ones of each species, whose state stays as it is (the
+ * reference's uploadState + insertData + copy); e's clump pebble ranges and
+ * solid clump indices are relative to e.  Then the search of step 0, which
+ * carries the contact springs over by (i, j) (neighborSearch(dev, 0, 1)),
+ * and, when solids were added, addBondData's later branch (:309-343): a
+ * contact of the new list becomes a bond when its pointed solid comes after
+ * the last bonded one and the initial-bond conditions hold.  No normals are
+ * estimated (setSolidNormal runs only at initialisation); a new solid's
+ * normal is e's.  Resolved differently from the reference:
+ *  - sumW0 of the existing fluid is kept, and a new particle's is its
+ *    kernel sum just after the append (the reference reallocates sumW0 as
+ *    zeros, deviceStructs.h:167, so its next re-initialisation divides by
+ *    zero);
+ *  - the infinite- and cylinder-wall spring histories of the existing solids
+ *    are kept (the reference rebuilds them empty, dataBase.cpp:259-260);
+ *  - with no bond yet, every contact may become one (the reference reads the
+ *    last element of an empty vector there, :313);
+ *  - walls, materials, contact tables, domain and gravity stay those of the
+ *    scene (e must hold no walls).                                       */
+static double shepard_sum(const orc_ctx* o, int i);
+int orc_append_scene(orc_ctx* o, const dsph_scene* e) {
+    if (!o || !e || !o->initialized) return DSPH_E_STATE;
+    if (e->n_fluid < 0 || e->n_solid < 0 || e->n_clump < 0 || e->n_wall != 0 || e->n_cylinder != 0)
+        return DSPH_E_INVALID;
+    const int nf0 = o->nf, ns0 = o->ns, nc0 = o->nc;
+    const int nf1 = e->n_fluid, ns1 = e->n_solid, nc1 = e->n_clump;
+    for (int k = 0; k < nc1; k++)
+        if (e->clump_pebble_start[k] < 0 || e->clump_pebble_end[k] > ns1 ||
+            e->clump_pebble_start[k] > e->clump_pebble_end[k])
+            return DSPH_E_INVALID;
+    for (int i = 0; i < ns1; i++) {
+        const int m = e->solid_material ? e->solid_material[i] : 0;
+        const bool dummy = e->solid_dummy && e->solid_dummy[i];
+        if (!dummy && (m < 0 || m >= o->nmat)) return DSPH_E_INVALID;
+    }
+    for (int i = 0; i < nf1; i++) {
+        o->fpos.push_back(ld3(e->fluid_pos, i));
+        o->fvel.push_back(e->fluid_vel ? ld3(e->fluid_vel, i) : mk(0, 0, 0));
+        o->facc.push_back(mk(0, 0, 0));
+        o->fh.push_back(e->fluid_h[i]);
+        o->rho0.push_back(e->fluid_rho0[i]);
+        o->drho.push_back(e->fluid_drho ? e->fluid_drho[i] : 0.0);
+        o->c.push_back(e->fluid_c[i]);
+        o->nu.push_back(e->fluid_nu[i]);
+        o->alpha.push_back(e->fluid_alpha ? e->fluid_alpha[i] : 0.0);
+        o->sumw0.push_back(0.0);
+    }
+    o->nf = nf0 + nf1;
+    for (int i = 0; i < ns1; i++) {
+        o->spos.push_back(ld3(e->solid_pos, i));
+        o->svel.push_back(e->solid_vel ? ld3(e->solid_vel, i) : mk(0, 0, 0));
+        o->sacc.push_back(mk(0, 0, 0));
+        o->sw.push_back(e->solid_angvel ? ld3(e->solid_angvel, i) : mk(0, 0, 0));
+        o->storq.push_back(mk(0, 0, 0));
+        o->snorm.push_back(e->solid_normal ? ld3(e->solid_normal, i) : mk(0, 0, 0));
+        o->srad.push_back(e->solid_radius[i]);
+        o->seff.push_back(e->solid_eff_radius[i]);
+        o->sinvm.push_back(e->solid_inv_mass[i]);
+        o->smat.push_back(e->solid_material ? e->solid_material[i] : 0);
+        o->sclus.push_back(e->solid_cluster ? e->solid_cluster[i] : -1);
+        const int cl = e->solid_clump ? e->solid_clump[i] : -1;
+        o->sclump.push_back(cl >= 0 ? cl + nc0 : -1);
+        o->sdummy.push_back(e->solid_dummy ? e->solid_dummy[i] : 0);
+    }
+    o->ns = ns0 + ns1;
+    for (int k = 0; k < nc1; k++) {
+        o->cpos.push_back(ld3(e->clump_centroid, k));
+        o->cvel.push_back(e->clump_vel ? ld3(e->clump_vel, k) : mk(0, 0, 0));
+        o->cacc.push_back(mk(0, 0, 0));
+        o->cw.push_back(e->clump_angvel ? ld3(e->clump_angvel, k) : mk(0, 0, 0));
+        o->ctorq.push_back(mk(0, 0, 0));
+        if (e->clump_orientation) {
+            const double* q = e->clump_orientation + 4 * k;
+            o->cq.push_back(Quat{q[0], q[1], q[2], q[3]});
+        } else {
+            o->cq.push_back(Quat{1, 0, 0, 0});
+        }
+        const double* I = e->clump_inv_inertia + 6 * k;
+        o->cinvI.push_back(Sym{I[0], I[1], I[2], I[3], I[4], I[5]});
+        o->cinvm.push_back(e->clump_inv_mass[k]);
+        o->cstart.push_back(e->clump_pebble_start[k] + ns0);
+        o->cend.push_back(e->clump_pebble_end[k] + ns0);
+    }
+    o->nc = nc0 + nc1;
+    /* per-solid wall histories are solid-major: the new solids' start empty */
+    const Springs zero{mk(0, 0, 0), mk(0, 0, 0), mk(0, 0, 0)};
+    o->wallHist.resize((size_t)o->ns * o->nw, zero);
+    o->cylHist.resize((size_t)o->ns * o->ny, zero);
+    /* the contact list's springs carry over by (i, j): its ranges per
+       pointing solid cover the new solids too, with no pairs */
+    o->ss.rstart.resize(o->ns, -1);
+    o->ss.rend.resize(o->ns, -1);
+    orc_neighbor_search(o, 0, 1);
+    if (ns1 > 0) {
+        int last = -1;  /* the last bonded pointed solid */
+        for (int i : o->bI) last = std::max(last, i);
+        const size_t nb0 = o->bI.size();
+        add_bonds(o);
+        /* keep the old bonds and the new ones past the last bonded solid */
+        size_t w = nb0;
+        for (size_t k = nb0; k < o->bI.size(); k++) {
+            if (o->bI[k] <= last) continue;
+            o->bN[w] = o->bN[k]; o->bPt[w] = o->bPt[k]; o->bFs[w] = o->bFs[k]; o->bTb[w] = o->bTb[k];
+            o->bFn[w] = o->bFn[k]; o->bTt[w] = o->bTt[k];
+            o->bI[w] = o->bI[k]; o->bJ[w] = o->bJ[k]; o->bOn[w] = o->bOn[k];
+            w++;
+        }
+        o->bN.resize(w); o->bPt.resize(w); o->bFs.resize(w); o->bTb.resize(w);
+        o->bFn.resize(w); o->bTt.resize(w); o->bI.resize(w); o->bJ.resize(w); o->bOn.resize(w);
+    }
+    for (int i = nf0; i < o->nf; i++) o->sumw0[i] = shepard_sum(o, i);
+    return 0;
+}
+
 /* ---------------- fluid phase: integrate.cu:3-421 ---------------------- */
 
 /* densityReinitialization (integrate.cu:3-53) */
+/* the Shepard kernel sum of fluid particle i over its lists
+ * (integrate.cu:22-44) */
+static double shepard_sum(const orc_ctx* o, int i) {
+    V3 ri = o->fpos[i];
+    double hi = o->fh[i];
+    double V0 = std::pow(hi / 1.3, 3);
+    double sw = wendland(0, hi);
+    for (int k = o->ff.seg_begin(i); k < o->ff.seg_end(i); k++)
+        sw += wendland(len(ri - o->fpos[o->ff.pointing[k]]), hi);
+    if (o->ff.rstart[i] != -1)
+        for (int k = o->ff.rstart[i]; k < o->ff.rend[i]; k++)
+            sw += wendland(len(ri - o->fpos[o->ff.pointed[o->ff.tindex[k]]]), hi);
+    for (int k = o->fs.seg_begin(i); k < o->fs.seg_end(i); k++) {
+        int a = o->fs.pointing[k];
+        double Va = std::pow(2. * o->srad[a], 3);
+        sw += wendland(len(ri - o->spos[a]), hi) * Va / V0;
+    }
+    return sw;
+}
+
 static void density_reinit(orc_ctx* o, int i_step) {
     std::vector<double> out(o->drho);
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < o->nf; i++) {
-        V3 ri = o->fpos[i];
-        double hi = o->fh[i];
-        double V0 = std::pow(hi / 1.3, 3);
-        double sw = wendland(0, hi);
         double rhos = o->rho0[i] + o->drho[i];
-        for (int k = o->ff.seg_begin(i); k < o->ff.seg_end(i); k++)
-            sw += wendland(len(ri - o->fpos[o->ff.pointing[k]]), hi);
-        if (o->ff.rstart[i] != -1)
-            for (int k = o->ff.rstart[i]; k < o->ff.rend[i]; k++)
-                sw += wendland(len(ri - o->fpos[o->ff.pointed[o->ff.tindex[k]]]), hi);
-        for (int k = o->fs.seg_begin(i); k < o->fs.seg_end(i); k++) {
-            int a = o->fs.pointing[k];
-            double Va = std::pow(2. * o->srad[a], 3);
-            sw += wendland(len(ri - o->spos[a]), hi) * Va / V0;
-        }
+        const double sw = shepard_sum(o, i);
         if (i_step == 1) {
             o->sumw0[i] = sw;
         } else {

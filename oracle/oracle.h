@@ -33,6 +33,8 @@ int      orc_fluid_integrate(orc_ctx* o, double dt, int32_t i_step, int32_t gap)
 int      orc_contact_forces(orc_ctx* o, double dt);
 int      orc_solid_after_contact(orc_ctx* o, double dt);
 int      orc_step(orc_ctx* o, int32_t i_step, double dt, int32_t gap);
+/* particles added after the run started (dsph_append_scene's semantics) */
+int      orc_append_scene(orc_ctx* o, const dsph_scene* extra);
 /* runtime mutation between the contact phase and solid_after_contact */
 int      orc_add_external_force(orc_ctx* o, int32_t solid_index,
                                 const double force[3]);

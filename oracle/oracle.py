@@ -30,6 +30,8 @@ def _load():
     lib.orc_create.restype = C.c_void_p
     lib.orc_destroy.argtypes = [C.c_void_p]
     lib.orc_upload_scene.argtypes = [C.c_void_p, C.POINTER(CScene)]
+    lib.orc_append_scene.argtypes = [C.c_void_p, C.POINTER(CScene)]
+    lib.orc_append_scene.restype = C.c_int
     for name, args in [
         ("orc_set_flags", [C.c_uint32]), ("orc_set_threads", [C.c_int]),
         ("orc_initialize", [C.c_int]), ("orc_neighbor_search", [C.c_int32, C.c_int32]),
@@ -149,6 +151,14 @@ class Oracle:
         cs = scene.cstruct()
         self._ok(self.L.orc_upload_scene(self.h, C.byref(cs)), "upload")
         self.nf, self.ns, self.nc = scene.n_fluid, scene.n_solid, scene.n_clump
+
+    def append(self, extra):
+        """Particles added after the run started (orc_append_scene)."""
+        cs = extra.cstruct()
+        self._ok(self.L.orc_append_scene(self.h, C.byref(cs)), "append")
+        self.nf += extra.n_fluid
+        self.ns += extra.n_solid
+        self.nc += extra.n_clump
 
     def set_flags(self, flags):
         self.L.orc_set_flags(self.h, flags)
