@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -53,7 +54,14 @@ const char* kFamNames[F_NFAM] = {"grid_hash_sort", "fluid_search",  "fluid_solid
 
 }  // namespace
 
+namespace {
+struct Merged;
+}
+
 struct dsph_ctx {
+    // the scene as uploaded (the static arrays dsph_append_scene uploads
+    // again), or null after a batch
+    std::unique_ptr<Merged> host;
     int dev = 0;
     hipStream_t st = nullptr;
     int prec = DSPH_PREC_MIXED;
@@ -1626,6 +1634,7 @@ int dsph_upload_batch(dsph_ctx* c, const dsph_scene* scenes, int32_t n_scenes) {
 
         invalidate_graphs(c);
         free_all(c);
+        c->host.reset();
         c->L = Lists();
         c->B = Bonds();
         c->so = Solids();
@@ -2156,6 +2165,7 @@ int dsph_upload_batch(dsph_ctx* c, const dsph_scene* scenes, int32_t n_scenes) {
         HIPCHK(hipStreamSynchronize(c->st));
         HIPCHK(hipGetLastError());
         c->uploaded = true;
+        if (n_scenes == 1) c->host = std::make_unique<Merged>(std::move(M));
         return DSPH_OK;
     } catch (const ApiError& e) {
         return fail(c, e);
@@ -2177,32 +2187,36 @@ int dsph_set_capacity(dsph_ctx* c, int32_t ff, int64_t fs, int64_t ss) {
     }
 }
 
+namespace {
+// addBondData, initial branch (dataBase.cpp:270-308), on the device: bonds
+// are the contacts of the current list in one cluster that touch
+// physically, in the contact list's order (flags, scan, fill)
+void initial_bonds(dsph_ctx* c) {
+    const int64_t cap = std::max<int64_t>(c->L.ss_cap, 1);
+    int* flag = dalloc<int>(c, (size_t)cap);
+    int* off = dalloc<int>(c, (size_t)cap + 1);
+    int* tmp = dalloc<int>(c, (size_t)scan_tmp_size(cap) + 64);
+    const DemArgs da = dem_args(c);
+    fill_int(c->st, flag, cap, 0);
+    dem_bond_flags(c->st, da, flag);
+    exclusive_scan(c->st, flag, off, cap, tmp);
+    int nb = 0;
+    d2h(c, &nb, off + cap, 1);
+    HIPCHK(hipStreamSynchronize(c->st));
+    alloc_bonds(c, nb);
+    dem_bond_fill(c->st, dem_args(c), off, c->B);
+    HIPCHK(hipStreamSynchronize(c->st));
+    dfree(c, flag);
+    dfree(c, off);
+    dfree(c, tmp);
+}
+}  // namespace
+
 int dsph_initialize(dsph_ctx* c, int compute_normals) {
     if (!c || !c->uploaded) return DSPH_E_STATE;
     try {
         search_checked(c, 0, 1);
-        // addBondData, initial branch (dataBase.cpp:270-308), on the device:
-        // bonds are the initial contacts in one cluster that touch
-        // physically, in the contact list's order (flags, scan, fill)
-        {
-            const int64_t cap = std::max<int64_t>(c->L.ss_cap, 1);
-            int* flag = dalloc<int>(c, (size_t)cap);
-            int* off = dalloc<int>(c, (size_t)cap + 1);
-            int* tmp = dalloc<int>(c, (size_t)scan_tmp_size(cap) + 64);
-            const DemArgs da = dem_args(c);
-            fill_int(c->st, flag, cap, 0);
-            dem_bond_flags(c->st, da, flag);
-            exclusive_scan(c->st, flag, off, cap, tmp);
-            int nb = 0;
-            d2h(c, &nb, off + cap, 1);
-            HIPCHK(hipStreamSynchronize(c->st));
-            alloc_bonds(c, nb);
-            dem_bond_fill(c->st, dem_args(c), off, c->B);
-            HIPCHK(hipStreamSynchronize(c->st));
-            dfree(c, flag);
-            dfree(c, off);
-            dfree(c, tmp);
-        }
+        initial_bonds(c);
         if (compute_normals && c->ns > 0) {
             // the solid grid of the initial positions (the gated search sorts
             // only on fluid steps)
@@ -2864,6 +2878,263 @@ int dsph_frame_write(dsph_ctx* c, int32_t slot, const char* dir, int32_t frame) 
     }
     F.pending = false;
     return ok ? DSPH_OK : DSPH_E_IO;
+}
+
+// ------------------------------------------------------- mid-run additions
+extern "C++" {
+namespace {
+// the bond arrays on the host
+struct HostBonds {
+    std::vector<int> i, j, on;
+    std::vector<double4> nrm, pt, Fs, Tb;
+    std::vector<double> Fn, Tt;
+    void get(dsph_ctx* c) {
+        const Bonds& B = c->B;
+        const size_t n = (size_t)B.n;
+        i.resize(n); j.resize(n); on.resize(n); nrm.resize(n); pt.resize(n); Fs.resize(n); Tb.resize(n);
+        Fn.resize(n); Tt.resize(n);
+        d2h(c, i.data(), B.i, n); d2h(c, j.data(), B.j, n); d2h(c, on.data(), B.on, n);
+        d2h(c, nrm.data(), B.nrm, n); d2h(c, pt.data(), B.pt, n); d2h(c, Fs.data(), B.Fs, n);
+        d2h(c, Tb.data(), B.Tb, n); d2h(c, Fn.data(), B.Fn, n); d2h(c, Tt.data(), B.Tt, n);
+        HIPCHK(hipStreamSynchronize(c->st));
+    }
+    void append(const HostBonds& o, size_t k) {
+        i.push_back(o.i[k]); j.push_back(o.j[k]); on.push_back(o.on[k]); nrm.push_back(o.nrm[k]);
+        pt.push_back(o.pt[k]); Fs.push_back(o.Fs[k]); Tb.push_back(o.Tb[k]); Fn.push_back(o.Fn[k]);
+        Tt.push_back(o.Tt[k]);
+    }
+    void put(dsph_ctx* c) const {
+        const size_t n = i.size();
+        alloc_bonds(c, (int)n);
+        const Bonds& B = c->B;
+        h2d(c, B.i, i.data(), n); h2d(c, B.j, j.data(), n); h2d(c, B.on, on.data(), n);
+        h2d(c, B.nrm, nrm.data(), n); h2d(c, B.pt, pt.data(), n); h2d(c, B.Fs, Fs.data(), n);
+        h2d(c, B.Tb, Tb.data(), n); h2d(c, B.Fn, Fn.data(), n); h2d(c, B.Tt, Tt.data(), n);
+        HIPCHK(hipStreamSynchronize(c->st));
+    }
+};
+template <class T>
+void cat(std::vector<T>& v, const std::vector<T>& a, const T* b, size_t nb, size_t width, T dflt) {
+    v = a;
+    put(v, b, nb, width, dflt);
+}
+// xyz of caller-order arrays [3n] into the .xyz of device quads (w kept)
+void put_xyz(dsph_ctx* c, double4* d, const std::vector<double>& v, size_t n) {
+    std::vector<double4> q(n);
+    d2h(c, q.data(), d, n);
+    HIPCHK(hipStreamSynchronize(c->st));
+    for (size_t k = 0; k < n; k++) {
+        q[k].x = v[3 * k];
+        q[k].y = v[3 * k + 1];
+        q[k].z = v[3 * k + 2];
+    }
+    h2d(c, d, q.data(), n);
+}
+}  // namespace
+}  // extern "C++"
+
+int dsph_append_scene(dsph_ctx* c, const dsph_scene* e) {
+    API_GUARD(c);
+    if (!e) return DSPH_E_INVALID;
+    if (!c->initialized || !c->host) return DSPH_E_STATE;
+    try {
+        validate_scene(e);
+        require(e->n_wall == 0 && e->n_cylinder == 0, "walls cannot be added after the start");
+        const int nf0 = c->nf, ns0 = c->ns, nc0 = c->nc;
+        const int nf1 = e->n_fluid, ns1 = e->n_solid, nc1 = e->n_clump;
+        for (int k = 0; k < nc1; k++)
+            require(e->clump_pebble_start[k] >= 0 && e->clump_pebble_end[k] <= ns1 &&
+                        e->clump_pebble_start[k] <= e->clump_pebble_end[k],
+                    "an added clump's pebbles must be added solids");
+        HIPCHK(hipStreamSynchronize(c->st));
+        if (c->cst) HIPCHK(hipStreamSynchronize(c->cst));
+        // 1. the running state (caller order), the contact list with its
+        // springs, the wall histories, the bonds and the contact tables
+        std::vector<double> fpos(3 * nf0), fvel(3 * nf0), facc(3 * nf0), fdrho(nf0), fsw(nf0);
+        std::vector<double> spos(3 * ns0), svel(3 * ns0), sacc(3 * ns0), sw(3 * ns0), stq(3 * ns0),
+            snrm(3 * ns0);
+        std::vector<double> cpos(3 * nc0), cvel(3 * nc0), cacc(3 * nc0), cw(3 * nc0), ctq(3 * nc0),
+            cq(4 * nc0);
+        auto ok = [](int rc) {
+            if (rc != DSPH_OK) throw ApiError{rc, "state read-back failed"};
+        };
+        if (nf0) ok(dsph_download_fluid(c, fpos.data(), fvel.data(), facc.data(), fdrho.data(), fsw.data()));
+        if (ns0)
+            ok(dsph_download_solid(c, spos.data(), svel.data(), sacc.data(), sw.data(), stq.data(), snrm.data()));
+        if (nc0)
+            ok(dsph_download_clump(c, cpos.data(), cvel.data(), cacc.data(), cw.data(), ctq.data(), cq.data()));
+        int tss = 0;
+        if (ns0) d2h(c, &tss, c->L.ssoff + ns0, 1);
+        HIPCHK(hipStreamSynchronize(c->st));
+        const int64_t nss = std::min<int64_t>(tss, c->L.ss_cap);
+        std::vector<int> ssi(nss), ssj(nss);
+        std::vector<double> ssS(9 * nss);
+        d2h(c, ssi.data(), c->L.ss_i, nss);
+        d2h(c, ssj.data(), c->L.ss_j, nss);
+        d2h(c, ssS.data(), c->L.ssS, 9 * nss);
+        const size_t nwh = 9 * (size_t)ns0 * c->nwmax, nyh = 9 * (size_t)ns0 * c->nymax;
+        std::vector<double> wh(nwh), yh(nyh);
+        d2h(c, wh.data(), c->whist, nwh);
+        d2h(c, yh.data(), c->yhist, nyh);
+        const size_t ncb = (size_t)c->tb.ncomb;
+        std::vector<double> hz(8 * ncb), ln(11 * ncb), bd(6 * ncb);
+        d2h(c, hz.data(), c->tb.hertz, 8 * ncb);
+        d2h(c, ln.data(), c->tb.linear, 11 * ncb);
+        d2h(c, bd.data(), c->tb.bonded, 6 * ncb);
+        HostBonds old;
+        old.get(c);
+        // 2. the scene again: the existing particles' static arrays and
+        // running state, then the added ones
+        const Merged& H = *c->host;
+        std::vector<double> Fpos, Fvel, Fh, Frho0, Fdrho, Fc, Fnu, Falpha;
+        cat(Fpos, fpos, e->fluid_pos, nf1, 3, 0.0);
+        cat(Fvel, fvel, e->fluid_vel, nf1, 3, 0.0);
+        cat(Fh, H.fh, e->fluid_h, nf1, 1, 0.0);
+        cat(Frho0, H.frho0, e->fluid_rho0, nf1, 1, 0.0);
+        cat(Fdrho, fdrho, e->fluid_drho, nf1, 1, 0.0);
+        cat(Fc, H.fc, e->fluid_c, nf1, 1, 0.0);
+        cat(Fnu, H.fnu, e->fluid_nu, nf1, 1, 0.0);
+        cat(Falpha, H.falpha, e->fluid_alpha, nf1, 1, 0.0);
+        std::vector<double> Spos, Svel, Sw, Snrm, Srad, Seff, Sinvm;
+        std::vector<int> Smat, Sclus, Sclump, Sdum;
+        cat(Spos, spos, e->solid_pos, ns1, 3, 0.0);
+        cat(Svel, svel, e->solid_vel, ns1, 3, 0.0);
+        cat(Sw, sw, e->solid_angvel, ns1, 3, 0.0);
+        cat(Snrm, snrm, e->solid_normal, ns1, 3, 0.0);
+        cat(Srad, H.srad, e->solid_radius, ns1, 1, 0.0);
+        cat(Seff, H.seff, e->solid_eff_radius, ns1, 1, 0.0);
+        cat(Sinvm, H.sinvm, e->solid_inv_mass, ns1, 1, 0.0);
+        cat(Smat, H.smat, e->solid_material, ns1, 1, 0);
+        cat(Sclus, H.sclus, e->solid_cluster, ns1, 1, -1);
+        cat(Sdum, H.sdum, e->solid_dummy, ns1, 1, 0);
+        Sclump = H.sclump;
+        for (int i = 0; i < ns1; i++) {
+            const int k = e->solid_clump ? e->solid_clump[i] : -1;
+            Sclump.push_back(k >= 0 ? k + nc0 : -1);
+        }
+        std::vector<double> Cpos, Cvel, Cw, Cq, CinvI, Cinvm;
+        std::vector<int> Cstart = H.cstart, Cend = H.cend;
+        cat(Cpos, cpos, e->clump_centroid, nc1, 3, 0.0);
+        cat(Cvel, cvel, e->clump_vel, nc1, 3, 0.0);
+        cat(Cw, cw, e->clump_angvel, nc1, 3, 0.0);
+        Cq = cq;
+        for (int k = 0; k < nc1; k++)
+            for (int m = 0; m < 4; m++)
+                Cq.push_back(e->clump_orientation ? e->clump_orientation[4 * k + m] : (m == 0 ? 1.0 : 0.0));
+        cat(CinvI, H.cinvI, e->clump_inv_inertia, nc1, 6, 0.0);
+        cat(Cinvm, H.cinvm, e->clump_inv_mass, nc1, 1, 0.0);
+        for (int k = 0; k < nc1; k++) {
+            Cstart.push_back(e->clump_pebble_start[k] + ns0);
+            Cend.push_back(e->clump_pebble_end[k] + ns0);
+        }
+        dsph_scene S = H.flat;
+        S.gravity[0] = c->gv[0];
+        S.gravity[1] = c->gv[1];
+        S.gravity[2] = c->gv[2];
+        S.n_fluid = nf0 + nf1;
+        S.fluid_pos = Fpos.data(); S.fluid_vel = Fvel.data(); S.fluid_h = Fh.data();
+        S.fluid_rho0 = Frho0.data(); S.fluid_drho = Fdrho.data(); S.fluid_c = Fc.data();
+        S.fluid_nu = Fnu.data(); S.fluid_alpha = Falpha.data();
+        S.n_solid = ns0 + ns1;
+        S.solid_pos = Spos.data(); S.solid_vel = Svel.data(); S.solid_angvel = Sw.data();
+        S.solid_normal = Snrm.data(); S.solid_radius = Srad.data(); S.solid_eff_radius = Seff.data();
+        S.solid_inv_mass = Sinvm.data(); S.solid_material = Smat.data(); S.solid_cluster = Sclus.data();
+        S.solid_clump = Sclump.data(); S.solid_dummy = Sdum.data();
+        S.n_clump = nc0 + nc1;
+        S.clump_centroid = Cpos.data(); S.clump_vel = Cvel.data(); S.clump_angvel = Cw.data();
+        S.clump_orientation = Cq.data(); S.clump_inv_inertia = CinvI.data(); S.clump_inv_mass = Cinvm.data();
+        S.clump_pebble_start = Cstart.data(); S.clump_pebble_end = Cend.data();
+        S.hertz = hz.data();
+        S.linear = ln.data();
+        S.bonded = bd.data();
+        const int rc = dsph_upload_batch(c, &S, 1);
+        if (rc != DSPH_OK) return rc;
+        // 3. the contact list as it was, for the search to carry its springs
+        // over by (i, j): a CSR by pointed solid (the new solids have none)
+        if (nss > 0) {
+            if (nss > c->L.ss_cap) alloc_ss(c, nss * 3 / 2 + 1024, false);
+            std::vector<int64_t> ord(nss);
+            for (int64_t k = 0; k < nss; k++) ord[k] = k;
+            std::stable_sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) { return ssi[a] < ssi[b]; });
+            std::vector<int> cnt(c->ns, 0), off(c->ns + 1, 0), pi(nss), pj(nss);
+            std::vector<double> pS(9 * nss);
+            for (int64_t k = 0; k < nss; k++) {
+                const int64_t o = ord[k];
+                pi[k] = ssi[o];
+                pj[k] = ssj[o];
+                for (int m = 0; m < 9; m++) pS[9 * k + m] = ssS[9 * o + m];
+                cnt[pi[k]]++;
+            }
+            for (int i = 0; i < c->ns; i++) off[i + 1] = off[i] + cnt[i];
+            h2d(c, c->L.sscnt, cnt.data(), c->ns);
+            h2d(c, c->L.ssoff, off.data(), c->ns + 1);
+            h2d(c, c->L.ss_i, pi.data(), nss);
+            h2d(c, c->L.ss_j, pj.data(), nss);
+            h2d(c, c->L.ssS, pS.data(), 9 * nss);
+            HIPCHK(hipStreamSynchronize(c->st));
+        }
+        if (nwh) h2d(c, c->whist, wh.data(), nwh);
+        if (nyh) h2d(c, c->yhist, yh.data(), nyh);
+        search_checked(c, 0, 1);
+        // 4. bonds: the old ones as they were; with solids added, the
+        // contacts that addBondData's later branch (dataBase.cpp:309-343)
+        // bonds: pointed after the last bonded solid, touching, one cluster
+        if (ns1 > 0) {
+            int last = -1;
+            for (int i : old.i) last = std::max(last, i);
+            initial_bonds(c);
+            HostBonds fresh;
+            fresh.get(c);
+            for (size_t k = 0; k < fresh.i.size(); k++)
+                if (fresh.i[k] > last) old.append(fresh, k);
+        }
+        old.put(c);
+        // 5. state the scene does not carry: accelerations, torques, sumW0
+        // (the existing particles keep theirs, a new particle starts from
+        // its kernel sum now)
+        if (c->nf > 0) {
+            const int nf = c->nf;
+            FluidSet& f = c->fset[0];
+            density_reinit(c->st, fluid_args(c), 1, c->mixed());
+            std::vector<int> orig(nf);
+            std::vector<double> w(nf);
+            std::vector<double4> a(nf);
+            d2h(c, orig.data(), f.orig, nf);
+            d2h(c, w.data(), f.sumw0, nf);
+            d2h(c, a.data(), f.acc, nf);
+            HIPCHK(hipStreamSynchronize(c->st));
+            for (int sl = 0; sl < nf; sl++) {
+                const int i = orig[sl];
+                if (i >= nf0) continue;
+                w[sl] = fsw[i];
+                a[sl].x = facc[3 * i];
+                a[sl].y = facc[3 * i + 1];
+                a[sl].z = facc[3 * i + 2];
+            }
+            h2d(c, f.sumw0, w.data(), nf);
+            h2d(c, f.acc, a.data(), nf);
+        }
+        if (ns0) {
+            std::vector<double> A(sacc), T(stq);
+            A.resize(3 * (size_t)c->ns, 0.0);
+            T.resize(3 * (size_t)c->ns, 0.0);
+            put_xyz(c, c->so.acc, A, c->ns);
+            put_xyz(c, c->so.tq, T, c->ns);
+        }
+        if (nc0) {
+            std::vector<double> A(cacc), T(ctq);
+            A.resize(3 * (size_t)c->nc, 0.0);
+            T.resize(3 * (size_t)c->nc, 0.0);
+            put_xyz(c, c->cl.acc, A, c->nc);
+            put_xyz(c, c->cl.tq, T, c->nc);
+        }
+        HIPCHK(hipStreamSynchronize(c->st));
+        invalidate_graphs(c);
+        c->initialized = true;
+        return DSPH_OK;
+    } catch (const ApiError& ex) {
+        return fail(c, ex);
+    }
 }
 
 // ------------------------------------------------------------- read-back

@@ -69,6 +69,7 @@ _SIGS = {
     "dsph_download_solid": [_dp] * 6,
     "dsph_download_clump": [_dp] * 6,
     "dsph_download_fluid_solid_force": [_dp, C.c_int64, _i64p],
+    "dsph_append_scene": [C.POINTER(CScene)],
     "dsph_frame_begin": [C.c_int32, C.c_int32, C.c_double, C.c_uint32],
     "dsph_frame_write": [C.c_int32, C.c_char_p, C.c_int32],
     "dsph_download_contacts": [_dp] * 5 + [C.c_int64, _i64p],
@@ -164,6 +165,16 @@ class Solver:
         cs = scene.cstruct()
         self._call("dsph_upload_scene", C.byref(cs))
         self.nf, self.ns, self.nc = scene.n_fluid, scene.n_solid, scene.n_clump
+
+    def append(self, extra: Scene):
+        """Particles added after the run started (dsph_append_scene): extra's
+        fluid, solids and clumps go after the existing ones, which keep
+        their state."""
+        cs = extra.cstruct()
+        self._call("dsph_append_scene", C.byref(cs))
+        self.nf += extra.n_fluid
+        self.ns += extra.n_solid
+        self.nc += extra.n_clump
 
     def upload_batch(self, scenes):
         """Independent scenes advanced together, each on its own grid; their

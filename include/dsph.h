@@ -348,6 +348,31 @@ int dsph_download_clump(dsph_ctx* ctx, double* centroid, double* vel,
  * DSPH_LIST_FLUID_SOLID list order; n = number of pairs                    */
 int dsph_download_fluid_solid_force(dsph_ctx* ctx, double* force, int64_t cap,
                                     int64_t* n_out);
+/* ---- particles added after the run started ----------------------------
+ * replaces: dataBase::addFluid / addSolid / addCluster / addClump called
+ * after step 0 (dataBase.cpp:113-117, 226-268: uploadState, insertData and
+ * copy of the whole device data, setSpatialGrids, neighborSearch(dev, 0, 1))
+ * and addBondData's later branch (dataBase.cpp:309-343).  The fluid,
+ * solids and clumps of extra go after the existing ones of each species
+ * (caller indices continue; extra's clump pebble ranges and solid clump
+ * indices count within extra; cluster ids are taken as given).  Every
+ * existing particle keeps its state: positions, velocities, accelerations,
+ * dRho, sumW0, angular state, normals, contact springs (carried over by
+ * (i, j)), wall spring histories and bonds.  The lists are searched again
+ * at step 0.  With solids added, a contact of the new list becomes a bond
+ * when its pointed solid comes after the last bonded one, the two touch
+ * and share a cluster with a bonded modulus.  New solids keep extra's
+ * normals (none are estimated after the start).  Differences from the
+ * reference: sumW0 is kept, and a new fluid particle's is its kernel sum
+ * at the append (the reference reallocates sumW0 as zeros, so its next
+ * re-initialisation divides by zero); wall spring histories are kept (the
+ * reference rebuilds them empty); with no bond yet any contact may become
+ * one.  extra's walls must be empty (DSPH_E_INVALID); its domain, gravity
+ * and contact tables are ignored.  Needs an initialised context of one
+ * scene (DSPH_E_STATE otherwise, and after dsph_upload_batch).  Graphs are
+ * captured again on the next dsph_step_n.                                 */
+int dsph_append_scene(dsph_ctx* ctx, const dsph_scene* extra);
+
 /* ---- frames: the VTU output of the step loop -------------------------
  * replaces: solverBase::outputFluidVTU, outputSolidVTU and
  * outputFluid2SolidVTU (solverBase.cpp:69-333), which copy the device state
