@@ -196,19 +196,55 @@ void dataBase::uploadContactModels() {
 }
 
 // ----------------------------------------------------------------- builders
-void dataBase::addFluidData(const HostFluid& f) {
-    if (simPara.iStep > 0) {
-        std::cout << "Error: particles can only be added before the simulation starts." << std::endl;
+// Particles added after the start (dataBase.cpp:113-117, 226-268) go to
+// the device in one dsph_append_scene: the running particles keep their
+// state, the lists are searched again and an added cluster is bonded.
+// The host mirrors grow with them; their state is read back on demand.
+void dataBase::appendToDevice(const HostFluid& f, const HostSolid& o, const HostClump& c) {
+    if (!dev) {
+        std::cout << "Error: no device data to add particles to." << std::endl;
         return;
     }
+    dsph_scene e{};
+    e.n_fluid = f.points.num;
+    e.fluid_pos = xyz(f.points.position);
+    e.fluid_vel = xyz(f.dyn.velocities);
+    e.fluid_h = f.points.effectiveRadii.data();
+    e.fluid_rho0 = f.rho0.data();
+    e.fluid_drho = f.dRho.data();
+    e.fluid_c = f.c.data();
+    e.fluid_nu = f.nu.data();
+    e.n_solid = o.points.num;
+    e.solid_pos = xyz(o.points.position);
+    e.solid_vel = xyz(o.dyn.velocities);
+    e.solid_angvel = xyz(o.angularVelocities);
+    e.solid_normal = xyz(o.normal);
+    e.solid_radius = o.radius.data();
+    e.solid_eff_radius = o.points.effectiveRadii.data();
+    e.solid_inv_mass = o.inverseMass.data();
+    e.solid_material = o.materialID.data();
+    e.solid_cluster = o.clusterID.data();
+    e.solid_clump = o.clumpID.data();
+    e.solid_dummy = o.isDummyParticle.data();
+    e.n_clump = c.num;
+    e.clump_centroid = xyz(c.centroidPosition);
+    e.clump_vel = xyz(c.dyn.velocities);
+    e.clump_angvel = xyz(c.angularVelocities);
+    e.clump_orientation = c.orientations.empty() ? nullptr : &c.orientations[0].q0;
+    e.clump_inv_inertia = c.inverseInertiaTensor.empty() ? nullptr : &c.inverseInertiaTensor[0].xx;
+    e.clump_inv_mass = c.inverseMass.data();
+    e.clump_pebble_start = c.pebbleStartIndex.data();
+    e.clump_pebble_end = c.pebbleEndIndex.data();
+    check(dsph_append_scene(dev, &e));
+}
+
+void dataBase::addFluidData(const HostFluid& f) {
+    if (simPara.iStep > 0) appendToDevice(f, HostSolid(0), HostClump(0));
     hos.fluids.append(f);
 }
 
 void dataBase::addSolidData(const HostSolid& s) {
-    if (simPara.iStep > 0) {
-        std::cout << "Error: particles can only be added before the simulation starts." << std::endl;
-        return;
-    }
+    if (simPara.iStep > 0) appendToDevice(HostFluid(0), s, HostClump(0));
     hos.solids.append(s);
 }
 
@@ -318,10 +354,6 @@ void dataBase::addClump(std::vector<double3> p, std::vector<double> radius, doub
         std::cout << "Error: material index exceeds the number of materials." << std::endl;
         return;
     }
-    if (simPara.iStep > 0) {
-        std::cout << "Error: particles can only be added before the simulation starts." << std::endl;
-        return;
-    }
     HostClump c(1);
     c.centroidPosition[0] = centroidPosition;
     c.dyn.velocities[0] = velocity;
@@ -345,7 +377,16 @@ void dataBase::addClump(std::vector<double3> p, std::vector<double> radius, doub
         s.clumpID[i] = clumpID;
         if (density > 0. && radius[i] > 0) s.inverseMass[i] = 1. / (std::pow(2. * radius[i], 3) * density);
     }
-    addSolidData(s);
+    if (simPara.iStep > 0) {
+        // pebbles and clump in one addition, the pebble range within it
+        HostClump r = c;
+        r.pebbleStartIndex[0] = 0;
+        r.pebbleEndIndex[0] = s.points.num;
+        for (size_t i = 0; i < p.size(); i++) s.clumpID[i] = 0;
+        appendToDevice(HostFluid(0), s, r);
+        for (size_t i = 0; i < p.size(); i++) s.clumpID[i] = clumpID;
+    }
+    hos.solids.append(s);
     c.pebbleEndIndex[0] = hos.solids.points.num;
     hos.clumps.append(c);
 }

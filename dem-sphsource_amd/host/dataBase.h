@@ -117,6 +117,8 @@ protected:
 private:
     void addFluidData(const HostFluid& f);
     void addSolidData(const HostSolid& s);
+    // particles added after the start, to the device in one addition
+    void appendToDevice(const HostFluid& f, const HostSolid& o, const HostClump& c);
     void uploadContactModels();
     int precision_ = DSPH_PREC_FP64;
     unsigned flags_ = 0;
