@@ -2938,8 +2938,18 @@ int dsph_append_scene(dsph_ctx* c, const dsph_scene* e) {
     if (!e) return DSPH_E_INVALID;
     if (!c->initialized || !c->host) return DSPH_E_STATE;
     try {
-        validate_scene(e);
         require(e->n_wall == 0 && e->n_cylinder == 0, "walls cannot be added after the start");
+        {
+            // checked as a part of the running scene: its domain and
+            // materials (extra's own are ignored)
+            dsph_scene ev = *e;
+            const dsph_scene& run = c->host->flat;
+            std::memcpy(ev.domain_origin, run.domain_origin, sizeof(ev.domain_origin));
+            std::memcpy(ev.domain_size, run.domain_size, sizeof(ev.domain_size));
+            ev.n_material = run.n_material;
+            ev.hertz = ev.linear = ev.bonded = nullptr;
+            validate_scene(&ev);
+        }
         const int nf0 = c->nf, ns0 = c->ns, nc0 = c->nc;
         const int nf1 = e->n_fluid, ns1 = e->n_solid, nc1 = e->n_clump;
         for (int k = 0; k < nc1; k++)

@@ -34,6 +34,18 @@ public:
         setPrecision(precision);
     }
 
+    // a step at whose contact phase a block of 3 x 3 x 3 fluid particles is
+    // poured in above the water (addFluid after the start), or -1
+    int pourAt = -1;
+
+    void handleDataAfterCalculateContact() override {
+        if (simPara.iStep != pourAt) return;
+        const double dx = particle_spacing;
+        addFluid(getRegularPackedPoints(make_double3(2. * dx, 2. * dx, size_.fluid.z + 2. * dx),
+                                        make_double3(3. * dx, 3. * dx, 3. * dx), dx),
+                 make_double3(0, 0, 0), 1.3 * dx, 1000., 30., 1.e-6);
+    }
+
     double particle_spacing = 0.025;
     double particle_radius = 0.5 * particle_spacing;
     double E = 0.3e9;
@@ -132,6 +144,7 @@ int main(int argc, char** argv) {
     }
     try {
         damBreak problem(sz, steps, prec);
+        if (argc > 5 && std::strcmp(argv[5], "pour") == 0) problem.pourAt = 2;
         const auto t0 = std::chrono::steady_clock::now();
         problem.solve();
         const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

@@ -303,6 +303,30 @@ def test_cpp_dam_break_case_matches_python_path(tmp_path):
     assert read_vtu(d / "fluid_0000.vtu")["STEP"][0] == 0
 
 
+def test_cpp_dam_break_case_adds_fluid_mid_run(tmp_path):
+    """The C++ host API's addFluid after the start (dataBase.cpp:226-240 of
+    the reference) through dsph_append_scene: the case pours 27 fluid
+    particles in at step 2's contact phase; the run goes on with them."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(dsph.LIB_PATH), "_build", "damBreak")
+    out = tmp_path / "state.bin"
+    steps = CASE.gap + 3
+    note(f"c++ case driver damBreak small with a pour, {steps} steps")
+    r = subprocess.run([exe, "small", str(steps), str(dsph.PREC_FP64), str(out), "pour"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = out.read_bytes()
+    nf, ns, nc, gap = (int(v) for v in np.frombuffer(raw[:16], np.int32))
+    assert (nf, ns, nc) == (CASE.n_fluid + 27, CASE.n_solid, CASE.n_clump)
+    d = np.frombuffer(raw[16:], np.float64)
+    fpos = d[:3 * nf].reshape(nf, 3)
+    fvel = d[3 * nf:6 * nf].reshape(nf, 3)
+    assert np.all(np.isfinite(fpos)) and np.all(np.isfinite(fvel))
+    # the poured block fell under gravity over the fluid step after it
+    assert np.all(fvel[CASE.n_fluid:, 2] < 0)
+
+
 # ----------------------------------------------------------- cylinder walls
 def test_rotating_drum_cylinder_walls_match_the_oracle():
     """Spheres in a turning drum (a cylinder wall, inside contacts) with a
