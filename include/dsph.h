@@ -400,7 +400,7 @@ int dsph_append_scene(dsph_ctx* ctx, const dsph_scene* extra);
  * DSPH_FRAME_SLOTS slots until it is written: dsph_frame_write uses only its
  * slot, so it may run on another thread while the caller steps and begins
  * frames in the other slot (not while dsph_frame_begin fills the same slot,
- * and not across dsph_upload_* / dsph_destroy).  It reports by return code
+ * and not across dsph_upload_*, dsph_append_scene or dsph_destroy).  It reports by return code
  * only (dsph_last_error is not updated): DSPH_E_STATE when the slot holds no
  * frame, DSPH_E_IO when a file cannot be written.  No reference counterpart
  * for the split (the reference's output is synchronous).                 */
