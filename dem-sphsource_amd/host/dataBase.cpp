@@ -205,6 +205,7 @@ void dataBase::appendToDevice(const HostFluid& f, const HostSolid& o, const Host
         std::cout << "Error: no device data to add particles to." << std::endl;
         return;
     }
+    beforeDeviceRebuild();
     dsph_scene e{};
     e.n_fluid = f.points.num;
     e.fluid_pos = xyz(f.points.position);

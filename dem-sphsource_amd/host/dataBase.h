@@ -119,6 +119,13 @@ private:
     void addSolidData(const HostSolid& s);
     // particles added after the start, to the device in one addition
     void appendToDevice(const HostFluid& f, const HostSolid& o, const HostClump& c);
+
+protected:
+    // called before the device data is rebuilt for an addition: work that
+    // reads it from other threads (frame writers) must end first
+    virtual void beforeDeviceRebuild() {}
+
+private:
     void uploadContactModels();
     int precision_ = DSPH_PREC_FP64;
     unsigned flags_ = 0;

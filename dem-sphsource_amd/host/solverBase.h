@@ -34,6 +34,9 @@ protected:
 
     const std::string& outputDirectory() const { return dir; }
 
+    // an addition rebuilds the device data: the frame writers finish first
+    void beforeDeviceRebuild() override { joinFrames(); }
+
 private:
     std::string dir = "problem";
 
